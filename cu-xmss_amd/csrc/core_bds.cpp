@@ -1,0 +1,611 @@
+// core_bds.cpp -- the BDS core (drop-in for cuxmss/xmss_core_fast.c): the
+// secret key carries the serialized traversal state, so a signature costs at
+// most (h - k)/2 leaf computations instead of a full tree.
+//
+// The traversal state machine runs on the host over the sk bytes, exactly as
+// the reference maps its state onto them (xmss_core_fast.c:38-136): node
+// arrays are views into sk, integer fields are read on load and written back
+// on store, so fields the reference never assigns keep the caller's bytes.
+// Every hash it needs -- leaves (gen_leaf_wots), node merges (thash_h), WOTS
+// signatures, R and H_msg, and the keygen trees -- is computed on the GPU.
+// Leaves are fetched in batches: each time a leaf is missing, the leaves the
+// state can ask for next (the next leaf of every unfinished treehash
+// instance, the next NEXT-tree leaves) are computed in one launch.
+#include <string.h>
+
+#include <map>
+#include <vector>
+
+#include "../../include/randombytes.h"
+#include "../../include/xmss_core.h"
+#include "core_common.h"
+
+using namespace xm;
+
+namespace {
+
+constexpr uint32_t ADDR_OTS = 0, ADDR_LTREE = 1, ADDR_HASHTREE = 2;
+
+struct TreehashInst {
+    unsigned char h;
+    unsigned long next_idx;
+    unsigned char stackusage;
+    unsigned char completed;
+    unsigned char* node;
+};
+
+struct BdsState {
+    unsigned char* stack;
+    unsigned int stackoffset;
+    unsigned char* stacklevels;
+    unsigned char* auth;
+    unsigned char* keep;
+    TreehashInst* th;
+    unsigned char* retain;
+    unsigned int next_leaf;
+};
+
+// GPU-backed hashing for one key during one call.
+struct Hasher {
+    Engine* e;
+    int fam;
+    unsigned n;
+    // leaves computed so far in this call, keyed by (layer, tree, leaf)
+    std::map<std::tuple<uint32_t, uint64_t, uint32_t>, std::vector<unsigned char>> cache;
+    std::vector<LeafRun> want;  // speculative leaves for the next miss
+
+    void leaf(unsigned char* out, const uint32_t addr[8], uint32_t leaf_idx) {
+        const uint64_t tree = ((uint64_t)addr[1] << 32) | addr[2];
+        auto k = std::make_tuple(addr[0], tree, leaf_idx);
+        auto it = cache.find(k);
+        if (it == cache.end()) {
+            std::vector<LeafRun> runs{{0, addr[0], tree, leaf_idx}};
+            for (const LeafRun& r : want) {
+                if (!cache.count(std::make_tuple(r.layer, r.tree, r.leaf0)) &&
+                    !(r.layer == addr[0] && r.tree == tree && r.leaf0 == leaf_idx))
+                    runs.push_back(r);
+            }
+            want.clear();
+            std::vector<unsigned char> out_all(runs.size() * n);
+            e->leaves(fam, (int)runs.size(), runs.data(), 1, out_all.data());
+            for (size_t i = 0; i < runs.size(); ++i) {
+                cache[std::make_tuple(runs[i].layer, runs[i].tree, runs[i].leaf0)] =
+                    std::vector<unsigned char>(out_all.begin() + i * n, out_all.begin() + (i + 1) * n);
+            }
+            it = cache.find(k);
+        }
+        memcpy(out, it->second.data(), n);
+    }
+    void thash_h(unsigned char* out, const unsigned char* in, const uint32_t adrs[8]) {
+        ThashReq q;
+        q.key = 0;
+        memcpy(q.adrs, adrs, 32);
+        e->thash_h(fam, 1, &q, in, out);
+    }
+};
+
+void subtree_addrs(const uint32_t addr[8], uint32_t nd[8]) {
+    memset(nd, 0, 32);
+    nd[0] = addr[0];
+    nd[1] = addr[1];
+    nd[2] = addr[2];
+    nd[3] = ADDR_HASHTREE;
+}
+
+// xmssmt_deserialize_state (xmss_core_fast.c:80-136)
+void load_states(const xmss_params* p, BdsState* st, TreehashInst* ths, unsigned char** wots_sigs,
+                 unsigned char* sk) {
+    const unsigned nth = p->tree_height - p->bds_k;
+    sk += p->index_bytes + 4 * p->n;
+    for (unsigned i = 0; i < 2 * p->d - 1; ++i) {
+        st[i].th = ths + i * nth;
+        st[i].stack = sk;
+        sk += (p->tree_height + 1) * p->n;
+        st[i].stackoffset = (unsigned)bytes_to_ull(sk, 4);
+        sk += 4;
+        st[i].stacklevels = sk;
+        sk += p->tree_height + 1;
+        st[i].auth = sk;
+        sk += p->tree_height * p->n;
+        st[i].keep = sk;
+        sk += (p->tree_height >> 1) * p->n;
+        for (unsigned j = 0; j < nth; ++j) {
+            st[i].th[j].h = (unsigned char)bytes_to_ull(sk, 1);
+            st[i].th[j].next_idx = (unsigned long)bytes_to_ull(sk + 1, 4);
+            st[i].th[j].stackusage = (unsigned char)bytes_to_ull(sk + 5, 1);
+            st[i].th[j].completed = (unsigned char)bytes_to_ull(sk + 6, 1);
+            st[i].th[j].node = sk + 7;
+            sk += 7 + p->n;
+        }
+        st[i].retain = sk;
+        sk += ((1 << p->bds_k) - p->bds_k - 1) * p->n;
+        st[i].next_leaf = (unsigned)bytes_to_ull(sk, 4);
+        sk += 4;
+    }
+    if (wots_sigs) *wots_sigs = sk;
+}
+
+// xmssmt_serialize_state (xmss_core_fast.c:38-78)
+void store_states(const xmss_params* p, unsigned char* sk, const BdsState* st) {
+    const unsigned nth = p->tree_height - p->bds_k;
+    sk += p->index_bytes + 4 * p->n;
+    for (unsigned i = 0; i < 2 * p->d - 1; ++i) {
+        sk += (p->tree_height + 1) * p->n;
+        ull_to_bytes(sk, 4, st[i].stackoffset);
+        sk += 4 + p->tree_height + 1 + p->tree_height * p->n + (p->tree_height >> 1) * p->n;
+        for (unsigned j = 0; j < nth; ++j) {
+            ull_to_bytes(sk, 1, st[i].th[j].h);
+            ull_to_bytes(sk + 1, 4, st[i].th[j].next_idx);
+            ull_to_bytes(sk + 5, 1, st[i].th[j].stackusage);
+            ull_to_bytes(sk + 6, 1, st[i].th[j].completed);
+            sk += 7 + p->n;
+        }
+        sk += ((1 << p->bds_k) - p->bds_k - 1) * p->n;
+        ull_to_bytes(sk, 4, st[i].next_leaf);
+        sk += 4;
+    }
+}
+
+void swap_bytes(unsigned char* a, unsigned char* b, size_t len) {
+    for (size_t i = 0; i < len; ++i) std::swap(a[i], b[i]);
+}
+
+// deep_state_swap (xmss_core_fast.c:165-193): swaps contents, not pointers
+void swap_states(const xmss_params* p, BdsState* a, BdsState* b) {
+    swap_bytes(a->stack, b->stack, (p->tree_height + 1) * p->n);
+    std::swap(a->stackoffset, b->stackoffset);
+    swap_bytes(a->stacklevels, b->stacklevels, p->tree_height + 1);
+    swap_bytes(a->auth, b->auth, p->tree_height * p->n);
+    swap_bytes(a->keep, b->keep, (p->tree_height >> 1) * p->n);
+    for (unsigned i = 0; i < p->tree_height - p->bds_k; ++i) {
+        std::swap(a->th[i].h, b->th[i].h);
+        std::swap(a->th[i].next_idx, b->th[i].next_idx);
+        std::swap(a->th[i].stackusage, b->th[i].stackusage);
+        std::swap(a->th[i].completed, b->th[i].completed);
+        swap_bytes(a->th[i].node, b->th[i].node, p->n);
+    }
+    swap_bytes(a->retain, b->retain, ((1 << p->bds_k) - p->bds_k - 1) * p->n);
+    std::swap(a->next_leaf, b->next_leaf);
+}
+
+// Where a right child of height nodeh and index j goes when the stack merges
+// it (xmss_core_fast.c:262-269 and :534-541).
+unsigned char* record_slot(const xmss_params* p, BdsState* st, unsigned nodeh, unsigned long j) {
+    if (j == 1) return st->auth + nodeh * p->n;
+    if (nodeh < p->tree_height - p->bds_k && j == 3) return st->th[nodeh].node;
+    if (nodeh >= p->tree_height - p->bds_k)
+        return st->retain + ((1 << (p->tree_height - 1 - nodeh)) + nodeh - p->tree_height + ((j - 3) >> 1)) * p->n;
+    return nullptr;
+}
+
+// treehash_init (xmss_core_fast.c:216-285) for several subtrees at once: the
+// whole subtree is built on the GPU and the nodes the stack walk would have
+// recorded (auth: index 1, treehash nodes: index 3, retain: odd indices >= 3
+// of the top bds_k heights) are read back.  Returns the roots.
+void treehash_init_many(const xmss_params* p, Engine* e, int fam, unsigned nt, const uint32_t* layers,
+                        BdsState* const* states, unsigned char* roots) {
+    const unsigned th = p->tree_height, n = p->n;
+    std::vector<LeafRun> runs(nt);
+    for (unsigned t = 0; t < nt; ++t) runs[t] = {0, layers[t], 0, 0};
+    e->build_trees(fam, th, (int)nt, runs.data());
+    std::vector<NodeRef> refs;
+    std::vector<unsigned char*> dst;
+    for (unsigned t = 0; t < nt; ++t) {
+        BdsState* st = states[t];
+        for (unsigned i = 0; i < th - p->bds_k; ++i) {
+            st->th[i].h = (unsigned char)i;
+            st->th[i].completed = 1;
+            st->th[i].stackusage = 0;
+        }
+        for (unsigned h = 0; h < th; ++h) {
+            for (unsigned long j = 1; j < (1ul << (th - h)); j += 2) {
+                unsigned char* slot = record_slot(p, st, h, j);
+                if (slot) {
+                    refs.push_back({t, h, (uint32_t)j});
+                    dst.push_back(slot);
+                }
+                if (h < th - p->bds_k && j >= 3) break;  // only index 1 and 3 below the retain rows
+            }
+        }
+        refs.push_back({t, th, 0});
+        dst.push_back(roots + (size_t)t * n);
+    }
+    std::vector<unsigned char> got(refs.size() * n);
+    e->fetch_nodes(fam, th, (int)refs.size(), refs.data(), got.data());
+    for (size_t i = 0; i < refs.size(); ++i) memcpy(dst[i], &got[i * n], n);
+}
+
+unsigned treehash_minheight_on_stack(const xmss_params* p, const BdsState* st, const TreehashInst* t) {
+    unsigned r = p->tree_height;
+    for (unsigned i = 0; i < t->stackusage; ++i) {
+        if (st->stacklevels[st->stackoffset - i - 1] < r) r = st->stacklevels[st->stackoffset - i - 1];
+    }
+    return r;
+}
+
+// treehash_update (xmss_core_fast.c:406-452)
+void treehash_update(const xmss_params* p, Hasher& H, TreehashInst* t, BdsState* st, const uint32_t addr[8]) {
+    const unsigned n = p->n;
+    uint32_t nd[8];
+    subtree_addrs(addr, nd);
+    unsigned char nb[2 * 64];
+    unsigned nodeheight = 0;
+    H.leaf(nb, addr, (uint32_t)t->next_idx);
+    while (t->stackusage > 0 && st->stacklevels[st->stackoffset - 1] == nodeheight) {
+        memcpy(nb + n, nb, n);
+        memcpy(nb, st->stack + (st->stackoffset - 1) * n, n);
+        nd[5] = nodeheight;
+        nd[6] = (uint32_t)(t->next_idx >> (nodeheight + 1));
+        H.thash_h(nb, nb, nd);
+        ++nodeheight;
+        --t->stackusage;
+        --st->stackoffset;
+    }
+    if (nodeheight == t->h) {
+        memcpy(t->node, nb, n);
+        t->completed = 1;
+    } else {
+        memcpy(st->stack + st->stackoffset * n, nb, n);
+        ++t->stackusage;
+        st->stacklevels[st->stackoffset] = (unsigned char)nodeheight;
+        ++st->stackoffset;
+        ++t->next_idx;
+    }
+}
+
+// Leaves the update loop below can ask for: the next leaf of every unfinished
+// treehash instance (up to `budget` consecutive ones each).
+void want_treehash_leaves(const xmss_params* p, Hasher& H, const BdsState* st, unsigned budget,
+                          const uint32_t addr[8]) {
+    const uint64_t tree = ((uint64_t)addr[1] << 32) | addr[2];
+    for (unsigned i = 0; i < p->tree_height - p->bds_k; ++i) {
+        if (st->th[i].completed) continue;
+        const unsigned long span = 1ul << st->th[i].h;
+        for (unsigned k = 0; k < budget && k < span; ++k) {
+            const unsigned long leaf = st->th[i].next_idx + k;
+            if (leaf >= (1ul << p->tree_height)) break;
+            H.want.push_back({0, addr[0], tree, (uint32_t)leaf});
+        }
+    }
+}
+
+// bds_treehash_update (xmss_core_fast.c:458-491).  The reference returns
+// `char`; the result is narrowed and sign-extended the same way.
+unsigned bds_treehash_update(const xmss_params* p, Hasher& H, BdsState* st, unsigned updates,
+                             const uint32_t addr[8]) {
+    const unsigned nth = p->tree_height - p->bds_k;
+    unsigned used = 0;
+    want_treehash_leaves(p, H, st, updates < 16 ? updates : 16, addr);
+    for (unsigned j = 0; j < updates; ++j) {
+        unsigned l_min = p->tree_height, level = nth;
+        for (unsigned i = 0; i < nth; ++i) {
+            unsigned low;
+            if (st->th[i].completed) low = p->tree_height;
+            else if (st->th[i].stackusage == 0) low = i;
+            else low = treehash_minheight_on_stack(p, st, &st->th[i]);
+            if (low < l_min) {
+                level = i;
+                l_min = low;
+            }
+        }
+        if (level == nth) break;
+        treehash_update(p, H, &st->th[level], st, addr);
+        ++used;
+    }
+    return (unsigned)(int)(signed char)(updates - used);
+}
+
+// bds_state_update (xmss_core_fast.c:497-552)
+int bds_state_update(const xmss_params* p, Hasher& H, BdsState* st, const uint32_t addr[8]) {
+    const unsigned n = p->n;
+    const int idx = (int)st->next_leaf;
+    if (idx == 1 << p->tree_height) return -1;
+    uint32_t nd[8];
+    subtree_addrs(addr, nd);
+    H.leaf(st->stack + st->stackoffset * n, addr, (uint32_t)idx);
+    st->stacklevels[st->stackoffset] = 0;
+    ++st->stackoffset;
+    while (st->stackoffset > 1 && st->stacklevels[st->stackoffset - 1] == st->stacklevels[st->stackoffset - 2]) {
+        const unsigned nodeh = st->stacklevels[st->stackoffset - 1];
+        unsigned char* slot = record_slot(p, st, nodeh, (unsigned long)(idx >> nodeh));
+        if (slot) memcpy(slot, st->stack + (st->stackoffset - 1) * n, n);
+        nd[5] = st->stacklevels[st->stackoffset - 1];
+        nd[6] = (uint32_t)(idx >> (st->stacklevels[st->stackoffset - 1] + 1));
+        H.thash_h(st->stack + (st->stackoffset - 2) * n, st->stack + (st->stackoffset - 2) * n, nd);
+        ++st->stacklevels[st->stackoffset - 2];
+        --st->stackoffset;
+    }
+    ++st->next_leaf;
+    return 0;
+}
+
+// bds_round (xmss_core_fast.c:559-626)
+void bds_round(const xmss_params* p, Hasher& H, BdsState* st, unsigned long leaf_idx, const uint32_t addr[8]) {
+    const unsigned n = p->n, th = p->tree_height, nth = th - p->bds_k;
+    unsigned tau = th;
+    unsigned char buf[2 * 64];
+    uint32_t nd[8];
+    subtree_addrs(addr, nd);
+    for (unsigned i = 0; i < th; ++i) {
+        if (!((leaf_idx >> i) & 1)) {
+            tau = i;
+            break;
+        }
+    }
+    if (tau > 0) {
+        memcpy(buf, st->auth + (tau - 1) * n, n);
+        memcpy(buf + n, st->keep + ((tau - 1) >> 1) * n, n);
+    }
+    if (!((leaf_idx >> (tau + 1)) & 1) && (tau < th - 1)) {
+        memcpy(st->keep + (tau >> 1) * n, st->auth + tau * n, n);
+    }
+    if (tau == 0) {
+        H.leaf(st->auth, addr, (uint32_t)leaf_idx);
+    } else {
+        nd[5] = tau - 1;
+        nd[6] = (uint32_t)(leaf_idx >> tau);
+        H.thash_h(st->auth + tau * n, buf, nd);
+        for (unsigned i = 0; i < tau; ++i) {
+            if (i < nth) {
+                memcpy(st->auth + i * n, st->th[i].node, n);
+            } else {
+                const unsigned offset = (1 << (th - 1 - i)) + i - th;
+                const unsigned rowidx = (unsigned)(((leaf_idx >> i) - 1) >> 1);
+                memcpy(st->auth + i * n, st->retain + (offset + rowidx) * n, n);
+            }
+        }
+        const unsigned lim = tau < nth ? tau : nth;
+        for (unsigned i = 0; i < lim; ++i) {
+            const unsigned startidx = (unsigned)(leaf_idx + 1 + 3 * (1u << i));
+            if (startidx < 1u << th) {
+                st->th[i].h = (unsigned char)i;
+                st->th[i].next_idx = startidx;
+                st->th[i].completed = 0;
+                st->th[i].stackusage = 0;
+            }
+        }
+    }
+}
+
+int prepare_engine(const xmss_params* p, Engine** e, int* fam) {
+    *fam = fam_of(p->func, p->n);
+    if (*fam < 0) return -1;
+    *e = Engine::get();
+    return *e ? 0 : XMSS_ENODEV;
+}
+
+// R, H_msg and the layer-0 WOTS signature of a BDS sign, all on the GPU.
+void sign_front(const xmss_params* p, Engine* e, int fam, unsigned long long idx, const unsigned char* pub_root,
+                unsigned char* sm, const unsigned char* m_in_sm, unsigned long long mlen, unsigned char* R,
+                unsigned char* wots_out, uint32_t layer, uint64_t tree, uint32_t ots) {
+    unsigned char mh[64];
+    MsgReq mr{0, 1, idx, nullptr, pub_root, m_in_sm, mlen};
+    e->hmsg(fam, 1, &mr, R, mh);
+    WotsReq w{0, layer, tree, ots, WOTS_SIGN};
+    e->wots(fam, 1, &w, mh, nullptr, wots_out);
+    (void)sm;
+}
+
+}  // namespace
+
+extern "C" {
+
+// xmss_core_fast.c:635-649
+unsigned long long xmss_xmssmt_core_sk_bytes(const xmss_params* p) {
+    return p->index_bytes + 4 * p->n +
+           (2 * p->d - 1) * ((p->tree_height + 1) * p->n + 4 + p->tree_height + 1 + p->tree_height * p->n +
+                             (p->tree_height >> 1) * p->n + (p->tree_height - p->bds_k) * (7 + p->n) +
+                             ((1 << p->bds_k) - p->bds_k - 1) * p->n + 4) +
+           (p->d - 1) * p->wots_sig_bytes;
+}
+
+// xmss_core_fast.c:656-701.  As in the reference, randombytes is not used:
+// SK_SEED = SK_PRF = 0x02.., PUB_SEED = 0x04.. (:678-687).
+int xmss_core_keypair(const xmss_params* params, unsigned char* pk, unsigned char* sk) {
+    const xmss_params* p = params;
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    Engine* e;
+    int fam;
+    if (int rc = prepare_engine(p, &e, &fam)) return rc;
+    std::vector<TreehashInst> ths(p->tree_height - p->bds_k + 1);
+    BdsState st;
+    load_states(p, &st, ths.data(), nullptr, sk);
+    st.stackoffset = 0;
+    st.next_leaf = 0;
+    sk[0] = sk[1] = sk[2] = sk[3] = 0;
+    memset(sk + p->index_bytes, 2, 2 * p->n);
+    memset(sk + p->index_bytes + 3 * p->n, 4, p->n);
+    memcpy(pk + p->n, sk + p->index_bytes + 3 * p->n, p->n);
+    KeySeeds ks{sk + p->index_bytes, sk + p->index_bytes + p->n, sk + p->index_bytes + 3 * p->n};
+    e->set_keys(fam, 1, &ks);
+    uint32_t layer = 0;
+    BdsState* sp = &st;
+    treehash_init_many(p, e, fam, 1, &layer, &sp, pk);
+    memcpy(sk + p->index_bytes + 2 * p->n, pk, p->n);
+    store_states(p, sk, &st);
+    return 0;
+}
+
+// xmss_core_fast.c:710-851
+int xmss_core_sign(const xmss_params* params, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
+                   const unsigned char* m, unsigned long long mlen) {
+    const xmss_params* p = params;
+    const unsigned n = p->n, th = p->tree_height;
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    Engine* e;
+    int fam;
+    if (int rc = prepare_engine(p, &e, &fam)) return rc;
+    std::vector<TreehashInst> ths(th - p->bds_k + 1);
+    BdsState st;
+    load_states(p, &st, ths.data(), nullptr, sk);
+    const unsigned long idx = ((unsigned long)sk[0] << 24) | ((unsigned long)sk[1] << 16) |
+                              ((unsigned long)sk[2] << 8) | sk[3];
+    if (index_check(p, sk, idx)) return -2;
+    unsigned char sk_seed[64], sk_prf[64], pub_seed[64], pub_root[64];
+    memcpy(sk_seed, sk + p->index_bytes, n);
+    memcpy(sk_prf, sk + p->index_bytes + n, n);
+    memcpy(pub_seed, sk + p->index_bytes + 3 * n, n);
+    memcpy(pub_root, sk + p->index_bytes + 2 * n, n);
+    sk[0] = ((idx + 1) >> 24) & 255;
+    sk[1] = ((idx + 1) >> 16) & 255;
+    sk[2] = ((idx + 1) >> 8) & 255;
+    sk[3] = (idx + 1) & 255;
+    KeySeeds ks{sk_seed, sk_prf, pub_seed};
+    e->set_keys(fam, 1, &ks);
+    memmove(sm + p->sig_bytes, m, mlen);
+    unsigned char R[64];
+    sign_front(p, e, fam, idx, pub_root, sm, sm + p->sig_bytes, mlen, R, sm + 4 + n, 0, 0, (uint32_t)idx);
+    sm[0] = (idx >> 24) & 255;
+    sm[1] = (idx >> 16) & 255;
+    sm[2] = (idx >> 8) & 255;
+    sm[3] = idx & 255;
+    memcpy(sm + 4, R, n);
+    unsigned char* s = sm + 4 + n + p->wots_sig_bytes;
+    memcpy(s, st.auth, th * n);
+    uint32_t ots_addr[8] = {0, 0, 0, ADDR_OTS, (uint32_t)idx, 0, 0, 0};
+    if (idx < (1UL << th) - 1) {
+        Hasher H{e, fam, n, {}, {}};
+        bds_round(p, H, &st, idx, ots_addr);
+        bds_treehash_update(p, H, &st, (th - p->bds_k) >> 1, ots_addr);
+    }
+    *smlen = p->sig_bytes + mlen;
+    store_states(p, sk, &st);
+    return 0;
+}
+
+// xmss_core_fast.c:858-908
+int xmssmt_core_keypair(const xmss_params* params, unsigned char* pk, unsigned char* sk) {
+    const xmss_params* p = params;
+    const unsigned n = p->n, d = p->d;
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    Engine* e;
+    int fam;
+    if (int rc = prepare_engine(p, &e, &fam)) return rc;
+    const unsigned nth = p->tree_height - p->bds_k;
+    std::vector<BdsState> st(2 * d - 1);
+    std::vector<TreehashInst> ths((2 * d - 1) * nth + 1);
+    unsigned char* wots_sigs;
+    load_states(p, st.data(), ths.data(), &wots_sigs, sk);
+    for (unsigned i = 0; i < 2 * d - 1; ++i) {
+        st[i].stackoffset = 0;
+        st[i].next_leaf = 0;
+    }
+    memset(sk, 0, p->index_bytes);
+    randombytes(sk + p->index_bytes, 2 * n);
+    randombytes(sk + p->index_bytes + 3 * n, n);
+    memcpy(pk + n, sk + p->index_bytes + 3 * n, n);
+    KeySeeds ks{sk + p->index_bytes, sk + p->index_bytes + n, sk + p->index_bytes + 3 * n};
+    e->set_keys(fam, 1, &ks);
+    // the d layer-0..d-1 trees (tree address 0) are independent: one build
+    std::vector<uint32_t> layers(d);
+    std::vector<BdsState*> sps(d);
+    for (unsigned i = 0; i < d; ++i) {
+        layers[i] = i;
+        sps[i] = &st[i];
+    }
+    std::vector<unsigned char> roots((size_t)d * n);
+    treehash_init_many(p, e, fam, d, layers.data(), sps.data(), roots.data());
+    // layer i+1 signs the root of layer i (leaf 0 of tree 0)
+    if (d > 1) {
+        std::vector<WotsReq> w(d - 1);
+        for (unsigned i = 0; i + 1 < d; ++i) w[i] = {0, i + 1, 0, 0, WOTS_SIGN};
+        e->wots(fam, (int)(d - 1), w.data(), roots.data(), nullptr, wots_sigs);
+    }
+    memcpy(pk, &roots[(size_t)(d - 1) * n], n);
+    memcpy(sk + p->index_bytes + 2 * n, pk, n);
+    store_states(p, sk, st.data());
+    return 0;
+}
+
+// xmss_core_fast.c:917-1121
+int xmssmt_core_sign(const xmss_params* params, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
+                     const unsigned char* m, unsigned long long mlen) {
+    const xmss_params* p = params;
+    const unsigned n = p->n, th = p->tree_height, d = p->d, nth = th - p->bds_k;
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    Engine* e;
+    int fam;
+    if (int rc = prepare_engine(p, &e, &fam)) return rc;
+    std::vector<BdsState> st(2 * d - 1);
+    std::vector<TreehashInst> ths((2 * d - 1) * nth + 1);
+    unsigned char* wots_sigs;
+    load_states(p, st.data(), ths.data(), &wots_sigs, sk);
+    unsigned long long idx = 0;
+    for (unsigned i = 0; i < p->index_bytes; ++i) idx |= ((unsigned long long)sk[i]) << 8 * (p->index_bytes - 1 - i);
+    if (index_check(p, sk, idx)) return -2;
+    unsigned char sk_seed[64], sk_prf[64], pub_seed[64], pub_root[64];
+    memcpy(sk_seed, sk + p->index_bytes, n);
+    memcpy(sk_prf, sk + p->index_bytes + n, n);
+    memcpy(pub_seed, sk + p->index_bytes + 3 * n, n);
+    memcpy(pub_root, sk + p->index_bytes + 2 * n, n);
+    for (unsigned i = 0; i < p->index_bytes; ++i) sk[i] = ((idx + 1) >> 8 * (p->index_bytes - 1 - i)) & 255;
+    KeySeeds ks{sk_seed, sk_prf, pub_seed};
+    e->set_keys(fam, 1, &ks);
+    memmove(sm + p->sig_bytes, m, mlen);
+    uint64_t idx_tree = idx >> th;
+    uint32_t idx_leaf = (uint32_t)(idx & ((1u << th) - 1));
+    unsigned char R[64];
+    sign_front(p, e, fam, idx, pub_root, sm, sm + p->sig_bytes, mlen, R, sm + p->index_bytes + n, 0, idx_tree,
+               idx_leaf);
+    for (unsigned i = 0; i < p->index_bytes; ++i) sm[i] = (idx >> 8 * (p->index_bytes - 1 - i)) & 255;
+    memcpy(sm + p->index_bytes, R, n);
+    unsigned char* s = sm + p->index_bytes + n + p->wots_sig_bytes;
+    memcpy(s, st[0].auth, th * n);
+    s += th * n;
+    for (unsigned i = 1; i < d; ++i) {
+        memcpy(s, wots_sigs + (i - 1) * p->wots_sig_bytes, p->wots_sig_bytes);
+        s += p->wots_sig_bytes;
+        memcpy(s, st[i].auth, th * n);
+        s += th * n;
+    }
+
+    Hasher H{e, fam, n, {}, {}};
+    uint32_t addr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t ots_addr[8] = {0, 0, 0, ADDR_OTS, 0, 0, 0, 0};
+    int needswap_upto = -1;
+    unsigned updates = (th - p->bds_k) >> 1;
+    addr[1] = (uint32_t)((idx_tree + 1) >> 32);
+    addr[2] = (uint32_t)(idx_tree + 1);
+    if ((1 + idx_tree) * (1ULL << th) + idx_leaf < (1ULL << p->full_height)) {
+        bds_state_update(p, H, &st[d], addr);
+    }
+    for (unsigned i = 0; i < d; ++i) {
+        if (!(((idx + 1) & ((1ULL << ((i + 1) * th)) - 1)) == 0)) {
+            idx_leaf = (uint32_t)((idx >> (th * i)) & ((1u << th) - 1));
+            idx_tree = idx >> (th * (i + 1));
+            addr[0] = i;
+            addr[1] = (uint32_t)(idx_tree >> 32);
+            addr[2] = (uint32_t)idx_tree;
+            if ((int)i == needswap_upto + 1) bds_round(p, H, &st[i], idx_leaf, addr);
+            updates = bds_treehash_update(p, H, &st[i], updates, addr);
+            addr[1] = (uint32_t)((idx_tree + 1) >> 32);
+            addr[2] = (uint32_t)(idx_tree + 1);
+            if ((1 + idx_tree) * (1ULL << th) + idx_leaf < (1ULL << (p->full_height - th * i))) {
+                if (i > 0 && updates > 0 && st[d + i].next_leaf < (1ULL << p->full_height)) {
+                    bds_state_update(p, H, &st[d + i], addr);
+                    --updates;
+                }
+            }
+        } else if (idx < (1ULL << p->full_height) - 1) {
+            swap_states(p, &st[d + i], &st[i]);
+            const uint64_t t = (idx + 1) >> ((i + 2) * th);
+            const uint32_t ots = (uint32_t)(((idx >> ((i + 1) * th)) + 1) & ((1u << th) - 1));
+            ots_addr[0] = i + 1;
+            ots_addr[1] = (uint32_t)(t >> 32);
+            ots_addr[2] = (uint32_t)t;
+            ots_addr[4] = ots;
+            WotsReq w{0, i + 1, t, ots, WOTS_SIGN};
+            e->wots(fam, 1, &w, st[i].stack, nullptr, wots_sigs + i * p->wots_sig_bytes);
+            st[d + i].stackoffset = 0;
+            st[d + i].next_leaf = 0;
+            --updates;  // WOTS signing counts as one update (:1107)
+            needswap_upto = (int)i;
+            for (unsigned j = 0; j < nth; ++j) st[i].th[j].completed = 1;
+        }
+    }
+    *smlen = p->sig_bytes + mlen;
+    store_states(p, sk, st.data());
+    return 0;
+}
+
+}  // extern "C"

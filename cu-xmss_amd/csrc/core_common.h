@@ -1,0 +1,51 @@
+// core_common.h -- helpers shared by the plain and BDS cores.
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "../../include/params.h"
+#include "engine.h"
+
+namespace xm {
+
+// utils.c:6-30
+inline void ull_to_bytes(unsigned char* out, unsigned outlen, unsigned long long in) {
+    for (int i = (int)outlen - 1; i >= 0; --i) {
+        out[i] = (unsigned char)(in & 0xff);
+        in >>= 8;
+    }
+}
+inline unsigned long long bytes_to_ull(const unsigned char* in, unsigned inlen) {
+    unsigned long long r = 0;
+    for (unsigned i = 0; i < inlen; ++i) r |= ((unsigned long long)in[i]) << (8 * (inlen - 1 - i));
+    return r;
+}
+
+// Index check of both cores (xmss_core.c:222-231, xmss_core_fast.c:742-751,
+// :970-979): from idx = 2^h - 1 on the key is wiped (index bytes 0xFF, the
+// rest zero); returns -2 when no one-time key is left, 0 otherwise (the last
+// index still signs, with the wiped key).
+inline int index_check(const xmss_params* p, unsigned char* sk, unsigned long long idx) {
+    if (idx >= ((1ULL << p->full_height) - 1)) {
+        memset(sk, 0xFF, p->index_bytes);
+        memset(sk + p->index_bytes, 0, (size_t)(p->sk_bytes - p->index_bytes));
+        if (idx > ((1ULL << p->full_height) - 1)) return -2;
+        if (p->full_height == 64 && idx == ((1ULL << p->full_height) - 1)) return -2;
+    }
+    return 0;
+}
+
+// All device work of the library is serialised on the engine of the current
+// device (one stream); callers on other threads wait here.
+std::mutex& engine_mutex();
+
+// Return code when no GPU is usable (the library has no CPU path).
+constexpr int XMSS_ENODEV = -3;
+
+// Shared verification (xmss_commons.c:136-215).
+int core_sign_open(const xmss_params* p, unsigned char* m, unsigned long long* mlen, const unsigned char* sm,
+                   unsigned long long smlen, const unsigned char* pk);
+
+}  // namespace xm

@@ -1,0 +1,524 @@
+// dev_family.h -- the tweakable-hash layer of XMSS on the device, one policy
+// class per hash family (the (func, n) pairs that core_hash accepts,
+// cuxmss/hash.c:29-65).
+//
+// Each policy provides, on nodes held in registers:
+//   prepare(key)                 per-key precomputation (SHA-2 midstates)
+//   prf_keygen(key, adrs)        hash.c:87-98   PRF_keygen(SK_SEED, PUB_SEED || ADRS)
+//   thash_f(key, adrs, x)        hash.c:157-183 F with key and bitmask from PRF
+//   thash_h(key, adrs, l, r)     hash.c:125-155 H with key and two bitmasks
+//   prf_idx(key, idx)            hash.c:70-81   R = PRF(SK_PRF, toByte(idx, 32))
+//   load / store                 node <-> n bytes in device memory
+//   hash_bytes(p, len)           core_hash over a byte string in device memory
+//
+// ADRS is passed as its eight 32-bit values (hash_address.c).  Word 7
+// (key_and_mask) is set inside thash_f / thash_h as the reference does.
+//
+// Message words that are constant (domain-separation pads, padding, zero
+// words) are written as literals so that the fully unrolled compressions fold
+// them; for n = 32 (SHA-256) and n = 64 (SHA-512) the one-block prefix
+// toByte(pad) || KEY of every PRF is absorbed once per key (midstate).
+#pragma once
+#include "dev_hash.h"
+
+namespace xm {
+
+enum : int { PAD_F = 0, PAD_H = 1, PAD_HASH = 2, PAD_PRF = 3, PAD_PRF_KEYGEN = 4 };
+
+// ===========================================================================
+// SHA-256, n = 24 (pad 4, output truncated, hash.c:36-39) and n = 32
+// ===========================================================================
+template <int N>
+struct Sha256Fam {
+    static constexpr int kN = N;
+    static constexpr int NW = N / 4;         // node words (big-endian values)
+    static constexpr int LEN = 8 * N / 4 + 3;  // wots_len for w = 16
+    static constexpr int BLOCK = 64;
+    using Word = uint32_t;
+    struct Node { uint32_t w[NW]; };
+    struct Key {
+        uint32_t pub[NW], sks[NW], prfk[NW];
+        uint32_t ms_pub[8], ms_sks[8], ms_prfk[8];  // n = 32: state after toByte(pad,32)||key
+    };
+
+    __device__ static void load(Node& x, const uint8_t* p) {
+        const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) x.w[i] = bswap32(q[i]);
+    }
+    __device__ static void store(const Node& x, uint8_t* p) {
+        uint32_t* q = reinterpret_cast<uint32_t*>(p);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) q[i] = bswap32(x.w[i]);
+    }
+
+    __device__ static void midstate(uint32_t out[8], uint32_t pad, const uint32_t k[NW]) {
+        uint32_t w[16];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) w[i] = 0;
+        w[7] = pad;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) w[8 + i] = k[i];
+        sha256_iv(out);
+        sha256_compress(out, w);
+    }
+    __device__ static void prepare(Key& k) {
+        if constexpr (N == 32) {
+            midstate(k.ms_pub, PAD_PRF, k.pub);
+            midstate(k.ms_sks, PAD_PRF_KEYGEN, k.sks);
+            midstate(k.ms_prfk, PAD_PRF, k.prfk);
+        }
+    }
+
+    // PRF(KEY, in) with in = 8 words; ms/key select the key (hash.c:70-81).
+    __device__ static void prf8(Node& out, const uint32_t ms[8], const uint32_t key[NW],
+                                const uint32_t (&in)[8]) {
+        uint32_t st[8];
+        if constexpr (N == 32) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) st[i] = ms[i];
+            sha256_tail<8>(st, in, 96);
+        } else {
+            uint32_t m[1 + NW + 8];
+            m[0] = PAD_PRF;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) m[1 + i] = key[i];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) m[1 + NW + i] = in[i];
+            sha256_iv(st);
+            sha256_tail<1 + NW + 8>(st, m, 4 + N + 32);
+        }
+#pragma unroll
+        for (int i = 0; i < NW; ++i) out.w[i] = st[i];
+    }
+
+    __device__ static void prf_keygen(Node& out, const Key& k, const uint32_t (&a)[8]) {
+        uint32_t st[8];
+        if constexpr (N == 32) {
+            uint32_t m[16];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) { m[i] = k.pub[i]; m[8 + i] = a[i]; }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) st[i] = k.ms_sks[i];
+            sha256_tail<16>(st, m, 128);
+        } else {
+            uint32_t m[1 + 2 * NW + 8];
+            m[0] = PAD_PRF_KEYGEN;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) { m[1 + i] = k.sks[i]; m[1 + NW + i] = k.pub[i]; }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) m[1 + 2 * NW + i] = a[i];
+            sha256_iv(st);
+            sha256_tail<1 + 2 * NW + 8>(st, m, 4 + 2 * N + 32);
+        }
+#pragma unroll
+        for (int i = 0; i < NW; ++i) out.w[i] = st[i];
+    }
+
+    __device__ static void thash_f(Node& out, const Key& k, uint32_t (&a)[8], const Node& x) {
+        Node key, mask;
+        a[7] = 0;
+        prf8(key, k.ms_pub, k.pub, a);
+        a[7] = 1;
+        prf8(mask, k.ms_pub, k.pub, a);
+        constexpr int P = (N == 32) ? 8 : 1;
+        uint32_t m[P + 2 * NW];
+#pragma unroll
+        for (int i = 0; i < P; ++i) m[i] = 0;  // toByte(0, pad)
+#pragma unroll
+        for (int i = 0; i < NW; ++i) { m[P + i] = key.w[i]; m[P + NW + i] = x.w[i] ^ mask.w[i]; }
+        uint32_t st[8];
+        sha256_iv(st);
+        sha256_tail<P + 2 * NW>(st, m, 4 * P + 2 * N);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) out.w[i] = st[i];
+    }
+
+    __device__ static void thash_h(Node& out, const Key& k, uint32_t (&a)[8], const Node& l,
+                                   const Node& r) {
+        Node key, m0, m1;
+        a[7] = 0;
+        prf8(key, k.ms_pub, k.pub, a);
+        a[7] = 1;
+        prf8(m0, k.ms_pub, k.pub, a);
+        a[7] = 2;
+        prf8(m1, k.ms_pub, k.pub, a);
+        constexpr int P = (N == 32) ? 8 : 1;
+        uint32_t m[P + 3 * NW];
+#pragma unroll
+        for (int i = 0; i < P - 1; ++i) m[i] = 0;
+        m[P - 1] = PAD_H;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            m[P + i] = key.w[i];
+            m[P + NW + i] = l.w[i] ^ m0.w[i];
+            m[P + 2 * NW + i] = r.w[i] ^ m1.w[i];
+        }
+        uint32_t st[8];
+        sha256_iv(st);
+        sha256_tail<P + 3 * NW>(st, m, 4 * P + 3 * N);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) out.w[i] = st[i];
+    }
+
+    __device__ static void prf_idx(Node& out, const Key& k, uint64_t idx) {
+        uint32_t in[8] = {0, 0, 0, 0, 0, 0, (uint32_t)(idx >> 32), (uint32_t)idx};
+        prf8(out, k.ms_prfk, k.prfk, in);
+    }
+
+    // SHA-256 of len bytes at p (p 4-byte aligned, readable up to the next
+    // multiple of 4), output truncated to n bytes.
+    __device__ static void hash_bytes(Node& out, const uint8_t* p, uint32_t len) {
+        uint32_t st[8];
+        sha256_iv(st);
+        const uint32_t nb = (len + 9 + 63) / 64;
+        const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+        for (uint32_t b = 0; b < nb; ++b) {
+            uint32_t w[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const uint32_t o = 64 * b + 4 * j;
+                uint32_t v;
+                if (o + 4 <= len) {
+                    v = bswap32(q[o / 4]);
+                } else if (o < len) {
+                    const uint32_t keep = len - o;  // 1..3 bytes
+                    v = bswap32(q[o / 4]) & (0xFFFFFFFFu << (32 - 8 * keep));
+                    v |= 0x80u << (24 - 8 * keep);
+                } else if (o == len) {
+                    v = 0x80000000u;
+                } else {
+                    v = 0;
+                }
+                w[j] = v;
+            }
+            if (b == nb - 1) {
+                w[14] = 0;
+                w[15] = len * 8u;
+            }
+            sha256_compress(st, w);
+        }
+#pragma unroll
+        for (int i = 0; i < NW; ++i) out.w[i] = st[i];
+    }
+};
+
+// ===========================================================================
+// SHA-512, n = 64 (pad 64), vendored self_SHA512 in the reference
+// (hash.c:54-56).  Node words are big-endian 64-bit values.
+// ===========================================================================
+struct Sha512Fam {
+    static constexpr int kN = 64;
+    static constexpr int NW = 8;
+    static constexpr int LEN = 8 * 64 / 4 + 3;  // 131
+    using Word = uint64_t;
+    struct Node { uint64_t w[8]; };
+    struct Key {
+        uint64_t pub[8], sks[8], prfk[8];
+        uint64_t ms_pub[8], ms_sks[8], ms_prfk[8];
+    };
+
+    __device__ static uint64_t bswap64(uint64_t x) { return __builtin_bswap64(x); }
+    __device__ static void load(Node& x, const uint8_t* p) {
+        const uint64_t* q = reinterpret_cast<const uint64_t*>(p);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x.w[i] = bswap64(q[i]);
+    }
+    __device__ static void store(const Node& x, uint8_t* p) {
+        uint64_t* q = reinterpret_cast<uint64_t*>(p);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) q[i] = bswap64(x.w[i]);
+    }
+    __device__ static void adrs64(uint64_t (&o)[4], const uint32_t (&a)[8]) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o[i] = ((uint64_t)a[2 * i] << 32) | a[2 * i + 1];
+    }
+    __device__ static void midstate(uint64_t out[8], uint64_t pad, const uint64_t k[8]) {
+        uint64_t w[16];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) w[i] = 0;
+        w[7] = pad;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) w[8 + i] = k[i];
+        sha512_iv(out);
+        sha512_compress(out, w);
+    }
+    __device__ static void prepare(Key& k) {
+        midstate(k.ms_pub, PAD_PRF, k.pub);
+        midstate(k.ms_sks, PAD_PRF_KEYGEN, k.sks);
+        midstate(k.ms_prfk, PAD_PRF, k.prfk);
+    }
+    __device__ static void prf4(Node& out, const uint64_t ms[8], const uint64_t (&in)[4]) {
+        uint64_t st[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) st[i] = ms[i];
+        sha512_tail<4>(st, in, 128 + 32);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) out.w[i] = st[i];
+    }
+    __device__ static void prf_keygen(Node& out, const Key& k, const uint32_t (&a)[8]) {
+        uint64_t m[12], ad[4];
+        adrs64(ad, a);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) m[i] = k.pub[i];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) m[8 + i] = ad[i];
+        uint64_t st[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) st[i] = k.ms_sks[i];
+        sha512_tail<12>(st, m, 128 + 96);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) out.w[i] = st[i];
+    }
+    __device__ static void thash_f(Node& out, const Key& k, uint32_t (&a)[8], const Node& x) {
+        Node key, mask;
+        uint64_t ad[4];
+        a[7] = 0;
+        adrs64(ad, a);
+        prf4(key, k.ms_pub, ad);
+        a[7] = 1;
+        adrs64(ad, a);
+        prf4(mask, k.ms_pub, ad);
+        uint64_t m[24];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) { m[i] = 0; m[8 + i] = key.w[i]; m[16 + i] = x.w[i] ^ mask.w[i]; }
+        uint64_t st[8];
+        sha512_iv(st);
+        sha512_tail<24>(st, m, 192);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) out.w[i] = st[i];
+    }
+    __device__ static void thash_h(Node& out, const Key& k, uint32_t (&a)[8], const Node& l,
+                                   const Node& r) {
+        Node key, m0, m1;
+        uint64_t ad[4];
+        a[7] = 0;
+        adrs64(ad, a);
+        prf4(key, k.ms_pub, ad);
+        a[7] = 1;
+        adrs64(ad, a);
+        prf4(m0, k.ms_pub, ad);
+        a[7] = 2;
+        adrs64(ad, a);
+        prf4(m1, k.ms_pub, ad);
+        uint64_t m[32];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            m[i] = (i == 7) ? (uint64_t)PAD_H : 0;
+            m[8 + i] = key.w[i];
+            m[16 + i] = l.w[i] ^ m0.w[i];
+            m[24 + i] = r.w[i] ^ m1.w[i];
+        }
+        uint64_t st[8];
+        sha512_iv(st);
+        sha512_tail<32>(st, m, 256);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) out.w[i] = st[i];
+    }
+    __device__ static void prf_idx(Node& out, const Key& k, uint64_t idx) {
+        uint64_t in[4] = {0, 0, 0, idx};
+        prf4(out, k.ms_prfk, in);
+    }
+    __device__ static void hash_bytes(Node& out, const uint8_t* p, uint32_t len) {
+        uint64_t st[8];
+        sha512_iv(st);
+        const uint32_t nb = (len + 17 + 127) / 128;
+        const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+        for (uint32_t b = 0; b < nb; ++b) {
+            uint64_t w[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                uint64_t v = 0;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {  // two 32-bit halves, big-endian
+                    const uint32_t o = 128 * b + 8 * j + 4 * h;
+                    uint32_t u;
+                    if (o + 4 <= len) {
+                        u = bswap32(q[o / 4]);
+                    } else if (o < len) {
+                        const uint32_t keep = len - o;
+                        u = (bswap32(q[o / 4]) & (0xFFFFFFFFu << (32 - 8 * keep))) | (0x80u << (24 - 8 * keep));
+                    } else if (o == len) {
+                        u = 0x80000000u;
+                    } else {
+                        u = 0;
+                    }
+                    v |= (uint64_t)u << (h ? 0 : 32);
+                }
+                w[j] = v;
+            }
+            if (b == nb - 1) {
+                w[14] = 0;
+                w[15] = (uint64_t)len * 8u;
+            }
+            sha512_compress(st, w);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) out.w[i] = st[i];
+    }
+};
+
+// ===========================================================================
+// SHAKE128 (rate 168) / SHAKE256 (rate 136), n = 24 (pad 4), 32, 64
+// (hash.c:41-59).  Messages and nodes are kept as little-endian 32-bit words
+// of the byte string; pairs of words form the 64-bit Keccak lanes.
+// ===========================================================================
+template <int RATE, int N>
+struct ShakeFam {
+    static constexpr int kN = N;
+    static constexpr int NW = N / 4;
+    static constexpr int LEN = 8 * N / 4 + 3;
+    static constexpr int PADW = (N == 24) ? 1 : N / 4;  // padding_len in words
+    using Word = uint32_t;
+    struct Node { uint32_t w[NW]; };
+    struct Key { uint32_t pub[NW], sks[NW], prfk[NW]; };
+
+    __device__ static void load(Node& x, const uint8_t* p) {
+        const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) x.w[i] = q[i];
+    }
+    __device__ static void store(const Node& x, uint8_t* p) {
+        uint32_t* q = reinterpret_cast<uint32_t*>(p);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) q[i] = x.w[i];
+    }
+    __device__ static void prepare(Key&) {}
+
+    // SHAKE over a message of M words (4*M bytes, compile-time length).
+    template <int M>
+    __device__ static void shake_words(Node& out, const uint32_t (&m)[M]) {
+        constexpr int LEN_B = 4 * M;
+        constexpr int NBLK = LEN_B / RATE + 1;
+        uint64_t s[25];
+#pragma unroll
+        for (int i = 0; i < 25; ++i) s[i] = 0;
+#pragma unroll
+        for (int b = 0; b < NBLK; ++b) {
+#pragma unroll
+            for (int j = 0; j < RATE / 8; ++j) {
+                uint32_t lohi[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int byte = b * RATE + 8 * j + 4 * h;
+                    uint32_t v = (byte / 4 < M) ? m[byte / 4] : 0u;
+                    // domain separation 0x1F right after the message, 0x80 in
+                    // the last byte of the last block (pad10*1)
+                    if (b == NBLK - 1) {
+                        if (byte <= LEN_B && LEN_B < byte + 4) v ^= 0x1Fu << (8 * (LEN_B - byte));
+                        if (byte <= NBLK * RATE - 1 && NBLK * RATE - 1 < byte + 4)
+                            v ^= 0x80u << (8 * (NBLK * RATE - 1 - byte));
+                    }
+                    lohi[h] = v;
+                }
+                s[j] ^= ((uint64_t)lohi[1] << 32) | lohi[0];
+            }
+            keccak_f1600(s);
+        }
+#pragma unroll
+        for (int i = 0; i < NW; ++i) out.w[i] = (i & 1) ? (uint32_t)(s[i / 2] >> 32) : (uint32_t)s[i / 2];
+    }
+
+    __device__ static void pad_words(uint32_t* m, uint32_t v) {
+#pragma unroll
+        for (int i = 0; i < PADW - 1; ++i) m[i] = 0;
+        m[PADW - 1] = v << 24;  // toByte(v, pad): last byte holds v
+    }
+    __device__ static void adrs_le(uint32_t* m, const uint32_t (&a)[8]) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) m[i] = bswap32(a[i]);
+    }
+    __device__ static void prf8(Node& out, const uint32_t key[NW], const uint32_t (&in_le)[8]) {
+        uint32_t m[PADW + NW + 8];
+        pad_words(m, PAD_PRF);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) m[PADW + i] = key[i];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) m[PADW + NW + i] = in_le[i];
+        shake_words(out, m);
+    }
+    __device__ static void prf_keygen(Node& out, const Key& k, const uint32_t (&a)[8]) {
+        uint32_t m[PADW + 2 * NW + 8];
+        pad_words(m, PAD_PRF_KEYGEN);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) { m[PADW + i] = k.sks[i]; m[PADW + NW + i] = k.pub[i]; }
+        uint32_t al[8];
+        adrs_le(al, a);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) m[PADW + 2 * NW + i] = al[i];
+        shake_words(out, m);
+    }
+    __device__ static void thash_f(Node& out, const Key& k, uint32_t (&a)[8], const Node& x) {
+        Node key, mask;
+        uint32_t al[8];
+        a[7] = 0;
+        adrs_le(al, a);
+        prf8(key, k.pub, al);
+        a[7] = 1;
+        adrs_le(al, a);
+        prf8(mask, k.pub, al);
+        uint32_t m[PADW + 2 * NW];
+        pad_words(m, PAD_F);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) { m[PADW + i] = key.w[i]; m[PADW + NW + i] = x.w[i] ^ mask.w[i]; }
+        shake_words(out, m);
+    }
+    __device__ static void thash_h(Node& out, const Key& k, uint32_t (&a)[8], const Node& l,
+                                   const Node& r) {
+        Node key, m0, m1;
+        uint32_t al[8];
+        a[7] = 0;
+        adrs_le(al, a);
+        prf8(key, k.pub, al);
+        a[7] = 1;
+        adrs_le(al, a);
+        prf8(m0, k.pub, al);
+        a[7] = 2;
+        adrs_le(al, a);
+        prf8(m1, k.pub, al);
+        uint32_t m[PADW + 3 * NW];
+        pad_words(m, PAD_H);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            m[PADW + i] = key.w[i];
+            m[PADW + NW + i] = l.w[i] ^ m0.w[i];
+            m[PADW + 2 * NW + i] = r.w[i] ^ m1.w[i];
+        }
+        shake_words(out, m);
+    }
+    __device__ static void prf_idx(Node& out, const Key& k, uint64_t idx) {
+        uint32_t in[8] = {0, 0, 0, 0, 0, 0, bswap32((uint32_t)(idx >> 32)), bswap32((uint32_t)idx)};
+        prf8(out, k.prfk, in);
+    }
+    __device__ static void hash_bytes(Node& out, const uint8_t* p, uint32_t len) {
+        uint64_t s[25];
+#pragma unroll
+        for (int i = 0; i < 25; ++i) s[i] = 0;
+        const uint32_t nb = len / RATE + 1;
+        const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+        for (uint32_t b = 0; b < nb; ++b) {
+#pragma unroll
+            for (int j = 0; j < RATE / 8; ++j) {
+                uint32_t lohi[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const uint32_t o = b * RATE + 8 * j + 4 * h;
+                    uint32_t v;
+                    if (o + 4 <= len) v = q[o / 4];
+                    else if (o < len) v = q[o / 4] & (0xFFFFFFFFu >> (32 - 8 * (len - o)));
+                    else v = 0;
+                    if (o <= len && len < o + 4) v ^= 0x1Fu << (8 * (len - o));
+                    const uint32_t last = nb * RATE - 1;
+                    if (o <= last && last < o + 4) v ^= 0x80u << (8 * (last - o));
+                    lohi[h] = v;
+                }
+                s[j] ^= ((uint64_t)lohi[1] << 32) | lohi[0];
+            }
+            keccak_f1600(s);
+        }
+#pragma unroll
+        for (int i = 0; i < NW; ++i) out.w[i] = (i & 1) ? (uint32_t)(s[i / 2] >> 32) : (uint32_t)s[i / 2];
+    }
+};
+
+}  // namespace xm

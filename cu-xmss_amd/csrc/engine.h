@@ -1,0 +1,182 @@
+// engine.h -- host-side interface of the device engine (internal to the
+// library; the public C ABI lives in include/).
+//
+// The engine owns one HIP stream on the current device and a grow-only
+// workspace.  Every call is synchronous: inputs are host byte arrays, outputs
+// are written back to host byte arrays before the call returns.  The cores
+// (core_plain.cpp, core_bds.cpp, batch.cpp) express all hashing through these
+// calls; no hash is ever computed on the host.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace xm {
+
+// Hash families: the (func, n) pairs accepted by core_hash (cuxmss/hash.c:29-65).
+enum Fam : int {
+    F_SHA2_24 = 0,
+    F_SHA2_32,
+    F_SHA2_64,
+    F_SHAKE128_32,
+    F_SHAKE256_24,
+    F_SHAKE256_32,
+    F_SHAKE256_64,
+    F_COUNT
+};
+
+// func codes of xmss_params.func (params.h:7-9)
+int fam_of(unsigned func, unsigned n);  // -1 when core_hash would fail
+unsigned fam_n(int fam);
+unsigned fam_len(int fam);              // wots_len for w = 16
+unsigned fam_pad(int fam);              // padding_len
+
+struct KeySeeds {
+    const uint8_t* sk_seed;  // n bytes or nullptr (zeros)
+    const uint8_t* sk_prf;
+    const uint8_t* pub_seed;
+};
+
+// A run of leaves of one (layer, tree) of one key: leaves leaf0 .. leaf0+count-1.
+struct LeafRun {
+    uint32_t key;
+    uint32_t layer;
+    uint64_t tree;
+    uint32_t leaf0;
+};
+
+// A node of a built subtree: tree number in the last build_trees call,
+// height (0 = leaves) and index at that height.
+struct NodeRef {
+    uint32_t tree;
+    uint32_t level;
+    uint32_t index;
+};
+
+enum WotsMode : uint32_t { WOTS_PKGEN = 0, WOTS_SIGN = 1, WOTS_VERIFY = 2 };
+
+struct WotsReq {
+    uint32_t key, layer;
+    uint64_t tree;
+    uint32_t ots;
+    uint32_t mode;  // WotsMode
+};
+
+struct LtreeReq {
+    uint32_t key, layer;
+    uint64_t tree;
+    uint32_t ltree;
+};
+
+struct RootReq {
+    uint32_t key, layer;
+    uint64_t tree;
+    uint32_t leaf_idx;
+};
+
+struct ThashReq {
+    uint32_t key;
+    uint32_t adrs[8];
+};
+
+// H_msg request: byte string prefix || M with the prefix laid out as in
+// hash_message (cuxmss/hash.c:107-120).  For sign requests R is computed on
+// the device from SK_PRF and idx and written into the prefix first.
+struct MsgReq {
+    uint32_t key;
+    uint32_t sign;
+    uint64_t idx;
+    const uint8_t* R;     // verify: n bytes from the signature; sign: unused
+    const uint8_t* root;  // n bytes
+    const uint8_t* m;
+    uint64_t mlen;
+};
+
+class Engine {
+public:
+    // The engine of the current HIP device, created on first use.  Returns
+    // nullptr (and prints why) when no usable GPU is present.
+    static Engine* get();
+
+    void set_keys(int fam, int nkeys, const KeySeeds* seeds);
+
+    // Full subtrees of height th, one per run (leaf0 must be 0).  All node
+    // layers stay on the device for fetch_nodes until the next call.
+    void build_trees(int fam, unsigned th, int nruns, const LeafRun* runs);
+    void fetch_nodes(int fam, unsigned th, int nrefs, const NodeRef* refs, uint8_t* out);
+
+    // gen_leaf_wots for `count` consecutive leaves of each run (xmss_commons.c:104-113).
+    void leaves(int fam, int nruns, const LeafRun* runs, unsigned count, uint8_t* out);
+
+    void wots(int fam, int n, const WotsReq* reqs, const uint8_t* msgs, const uint8_t* in,
+              uint8_t* out);
+    // One layer of signature verification per request (xmss_commons.c:362-375):
+    // WOTS pk from the signature, L-tree, then the auth-path walk -> root.
+    void verify_layer(int fam, unsigned th, int n, const RootReq* reqs, const uint8_t* msgs,
+                      const uint8_t* sigs, const uint8_t* auth, uint8_t* roots);
+    void ltree(int fam, int n, const LtreeReq* reqs, const uint8_t* pks, uint8_t* out);
+    void compute_root(int fam, unsigned th, int n, const RootReq* reqs, const uint8_t* leaves,
+                      const uint8_t* auth, uint8_t* out);
+    void thash_h(int fam, int n, const ThashReq* reqs, const uint8_t* in, uint8_t* out);
+    void hmsg(int fam, int n, const MsgReq* reqs, uint8_t* R_out, uint8_t* digest_out);
+
+    // ---- device-resident batches (plain-core sign, verify) ----
+    // A batch is staged to the device by *_prepare, computed by run() with no
+    // host traffic, and read back by collect().  All requests of a batch share
+    // one parameter set.
+    struct SignItem {
+        const uint8_t* sk_seed;
+        const uint8_t* sk_prf;
+        const uint8_t* pub_seed;
+        const uint8_t* root;  // pub root (H_msg input)
+        uint64_t idx;         // signature index
+        const uint8_t* m;
+        uint64_t mlen;
+    };
+    struct VerifyItem {
+        const uint8_t* pk;  // root || PUB_SEED
+        uint64_t idx;
+        const uint8_t* R;
+        const uint8_t* wots;  // d consecutive (WOTS sig || auth path) layers, as in sm
+        const uint8_t* m;
+        uint64_t mlen;
+    };
+    struct Batch;
+    Batch* sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items);
+    Batch* verify_prepare(int fam, unsigned th, unsigned d, int count, const VerifyItem* items);
+    void run(Batch* b);
+    // sign: per item R (n) || d x (WOTS sig || auth path) written to out[i]
+    //       (sig_bytes - index_bytes bytes each).
+    // verify: ok[i] = 1 when the recomputed root equals the pk root.
+    void collect_sign(Batch* b, uint8_t* const* out);
+    void collect_verify(Batch* b, int* ok);
+    void free_batch(Batch* b);
+
+    // Per-kernel HIP-event timing (off by default).
+    void set_profiling(bool on);
+    bool profiling() const { return prof_on_; }
+    struct KStat {
+        std::string name;
+        uint64_t count;
+        double total_ms;
+    };
+    std::vector<KStat> stats() const;
+    void reset_stats();
+
+    void* stream() const { return stream_; }
+    int device() const { return dev_; }
+
+    // implementation detail, public for the launch helpers
+    struct Impl;
+    Impl* impl_;
+
+private:
+    Engine();
+    void* stream_ = nullptr;
+    int dev_ = 0;
+    bool prof_on_ = false;
+};
+
+}  // namespace xm

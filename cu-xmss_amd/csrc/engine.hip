@@ -1,0 +1,656 @@
+// engine.hip -- the device engine: stream, grow-only workspace, host <-> device
+// staging, and the sequencing of kernel launches for each operation.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+
+#include "engine.h"
+#include "launch.h"
+
+namespace xm {
+
+[[noreturn]] void fatal(const char* what) {
+    fprintf(stderr, "xmss_mi355x: fatal: %s\n", what);
+    fflush(stderr);
+    abort();
+}
+
+int fam_of(unsigned func, unsigned n) {
+    // the (func, n) pairs core_hash accepts (cuxmss/hash.c:36-63)
+    if (func == 0 && n == 24) return F_SHA2_24;
+    if (func == 0 && n == 32) return F_SHA2_32;
+    if (func == 0 && n == 64) return F_SHA2_64;
+    if (func == 1 && n == 32) return F_SHAKE128_32;
+    if (func == 2 && n == 24) return F_SHAKE256_24;
+    if (func == 2 && n == 32) return F_SHAKE256_32;
+    if (func == 2 && n == 64) return F_SHAKE256_64;
+    return -1;
+}
+unsigned fam_n(int fam) {
+    static const unsigned n[F_COUNT] = {24, 32, 64, 32, 24, 32, 64};
+    return n[fam];
+}
+unsigned fam_len(int fam) { return 2 * fam_n(fam) + 3; }
+unsigned fam_pad(int fam) { return fam_n(fam) == 24 ? 4 : fam_n(fam); }
+
+static FamOps* ops_of(int fam) {
+    switch (fam) {
+        case F_SHA2_24: return ops_sha2_24();
+        case F_SHA2_32: return ops_sha2_32();
+        case F_SHA2_64: return ops_sha2_64();
+        case F_SHAKE128_32: return ops_shake128_32();
+        case F_SHAKE256_24: return ops_shake256_24();
+        case F_SHAKE256_32: return ops_shake256_32();
+        case F_SHAKE256_64: return ops_shake256_64();
+    }
+    fatal("unsupported hash family");
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    template <class T = uint8_t>
+    T* need(size_t bytes) {
+        bytes = std::max<size_t>(bytes, 64);
+        if (bytes > cap) {
+            if (p) HIPCK(hipFree(p));
+            size_t c = std::max(bytes, cap + cap / 2);
+            c = (c + 4095) & ~(size_t)4095;
+            HIPCK(hipMalloc(&p, c));
+            cap = c;
+        }
+        return (T*)p;
+    }
+};
+
+struct Engine::Impl {
+    Launcher L;
+    Engine::Batch* spare[2] = {nullptr, nullptr};  // reusable verify / sign batches
+    DevBuf keys, seeds, desc, ends, tmp, nodes, a, b, c, d, idx;
+    int key_fam = -1;
+    int nkeys = 0;
+    void sync() {
+        HIPCK(hipStreamSynchronize(L.s));
+        L.harvest();
+    }
+    void h2d(void* dst, const void* src, size_t n) {
+        if (n) HIPCK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, L.s));
+    }
+    void d2h(void* dst, const void* src, size_t n) {
+        if (n) HIPCK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, L.s));
+    }
+    FamOps* ops(int fam) {
+        if (fam != key_fam) fatal("engine call for a hash family other than the loaded keys");
+        return ops_of(fam);
+    }
+};
+
+Engine::Engine() : impl_(new Impl) {
+    HIPCK(hipGetDevice(&dev_));
+    HIPCK(hipStreamCreateWithFlags(&impl_->L.s, hipStreamNonBlocking));
+    stream_ = impl_->L.s;
+}
+
+Engine* Engine::get() {
+    static std::mutex mu;
+    static std::map<int, Engine*> engines;
+    std::lock_guard<std::mutex> lk(mu);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        fprintf(stderr, "xmss_mi355x: no HIP device available (this library has no CPU path)\n");
+        fflush(stderr);
+        return nullptr;
+    }
+    int dev = 0;
+    HIPCK(hipGetDevice(&dev));
+    auto it = engines.find(dev);
+    if (it != engines.end()) return it->second;
+    Engine* e = new Engine();
+    engines[dev] = e;
+    return e;
+}
+
+void Engine::set_profiling(bool on) {
+    prof_on_ = on;
+    impl_->L.prof = on;
+}
+std::vector<Engine::KStat> Engine::stats() const {
+    std::vector<KStat> v;
+    for (auto& kv : impl_->L.stat) v.push_back({kv.first, kv.second.first, kv.second.second});
+    return v;
+}
+void Engine::reset_stats() { impl_->L.stat.clear(); }
+
+void Engine::set_keys(int fam, int nkeys, const KeySeeds* seeds) {
+    Impl* I = impl_;
+    const unsigned n = fam_n(fam);
+    std::vector<uint8_t> h((size_t)nkeys * 3 * n, 0);
+    for (int i = 0; i < nkeys; ++i) {
+        uint8_t* d = &h[(size_t)i * 3 * n];
+        if (seeds[i].sk_seed) memcpy(d, seeds[i].sk_seed, n);
+        if (seeds[i].sk_prf) memcpy(d + n, seeds[i].sk_prf, n);
+        if (seeds[i].pub_seed) memcpy(d + 2 * n, seeds[i].pub_seed, n);
+    }
+    uint8_t* ds = I->seeds.need(h.size());
+    I->h2d(ds, h.data(), h.size());
+    FamOps* o = ops_of(fam);
+    void* dk = I->keys.need(o->key_bytes() * (size_t)nkeys);
+    o->prepare(I->L, ds, nkeys, dk);
+    I->key_fam = fam;
+    I->nkeys = nkeys;
+}
+
+static std::vector<DRun> pack_runs(int n, const LeafRun* runs) {
+    std::vector<DRun> v(n);
+    for (int i = 0; i < n; ++i) {
+        v[i] = {runs[i].key, runs[i].layer, (uint32_t)(runs[i].tree >> 32), (uint32_t)runs[i].tree,
+                runs[i].leaf0, 0, 0, 0};
+    }
+    return v;
+}
+
+// Byte budget of the chain-end staging buffer per launch group.
+static const size_t kEndsBudget = (size_t)1 << 30;
+
+void Engine::build_trees(int fam, unsigned th, int nruns, const LeafRun* runs) {
+    Impl* I = impl_;
+    FamOps* o = I->ops(fam);
+    const unsigned n = fam_n(fam), len = fam_len(fam);
+    const uint64_t stride = (2ull << th) - 1;
+    const uint64_t leaves = 1ull << th;
+    const uint64_t per_leaf = (uint64_t)len * n;
+    std::vector<DRun> d = pack_runs(nruns, runs);
+    DRun* drun = I->desc.need<DRun>(sizeof(DRun) * d.size());
+    I->h2d(drun, d.data(), sizeof(DRun) * d.size());
+    uint8_t* nodes = I->nodes.need(stride * n * (size_t)nruns);
+    const void* dk = I->keys.p;
+    for (int r0 = 0; r0 < nruns;) {
+        int rg = std::min<int>(nruns - r0, 65535);
+        uint64_t lpp = leaves;
+        while ((uint64_t)rg * lpp * per_leaf > kEndsBudget && lpp > 64) lpp >>= 1;
+        while ((uint64_t)rg * lpp * per_leaf > kEndsBudget && rg > 1) rg = (rg + 1) / 2;
+        uint8_t* ends = I->ends.need((size_t)rg * lpp * per_leaf);
+        uint8_t* tmp = I->tmp.need((size_t)rg * lpp * ((len + 1) / 2) * n);
+        uint8_t* tnodes = nodes + (size_t)r0 * stride * n;
+        for (uint64_t l0 = 0; l0 < leaves; l0 += lpp) {
+            const uint32_t cnt = (uint32_t)std::min<uint64_t>(lpp, leaves - l0);
+            o->chains(I->L, dk, drun + r0, rg, (uint32_t)l0, cnt, ends);
+            // L-tree height by height, ping-ponging ends <-> tmp
+            uint32_t l = len, height = 0;
+            bool from_a = true;
+            const uint32_t sa = len, sb = (len + 1) / 2;
+            while (l > 1) {
+                o->ltree_level(I->L, dk, drun + r0, rg, (uint32_t)l0, cnt, l, height, from_a ? ends : tmp,
+                               from_a ? sa : sb, from_a ? tmp : ends, from_a ? sb : sa, tnodes, stride, l0);
+                from_a = !from_a;
+                l = (l + 1) / 2;
+                ++height;
+            }
+        }
+        for (unsigned lev = 0; lev < th; ++lev) o->level(I->L, dk, drun + r0, rg, th, lev, tnodes);
+        r0 += rg;
+    }
+    I->sync();
+}
+
+void Engine::fetch_nodes(int fam, unsigned th, int nrefs, const NodeRef* refs, uint8_t* out) {
+    Impl* I = impl_;
+    const unsigned n = fam_n(fam);
+    const uint64_t stride = (2ull << th) - 1;
+    std::vector<uint64_t> idx(nrefs);
+    for (int i = 0; i < nrefs; ++i) {
+        idx[i] = refs[i].tree * stride + ((2ull << th) - (2ull << (th - refs[i].level))) + refs[i].index;
+    }
+    uint64_t* didx = I->idx.need<uint64_t>(sizeof(uint64_t) * (size_t)nrefs);
+    uint8_t* dout = I->b.need((size_t)n * nrefs);
+    I->h2d(didx, idx.data(), sizeof(uint64_t) * nrefs);
+    I->L.run("gather", k_gather, dim3(cdiv((uint64_t)nrefs * (n / 4), WG)), dim3(WG),
+             (const uint8_t*)I->nodes.p, (const uint64_t*)didx, (uint32_t)nrefs, n, dout);
+    I->d2h(out, dout, (size_t)n * nrefs);
+    I->sync();
+}
+
+void Engine::leaves(int fam, int nruns, const LeafRun* runs, unsigned count, uint8_t* out) {
+    Impl* I = impl_;
+    FamOps* o = I->ops(fam);
+    const unsigned n = fam_n(fam);
+    const size_t nl = (size_t)nruns * count;
+    std::vector<DLeaf> d(nl);
+    for (int i = 0; i < nruns; ++i) {
+        for (unsigned j = 0; j < count; ++j) {
+            d[(size_t)i * count + j] = {runs[i].key, runs[i].layer, (uint32_t)(runs[i].tree >> 32),
+                                        (uint32_t)runs[i].tree, runs[i].leaf0 + j, 0, 0, 0};
+        }
+    }
+    DLeaf* dd = I->desc.need<DLeaf>(sizeof(DLeaf) * nl);
+    uint8_t* dout = I->b.need(nl * n);
+    I->h2d(dd, d.data(), sizeof(DLeaf) * nl);
+    o->leaf_wg(I->L, I->keys.p, dd, (uint32_t)nl, false, nullptr, nullptr, nullptr, 0, dout);
+    I->d2h(out, dout, nl * n);
+    I->sync();
+}
+
+void Engine::verify_layer(int fam, unsigned th, int nreq, const RootReq* reqs, const uint8_t* msgs,
+                          const uint8_t* sigs, const uint8_t* auth, uint8_t* roots) {
+    Impl* I = impl_;
+    FamOps* o = I->ops(fam);
+    const unsigned n = fam_n(fam), len = fam_len(fam);
+    std::vector<DLeaf> d(nreq);
+    for (int i = 0; i < nreq; ++i) {
+        d[i] = {reqs[i].key, reqs[i].layer, (uint32_t)(reqs[i].tree >> 32), (uint32_t)reqs[i].tree,
+                reqs[i].leaf_idx, 0, 0, 0};
+    }
+    DLeaf* dd = I->desc.need<DLeaf>(sizeof(DLeaf) * nreq);
+    uint8_t* dm = I->a.need((size_t)nreq * n);
+    uint8_t* ds = I->c.need((size_t)nreq * len * n);
+    uint8_t* da = I->d.need((size_t)nreq * th * n);
+    uint8_t* dout = I->b.need((size_t)nreq * n);
+    I->h2d(dd, d.data(), sizeof(DLeaf) * nreq);
+    I->h2d(dm, msgs, (size_t)nreq * n);
+    I->h2d(ds, sigs, (size_t)nreq * len * n);
+    I->h2d(da, auth, (size_t)nreq * th * n);
+    o->leaf_wg(I->L, I->keys.p, dd, (uint32_t)nreq, true, dm, ds, da, th, dout);
+    I->d2h(roots, dout, (size_t)nreq * n);
+    I->sync();
+}
+
+void Engine::wots(int fam, int nreq, const WotsReq* reqs, const uint8_t* msgs, const uint8_t* in,
+                  uint8_t* out) {
+    Impl* I = impl_;
+    FamOps* o = I->ops(fam);
+    const unsigned n = fam_n(fam), len = fam_len(fam);
+    std::vector<DWots> d(nreq);
+    for (int i = 0; i < nreq; ++i) {
+        d[i] = {reqs[i].key, reqs[i].layer, (uint32_t)(reqs[i].tree >> 32), (uint32_t)reqs[i].tree,
+                reqs[i].ots, reqs[i].mode, 0, 0};
+    }
+    DWots* dd = I->desc.need<DWots>(sizeof(DWots) * d.size());
+    uint8_t* dm = I->a.need((size_t)nreq * n);
+    uint8_t* din = I->c.need((size_t)nreq * len * n);
+    uint8_t* dout = I->b.need((size_t)nreq * len * n);
+    I->h2d(dd, d.data(), sizeof(DWots) * d.size());
+    I->h2d(dm, msgs, (size_t)nreq * n);
+    if (in) I->h2d(din, in, (size_t)nreq * len * n);
+    o->wots(I->L, I->keys.p, dd, (uint32_t)nreq, dm, din, dout);
+    I->d2h(out, dout, (size_t)nreq * len * n);
+    I->sync();
+}
+
+void Engine::thash_h(int fam, int nreq, const ThashReq* reqs, const uint8_t* in, uint8_t* out) {
+    Impl* I = impl_;
+    FamOps* o = I->ops(fam);
+    const unsigned n = fam_n(fam);
+    std::vector<DThash> d(nreq);
+    for (int i = 0; i < nreq; ++i) {
+        memset(&d[i], 0, sizeof(DThash));
+        d[i].key = reqs[i].key;
+        memcpy(d[i].adrs, reqs[i].adrs, 32);
+    }
+    DThash* dd = I->desc.need<DThash>(sizeof(DThash) * d.size());
+    uint8_t* di = I->a.need((size_t)nreq * 2 * n);
+    uint8_t* dout = I->b.need((size_t)nreq * n);
+    I->h2d(dd, d.data(), sizeof(DThash) * d.size());
+    I->h2d(di, in, (size_t)nreq * 2 * n);
+    o->thash_h(I->L, I->keys.p, dd, (uint32_t)nreq, di, dout);
+    I->d2h(out, dout, (size_t)nreq * n);
+    I->sync();
+}
+
+void Engine::hmsg(int fam, int nreq, const MsgReq* reqs, uint8_t* R_out, uint8_t* dig_out) {
+    Impl* I = impl_;
+    FamOps* o = I->ops(fam);
+    const unsigned n = fam_n(fam), pad = fam_pad(fam);
+    const size_t pb = pad + 3 * n;
+    std::vector<DMsg> d(nreq);
+    size_t total = 0;
+    for (int i = 0; i < nreq; ++i) {
+        d[i] = {reqs[i].key, reqs[i].sign, reqs[i].idx, (uint64_t)total, (uint32_t)(pb + reqs[i].mlen), 0};
+        total += (pb + reqs[i].mlen + 7) & ~(size_t)7;
+    }
+    total += 8;
+    // toByte(2, pad) || R || root || toByte(idx, n) || M  (hash.c:107-120)
+    std::vector<uint8_t> blob(total, 0);
+    for (int i = 0; i < nreq; ++i) {
+        uint8_t* p = &blob[d[i].off];
+        p[pad - 1] = 2;
+        if (!reqs[i].sign && reqs[i].R) memcpy(p + pad, reqs[i].R, n);
+        memcpy(p + pad + n, reqs[i].root, n);
+        uint64_t v = reqs[i].idx;
+        for (int b = 0; b < 8; ++b) {
+            p[pad + 3 * n - 1 - b] = (uint8_t)(v & 0xff);
+            v >>= 8;
+        }
+        if (reqs[i].mlen) memcpy(p + pb, reqs[i].m, reqs[i].mlen);
+    }
+    DMsg* dd = I->desc.need<DMsg>(sizeof(DMsg) * d.size());
+    uint8_t* db = I->c.need(total);
+    uint8_t* dR = I->a.need((size_t)nreq * n);
+    uint8_t* dD = I->b.need((size_t)nreq * n);
+    I->h2d(dd, d.data(), sizeof(DMsg) * d.size());
+    I->h2d(db, blob.data(), total);
+    o->hmsg(I->L, I->keys.p, dd, (uint32_t)nreq, pad, db, dR, dD);
+    if (R_out) I->d2h(R_out, dR, (size_t)nreq * n);
+    I->d2h(dig_out, dD, (size_t)nreq * n);
+    I->sync();
+}
+
+__global__ __launch_bounds__(WG) void k_gather(const uint8_t* __restrict__ src,
+                                               const uint64_t* __restrict__ idx, uint32_t n,
+                                               uint32_t nbytes, uint8_t* __restrict__ dst) {
+    const uint32_t words = nbytes / 4;
+    const uint32_t g = blockIdx.x * WG + threadIdx.x;
+    if (g >= n * words) return;
+    const uint32_t item = g / words, w = g % words;
+    reinterpret_cast<uint32_t*>(dst)[(size_t)item * words + w] =
+        reinterpret_cast<const uint32_t*>(src)[idx[item] * words + w];
+}
+
+}  // namespace xm
+
+namespace xm {
+
+// Copies node i of a flat node array into out: used to lay out auth paths
+// and roots of built subtrees (see Batch::gather_idx).
+struct Engine::Batch {
+    bool sign = false;
+    int fam = 0;
+    unsigned th = 0, d = 0, n = 0, len = 0;
+    int count = 0;
+    DevBuf keys, seeds, msg_desc, blob, R, dig, runs, nodes, ends, tmp, gidx, auth, wdesc, wots, vdesc,
+        roots, ok, pkroot;
+    size_t nodes_stride = 0;
+};
+
+__global__ __launch_bounds__(WG) void k_compare(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b,
+                                                uint32_t n, uint32_t nbytes, int* __restrict__ ok) {
+    const uint32_t i = blockIdx.x * WG + threadIdx.x;
+    if (i >= n) return;
+    int eq = 1;
+    for (uint32_t j = 0; j < nbytes; ++j) eq &= (a[(size_t)i * nbytes + j] == b[(size_t)i * nbytes + j]);
+    ok[i] = eq;
+}
+
+static void stage_msgs(Engine::Impl* I, DevBuf& desc, DevBuf& blob, int fam, int count, bool sign,
+                       const std::vector<MsgReq>& reqs) {
+    const unsigned n = fam_n(fam), pad = fam_pad(fam);
+    const size_t pb = pad + 3 * n;
+    std::vector<DMsg> d(count);
+    size_t total = 0;
+    for (int i = 0; i < count; ++i) {
+        d[i] = {reqs[i].key, sign ? 1u : 0u, reqs[i].idx, (uint64_t)total, (uint32_t)(pb + reqs[i].mlen), 0};
+        total += (pb + reqs[i].mlen + 7) & ~(size_t)7;
+    }
+    total += 8;
+    std::vector<uint8_t> h(total, 0);
+    for (int i = 0; i < count; ++i) {
+        uint8_t* p = &h[d[i].off];
+        p[pad - 1] = 2;
+        if (!sign) memcpy(p + pad, reqs[i].R, n);
+        memcpy(p + pad + n, reqs[i].root, n);
+        uint64_t v = reqs[i].idx;
+        for (int b = 0; b < 8; ++b) {
+            p[pad + 3 * n - 1 - b] = (uint8_t)(v & 0xff);
+            v >>= 8;
+        }
+        if (reqs[i].mlen) memcpy(p + pb, reqs[i].m, reqs[i].mlen);
+    }
+    DMsg* dd = desc.need<DMsg>(sizeof(DMsg) * count);
+    uint8_t* db = blob.need(total);
+    I->h2d(dd, d.data(), sizeof(DMsg) * count);
+    I->h2d(db, h.data(), total);
+}
+
+static void stage_keys(Engine::Impl* I, Engine::Batch* b, const std::vector<KeySeeds>& ks) {
+    const unsigned n = b->n;
+    std::vector<uint8_t> h((size_t)ks.size() * 3 * n, 0);
+    for (size_t i = 0; i < ks.size(); ++i) {
+        uint8_t* d = &h[i * 3 * n];
+        if (ks[i].sk_seed) memcpy(d, ks[i].sk_seed, n);
+        if (ks[i].sk_prf) memcpy(d + n, ks[i].sk_prf, n);
+        if (ks[i].pub_seed) memcpy(d + 2 * n, ks[i].pub_seed, n);
+    }
+    uint8_t* ds = b->seeds.need(h.size());
+    I->h2d(ds, h.data(), h.size());
+    FamOps* o = ops_of(b->fam);
+    void* dk = b->keys.need(o->key_bytes() * ks.size());
+    o->prepare(I->L, ds, (int)ks.size(), dk);
+}
+
+static Engine::Batch* take_batch(Engine::Impl* I, bool sign) {
+    Engine::Batch* b = I->spare[sign];
+    I->spare[sign] = nullptr;
+    return b ? b : new Engine::Batch;
+}
+
+Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items) {
+    Impl* I = impl_;
+    Batch* b = take_batch(I, true);
+    b->sign = true;
+    b->fam = fam;
+    b->th = th;
+    b->d = d;
+    b->n = fam_n(fam);
+    b->len = fam_len(fam);
+    b->count = count;
+    const unsigned n = b->n;
+    std::vector<KeySeeds> ks(count);
+    std::vector<MsgReq> mr(count);
+    for (int i = 0; i < count; ++i) {
+        ks[i] = {items[i].sk_seed, items[i].sk_prf, items[i].pub_seed};
+        mr[i] = {(uint32_t)i, 1, items[i].idx, nullptr, items[i].root, items[i].m, items[i].mlen};
+    }
+    stage_keys(I, b, ks);
+    stage_msgs(I, b->msg_desc, b->blob, fam, count, true, mr);
+    // one subtree per (item, layer): layer i, tree idx >> (th * (i + 1))
+    std::vector<DRun> runs((size_t)count * d);
+    std::vector<DWots> wd((size_t)count * d);
+    std::vector<uint64_t> gi((size_t)count * d * (th + 1));
+    b->nodes_stride = (2ull << th) - 1;
+    for (int i = 0; i < count; ++i) {
+        uint64_t idx = items[i].idx;
+        for (unsigned L = 0; L < d; ++L) {
+            const uint32_t leaf = (uint32_t)(idx & ((1ull << th) - 1));
+            idx >>= th;
+            const size_t t = (size_t)L * count + i;  // layer-major
+            runs[t] = {(uint32_t)i, L, (uint32_t)(idx >> 32), (uint32_t)idx, 0, 0, 0, 0};
+            wd[t] = {(uint32_t)i, L, (uint32_t)(idx >> 32), (uint32_t)idx, leaf, MODE_SIGN, 0, 0};
+            // auth path (th nodes) then the root, gathered into auth[t]
+            for (unsigned j = 0; j < th; ++j) {
+                gi[t * (th + 1) + j] = t * b->nodes_stride + ((2ull << th) - (2ull << (th - j))) + ((leaf >> j) ^ 1);
+            }
+            gi[t * (th + 1) + th] = t * b->nodes_stride + b->nodes_stride - 1;
+        }
+    }
+    DRun* dr = b->runs.need<DRun>(sizeof(DRun) * runs.size());
+    I->h2d(dr, runs.data(), sizeof(DRun) * runs.size());
+    DWots* dw = b->wdesc.need<DWots>(sizeof(DWots) * wd.size());
+    I->h2d(dw, wd.data(), sizeof(DWots) * wd.size());
+    uint64_t* dg = b->gidx.need<uint64_t>(sizeof(uint64_t) * gi.size());
+    I->h2d(dg, gi.data(), sizeof(uint64_t) * gi.size());
+    b->R.need((size_t)count * n);
+    b->dig.need((size_t)count * n);
+    b->nodes.need(b->nodes_stride * n * (size_t)count * d);
+    b->auth.need(gi.size() * n);
+    b->wots.need((size_t)count * d * b->len * n);
+    b->roots.need((size_t)count * d * n);  // layer-major roots
+    I->sync();
+    return b;
+}
+
+Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int count, const VerifyItem* items) {
+    Impl* I = impl_;
+    Batch* b = take_batch(I, false);
+    b->sign = false;
+    b->fam = fam;
+    b->th = th;
+    b->d = d;
+    b->n = fam_n(fam);
+    b->len = fam_len(fam);
+    b->count = count;
+    const unsigned n = b->n, len = b->len;
+    std::vector<KeySeeds> ks(count);
+    std::vector<MsgReq> mr(count);
+    for (int i = 0; i < count; ++i) {
+        ks[i] = {nullptr, nullptr, items[i].pk + n};
+        mr[i] = {(uint32_t)i, 0, items[i].idx, items[i].R, items[i].pk, items[i].m, items[i].mlen};
+    }
+    stage_keys(I, b, ks);
+    stage_msgs(I, b->msg_desc, b->blob, fam, count, false, mr);
+    std::vector<DLeaf> vd((size_t)count * d);
+    std::vector<uint8_t> sigs((size_t)count * d * len * n), auth((size_t)count * d * th * n), pkr((size_t)count * n);
+    for (int i = 0; i < count; ++i) {
+        uint64_t idx = items[i].idx;
+        const uint8_t* p = items[i].wots;
+        for (unsigned L = 0; L < d; ++L) {
+            const uint32_t leaf = (uint32_t)(idx & ((1ull << th) - 1));
+            idx >>= th;
+            const size_t t = (size_t)L * count + i;
+            vd[t] = {(uint32_t)i, L, (uint32_t)(idx >> 32), (uint32_t)idx, leaf, 0, 0, 0};
+            memcpy(&sigs[t * len * n], p, (size_t)len * n);
+            p += (size_t)len * n;
+            memcpy(&auth[t * th * n], p, (size_t)th * n);
+            p += (size_t)th * n;
+        }
+        memcpy(&pkr[(size_t)i * n], items[i].pk, n);
+    }
+    DLeaf* dv = b->vdesc.need<DLeaf>(sizeof(DLeaf) * vd.size());
+    I->h2d(dv, vd.data(), sizeof(DLeaf) * vd.size());
+    uint8_t* dws = b->wots.need(sigs.size());
+    I->h2d(dws, sigs.data(), sigs.size());
+    uint8_t* da = b->auth.need(auth.size());
+    I->h2d(da, auth.data(), auth.size());
+    uint8_t* dp = b->pkroot.need(pkr.size());
+    I->h2d(dp, pkr.data(), pkr.size());
+    b->dig.need((size_t)count * n);
+    b->roots.need((size_t)count * d * n);
+    b->ok.need(sizeof(int) * (size_t)count);
+    I->sync();
+    return b;
+}
+
+void Engine::run(Batch* b) {
+    Impl* I = impl_;
+    FamOps* o = ops_of(b->fam);
+    const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
+    const int count = b->count;
+    const void* dk = b->keys.p;
+    o->hmsg(I->L, dk, (const DMsg*)b->msg_desc.p, (uint32_t)count, fam_pad(b->fam), (uint8_t*)b->blob.p,
+            (uint8_t*)b->R.p, (uint8_t*)b->dig.p);
+    if (b->sign) {
+        // all subtrees (count x d) in one pipeline
+        const int nruns = count * (int)d;
+        const uint64_t leaves = 1ull << th;
+        const uint64_t per_leaf = (uint64_t)len * n;
+        const uint64_t stride = b->nodes_stride;
+        uint8_t* nodes = (uint8_t*)b->nodes.p;
+        const DRun* drun = (const DRun*)b->runs.p;
+        for (int r0 = 0; r0 < nruns;) {
+            int rg = std::min<int>(nruns - r0, 65535);
+            uint64_t lpp = leaves;
+            while ((uint64_t)rg * lpp * per_leaf > kEndsBudget && lpp > 64) lpp >>= 1;
+            while ((uint64_t)rg * lpp * per_leaf > kEndsBudget && rg > 1) rg = (rg + 1) / 2;
+            uint8_t* ends = b->ends.need((size_t)rg * lpp * per_leaf);
+            uint8_t* tmp = b->tmp.need((size_t)rg * lpp * ((len + 1) / 2) * n);
+            uint8_t* tnodes = nodes + (size_t)r0 * stride * n;
+            for (uint64_t l0 = 0; l0 < leaves; l0 += lpp) {
+                const uint32_t cnt = (uint32_t)std::min<uint64_t>(lpp, leaves - l0);
+                o->chains(I->L, dk, drun + r0, rg, (uint32_t)l0, cnt, ends);
+                uint32_t l = len, height = 0;
+                bool from_a = true;
+                const uint32_t sa = len, sb = (len + 1) / 2;
+                while (l > 1) {
+                    o->ltree_level(I->L, dk, drun + r0, rg, (uint32_t)l0, cnt, l, height, from_a ? ends : tmp,
+                                   from_a ? sa : sb, from_a ? tmp : ends, from_a ? sb : sa, tnodes, stride, l0);
+                    from_a = !from_a;
+                    l = (l + 1) / 2;
+                    ++height;
+                }
+            }
+            for (unsigned lev = 0; lev < th; ++lev) o->level(I->L, dk, drun + r0, rg, th, lev, tnodes);
+            r0 += rg;
+        }
+        // auth paths and roots: auth[t] = th auth nodes || root, t = layer * count + item
+        const uint32_t ng = (uint32_t)(nruns * (th + 1));
+        I->L.run("gather", k_gather, dim3(cdiv((uint64_t)ng * (n / 4), WG)), dim3(WG), (const uint8_t*)nodes,
+                 (const uint64_t*)b->gidx.p, ng, n, (uint8_t*)b->auth.p);
+        // WOTS signatures layer by layer; layer L signs the root of layer L-1
+        // (the digest for layer 0).  Roots sit at the end of each auth record.
+        uint8_t* msgs = (uint8_t*)b->roots.p;
+        const size_t rec = (size_t)(th + 1) * n;
+        for (unsigned L = 0; L < d; ++L) {
+            const uint8_t* msg_l;
+            if (L == 0) {
+                msg_l = (const uint8_t*)b->dig.p;
+            } else {
+                // compact roots of layer L-1 into msgs[L-1]
+                uint8_t* dst = msgs + (size_t)(L - 1) * count * n;
+                HIPCK(hipMemcpy2DAsync(dst, n, (const uint8_t*)b->auth.p + ((size_t)(L - 1) * count) * rec + th * n,
+                                       rec, n, count, hipMemcpyDeviceToDevice, I->L.s));
+                msg_l = dst;
+            }
+            o->wots(I->L, dk, (const DWots*)b->wdesc.p + (size_t)L * count, (uint32_t)count, msg_l, nullptr,
+                    (uint8_t*)b->wots.p + (size_t)L * count * len * n);
+        }
+    } else {
+        for (unsigned L = 0; L < d; ++L) {
+            const uint8_t* msg_l = L == 0 ? (const uint8_t*)b->dig.p : (const uint8_t*)b->roots.p + (size_t)(L - 1) * count * n;
+            o->leaf_wg(I->L, dk, (const DLeaf*)b->vdesc.p + (size_t)L * count, (uint32_t)count, true, msg_l,
+                       (const uint8_t*)b->wots.p + (size_t)L * count * len * n,
+                       (const uint8_t*)b->auth.p + (size_t)L * count * th * n, th,
+                       (uint8_t*)b->roots.p + (size_t)L * count * n);
+        }
+        I->L.run("compare", k_compare, dim3(cdiv(count, WG)), dim3(WG),
+                 (const uint8_t*)b->roots.p + (size_t)(d - 1) * count * n, (const uint8_t*)b->pkroot.p,
+                 (uint32_t)count, n, (int*)b->ok.p);
+    }
+    I->sync();
+}
+
+void Engine::collect_sign(Batch* b, uint8_t* const* out) {
+    Impl* I = impl_;
+    const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
+    const int count = b->count;
+    std::vector<uint8_t> R((size_t)count * n), auth((size_t)count * d * (th + 1) * n), w((size_t)count * d * len * n);
+    I->d2h(R.data(), b->R.p, R.size());
+    I->d2h(auth.data(), b->auth.p, auth.size());
+    I->d2h(w.data(), b->wots.p, w.size());
+    I->sync();
+    for (int i = 0; i < count; ++i) {
+        uint8_t* p = out[i];
+        memcpy(p, &R[(size_t)i * n], n);
+        p += n;
+        for (unsigned L = 0; L < d; ++L) {
+            const size_t t = (size_t)L * count + i;
+            memcpy(p, &w[t * len * n], (size_t)len * n);
+            p += (size_t)len * n;
+            memcpy(p, &auth[t * (th + 1) * n], (size_t)th * n);
+            p += (size_t)th * n;
+        }
+    }
+}
+
+void Engine::collect_verify(Batch* b, int* ok) {
+    Impl* I = impl_;
+    I->d2h(ok, b->ok.p, sizeof(int) * (size_t)b->count);
+    I->sync();
+}
+
+void Engine::free_batch(Batch* b) {
+    if (!impl_->spare[b->sign]) {  // keep one of each kind: its buffers only grow
+        impl_->spare[b->sign] = b;
+        return;
+    }
+    DevBuf* bufs[] = {&b->keys, &b->seeds, &b->msg_desc, &b->blob, &b->R, &b->dig, &b->runs, &b->nodes,
+                      &b->ends, &b->tmp, &b->gidx, &b->auth, &b->wdesc, &b->wots, &b->vdesc, &b->roots,
+                      &b->ok, &b->pkroot};
+    for (DevBuf* x : bufs) {
+        if (x->p) HIPCK(hipFree(x->p));
+        x->p = nullptr;
+    }
+    delete b;
+}
+
+}  // namespace xm

@@ -1,0 +1,249 @@
+// ext.cpp -- additions beyond the reference API (include/xmss_mi355x.h):
+// batches of independent keys, device check, kernel timing.
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/xmss_core.h"
+#include "../../include/xmss_mi355x.h"
+#include "core_common.h"
+
+using namespace xm;
+
+struct xmss_mi355x_job {
+    bool sign = false;
+    xmss_params p{};
+    int fam = -1;
+    size_t count = 0;
+    std::vector<int> rets;
+    std::vector<size_t> live;  // inputs that reach the device
+    Engine::Batch* b = nullptr;
+    // sign
+    std::vector<unsigned char> idxbytes;
+    const unsigned char* const* ms = nullptr;
+    const unsigned long long* mlens = nullptr;
+    // verify
+    const unsigned char* const* sms = nullptr;
+    const unsigned long long* smlens = nullptr;
+    const unsigned char* const* pks = nullptr;
+};
+
+namespace {
+uint32_t oid_of(const unsigned char* b) {
+    return ((uint32_t)b[0] << 24) | ((uint32_t)b[1] << 16) | ((uint32_t)b[2] << 8) | b[3];
+}
+int parse(int mt, uint32_t oid, xmss_params* p) { return mt ? xmssmt_parse_oid(p, oid) : xmss_parse_oid(p, oid); }
+}  // namespace
+
+extern "C" {
+
+int xmss_mi355x_device_ok(void) {
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    return Engine::get() != nullptr;
+}
+
+void xmss_mi355x_profile(int on) {
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    if (Engine* e = Engine::get()) e->set_profiling(on != 0);
+}
+
+void xmss_mi355x_reset_stats(void) {
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    if (Engine* e = Engine::get()) e->reset_stats();
+}
+
+size_t xmss_mi355x_kernel_stats(char* buf, size_t len) {
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    std::string s;
+    if (Engine* e = Engine::get()) {
+        for (auto& k : e->stats()) {
+            char line[256];
+            snprintf(line, sizeof line, "%s %llu %.6f\n", k.name.c_str(), (unsigned long long)k.count, k.total_ms);
+            s += line;
+        }
+    }
+    if (buf && len) {
+        const size_t m = std::min(len - 1, s.size());
+        memcpy(buf, s.data(), m);
+        buf[m] = 0;
+    }
+    return s.size() + 1;
+}
+
+xmss_mi355x_job* xmss_mi355x_sign_prepare(int mt, unsigned char* const* sks, const unsigned char* const* ms,
+                                          const unsigned long long* mlens, size_t count) {
+    xmss_mi355x_job* j = new xmss_mi355x_job;
+    j->sign = true;
+    j->count = count;
+    j->ms = ms;
+    j->mlens = mlens;
+    j->rets.assign(count, -1);
+    if (count == 0) return j;
+    const uint32_t oid0 = oid_of(sks[0]);
+    if (parse(mt, oid0, &j->p)) return j;
+    const xmss_params& p = j->p;
+    const unsigned n = p.n, ib = p.index_bytes;
+    j->fam = fam_of(p.func, n);
+    j->idxbytes.assign(count * ib, 0);
+    std::vector<Engine::SignItem> items;
+    for (size_t i = 0; i < count; ++i) {
+        unsigned char* sk = sks[i];
+        if (oid_of(sk) != oid0) continue;  // one parameter set per batch
+        sk += XMSS_OID_LEN;
+        const unsigned long long idx = (unsigned long)bytes_to_ull(sk, ib);
+        if (index_check(&p, sk, idx)) {
+            j->rets[i] = -2;
+            continue;
+        }
+        memcpy(&j->idxbytes[i * ib], sk, ib);
+        ull_to_bytes(sk, ib, idx + 1);
+        items.push_back({sk + ib, sk + ib + n, sk + ib + 3 * n, sk + ib + 2 * n, idx, ms[i], mlens[i]});
+        j->live.push_back(i);
+        j->rets[i] = 0;
+    }
+    if (!items.empty() && j->fam >= 0) {
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        Engine* e = Engine::get();
+        if (!e) {
+            for (size_t i : j->live) j->rets[i] = XMSS_ENODEV;
+            j->live.clear();
+            return j;
+        }
+        j->b = e->sign_prepare(j->fam, p.tree_height, p.d, (int)items.size(), items.data());
+    }
+    return j;
+}
+
+xmss_mi355x_job* xmss_mi355x_verify_prepare(int mt, const unsigned char* const* sms,
+                                            const unsigned long long* smlens, const unsigned char* const* pks,
+                                            size_t count) {
+    xmss_mi355x_job* j = new xmss_mi355x_job;
+    j->sign = false;
+    j->count = count;
+    j->sms = sms;
+    j->smlens = smlens;
+    j->pks = pks;
+    j->rets.assign(count, -1);
+    if (count == 0) return j;
+    const uint32_t oid0 = oid_of(pks[0]);
+    if (parse(mt, oid0, &j->p)) return j;
+    const xmss_params& p = j->p;
+    const unsigned n = p.n, ib = p.index_bytes;
+    j->fam = fam_of(p.func, n);
+    std::vector<Engine::VerifyItem> items;
+    for (size_t i = 0; i < count; ++i) {
+        if (oid_of(pks[i]) != oid0 || smlens[i] < p.sig_bytes) continue;
+        const unsigned char* sm = sms[i];
+        items.push_back({pks[i] + XMSS_OID_LEN, bytes_to_ull(sm, ib), sm + ib, sm + ib + n, sm + p.sig_bytes,
+                         smlens[i] - p.sig_bytes});
+        j->live.push_back(i);
+    }
+    if (!items.empty() && j->fam >= 0) {
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        Engine* e = Engine::get();
+        if (!e) {
+            for (size_t i : j->live) j->rets[i] = XMSS_ENODEV;
+            j->live.clear();
+            return j;
+        }
+        j->b = e->verify_prepare(j->fam, p.tree_height, p.d, (int)items.size(), items.data());
+    }
+    return j;
+}
+
+int xmss_mi355x_job_run(xmss_mi355x_job* j) {
+    if (!j->b) return 0;
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    Engine* e = Engine::get();
+    if (!e) return XMSS_ENODEV;
+    e->run(j->b);
+    return 0;
+}
+
+int xmss_mi355x_sign_collect(xmss_mi355x_job* j, unsigned char* const* sms, unsigned long long* smlens,
+                             int* rets) {
+    const xmss_params& p = j->p;
+    if (j->b) {
+        std::vector<unsigned char*> outs;
+        for (size_t i : j->live) outs.push_back(sms[i] + p.index_bytes);
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        Engine::get()->collect_sign(j->b, outs.data());
+    }
+    for (size_t i = 0; i < j->count; ++i) {
+        rets[i] = j->rets[i];
+        if (j->rets[i] == 0 || j->rets[i] == -2) {
+            // message after the signature, smlen set even for -2 (xmss_core.c:205-206)
+            memmove(sms[i] + p.sig_bytes, j->ms[i], j->mlens[i]);
+            smlens[i] = p.sig_bytes + j->mlens[i];
+        }
+        if (j->rets[i] == 0) memcpy(sms[i], &j->idxbytes[i * p.index_bytes], p.index_bytes);
+    }
+    return 0;
+}
+
+int xmss_mi355x_verify_collect(xmss_mi355x_job* j, unsigned char* const* ms, unsigned long long* mlens,
+                               int* rets) {
+    const xmss_params& p = j->p;
+    std::vector<int> ok(j->live.size(), 0);
+    if (j->b) {
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        Engine::get()->collect_verify(j->b, ok.data());
+    }
+    for (size_t i = 0; i < j->count; ++i) {
+        rets[i] = j->rets[i];
+        if (rets[i] == -1) mlens[i] = 0;
+    }
+    for (size_t k = 0; k < j->live.size(); ++k) {
+        const size_t i = j->live[k];
+        const unsigned char* sm = j->sms[i];
+        unsigned char* m = ms[i];
+        const unsigned long long mlen = j->smlens[i] - p.sig_bytes;
+        memmove(m + p.sig_bytes, sm + p.sig_bytes, mlen);
+        unsigned char* pre = m + p.sig_bytes - p.padding_len - 3 * p.n;
+        ull_to_bytes(pre, p.padding_len, 2);
+        memcpy(pre + p.padding_len, sm + p.index_bytes, p.n);
+        memcpy(pre + p.padding_len + p.n, j->pks[i] + XMSS_OID_LEN, p.n);
+        ull_to_bytes(pre + p.padding_len + 2 * p.n, p.n, bytes_to_ull(sm, p.index_bytes));
+        if (ok[k]) {
+            memmove(m, sm + p.sig_bytes, mlen);
+            mlens[i] = mlen;
+            rets[i] = 0;
+        } else {
+            memset(m, 0, mlen);
+            mlens[i] = 0;
+            rets[i] = -1;
+        }
+    }
+    return 0;
+}
+
+void xmss_mi355x_job_free(xmss_mi355x_job* j) {
+    if (j->b) {
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        Engine::get()->free_batch(j->b);
+    }
+    delete j;
+}
+
+int xmss_batch_sign(int mt, unsigned char* const* sks, unsigned char* const* sms, unsigned long long* smlens,
+                    const unsigned char* const* ms, const unsigned long long* mlens, int* rets, size_t count) {
+    xmss_mi355x_job* j = xmss_mi355x_sign_prepare(mt, sks, ms, mlens, count);
+    int rc = xmss_mi355x_job_run(j);
+    if (rc == 0) rc = xmss_mi355x_sign_collect(j, sms, smlens, rets);
+    xmss_mi355x_job_free(j);
+    return rc;
+}
+
+int xmss_batch_sign_open(int mt, unsigned char* const* ms, unsigned long long* mlens,
+                         const unsigned char* const* sms, const unsigned long long* smlens,
+                         const unsigned char* const* pks, int* rets, size_t count) {
+    xmss_mi355x_job* j = xmss_mi355x_verify_prepare(mt, sms, smlens, pks, count);
+    int rc = xmss_mi355x_job_run(j);
+    if (rc == 0) rc = xmss_mi355x_verify_collect(j, ms, mlens, rets);
+    xmss_mi355x_job_free(j);
+    return rc;
+}
+
+}  // extern "C"

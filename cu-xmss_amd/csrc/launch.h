@@ -1,0 +1,167 @@
+// launch.h -- kernel launch helper with optional per-kernel HIP-event timing,
+// and the per-family launch table (one implementation per fam_*.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace xm {
+
+[[noreturn]] void fatal(const char* what);
+
+#define HIPCK(x)                                                                  \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess) {                                                   \
+            fprintf(stderr, "xmss_mi355x: HIP error %s at %s:%d (%s)\n",          \
+                    hipGetErrorString(e_), __FILE__, __LINE__, #x);               \
+            fflush(stderr);                                                       \
+            abort();                                                              \
+        }                                                                         \
+    } while (0)
+
+struct Launcher {
+    hipStream_t s = nullptr;
+    bool prof = false;
+    struct Pending {
+        const char* name;
+        hipEvent_t e0, e1;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<uint64_t, double>> stat;
+
+    hipEvent_t ev() {
+        if (pool.empty()) {
+            hipEvent_t e;
+            HIPCK(hipEventCreate(&e));
+            return e;
+        }
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    // Events bracket each launch on the same stream, so elapsed time between
+    // them is that kernel's duration.
+    template <typename K, typename... Args>
+    void run(const char* name, K kernel, dim3 grid, dim3 block, Args... args) {
+        if (grid.x == 0 || grid.y == 0) return;
+        hipEvent_t e0 = nullptr;
+        if (prof) {
+            e0 = ev();
+            HIPCK(hipEventRecord(e0, s));
+        }
+        hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+        HIPCK(hipGetLastError());
+        if (prof) {
+            hipEvent_t e1 = ev();
+            HIPCK(hipEventRecord(e1, s));
+            pending.push_back({name, e0, e1});
+        }
+    }
+    void harvest() {
+        for (auto& p : pending) {
+            float ms = 0;
+            HIPCK(hipEventElapsedTime(&ms, p.e0, p.e1));
+            auto& st = stat[p.name];
+            st.first += 1;
+            st.second += ms;
+            pool.push_back(p.e0);
+            pool.push_back(p.e1);
+        }
+        pending.clear();
+    }
+};
+
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Kernel launches of one hash family; all pointers are device pointers.
+struct FamOps {
+    virtual ~FamOps() {}
+    virtual size_t key_bytes() const = 0;
+    virtual void prepare(Launcher& L, const uint8_t* seeds, int nkeys, void* keys) = 0;
+    virtual void chains(Launcher& L, const void* keys, const DRun* runs, int nruns, uint32_t leaf_off,
+                        uint32_t leaves, uint8_t* ends) = 0;
+    virtual void ltree_level(Launcher& L, const void* keys, const DRun* runs, int nruns, uint32_t leaf_off,
+                             uint32_t leaves, uint32_t l, uint32_t height, const uint8_t* src,
+                             uint32_t src_stride, uint8_t* dst, uint32_t dst_stride, uint8_t* out,
+                             uint64_t out_stride, uint64_t out_base) = 0;
+    virtual void level(Launcher& L, const void* keys, const DRun* runs, int nruns, uint32_t th,
+                       uint32_t level, uint8_t* nodes) = 0;
+    virtual void wots(Launcher& L, const void* keys, const DWots* reqs, uint32_t n, const uint8_t* msgs,
+                      const uint8_t* in, uint8_t* out) = 0;
+    virtual void leaf_wg(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, bool verify,
+                         const uint8_t* msgs, const uint8_t* sigs, const uint8_t* auth, uint32_t th,
+                         uint8_t* out) = 0;
+    virtual void thash_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, const uint8_t* in,
+                         uint8_t* out) = 0;
+    virtual void hmsg(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad,
+                      uint8_t* blob, uint8_t* R, uint8_t* dig) = 0;
+};
+
+template <class F>
+struct FamOpsT final : FamOps {
+    using K = typename F::Key;
+    size_t key_bytes() const override { return sizeof(K); }
+    void prepare(Launcher& L, const uint8_t* seeds, int nkeys, void* keys) override {
+        L.run("prepare", k_prepare<F>, dim3(cdiv(nkeys, WG)), dim3(WG), seeds, nkeys, (K*)keys);
+    }
+    void chains(Launcher& L, const void* keys, const DRun* runs, int nruns, uint32_t leaf_off,
+                uint32_t leaves, uint8_t* ends) override {
+        L.run("chains", k_chains<F>, dim3(cdiv((uint64_t)leaves * F::LEN, WG), nruns), dim3(WG),
+              (const K*)keys, runs, leaf_off, leaves, ends);
+    }
+    void ltree_level(Launcher& L, const void* keys, const DRun* runs, int nruns, uint32_t leaf_off,
+                     uint32_t leaves, uint32_t l, uint32_t height, const uint8_t* src, uint32_t src_stride,
+                     uint8_t* dst, uint32_t dst_stride, uint8_t* out, uint64_t out_stride,
+                     uint64_t out_base) override {
+        L.run("ltree", k_ltree_level<F>, dim3(cdiv((uint64_t)leaves * ((l + 1) / 2), WG), nruns), dim3(WG),
+              (const K*)keys, runs, leaf_off, leaves, l, height, src, src_stride, dst, dst_stride, out,
+              out_stride, out_base);
+    }
+    void level(Launcher& L, const void* keys, const DRun* runs, int nruns, uint32_t th, uint32_t lev,
+               uint8_t* nodes) override {
+        L.run("level", k_level<F>, dim3(cdiv(1ull << (th - lev - 1), WG), nruns), dim3(WG), (const K*)keys,
+              runs, th, lev, nodes);
+    }
+    void wots(Launcher& L, const void* keys, const DWots* reqs, uint32_t n, const uint8_t* msgs,
+              const uint8_t* in, uint8_t* out) override {
+        L.run("wots", k_wots<F>, dim3(cdiv((uint64_t)n * F::LEN, WG)), dim3(WG), (const K*)keys, reqs, n, msgs,
+              in, out);
+    }
+    void leaf_wg(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, bool verify,
+                 const uint8_t* msgs, const uint8_t* sigs, const uint8_t* auth, uint32_t th,
+                 uint8_t* out) override {
+        const dim3 block(((F::LEN + 63) / 64) * 64);
+        if (verify)
+            L.run("verify_wg", k_leaf_wg<F, true>, dim3(n), block, (const K*)keys, reqs, msgs, sigs, auth, th,
+                  out);
+        else
+            L.run("leaf_wg", k_leaf_wg<F, false>, dim3(n), block, (const K*)keys, reqs, msgs, sigs, auth, th,
+                  out);
+    }
+    void thash_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, const uint8_t* in,
+                 uint8_t* out) override {
+        L.run("thash_h", k_thash_h<F>, dim3(cdiv(n, WG)), dim3(WG), (const K*)keys, reqs, n, in, out);
+    }
+    void hmsg(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad, uint8_t* blob,
+              uint8_t* R, uint8_t* dig) override {
+        L.run("hmsg", k_hmsg<F>, dim3(cdiv(n, WG)), dim3(WG), (const K*)keys, reqs, n, pad, blob, R, dig);
+    }
+};
+
+FamOps* ops_sha2_24();
+FamOps* ops_sha2_32();
+FamOps* ops_sha2_64();
+FamOps* ops_shake128_32();
+FamOps* ops_shake256_24();
+FamOps* ops_shake256_32();
+FamOps* ops_shake256_64();
+
+}  // namespace xm

@@ -1,0 +1,70 @@
+/*
+ * xmss_mi355x.h -- additions of libxmss_mi355x beyond the reference API.
+ *
+ * None of these has a counterpart in cuxmss/; they are additive and leave
+ * the drop-in functions of xmss.h / xmss_core.h / params.h unchanged.
+ */
+#ifndef XMSS_MI355X_H
+#define XMSS_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Randomness for keypairs: a reproducible SplitMix64 byte stream from `seed`
+ * (each 64-bit output as 8 little-endian bytes), or /dev/urandom (default).
+ * The reference links a different randombytes() object for the same purpose. */
+void xmss_mi355x_rng_seed(uint64_t seed);
+void xmss_mi355x_rng_urandom(void);
+
+/* 1 when a HIP device is usable, 0 otherwise (then every hashing call
+ * returns -3). */
+int xmss_mi355x_device_ok(void);
+
+/* Per-kernel timing with HIP events on the library's stream. */
+void xmss_mi355x_profile(int on);
+void xmss_mi355x_reset_stats(void);
+/* Writes "name count total_ms\n" lines; returns the length needed. */
+size_t xmss_mi355x_kernel_stats(char *buf, size_t len);
+
+/* ---- batches of independent keys, one parameter set per batch ----------
+ * Plain-core signing of `count` OID-prefixed secret keys (each updated in
+ * place exactly as xmss[mt]_sign would) and verification of `count`
+ * signed messages.  The three-step form stages inputs in HBM (prepare),
+ * computes with no host traffic (run) and copies results out (collect);
+ * rets[i] carries what xmss[mt]_sign / _sign_open would have returned. */
+typedef struct xmss_mi355x_job xmss_mi355x_job;
+
+xmss_mi355x_job *xmss_mi355x_sign_prepare(int mt, unsigned char *const *sks,
+                                          const unsigned char *const *ms,
+                                          const unsigned long long *mlens,
+                                          size_t count);
+xmss_mi355x_job *xmss_mi355x_verify_prepare(int mt, const unsigned char *const *sms,
+                                            const unsigned long long *smlens,
+                                            const unsigned char *const *pks,
+                                            size_t count);
+int xmss_mi355x_job_run(xmss_mi355x_job *job);
+int xmss_mi355x_sign_collect(xmss_mi355x_job *job, unsigned char *const *sms,
+                             unsigned long long *smlens, int *rets);
+/* ms[i] must hold smlens[i] bytes, as for xmss_sign_open. */
+int xmss_mi355x_verify_collect(xmss_mi355x_job *job, unsigned char *const *ms,
+                               unsigned long long *mlens, int *rets);
+void xmss_mi355x_job_free(xmss_mi355x_job *job);
+
+/* One-call forms of the above. */
+int xmss_batch_sign(int mt, unsigned char *const *sks, unsigned char *const *sms,
+                    unsigned long long *smlens, const unsigned char *const *ms,
+                    const unsigned long long *mlens, int *rets, size_t count);
+int xmss_batch_sign_open(int mt, unsigned char *const *ms, unsigned long long *mlens,
+                         const unsigned char *const *sms,
+                         const unsigned long long *smlens,
+                         const unsigned char *const *pks, int *rets, size_t count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

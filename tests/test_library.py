@@ -1,0 +1,47 @@
+"""The C-ABI libraries load and export every function include/*.h declares
+(no GPU work).  On a machine without a GPU the hashing calls fail loudly
+with -3 instead of falling back to a CPU path."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from ffi import PRODUCT_DIR, ROOT, buf, ull
+
+HEADERS = ["xmss.h", "xmss_core.h", "params.h", "randombytes.h", "xmss_mi355x.h"]
+LIBS = ["libxmss_mi355x.so", "libxmss_mi355x_fast.so"]
+
+
+def declared():
+    names = set()
+    for h in HEADERS:
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b([a-z_][a-z0-9_]*)\s*\(", text):
+            nm = m.group(1)
+            if nm not in ("defined", "sizeof") and not nm.startswith("__"):
+                names.add(nm)
+    return names
+
+
+@pytest.mark.parametrize("lib", LIBS)
+def test_exports_every_declared_symbol(lib):
+    so = C.CDLL(os.path.join(PRODUCT_DIR, lib))
+    names = declared()
+    assert "xmss_keypair" in names and "xmssmt_core_sign_open" in names and "xmss_batch_sign" in names
+    plain_only = {"xmssmt_core_seed_keypair"}
+    for nm in sorted(names):
+        if lib.endswith("_fast.so") and nm in plain_only:
+            continue  # the BDS core has no seed keypair, as in the reference
+        assert hasattr(so, nm), "%s does not export %s" % (lib, nm)
+
+
+def test_no_cpu_fallback_without_gpu():
+    """Without a usable device the library reports it and returns -3."""
+    so = C.CDLL(os.path.join(PRODUCT_DIR, "libxmss_mi355x.so"))
+    so.xmss_mi355x_device_ok.restype = C.c_int
+    if so.xmss_mi355x_device_ok() == 1:
+        pytest.skip("a GPU is present")
+    pk, sk = buf(100), buf(200)
+    assert so.xmss_keypair(pk, sk, 1) == -3
