@@ -1,0 +1,333 @@
+"""Benchmark of the MI355X XMSS engine on its headline workloads.
+
+Metric: XMSS-SHA2_16_256 keygen seconds; signs/s + verifies/s at batch 4096;
+p99 sign latency.
+
+  step     one XMSS-SHA2_16_256 keygen (65,536 leaves, 298,582,010 SHA-256
+           compressions): the value, in seconds.
+  batch    4,096 independent XMSS-SHA2_10_256 keys, one plain-core signature
+           each at a random leaf, messages log-uniform 32 B - 4 KB; then the
+           4,096 verifications.  Inputs are staged in HBM before the timed
+           region (xmss_mi355x_sign_prepare), results copied out after it.
+  latency  1,024 sequential BDS-core signatures of one XMSS-SHA2_16_256 key
+           (p50 / p99 per call, each a full xmss_sign call).
+
+Prints one JSON line.  Kernel durations come from HIP events recorded on the
+library's stream around each launch.
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import random
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+from ffi import Api, ORACLE_SO, PRODUCT_DIR, REF_DIR, Oracle, buf, load_ref, raw  # noqa: E402
+
+METRIC = "XMSS-SHA2_16_256 keygen s; signs/s + verifies/s at batch 4096; p99 sign latency"
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # INT32 lane-ops/s of MI355X (256 CU x 64 lanes x 2.4 GHz)
+OPS_PER_SHA256 = 1384  # VALU ops of one SHA-256 compression (64 x 14 round + 48 x 10 schedule + 8)
+SEED = 0x584D5353
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_product(core):
+    api = Api(os.path.join(PRODUCT_DIR, "libxmss_mi355x_fast.so" if core else "libxmss_mi355x.so"), core)
+    L = api.lib
+    L.xmss_mi355x_device_ok.restype = C.c_int
+    L.xmss_mi355x_profile.argtypes = [C.c_int]
+    L.xmss_mi355x_kernel_stats.argtypes = [C.c_char_p, C.c_size_t]
+    L.xmss_mi355x_kernel_stats.restype = C.c_size_t
+    L.xmss_mi355x_rng_seed.argtypes = [C.c_uint64]
+    for nm in ("xmss_mi355x_sign_prepare", "xmss_mi355x_verify_prepare"):
+        getattr(L, nm).restype = C.c_void_p
+        getattr(L, nm).argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t] if "sign" in nm else \
+            [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    L.xmss_mi355x_job_run.argtypes = [C.c_void_p]
+    L.xmss_mi355x_sign_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.xmss_mi355x_verify_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.xmss_mi355x_job_free.argtypes = [C.c_void_p]
+    return api
+
+
+def kernel_stats(L):
+    n = L.xmss_mi355x_kernel_stats(None, 0)
+    b = C.create_string_buffer(n + 1)
+    L.xmss_mi355x_kernel_stats(b, n + 1)
+    out = {}
+    for line in b.value.decode().splitlines():
+        name, cnt, ms = line.split()
+        out[name] = (int(cnt), float(ms))
+    return out
+
+
+def ptrs(bufs):
+    a = (C.c_void_p * len(bufs))()
+    for i, b in enumerate(bufs):
+        a[i] = C.cast(b, C.c_void_p)
+    return a
+
+
+def keygen_h16(api):
+    """One XMSS-SHA2_16_256 keygen (plain core: seed keypair, xmss_core.c:134)."""
+    rc, pk, sk = api.seed_keypair(0x02, 0, bytes(range(96)))
+    assert rc == 0
+    return pk
+
+
+def bench_keygen(api, steps, warmup):
+    L = api.lib
+    for _ in range(warmup):
+        keygen_h16(api)
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        pk = keygen_h16(api)
+        times.append(time.perf_counter() - t0)
+    # one profiled keygen: per-kernel HIP-event durations
+    L.xmss_mi355x_reset_stats()
+    L.xmss_mi355x_profile(1)
+    keygen_h16(api)
+    L.xmss_mi355x_profile(0)
+    st = kernel_stats(L)
+    L.xmss_mi355x_reset_stats()
+    return times, st, pk
+
+
+def batch_inputs(count, rng):
+    """XMSS-SHA2_10_256 keys from their own 96-byte seeds, a uniform leaf in
+    [0, 1023] each and log-uniform 32 B - 4 KB messages."""
+    seeds = bytes(rng.getrandbits(8) for _ in range(96 * count))
+    idxs = [rng.randrange(0, 1024) for _ in range(count)]
+    msgs = [bytes(rng.getrandbits(8) for _ in range(int(32 * 128 ** rng.random()))) for _ in range(count)]
+    return seeds, idxs, msgs
+
+
+def bench_batch(api, count):
+    """Batch keygen, then one plain-core signature per key, then verification
+    of all of them.  Returns timings, kernel stats and the verify results."""
+    L = api.lib
+    L.xmss_batch_keypair.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_char_p, C.c_uint32, C.c_size_t]
+    rng = random.Random(SEED + 7)
+    seeds, idxs, msgs = batch_inputs(count, rng)
+    p = api.params(0x01, 0)
+    pks = [buf(4 + p.pk_bytes) for _ in range(count)]
+    sks = [buf(4 + p.sk_bytes) for _ in range(count)]
+    t0 = time.perf_counter()
+    assert L.xmss_batch_keypair(0, ptrs(pks), ptrs(sks), seeds, 0x01, count) == 0
+    keygen_t = time.perf_counter() - t0
+    for i in range(count):
+        sks[i][4:8] = list(idxs[i].to_bytes(4, "big"))
+    ms = [buf(m) for m in msgs]
+    mlens = (C.c_ulonglong * count)(*[len(m) for m in msgs])
+    sms = [buf(p.sig_bytes + len(m)) for m in msgs]
+    smlens = (C.c_ulonglong * count)()
+    rets = (C.c_int * count)()
+    t0 = time.perf_counter()
+    job = L.xmss_mi355x_sign_prepare(0, ptrs(sks), ptrs(ms), mlens, count)
+    t1 = time.perf_counter()
+    L.xmss_mi355x_reset_stats()
+    L.xmss_mi355x_profile(1)
+    L.xmss_mi355x_job_run(job)
+    t2 = time.perf_counter()
+    L.xmss_mi355x_profile(0)
+    st = kernel_stats(L)
+    L.xmss_mi355x_sign_collect(job, ptrs(sms), smlens, rets)
+    L.xmss_mi355x_job_free(job)
+    t3 = time.perf_counter()
+    assert all(rets[i] == 0 for i in range(count))
+    sign_run, sign_total = t2 - t1, t3 - t0
+
+    sm_bytes = [raw(sms[i], smlens[i]) for i in range(count)]
+    smb = [buf(x) for x in sm_bytes]
+    outs = [buf(len(x)) for x in sm_bytes]
+    smls = (C.c_ulonglong * count)(*[len(x) for x in sm_bytes])
+    mls = (C.c_ulonglong * count)()
+    vr = (C.c_int * count)()
+    job = L.xmss_mi355x_verify_prepare(0, ptrs(smb), smls, ptrs(pks), count)
+    t0 = time.perf_counter()
+    L.xmss_mi355x_job_run(job)
+    ver_t = time.perf_counter() - t0
+    L.xmss_mi355x_verify_collect(job, ptrs(outs), mls, vr)
+    L.xmss_mi355x_job_free(job)
+    # leaf 1023 is the last index: the reference signs it with the wiped key,
+    # so exactly those signatures must be rejected (xmss_core.c:222-231)
+    for i in range(count):
+        want = 0 if idxs[i] < 1023 else -1
+        assert vr[i] == want, (i, idxs[i], vr[i])
+        if want == 0:
+            assert raw(outs[i], mls[i]) == msgs[i]
+    return keygen_t, sign_run, sign_total, ver_t, st, sum(1 for i in idxs if i == 1023)
+
+
+def bench_latency(nsigns):
+    """Sequential BDS signatures of one XMSS-SHA2_16_256 key (fast core)."""
+    api = load_product(1)
+    p = api.params(0x02, 0)
+    rc, pk, sk = api.keypair(0x02, 0, sk_init=bytes(4 + p.sk_bytes))
+    assert rc == 0
+    rng = random.Random(SEED + 9)
+    lat = []
+    last = None
+    for _ in range(nsigns):
+        m = bytes(rng.getrandbits(8) for _ in range(32))
+        t0 = time.perf_counter()
+        rc, sm, sk = api.sign(sk, m, 0)
+        lat.append(time.perf_counter() - t0)
+        assert rc == 0
+        last = (sm, m)
+    vlat = []
+    for _ in range(16):
+        t0 = time.perf_counter()
+        vr, mm = api.sign_open(last[0], pk, 0)
+        vlat.append(time.perf_counter() - t0)
+        assert vr == 0 and mm == last[1]
+    lat.sort()
+    vlat.sort()
+    q = lambda v, f: v[min(len(v) - 1, int(math.ceil(f * len(v))) - 1)]
+    return q(lat, 0.5), q(lat, 0.99), q(vlat, 0.5)
+
+
+def cpu_baseline():
+    """The reference C code (oracle/_ref, xmss_core.c keygen) on one host core:
+    XMSS-SHA2_10_256 keygens, scaled by 64.0 = 298,582,010 / 4,665,338
+    compressions to an XMSS-SHA2_16_256 keygen."""
+    ref = load_ref(0)
+    kind = "reference"
+    if ref is None:
+        kind = "port"
+        orc = Oracle(ORACLE_SO)
+        run = lambda: orc.seed_keypair(0x01, 0, bytes(range(96)))
+    else:
+        run = lambda: ref.seed_keypair(0x01, 0, bytes(range(96)))
+    n, t0 = 0, time.perf_counter()
+    while n < 2 or time.perf_counter() - t0 < 8.0:
+        run()
+        n += 1
+        if n >= 6:
+            break
+    dt = (time.perf_counter() - t0) / n
+    scale = 298582010 / 4665338
+    return {"value": round(dt * scale, 3), "unit": "s per XMSS-SHA2_16_256 keygen", "cores": 1, "kind": kind,
+            "sample": "%d XMSS-SHA2_10_256 keygens (%.2f s each) on 1 core, x%.2f compressions" % (n, dt, scale)}
+
+
+def copy_calibration(nbytes):
+    """One plain device-to-device copy of nbytes (reads nbytes, writes nbytes):
+    the reference point for HBM byte counters."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpyDtoD(b, a, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy calibration: %d bytes device-to-device" % nbytes)
+
+
+def profiled_traffic(kernel="k_chains"):
+    """HBM bytes per launch of the keygen chain kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated on a copy kernel)."""
+    path = os.path.join(ROOT, "profiles", "keygen_xmss_sha2_16_256_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch %s:" % kernel):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--latency-signs", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="MiB of one device-to-device copy, run before the keygens (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: keys are independent and shard by key id with no "
+              "communication, so no multi-GPU run is defined here (use --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    api = load_product(0)
+    if api.lib.xmss_mi355x_device_ok() != 1:
+        raise SystemExit("no usable GPU")
+    if args.copy_calib:
+        copy_calibration(args.copy_calib << 20)
+    log("keygen XMSS-SHA2_16_256 ...")
+    times, kst, pk = bench_keygen(api, args.steps, args.warmup)
+    ms_per_step = 1000 * sum(times) / len(times)
+    chains = kst.get("chains")
+    leaves = 1 << 16
+    comps_chains = leaves * 67 * 62  # PRF_keygen (2) + 15 thash_f (4 each) per chain
+    roof = None
+    if chains:
+        avg_ms = chains[1] / chains[0]
+        achieved = comps_chains * OPS_PER_SHA256 / (avg_ms / 1000) / 1e12
+        roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": profiled_traffic(),
+                "kernel": "k_chains<Sha256Fam<32>>", "kernel_ms": round(avg_ms, 4),
+                "algorithmic_ops": comps_chains * OPS_PER_SHA256}
+    total_comps = 298582010
+    keygen_frac = total_comps * OPS_PER_SHA256 / (ms_per_step / 1000) / 1e12 / VALU_PEAK_TOPS
+
+    bkeygen_t = sign_run = sign_total = ver_t = None
+    bst, n_last = {}, None
+    if args.batch:
+        log("batch keygen/sign/verify x%d ..." % args.batch)
+        bkeygen_t, sign_run, sign_total, ver_t, bst, n_last = bench_batch(api, args.batch)
+    rate = lambda t: None if not t else round(args.batch / t, 2)
+    log("latency (BDS) x%d ..." % args.latency_signs)
+    p50, p99, v50 = bench_latency(args.latency_signs) if args.latency_signs else (None, None, None)
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms_per_step / 1000, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded keys and messages)",
+        "config": {"workload": "XMSS-SHA2_16_256 keygen (65,536 leaves); batch 4096 x XMSS-SHA2_10_256 "
+                               "plain sign + verify; 1024 sequential BDS signs",
+                   "parallelism": "single GPU"},
+        "keygen_valu_frac": round(keygen_frac, 4),
+        "keygens_per_s_batch4096": rate(bkeygen_t),
+        "signs_per_s_batch4096": rate(sign_run),
+        "signs_per_s_batch4096_incl_pcie": rate(sign_total),
+        "verifies_per_s_batch4096": rate(ver_t),
+        "p50_sign_ms": None if p50 is None else round(1000 * p50, 3),
+        "p99_sign_ms": None if p99 is None else round(1000 * p99, 3),
+        "p50_verify_ms": None if v50 is None else round(1000 * v50, 3),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels_keygen_ms": {k: round(v[1], 4) for k, v in kst.items()},
+        "kernels_batch_sign_ms": {k: round(v[1], 3) for k, v in bst.items()},
+        "batch_last_index_rejected": n_last,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

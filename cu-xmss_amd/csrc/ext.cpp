@@ -20,14 +20,15 @@ struct xmss_mi355x_job {
     std::vector<int> rets;
     std::vector<size_t> live;  // inputs that reach the device
     Engine::Batch* b = nullptr;
-    // sign
+    // sign (the pointer arrays are copied; the buffers they point to must
+    // stay valid until collect)
     std::vector<unsigned char> idxbytes;
-    const unsigned char* const* ms = nullptr;
-    const unsigned long long* mlens = nullptr;
+    std::vector<const unsigned char*> ms;
+    std::vector<unsigned long long> mlens;
     // verify
-    const unsigned char* const* sms = nullptr;
-    const unsigned long long* smlens = nullptr;
-    const unsigned char* const* pks = nullptr;
+    std::vector<const unsigned char*> sms;
+    std::vector<unsigned long long> smlens;
+    std::vector<const unsigned char*> pks;
 };
 
 namespace {
@@ -77,8 +78,8 @@ xmss_mi355x_job* xmss_mi355x_sign_prepare(int mt, unsigned char* const* sks, con
     xmss_mi355x_job* j = new xmss_mi355x_job;
     j->sign = true;
     j->count = count;
-    j->ms = ms;
-    j->mlens = mlens;
+    j->ms.assign(ms, ms + count);
+    j->mlens.assign(mlens, mlens + count);
     j->rets.assign(count, -1);
     if (count == 0) return j;
     const uint32_t oid0 = oid_of(sks[0]);
@@ -122,9 +123,9 @@ xmss_mi355x_job* xmss_mi355x_verify_prepare(int mt, const unsigned char* const* 
     xmss_mi355x_job* j = new xmss_mi355x_job;
     j->sign = false;
     j->count = count;
-    j->sms = sms;
-    j->smlens = smlens;
-    j->pks = pks;
+    j->sms.assign(sms, sms + count);
+    j->smlens.assign(smlens, smlens + count);
+    j->pks.assign(pks, pks + count);
     j->rets.assign(count, -1);
     if (count == 0) return j;
     const uint32_t oid0 = oid_of(pks[0]);
@@ -225,6 +226,48 @@ void xmss_mi355x_job_free(xmss_mi355x_job* j) {
         Engine::get()->free_batch(j->b);
     }
     delete j;
+}
+
+int xmss_batch_keypair(int mt, unsigned char* const* pks, unsigned char* const* sks, const unsigned char* seeds,
+                       uint32_t oid, size_t count) {
+    xmss_params p;
+    if (parse(mt, oid, &p)) return -1;
+    const unsigned n = p.n, ib = p.index_bytes;
+    const int fam = fam_of(p.func, n);
+    if (fam < 0) return -1;
+    if (count == 0) return 0;
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    Engine* e = Engine::get();
+    if (!e) return XMSS_ENODEV;
+    std::vector<KeySeeds> ks(count);
+    std::vector<LeafRun> runs(count);
+    std::vector<NodeRef> refs(count);
+    for (size_t i = 0; i < count; ++i) {
+        const unsigned char* s = seeds + i * 3 * n;
+        ks[i] = {s, s + n, s + 2 * n};
+        runs[i] = {(uint32_t)i, p.d - 1, 0, 0};
+        refs[i] = {(uint32_t)i, p.tree_height, 0};
+    }
+    e->set_keys(fam, (int)count, ks.data());
+    e->build_trees(fam, p.tree_height, (int)count, runs.data());
+    std::vector<unsigned char> roots(count * n);
+    e->fetch_nodes(fam, p.tree_height, (int)count, refs.data(), roots.data());
+    for (size_t i = 0; i < count; ++i) {
+        const unsigned char* s = seeds + i * 3 * n;
+        unsigned char* pk = pks[i];
+        unsigned char* sk = sks[i];
+        for (unsigned b = 0; b < XMSS_OID_LEN; ++b) pk[XMSS_OID_LEN - b - 1] = sk[XMSS_OID_LEN - b - 1] = (oid >> (8 * b)) & 0xFF;
+        pk += XMSS_OID_LEN;
+        sk += XMSS_OID_LEN;
+        // xmss_core.c:146-158 layout
+        memset(sk, 0, ib);
+        memcpy(sk + ib, s, 2 * n);
+        memcpy(sk + ib + 3 * n, s + 2 * n, n);
+        memcpy(pk + n, s + 2 * n, n);
+        memcpy(pk, &roots[i * n], n);
+        memcpy(sk + ib + 2 * n, pk, n);
+    }
+    return 0;
 }
 
 int xmss_batch_sign(int mt, unsigned char* const* sks, unsigned char* const* sms, unsigned long long* smlens,

@@ -35,7 +35,9 @@ size_t xmss_mi355x_kernel_stats(char *buf, size_t len);
  * place exactly as xmss[mt]_sign would) and verification of `count`
  * signed messages.  The three-step form stages inputs in HBM (prepare),
  * computes with no host traffic (run) and copies results out (collect);
- * rets[i] carries what xmss[mt]_sign / _sign_open would have returned. */
+ * rets[i] carries what xmss[mt]_sign / _sign_open would have returned.
+ * The pointer arrays are copied by prepare; the buffers they point to (keys,
+ * messages, signatures) must stay valid until collect returns. */
 typedef struct xmss_mi355x_job xmss_mi355x_job;
 
 xmss_mi355x_job *xmss_mi355x_sign_prepare(int mt, unsigned char *const *sks,
@@ -53,6 +55,13 @@ int xmss_mi355x_sign_collect(xmss_mi355x_job *job, unsigned char *const *sms,
 int xmss_mi355x_verify_collect(xmss_mi355x_job *job, unsigned char *const *ms,
                                unsigned long long *mlens, int *rets);
 void xmss_mi355x_job_free(xmss_mi355x_job *job);
+
+/* Plain-core keypairs of `count` independent keys of one parameter set from
+ * caller-supplied seeds (3n bytes each: SK_SEED || SK_PRF || PUB_SEED), as
+ * xmssmt_core_seed_keypair does per key, with the OID prefix of
+ * xmss[mt]_keypair.  All top subtrees are built in one GPU pipeline. */
+int xmss_batch_keypair(int mt, unsigned char *const *pks, unsigned char *const *sks,
+                       const unsigned char *seeds, uint32_t oid, size_t count);
 
 /* One-call forms of the above. */
 int xmss_batch_sign(int mt, unsigned char *const *sks, unsigned char *const *sms,

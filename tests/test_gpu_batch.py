@@ -56,3 +56,35 @@ def test_batch_sign_and_verify_match_reference():
         assert vr[i] == want, "verify item %d" % i
         if want == 0:
             assert raw(outs[i], mls[i]) == msgs[i]
+
+
+def test_three_step_job_api_with_temporary_pointer_arrays():
+    """prepare / run / collect with pointer arrays that do not outlive the
+    prepare call (the job keeps its own copies)."""
+    g = golden("plain_batch.json")
+    lib = product(0)
+    L = lib.lib
+    L.xmss_mi355x_sign_prepare.restype = C.c_void_p
+    L.xmss_mi355x_sign_prepare.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    L.xmss_mi355x_job_run.argtypes = [C.c_void_p]
+    L.xmss_mi355x_sign_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.xmss_mi355x_job_free.argtypes = [C.c_void_p]
+    p = lib.params(g["oid"], 0)
+    items = g["items"][:4]
+    k = len(items)
+    sks = [buf(bytes.fromhex(it["sk_in"])) for it in items]
+    msgs = [bytes.fromhex(it["msg"]) for it in items]
+    mbufs = [buf(m) for m in msgs]
+    mlens = (C.c_ulonglong * k)(*[len(m) for m in msgs])
+    job = L.xmss_mi355x_sign_prepare(0, _arr(sks), _arr(mbufs), mlens, k)
+    import gc
+    gc.collect()
+    assert L.xmss_mi355x_job_run(job) == 0
+    sms = [buf(p.sig_bytes + len(m)) for m in msgs]
+    smlens = (C.c_ulonglong * k)()
+    rets = (C.c_int * k)()
+    assert L.xmss_mi355x_sign_collect(job, _arr(sms), smlens, rets) == 0
+    L.xmss_mi355x_job_free(job)
+    for i, it in enumerate(items):
+        assert rets[i] == it["rc"]
+        assert raw(sms[i], smlens[i]).hex() == it["sm"]
