@@ -31,6 +31,7 @@ enum : int { PAD_F = 0, PAD_H = 1, PAD_HASH = 2, PAD_PRF = 3, PAD_PRF_KEYGEN = 4
 template <int N>
 struct Sha256Fam {
     static constexpr int kN = N;
+    static constexpr bool kChainPrf = (N == 32);  // chain_prf_init / chain_prf_step available
     static constexpr int NW = N / 4;         // node words (big-endian values)
     static constexpr int LEN = 8 * N / 4 + 3;  // wots_len for w = 16
     static constexpr int BLOCK = 64;
@@ -92,6 +93,130 @@ struct Sha256Fam {
         for (int i = 0; i < NW; ++i) out.w[i] = st[i];
     }
 
+    // ---- n = 32: PRF(PUB_SEED, ADRS) computations that share work ----------
+    // The PRF message block after the midstate is ADRS || padding: words 0..7
+    // are ADRS, 8..15 constants.  Two PRFs whose ADRS agree in words 0..K-1
+    // share rounds 0..K-1; schedule words W16..W20 depend only on words 0..5
+    // and the constants, W21 also on word 6 (the hash address of a chain step).
+
+    // State after rounds 0..5 and W16..W20 for one WOTS chain (ADRS words
+    // 0..5 = layer, tree, type, ots, chain are fixed along the chain).
+    struct ChainPrf {
+        uint32_t v6[8];   // working variables after round 5
+        uint32_t w[16];   // W16..W20 in slots 0..4, W5 in slot 5 (for W21)
+    };
+    __device__ static void chain_prf_init(ChainPrf& c, const Key& k, const uint32_t (&a)[8]) {
+        uint32_t w[16];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) w[i] = a[i];
+        w[6] = 0;
+        w[7] = 0;
+        w[8] = 0x80000000u;
+#pragma unroll
+        for (int i = 9; i < 15; ++i) w[i] = 0;
+        w[15] = 96 * 8;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) c.v6[i] = k.ms_pub[i];
+        sha256_rounds<0, 6>(c.v6, w);
+        // W16..W20 (independent of words 6 and 7)
+        uint32_t x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = w[i];
+#pragma unroll
+        for (int t = 16; t < 21; ++t) x[t & 15] = sha256_sched(x, t);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) c.w[i] = x[i];
+        c.w[5] = a[5];
+    }
+
+    // key = PRF(ADRS with hash = i, km = 0), mask = PRF(km = 1) for one chain step.
+    __device__ static void chain_prf_step(Node& key, Node& mask, const Key& k, const ChainPrf& c,
+                                          uint32_t i) {
+        uint32_t w[16];
+        w[6] = i;
+        w[7] = 0;
+        w[8] = 0x80000000u;
+#pragma unroll
+        for (int j = 9; j < 15; ++j) w[j] = 0;
+        w[15] = 96 * 8;
+        // W21 = sigma1(W19) + W14 + sigma0(W6) + W5, shared by key and mask
+        const uint32_t w19 = c.w[3];
+        const uint32_t w21 = xor3(rotr32(w19, 17), rotr32(w19, 19), w19 >> 10) + 0u +
+                             xor3(rotr32(i, 7), rotr32(i, 18), i >> 3) + c.w[5];
+        w[0] = c.w[0]; w[1] = c.w[1]; w[2] = c.w[2]; w[3] = c.w[3]; w[4] = c.w[4];
+        w[5] = w21;
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = c.v6[j];
+        sha256_rounds<6, 7>(v, w);  // round 6 (W6 = i) is shared as well
+        uint32_t v1[8], w1[16];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v1[j] = v[j];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) w1[j] = w[j];
+        w1[7] = 1;
+        sha256_rounds<7, 64, 22>(v, w);
+        sha256_rounds<7, 64, 22>(v1, w1);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            key.w[j] = k.ms_pub[j] + v[j];
+            mask.w[j] = k.ms_pub[j] + v1[j];
+        }
+    }
+
+    // F(key, x ^ mask) given key and mask (the outer hash of thash_f).
+    __device__ static void f_outer(Node& out, const Node& key, const Node& mask, const Node& x) {
+        constexpr int P = (N == 32) ? 8 : 1;
+        uint32_t m[P + 2 * NW];
+#pragma unroll
+        for (int i = 0; i < P; ++i) m[i] = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) { m[P + i] = key.w[i]; m[P + NW + i] = x.w[i] ^ mask.w[i]; }
+        uint32_t st[8];
+        sha256_iv(st);
+        sha256_tail<P + 2 * NW>(st, m, 4 * P + 2 * N);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) out.w[i] = st[i];
+    }
+
+    // M PRFs of one ADRS with key_and_mask = 0 .. M-1 (thash_f: key, mask;
+    // thash_h: key, mask0, mask1): ADRS words 0..6 are common, so rounds
+    // 0..6 and W16..W21 are computed once.
+    template <int M>
+    __device__ static void prf_km(Node* outs, const Key& k, const uint32_t (&a)[8]) {
+        uint32_t w[16];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) w[i] = a[i];
+        w[7] = 0;
+        w[8] = 0x80000000u;
+#pragma unroll
+        for (int i = 9; i < 15; ++i) w[i] = 0;
+        w[15] = 96 * 8;
+        uint32_t v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = k.ms_pub[i];
+        sha256_rounds<0, 7>(v, w);
+        uint32_t x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = w[i];
+#pragma unroll
+        for (int t = 16; t < 22; ++t) x[t & 15] = sha256_sched(x, t);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) w[i] = x[i];  // W16..W21 into slots 0..5
+#pragma unroll
+        for (int km = 0; km < M; ++km) {
+            uint32_t vv[8], ww[16];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) vv[j] = v[j];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) ww[j] = w[j];
+            ww[7] = (uint32_t)km;
+            sha256_rounds<7, 64, 22>(vv, ww);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) outs[km].w[j] = k.ms_pub[j] + vv[j];
+        }
+    }
+
     __device__ static void prf_keygen(Node& out, const Key& k, const uint32_t (&a)[8]) {
         uint32_t st[8];
         if constexpr (N == 32) {
@@ -117,10 +242,18 @@ struct Sha256Fam {
 
     __device__ static void thash_f(Node& out, const Key& k, uint32_t (&a)[8], const Node& x) {
         Node key, mask;
-        a[7] = 0;
-        prf8(key, k.ms_pub, k.pub, a);
-        a[7] = 1;
-        prf8(mask, k.ms_pub, k.pub, a);
+        if constexpr (N == 32) {
+            Node km[2];
+            prf_km<2>(km, k, a);
+            key = km[0];
+            mask = km[1];
+            a[7] = 1;
+        } else {
+            a[7] = 0;
+            prf8(key, k.ms_pub, k.pub, a);
+            a[7] = 1;
+            prf8(mask, k.ms_pub, k.pub, a);
+        }
         constexpr int P = (N == 32) ? 8 : 1;
         uint32_t m[P + 2 * NW];
 #pragma unroll
@@ -137,12 +270,21 @@ struct Sha256Fam {
     __device__ static void thash_h(Node& out, const Key& k, uint32_t (&a)[8], const Node& l,
                                    const Node& r) {
         Node key, m0, m1;
-        a[7] = 0;
-        prf8(key, k.ms_pub, k.pub, a);
-        a[7] = 1;
-        prf8(m0, k.ms_pub, k.pub, a);
-        a[7] = 2;
-        prf8(m1, k.ms_pub, k.pub, a);
+        if constexpr (N == 32) {
+            Node km[3];
+            prf_km<3>(km, k, a);
+            key = km[0];
+            m0 = km[1];
+            m1 = km[2];
+            a[7] = 2;
+        } else {
+            a[7] = 0;
+            prf8(key, k.ms_pub, k.pub, a);
+            a[7] = 1;
+            prf8(m0, k.ms_pub, k.pub, a);
+            a[7] = 2;
+            prf8(m1, k.ms_pub, k.pub, a);
+        }
         constexpr int P = (N == 32) ? 8 : 1;
         uint32_t m[P + 3 * NW];
 #pragma unroll
@@ -209,6 +351,7 @@ struct Sha256Fam {
 // ===========================================================================
 struct Sha512Fam {
     static constexpr int kN = 64;
+    static constexpr bool kChainPrf = false;
     static constexpr int NW = 8;
     static constexpr int LEN = 8 * 64 / 4 + 3;  // 131
     using Word = uint64_t;
@@ -366,6 +509,7 @@ struct Sha512Fam {
 template <int RATE, int N>
 struct ShakeFam {
     static constexpr int kN = N;
+    static constexpr bool kChainPrf = false;
     static constexpr int NW = N / 4;
     static constexpr int LEN = 8 * N / 4 + 3;
     static constexpr int PADW = (N == 24) ? 1 : N / 4;  // padding_len in words

@@ -57,18 +57,25 @@ struct Sha256Consts {
                                        0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
 };
 
-// One compression: st <- st + F(st, w).  w is clobbered (rolling schedule).
-__device__ __forceinline__ void sha256_compress(uint32_t st[8], uint32_t w[16]) {
-    uint32_t a = st[0], b = st[1], c = st[2], d = st[3];
-    uint32_t e = st[4], f = st[5], g = st[6], h = st[7];
+// Message-schedule word t >= 16 in the rolling 16-word buffer.
+__device__ __forceinline__ uint32_t sha256_sched(const uint32_t w[16], int t) {
+    const uint32_t w15 = w[(t + 1) & 15], w2 = w[(t + 14) & 15];
+    const uint32_t s0 = xor3(rotr32(w15, 7), rotr32(w15, 18), w15 >> 3);
+    const uint32_t s1 = xor3(rotr32(w2, 17), rotr32(w2, 19), w2 >> 10);
+    return w[t & 15] + s0 + s1 + w[(t + 9) & 15];
+}
+
+// Rounds T0 .. T1-1 on the working variables v = {a, ..., h}.  Rounds from
+// SCHED on extend the schedule in the rolling buffer first; rounds before
+// SCHED (and >= 16) take w[t & 15] as already holding W[t], which lets a
+// caller supply schedule words it computed once for many compressions.
+template <int T0, int T1, int SCHED = 16>
+__device__ __forceinline__ void sha256_rounds(uint32_t v[8], uint32_t w[16]) {
+    uint32_t a = v[0], b = v[1], c = v[2], d = v[3];
+    uint32_t e = v[4], f = v[5], g = v[6], h = v[7];
 #pragma unroll
-    for (int t = 0; t < 64; ++t) {
-        if (t >= 16) {
-            const uint32_t w15 = w[(t + 1) & 15], w2 = w[(t + 14) & 15];
-            const uint32_t s0 = xor3(rotr32(w15, 7), rotr32(w15, 18), w15 >> 3);
-            const uint32_t s1 = xor3(rotr32(w2, 17), rotr32(w2, 19), w2 >> 10);
-            w[t & 15] = w[t & 15] + s0 + s1 + w[(t + 9) & 15];
-        }
+    for (int t = T0; t < T1; ++t) {
+        if (t >= 16 && t >= SCHED) w[t & 15] = sha256_sched(w, t);
         const uint32_t S1 = xor3(rotr32(e, 6), rotr32(e, 11), rotr32(e, 25));
         const uint32_t ch = (e & f) ^ (~e & g);
         const uint32_t t1 = h + S1 + ch + Sha256Consts::K[t] + w[t & 15];
@@ -77,8 +84,17 @@ __device__ __forceinline__ void sha256_compress(uint32_t st[8], uint32_t w[16]) 
         h = g; g = f; f = e; e = d + t1;
         d = c; c = b; b = a; a = t1 + t2;
     }
-    st[0] += a; st[1] += b; st[2] += c; st[3] += d;
-    st[4] += e; st[5] += f; st[6] += g; st[7] += h;
+    v[0] = a; v[1] = b; v[2] = c; v[3] = d; v[4] = e; v[5] = f; v[6] = g; v[7] = h;
+}
+
+// One compression: st <- st + F(st, w).  w is clobbered (rolling schedule).
+__device__ __forceinline__ void sha256_compress(uint32_t st[8], uint32_t w[16]) {
+    uint32_t v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = st[i];
+    sha256_rounds<0, 64>(v, w);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) st[i] += v[i];
 }
 
 __device__ __forceinline__ void sha256_iv(uint32_t st[8]) {

@@ -57,6 +57,28 @@ __global__ __launch_bounds__(WG) void k_prepare(const uint8_t* __restrict__ seed
     keys[i] = k;
 }
 
+// gen_chain (wots.c:38-53): `steps` applications of thash_f from position
+// `start`; a[0..5] hold the chain's ADRS.  Families that can share the
+// ADRS-invariant PRF rounds along the chain do so.
+template <class F>
+__device__ __forceinline__ void run_chain(typename F::Node& x, const typename F::Key& k, uint32_t (&a)[8],
+                                          uint32_t start, uint32_t steps) {
+    if constexpr (F::kChainPrf) {
+        typename F::ChainPrf cp;
+        F::chain_prf_init(cp, k, a);
+        for (uint32_t i = start; i < start + steps; ++i) {
+            typename F::Node key, mask;
+            F::chain_prf_step(key, mask, k, cp, i);
+            F::f_outer(x, key, mask, x);
+        }
+    } else {
+        for (uint32_t i = start; i < start + steps; ++i) {
+            a[6] = i;
+            F::thash_f(x, k, a, x);
+        }
+    }
+}
+
 // Chain ends of `leaves` consecutive leaves per run, from run.leaf0 + leaf_off.
 template <class F>
 __global__ __launch_bounds__(WG) void k_chains(const typename F::Key* __restrict__ keys,
@@ -72,10 +94,7 @@ __global__ __launch_bounds__(WG) void k_chains(const typename F::Key* __restrict
                      chain, 0, 0};
     typename F::Node x;
     F::prf_keygen(x, k, a);
-    for (uint32_t i = 0; i < 15; ++i) {
-        a[6] = i;
-        F::thash_f(x, k, a, x);
-    }
+    run_chain<F>(x, k, a, 0, 15);
     F::store(x, ends + ((size_t)r * leaves * F::LEN + item) * F::kN);
 }
 
@@ -178,10 +197,7 @@ __global__ __launch_bounds__(WG) void k_wots(const typename F::Key* __restrict__
         if (q.mode == MODE_SIGN) steps = chain_len<F>(msgs + (size_t)r * F::kN, c);
         F::prf_keygen(x, k, a);
     }
-    for (uint32_t i = start; i < start + steps; ++i) {
-        a[6] = i;
-        F::thash_f(x, k, a, x);
-    }
+    run_chain<F>(x, k, a, start, steps);
     F::store(x, out + (size_t)g * F::kN);
 }
 
@@ -212,10 +228,7 @@ __global__ __launch_bounds__(192) void k_leaf_wg(const typename F::Key* __restri
         } else {
             F::prf_keygen(x, k, a);
         }
-        for (uint32_t i = start; i < start + steps; ++i) {
-            a[6] = i;
-            F::thash_f(x, k, a, x);
-        }
+        run_chain<F>(x, k, a, start, steps);
         pk[c] = x;
     }
     __syncthreads();
