@@ -54,6 +54,7 @@ def load_product(core):
     L.xmss_mi355x_sign_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.xmss_mi355x_verify_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.xmss_mi355x_job_free.argtypes = [C.c_void_p]
+    L.xmss_batch_keypair.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_char_p, C.c_uint32, C.c_size_t]
     return api
 
 
@@ -114,7 +115,6 @@ def bench_batch(api, count):
     """Batch keygen, then one plain-core signature per key, then verification
     of all of them.  Returns timings, kernel stats and the verify results."""
     L = api.lib
-    L.xmss_batch_keypair.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_char_p, C.c_uint32, C.c_size_t]
     rng = random.Random(SEED + 7)
     seeds, idxs, msgs = batch_inputs(count, rng)
     p = api.params(0x01, 0)
@@ -165,6 +165,60 @@ def bench_batch(api, count):
         if want == 0:
             assert raw(outs[i], mls[i]) == msgs[i]
     return keygen_t, sign_run, sign_total, ver_t, st, sum(1 for i in idxs if i == 1023)
+
+
+MIXED = [("XMSSMT-SHA2_20/2_256", 1, 0x01), ("XMSS-SHA2_10_512", 0, 0x04), ("XMSS-SHAKE_10_256", 0, 0x07)]
+
+
+def bench_mixed(api, per_set):
+    """The mixed-parameter batch: per_set keys of each of XMSSMT-SHA2_20/2_256,
+    XMSS-SHA2_10_512 and XMSS-SHAKE_10_256, one plain-core signature each at
+    a uniform leaf with a log-uniform 32 B - 4 KB message, then verification.
+    One batch per parameter set, run back to back; the time is their sum."""
+    L = api.lib
+    rng = random.Random(SEED + 11)
+    res = {}
+    total_sign = total_ver = 0.0
+    for name, mt, oid in MIXED:
+        p = api.params(oid, mt)
+        seeds = bytes(rng.getrandbits(8) for _ in range(3 * p.n * per_set))
+        pks = [buf(4 + p.pk_bytes) for _ in range(per_set)]
+        sks = [buf(4 + p.sk_bytes) for _ in range(per_set)]
+        assert L.xmss_batch_keypair(mt, ptrs(pks), ptrs(sks), seeds, oid, per_set) == 0
+        idxs = [rng.randrange(0, (1 << p.full_height) - 1) for _ in range(per_set)]
+        for i in range(per_set):
+            sks[i][4:4 + p.index_bytes] = list(idxs[i].to_bytes(p.index_bytes, "big"))
+        msgs = [bytes(rng.getrandbits(8) for _ in range(int(32 * 128 ** rng.random()))) for _ in range(per_set)]
+        ms = [buf(m) for m in msgs]
+        mlens = (C.c_ulonglong * per_set)(*[len(m) for m in msgs])
+        sms = [buf(p.sig_bytes + len(m)) for m in msgs]
+        smlens = (C.c_ulonglong * per_set)()
+        rets = (C.c_int * per_set)()
+        job = L.xmss_mi355x_sign_prepare(mt, ptrs(sks), ptrs(ms), mlens, per_set)
+        t0 = time.perf_counter()
+        L.xmss_mi355x_job_run(job)
+        ts = time.perf_counter() - t0
+        L.xmss_mi355x_sign_collect(job, ptrs(sms), smlens, rets)
+        L.xmss_mi355x_job_free(job)
+        assert all(rets[i] == 0 for i in range(per_set))
+        smb = [buf(raw(sms[i], smlens[i])) for i in range(per_set)]
+        outs = [buf(smlens[i]) for i in range(per_set)]
+        mls = (C.c_ulonglong * per_set)()
+        vr = (C.c_int * per_set)()
+        job = L.xmss_mi355x_verify_prepare(mt, ptrs(smb), smlens, ptrs(pks), per_set)
+        t0 = time.perf_counter()
+        L.xmss_mi355x_job_run(job)
+        tv = time.perf_counter() - t0
+        L.xmss_mi355x_verify_collect(job, ptrs(outs), mls, vr)
+        L.xmss_mi355x_job_free(job)
+        assert all(vr[i] == 0 for i in range(per_set))
+        res[name] = {"signs_per_s": round(per_set / ts, 1), "verifies_per_s": round(per_set / tv, 1)}
+        total_sign += ts
+        total_ver += tv
+        log("  %s: sign %.3f s, verify %.4f s" % (name, ts, tv))
+    res["total_sign_s"] = round(total_sign, 4)
+    res["total_verify_s"] = round(total_ver, 5)
+    return res
 
 
 def bench_latency(nsigns):
@@ -256,6 +310,7 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--latency-signs", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed", type=int, default=1024, help="keys per parameter set of the mixed batch (0: skip)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="MiB of one device-to-device copy, run before the keygens (HBM counter calibration)")
     args = ap.parse_args()
@@ -292,6 +347,10 @@ def main():
         log("batch keygen/sign/verify x%d ..." % args.batch)
         bkeygen_t, sign_run, sign_total, ver_t, bst, n_last = bench_batch(api, args.batch)
     rate = lambda t: None if not t else round(args.batch / t, 2)
+    mixed = None
+    if args.mixed:
+        log("mixed batch x%d per set ..." % args.mixed)
+        mixed = bench_mixed(api, args.mixed)
     log("latency (BDS) x%d ..." % args.latency_signs)
     p50, p99, v50 = bench_latency(args.latency_signs) if args.latency_signs else (None, None, None)
     cpu = None if args.no_cpu else cpu_baseline()
@@ -325,6 +384,7 @@ def main():
         "kernels_keygen_ms": {k: round(v[1], 4) for k, v in kst.items()},
         "kernels_batch_sign_ms": {k: round(v[1], 3) for k, v in bst.items()},
         "batch_last_index_rejected": n_last,
+        "mixed_batch": mixed,
     }
     print(json.dumps(out), flush=True)
 

@@ -413,48 +413,73 @@ struct Sha512Fam {
 #pragma unroll
         for (int i = 0; i < 8; ++i) out.w[i] = st[i];
     }
-    __device__ static void thash_f(Node& out, const Key& k, uint32_t (&a)[8], const Node& x) {
-        Node key, mask;
-        uint64_t ad[4];
-        a[7] = 0;
-        adrs64(ad, a);
-        prf4(key, k.ms_pub, ad);
-        a[7] = 1;
-        adrs64(ad, a);
-        prf4(mask, k.ms_pub, ad);
-        uint64_t m[24];
+    // The outer hash absorbs toByte(pad) || key first and the masked input
+    // after it, so each PRF result is consumed as soon as it exists: at most
+    // one 512-bit PRF output is live next to the running state.
+    __device__ static void absorb_block(uint64_t st[8], const uint64_t (&lo)[8], const uint64_t (&hi)[8]) {
+        uint64_t w[16];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) { m[i] = 0; m[8 + i] = key.w[i]; m[16 + i] = x.w[i] ^ mask.w[i]; }
+        for (int i = 0; i < 8; ++i) { w[i] = lo[i]; w[8 + i] = hi[i]; }
+        sha512_compress(st, w);
+    }
+    __device__ static void thash_f(Node& out, const Key& k, uint32_t (&a)[8], const Node& x) {
+        Node t;
+        uint64_t ad[4];
         uint64_t st[8];
         sha512_iv(st);
-        sha512_tail<24>(st, m, 192);
+        a[7] = 0;
+        adrs64(ad, a);
+        prf4(t, k.ms_pub, ad);  // key
+        {
+            const uint64_t z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            absorb_block(st, z, t.w);
+        }
+        a[7] = 1;
+        adrs64(ad, a);
+        prf4(t, k.ms_pub, ad);  // mask
+        {
+            uint64_t xm[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) xm[i] = x.w[i] ^ t.w[i];
+            const uint64_t pad[8] = {0x8000000000000000ULL, 0, 0, 0, 0, 0, 0, 192 * 8};
+            absorb_block(st, xm, pad);
+        }
 #pragma unroll
         for (int i = 0; i < 8; ++i) out.w[i] = st[i];
     }
     __device__ static void thash_h(Node& out, const Key& k, uint32_t (&a)[8], const Node& l,
                                    const Node& r) {
-        Node key, m0, m1;
+        Node t;
         uint64_t ad[4];
-        a[7] = 0;
-        adrs64(ad, a);
-        prf4(key, k.ms_pub, ad);
-        a[7] = 1;
-        adrs64(ad, a);
-        prf4(m0, k.ms_pub, ad);
-        a[7] = 2;
-        adrs64(ad, a);
-        prf4(m1, k.ms_pub, ad);
-        uint64_t m[32];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            m[i] = (i == 7) ? (uint64_t)PAD_H : 0;
-            m[8 + i] = key.w[i];
-            m[16 + i] = l.w[i] ^ m0.w[i];
-            m[24 + i] = r.w[i] ^ m1.w[i];
-        }
         uint64_t st[8];
         sha512_iv(st);
-        sha512_tail<32>(st, m, 256);
+        a[7] = 0;
+        adrs64(ad, a);
+        prf4(t, k.ms_pub, ad);  // key
+        {
+            const uint64_t p[8] = {0, 0, 0, 0, 0, 0, 0, (uint64_t)PAD_H};
+            absorb_block(st, p, t.w);
+        }
+        uint64_t lm[8];
+        a[7] = 1;
+        adrs64(ad, a);
+        prf4(t, k.ms_pub, ad);  // mask 0
+#pragma unroll
+        for (int i = 0; i < 8; ++i) lm[i] = l.w[i] ^ t.w[i];
+        a[7] = 2;
+        adrs64(ad, a);
+        prf4(t, k.ms_pub, ad);  // mask 1
+        {
+            uint64_t rm[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) rm[i] = r.w[i] ^ t.w[i];
+            absorb_block(st, lm, rm);
+        }
+        {
+            const uint64_t p0[8] = {0x8000000000000000ULL, 0, 0, 0, 0, 0, 0, 0};
+            const uint64_t p1[8] = {0, 0, 0, 0, 0, 0, 0, 256 * 8};
+            absorb_block(st, p0, p1);
+        }
 #pragma unroll
         for (int i = 0; i < 8; ++i) out.w[i] = st[i];
     }

@@ -184,27 +184,72 @@ __device__ __forceinline__ uint64_t maj3_64(uint64_t a, uint64_t b, uint64_t c) 
     return ((uint64_t)hi << 32) | lo;
 }
 
-__device__ __forceinline__ void sha512_compress(uint64_t st[8], uint64_t w[16]) {
-    uint64_t a = st[0], b = st[1], c = st[2], d = st[3];
-    uint64_t e = st[4], f = st[5], g = st[6], h = st[7];
-#pragma unroll
-    for (int t = 0; t < 80; ++t) {
-        if (t >= 16) {
-            const uint64_t w15 = w[(t + 1) & 15], w2 = w[(t + 14) & 15];
-            const uint64_t s0 = xor3_64(rotr64(w15, 1), rotr64(w15, 8), w15 >> 7);
-            const uint64_t s1 = xor3_64(rotr64(w2, 19), rotr64(w2, 61), w2 >> 6);
-            w[t & 15] = w[t & 15] + s0 + s1 + w[(t + 9) & 15];
-        }
-        const uint64_t S1 = xor3_64(rotr64(e, 14), rotr64(e, 18), rotr64(e, 41));
-        const uint64_t ch = (e & f) ^ (~e & g);
-        const uint64_t t1 = h + S1 + ch + Sha512Consts::K[t] + w[t & 15];
-        const uint64_t S0 = xor3_64(rotr64(a, 28), rotr64(a, 34), rotr64(a, 39));
-        const uint64_t t2 = S0 + maj3_64(a, b, c);
-        h = g; g = f; f = e; e = d + t1;
-        d = c; c = b; b = a; a = t1 + t2;
+__device__ __constant__ static const uint64_t kSha512K[80] = {
+    0x428a2f98d728ae22ULL, 0x7137449123ef65cdULL, 0xb5c0fbcfec4d3b2fULL, 0xe9b5dba58189dbbcULL,
+    0x3956c25bf348b538ULL, 0x59f111f1b605d019ULL, 0x923f82a4af194f9bULL, 0xab1c5ed5da6d8118ULL,
+    0xd807aa98a3030242ULL, 0x12835b0145706fbeULL, 0x243185be4ee4b28cULL, 0x550c7dc3d5ffb4e2ULL,
+    0x72be5d74f27b896fULL, 0x80deb1fe3b1696b1ULL, 0x9bdc06a725c71235ULL, 0xc19bf174cf692694ULL,
+    0xe49b69c19ef14ad2ULL, 0xefbe4786384f25e3ULL, 0x0fc19dc68b8cd5b5ULL, 0x240ca1cc77ac9c65ULL,
+    0x2de92c6f592b0275ULL, 0x4a7484aa6ea6e483ULL, 0x5cb0a9dcbd41fbd4ULL, 0x76f988da831153b5ULL,
+    0x983e5152ee66dfabULL, 0xa831c66d2db43210ULL, 0xb00327c898fb213fULL, 0xbf597fc7beef0ee4ULL,
+    0xc6e00bf33da88fc2ULL, 0xd5a79147930aa725ULL, 0x06ca6351e003826fULL, 0x142929670a0e6e70ULL,
+    0x27b70a8546d22ffcULL, 0x2e1b21385c26c926ULL, 0x4d2c6dfc5ac42aedULL, 0x53380d139d95b3dfULL,
+    0x650a73548baf63deULL, 0x766a0abb3c77b2a8ULL, 0x81c2c92e47edaee6ULL, 0x92722c851482353bULL,
+    0xa2bfe8a14cf10364ULL, 0xa81a664bbc423001ULL, 0xc24b8b70d0f89791ULL, 0xc76c51a30654be30ULL,
+    0xd192e819d6ef5218ULL, 0xd69906245565a910ULL, 0xf40e35855771202aULL, 0x106aa07032bbd1b8ULL,
+    0x19a4c116b8d2d0c8ULL, 0x1e376c085141ab53ULL, 0x2748774cdf8eeb99ULL, 0x34b0bcb5e19b48a8ULL,
+    0x391c0cb3c5c95a63ULL, 0x4ed8aa4ae3418acbULL, 0x5b9cca4f7763e373ULL, 0x682e6ff3d6b2b8a3ULL,
+    0x748f82ee5defb2fcULL, 0x78a5636f43172f60ULL, 0x84c87814a1f0ab72ULL, 0x8cc702081a6439ecULL,
+    0x90befffa23631e28ULL, 0xa4506cebde82bde9ULL, 0xbef9a3f7b2c67915ULL, 0xc67178f2e372532bULL,
+    0xca273eceea26619cULL, 0xd186b8c721c0c207ULL, 0xeada7dd6cde0eb1eULL, 0xf57d4f7fee6ed178ULL,
+    0x06f067aa72176fbaULL, 0x0a637dc5a2c898a6ULL, 0x113f9804bef90daeULL, 0x1b710b35131c471bULL,
+    0x28db77f523047d84ULL, 0x32caab7b40c72493ULL, 0x3c9ebe0a15c9bebcULL, 0x431d67c49c100d4cULL,
+    0x4cc5d4becb3e42b6ULL, 0x597f299cfc657e2aULL, 0x5fcb6fab3ad6faecULL, 0x6c44198c4a475817ULL};
+
+template <bool SCHED>
+__device__ __forceinline__ void sha512_round(uint64_t* v, uint64_t* w, int t, int j, uint64_t k) {
+    // v holds a..h rotated so that round j of an 8-round group reads fixed slots
+    uint64_t& a = v[(8 - j) & 7];
+    uint64_t& b = v[(9 - j) & 7];
+    uint64_t& c = v[(10 - j) & 7];
+    uint64_t& d = v[(11 - j) & 7];
+    uint64_t& e = v[(12 - j) & 7];
+    uint64_t& f = v[(13 - j) & 7];
+    uint64_t& g = v[(14 - j) & 7];
+    uint64_t& h = v[(15 - j) & 7];
+    (void)t;
+    const int s = j & 15;
+    if (SCHED) {
+        const uint64_t w15 = w[(s + 1) & 15], w2 = w[(s + 14) & 15];
+        const uint64_t s0 = xor3_64(rotr64(w15, 1), rotr64(w15, 8), w15 >> 7);
+        const uint64_t s1 = xor3_64(rotr64(w2, 19), rotr64(w2, 61), w2 >> 6);
+        w[s] = w[s] + s0 + s1 + w[(s + 9) & 15];
     }
-    st[0] += a; st[1] += b; st[2] += c; st[3] += d;
-    st[4] += e; st[5] += f; st[6] += g; st[7] += h;
+    const uint64_t S1 = xor3_64(rotr64(e, 14), rotr64(e, 18), rotr64(e, 41));
+    const uint64_t ch = (e & f) ^ (~e & g);
+    const uint64_t t1 = h + S1 + ch + k + w[s];
+    const uint64_t S0 = xor3_64(rotr64(a, 28), rotr64(a, 34), rotr64(a, 39));
+    d += t1;
+    h = t1 + S0 + maj3_64(a, b, c);
+}
+
+// One SHA-512 compression.  Rounds 0..15 are unrolled (constant message
+// words fold); rounds 16..79 run as a loop of 16-round groups with K read
+// through the scalar cache, which keeps the scheduling window -- and so the
+// number of live 64-bit values -- small enough to avoid spills.
+__device__ __forceinline__ void sha512_compress(uint64_t st[8], uint64_t w[16]) {
+    uint64_t v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = st[i];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) sha512_round<false>(v, w, j, j, Sha512Consts::K[j]);
+#pragma unroll 1
+    for (int t0 = 16; t0 < 80; t0 += 16) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) sha512_round<true>(v, w, t0 + j, j, kSha512K[t0 + j]);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) st[i] += v[i];
 }
 
 __device__ __forceinline__ void sha512_iv(uint64_t st[8]) {
