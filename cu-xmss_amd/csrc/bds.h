@@ -1,0 +1,65 @@
+// bds.h -- the BDS signing state machine (core_bds.cpp) and the hashing
+// backend it runs on.  The C API drives it with a backend that launches on the
+// engine directly; the serving loop (serve.cpp) with one that merges the work
+// of many keys into shared launches.
+#pragma once
+#include <stdint.h>
+
+#include <tuple>
+#include <vector>
+
+#include "../../include/params.h"
+
+namespace xm {
+
+struct LeafId {
+    uint32_t layer;
+    uint64_t tree;
+    uint32_t leaf;
+    bool operator<(const LeafId& o) const {
+        return std::tie(layer, tree, leaf) < std::tie(o.layer, o.tree, o.leaf);
+    }
+    bool operator==(const LeafId& o) const { return layer == o.layer && tree == o.tree && leaf == o.leaf; }
+};
+
+// Every hash one BDS signature needs, for the key bound last.
+struct BdsOps {
+    virtual ~BdsOps() {}
+    // The key the following calls hash for (SK_SEED, SK_PRF, PUB_SEED as the
+    // signature reads them from sk, i.e. after a possible wipe).
+    virtual void bind_key(const unsigned char* sk_seed, const unsigned char* sk_prf,
+                          const unsigned char* pub_seed) = 0;
+    // gen_leaf_wots of one leaf (xmss_commons.c:104-113)
+    virtual void leaf(unsigned char* out, const LeafId& id) = 0;
+    // thash_h with a full ADRS (hash.c:125-155)
+    virtual void thash_h(unsigned char* out, const unsigned char* in, const uint32_t adrs[8]) = 0;
+    // wots_sign of an n-byte message at (layer, tree, ots) (wots.c:136-154)
+    virtual void wots_sign(unsigned char* sig, const unsigned char* msg, uint32_t layer, uint64_t tree,
+                           uint32_t ots) = 0;
+    // R = PRF(SK_PRF, idx) and H_msg(R, root, idx, M) (xmss_core_fast.c:788-797)
+    virtual void hmsg_sign(unsigned char* R, unsigned char* digest, uint64_t idx, const unsigned char* root,
+                           const unsigned char* m, uint64_t mlen) = 0;
+};
+
+// Backend that computes nothing and records which leaves were asked for.
+struct RecordingOps final : BdsOps {
+    unsigned n;
+    std::vector<LeafId> leaves;
+    explicit RecordingOps(unsigned n_) : n(n_) {}
+    void bind_key(const unsigned char*, const unsigned char*, const unsigned char*) override {}
+    void leaf(unsigned char* out, const LeafId& id) override;
+    void thash_h(unsigned char* out, const unsigned char*, const uint32_t*) override;
+    void wots_sign(unsigned char* sig, const unsigned char*, uint32_t, uint64_t, uint32_t) override;
+    void hmsg_sign(unsigned char* R, unsigned char* digest, uint64_t, const unsigned char*, const unsigned char*,
+                   uint64_t) override;
+};
+
+// One BDS signature (xmss_core_fast.c:710-851 for d = 1, :917-1121 for d > 1)
+// with all hashing on `ops`.  sk is the OID-less secret key.
+int bds_sign(const xmss_params* p, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
+             const unsigned char* m, unsigned long long mlen, BdsOps& ops);
+
+// The leaves bds_sign would consume for this sk (a dry run on a copy).
+std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* sk, unsigned long long mlen);
+
+}  // namespace xm

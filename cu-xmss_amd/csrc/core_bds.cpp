@@ -7,17 +7,22 @@
 // arrays are views into sk, integer fields are read on load and written back
 // on store, so fields the reference never assigns keep the caller's bytes.
 // Every hash it needs -- leaves (gen_leaf_wots), node merges (thash_h), WOTS
-// signatures, R and H_msg, and the keygen trees -- is computed on the GPU.
-// Leaves are fetched in batches: each time a leaf is missing, the leaves the
-// state can ask for next (the next leaf of every unfinished treehash
-// instance, the next NEXT-tree leaves) are computed in one launch.
+// signatures, R and H_msg, and the keygen trees -- is computed on the GPU,
+// through a BdsOps backend (bds.h).
+//
+// Control flow never depends on hash values, only on the integer state, so
+// a signature first runs on a copy of sk with a recording backend: that dry
+// run lists exactly the leaves the signature will consume, which are then
+// computed in a single launch before the real run.
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <vector>
 
 #include "../../include/randombytes.h"
 #include "../../include/xmss_core.h"
+#include "bds.h"
 #include "core_common.h"
 
 using namespace xm;
@@ -43,45 +48,6 @@ struct BdsState {
     TreehashInst* th;
     unsigned char* retain;
     unsigned int next_leaf;
-};
-
-// GPU-backed hashing for one key during one call.
-struct Hasher {
-    Engine* e;
-    int fam;
-    unsigned n;
-    // leaves computed so far in this call, keyed by (layer, tree, leaf)
-    std::map<std::tuple<uint32_t, uint64_t, uint32_t>, std::vector<unsigned char>> cache;
-    std::vector<LeafRun> want;  // speculative leaves for the next miss
-
-    void leaf(unsigned char* out, const uint32_t addr[8], uint32_t leaf_idx) {
-        const uint64_t tree = ((uint64_t)addr[1] << 32) | addr[2];
-        auto k = std::make_tuple(addr[0], tree, leaf_idx);
-        auto it = cache.find(k);
-        if (it == cache.end()) {
-            std::vector<LeafRun> runs{{0, addr[0], tree, leaf_idx}};
-            for (const LeafRun& r : want) {
-                if (!cache.count(std::make_tuple(r.layer, r.tree, r.leaf0)) &&
-                    !(r.layer == addr[0] && r.tree == tree && r.leaf0 == leaf_idx))
-                    runs.push_back(r);
-            }
-            want.clear();
-            std::vector<unsigned char> out_all(runs.size() * n);
-            e->leaves(fam, (int)runs.size(), runs.data(), 1, out_all.data());
-            for (size_t i = 0; i < runs.size(); ++i) {
-                cache[std::make_tuple(runs[i].layer, runs[i].tree, runs[i].leaf0)] =
-                    std::vector<unsigned char>(out_all.begin() + i * n, out_all.begin() + (i + 1) * n);
-            }
-            it = cache.find(k);
-        }
-        memcpy(out, it->second.data(), n);
-    }
-    void thash_h(unsigned char* out, const unsigned char* in, const uint32_t adrs[8]) {
-        ThashReq q;
-        q.key = 0;
-        memcpy(q.adrs, adrs, 32);
-        e->thash_h(fam, 1, &q, in, out);
-    }
 };
 
 void subtree_addrs(const uint32_t addr[8], uint32_t nd[8]) {
@@ -224,13 +190,13 @@ unsigned treehash_minheight_on_stack(const xmss_params* p, const BdsState* st, c
 }
 
 // treehash_update (xmss_core_fast.c:406-452)
-void treehash_update(const xmss_params* p, Hasher& H, TreehashInst* t, BdsState* st, const uint32_t addr[8]) {
+void treehash_update(const xmss_params* p, BdsOps& H, TreehashInst* t, BdsState* st, const uint32_t addr[8]) {
     const unsigned n = p->n;
     uint32_t nd[8];
     subtree_addrs(addr, nd);
     unsigned char nb[2 * 64];
     unsigned nodeheight = 0;
-    H.leaf(nb, addr, (uint32_t)t->next_idx);
+    H.leaf(nb, {addr[0], ((uint64_t)addr[1] << 32) | addr[2], (uint32_t)t->next_idx});
     while (t->stackusage > 0 && st->stacklevels[st->stackoffset - 1] == nodeheight) {
         memcpy(nb + n, nb, n);
         memcpy(nb, st->stack + (st->stackoffset - 1) * n, n);
@@ -253,29 +219,12 @@ void treehash_update(const xmss_params* p, Hasher& H, TreehashInst* t, BdsState*
     }
 }
 
-// Leaves the update loop below can ask for: the next leaf of every unfinished
-// treehash instance (up to `budget` consecutive ones each).
-void want_treehash_leaves(const xmss_params* p, Hasher& H, const BdsState* st, unsigned budget,
-                          const uint32_t addr[8]) {
-    const uint64_t tree = ((uint64_t)addr[1] << 32) | addr[2];
-    for (unsigned i = 0; i < p->tree_height - p->bds_k; ++i) {
-        if (st->th[i].completed) continue;
-        const unsigned long span = 1ul << st->th[i].h;
-        for (unsigned k = 0; k < budget && k < span; ++k) {
-            const unsigned long leaf = st->th[i].next_idx + k;
-            if (leaf >= (1ul << p->tree_height)) break;
-            H.want.push_back({0, addr[0], tree, (uint32_t)leaf});
-        }
-    }
-}
-
 // bds_treehash_update (xmss_core_fast.c:458-491).  The reference returns
 // `char`; the result is narrowed and sign-extended the same way.
-unsigned bds_treehash_update(const xmss_params* p, Hasher& H, BdsState* st, unsigned updates,
+unsigned bds_treehash_update(const xmss_params* p, BdsOps& H, BdsState* st, unsigned updates,
                              const uint32_t addr[8]) {
     const unsigned nth = p->tree_height - p->bds_k;
     unsigned used = 0;
-    want_treehash_leaves(p, H, st, updates < 16 ? updates : 16, addr);
     for (unsigned j = 0; j < updates; ++j) {
         unsigned l_min = p->tree_height, level = nth;
         for (unsigned i = 0; i < nth; ++i) {
@@ -296,13 +245,13 @@ unsigned bds_treehash_update(const xmss_params* p, Hasher& H, BdsState* st, unsi
 }
 
 // bds_state_update (xmss_core_fast.c:497-552)
-int bds_state_update(const xmss_params* p, Hasher& H, BdsState* st, const uint32_t addr[8]) {
+int bds_state_update(const xmss_params* p, BdsOps& H, BdsState* st, const uint32_t addr[8]) {
     const unsigned n = p->n;
     const int idx = (int)st->next_leaf;
     if (idx == 1 << p->tree_height) return -1;
     uint32_t nd[8];
     subtree_addrs(addr, nd);
-    H.leaf(st->stack + st->stackoffset * n, addr, (uint32_t)idx);
+    H.leaf(st->stack + st->stackoffset * n, {addr[0], ((uint64_t)addr[1] << 32) | addr[2], (uint32_t)idx});
     st->stacklevels[st->stackoffset] = 0;
     ++st->stackoffset;
     while (st->stackoffset > 1 && st->stacklevels[st->stackoffset - 1] == st->stacklevels[st->stackoffset - 2]) {
@@ -320,7 +269,7 @@ int bds_state_update(const xmss_params* p, Hasher& H, BdsState* st, const uint32
 }
 
 // bds_round (xmss_core_fast.c:559-626)
-void bds_round(const xmss_params* p, Hasher& H, BdsState* st, unsigned long leaf_idx, const uint32_t addr[8]) {
+void bds_round(const xmss_params* p, BdsOps& H, BdsState* st, unsigned long leaf_idx, const uint32_t addr[8]) {
     const unsigned n = p->n, th = p->tree_height, nth = th - p->bds_k;
     unsigned tau = th;
     unsigned char buf[2 * 64];
@@ -340,7 +289,7 @@ void bds_round(const xmss_params* p, Hasher& H, BdsState* st, unsigned long leaf
         memcpy(st->keep + (tau >> 1) * n, st->auth + tau * n, n);
     }
     if (tau == 0) {
-        H.leaf(st->auth, addr, (uint32_t)leaf_idx);
+        H.leaf(st->auth, {addr[0], ((uint64_t)addr[1] << 32) | addr[2], (uint32_t)leaf_idx});
     } else {
         nd[5] = tau - 1;
         nd[6] = (uint32_t)(leaf_idx >> tau);
@@ -374,19 +323,210 @@ int prepare_engine(const xmss_params* p, Engine** e, int* fam) {
     return *e ? 0 : XMSS_ENODEV;
 }
 
-// R, H_msg and the layer-0 WOTS signature of a BDS sign, all on the GPU.
-void sign_front(const xmss_params* p, Engine* e, int fam, unsigned long long idx, const unsigned char* pub_root,
-                unsigned char* sm, const unsigned char* m_in_sm, unsigned long long mlen, unsigned char* R,
-                unsigned char* wots_out, uint32_t layer, uint64_t tree, uint32_t ots) {
-    unsigned char mh[64];
-    MsgReq mr{0, 1, idx, nullptr, pub_root, m_in_sm, mlen};
-    e->hmsg(fam, 1, &mr, R, mh);
-    WotsReq w{0, layer, tree, ots, WOTS_SIGN};
-    e->wots(fam, 1, &w, mh, nullptr, wots_out);
-    (void)sm;
+}  // namespace
+
+namespace xm {
+
+void RecordingOps::leaf(unsigned char* out, const LeafId& id) {
+    leaves.push_back(id);
+    memset(out, 0, n);
+}
+void RecordingOps::thash_h(unsigned char* out, const unsigned char*, const uint32_t*) { memset(out, 0, n); }
+void RecordingOps::wots_sign(unsigned char* sig, const unsigned char*, uint32_t, uint64_t, uint32_t) {
+    memset(sig, 0, (size_t)(2 * n + 3) * n);
+}
+void RecordingOps::hmsg_sign(unsigned char* R, unsigned char* digest, uint64_t, const unsigned char*,
+                             const unsigned char*, uint64_t) {
+    memset(R, 0, n);
+    memset(digest, 0, n);
 }
 
-}  // namespace
+// xmss_core_sign (xmss_core_fast.c:710-851)
+static int sign_xmss(const xmss_params* p, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
+                     const unsigned char* m, unsigned long long mlen, BdsOps& ops) {
+    const unsigned n = p->n, th = p->tree_height;
+    std::vector<TreehashInst> ths(th - p->bds_k + 1);
+    BdsState st;
+    load_states(p, &st, ths.data(), nullptr, sk);
+    const unsigned long idx = ((unsigned long)sk[0] << 24) | ((unsigned long)sk[1] << 16) |
+                              ((unsigned long)sk[2] << 8) | sk[3];
+    if (index_check(p, sk, idx)) return -2;
+    unsigned char sk_seed[64], sk_prf[64], pub_seed[64], pub_root[64];
+    memcpy(sk_seed, sk + p->index_bytes, n);
+    memcpy(sk_prf, sk + p->index_bytes + n, n);
+    memcpy(pub_seed, sk + p->index_bytes + 3 * n, n);
+    memcpy(pub_root, sk + p->index_bytes + 2 * n, n);
+    sk[0] = ((idx + 1) >> 24) & 255;
+    sk[1] = ((idx + 1) >> 16) & 255;
+    sk[2] = ((idx + 1) >> 8) & 255;
+    sk[3] = (idx + 1) & 255;
+    ops.bind_key(sk_seed, sk_prf, pub_seed);
+    memmove(sm + p->sig_bytes, m, mlen);
+    unsigned char R[64], mh[64];
+    ops.hmsg_sign(R, mh, idx, pub_root, sm + p->sig_bytes, mlen);
+    ops.wots_sign(sm + 4 + n, mh, 0, 0, (uint32_t)idx);
+    sm[0] = (idx >> 24) & 255;
+    sm[1] = (idx >> 16) & 255;
+    sm[2] = (idx >> 8) & 255;
+    sm[3] = idx & 255;
+    memcpy(sm + 4, R, n);
+    memcpy(sm + 4 + n + p->wots_sig_bytes, st.auth, th * n);
+    const uint32_t ots_addr[8] = {0, 0, 0, 0, (uint32_t)idx, 0, 0, 0};
+    if (idx < (1UL << th) - 1) {
+        bds_round(p, ops, &st, idx, ots_addr);
+        bds_treehash_update(p, ops, &st, (th - p->bds_k) >> 1, ots_addr);
+    }
+    *smlen = p->sig_bytes + mlen;
+    store_states(p, sk, &st);
+    return 0;
+}
+
+// xmssmt_core_sign (xmss_core_fast.c:917-1121)
+static int sign_xmssmt(const xmss_params* p, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
+                       const unsigned char* m, unsigned long long mlen, BdsOps& ops) {
+    const unsigned n = p->n, th = p->tree_height, d = p->d, nth = th - p->bds_k;
+    std::vector<BdsState> st(2 * d - 1);
+    std::vector<TreehashInst> ths((2 * d - 1) * nth + 1);
+    unsigned char* wots_sigs;
+    load_states(p, st.data(), ths.data(), &wots_sigs, sk);
+    unsigned long long idx = 0;
+    for (unsigned i = 0; i < p->index_bytes; ++i) idx |= ((unsigned long long)sk[i]) << 8 * (p->index_bytes - 1 - i);
+    if (index_check(p, sk, idx)) return -2;
+    unsigned char sk_seed[64], sk_prf[64], pub_seed[64], pub_root[64];
+    memcpy(sk_seed, sk + p->index_bytes, n);
+    memcpy(sk_prf, sk + p->index_bytes + n, n);
+    memcpy(pub_seed, sk + p->index_bytes + 3 * n, n);
+    memcpy(pub_root, sk + p->index_bytes + 2 * n, n);
+    for (unsigned i = 0; i < p->index_bytes; ++i) sk[i] = ((idx + 1) >> 8 * (p->index_bytes - 1 - i)) & 255;
+    ops.bind_key(sk_seed, sk_prf, pub_seed);
+    memmove(sm + p->sig_bytes, m, mlen);
+    uint64_t idx_tree = idx >> th;
+    uint32_t idx_leaf = (uint32_t)(idx & ((1u << th) - 1));
+    unsigned char R[64], mh[64];
+    ops.hmsg_sign(R, mh, idx, pub_root, sm + p->sig_bytes, mlen);
+    ops.wots_sign(sm + p->index_bytes + n, mh, 0, idx_tree, idx_leaf);
+    for (unsigned i = 0; i < p->index_bytes; ++i) sm[i] = (idx >> 8 * (p->index_bytes - 1 - i)) & 255;
+    memcpy(sm + p->index_bytes, R, n);
+    unsigned char* s = sm + p->index_bytes + n + p->wots_sig_bytes;
+    memcpy(s, st[0].auth, th * n);
+    s += th * n;
+    for (unsigned i = 1; i < d; ++i) {
+        memcpy(s, wots_sigs + (i - 1) * p->wots_sig_bytes, p->wots_sig_bytes);
+        s += p->wots_sig_bytes;
+        memcpy(s, st[i].auth, th * n);
+        s += th * n;
+    }
+
+    uint32_t addr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int needswap_upto = -1;
+    unsigned updates = (th - p->bds_k) >> 1;
+    addr[1] = (uint32_t)((idx_tree + 1) >> 32);
+    addr[2] = (uint32_t)(idx_tree + 1);
+    if ((1 + idx_tree) * (1ULL << th) + idx_leaf < (1ULL << p->full_height)) {
+        bds_state_update(p, ops, &st[d], addr);
+    }
+    for (unsigned i = 0; i < d; ++i) {
+        if (!(((idx + 1) & ((1ULL << ((i + 1) * th)) - 1)) == 0)) {
+            idx_leaf = (uint32_t)((idx >> (th * i)) & ((1u << th) - 1));
+            idx_tree = idx >> (th * (i + 1));
+            addr[0] = i;
+            addr[1] = (uint32_t)(idx_tree >> 32);
+            addr[2] = (uint32_t)idx_tree;
+            if ((int)i == needswap_upto + 1) bds_round(p, ops, &st[i], idx_leaf, addr);
+            updates = bds_treehash_update(p, ops, &st[i], updates, addr);
+            addr[1] = (uint32_t)((idx_tree + 1) >> 32);
+            addr[2] = (uint32_t)(idx_tree + 1);
+            if ((1 + idx_tree) * (1ULL << th) + idx_leaf < (1ULL << (p->full_height - th * i))) {
+                if (i > 0 && updates > 0 && st[d + i].next_leaf < (1ULL << p->full_height)) {
+                    bds_state_update(p, ops, &st[d + i], addr);
+                    --updates;
+                }
+            }
+        } else if (idx < (1ULL << p->full_height) - 1) {
+            swap_states(p, &st[d + i], &st[i]);
+            const uint64_t t = (idx + 1) >> ((i + 2) * th);
+            const uint32_t ots = (uint32_t)(((idx >> ((i + 1) * th)) + 1) & ((1u << th) - 1));
+            ops.wots_sign(wots_sigs + i * p->wots_sig_bytes, st[i].stack, i + 1, t, ots);
+            st[d + i].stackoffset = 0;
+            st[d + i].next_leaf = 0;
+            --updates;  // WOTS signing counts as one update (:1107)
+            needswap_upto = (int)i;
+            for (unsigned j = 0; j < nth; ++j) st[i].th[j].completed = 1;
+        }
+    }
+    *smlen = p->sig_bytes + mlen;
+    store_states(p, sk, st.data());
+    return 0;
+}
+
+int bds_sign(const xmss_params* p, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
+             const unsigned char* m, unsigned long long mlen, BdsOps& ops) {
+    return p->d == 1 ? sign_xmss(p, sk, sm, smlen, m, mlen, ops) : sign_xmssmt(p, sk, sm, smlen, m, mlen, ops);
+}
+
+std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* sk, unsigned long long mlen) {
+    std::vector<unsigned char> skc(sk, sk + p->sk_bytes);
+    std::vector<unsigned char> smc(p->sig_bytes + mlen + 1), mc(mlen + 1, 0);
+    unsigned long long l = 0;
+    RecordingOps rec(p->n);
+    bds_sign(p, skc.data(), smc.data(), &l, mc.data(), mlen, rec);
+    std::vector<LeafId> v = rec.leaves;
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    return v;
+}
+
+// Backend of the C API: one key, launches straight on the engine; the
+// planned leaves are computed in one launch when the key is bound.
+struct DirectOps final : BdsOps {
+    Engine* e;
+    int fam;
+    unsigned n;
+    std::vector<LeafId> plan;
+    std::map<LeafId, std::vector<unsigned char>> cache;
+    DirectOps(Engine* e_, int fam_, unsigned n_, std::vector<LeafId> plan_)
+        : e(e_), fam(fam_), n(n_), plan(std::move(plan_)) {}
+    void bind_key(const unsigned char* sk_seed, const unsigned char* sk_prf, const unsigned char* pub_seed) override {
+        KeySeeds ks{sk_seed, sk_prf, pub_seed};
+        e->set_keys(fam, 1, &ks);
+        fetch(plan);
+    }
+    void fetch(const std::vector<LeafId>& ids) {
+        if (ids.empty()) return;
+        std::vector<LeafRun> runs(ids.size());
+        for (size_t i = 0; i < ids.size(); ++i) runs[i] = {0, ids[i].layer, ids[i].tree, ids[i].leaf};
+        std::vector<unsigned char> out(ids.size() * n);
+        e->leaves(fam, (int)runs.size(), runs.data(), 1, out.data());
+        for (size_t i = 0; i < ids.size(); ++i)
+            cache[ids[i]] = std::vector<unsigned char>(out.begin() + i * n, out.begin() + (i + 1) * n);
+    }
+    void leaf(unsigned char* out, const LeafId& id) override {
+        auto it = cache.find(id);
+        if (it == cache.end()) {
+            fetch({id});
+            it = cache.find(id);
+        }
+        memcpy(out, it->second.data(), n);
+    }
+    void thash_h(unsigned char* out, const unsigned char* in, const uint32_t adrs[8]) override {
+        ThashReq q;
+        q.key = 0;
+        memcpy(q.adrs, adrs, 32);
+        e->thash_h(fam, 1, &q, in, out);
+    }
+    void wots_sign(unsigned char* sig, const unsigned char* msg, uint32_t layer, uint64_t tree,
+                   uint32_t ots) override {
+        WotsReq w{0, layer, tree, ots, WOTS_SIGN};
+        e->wots(fam, 1, &w, msg, nullptr, sig);
+    }
+    void hmsg_sign(unsigned char* R, unsigned char* digest, uint64_t idx, const unsigned char* root,
+                   const unsigned char* m, uint64_t mlen) override {
+        MsgReq mr{0, 1, idx, nullptr, root, m, mlen};
+        e->hmsg(fam, 1, &mr, R, digest);
+    }
+};
+
+}  // namespace xm
 
 extern "C" {
 
@@ -422,53 +562,6 @@ int xmss_core_keypair(const xmss_params* params, unsigned char* pk, unsigned cha
     BdsState* sp = &st;
     treehash_init_many(p, e, fam, 1, &layer, &sp, pk);
     memcpy(sk + p->index_bytes + 2 * p->n, pk, p->n);
-    store_states(p, sk, &st);
-    return 0;
-}
-
-// xmss_core_fast.c:710-851
-int xmss_core_sign(const xmss_params* params, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
-                   const unsigned char* m, unsigned long long mlen) {
-    const xmss_params* p = params;
-    const unsigned n = p->n, th = p->tree_height;
-    std::lock_guard<std::mutex> lk(engine_mutex());
-    Engine* e;
-    int fam;
-    if (int rc = prepare_engine(p, &e, &fam)) return rc;
-    std::vector<TreehashInst> ths(th - p->bds_k + 1);
-    BdsState st;
-    load_states(p, &st, ths.data(), nullptr, sk);
-    const unsigned long idx = ((unsigned long)sk[0] << 24) | ((unsigned long)sk[1] << 16) |
-                              ((unsigned long)sk[2] << 8) | sk[3];
-    if (index_check(p, sk, idx)) return -2;
-    unsigned char sk_seed[64], sk_prf[64], pub_seed[64], pub_root[64];
-    memcpy(sk_seed, sk + p->index_bytes, n);
-    memcpy(sk_prf, sk + p->index_bytes + n, n);
-    memcpy(pub_seed, sk + p->index_bytes + 3 * n, n);
-    memcpy(pub_root, sk + p->index_bytes + 2 * n, n);
-    sk[0] = ((idx + 1) >> 24) & 255;
-    sk[1] = ((idx + 1) >> 16) & 255;
-    sk[2] = ((idx + 1) >> 8) & 255;
-    sk[3] = (idx + 1) & 255;
-    KeySeeds ks{sk_seed, sk_prf, pub_seed};
-    e->set_keys(fam, 1, &ks);
-    memmove(sm + p->sig_bytes, m, mlen);
-    unsigned char R[64];
-    sign_front(p, e, fam, idx, pub_root, sm, sm + p->sig_bytes, mlen, R, sm + 4 + n, 0, 0, (uint32_t)idx);
-    sm[0] = (idx >> 24) & 255;
-    sm[1] = (idx >> 16) & 255;
-    sm[2] = (idx >> 8) & 255;
-    sm[3] = idx & 255;
-    memcpy(sm + 4, R, n);
-    unsigned char* s = sm + 4 + n + p->wots_sig_bytes;
-    memcpy(s, st.auth, th * n);
-    uint32_t ots_addr[8] = {0, 0, 0, ADDR_OTS, (uint32_t)idx, 0, 0, 0};
-    if (idx < (1UL << th) - 1) {
-        Hasher H{e, fam, n, {}, {}};
-        bds_round(p, H, &st, idx, ots_addr);
-        bds_treehash_update(p, H, &st, (th - p->bds_k) >> 1, ots_addr);
-    }
-    *smlen = p->sig_bytes + mlen;
     store_states(p, sk, &st);
     return 0;
 }
@@ -517,95 +610,26 @@ int xmssmt_core_keypair(const xmss_params* params, unsigned char* pk, unsigned c
     return 0;
 }
 
-// xmss_core_fast.c:917-1121
-int xmssmt_core_sign(const xmss_params* params, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
-                     const unsigned char* m, unsigned long long mlen) {
-    const xmss_params* p = params;
-    const unsigned n = p->n, th = p->tree_height, d = p->d, nth = th - p->bds_k;
+static int bds_sign_api(const xmss_params* p, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
+                        const unsigned char* m, unsigned long long mlen) {
     std::lock_guard<std::mutex> lk(engine_mutex());
     Engine* e;
     int fam;
     if (int rc = prepare_engine(p, &e, &fam)) return rc;
-    std::vector<BdsState> st(2 * d - 1);
-    std::vector<TreehashInst> ths((2 * d - 1) * nth + 1);
-    unsigned char* wots_sigs;
-    load_states(p, st.data(), ths.data(), &wots_sigs, sk);
-    unsigned long long idx = 0;
-    for (unsigned i = 0; i < p->index_bytes; ++i) idx |= ((unsigned long long)sk[i]) << 8 * (p->index_bytes - 1 - i);
-    if (index_check(p, sk, idx)) return -2;
-    unsigned char sk_seed[64], sk_prf[64], pub_seed[64], pub_root[64];
-    memcpy(sk_seed, sk + p->index_bytes, n);
-    memcpy(sk_prf, sk + p->index_bytes + n, n);
-    memcpy(pub_seed, sk + p->index_bytes + 3 * n, n);
-    memcpy(pub_root, sk + p->index_bytes + 2 * n, n);
-    for (unsigned i = 0; i < p->index_bytes; ++i) sk[i] = ((idx + 1) >> 8 * (p->index_bytes - 1 - i)) & 255;
-    KeySeeds ks{sk_seed, sk_prf, pub_seed};
-    e->set_keys(fam, 1, &ks);
-    memmove(sm + p->sig_bytes, m, mlen);
-    uint64_t idx_tree = idx >> th;
-    uint32_t idx_leaf = (uint32_t)(idx & ((1u << th) - 1));
-    unsigned char R[64];
-    sign_front(p, e, fam, idx, pub_root, sm, sm + p->sig_bytes, mlen, R, sm + p->index_bytes + n, 0, idx_tree,
-               idx_leaf);
-    for (unsigned i = 0; i < p->index_bytes; ++i) sm[i] = (idx >> 8 * (p->index_bytes - 1 - i)) & 255;
-    memcpy(sm + p->index_bytes, R, n);
-    unsigned char* s = sm + p->index_bytes + n + p->wots_sig_bytes;
-    memcpy(s, st[0].auth, th * n);
-    s += th * n;
-    for (unsigned i = 1; i < d; ++i) {
-        memcpy(s, wots_sigs + (i - 1) * p->wots_sig_bytes, p->wots_sig_bytes);
-        s += p->wots_sig_bytes;
-        memcpy(s, st[i].auth, th * n);
-        s += th * n;
-    }
+    DirectOps ops(e, fam, p->n, bds_plan_leaves(p, sk, mlen));
+    return bds_sign(p, sk, sm, smlen, m, mlen, ops);
+}
 
-    Hasher H{e, fam, n, {}, {}};
-    uint32_t addr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    uint32_t ots_addr[8] = {0, 0, 0, ADDR_OTS, 0, 0, 0, 0};
-    int needswap_upto = -1;
-    unsigned updates = (th - p->bds_k) >> 1;
-    addr[1] = (uint32_t)((idx_tree + 1) >> 32);
-    addr[2] = (uint32_t)(idx_tree + 1);
-    if ((1 + idx_tree) * (1ULL << th) + idx_leaf < (1ULL << p->full_height)) {
-        bds_state_update(p, H, &st[d], addr);
-    }
-    for (unsigned i = 0; i < d; ++i) {
-        if (!(((idx + 1) & ((1ULL << ((i + 1) * th)) - 1)) == 0)) {
-            idx_leaf = (uint32_t)((idx >> (th * i)) & ((1u << th) - 1));
-            idx_tree = idx >> (th * (i + 1));
-            addr[0] = i;
-            addr[1] = (uint32_t)(idx_tree >> 32);
-            addr[2] = (uint32_t)idx_tree;
-            if ((int)i == needswap_upto + 1) bds_round(p, H, &st[i], idx_leaf, addr);
-            updates = bds_treehash_update(p, H, &st[i], updates, addr);
-            addr[1] = (uint32_t)((idx_tree + 1) >> 32);
-            addr[2] = (uint32_t)(idx_tree + 1);
-            if ((1 + idx_tree) * (1ULL << th) + idx_leaf < (1ULL << (p->full_height - th * i))) {
-                if (i > 0 && updates > 0 && st[d + i].next_leaf < (1ULL << p->full_height)) {
-                    bds_state_update(p, H, &st[d + i], addr);
-                    --updates;
-                }
-            }
-        } else if (idx < (1ULL << p->full_height) - 1) {
-            swap_states(p, &st[d + i], &st[i]);
-            const uint64_t t = (idx + 1) >> ((i + 2) * th);
-            const uint32_t ots = (uint32_t)(((idx >> ((i + 1) * th)) + 1) & ((1u << th) - 1));
-            ots_addr[0] = i + 1;
-            ots_addr[1] = (uint32_t)(t >> 32);
-            ots_addr[2] = (uint32_t)t;
-            ots_addr[4] = ots;
-            WotsReq w{0, i + 1, t, ots, WOTS_SIGN};
-            e->wots(fam, 1, &w, st[i].stack, nullptr, wots_sigs + i * p->wots_sig_bytes);
-            st[d + i].stackoffset = 0;
-            st[d + i].next_leaf = 0;
-            --updates;  // WOTS signing counts as one update (:1107)
-            needswap_upto = (int)i;
-            for (unsigned j = 0; j < nth; ++j) st[i].th[j].completed = 1;
-        }
-    }
-    *smlen = p->sig_bytes + mlen;
-    store_states(p, sk, st.data());
-    return 0;
+// xmss_core_fast.c:710-851
+int xmss_core_sign(const xmss_params* params, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
+                   const unsigned char* m, unsigned long long mlen) {
+    return bds_sign_api(params, sk, sm, smlen, m, mlen);
+}
+
+// xmss_core_fast.c:917-1121
+int xmssmt_core_sign(const xmss_params* params, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
+                     const unsigned char* m, unsigned long long mlen) {
+    return bds_sign_api(params, sk, sm, smlen, m, mlen);
 }
 
 }  // extern "C"

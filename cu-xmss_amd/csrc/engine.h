@@ -102,6 +102,14 @@ public:
 
     void set_keys(int fam, int nkeys, const KeySeeds* seeds);
 
+    // Device-resident key contexts that outlive a call (the serving pool):
+    // pool_add prepares one key and returns its slot; after pool_use(fam) the
+    // request `key` fields of the calls below index the pool of that family
+    // (until the next set_keys).
+    int pool_add(int fam, const KeySeeds& seeds);
+    void pool_use(int fam);
+    size_t pool_key_bytes(int fam) const;
+
     // Full subtrees of height th, one per run (leaf0 must be 0).  All node
     // layers stay on the device for fetch_nodes until the next call.
     void build_trees(int fam, unsigned th, int nruns, const LeafRun* runs);

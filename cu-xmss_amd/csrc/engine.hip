@@ -66,8 +66,17 @@ struct DevBuf {
     }
 };
 
+struct KeyPool {
+    void* p = nullptr;
+    size_t cap = 0;  // keys
+    int n = 0;
+};
+
 struct Engine::Impl {
     Launcher L;
+    KeyPool pool[F_COUNT];
+    const void* kp_override = nullptr;  // pool in use, or null for `keys`
+    const void* kp() const { return kp_override ? kp_override : keys.p; }
     Engine::Batch* spare[2] = {nullptr, nullptr};  // reusable verify / sign batches
     DevBuf keys, seeds, desc, ends, tmp, nodes, a, b, c, d, idx;
     int key_fam = -1;
@@ -141,7 +150,45 @@ void Engine::set_keys(int fam, int nkeys, const KeySeeds* seeds) {
     o->prepare(I->L, ds, nkeys, dk);
     I->key_fam = fam;
     I->nkeys = nkeys;
+    I->kp_override = nullptr;
 }
+
+int Engine::pool_add(int fam, const KeySeeds& s) {
+    Impl* I = impl_;
+    FamOps* o = ops_of(fam);
+    KeyPool& P = I->pool[fam];
+    const size_t kb = o->key_bytes();
+    if ((size_t)P.n == P.cap) {  // grow, keeping the prepared contexts
+        const size_t cap = std::max<size_t>(64, P.cap * 2);
+        void* np = nullptr;
+        HIPCK(hipMalloc(&np, cap * kb));
+        if (P.p) {
+            HIPCK(hipMemcpyAsync(np, P.p, (size_t)P.n * kb, hipMemcpyDeviceToDevice, I->L.s));
+            HIPCK(hipStreamSynchronize(I->L.s));
+            HIPCK(hipFree(P.p));
+        }
+        P.p = np;
+        P.cap = cap;
+        if (I->key_fam == fam && I->kp_override) I->kp_override = P.p;
+    }
+    const unsigned n = fam_n(fam);
+    std::vector<uint8_t> h(3 * n, 0);
+    if (s.sk_seed) memcpy(&h[0], s.sk_seed, n);
+    if (s.sk_prf) memcpy(&h[n], s.sk_prf, n);
+    if (s.pub_seed) memcpy(&h[2 * n], s.pub_seed, n);
+    uint8_t* ds = I->seeds.need(h.size());
+    I->h2d(ds, h.data(), h.size());
+    o->prepare(I->L, ds, 1, (uint8_t*)P.p + (size_t)P.n * kb);
+    I->sync();
+    return P.n++;
+}
+
+void Engine::pool_use(int fam) {
+    impl_->key_fam = fam;
+    impl_->kp_override = impl_->pool[fam].p;
+}
+
+size_t Engine::pool_key_bytes(int fam) const { return ops_of(fam)->key_bytes(); }
 
 static std::vector<DRun> pack_runs(int n, const LeafRun* runs) {
     std::vector<DRun> v(n);
@@ -166,7 +213,7 @@ void Engine::build_trees(int fam, unsigned th, int nruns, const LeafRun* runs) {
     DRun* drun = I->desc.need<DRun>(sizeof(DRun) * d.size());
     I->h2d(drun, d.data(), sizeof(DRun) * d.size());
     uint8_t* nodes = I->nodes.need(stride * n * (size_t)nruns);
-    const void* dk = I->keys.p;
+    const void* dk = I->kp();
     for (int r0 = 0; r0 < nruns;) {
         int rg = std::min<int>(nruns - r0, 65535);
         uint64_t lpp = leaves;
@@ -228,7 +275,7 @@ void Engine::leaves(int fam, int nruns, const LeafRun* runs, unsigned count, uin
     DLeaf* dd = I->desc.need<DLeaf>(sizeof(DLeaf) * nl);
     uint8_t* dout = I->b.need(nl * n);
     I->h2d(dd, d.data(), sizeof(DLeaf) * nl);
-    o->leaf_wg(I->L, I->keys.p, dd, (uint32_t)nl, false, nullptr, nullptr, nullptr, 0, dout);
+    o->leaf_wg(I->L, I->kp(), dd, (uint32_t)nl, false, nullptr, nullptr, nullptr, 0, dout);
     I->d2h(out, dout, nl * n);
     I->sync();
 }
@@ -252,7 +299,7 @@ void Engine::verify_layer(int fam, unsigned th, int nreq, const RootReq* reqs, c
     I->h2d(dm, msgs, (size_t)nreq * n);
     I->h2d(ds, sigs, (size_t)nreq * len * n);
     I->h2d(da, auth, (size_t)nreq * th * n);
-    o->leaf_wg(I->L, I->keys.p, dd, (uint32_t)nreq, true, dm, ds, da, th, dout);
+    o->leaf_wg(I->L, I->kp(), dd, (uint32_t)nreq, true, dm, ds, da, th, dout);
     I->d2h(roots, dout, (size_t)nreq * n);
     I->sync();
 }
@@ -274,7 +321,7 @@ void Engine::wots(int fam, int nreq, const WotsReq* reqs, const uint8_t* msgs, c
     I->h2d(dd, d.data(), sizeof(DWots) * d.size());
     I->h2d(dm, msgs, (size_t)nreq * n);
     if (in) I->h2d(din, in, (size_t)nreq * len * n);
-    o->wots(I->L, I->keys.p, dd, (uint32_t)nreq, dm, din, dout);
+    o->wots(I->L, I->kp(), dd, (uint32_t)nreq, dm, din, dout);
     I->d2h(out, dout, (size_t)nreq * len * n);
     I->sync();
 }
@@ -294,7 +341,7 @@ void Engine::thash_h(int fam, int nreq, const ThashReq* reqs, const uint8_t* in,
     uint8_t* dout = I->b.need((size_t)nreq * n);
     I->h2d(dd, d.data(), sizeof(DThash) * d.size());
     I->h2d(di, in, (size_t)nreq * 2 * n);
-    o->thash_h(I->L, I->keys.p, dd, (uint32_t)nreq, di, dout);
+    o->thash_h(I->L, I->kp(), dd, (uint32_t)nreq, di, dout);
     I->d2h(out, dout, (size_t)nreq * n);
     I->sync();
 }
@@ -331,7 +378,7 @@ void Engine::hmsg(int fam, int nreq, const MsgReq* reqs, uint8_t* R_out, uint8_t
     uint8_t* dD = I->b.need((size_t)nreq * n);
     I->h2d(dd, d.data(), sizeof(DMsg) * d.size());
     I->h2d(db, blob.data(), total);
-    o->hmsg(I->L, I->keys.p, dd, (uint32_t)nreq, pad, db, dR, dD);
+    o->hmsg(I->L, I->kp(), dd, (uint32_t)nreq, pad, db, dR, dD);
     if (R_out) I->d2h(R_out, dR, (size_t)nreq * n);
     I->d2h(dig_out, dD, (size_t)nreq * n);
     I->sync();

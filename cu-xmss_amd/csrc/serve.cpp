@@ -1,0 +1,572 @@
+// serve.cpp -- a serving loop with continuous batching (include/xmss_mi355x.h,
+// xmss_mi355x_server_*).
+//
+// Requests (BDS signatures, verifications) arrive at any time.  Each runs on
+// a worker thread as ordinary BDS code (bds_sign, core_bds.cpp) over a
+// backend whose hashing calls are handed to one dispatcher: the dispatcher
+// waits until every running request is blocked on the device (or a short
+// deadline passes), then executes all waiting calls as one launch per kind
+// and hash family -- the leaves, merges, WOTS signatures, H_msg and
+// verifications of many keys share launches, and a request that arrives while
+// others are on the GPU joins the next launch.  Signatures of one key run in
+// arrival order (its sk is a sequential state); other keys are independent.
+//
+// Key contexts (the PRF midstates of SK_SEED, SK_PRF, PUB_SEED) live in an
+// HBM pool for the lifetime of the key.  Admission: every request reserves its
+// device working set (message, signature, planned leaves, and the key context
+// the first time its key is placed) against a fixed HBM budget and waits in
+// FIFO order while the budget is exhausted.
+#include <string.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/xmss_core.h"
+#include "../../include/xmss_mi355x.h"
+#include "bds.h"
+#include "core_common.h"
+
+using namespace xm;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// ---------------------------------------------------------------- dispatcher
+struct Call {
+    enum Kind { LEAVES, THASH, WOTS, HMSG, VERIFY } kind;
+    int fam = 0;
+    int slot = 0;
+    unsigned n = 0;
+    // LEAVES
+    std::vector<LeafId> ids;
+    // THASH
+    uint32_t adrs[8];
+    const unsigned char* in = nullptr;
+    // WOTS
+    uint32_t layer = 0;
+    uint64_t tree = 0;
+    uint32_t ots = 0;
+    const unsigned char* msg = nullptr;
+    // HMSG
+    uint64_t idx = 0;
+    const unsigned char* root = nullptr;
+    const unsigned char* m = nullptr;
+    uint64_t mlen = 0;
+    unsigned char* out2 = nullptr;
+    // VERIFY
+    const xmss_params* p = nullptr;
+    const unsigned char* pk = nullptr;  // OID-less
+    const unsigned char* sm = nullptr;
+    uint64_t smlen = 0;
+    int ok = 0;
+    // result
+    unsigned char* out = nullptr;
+    bool done = false;
+};
+
+class Dispatcher {
+public:
+    explicit Dispatcher(Engine* e) : e_(e), th_([this] { loop(); }) {}
+    ~Dispatcher() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        th_.join();
+    }
+    void enter() {
+        std::lock_guard<std::mutex> lk(mu_);
+        ++active_;
+    }
+    void leave() {
+        std::lock_guard<std::mutex> lk(mu_);
+        --active_;
+        cv_.notify_all();
+    }
+    // Blocks until the dispatcher has executed the call.
+    void run(Call* c) {
+        std::unique_lock<std::mutex> lk(mu_);
+        pending_.push_back(c);
+        if (pending_.size() == 1) first_ = Clock::now();
+        cv_.notify_all();
+        cv_done_.wait(lk, [c] { return c->done; });
+    }
+    uint64_t launches() const { return launches_; }
+    uint64_t calls() const { return calls_; }
+
+private:
+    void loop() {
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            // every running request waiting on the device, or 200 us since the oldest call
+            cv_.wait_for(lk, std::chrono::microseconds(200), [this] {
+                return stop_ || (!pending_.empty() && (int)pending_.size() >= active_);
+            });
+            if (stop_ && pending_.empty()) return;
+            if (pending_.empty()) continue;
+            if ((int)pending_.size() < active_ && Clock::now() - first_ < std::chrono::microseconds(200)) continue;
+            std::vector<Call*> batch;
+            batch.swap(pending_);
+            lk.unlock();
+            execute(batch);
+            lk.lock();
+            for (Call* c : batch) c->done = true;
+            calls_ += batch.size();
+            cv_done_.notify_all();
+        }
+    }
+
+    void execute(std::vector<Call*>& batch) {
+        std::lock_guard<std::mutex> elk(engine_mutex());
+        std::map<std::pair<int, int>, std::vector<Call*>> groups;  // (fam, kind)
+        for (Call* c : batch) groups[{c->fam, (int)c->kind}].push_back(c);
+        for (auto& g : groups) {
+            const int fam = g.first.first;
+            std::vector<Call*>& cs = g.second;
+            const unsigned n = fam_n(fam);
+            switch ((Call::Kind)g.first.second) {
+                case Call::LEAVES: {
+                    e_->pool_use(fam);
+                    std::vector<LeafRun> runs;
+                    for (Call* c : cs)
+                        for (const LeafId& id : c->ids) runs.push_back({(uint32_t)c->slot, id.layer, id.tree, id.leaf});
+                    std::vector<unsigned char> out(runs.size() * n);
+                    if (!runs.empty()) e_->leaves(fam, (int)runs.size(), runs.data(), 1, out.data());
+                    size_t k = 0;
+                    for (Call* c : cs) {
+                        memcpy(c->out, &out[k * n], c->ids.size() * n);
+                        k += c->ids.size();
+                    }
+                    break;
+                }
+                case Call::THASH: {
+                    e_->pool_use(fam);
+                    std::vector<ThashReq> q(cs.size());
+                    std::vector<unsigned char> in(cs.size() * 2 * n), out(cs.size() * n);
+                    for (size_t i = 0; i < cs.size(); ++i) {
+                        q[i].key = (uint32_t)cs[i]->slot;
+                        memcpy(q[i].adrs, cs[i]->adrs, 32);
+                        memcpy(&in[i * 2 * n], cs[i]->in, 2 * n);
+                    }
+                    e_->thash_h(fam, (int)cs.size(), q.data(), in.data(), out.data());
+                    for (size_t i = 0; i < cs.size(); ++i) memcpy(cs[i]->out, &out[i * n], n);
+                    break;
+                }
+                case Call::WOTS: {
+                    e_->pool_use(fam);
+                    const size_t sb = (size_t)fam_len(fam) * n;
+                    std::vector<WotsReq> q(cs.size());
+                    std::vector<unsigned char> msgs(cs.size() * n), out(cs.size() * sb);
+                    for (size_t i = 0; i < cs.size(); ++i) {
+                        q[i] = {(uint32_t)cs[i]->slot, cs[i]->layer, cs[i]->tree, cs[i]->ots, WOTS_SIGN};
+                        memcpy(&msgs[i * n], cs[i]->msg, n);
+                    }
+                    e_->wots(fam, (int)cs.size(), q.data(), msgs.data(), nullptr, out.data());
+                    for (size_t i = 0; i < cs.size(); ++i) memcpy(cs[i]->out, &out[i * sb], sb);
+                    break;
+                }
+                case Call::HMSG: {
+                    e_->pool_use(fam);
+                    std::vector<MsgReq> q(cs.size());
+                    std::vector<unsigned char> R(cs.size() * n), D(cs.size() * n);
+                    for (size_t i = 0; i < cs.size(); ++i)
+                        q[i] = {(uint32_t)cs[i]->slot, 1, cs[i]->idx, nullptr, cs[i]->root, cs[i]->m, cs[i]->mlen};
+                    e_->hmsg(fam, (int)cs.size(), q.data(), R.data(), D.data());
+                    for (size_t i = 0; i < cs.size(); ++i) {
+                        memcpy(cs[i]->out, &R[i * n], n);
+                        memcpy(cs[i]->out2, &D[i * n], n);
+                    }
+                    break;
+                }
+                case Call::VERIFY: {
+                    // one verify batch per parameter set
+                    std::map<std::tuple<unsigned, unsigned, unsigned>, std::vector<Call*>> by;
+                    for (Call* c : cs) by[std::make_tuple(c->p->tree_height, c->p->d, c->p->full_height)].push_back(c);
+                    for (auto& v : by) {
+                        const xmss_params* p = v.second[0]->p;
+                        std::vector<Engine::VerifyItem> it;
+                        for (Call* c : v.second) {
+                            it.push_back({c->pk, bytes_to_ull(c->sm, p->index_bytes), c->sm + p->index_bytes,
+                                          c->sm + p->index_bytes + p->n, c->sm + p->sig_bytes,
+                                          c->smlen - p->sig_bytes});
+                        }
+                        Engine::Batch* b = e_->verify_prepare(fam, p->tree_height, p->d, (int)it.size(), it.data());
+                        e_->run(b);
+                        std::vector<int> ok(it.size());
+                        e_->collect_verify(b, ok.data());
+                        e_->free_batch(b);
+                        for (size_t i = 0; i < v.second.size(); ++i) v.second[i]->ok = ok[i];
+                    }
+                    break;
+                }
+            }
+            ++launches_;
+        }
+    }
+
+    Engine* e_;
+    std::mutex mu_;
+    std::condition_variable cv_, cv_done_;
+    std::vector<Call*> pending_;
+    int active_ = 0;
+    bool stop_ = false;
+    Clock::time_point first_;
+    uint64_t launches_ = 0, calls_ = 0;
+    std::thread th_;
+};
+
+// BDS backend of one in-flight signature: hashing goes through the dispatcher.
+struct ServeOps final : BdsOps {
+    Dispatcher* d;
+    int fam;
+    unsigned n;
+    int slot;
+    std::function<int(const unsigned char*, const unsigned char*, const unsigned char*)> slot_for;
+    std::vector<LeafId> plan;
+    std::map<LeafId, std::vector<unsigned char>> cache;
+
+    void bind_key(const unsigned char* a, const unsigned char* b, const unsigned char* c) override {
+        slot = slot_for(a, b, c);
+        if (plan.empty()) return;
+        Call call;
+        call.kind = Call::LEAVES;
+        call.fam = fam;
+        call.slot = slot;
+        call.ids = plan;
+        std::vector<unsigned char> out(plan.size() * n);
+        call.out = out.data();
+        d->run(&call);
+        for (size_t i = 0; i < plan.size(); ++i)
+            cache[plan[i]] = std::vector<unsigned char>(out.begin() + i * n, out.begin() + (i + 1) * n);
+    }
+    void leaf(unsigned char* out, const LeafId& id) override {
+        auto it = cache.find(id);
+        if (it == cache.end()) {  // not planned (cannot happen: control flow is value-independent)
+            Call call;
+            call.kind = Call::LEAVES;
+            call.fam = fam;
+            call.slot = slot;
+            call.ids = {id};
+            call.out = out;
+            d->run(&call);
+            return;
+        }
+        memcpy(out, it->second.data(), n);
+    }
+    void thash_h(unsigned char* out, const unsigned char* in, const uint32_t adrs[8]) override {
+        Call call;
+        call.kind = Call::THASH;
+        call.fam = fam;
+        call.slot = slot;
+        memcpy(call.adrs, adrs, 32);
+        call.in = in;
+        std::vector<unsigned char> tmp(n);
+        call.out = tmp.data();
+        d->run(&call);
+        memcpy(out, tmp.data(), n);
+    }
+    void wots_sign(unsigned char* sig, const unsigned char* msg, uint32_t layer, uint64_t tree,
+                   uint32_t ots) override {
+        Call call;
+        call.kind = Call::WOTS;
+        call.fam = fam;
+        call.slot = slot;
+        call.layer = layer;
+        call.tree = tree;
+        call.ots = ots;
+        std::vector<unsigned char> m(msg, msg + n);
+        call.msg = m.data();
+        call.out = sig;
+        d->run(&call);
+    }
+    void hmsg_sign(unsigned char* R, unsigned char* digest, uint64_t idx, const unsigned char* root,
+                   const unsigned char* m, uint64_t mlen) override {
+        Call call;
+        call.kind = Call::HMSG;
+        call.fam = fam;
+        call.slot = slot;
+        call.idx = idx;
+        call.root = root;
+        call.m = m;
+        call.mlen = mlen;
+        call.out = R;
+        call.out2 = digest;
+        d->run(&call);
+    }
+};
+
+struct Key {
+    int mt = 0;
+    xmss_params p{};
+    int fam = -1;
+    std::vector<unsigned char> sk, pk;  // OID-prefixed
+    int slot = -1;                      // pool slot of the key's own seeds
+    bool busy = false;
+};
+
+struct Request {
+    uint64_t id = 0;
+    bool sign = true;
+    int key = 0;
+    std::vector<unsigned char> data;  // sign: message; verify: sm
+    std::vector<unsigned char> out;   // sign: sm; verify: m
+    unsigned long long outlen = 0;
+    int ret = 0;
+    size_t reserve = 0;
+    Clock::time_point t0, t1;
+    bool done = false;
+};
+
+}  // namespace
+
+struct xmss_mi355x_server {
+    Engine* e = nullptr;
+    std::unique_ptr<Dispatcher> disp;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::vector<Key> keys;
+    std::deque<Request*> queue;  // arrival order
+    std::map<uint64_t, Request*> reqs;
+    uint64_t next_id = 1;
+    size_t budget = 0, reserved = 0, peak = 0;
+    uint64_t admission_waits = 0;
+    bool stop = false;
+    std::vector<std::thread> workers;
+
+    // The request to run next: the oldest one whose key is free, if the
+    // budget admits it (FIFO: a request that does not fit holds back the rest).
+    Request* pick() {
+        for (auto it = queue.begin(); it != queue.end(); ++it) {
+            Request* r = *it;
+            if (r->sign && keys[r->key].busy) continue;
+            if (reserved + r->reserve > budget && reserved > 0) {
+                ++admission_waits;
+                return nullptr;
+            }
+            queue.erase(it);
+            return r;
+        }
+        return nullptr;
+    }
+
+    void work() {
+        for (;;) {
+            Request* r = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stop || (r = pick()) != nullptr; });
+                if (!r) return;
+                reserved += r->reserve;
+                peak = std::max(peak, reserved);
+                if (r->sign) keys[r->key].busy = true;
+            }
+            disp->enter();
+            execute(r);
+            disp->leave();
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                reserved -= r->reserve;
+                if (r->sign) keys[r->key].busy = false;
+                r->t1 = Clock::now();
+                r->done = true;
+            }
+            cv_work.notify_all();
+            cv_done.notify_all();
+        }
+    }
+
+    int slot_for(Key& k, const unsigned char* a, const unsigned char* b, const unsigned char* c) {
+        const unsigned n = k.p.n, ib = k.p.index_bytes;
+        const unsigned char* s = k.pk.data();  // unused; seeds come from sk
+        (void)s;
+        std::lock_guard<std::mutex> elk(engine_mutex());
+        if (k.slot >= 0 && !memcmp(a, seeds_of(k).data(), n) && !memcmp(b, seeds_of(k).data() + n, n) &&
+            !memcmp(c, seeds_of(k).data() + 2 * n, n))
+            return k.slot;
+        (void)ib;
+        // a wiped key (last index) signs with zero seeds: a context of its own
+        KeySeeds ks{a, b, c};
+        return e->pool_add(k.fam, ks);
+    }
+    std::vector<unsigned char> seeds_of(const Key& k) {
+        const unsigned n = k.p.n, ib = k.p.index_bytes;
+        const unsigned char* sk = k.sk.data() + XMSS_OID_LEN;
+        std::vector<unsigned char> v(3 * n);
+        memcpy(&v[0], sk + ib, n);
+        memcpy(&v[n], sk + ib + n, n);
+        memcpy(&v[2 * n], sk + ib + 3 * n, n);
+        return v;
+    }
+
+    void execute(Request* r) {
+        Key& k = keys[r->key];
+        const xmss_params& p = k.p;
+        if (r->sign) {
+            // the key's sk is owned by this request while busy
+            unsigned char* sk = k.sk.data() + XMSS_OID_LEN;
+            r->out.assign(p.sig_bytes + r->data.size(), 0);
+            ServeOps ops;
+            ops.d = disp.get();
+            ops.fam = k.fam;
+            ops.n = p.n;
+            ops.slot = k.slot;
+            ops.slot_for = [&](const unsigned char* a, const unsigned char* b, const unsigned char* c) {
+                return slot_for(k, a, b, c);
+            };
+            ops.plan = bds_plan_leaves(&p, sk, r->data.size());
+            r->ret = bds_sign(&p, sk, r->out.data(), &r->outlen, r->data.data(), r->data.size(), ops);
+        } else {
+            const unsigned long long smlen = r->data.size();
+            r->out.assign(smlen + 1, 0);
+            if (smlen < p.sig_bytes) {
+                r->ret = -1;
+                r->outlen = 0;
+                return;
+            }
+            Call call;
+            call.kind = Call::VERIFY;
+            call.fam = k.fam;
+            call.p = &p;
+            call.pk = k.pk.data() + XMSS_OID_LEN;
+            call.sm = r->data.data();
+            call.smlen = smlen;
+            disp->run(&call);
+            const unsigned long long mlen = smlen - p.sig_bytes;
+            if (call.ok) {
+                memcpy(r->out.data(), r->data.data() + p.sig_bytes, mlen);
+                r->outlen = mlen;
+                r->ret = 0;
+            } else {
+                r->outlen = 0;
+                r->ret = -1;
+            }
+        }
+    }
+};
+
+extern "C" {
+
+xmss_mi355x_server* xmss_mi355x_server_create(size_t hbm_budget_bytes, int workers) {
+    Engine* e;
+    {
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        e = Engine::get();
+    }
+    if (!e) return nullptr;
+    xmss_mi355x_server* s = new xmss_mi355x_server;
+    s->e = e;
+    s->budget = hbm_budget_bytes;
+    s->disp.reset(new Dispatcher(e));
+    if (workers <= 0) workers = 256;
+    for (int i = 0; i < workers; ++i) s->workers.emplace_back([s] { s->work(); });
+    return s;
+}
+
+int xmss_mi355x_server_add_key(xmss_mi355x_server* s, int mt, const unsigned char* sk, const unsigned char* pk) {
+    Key k;
+    k.mt = mt;
+    const uint32_t oid = ((uint32_t)pk[0] << 24) | ((uint32_t)pk[1] << 16) | ((uint32_t)pk[2] << 8) | pk[3];
+    if (mt ? xmssmt_parse_oid(&k.p, oid) : xmss_parse_oid(&k.p, oid)) return -1;
+    k.fam = fam_of(k.p.func, k.p.n);
+    if (k.fam < 0) return -1;
+    k.sk.assign(sk, sk + XMSS_OID_LEN + k.p.sk_bytes);
+    k.pk.assign(pk, pk + XMSS_OID_LEN + k.p.pk_bytes);
+    std::vector<unsigned char> seeds = s->seeds_of(k);
+    const unsigned n = k.p.n;
+    KeySeeds ks{&seeds[0], &seeds[n], &seeds[2 * n]};
+    {
+        std::lock_guard<std::mutex> elk(engine_mutex());
+        k.slot = s->e->pool_add(k.fam, ks);
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->keys.push_back(std::move(k));
+    s->reserved += s->e->pool_key_bytes(s->keys.back().fam);  // resident key context
+    s->peak = std::max(s->peak, s->reserved);
+    return (int)s->keys.size() - 1;
+}
+
+static uint64_t submit(xmss_mi355x_server* s, bool sign, int key, const unsigned char* d, unsigned long long len) {
+    Request* r = new Request;
+    r->sign = sign;
+    r->key = key;
+    r->data.assign(d, d + len);
+    r->t0 = Clock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (key < 0 || key >= (int)s->keys.size()) {
+        delete r;
+        return 0;
+    }
+    const xmss_params& p = s->keys[key].p;
+    // device working set: message + signature staging, planned leaves, H_msg prefix
+    r->reserve = (size_t)p.sig_bytes + len + (size_t)(p.tree_height + 1) * p.n * 2 + 4 * p.n;
+    r->id = s->next_id++;
+    s->reqs[r->id] = r;
+    s->queue.push_back(r);
+    s->cv_work.notify_one();
+    return r->id;
+}
+
+uint64_t xmss_mi355x_server_sign(xmss_mi355x_server* s, int key, const unsigned char* m, unsigned long long mlen) {
+    return submit(s, true, key, m, mlen);
+}
+
+uint64_t xmss_mi355x_server_verify(xmss_mi355x_server* s, int key, const unsigned char* sm,
+                                   unsigned long long smlen) {
+    return submit(s, false, key, sm, smlen);
+}
+
+int xmss_mi355x_server_result(xmss_mi355x_server* s, uint64_t id, unsigned char* out, unsigned long long* outlen,
+                              double* latency_ms) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return -4;
+    Request* r = it->second;
+    s->cv_done.wait(lk, [r] { return r->done; });
+    if (out) memcpy(out, r->out.data(), r->outlen);
+    if (outlen) *outlen = r->outlen;
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t1 - r->t0).count();
+    const int ret = r->ret;
+    s->reqs.erase(it);
+    delete r;
+    return ret;
+}
+
+int xmss_mi355x_server_get_sk(xmss_mi355x_server* s, int key, unsigned char* sk) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (key < 0 || key >= (int)s->keys.size()) return -1;
+    s->cv_done.wait(lk, [&] { return !s->keys[key].busy; });
+    memcpy(sk, s->keys[key].sk.data(), s->keys[key].sk.size());
+    return 0;
+}
+
+void xmss_mi355x_server_stats(xmss_mi355x_server* s, uint64_t* launches, uint64_t* calls, size_t* peak_bytes,
+                              uint64_t* admission_waits) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (launches) *launches = s->disp->launches();
+    if (calls) *calls = s->disp->calls();
+    if (peak_bytes) *peak_bytes = s->peak;
+    if (admission_waits) *admission_waits = s->admission_waits;
+}
+
+void xmss_mi355x_server_destroy(xmss_mi355x_server* s) {
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->workers) t.join();
+    s->disp.reset();
+    for (auto& kv : s->reqs) delete kv.second;
+    delete s;
+}
+
+}  // extern "C"

@@ -72,6 +72,35 @@ int xmss_batch_sign_open(int mt, unsigned char *const *ms, unsigned long long *m
                          const unsigned long long *smlens,
                          const unsigned char *const *pks, int *rets, size_t count);
 
+/* ---- serving loop (continuous batching of BDS-core requests) -------------
+ * Keys are added with their BDS-format secret key (as xmss[mt]_keypair of
+ * libxmss_mi355x_fast writes it) and public key; the server owns the key from
+ * then on.  Signatures of one key run in submission order and update its sk
+ * exactly as xmss[mt]_sign would; requests of different keys share GPU
+ * launches.  Every request reserves its device working set against
+ * hbm_budget_bytes (key contexts stay resident) and waits while the budget is
+ * used up.  Results come back with their latency (submission to completion). */
+typedef struct xmss_mi355x_server xmss_mi355x_server;
+
+xmss_mi355x_server *xmss_mi355x_server_create(size_t hbm_budget_bytes, int workers);
+/* Returns the key id, or -1 for an unknown OID. */
+int xmss_mi355x_server_add_key(xmss_mi355x_server *s, int mt, const unsigned char *sk,
+                               const unsigned char *pk);
+/* Return request ids (0 for an unknown key). */
+uint64_t xmss_mi355x_server_sign(xmss_mi355x_server *s, int key, const unsigned char *m,
+                                 unsigned long long mlen);
+uint64_t xmss_mi355x_server_verify(xmss_mi355x_server *s, int key, const unsigned char *sm,
+                                   unsigned long long smlen);
+/* Waits for a request; out receives sm (sign) or m (verify).  Returns what
+ * xmss[mt]_sign / _sign_open would have returned. */
+int xmss_mi355x_server_result(xmss_mi355x_server *s, uint64_t id, unsigned char *out,
+                              unsigned long long *outlen, double *latency_ms);
+/* Current secret key of a key id (waits for its running signature). */
+int xmss_mi355x_server_get_sk(xmss_mi355x_server *s, int key, unsigned char *sk);
+void xmss_mi355x_server_stats(xmss_mi355x_server *s, uint64_t *launches, uint64_t *calls,
+                              size_t *peak_bytes, uint64_t *admission_waits);
+void xmss_mi355x_server_destroy(xmss_mi355x_server *s);
+
 #ifdef __cplusplus
 }
 #endif

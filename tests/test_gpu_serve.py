@@ -1,0 +1,110 @@
+"""Serving loop (continuous batching of BDS requests across keys): every
+signature and every secret-key update must be byte-identical to the same
+request run alone through xmss[mt]_sign, whatever else shares its launches;
+verifications accept exactly the valid signatures."""
+import ctypes as C
+import random
+
+import pytest
+
+from common import product
+from ffi import buf, raw
+
+pytestmark = pytest.mark.gpu
+
+KEYS = [("XMSS-SHA2_10_256", 0, 0x01), ("XMSS-SHA2_10_256", 0, 0x01),
+        ("XMSSMT-SHA2_20/4_256", 1, 0x02), ("XMSSMT-SHA2_20/4_256", 1, 0x02),
+        ("XMSSMT-SHA2_40/8_256", 1, 0x05), ("XMSSMT-SHA2_20/2_192", 1, 0x21)]
+
+
+def _server_api(L):
+    L.xmss_mi355x_server_create.restype = C.c_void_p
+    L.xmss_mi355x_server_create.argtypes = [C.c_size_t, C.c_int]
+    L.xmss_mi355x_server_add_key.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    for nm in ("xmss_mi355x_server_sign", "xmss_mi355x_server_verify"):
+        getattr(L, nm).restype = C.c_uint64
+        getattr(L, nm).argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_ulonglong]
+    L.xmss_mi355x_server_result.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_ulonglong),
+                                            C.POINTER(C.c_double)]
+    L.xmss_mi355x_server_get_sk.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.xmss_mi355x_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_size_t), C.POINTER(C.c_uint64)]
+    L.xmss_mi355x_server_destroy.argtypes = [C.c_void_p]
+
+
+def _keys(lib):
+    keys = []
+    for i, (name, mt, oid) in enumerate(KEYS):
+        lib.lib.xmss_mi355x_rng_seed(1000 + i)
+        p = lib.params(oid, mt)
+        rc, pk, sk = lib.keypair(oid, mt, sk_init=bytes(4 + p.sk_bytes))
+        assert rc == 0
+        keys.append((mt, oid, p, pk, sk))
+    return keys
+
+
+def _alone(lib, keys, msgs):
+    """Each key's messages signed in order through the C API."""
+    out = []
+    for (mt, oid, p, pk, sk), ms in zip(keys, msgs):
+        sms = []
+        for m in ms:
+            rc, sm, sk = lib.sign(sk, m, mt)
+            assert rc == 0
+            sms.append(sm)
+        out.append((sms, sk))
+    return out
+
+
+@pytest.mark.parametrize("budget", [1 << 30, 40000])
+def test_serve_mix_equals_alone(budget):
+    lib = product(1)
+    L = lib.lib
+    _server_api(L)
+    keys = _keys(lib)
+    rng = random.Random(77)
+    nsig = 40  # crosses layer-0 subtree switches of the th = 5 keys (idx 31 -> 32)
+    msgs = [[bytes(rng.getrandbits(8) for _ in range(int(32 * 128 ** rng.random()))) for _ in range(nsig)]
+            for _ in keys]
+    want = _alone(lib, keys, msgs)
+
+    srv = L.xmss_mi355x_server_create(budget, 64)
+    assert srv
+    ids = [L.xmss_mi355x_server_add_key(srv, mt, buf(sk), buf(pk)) for (mt, oid, p, pk, sk) in keys]
+    assert ids == list(range(len(keys)))
+    # interleave: round-robin over keys, with verifications of signatures
+    # already produced mixed in
+    reqs = []
+    order = [(k, j) for j in range(nsig) for k in range(len(keys))]
+    for k, j in order:
+        reqs.append(("s", k, j, L.xmss_mi355x_server_sign(srv, k, msgs[k][j], len(msgs[k][j]))))
+        if j % 5 == 4:
+            sm = want[k][0][j - 3]
+            reqs.append(("v", k, j - 3, L.xmss_mi355x_server_verify(srv, k, sm, len(sm))))
+    bad = bytearray(want[2][0][7])
+    bad[-1] ^= 1
+    reqs.append(("bad", 2, 7, L.xmss_mi355x_server_verify(srv, 2, bytes(bad), len(bad))))
+    for kind, k, j, rid in reqs:
+        p = keys[k][2]
+        cap = p.sig_bytes + 5000
+        out = buf(cap)
+        olen = C.c_ulonglong(0)
+        lat = C.c_double(0)
+        rc = L.xmss_mi355x_server_result(srv, rid, out, C.byref(olen), C.byref(lat))
+        if kind == "s":
+            assert rc == 0
+            assert raw(out, olen.value) == want[k][0][j], "key %d sign %d" % (k, j)
+        elif kind == "v":
+            assert rc == 0 and raw(out, olen.value) == msgs[k][j]
+        else:
+            assert rc == -1 and olen.value == 0
+    for k, (mt, oid, p, pk, sk) in enumerate(keys):
+        skb = buf(4 + p.sk_bytes)
+        assert L.xmss_mi355x_server_get_sk(srv, k, skb) == 0
+        assert raw(skb, 4 + p.sk_bytes) == want[k][1], "key %d final sk" % k
+    launches, calls, peak, waits = C.c_uint64(), C.c_uint64(), C.c_size_t(), C.c_uint64()
+    L.xmss_mi355x_server_stats(srv, C.byref(launches), C.byref(calls), C.byref(peak), C.byref(waits))
+    L.xmss_mi355x_server_destroy(srv)
+    assert calls.value > launches.value  # calls of different keys shared launches
+    if budget < 1 << 20:
+        assert waits.value > 0 and peak.value <= budget + 20000
