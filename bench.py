@@ -221,6 +221,105 @@ def bench_mixed(api, per_set):
     return res
 
 
+SERVE_SETS = [("XMSS-SHA2_16_256", 0, 0x02), ("XMSSMT-SHA2_20/4_256", 1, 0x02), ("XMSSMT-SHA2_40/4_256", 1, 0x04)]
+
+
+def bench_serve(rate, seconds, nkeys, budget):
+    """Serving: Poisson arrivals at `rate` requests/s for `seconds`, 70% sign /
+    30% verify, over nkeys BDS keys split evenly among XMSS-SHA2_16_256,
+    XMSSMT-SHA2_20/4_256 and XMSSMT-SHA2_40/4_256; messages log-uniform
+    32 B - 4 KB.  Latency is submission to completion, measured in the library."""
+    import queue
+    import threading
+    api = load_product(1)
+    L = api.lib
+    L.xmss_mi355x_server_create.restype = C.c_void_p
+    L.xmss_mi355x_server_create.argtypes = [C.c_size_t, C.c_int]
+    L.xmss_mi355x_server_add_key.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    for nm in ("xmss_mi355x_server_sign", "xmss_mi355x_server_verify"):
+        getattr(L, nm).restype = C.c_uint64
+        getattr(L, nm).argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_ulonglong]
+    L.xmss_mi355x_server_result.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_ulonglong),
+                                            C.POINTER(C.c_double)]
+    L.xmss_mi355x_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_size_t), C.POINTER(C.c_uint64)]
+    L.xmss_mi355x_server_destroy.argtypes = [C.c_void_p]
+    rng = random.Random(SEED + 13)
+    t0 = time.perf_counter()
+    keys = []
+    made = {}
+    for i in range(nkeys):
+        name, mt, oid = SERVE_SETS[i % 3]
+        p = api.params(oid, mt)
+        if mt == 0 and name in made:  # the XMSS BDS keypair uses fixed seeds: every such key is the same
+            pk, sk = made[name]
+        else:
+            L.xmss_mi355x_rng_seed(SEED + 100 + i)
+            rc, pk, sk = api.keypair(oid, mt, sk_init=bytes(4 + p.sk_bytes))
+            assert rc == 0
+            made[name] = (pk, sk)
+        keys.append((mt, p, pk, sk))
+    keygen_s = time.perf_counter() - t0
+    srv = L.xmss_mi355x_server_create(budget, 256)
+    for mt, p, pk, sk in keys:
+        L.xmss_mi355x_server_add_key(srv, mt, buf(sk), buf(pk))
+    pending = queue.Queue()
+    done_sigs = []  # (key, sm) available for verification
+    lat = {"s": [], "v": []}
+    fails = [0]
+
+    def collect():
+        while True:
+            item = pending.get()
+            if item is None:
+                return
+            kind, k, rid = item
+            p = keys[k][1]
+            out = buf(p.sig_bytes + 4200)
+            olen, ms = C.c_ulonglong(0), C.c_double(0)
+            rc = L.xmss_mi355x_server_result(srv, rid, out, C.byref(olen), C.byref(ms))
+            if rc != 0:
+                fails[0] += 1
+            lat[kind].append(ms.value)
+            if kind == "s" and rc == 0:
+                done_sigs.append((k, raw(out, olen.value)))
+
+    th = threading.Thread(target=collect)
+    th.start()
+    n = 0
+    start = time.perf_counter()
+    t_next = start
+    while True:
+        t_next += rng.expovariate(rate)
+        if t_next - start > seconds:
+            break
+        now = time.perf_counter()
+        if t_next > now:
+            time.sleep(t_next - now)
+        if rng.random() < 0.7 or not done_sigs:
+            k = rng.randrange(nkeys)
+            m = bytes(rng.getrandbits(8) for _ in range(int(32 * 128 ** rng.random())))
+            pending.put(("s", k, L.xmss_mi355x_server_sign(srv, k, m, len(m))))
+        else:
+            k, sm = done_sigs[rng.randrange(len(done_sigs))]
+            pending.put(("v", k, L.xmss_mi355x_server_verify(srv, k, sm, len(sm))))
+        n += 1
+    pending.put(None)
+    th.join()
+    elapsed = time.perf_counter() - start
+    launches, calls, peak, waits = C.c_uint64(), C.c_uint64(), C.c_size_t(), C.c_uint64()
+    L.xmss_mi355x_server_stats(srv, C.byref(launches), C.byref(calls), C.byref(peak), C.byref(waits))
+    L.xmss_mi355x_server_destroy(srv)
+    q = lambda v, f: sorted(v)[min(len(v) - 1, int(math.ceil(f * len(v))) - 1)] if v else None
+    return {"offered_per_s": rate, "seconds": seconds, "keys": nkeys, "requests": n,
+            "sustained_ops_per_s": round(n / elapsed, 1), "failures": fails[0],
+            "sign_p50_ms": round(q(lat["s"], 0.5), 3), "sign_p99_ms": round(q(lat["s"], 0.99), 3),
+            "verify_p50_ms": round(q(lat["v"], 0.5), 3) if lat["v"] else None,
+            "verify_p99_ms": round(q(lat["v"], 0.99), 3) if lat["v"] else None,
+            "hbm_budget_bytes": budget, "peak_reserved_bytes": peak.value, "admission_waits": waits.value,
+            "device_calls": calls.value, "launch_groups": launches.value, "key_setup_s": round(keygen_s, 2)}
+
+
 def bench_latency(nsigns):
     """Sequential BDS signatures of one XMSS-SHA2_16_256 key (fast core)."""
     api = load_product(1)
@@ -311,6 +410,9 @@ def main():
     ap.add_argument("--latency-signs", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", type=int, default=1024, help="keys per parameter set of the mixed batch (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=4000.0, help="offered requests/s of the serving run (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-budget", type=int, default=4 << 20, help="HBM budget of the serving run, bytes")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="MiB of one device-to-device copy, run before the keygens (HBM counter calibration)")
     args = ap.parse_args()
@@ -351,6 +453,10 @@ def main():
     if args.mixed:
         log("mixed batch x%d per set ..." % args.mixed)
         mixed = bench_mixed(api, args.mixed)
+    serve = None
+    if args.serve_rate:
+        log("serving at %.0f req/s for %.0f s ..." % (args.serve_rate, args.serve_seconds))
+        serve = bench_serve(args.serve_rate, args.serve_seconds, 256, args.serve_budget)
     log("latency (BDS) x%d ..." % args.latency_signs)
     p50, p99, v50 = bench_latency(args.latency_signs) if args.latency_signs else (None, None, None)
     cpu = None if args.no_cpu else cpu_baseline()
@@ -385,6 +491,7 @@ def main():
         "kernels_batch_sign_ms": {k: round(v[1], 3) for k, v in bst.items()},
         "batch_last_index_rejected": n_last,
         "mixed_batch": mixed,
+        "serving": serve,
     }
     print(json.dumps(out), flush=True)
 

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/params.h"
+#include "engine.h"
 
 namespace xm {
 
@@ -59,7 +60,25 @@ struct RecordingOps final : BdsOps {
 int bds_sign(const xmss_params* p, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
              const unsigned char* m, unsigned long long mlen, BdsOps& ops);
 
-// The leaves bds_sign would consume for this sk (a dry run on a copy).
-std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* sk, unsigned long long mlen);
+// The leaves bds_sign would consume for this sk (a dry run on a copy):
+// distinct and sorted, or every request in consumption order.
+std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* sk, unsigned long long mlen,
+                                    bool in_order = false);
+
+// A signature whose traversal runs on the device (Engine::bds_sign_batch).
+struct BdsDeviceSign {
+    std::vector<uint32_t> layer, leaf;
+    std::vector<uint64_t> tree;
+    std::vector<unsigned char> seeds;  // SK_SEED, SK_PRF, PUB_SEED, root
+    unsigned long long idx = 0;
+    Engine::BdsJob job{};
+};
+// Host part before the launch: index, plan, index advance, message copy.
+// Returns false for the last index / an exhausted key (use bds_sign).
+bool bds_device_prepare(const xmss_params* p, unsigned char* sk, unsigned char* sm, const unsigned char* m,
+                        unsigned long long mlen, BdsDeviceSign& ds);
+void bds_device_finish(const xmss_params* p, unsigned char* sm, unsigned long long* smlen, unsigned long long mlen,
+                       const BdsDeviceSign& ds);
+Engine::BdsShape bds_shape(const xmss_params* p);
 
 }  // namespace xm

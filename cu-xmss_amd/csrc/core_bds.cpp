@@ -464,16 +464,74 @@ int bds_sign(const xmss_params* p, unsigned char* sk, unsigned char* sm, unsigne
     return p->d == 1 ? sign_xmss(p, sk, sm, smlen, m, mlen, ops) : sign_xmssmt(p, sk, sm, smlen, m, mlen, ops);
 }
 
-std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* sk, unsigned long long mlen) {
+std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* sk, unsigned long long mlen,
+                                    bool in_order) {
     std::vector<unsigned char> skc(sk, sk + p->sk_bytes);
     std::vector<unsigned char> smc(p->sig_bytes + mlen + 1), mc(mlen + 1, 0);
     unsigned long long l = 0;
     RecordingOps rec(p->n);
     bds_sign(p, skc.data(), smc.data(), &l, mc.data(), mlen, rec);
     std::vector<LeafId> v = rec.leaves;
-    std::sort(v.begin(), v.end());
-    v.erase(std::unique(v.begin(), v.end()), v.end());
+    if (!in_order) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
     return v;
+}
+
+bool bds_device_prepare(const xmss_params* p, unsigned char* sk, unsigned char* sm, const unsigned char* m,
+                        unsigned long long mlen, BdsDeviceSign& ds) {
+    const unsigned n = p->n, ib = p->index_bytes;
+    unsigned long long idx = 0;
+    for (unsigned i = 0; i < ib; ++i) idx |= ((unsigned long long)sk[i]) << 8 * (ib - 1 - i);
+    if (idx >= (1ULL << p->full_height) - 1) return false;  // wipe / exhausted: host path
+    const std::vector<LeafId> plan = bds_plan_leaves(p, sk, mlen, true);
+    ds.layer.resize(plan.size());
+    ds.tree.resize(plan.size());
+    ds.leaf.resize(plan.size());
+    for (size_t i = 0; i < plan.size(); ++i) {
+        ds.layer[i] = plan[i].layer;
+        ds.tree[i] = plan[i].tree;
+        ds.leaf[i] = plan[i].leaf;
+    }
+    ds.seeds.assign(4 * n, 0);
+    memcpy(&ds.seeds[0], sk + ib, n);           // SK_SEED
+    memcpy(&ds.seeds[n], sk + ib + n, n);       // SK_PRF
+    memcpy(&ds.seeds[2 * n], sk + ib + 3 * n, n);  // PUB_SEED
+    memcpy(&ds.seeds[3 * n], sk + ib + 2 * n, n);  // root
+    // the index advances before the signature exists (xmss_core_fast.c:769-772, :986-988)
+    ull_to_bytes(sk, ib, idx + 1);
+    memmove(sm + p->sig_bytes, m, mlen);
+    ds.idx = idx;
+    ds.job = {};
+    ds.job.sk_seed = &ds.seeds[0];
+    ds.job.sk_prf = &ds.seeds[n];
+    ds.job.pub_seed = &ds.seeds[2 * n];
+    ds.job.pub_root = &ds.seeds[3 * n];
+    ds.job.sk = sk;
+    ds.job.idx = idx;
+    ds.job.m = sm + p->sig_bytes;
+    ds.job.mlen = mlen;
+    ds.job.plan_layer = ds.layer.data();
+    ds.job.plan_tree = ds.tree.data();
+    ds.job.plan_leaf = ds.leaf.data();
+    ds.job.nplan = (uint32_t)plan.size();
+    ds.job.R_out = sm + ib;
+    ds.job.wots0_out = sm + ib + n;
+    ds.job.tail_out = sm + ib + n + p->wots_sig_bytes;
+    ds.job.shape = bds_shape(p);
+    return true;
+}
+
+void bds_device_finish(const xmss_params* p, unsigned char* sm, unsigned long long* smlen, unsigned long long mlen,
+                       const BdsDeviceSign& ds) {
+    ull_to_bytes(sm, p->index_bytes, ds.idx);
+    *smlen = p->sig_bytes + mlen;
+}
+
+Engine::BdsShape bds_shape(const xmss_params* p) {
+    return {p->tree_height, p->d, p->full_height, p->index_bytes, p->bds_k, p->wots_sig_bytes, p->sig_bytes,
+            p->sk_bytes};
 }
 
 // Backend of the C API: one key, launches straight on the engine; the
@@ -616,7 +674,14 @@ static int bds_sign_api(const xmss_params* p, unsigned char* sk, unsigned char* 
     Engine* e;
     int fam;
     if (int rc = prepare_engine(p, &e, &fam)) return rc;
-    DirectOps ops(e, fam, p->n, bds_plan_leaves(p, sk, mlen));
+    BdsDeviceSign ds;
+    if (bds_device_prepare(p, sk, sm, m, mlen, ds)) {
+        e->bds_sign_batch(fam, 1, &ds.job, false);
+        bds_device_finish(p, sm, smlen, mlen, ds);
+        return 0;
+    }
+    // last index (wiped key) or exhausted key: no traversal work, host control
+    DirectOps ops(e, fam, p->n, bds_plan_leaves(p, sk, mlen, false));
     return bds_sign(p, sk, sm, smlen, m, mlen, ops);
 }
 

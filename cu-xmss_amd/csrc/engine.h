@@ -99,6 +99,10 @@ public:
     // The engine of the current HIP device, created on first use.  Returns
     // nullptr (and prints why) when no usable GPU is present.
     static Engine* get();
+    // A second engine on the current device with its own stream and buffers,
+    // for work that runs concurrently with the shared engine (the serving
+    // loop's verifications).  Not shared; the caller serialises its use.
+    static Engine* make_private();
 
     void set_keys(int fam, int nkeys, const KeySeeds* seeds);
 
@@ -161,6 +165,37 @@ public:
     void collect_sign(Batch* b, uint8_t* const* out);
     void collect_verify(Batch* b, int* ok);
     void free_batch(Batch* b);
+
+    // ---- BDS signatures with the traversal on the device ----
+    // For each job: the planned leaves (in the order the signature consumes
+    // them), R and H_msg, the layer-0 WOTS signature and the state update of
+    // the serialized sk run in four launches on one stream; sk is updated in
+    // place.  Keys: set from the seeds, or pool slots when use_pool.
+    struct BdsJob {
+        const uint8_t* sk_seed;
+        const uint8_t* sk_prf;
+        const uint8_t* pub_seed;
+        const uint8_t* pub_root;
+        uint32_t key;          // pool slot (use_pool)
+        uint8_t* sk;           // OID-less sk, index already advanced
+        uint64_t idx;          // index being signed
+        const uint8_t* m;
+        uint64_t mlen;
+        const uint32_t* plan_layer;  // planned leaves
+        const uint64_t* plan_tree;
+        const uint32_t* plan_leaf;
+        uint32_t nplan;
+        uint8_t* R_out;        // n
+        uint8_t* wots0_out;    // wots_sig_bytes
+        uint8_t* tail_out;     // sig_bytes - index_bytes - n - wots_sig_bytes
+        struct Shape {
+            unsigned th, d, full_height, index_bytes, bds_k, wots_sig_bytes, sig_bytes;
+            unsigned long long sk_bytes;
+        } shape;
+    };
+    using BdsShape = BdsJob::Shape;
+    // Jobs may mix parameter shapes of one hash family.
+    void bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool);
 
     // Per-kernel HIP-event timing (off by default).
     void set_profiling(bool on);

@@ -66,6 +66,8 @@ struct DevBuf {
     }
 };
 
+struct BdsBufs;
+
 struct KeyPool {
     void* p = nullptr;
     size_t cap = 0;  // keys
@@ -76,6 +78,7 @@ struct Engine::Impl {
     Launcher L;
     KeyPool pool[F_COUNT];
     const void* kp_override = nullptr;  // pool in use, or null for `keys`
+    BdsBufs* bds = nullptr;             // buffers of bds_sign_batch
     const void* kp() const { return kp_override ? kp_override : keys.p; }
     Engine::Batch* spare[2] = {nullptr, nullptr};  // reusable verify / sign batches
     DevBuf keys, seeds, desc, ends, tmp, nodes, a, b, c, d, idx;
@@ -99,7 +102,9 @@ struct Engine::Impl {
 
 Engine::Engine() : impl_(new Impl) {
     HIPCK(hipGetDevice(&dev_));
-    HIPCK(hipStreamCreateWithFlags(&impl_->L.s, hipStreamNonBlocking));
+    HIPCK(hipStreamCreateWithFlags(&impl_->L.s_main, hipStreamNonBlocking));
+    HIPCK(hipStreamCreateWithFlags(&impl_->L.s_aux, hipStreamNonBlocking));
+    impl_->L.s = impl_->L.s_main;
     stream_ = impl_->L.s;
 }
 
@@ -120,6 +125,12 @@ Engine* Engine::get() {
     Engine* e = new Engine();
     engines[dev] = e;
     return e;
+}
+
+Engine* Engine::make_private() {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return nullptr;
+    return new Engine();
 }
 
 void Engine::set_profiling(bool on) {
@@ -698,6 +709,127 @@ void Engine::free_batch(Batch* b) {
         x->p = nullptr;
     }
     delete b;
+}
+
+}  // namespace xm
+
+namespace xm {
+
+struct BdsBufs {
+    DevBuf jobs, leaves_desc, leaf_arena, msg_desc, blob, R, dig, wdesc, wots0, sk_arena, tails, scratch;
+};
+
+void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
+    Impl* I = impl_;
+    if (!I->bds) I->bds = new BdsBufs;
+    BdsBufs& B = *I->bds;
+    FamOps* o = ops_of(fam);
+    const unsigned n = fam_n(fam), len = fam_len(fam), pad = fam_pad(fam);
+    if (!use_pool) {
+        std::vector<KeySeeds> ks(count);
+        for (int i = 0; i < count; ++i) {
+            ks[i] = {jobs[i].sk_seed, jobs[i].sk_prf, jobs[i].pub_seed};
+            jobs[i].key = (uint32_t)i;
+        }
+        set_keys(fam, count, ks.data());
+    } else {
+        pool_use(fam);
+    }
+    const void* dk = I->kp();
+    // per-job layout: planned leaves, sk bytes, signature tail
+    std::vector<DLeaf> ld;
+    std::vector<DBds> jd(count);
+    std::vector<size_t> tail_bytes(count);
+    size_t sk_total = 0, tail_total = 0, lds = 0;
+    for (int i = 0; i < count; ++i) {
+        const BdsShape& s = jobs[i].shape;
+        jd[i].leaf_off = ld.size();
+        for (uint32_t k = 0; k < jobs[i].nplan; ++k) {
+            const uint64_t t = jobs[i].plan_tree[k];
+            ld.push_back({jobs[i].key, jobs[i].plan_layer[k], (uint32_t)(t >> 32), (uint32_t)t, jobs[i].plan_leaf[k],
+                          0, 0, 0});
+        }
+        tail_bytes[i] = (size_t)s.sig_bytes - s.index_bytes - n - s.wots_sig_bytes;
+        jd[i].sk_off = sk_total;
+        sk_total += (s.sk_bytes + 7) & ~7ull;
+        jd[i].tail_off = tail_total;
+        tail_total += tail_bytes[i];
+        jd[i].idx = jobs[i].idx;
+        jd[i].key = jobs[i].key;
+        jd[i].p0 = 0;
+        BdsP& P = jd[i].P;
+        P = {s.th, s.d, s.full_height, s.index_bytes, s.bds_k, s.wots_sig_bytes, s.sig_bytes, 0};
+        P.state_bytes = (s.th + 1) * n + 4 + s.th + 1 + s.th * n + (s.th >> 1) * n + (s.th - s.bds_k) * (7 + n) +
+                        ((1u << s.bds_k) - s.bds_k - 1) * n + 4;
+        lds = std::max(lds, (size_t)(2 * s.d - 1) * P.state_bytes);
+    }
+    // H_msg inputs
+    const size_t pb = pad + 3 * n;
+    std::vector<DMsg> md(count);
+    size_t total = 0;
+    for (int i = 0; i < count; ++i) {
+        md[i] = {jobs[i].key, 1u, jobs[i].idx, (uint64_t)total, (uint32_t)(pb + jobs[i].mlen), 0};
+        total += (pb + jobs[i].mlen + 7) & ~(size_t)7;
+    }
+    total += 8;
+    std::vector<uint8_t> hb(total, 0);
+    std::vector<DWots> wd(count);
+    for (int i = 0; i < count; ++i) {
+        uint8_t* p = &hb[md[i].off];
+        p[pad - 1] = 2;
+        memcpy(p + pad + n, jobs[i].pub_root, n);
+        uint64_t v = jobs[i].idx;
+        for (int b = 0; b < 8; ++b) {
+            p[pad + 3 * n - 1 - b] = (uint8_t)(v & 0xff);
+            v >>= 8;
+        }
+        if (jobs[i].mlen) memcpy(p + pb, jobs[i].m, jobs[i].mlen);
+        const unsigned th = jobs[i].shape.th;
+        const uint64_t tree = jobs[i].idx >> th;
+        wd[i] = {jobs[i].key, 0, (uint32_t)(tree >> 32), (uint32_t)tree,
+                 (uint32_t)(jobs[i].idx & ((1ull << th) - 1)), MODE_SIGN, 0, 0};
+    }
+    // stage everything, then: leaves on the main stream while R / H_msg and
+    // the layer-0 WOTS signatures run on the aux stream; the state update
+    // waits for both
+    uint8_t* leaf_arena = B.leaf_arena.need(ld.size() * n);
+    DLeaf* dld = B.leaves_desc.need<DLeaf>(sizeof(DLeaf) * std::max<size_t>(1, ld.size()));
+    DMsg* dmd = B.msg_desc.need<DMsg>(sizeof(DMsg) * count);
+    uint8_t* db = B.blob.need(total);
+    DWots* dw = B.wdesc.need<DWots>(sizeof(DWots) * count);
+    uint8_t* ska = B.sk_arena.need(sk_total);
+    DBds* djobs = B.jobs.need<DBds>(sizeof(DBds) * count);
+    I->h2d(dld, ld.data(), sizeof(DLeaf) * ld.size());
+    I->h2d(dmd, md.data(), sizeof(DMsg) * count);
+    I->h2d(db, hb.data(), total);
+    I->h2d(dw, wd.data(), sizeof(DWots) * count);
+    for (int i = 0; i < count; ++i) I->h2d(ska + jd[i].sk_off, jobs[i].sk, jobs[i].shape.sk_bytes);
+    I->h2d(djobs, jd.data(), sizeof(DBds) * count);
+    hipEvent_t staged = I->L.ev(), front = I->L.ev();
+    HIPCK(hipEventRecord(staged, I->L.s_main));
+    HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
+    I->L.s = I->L.s_aux;
+    o->hmsg(I->L, dk, dmd, (uint32_t)count, pad, db, B.R.need((size_t)count * n), B.dig.need((size_t)count * n));
+    o->wots(I->L, dk, dw, (uint32_t)count, (const uint8_t*)B.dig.p, nullptr, B.wots0.need((size_t)count * len * n));
+    HIPCK(hipEventRecord(front, I->L.s_aux));
+    I->L.s = I->L.s_main;
+    if (!ld.empty()) o->leaf_wg(I->L, dk, dld, (uint32_t)ld.size(), false, nullptr, nullptr, nullptr, 0, leaf_arena);
+    HIPCK(hipStreamWaitEvent(I->L.s_main, front, 0));
+    o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, B.tails.need(tail_total),
+                  B.scratch.need((size_t)count * 4 * n));
+    std::vector<uint8_t> R((size_t)count * n), W((size_t)count * len * n), T(tail_total);
+    I->d2h(R.data(), B.R.p, R.size());
+    I->d2h(W.data(), B.wots0.p, W.size());
+    I->d2h(T.data(), B.tails.p, T.size());
+    for (int i = 0; i < count; ++i) I->d2h(jobs[i].sk, ska + jd[i].sk_off, jobs[i].shape.sk_bytes);
+    I->sync();
+    I->L.pool.push_back(staged);
+    I->L.pool.push_back(front);
+    for (int i = 0; i < count; ++i) {
+        memcpy(jobs[i].R_out, &R[(size_t)i * n], n);
+        memcpy(jobs[i].wots0_out, &W[(size_t)i * len * n], (size_t)len * n);
+        memcpy(jobs[i].tail_out, &T[jd[i].tail_off], tail_bytes[i]);
+    }
 }
 
 }  // namespace xm

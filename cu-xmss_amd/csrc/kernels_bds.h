@@ -1,0 +1,390 @@
+// kernels_bds.h -- the BDS traversal state machine on the device.
+//
+// One workgroup per signature (ceil(LEN/64) waves) advances that key's
+// serialized state in place, in HBM: bds_round, bds_treehash_update,
+// treehash_update, bds_state_update and deep_state_swap of
+// cuxmss/xmss_core_fast.c:165-626,917-1121, working directly on the sk bytes
+// (the reference's (de)serialize pair, :38-136, maps the same fields; reading
+// and writing them in place is equivalent because nothing else touches them
+// during a signature).  The leaves the signature consumes were computed
+// before, in the order a host dry run of the same code requested them, so the
+// kernel reads them from a cursor.  Control flow is uniform across the
+// workgroup (every thread reads the same bytes); every write is made by
+// thread 0, or by the chain lanes for a WOTS signature, and followed by a
+// barrier.  thash_h runs on thread 0.  The serialized states are staged in
+// LDS for the launch (at most 28 KB for the largest parameter set); the
+// lower-layer WOTS signatures stay in HBM.
+#pragma once
+#include "kernels.h"
+
+namespace xm {
+
+struct BdsP {
+    uint32_t th, d, full_height, index_bytes, bds_k, wots_sig_bytes, sig_bytes;
+    uint32_t state_bytes;  // one serialized bds_state
+};
+
+struct DBds {
+    uint64_t sk_off;      // OID-less sk in the sk arena
+    uint64_t leaf_off;    // first planned leaf (node index) in the leaf arena
+    uint64_t tail_off;    // signature bytes after R and the layer-0 WOTS signature
+    uint64_t idx;         // index being signed
+    uint32_t key;         // key context
+    uint32_t p0;
+    BdsP P;               // parameter shape of this key
+};
+
+template <class F>
+struct BdsDev {
+    uint8_t* sk;       // the serialized states, staged in LDS for the launch
+    uint8_t* sk_glob;  // the OID-less sk in HBM (lower-layer WOTS signatures live there)
+    const BdsP& P;
+    const uint8_t* leaves;
+    uint32_t cur;
+    const typename F::Key& key;
+    uint32_t tid;
+
+    static constexpr uint32_t n = F::kN;
+
+    __device__ BdsDev(uint8_t* lds, uint8_t* skg, const BdsP& P_, const uint8_t* lv, const typename F::Key& k)
+        : sk(lds), sk_glob(skg), P(P_), leaves(lv), cur(0), key(k), tid(threadIdx.x) {}
+
+    // ---- layout (xmss_core_fast.c:38-78), offsets within the states region ----
+    __device__ size_t st(uint32_t i) const { return (size_t)i * P.state_bytes; }
+    __device__ size_t states_base() const { return P.index_bytes + 4 * n; }
+    __device__ size_t o_stack(uint32_t i) const { return st(i); }
+    __device__ size_t o_stackoffset(uint32_t i) const { return st(i) + (size_t)(P.th + 1) * n; }
+    __device__ size_t o_stacklevels(uint32_t i) const { return o_stackoffset(i) + 4; }
+    __device__ size_t o_auth(uint32_t i) const { return o_stacklevels(i) + P.th + 1; }
+    __device__ size_t o_keep(uint32_t i) const { return o_auth(i) + (size_t)P.th * n; }
+    __device__ size_t o_th(uint32_t i, uint32_t j) const { return o_keep(i) + (size_t)(P.th >> 1) * n + (size_t)j * (7 + n); }
+    __device__ size_t o_retain(uint32_t i) const { return o_th(i, P.th - P.bds_k); }
+    __device__ size_t o_next_leaf(uint32_t i) const {
+        return o_retain(i) + (size_t)((1u << P.bds_k) - P.bds_k - 1) * n;
+    }
+    __device__ uint8_t* wots_sigs() const { return sk_glob + states_base() + (size_t)(2 * P.d - 1) * P.state_bytes; }
+
+    __device__ uint32_t rd(size_t off, int bytes) const {
+        uint32_t v = 0;
+        for (int b = 0; b < bytes; ++b) v = (v << 8) | sk[off + b];
+        return v;
+    }
+    __device__ void wr(size_t off, int bytes, uint32_t v) {
+        if (tid == 0)
+            for (int b = bytes - 1; b >= 0; --b) {
+                sk[off + b] = (uint8_t)v;
+                v >>= 8;
+            }
+        __syncthreads();
+    }
+    // field accessors
+    __device__ uint32_t stackoffset(uint32_t i) const { return rd(o_stackoffset(i), 4); }
+    __device__ void set_stackoffset(uint32_t i, uint32_t v) { wr(o_stackoffset(i), 4, v); }
+    __device__ uint32_t level(uint32_t i, uint32_t k) const { return sk[o_stacklevels(i) + k]; }
+    __device__ void set_level(uint32_t i, uint32_t k, uint32_t v) { wr(o_stacklevels(i) + k, 1, v); }
+    __device__ uint32_t th_h(uint32_t i, uint32_t j) const { return sk[o_th(i, j)]; }
+    __device__ uint32_t th_next(uint32_t i, uint32_t j) const { return rd(o_th(i, j) + 1, 4); }
+    __device__ uint32_t th_usage(uint32_t i, uint32_t j) const { return sk[o_th(i, j) + 5]; }
+    __device__ uint32_t th_done(uint32_t i, uint32_t j) const { return sk[o_th(i, j) + 6]; }
+    __device__ size_t th_node(uint32_t i, uint32_t j) const { return o_th(i, j) + 7; }
+    __device__ uint32_t next_leaf(uint32_t i) const { return rd(o_next_leaf(i), 4); }
+
+    // node copies (n bytes) by thread 0, then a barrier
+    __device__ void copy(uint8_t* dst, const uint8_t* src) {
+        if (tid == 0)
+            for (uint32_t b = 0; b < n; ++b) dst[b] = src[b];
+        __syncthreads();
+    }
+    __device__ void swap(size_t a, size_t b, size_t len) {
+        if (tid == 0)
+            for (size_t x = 0; x < len; ++x) {
+                const uint8_t t = sk[a + x];
+                sk[a + x] = sk[b + x];
+                sk[b + x] = t;
+            }
+        __syncthreads();
+    }
+    // dst <- thash_h(in[0..2n)) at ADRS a (thread 0); dst may alias in
+    __device__ void hash_h(uint8_t* dst, const uint8_t* in, const uint32_t (&a)[8]) {
+        if (tid == 0) {
+            typename F::Node L, R, o;
+            uint32_t ad[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) ad[i] = a[i];
+            // node buffers are not 4-byte aligned in the serialized state: gather bytes
+            load_bytes(L, in);
+            load_bytes(R, in + n);
+            F::thash_h(o, key, ad, L, R);
+            store_bytes(o, dst);
+        }
+        __syncthreads();
+    }
+    __device__ static void load_bytes(typename F::Node& x, const uint8_t* p) {
+        alignas(8) uint8_t b[F::kN];
+        for (uint32_t i = 0; i < F::kN; ++i) b[i] = p[i];
+        F::load(x, b);
+    }
+    __device__ static void store_bytes(const typename F::Node& x, uint8_t* p) {
+        alignas(8) uint8_t b[F::kN];
+        F::store(x, b);
+        for (uint32_t i = 0; i < F::kN; ++i) p[i] = b[i];
+    }
+    __device__ void leaf(uint8_t* dst) {
+        copy(dst, leaves + (size_t)cur * n);
+        ++cur;
+    }
+
+    // treehash_minheight_on_stack (xmss_core_fast.c:195-207)
+    __device__ uint32_t minheight(uint32_t s, uint32_t j) const {
+        uint32_t r = P.th;
+        const uint32_t so = stackoffset(s), use = th_usage(s, j);
+        for (uint32_t i = 0; i < use; ++i) {
+            const uint32_t l = level(s, so - i - 1);
+            if (l < r) r = l;
+        }
+        return r;
+    }
+
+    // treehash_update (xmss_core_fast.c:406-452) on instance j of state s;
+    // nb is a 2n-byte scratch in global memory
+    __device__ void treehash_update(uint32_t s, uint32_t j, const uint32_t (&addr)[8], uint8_t* nb) {
+        uint32_t nd[8] = {addr[0], addr[1], addr[2], 2, 0, 0, 0, 0};
+        const uint32_t next = th_next(s, j);
+        uint32_t nodeheight = 0;
+        leaf(nb);
+        for (;;) {
+            const uint32_t use = th_usage(s, j);
+            const uint32_t so = stackoffset(s);
+            if (!(use > 0 && level(s, so - 1) == nodeheight)) break;
+            copy(nb + n, nb);
+            copy(nb, sk + o_stack(s) + (size_t)(so - 1) * n);
+            nd[5] = nodeheight;
+            nd[6] = next >> (nodeheight + 1);
+            hash_h(nb, nb, nd);
+            ++nodeheight;
+            wr(o_th(s, j) + 5, 1, use - 1);
+            set_stackoffset(s, so - 1);
+        }
+        if (nodeheight == th_h(s, j)) {
+            copy(sk + th_node(s, j), nb);
+            wr(o_th(s, j) + 6, 1, 1);
+        } else {
+            const uint32_t so = stackoffset(s);
+            copy(sk + o_stack(s) + (size_t)so * n, nb);
+            wr(o_th(s, j) + 5, 1, th_usage(s, j) + 1);
+            set_level(s, so, nodeheight);
+            set_stackoffset(s, so + 1);
+            wr(o_th(s, j) + 1, 4, next + 1);
+        }
+    }
+
+    // bds_treehash_update (xmss_core_fast.c:458-491), `char` return kept
+    __device__ uint32_t bds_treehash_update(uint32_t s, uint32_t updates, const uint32_t (&addr)[8], uint8_t* nb) {
+        const uint32_t nth = P.th - P.bds_k;
+        uint32_t used = 0;
+        for (uint32_t u = 0; u < updates; ++u) {
+            uint32_t l_min = P.th, lev = nth;
+            for (uint32_t i = 0; i < nth; ++i) {
+                uint32_t low;
+                if (th_done(s, i)) low = P.th;
+                else if (th_usage(s, i) == 0) low = i;
+                else low = minheight(s, i);
+                if (low < l_min) {
+                    lev = i;
+                    l_min = low;
+                }
+            }
+            if (lev == nth) break;
+            treehash_update(s, lev, addr, nb);
+            ++used;
+        }
+        return (uint32_t)(int)(signed char)(updates - used);
+    }
+
+    // where a merged right child of height h and index j is recorded (:262-269, :534-541)
+    __device__ uint8_t* record_slot(uint32_t s, uint32_t h, uint32_t j) {
+        if (j == 1) return sk + o_auth(s) + (size_t)h * n;
+        if (h < P.th - P.bds_k && j == 3) return sk + th_node(s, h);
+        if (h >= P.th - P.bds_k)
+            return sk + o_retain(s) + (size_t)((1u << (P.th - 1 - h)) + h - P.th + ((j - 3) >> 1)) * n;
+        return nullptr;
+    }
+
+    // bds_state_update (xmss_core_fast.c:497-552)
+    __device__ void bds_state_update(uint32_t s, const uint32_t (&addr)[8]) {
+        const uint32_t idx = next_leaf(s);
+        if (idx == (1u << P.th)) return;
+        uint32_t nd[8] = {addr[0], addr[1], addr[2], 2, 0, 0, 0, 0};
+        uint32_t so = stackoffset(s);
+        leaf(sk + o_stack(s) + (size_t)so * n);
+        set_level(s, so, 0);
+        set_stackoffset(s, ++so);
+        while (so > 1 && level(s, so - 1) == level(s, so - 2)) {
+            const uint32_t nodeh = level(s, so - 1);
+            uint8_t* slot = record_slot(s, nodeh, idx >> nodeh);
+            if (slot) copy(slot, sk + o_stack(s) + (size_t)(so - 1) * n);
+            nd[5] = nodeh;
+            nd[6] = idx >> (nodeh + 1);
+            hash_h(sk + o_stack(s) + (size_t)(so - 2) * n, sk + o_stack(s) + (size_t)(so - 2) * n, nd);
+            set_level(s, so - 2, level(s, so - 2) + 1);
+            set_stackoffset(s, --so);
+        }
+        wr(o_next_leaf(s), 4, idx + 1);
+    }
+
+    // bds_round (xmss_core_fast.c:559-626); buf is a 2n-byte scratch
+    __device__ void bds_round(uint32_t s, uint32_t leaf_idx, const uint32_t (&addr)[8], uint8_t* buf) {
+        const uint32_t th = P.th, nth = th - P.bds_k;
+        uint32_t tau = th;
+        for (uint32_t i = 0; i < th; ++i)
+            if (!((leaf_idx >> i) & 1)) {
+                tau = i;
+                break;
+            }
+        if (tau > 0) {
+            copy(buf, sk + o_auth(s) + (size_t)(tau - 1) * n);
+            copy(buf + n, sk + o_keep(s) + (size_t)((tau - 1) >> 1) * n);
+        }
+        if (!((leaf_idx >> (tau + 1)) & 1) && (tau < th - 1))
+            copy(sk + o_keep(s) + (size_t)(tau >> 1) * n, sk + o_auth(s) + (size_t)tau * n);
+        if (tau == 0) {
+            leaf(sk + o_auth(s));
+        } else {
+            uint32_t nd[8] = {addr[0], addr[1], addr[2], 2, 0, tau - 1, leaf_idx >> tau, 0};
+            hash_h(sk + o_auth(s) + (size_t)tau * n, buf, nd);
+            for (uint32_t i = 0; i < tau; ++i) {
+                if (i < nth) {
+                    copy(sk + o_auth(s) + (size_t)i * n, sk + th_node(s, i));
+                } else {
+                    const uint32_t offset = (1u << (th - 1 - i)) + i - th;
+                    const uint32_t rowidx = ((leaf_idx >> i) - 1) >> 1;
+                    copy(sk + o_auth(s) + (size_t)i * n, sk + o_retain(s) + (size_t)(offset + rowidx) * n);
+                }
+            }
+            const uint32_t lim = tau < nth ? tau : nth;
+            for (uint32_t i = 0; i < lim; ++i) {
+                const uint32_t startidx = leaf_idx + 1 + 3 * (1u << i);
+                if (startidx < (1u << th)) {
+                    wr(o_th(s, i), 1, i);
+                    wr(o_th(s, i) + 1, 4, startidx);
+                    wr(o_th(s, i) + 6, 1, 0);
+                    wr(o_th(s, i) + 5, 1, 0);
+                }
+            }
+        }
+    }
+
+    // deep_state_swap (xmss_core_fast.c:165-193): states occupy equal, disjoint byte ranges
+    __device__ void swap_states(uint32_t a, uint32_t b) { swap(st(a), st(b), P.state_bytes); }
+
+    // wots_sign of the n bytes at msg (in sk) into sig (in sk), all chain lanes
+    __device__ void wots_sign(uint8_t* sig, const uint8_t* msg, uint32_t layer, uint64_t tree, uint32_t ots,
+                              uint8_t* scratch) {
+        // copy the message to an aligned scratch first (it may move during the swap)
+        if (tid < n) scratch[tid] = msg[tid];
+        __syncthreads();
+        if (tid < (uint32_t)F::LEN) {
+            uint32_t a[8] = {layer, (uint32_t)(tree >> 32), (uint32_t)tree, 0, ots, tid, 0, 0};
+            typename F::Node x;
+            F::prf_keygen(x, key, a);
+            run_chain<F>(x, key, a, 0, chain_len<F>(scratch, tid));
+            store_bytes(x, sig + (size_t)tid * n);
+        }
+        __syncthreads();
+    }
+};
+
+// The state part of one BDS signature per workgroup (everything after R, H_msg
+// and the layer-0 WOTS signature, which run in their own launches): the auth
+// paths and lower-layer WOTS signatures go to the signature tail, then the
+// traversal updates the state.  `scratch` holds 4n bytes per workgroup.
+template <class F>
+__global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __restrict__ keys, const DBds* __restrict__ jobs,
+                             uint8_t* __restrict__ sk_arena, const uint8_t* __restrict__ leaf_arena,
+                             uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena) {
+    extern __shared__ uint8_t lds_states[];
+    const DBds job = jobs[blockIdx.x];
+    const BdsP& P = job.P;
+    uint8_t* skg = sk_arena + job.sk_off;
+    BdsDev<F> B(lds_states, skg, P, leaf_arena + job.leaf_off * F::kN, keys[job.key]);
+    uint8_t* scratch = scratch_arena + (size_t)blockIdx.x * 4 * F::kN;
+    uint8_t* tail = tails + job.tail_off;
+    const uint32_t n = F::kN, th = P.th, d = P.d;
+    const uint64_t idx = job.idx;
+    const uint32_t region = (2 * d - 1) * P.state_bytes;
+    // stage the serialized states in LDS; written back when the update is done
+    for (uint32_t b = threadIdx.x; b < region; b += blockDim.x) lds_states[b] = skg[B.states_base() + b];
+    __syncthreads();
+    struct WriteBack {
+        uint8_t* l;
+        uint8_t* g;
+        uint32_t len;
+        __device__ ~WriteBack() {
+            __syncthreads();
+            for (uint32_t b = threadIdx.x; b < len; b += blockDim.x) g[b] = l[b];
+        }
+    } wb{lds_states, skg + B.states_base(), region};
+    // signature tail: auth of layer 0, then (WOTS signature, auth) of layers 1..d-1
+    {
+        size_t o = 0;
+        for (uint32_t i = 0; i < d; ++i) {
+            if (i > 0) {
+                const uint8_t* w = B.wots_sigs() + (size_t)(i - 1) * P.wots_sig_bytes;
+                for (uint32_t b = threadIdx.x; b < P.wots_sig_bytes; b += blockDim.x) tail[o + b] = w[b];
+                o += P.wots_sig_bytes;
+            }
+            const uint8_t* a = B.sk + B.o_auth(i);
+            for (uint32_t b = threadIdx.x; b < th * n; b += blockDim.x) tail[o + b] = a[b];
+            o += (size_t)th * n;
+        }
+        __syncthreads();
+    }
+    if (d == 1) {
+        // xmss_core_sign (xmss_core_fast.c:836-839); the host routes the last index elsewhere
+        if (idx < (1ull << th) - 1) {
+            const uint32_t addr[8] = {0, 0, 0, 0, (uint32_t)idx, 0, 0, 0};
+            B.bds_round(0, (uint32_t)idx, addr, scratch);
+            B.bds_treehash_update(0, (th - P.bds_k) >> 1, addr, scratch);
+        }
+        return;
+    }
+    // xmssmt_core_sign (xmss_core_fast.c:1068-1113)
+    uint64_t idx_tree = idx >> th;
+    uint32_t idx_leaf = (uint32_t)(idx & ((1u << th) - 1));
+    uint32_t addr[8] = {0, (uint32_t)((idx_tree + 1) >> 32), (uint32_t)(idx_tree + 1), 0, 0, 0, 0, 0};
+    int needswap_upto = -1;
+    uint32_t updates = (th - P.bds_k) >> 1;
+    if ((1 + idx_tree) * (1ull << th) + idx_leaf < (1ull << P.full_height)) B.bds_state_update(d, addr);
+    for (uint32_t i = 0; i < d; ++i) {
+        if (!(((idx + 1) & ((1ull << ((i + 1) * th)) - 1)) == 0)) {
+            idx_leaf = (uint32_t)((idx >> (th * i)) & ((1u << th) - 1));
+            idx_tree = idx >> (th * (i + 1));
+            addr[0] = i;
+            addr[1] = (uint32_t)(idx_tree >> 32);
+            addr[2] = (uint32_t)idx_tree;
+            if ((int)i == needswap_upto + 1) B.bds_round(i, idx_leaf, addr, scratch);
+            updates = B.bds_treehash_update(i, updates, addr, scratch);
+            addr[1] = (uint32_t)((idx_tree + 1) >> 32);
+            addr[2] = (uint32_t)(idx_tree + 1);
+            if ((1 + idx_tree) * (1ull << th) + idx_leaf < (1ull << (P.full_height - th * i))) {
+                if (i > 0 && updates > 0 && B.next_leaf(d + i) < (1ull << P.full_height)) {
+                    B.bds_state_update(d + i, addr);
+                    --updates;
+                }
+            }
+        } else if (idx < (1ull << P.full_height) - 1) {
+            B.swap_states(d + i, i);
+            const uint64_t t = (idx + 1) >> ((i + 2) * th);
+            const uint32_t ots = (uint32_t)(((idx >> ((i + 1) * th)) + 1) & ((1u << th) - 1));
+            B.wots_sign(B.wots_sigs() + (size_t)i * P.wots_sig_bytes, B.sk + B.o_stack(i), i + 1, t, ots,
+                        scratch + 2 * n);
+            B.set_stackoffset(d + i, 0);
+            B.wr(B.o_next_leaf(d + i), 4, 0);
+            --updates;  // WOTS signing counts as one update (:1107)
+            needswap_upto = (int)i;
+            for (uint32_t j = 0; j < th - P.bds_k; ++j) B.wr(B.o_th(i, j) + 6, 1, 1);
+        }
+    }
+}
+
+}  // namespace xm

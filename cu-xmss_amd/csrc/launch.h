@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "kernels_bds.h"
 
 namespace xm {
 
@@ -28,6 +29,7 @@ namespace xm {
 
 struct Launcher {
     hipStream_t s = nullptr;
+    hipStream_t s_main = nullptr, s_aux = nullptr;  // s is one of the two
     bool prof = false;
     struct Pending {
         const char* name;
@@ -51,13 +53,17 @@ struct Launcher {
     // them is that kernel's duration.
     template <typename K, typename... Args>
     void run(const char* name, K kernel, dim3 grid, dim3 block, Args... args) {
+        run_lds(name, kernel, grid, block, 0, args...);
+    }
+    template <typename K, typename... Args>
+    void run_lds(const char* name, K kernel, dim3 grid, dim3 block, size_t lds, Args... args) {
         if (grid.x == 0 || grid.y == 0) return;
         hipEvent_t e0 = nullptr;
         if (prof) {
             e0 = ev();
             HIPCK(hipEventRecord(e0, s));
         }
-        hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
         HIPCK(hipGetLastError());
         if (prof) {
             hipEvent_t e1 = ev();
@@ -103,6 +109,8 @@ struct FamOps {
                          uint8_t* out) = 0;
     virtual void hmsg(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad,
                       uint8_t* blob, uint8_t* R, uint8_t* dig) = 0;
+    virtual void bds_update(Launcher& L, const void* keys, size_t lds, const DBds* jobs, uint32_t n,
+                            uint8_t* sk_arena, const uint8_t* leaf_arena, uint8_t* tails, uint8_t* scratch) = 0;
 };
 
 template <class F>
@@ -153,6 +161,11 @@ struct FamOpsT final : FamOps {
     void hmsg(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad, uint8_t* blob,
               uint8_t* R, uint8_t* dig) override {
         L.run("hmsg", k_hmsg<F>, dim3(cdiv(n, WG)), dim3(WG), (const K*)keys, reqs, n, pad, blob, R, dig);
+    }
+    void bds_update(Launcher& L, const void* keys, size_t lds, const DBds* jobs, uint32_t n,
+                    uint8_t* sk_arena, const uint8_t* leaf_arena, uint8_t* tails, uint8_t* scratch) override {
+        L.run_lds("bds_update", k_bds_update<F>, dim3(n), dim3(((F::LEN + 63) / 64) * 64), lds, (const K*)keys,
+                  jobs, sk_arena, leaf_arena, tails, scratch);
     }
 };
 

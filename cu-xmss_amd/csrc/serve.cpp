@@ -18,6 +18,7 @@
 // FIFO order while the budget is exhausted.
 #include <string.h>
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -41,7 +42,8 @@ using Clock = std::chrono::steady_clock;
 
 // ---------------------------------------------------------------- dispatcher
 struct Call {
-    enum Kind { LEAVES, THASH, WOTS, HMSG, VERIFY } kind;
+    enum Kind { LEAVES, THASH, WOTS, HMSG, VERIFY, BDS } kind;
+    Engine::BdsJob* job = nullptr;  // BDS
     int fam = 0;
     int slot = 0;
     unsigned n = 0;
@@ -74,7 +76,7 @@ struct Call {
 
 class Dispatcher {
 public:
-    explicit Dispatcher(Engine* e) : e_(e), th_([this] { loop(); }) {}
+    Dispatcher(Engine* e, Engine* ev) : e_(e), ev_(ev), th_([this] { loop(); }) {}
     ~Dispatcher() {
         {
             std::lock_guard<std::mutex> lk(mu_);
@@ -125,8 +127,23 @@ private:
         }
     }
 
+    // Verifications run on their own engine (stream) in a helper thread while
+    // the other kinds run on the shared engine.
     void execute(std::vector<Call*>& batch) {
-        std::lock_guard<std::mutex> elk(engine_mutex());
+        std::vector<Call*> ver, rest;
+        for (Call* c : batch) (c->kind == Call::VERIFY ? ver : rest).push_back(c);
+        std::thread tv;
+        if (!ver.empty()) {
+            if (ev_) tv = std::thread([this, &ver] { execute_on(ev_, ver, false); });
+            else rest.insert(rest.end(), ver.begin(), ver.end());
+        }
+        execute_on(e_, rest, true);
+        if (tv.joinable()) tv.join();
+    }
+
+    void execute_on(Engine* e_, std::vector<Call*>& batch, bool shared) {
+        std::unique_lock<std::mutex> elk(engine_mutex(), std::defer_lock);
+        if (shared) elk.lock();
         std::map<std::pair<int, int>, std::vector<Call*>> groups;  // (fam, kind)
         for (Call* c : batch) groups[{c->fam, (int)c->kind}].push_back(c);
         for (auto& g : groups) {
@@ -187,6 +204,13 @@ private:
                     }
                     break;
                 }
+                case Call::BDS: {
+                    // signatures with the traversal on the device: one batch for all shapes
+                    std::vector<Engine::BdsJob> jobs;
+                    for (Call* c : cs) jobs.push_back(*c->job);
+                    e_->bds_sign_batch(fam, (int)jobs.size(), jobs.data(), true);
+                    break;
+                }
                 case Call::VERIFY: {
                     // one verify batch per parameter set
                     std::map<std::tuple<unsigned, unsigned, unsigned>, std::vector<Call*>> by;
@@ -214,13 +238,15 @@ private:
     }
 
     Engine* e_;
+    Engine* ev_;
     std::mutex mu_;
     std::condition_variable cv_, cv_done_;
     std::vector<Call*> pending_;
     int active_ = 0;
     bool stop_ = false;
     Clock::time_point first_;
-    uint64_t launches_ = 0, calls_ = 0;
+    std::atomic<uint64_t> launches_{0};
+    uint64_t calls_ = 0;
     std::thread th_;
 };
 
@@ -414,6 +440,20 @@ struct xmss_mi355x_server {
             // the key's sk is owned by this request while busy
             unsigned char* sk = k.sk.data() + XMSS_OID_LEN;
             r->out.assign(p.sig_bytes + r->data.size(), 0);
+            BdsDeviceSign ds;
+            if (bds_device_prepare(&p, sk, r->out.data(), r->data.data(), r->data.size(), ds)) {
+                ds.job.key = (uint32_t)k.slot;
+                Call call;
+                call.kind = Call::BDS;
+                call.fam = k.fam;
+                call.p = &p;
+                call.job = &ds.job;
+                disp->run(&call);
+                bds_device_finish(&p, r->out.data(), &r->outlen, r->data.size(), ds);
+                r->ret = 0;
+                return;
+            }
+            // last index or exhausted key: host control, hashing through the dispatcher
             ServeOps ops;
             ops.d = disp.get();
             ops.fam = k.fam;
@@ -465,7 +505,7 @@ xmss_mi355x_server* xmss_mi355x_server_create(size_t hbm_budget_bytes, int worke
     xmss_mi355x_server* s = new xmss_mi355x_server;
     s->e = e;
     s->budget = hbm_budget_bytes;
-    s->disp.reset(new Dispatcher(e));
+    s->disp.reset(new Dispatcher(e, Engine::make_private()));
     if (workers <= 0) workers = 256;
     for (int i = 0; i < workers; ++i) s->workers.emplace_back([s] { s->work(); });
     return s;
