@@ -388,17 +388,34 @@ def copy_calibration(nbytes):
     log("copy calibration: %d bytes device-to-device" % nbytes)
 
 
-def profiled_traffic(kernel="k_chains"):
-    """HBM bytes per launch of the keygen chain kernel, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated on a copy kernel)."""
+def profiled(field, kernel="k_chains"):
+    """A per-launch figure of the keygen chain kernel from the committed
+    rocprofv3 PMC summary: `traffic_bytes_per_launch` (FETCH_SIZE + WRITE_SIZE,
+    calibrated on a copy kernel) or `valu_lane_ops_per_launch` (SQ_INSTS_VALU
+    x 64)."""
     path = os.path.join(ROOT, "profiles", "keygen_xmss_sha2_16_256_pmc.md")
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch %s:" % kernel):
+            if line.startswith("%s %s:" % (field, kernel)):
                 return int(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def valu_peak(api, runs=3):
+    """Measured int32 VALU issue rate (T lane-ops/s): best of `runs` launches
+    of the library's probe kernel (4096 workgroups x 256 lanes x 4000 x 96
+    independent v_add3 / v_alignbit / v_bitop3 instructions)."""
+    L = api.lib
+    L.xmss_mi355x_valu_peak.argtypes = [C.c_uint, C.c_uint, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    best = 0.0
+    for _ in range(runs):
+        rate, ms = C.c_double(0), C.c_double(0)
+        if L.xmss_mi355x_valu_peak(4096, 4000, C.byref(rate), C.byref(ms)) != 0:
+            return None
+        best = max(best, rate.value)
+    return best / 1e12
 
 
 def main():
@@ -435,11 +452,26 @@ def main():
     roof = None
     if chains:
         avg_ms = chains[1] / chains[0]
-        achieved = comps_chains * OPS_PER_SHA256 / (avg_ms / 1000) / 1e12
+        peak_meas = valu_peak(api)
+        # Work per launch: the VALU lane-ops the kernel's instruction stream
+        # performs (static count, equal to SQ_INSTS_VALU x 64 in the profile).
+        # The nominal count (62 compressions x 1384 per chain) is 13% higher:
+        # the kernel shares PRF rounds along each chain and the compiler folds
+        # rounds and schedule words whose inputs are constants.
+        ops = profiled("valu_lane_ops_per_launch")
+        nominal = comps_chains * OPS_PER_SHA256
+        rate = lambda o: o / (avg_ms / 1000) / 1e12
+        achieved = rate(ops if ops else nominal)
         roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": profiled_traffic(),
+                "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": profiled("traffic_bytes_per_launch"),
                 "kernel": "k_chains<Sha256Fam<32>>", "kernel_ms": round(avg_ms, 4),
-                "algorithmic_ops": comps_chains * OPS_PER_SHA256}
+                "ops_per_launch": ops if ops else nominal,
+                "ops_counted": "VALU lane-ops executed (SQ_INSTS_VALU x 64)" if ops else "nominal",
+                "peak_measured": None if peak_meas is None else round(peak_meas, 3),
+                "frac_of_measured_peak": None if not peak_meas else round(achieved / peak_meas, 4),
+                "nominal_ops_per_launch": nominal,
+                "nominal_rate": round(rate(nominal), 3),
+                "nominal_frac": round(rate(nominal) / VALU_PEAK_TOPS, 4)}
     total_comps = 298582010
     keygen_frac = total_comps * OPS_PER_SHA256 / (ms_per_step / 1000) / 1e12 / VALU_PEAK_TOPS
 

@@ -300,27 +300,107 @@ __device__ __forceinline__ uint64_t rotl64(uint64_t x, int n) {
     return n == 0 ? x : rotr64(x, 64 - n);
 }
 
-__device__ __forceinline__ void keccak_f1600(uint64_t s[25]) {
-    uint64_t b[25];
-    for (int r = 0; r < 24; ++r) {
-        uint64_t c[5], d[5];
-#pragma unroll
-        for (int x = 0; x < 5; ++x)
-            c[x] = xor3_64(xor3_64(s[x], s[x + 5], s[x + 10]), s[x + 15], s[x + 20]);
-#pragma unroll
-        for (int x = 0; x < 5; ++x) d[x] = c[(x + 4) % 5] ^ rotl64(c[(x + 1) % 5], 1);
-#pragma unroll
-        for (int y = 0; y < 5; ++y)
-#pragma unroll
-            for (int x = 0; x < 5; ++x)
-                b[y + 5 * ((2 * x + 3 * y) % 5)] = rotl64(s[x + 5 * y] ^ d[x], KeccakConsts::RHO[x + 5 * y]);
-#pragma unroll
-        for (int y = 0; y < 5; ++y)
-#pragma unroll
-            for (int x = 0; x < 5; ++x)
-                s[x + 5 * y] = b[x + 5 * y] ^ (~b[(x + 1) % 5 + 5 * y] & b[(x + 2) % 5 + 5 * y]);
-        s[0] ^= KeccakConsts::RC[r];
+// One 64-bit lane as two 32-bit halves: every step of the round maps onto
+// one 32-bit instruction per half.  Per round: theta 20 (column parities,
+// two v_bitop3 xor3 per half) + 10 (rotl 1 of the parities, v_alignbit) +
+// 50 (one xor3 per half-lane), rho 48 (two v_alignbit per lane, lane 0 is
+// not rotated), chi 50 (one v_bitop3 a ^ (~b & c) per half-lane), iota 2:
+// 180 + 2 instructions; pi is register renaming.
+template <int N>
+__device__ __forceinline__ void rotl64_halves(uint32_t& lo, uint32_t& hi, uint32_t l, uint32_t h) {
+    if constexpr (N == 0) {
+        lo = l;
+        hi = h;
+    } else if constexpr (N < 32) {
+        hi = __builtin_amdgcn_alignbit(h, l, 32 - N);
+        lo = __builtin_amdgcn_alignbit(l, h, 32 - N);
+    } else if constexpr (N == 32) {
+        lo = h;
+        hi = l;
+    } else {
+        hi = __builtin_amdgcn_alignbit(l, h, 64 - N);
+        lo = __builtin_amdgcn_alignbit(h, l, 64 - N);
     }
+}
+
+// a ^ (~b & c) as one v_bitop3_b32: the truth table is the function applied
+// to the patterns 0xF0, 0xCC, 0xAA of (src0, src1, src2).
+__device__ __forceinline__ uint32_t chi3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0xD2);
+}
+
+struct KeccakRC32 {
+    static constexpr uint32_t LO[24] = {
+        0x00000001u, 0x00008082u, 0x0000808au, 0x80008000u, 0x0000808bu, 0x80000001u, 0x80008081u, 0x00008009u,
+        0x0000008au, 0x00000088u, 0x80008009u, 0x8000000au, 0x8000808bu, 0x0000008bu, 0x00008089u, 0x00008003u,
+        0x00008002u, 0x00000080u, 0x0000800au, 0x8000000au, 0x80008081u, 0x00008080u, 0x80000001u, 0x80008008u};
+    static constexpr uint32_t HI[24] = {
+        0x00000000u, 0x00000000u, 0x80000000u, 0x80000000u, 0x00000000u, 0x00000000u, 0x80000000u, 0x80000000u,
+        0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x80000000u, 0x80000000u, 0x80000000u,
+        0x80000000u, 0x80000000u, 0x00000000u, 0x80000000u, 0x80000000u, 0x80000000u, 0x00000000u, 0x80000000u};
+};
+
+template <int X, int Y>
+__device__ __forceinline__ void keccak_theta_rho_pi(uint32_t (&BL)[25], uint32_t (&BH)[25], const uint32_t (&L)[25],
+                                                    const uint32_t (&H)[25], const uint32_t (&cl)[5],
+                                                    const uint32_t (&ch)[5], const uint32_t (&rl)[5],
+                                                    const uint32_t (&rh)[5]) {
+    constexpr int i = X + 5 * Y;
+    constexpr int j = Y + 5 * ((2 * X + 3 * Y) % 5);
+    const uint32_t l = xor3(L[i], cl[(X + 4) % 5], rl[(X + 1) % 5]);
+    const uint32_t h = xor3(H[i], ch[(X + 4) % 5], rh[(X + 1) % 5]);
+    rotl64_halves<KeccakConsts::RHO[i]>(BL[j], BH[j], l, h);
+}
+
+template <int Y>
+__device__ __forceinline__ void keccak_row(uint32_t (&BL)[25], uint32_t (&BH)[25], const uint32_t (&L)[25],
+                                           const uint32_t (&H)[25], const uint32_t (&cl)[5], const uint32_t (&ch)[5],
+                                           const uint32_t (&rl)[5], const uint32_t (&rh)[5]) {
+    keccak_theta_rho_pi<0, Y>(BL, BH, L, H, cl, ch, rl, rh);
+    keccak_theta_rho_pi<1, Y>(BL, BH, L, H, cl, ch, rl, rh);
+    keccak_theta_rho_pi<2, Y>(BL, BH, L, H, cl, ch, rl, rh);
+    keccak_theta_rho_pi<3, Y>(BL, BH, L, H, cl, ch, rl, rh);
+    keccak_theta_rho_pi<4, Y>(BL, BH, L, H, cl, ch, rl, rh);
+}
+
+__device__ __forceinline__ void keccak_f1600(uint64_t s[25]) {
+    uint32_t L[25], H[25];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) {
+        L[i] = (uint32_t)s[i];
+        H[i] = (uint32_t)(s[i] >> 32);
+    }
+#pragma unroll 2
+    for (int r = 0; r < 24; ++r) {
+        uint32_t cl[5], ch[5], rl[5], rh[5];
+#pragma unroll
+        for (int x = 0; x < 5; ++x) {
+            cl[x] = xor3(xor3(L[x], L[x + 5], L[x + 10]), L[x + 15], L[x + 20]);
+            ch[x] = xor3(xor3(H[x], H[x + 5], H[x + 10]), H[x + 15], H[x + 20]);
+        }
+#pragma unroll
+        for (int x = 0; x < 5; ++x) {  // rotl64(c[x], 1)
+            rl[x] = __builtin_amdgcn_alignbit(cl[x], ch[x], 31);
+            rh[x] = __builtin_amdgcn_alignbit(ch[x], cl[x], 31);
+        }
+        uint32_t BL[25], BH[25];
+        keccak_row<0>(BL, BH, L, H, cl, ch, rl, rh);
+        keccak_row<1>(BL, BH, L, H, cl, ch, rl, rh);
+        keccak_row<2>(BL, BH, L, H, cl, ch, rl, rh);
+        keccak_row<3>(BL, BH, L, H, cl, ch, rl, rh);
+        keccak_row<4>(BL, BH, L, H, cl, ch, rl, rh);
+#pragma unroll
+        for (int y = 0; y < 5; ++y)
+#pragma unroll
+            for (int x = 0; x < 5; ++x) {
+                L[x + 5 * y] = chi3(BL[x + 5 * y], BL[(x + 1) % 5 + 5 * y], BL[(x + 2) % 5 + 5 * y]);
+                H[x + 5 * y] = chi3(BH[x + 5 * y], BH[(x + 1) % 5 + 5 * y], BH[(x + 2) % 5 + 5 * y]);
+            }
+        L[0] ^= KeccakRC32::LO[r];
+        H[0] ^= KeccakRC32::HI[r];
+    }
+#pragma unroll
+    for (int i = 0; i < 25; ++i) s[i] = ((uint64_t)H[i] << 32) | L[i];
 }
 
 }  // namespace xm

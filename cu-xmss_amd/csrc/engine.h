@@ -27,6 +27,9 @@ enum Fam : int {
     F_COUNT
 };
 
+// VALU issue-rate probe (probe.hip): lane-ops/s and kernel ms of one launch.
+int valu_probe(uint32_t blocks, uint32_t iters, double* lane_ops_per_s, double* ms);
+
 // func codes of xmss_params.func (params.h:7-9)
 int fam_of(unsigned func, unsigned n);  // -1 when core_hash would fail
 unsigned fam_n(int fam);

@@ -45,6 +45,12 @@ int xmss_mi355x_device_ok(void) {
     return Engine::get() != nullptr;
 }
 
+int xmss_mi355x_valu_peak(unsigned blocks, unsigned iters, double* lane_ops_per_s, double* ms) {
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    if (!Engine::get()) return XMSS_ENODEV;
+    return valu_probe(blocks, iters, lane_ops_per_s, ms);
+}
+
 void xmss_mi355x_profile(int on) {
     std::lock_guard<std::mutex> lk(engine_mutex());
     if (Engine* e = Engine::get()) e->set_profiling(on != 0);

@@ -1,0 +1,74 @@
+// VALU issue-rate probe for the roofline of the hash kernels.
+//
+// Every lane runs `iters` x 96 independent 32-bit integer VALU instructions
+// in the mix of a SHA-256 round (v_add3_u32, v_alignbit_b32, v_bitop3_b32),
+// eight accumulators per lane so that no wave waits on its own results.  The
+// instruction count is fixed by the inline assembly, so
+//     lane-ops = blocks x 256 x iters x 96
+// and lane-ops / kernel time is the int32 VALU rate the chip sustains.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "launch.h"
+
+namespace xm {
+
+#define XM_MIX(x, y, z)                                     \
+    asm volatile(                                           \
+        "v_add3_u32 %0, %0, %1, %2\n\t"                     \
+        "v_alignbit_b32 %0, %0, %0, 7\n\t"                  \
+        "v_bitop3_b32 %0, %0, %1, %2 bitop3:0x96"           \
+        : "+v"(x)                                           \
+        : "v"(y), "v"(z))
+
+__global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, uint32_t* __restrict__ out) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    uint32_t a0 = t, a1 = t + 1, a2 = t + 2, a3 = t + 3, a4 = t + 4, a5 = t + 5, a6 = t + 6, a7 = t + 7;
+    const uint32_t y = t * 0x9e3779b9u, z = t ^ 0x5bd1e995u;
+    for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            XM_MIX(a0, y, z);
+            XM_MIX(a1, y, z);
+            XM_MIX(a2, y, z);
+            XM_MIX(a3, y, z);
+            XM_MIX(a4, y, z);
+            XM_MIX(a5, y, z);
+            XM_MIX(a6, y, z);
+            XM_MIX(a7, y, z);
+        }
+    }
+    out[t] = a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7;
+}
+
+#undef XM_MIX
+
+// Runs the probe once (after one warm-up launch) on its own stream of the
+// current device; returns lane-ops/s and the kernel time in ms.
+int valu_probe(uint32_t blocks, uint32_t iters, double* lane_ops_per_s, double* ms_out) {
+    uint32_t* out = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCK(hipMalloc(&out, (size_t)blocks * 256 * sizeof(uint32_t)));
+    HIPCK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIPCK(hipEventCreate(&e0));
+    HIPCK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, s, iters / 8 + 1, out);
+    HIPCK(hipGetLastError());
+    HIPCK(hipEventRecord(e0, s));
+    hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, s, iters, out);
+    HIPCK(hipGetLastError());
+    HIPCK(hipEventRecord(e1, s));
+    HIPCK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCK(hipEventElapsedTime(&ms, e0, e1));
+    HIPCK(hipEventDestroy(e0));
+    HIPCK(hipEventDestroy(e1));
+    HIPCK(hipStreamDestroy(s));
+    HIPCK(hipFree(out));
+    *ms_out = ms;
+    *lane_ops_per_s = (double)blocks * 256.0 * iters * 96.0 / (ms / 1000.0);
+    return 0;
+}
+
+}  // namespace xm

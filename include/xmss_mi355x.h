@@ -30,6 +30,12 @@ void xmss_mi355x_reset_stats(void);
 /* Writes "name count total_ms\n" lines; returns the length needed. */
 size_t xmss_mi355x_kernel_stats(char *buf, size_t len);
 
+/* Measures the int32 VALU issue rate of the device: `blocks` workgroups of
+ * 256 lanes each run iters x 96 independent 32-bit integer instructions
+ * (the SHA-256 round mix).  Writes lane-ops/s and the kernel time; returns
+ * 0, or -3 without a GPU.  Used as the measured peak of the roofline. */
+int xmss_mi355x_valu_peak(unsigned blocks, unsigned iters, double *lane_ops_per_s, double *ms);
+
 /* ---- batches of independent keys, one parameter set per batch ----------
  * Plain-core signing of `count` OID-prefixed secret keys (each updated in
  * place exactly as xmss[mt]_sign would) and verification of `count`
