@@ -404,9 +404,10 @@ def profiled(field, kernel="k_chains"):
 
 
 def valu_peak(api, runs=3):
-    """Measured int32 VALU issue rate (T lane-ops/s): best of `runs` launches
-    of the library's probe kernel (4096 workgroups x 256 lanes x 4000 x 96
-    independent v_add3 / v_alignbit / v_bitop3 instructions)."""
+    """Measured int32 VALU rate (T lane-ops/s) for the instruction mix of the
+    SHA-256 chain step: best of `runs` launches of the library's probe kernel
+    (4096 workgroups x 256 lanes x 4000 x 96 instructions, per 12: 5
+    v_alignbit, 2 v_add3, 3 v_bitop3, 1 v_add, 1 v_lshrrev)."""
     L = api.lib
     L.xmss_mi355x_valu_peak.argtypes = [C.c_uint, C.c_uint, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     best = 0.0
@@ -462,16 +463,21 @@ def main():
         nominal = comps_chains * OPS_PER_SHA256
         rate = lambda o: o / (avg_ms / 1000) / 1e12
         achieved = rate(ops if ops else nominal)
-        roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": profiled("traffic_bytes_per_launch"),
+        # AMD publishes no int32 VALU figure; 256 CU x 64 lanes x 2.4 GHz is
+        # the spec derivation, but gfx950 issues VOP2 ops and v_bitop3 faster
+        # than that, so the peak is the probe's rate for this kernel's mix.
+        peak = peak_meas if peak_meas else VALU_PEAK_TOPS
+        roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3),
+                "unit": "TOP/s", "frac": round(achieved / peak, 4), "traffic": profiled("traffic_bytes_per_launch"),
                 "kernel": "k_chains<Sha256Fam<32>>", "kernel_ms": round(avg_ms, 4),
                 "ops_per_launch": ops if ops else nominal,
                 "ops_counted": "VALU lane-ops executed (SQ_INSTS_VALU x 64)" if ops else "nominal",
-                "peak_measured": None if peak_meas is None else round(peak_meas, 3),
-                "frac_of_measured_peak": None if not peak_meas else round(achieved / peak_meas, 4),
+                "peak_source": "measured, chain-step instruction mix" if peak_meas else "spec",
+                "peak_spec": round(VALU_PEAK_TOPS, 2),
+                "frac_of_spec": round(achieved / VALU_PEAK_TOPS, 4),
                 "nominal_ops_per_launch": nominal,
                 "nominal_rate": round(rate(nominal), 3),
-                "nominal_frac": round(rate(nominal) / VALU_PEAK_TOPS, 4)}
+                "nominal_frac_of_spec": round(rate(nominal) / VALU_PEAK_TOPS, 4)}
     total_comps = 298582010
     keygen_frac = total_comps * OPS_PER_SHA256 / (ms_per_step / 1000) / 1e12 / VALU_PEAK_TOPS
 
@@ -509,7 +515,7 @@ def main():
         "config": {"workload": "XMSS-SHA2_16_256 keygen (65,536 leaves); batch 4096 x XMSS-SHA2_10_256 "
                                "plain sign + verify; 1024 sequential BDS signs",
                    "parallelism": "single GPU"},
-        "keygen_valu_frac": round(keygen_frac, 4),
+        "keygen_nominal_frac_of_spec": round(keygen_frac, 4),
         "keygens_per_s_batch4096": rate(bkeygen_t),
         "signs_per_s_batch4096": rate(sign_run),
         "signs_per_s_batch4096_incl_pcie": rate(sign_total),

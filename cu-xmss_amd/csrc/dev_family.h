@@ -32,6 +32,7 @@ template <int N>
 struct Sha256Fam {
     static constexpr int kN = N;
     static constexpr bool kChainPrf = (N == 32);  // chain_prf_init / chain_prf_step available
+    static constexpr int kChainWaves = 1;         // no occupancy request for k_chains
     static constexpr int NW = N / 4;         // node words (big-endian values)
     static constexpr int LEN = 8 * N / 4 + 3;  // wots_len for w = 16
     static constexpr int BLOCK = 64;
@@ -352,6 +353,7 @@ struct Sha256Fam {
 struct Sha512Fam {
     static constexpr int kN = 64;
     static constexpr bool kChainPrf = false;
+    static constexpr int kChainWaves = 4;  // k_chains within 128 VGPRs
     static constexpr int NW = 8;
     static constexpr int LEN = 8 * 64 / 4 + 3;  // 131
     using Word = uint64_t;
@@ -447,7 +449,7 @@ struct Sha512Fam {
 #pragma unroll
         for (int i = 0; i < 8; ++i) out.w[i] = st[i];
     }
-    __device__ static void thash_h(Node& out, const Key& k, uint32_t (&a)[8], const Node& l,
+    __device__ __forceinline__ static void thash_h(Node& out, const Key& k, uint32_t (&a)[8], const Node& l,
                                    const Node& r) {
         Node t;
         uint64_t ad[4];
@@ -535,6 +537,7 @@ template <int RATE, int N>
 struct ShakeFam {
     static constexpr int kN = N;
     static constexpr bool kChainPrf = false;
+    static constexpr int kChainWaves = 1;
     static constexpr int NW = N / 4;
     static constexpr int LEN = 8 * N / 4 + 3;
     static constexpr int PADW = (N == 24) ? 1 : N / 4;  // padding_len in words

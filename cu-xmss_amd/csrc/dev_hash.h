@@ -173,15 +173,30 @@ __device__ __forceinline__ uint64_t rotr64(uint64_t x, int n) {
     }
     return ((uint64_t)rhi << 32) | rlo;
 }
+// Two 32-bit halves as one 64-bit register pair.  The empty asm hides the
+// halves from the optimizer, so a following 64-bit add stays one
+// v_lshl_add_u64 instead of being split into an add of the low half (with a
+// zero high half) and a separate add into the high half.
+__device__ __forceinline__ uint64_t pair64(uint32_t lo, uint32_t hi) {
+    uint64_t x = ((uint64_t)hi << 32) | lo;
+    asm("" : "+v"(x));
+    return x;
+}
 __device__ __forceinline__ uint64_t xor3_64(uint64_t a, uint64_t b, uint64_t c) {
     const uint32_t lo = xor3((uint32_t)a, (uint32_t)b, (uint32_t)c);
     const uint32_t hi = xor3((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32));
-    return ((uint64_t)hi << 32) | lo;
+    return pair64(lo, hi);
 }
 __device__ __forceinline__ uint64_t maj3_64(uint64_t a, uint64_t b, uint64_t c) {
     const uint32_t lo = maj3((uint32_t)a, (uint32_t)b, (uint32_t)c);
     const uint32_t hi = maj3((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32));
-    return ((uint64_t)hi << 32) | lo;
+    return pair64(lo, hi);
+}
+// (e & f) ^ (~e & g) per half (one v_bitop3 each)
+__device__ __forceinline__ uint64_t ch_64(uint64_t e, uint64_t f, uint64_t g) {
+    const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)e, (uint32_t)f, (uint32_t)g, 0xCA);
+    const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(e >> 32), (uint32_t)(f >> 32), (uint32_t)(g >> 32), 0xCA);
+    return pair64(lo, hi);
 }
 
 __device__ __constant__ static const uint64_t kSha512K[80] = {
@@ -226,7 +241,7 @@ __device__ __forceinline__ void sha512_round(uint64_t* v, uint64_t* w, int t, in
         w[s] = w[s] + s0 + s1 + w[(s + 9) & 15];
     }
     const uint64_t S1 = xor3_64(rotr64(e, 14), rotr64(e, 18), rotr64(e, 41));
-    const uint64_t ch = (e & f) ^ (~e & g);
+    const uint64_t ch = ch_64(e, f, g);
     const uint64_t t1 = h + S1 + ch + k + w[s];
     const uint64_t S0 = xor3_64(rotr64(a, 28), rotr64(a, 34), rotr64(a, 39));
     d += t1;

@@ -81,7 +81,8 @@ __device__ __forceinline__ void run_chain(typename F::Node& x, const typename F:
 
 // Chain ends of `leaves` consecutive leaves per run, from run.leaf0 + leaf_off.
 template <class F>
-__global__ __launch_bounds__(WG) void k_chains(const typename F::Key* __restrict__ keys,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(F::kChainWaves)))
+void k_chains(const typename F::Key* __restrict__ keys,
                                                 const DRun* __restrict__ runs, uint32_t leaf_off,
                                                 uint32_t leaves, uint8_t* __restrict__ ends) {
     const uint32_t r = blockIdx.y;

@@ -1,11 +1,17 @@
 // VALU issue-rate probe for the roofline of the hash kernels.
 //
-// Every lane runs `iters` x 96 independent 32-bit integer VALU instructions
-// in the mix of a SHA-256 round (v_add3_u32, v_alignbit_b32, v_bitop3_b32),
-// eight accumulators per lane so that no wave waits on its own results.  The
-// instruction count is fixed by the inline assembly, so
+// gfx950 does not issue every 32-bit VALU instruction at one wave64 per four
+// cycles per SIMD: VOP2/VOP1 ops (v_add_u32, v_xor_b32, v_lshrrev_b32) and
+// v_bitop3_b32 sustain up to 1.6-1.9x the rate of three-operand VOP3 ops
+// (v_alignbit_b32, v_add3_u32, v_perm_b32, v_bfi_b32).  The attainable peak of
+// a hash kernel therefore depends on its instruction mix, and this probe
+// issues the mix of the SHA-256 WOTS chain step: per 12 instructions 5
+// v_alignbit_b32, 2 v_add3_u32, 3 v_bitop3_b32, 1 v_add_u32, 1 v_lshrrev_b32
+// (the chain step is 41.5% / 17.0% / 25.6% / 9.2% / 6.4% of these).  Eight
+// independent accumulators per lane keep every wave issuing; the count is
+// fixed by the inline assembly, so
 //     lane-ops = blocks x 256 x iters x 96
-// and lane-ops / kernel time is the int32 VALU rate the chip sustains.
+// and lane-ops / kernel time is the rate the chip sustains for that mix.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -13,11 +19,20 @@
 
 namespace xm {
 
-#define XM_MIX(x, y, z)                                     \
+#define XM_MIX12(x, y, z)                                   \
     asm volatile(                                           \
-        "v_add3_u32 %0, %0, %1, %2\n\t"                     \
         "v_alignbit_b32 %0, %0, %0, 7\n\t"                  \
-        "v_bitop3_b32 %0, %0, %1, %2 bitop3:0x96"           \
+        "v_bitop3_b32 %0, %0, %1, %2 bitop3:0x96\n\t"       \
+        "v_alignbit_b32 %0, %1, %0, 11\n\t"                 \
+        "v_add3_u32 %0, %0, %1, %2\n\t"                     \
+        "v_alignbit_b32 %0, %0, %0, 25\n\t"                 \
+        "v_bitop3_b32 %0, %0, %1, %2 bitop3:0xe8\n\t"       \
+        "v_lshrrev_b32 %0, 3, %0\n\t"                       \
+        "v_alignbit_b32 %0, %2, %0, 13\n\t"                 \
+        "v_add_u32 %0, %0, %1\n\t"                          \
+        "v_bitop3_b32 %0, %0, %1, %2 bitop3:0xca\n\t"       \
+        "v_alignbit_b32 %0, %0, %0, 22\n\t"                 \
+        "v_add3_u32 %0, %0, %1, %2"                          \
         : "+v"(x)                                           \
         : "v"(y), "v"(z))
 
@@ -26,22 +41,19 @@ __global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, uint32_t* __
     uint32_t a0 = t, a1 = t + 1, a2 = t + 2, a3 = t + 3, a4 = t + 4, a5 = t + 5, a6 = t + 6, a7 = t + 7;
     const uint32_t y = t * 0x9e3779b9u, z = t ^ 0x5bd1e995u;
     for (uint32_t i = 0; i < iters; ++i) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            XM_MIX(a0, y, z);
-            XM_MIX(a1, y, z);
-            XM_MIX(a2, y, z);
-            XM_MIX(a3, y, z);
-            XM_MIX(a4, y, z);
-            XM_MIX(a5, y, z);
-            XM_MIX(a6, y, z);
-            XM_MIX(a7, y, z);
-        }
+        XM_MIX12(a0, y, z);
+        XM_MIX12(a1, y, z);
+        XM_MIX12(a2, y, z);
+        XM_MIX12(a3, y, z);
+        XM_MIX12(a4, y, z);
+        XM_MIX12(a5, y, z);
+        XM_MIX12(a6, y, z);
+        XM_MIX12(a7, y, z);
     }
     out[t] = a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7;
 }
 
-#undef XM_MIX
+#undef XM_MIX12
 
 // Runs the probe once (after one warm-up launch) on its own stream of the
 // current device; returns lane-ops/s and the kernel time in ms.
