@@ -167,6 +167,40 @@ def bench_batch(api, count):
     return keygen_t, sign_run, sign_total, ver_t, st, sum(1 for i in idxs if i == 1023)
 
 
+def bench_plain_single(api, count):
+    """The plain one-request-per-launch form of the batch workload: `count`
+    XMSS-SHA2_10_256 keys (leaves uniform in [0, 1022], messages log-uniform
+    32 B - 4 KB), each signed and then verified by its own xmss_sign /
+    xmss_sign_open call through the plain-core library.  Returns signs/s,
+    verifies/s and p50/p99 per-call latency in ms."""
+    L = api.lib
+    rng = random.Random(SEED + 17)
+    seeds, idxs, msgs = batch_inputs(count, rng)
+    idxs = [min(i, 1022) for i in idxs]
+    p = api.params(0x01, 0)
+    pks = [buf(4 + p.pk_bytes) for _ in range(count)]
+    sks = [buf(4 + p.sk_bytes) for _ in range(count)]
+    assert L.xmss_batch_keypair(0, ptrs(pks), ptrs(sks), seeds, 0x01, count) == 0
+    slat, vlat, sms = [], [], []
+    for i in range(count):
+        sks[i][4:8] = list(idxs[i].to_bytes(4, "big"))
+        t0 = time.perf_counter()
+        rc, sm, _ = api.sign(raw(sks[i], 4 + p.sk_bytes), msgs[i], 0)
+        slat.append(time.perf_counter() - t0)
+        assert rc == 0
+        sms.append(sm)
+    for i in range(count):
+        t0 = time.perf_counter()
+        vr, mm = api.sign_open(sms[i], raw(pks[i], 4 + p.pk_bytes), 0)
+        vlat.append(time.perf_counter() - t0)
+        assert vr == 0 and mm == msgs[i]
+    q = lambda v, f: sorted(v)[min(len(v) - 1, int(math.ceil(f * len(v))) - 1)]
+    return {"keys": count, "signs_per_s": round(count / sum(slat), 1),
+            "verifies_per_s": round(count / sum(vlat), 1),
+            "sign_p50_ms": round(1000 * q(slat, 0.5), 3), "sign_p99_ms": round(1000 * q(slat, 0.99), 3),
+            "verify_p50_ms": round(1000 * q(vlat, 0.5), 3), "verify_p99_ms": round(1000 * q(vlat, 0.99), 3)}
+
+
 MIXED = [("XMSSMT-SHA2_20/2_256", 1, 0x01), ("XMSS-SHA2_10_512", 0, 0x04), ("XMSS-SHAKE_10_256", 0, 0x07)]
 
 
@@ -403,17 +437,23 @@ def profiled(field, kernel="k_chains"):
     return None
 
 
-def valu_peak(api, runs=3):
-    """Measured int32 VALU rate (T lane-ops/s) for the instruction mix of the
-    SHA-256 chain step: best of `runs` launches of the library's probe kernel
-    (4096 workgroups x 256 lanes x 4000 x 96 instructions, per 12: 5
-    v_alignbit, 2 v_add3, 3 v_bitop3, 1 v_add, 1 v_lshrrev)."""
+VALU_CLASSES = {1: "v_alignbit_b32", 2: "v_add3_u32", 3: "v_bitop3_b32", 4: "v_add_u32/v_xor_b32",
+                5: "v_perm_b32/v_bfi_b32"}
+
+
+def valu_peak(api, kind=0, runs=3):
+    """Measured int32 VALU rate (T lane-ops/s): best of `runs` launches of the
+    library's probe kernel (4096 workgroups x 256 lanes x 4000 x 96
+    instructions).  Kind 0 issues the instruction mix of the SHA-256 chain
+    step (per 12: 5 v_alignbit, 2 v_add3, 3 v_bitop3, 1 v_add, 1 v_lshrrev);
+    kinds 1-5 one instruction class each (VALU_CLASSES)."""
     L = api.lib
-    L.xmss_mi355x_valu_peak.argtypes = [C.c_uint, C.c_uint, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.xmss_mi355x_valu_probe.argtypes = [C.c_int, C.c_uint, C.c_uint, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]
     best = 0.0
     for _ in range(runs):
         rate, ms = C.c_double(0), C.c_double(0)
-        if L.xmss_mi355x_valu_peak(4096, 4000, C.byref(rate), C.byref(ms)) != 0:
+        if L.xmss_mi355x_valu_probe(kind, 4096, 4000, C.byref(rate), C.byref(ms)) != 0:
             return None
         best = max(best, rate.value)
     return best / 1e12
@@ -427,6 +467,8 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--latency-signs", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--single", type=int, default=256,
+                    help="keys signed one xmss_sign call each, beside the batch (0: skip)")
     ap.add_argument("--mixed", type=int, default=1024, help="keys per parameter set of the mixed batch (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4000.0, help="offered requests/s of the serving run (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
@@ -474,6 +516,7 @@ def main():
                 "ops_counted": "VALU lane-ops executed (SQ_INSTS_VALU x 64)" if ops else "nominal",
                 "peak_source": "measured, chain-step instruction mix" if peak_meas else "spec",
                 "peak_spec": round(VALU_PEAK_TOPS, 2),
+                "class_rates": {nm: round(valu_peak(api, k, runs=1) or 0, 2) for k, nm in VALU_CLASSES.items()},
                 "frac_of_spec": round(achieved / VALU_PEAK_TOPS, 4),
                 "nominal_ops_per_launch": nominal,
                 "nominal_rate": round(rate(nominal), 3),
@@ -487,6 +530,10 @@ def main():
         log("batch keygen/sign/verify x%d ..." % args.batch)
         bkeygen_t, sign_run, sign_total, ver_t, bst, n_last = bench_batch(api, args.batch)
     rate = lambda t: None if not t else round(args.batch / t, 2)
+    single = None
+    if args.batch and args.single:
+        log("plain one-request-per-launch x%d ..." % args.single)
+        single = bench_plain_single(api, args.single)
     mixed = None
     if args.mixed:
         log("mixed batch x%d per set ..." % args.mixed)
@@ -528,6 +575,7 @@ def main():
         "kernels_keygen_ms": {k: round(v[1], 4) for k, v in kst.items()},
         "kernels_batch_sign_ms": {k: round(v[1], 3) for k, v in bst.items()},
         "batch_last_index_rejected": n_last,
+        "plain_one_per_call": single,
         "mixed_batch": mixed,
         "serving": serve,
     }

@@ -27,8 +27,10 @@ enum Fam : int {
     F_COUNT
 };
 
-// VALU issue-rate probe (probe.hip): lane-ops/s and kernel ms of one launch.
-int valu_probe(uint32_t blocks, uint32_t iters, double* lane_ops_per_s, double* ms);
+// VALU issue-rate probe (probe.hip): lane-ops/s and kernel ms of one launch
+// of instruction mix `kind` (0: SHA-256 chain step; 1-5: single classes).
+constexpr int VALU_PROBE_KINDS = 6;
+int valu_probe(int kind, uint32_t blocks, uint32_t iters, double* lane_ops_per_s, double* ms);
 
 // func codes of xmss_params.func (params.h:7-9)
 int fam_of(unsigned func, unsigned n);  // -1 when core_hash would fail

@@ -46,9 +46,13 @@ int xmss_mi355x_device_ok(void) {
 }
 
 int xmss_mi355x_valu_peak(unsigned blocks, unsigned iters, double* lane_ops_per_s, double* ms) {
+    return xmss_mi355x_valu_probe(0, blocks, iters, lane_ops_per_s, ms);
+}
+
+int xmss_mi355x_valu_probe(int kind, unsigned blocks, unsigned iters, double* lane_ops_per_s, double* ms) {
     std::lock_guard<std::mutex> lk(engine_mutex());
     if (!Engine::get()) return XMSS_ENODEV;
-    return valu_probe(blocks, iters, lane_ops_per_s, ms);
+    return valu_probe(kind, blocks, iters, lane_ops_per_s, ms);
 }
 
 void xmss_mi355x_profile(int on) {

@@ -35,6 +35,10 @@ size_t xmss_mi355x_kernel_stats(char *buf, size_t len);
  * (the SHA-256 round mix).  Writes lane-ops/s and the kernel time; returns
  * 0, or -3 without a GPU.  Used as the measured peak of the roofline. */
 int xmss_mi355x_valu_peak(unsigned blocks, unsigned iters, double *lane_ops_per_s, double *ms);
+/* The same for one instruction class: kind 0 is the mix above, 1 v_alignbit_b32,
+ * 2 v_add3_u32, 3 v_bitop3_b32, 4 v_add_u32 / v_xor_b32, 5 v_perm_b32 / v_bfi_b32
+ * (96 instructions per lane and iteration each).  Returns -1 for an unknown kind. */
+int xmss_mi355x_valu_probe(int kind, unsigned blocks, unsigned iters, double *lane_ops_per_s, double *ms);
 
 /* ---- batches of independent keys, one parameter set per batch ----------
  * Plain-core signing of `count` OID-prefixed secret keys (each updated in
