@@ -304,6 +304,109 @@ struct Sha256Fam {
         for (int i = 0; i < NW; ++i) out.w[i] = st[i];
     }
 
+    // ---- split tweakable hashes ---------------------------------------
+    // Everything in thash_f / thash_h that depends only on the address -- the
+    // PRF key and masks and, for n = 32, the first block of the outer hash
+    // (toByte(0|1, 32) || key) -- is computed by pre_f / pre_h; fin_f / fin_h
+    // then finish with the chain value or the two children: one compression
+    // (F) or two (H) for n = 32.  fin(pre(a), x) == thash(a, x) bit for bit.
+    static constexpr int PREW = (N == 32) ? 8 : NW;  // midstate, or the key
+    struct FPre { uint32_t s[PREW]; Node mask; };
+    struct HPre { uint32_t s[PREW]; Node m0, m1; };
+
+    __device__ static void pre_f(FPre& p, const Key& k, uint32_t (&a)[8]) {
+        Node key;
+        if constexpr (N == 32) {
+            Node km[2];
+            prf_km<2>(km, k, a);
+            key = km[0];
+            p.mask = km[1];
+            uint32_t w[16];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) { w[i] = 0; w[8 + i] = key.w[i]; }  // toByte(0, 32) || key
+            sha256_iv(p.s);
+            sha256_compress(p.s, w);
+        } else {
+            a[7] = 0;
+            prf8(key, k.ms_pub, k.pub, a);
+            a[7] = 1;
+            prf8(p.mask, k.ms_pub, k.pub, a);
+#pragma unroll
+            for (int i = 0; i < NW; ++i) p.s[i] = key.w[i];
+        }
+    }
+    __device__ static void fin_f(Node& out, const FPre& p, const Node& x) {
+        uint32_t st[8];
+        if constexpr (N == 32) {
+            uint32_t m[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) { st[i] = p.s[i]; m[i] = x.w[i] ^ p.mask.w[i]; }
+            sha256_tail<8>(st, m, 96);
+        } else {
+            uint32_t m[1 + 2 * NW];
+            m[0] = PAD_F;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) { m[1 + i] = p.s[i]; m[1 + NW + i] = x.w[i] ^ p.mask.w[i]; }
+            sha256_iv(st);
+            sha256_tail<1 + 2 * NW>(st, m, 4 + 2 * N);
+        }
+#pragma unroll
+        for (int i = 0; i < NW; ++i) out.w[i] = st[i];
+    }
+    __device__ static void pre_h(HPre& p, const Key& k, uint32_t (&a)[8]) {
+        Node key;
+        if constexpr (N == 32) {
+            Node km[3];
+            prf_km<3>(km, k, a);
+            key = km[0];
+            p.m0 = km[1];
+            p.m1 = km[2];
+            uint32_t w[16];
+#pragma unroll
+            for (int i = 0; i < 7; ++i) w[i] = 0;
+            w[7] = PAD_H;  // toByte(1, 32) || key
+#pragma unroll
+            for (int i = 0; i < 8; ++i) w[8 + i] = key.w[i];
+            sha256_iv(p.s);
+            sha256_compress(p.s, w);
+        } else {
+            a[7] = 0;
+            prf8(key, k.ms_pub, k.pub, a);
+            a[7] = 1;
+            prf8(p.m0, k.ms_pub, k.pub, a);
+            a[7] = 2;
+            prf8(p.m1, k.ms_pub, k.pub, a);
+#pragma unroll
+            for (int i = 0; i < NW; ++i) p.s[i] = key.w[i];
+        }
+    }
+    __device__ static void fin_h(Node& out, const HPre& p, const Node& l, const Node& r) {
+        uint32_t st[8];
+        if constexpr (N == 32) {
+            uint32_t m[16];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                st[i] = p.s[i];
+                m[i] = l.w[i] ^ p.m0.w[i];
+                m[8 + i] = r.w[i] ^ p.m1.w[i];
+            }
+            sha256_tail<16>(st, m, 128);
+        } else {
+            uint32_t m[1 + 3 * NW];
+            m[0] = PAD_H;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) {
+                m[1 + i] = p.s[i];
+                m[1 + NW + i] = l.w[i] ^ p.m0.w[i];
+                m[1 + 2 * NW + i] = r.w[i] ^ p.m1.w[i];
+            }
+            sha256_iv(st);
+            sha256_tail<1 + 3 * NW>(st, m, 4 + 3 * N);
+        }
+#pragma unroll
+        for (int i = 0; i < NW; ++i) out.w[i] = st[i];
+    }
+
     __device__ static void prf_idx(Node& out, const Key& k, uint64_t idx) {
         uint32_t in[8] = {0, 0, 0, 0, 0, 0, (uint32_t)(idx >> 32), (uint32_t)idx};
         prf8(out, k.ms_prfk, k.prfk, in);
@@ -485,6 +588,64 @@ struct Sha512Fam {
 #pragma unroll
         for (int i = 0; i < 8; ++i) out.w[i] = st[i];
     }
+    // ---- split tweakable hashes (see Sha256Fam): pre_* leaves the state
+    // after toByte(0|1, 64) || key and the masks; fin_f is one compression,
+    // fin_h two.
+    struct FPre { uint64_t s[8]; Node mask; };
+    struct HPre { uint64_t s[8]; Node m0, m1; };
+    __device__ static void pre_f(FPre& p, const Key& k, uint32_t (&a)[8]) {
+        Node key;
+        uint64_t ad[4];
+        a[7] = 0;
+        adrs64(ad, a);
+        prf4(key, k.ms_pub, ad);
+        a[7] = 1;
+        adrs64(ad, a);
+        prf4(p.mask, k.ms_pub, ad);
+        const uint64_t z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        sha512_iv(p.s);
+        absorb_block(p.s, z, key.w);
+    }
+    __device__ static void fin_f(Node& out, const FPre& p, const Node& x) {
+        uint64_t st[8], xm[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) { st[i] = p.s[i]; xm[i] = x.w[i] ^ p.mask.w[i]; }
+        const uint64_t pad[8] = {0x8000000000000000ULL, 0, 0, 0, 0, 0, 0, 192 * 8};
+        absorb_block(st, xm, pad);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) out.w[i] = st[i];
+    }
+    __device__ static void pre_h(HPre& p, const Key& k, uint32_t (&a)[8]) {
+        Node key;
+        uint64_t ad[4];
+        a[7] = 0;
+        adrs64(ad, a);
+        prf4(key, k.ms_pub, ad);
+        a[7] = 1;
+        adrs64(ad, a);
+        prf4(p.m0, k.ms_pub, ad);
+        a[7] = 2;
+        adrs64(ad, a);
+        prf4(p.m1, k.ms_pub, ad);
+        const uint64_t pw[8] = {0, 0, 0, 0, 0, 0, 0, (uint64_t)PAD_H};
+        sha512_iv(p.s);
+        absorb_block(p.s, pw, key.w);
+    }
+    __device__ static void fin_h(Node& out, const HPre& p, const Node& l, const Node& r) {
+        uint64_t st[8], lm[8], rm[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            st[i] = p.s[i];
+            lm[i] = l.w[i] ^ p.m0.w[i];
+            rm[i] = r.w[i] ^ p.m1.w[i];
+        }
+        absorb_block(st, lm, rm);
+        const uint64_t p0[8] = {0x8000000000000000ULL, 0, 0, 0, 0, 0, 0, 0};
+        const uint64_t p1[8] = {0, 0, 0, 0, 0, 0, 0, 256 * 8};
+        absorb_block(st, p0, p1);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) out.w[i] = st[i];
+    }
     __device__ static void prf_idx(Node& out, const Key& k, uint64_t idx) {
         uint64_t in[4] = {0, 0, 0, idx};
         prf4(out, k.ms_prfk, in);
@@ -655,6 +816,50 @@ struct ShakeFam {
             m[PADW + i] = key.w[i];
             m[PADW + NW + i] = l.w[i] ^ m0.w[i];
             m[PADW + 2 * NW + i] = r.w[i] ^ m1.w[i];
+        }
+        shake_words(out, m);
+    }
+    // ---- split tweakable hashes (see Sha256Fam): the sponge absorbs key and
+    // masked input in one block, so pre_* keeps key and masks and fin_* is
+    // the outer permutation.
+    struct FPre { Node key, mask; };
+    struct HPre { Node key, m0, m1; };
+    __device__ static void pre_f(FPre& p, const Key& k, uint32_t (&a)[8]) {
+        uint32_t al[8];
+        a[7] = 0;
+        adrs_le(al, a);
+        prf8(p.key, k.pub, al);
+        a[7] = 1;
+        adrs_le(al, a);
+        prf8(p.mask, k.pub, al);
+    }
+    __device__ static void fin_f(Node& out, const FPre& p, const Node& x) {
+        uint32_t m[PADW + 2 * NW];
+        pad_words(m, PAD_F);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) { m[PADW + i] = p.key.w[i]; m[PADW + NW + i] = x.w[i] ^ p.mask.w[i]; }
+        shake_words(out, m);
+    }
+    __device__ static void pre_h(HPre& p, const Key& k, uint32_t (&a)[8]) {
+        uint32_t al[8];
+        a[7] = 0;
+        adrs_le(al, a);
+        prf8(p.key, k.pub, al);
+        a[7] = 1;
+        adrs_le(al, a);
+        prf8(p.m0, k.pub, al);
+        a[7] = 2;
+        adrs_le(al, a);
+        prf8(p.m1, k.pub, al);
+    }
+    __device__ static void fin_h(Node& out, const HPre& p, const Node& l, const Node& r) {
+        uint32_t m[PADW + 3 * NW];
+        pad_words(m, PAD_H);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            m[PADW + i] = p.key.w[i];
+            m[PADW + NW + i] = l.w[i] ^ p.m0.w[i];
+            m[PADW + 2 * NW + i] = r.w[i] ^ p.m1.w[i];
         }
         shake_words(out, m);
     }

@@ -49,6 +49,11 @@ static FamOps* ops_of(int fam) {
     fatal("unsupported hash family");
 }
 
+// Up to this many leaves / WOTS signatures / verify layers per launch go
+// through the two-launch latency kernels (kernels_lat.h); larger batches fill
+// the GPU with the one-launch kernels.
+constexpr size_t kLatMax = 1024;
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
@@ -81,7 +86,7 @@ struct Engine::Impl {
     BdsBufs* bds = nullptr;             // buffers of bds_sign_batch
     const void* kp() const { return kp_override ? kp_override : keys.p; }
     Engine::Batch* spare[2] = {nullptr, nullptr};  // reusable verify / sign batches
-    DevBuf keys, seeds, desc, ends, tmp, nodes, a, b, c, d, idx;
+    DevBuf keys, seeds, desc, ends, tmp, nodes, a, b, c, d, idx, lat;
     int key_fam = -1;
     int nkeys = 0;
     void sync() {
@@ -286,7 +291,11 @@ void Engine::leaves(int fam, int nruns, const LeafRun* runs, unsigned count, uin
     DLeaf* dd = I->desc.need<DLeaf>(sizeof(DLeaf) * nl);
     uint8_t* dout = I->b.need(nl * n);
     I->h2d(dd, d.data(), sizeof(DLeaf) * nl);
-    o->leaf_wg(I->L, I->kp(), dd, (uint32_t)nl, false, nullptr, nullptr, nullptr, 0, dout);
+    if (nl <= kLatMax)
+        o->lat(I->L, I->kp(), LAT_LEAF, dd, (uint32_t)nl, nullptr, nullptr, nullptr, 0,
+               I->lat.need(nl * o->lat_bytes(0)), dout);
+    else
+        o->leaf_wg(I->L, I->kp(), dd, (uint32_t)nl, false, nullptr, nullptr, nullptr, 0, dout);
     I->d2h(out, dout, nl * n);
     I->sync();
 }
@@ -310,7 +319,11 @@ void Engine::verify_layer(int fam, unsigned th, int nreq, const RootReq* reqs, c
     I->h2d(dm, msgs, (size_t)nreq * n);
     I->h2d(ds, sigs, (size_t)nreq * len * n);
     I->h2d(da, auth, (size_t)nreq * th * n);
-    o->leaf_wg(I->L, I->kp(), dd, (uint32_t)nreq, true, dm, ds, da, th, dout);
+    if (nreq <= (int)kLatMax)
+        o->lat(I->L, I->kp(), LAT_VERIFY, dd, (uint32_t)nreq, dm, ds, da, th,
+               I->lat.need((size_t)nreq * o->lat_bytes(th)), dout);
+    else
+        o->leaf_wg(I->L, I->kp(), dd, (uint32_t)nreq, true, dm, ds, da, th, dout);
     I->d2h(roots, dout, (size_t)nreq * n);
     I->sync();
 }
@@ -329,6 +342,20 @@ void Engine::wots(int fam, int nreq, const WotsReq* reqs, const uint8_t* msgs, c
     uint8_t* dm = I->a.need((size_t)nreq * n);
     uint8_t* din = I->c.need((size_t)nreq * len * n);
     uint8_t* dout = I->b.need((size_t)nreq * len * n);
+    bool all_sign = nreq <= (int)kLatMax;
+    for (int i = 0; i < nreq && all_sign; ++i) all_sign = reqs[i].mode == MODE_SIGN;
+    if (all_sign) {
+        std::vector<DLeaf> dl(nreq);
+        for (int i = 0; i < nreq; ++i) dl[i] = {d[i].key, d[i].layer, d[i].tree_hi, d[i].tree_lo, d[i].ots, 0, 0, 0};
+        DLeaf* ddl = I->desc.need<DLeaf>(sizeof(DLeaf) * nreq);
+        I->h2d(ddl, dl.data(), sizeof(DLeaf) * nreq);
+        I->h2d(dm, msgs, (size_t)nreq * n);
+        o->lat(I->L, I->kp(), LAT_SIGN, ddl, (uint32_t)nreq, dm, nullptr, nullptr, 0,
+               I->lat.need((size_t)nreq * o->lat_bytes(0)), dout);
+        I->d2h(out, dout, (size_t)nreq * len * n);
+        I->sync();
+        return;
+    }
     I->h2d(dd, d.data(), sizeof(DWots) * d.size());
     I->h2d(dm, msgs, (size_t)nreq * n);
     if (in) I->h2d(din, in, (size_t)nreq * len * n);
@@ -655,10 +682,16 @@ void Engine::run(Batch* b) {
     } else {
         for (unsigned L = 0; L < d; ++L) {
             const uint8_t* msg_l = L == 0 ? (const uint8_t*)b->dig.p : (const uint8_t*)b->roots.p + (size_t)(L - 1) * count * n;
-            o->leaf_wg(I->L, dk, (const DLeaf*)b->vdesc.p + (size_t)L * count, (uint32_t)count, true, msg_l,
+            if (count <= (int)kLatMax)
+                o->lat(I->L, dk, LAT_VERIFY, (const DLeaf*)b->vdesc.p + (size_t)L * count, (uint32_t)count, msg_l,
                        (const uint8_t*)b->wots.p + (size_t)L * count * len * n,
                        (const uint8_t*)b->auth.p + (size_t)L * count * th * n, th,
-                       (uint8_t*)b->roots.p + (size_t)L * count * n);
+                       I->lat.need((size_t)count * o->lat_bytes(th)), (uint8_t*)b->roots.p + (size_t)L * count * n);
+            else
+                o->leaf_wg(I->L, dk, (const DLeaf*)b->vdesc.p + (size_t)L * count, (uint32_t)count, true, msg_l,
+                           (const uint8_t*)b->wots.p + (size_t)L * count * len * n,
+                           (const uint8_t*)b->auth.p + (size_t)L * count * th * n, th,
+                           (uint8_t*)b->roots.p + (size_t)L * count * n);
         }
         I->L.run("compare", k_compare, dim3(cdiv(count, WG)), dim3(WG),
                  (const uint8_t*)b->roots.p + (size_t)(d - 1) * count * n, (const uint8_t*)b->pkroot.p,
@@ -716,7 +749,8 @@ void Engine::free_batch(Batch* b) {
 namespace xm {
 
 struct BdsBufs {
-    DevBuf jobs, leaves_desc, leaf_arena, msg_desc, blob, R, dig, wdesc, wots0, sk_arena, tails, scratch;
+    DevBuf jobs, leaves_desc, leaf_arena, msg_desc, blob, R, dig, wdesc, wots0, sk_arena, tails, scratch, lat_leaf,
+        lat_wots;
 };
 
 void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
@@ -773,7 +807,7 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
     }
     total += 8;
     std::vector<uint8_t> hb(total, 0);
-    std::vector<DWots> wd(count);
+    std::vector<DLeaf> wd(count);
     for (int i = 0; i < count; ++i) {
         uint8_t* p = &hb[md[i].off];
         p[pad - 1] = 2;
@@ -787,7 +821,7 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
         const unsigned th = jobs[i].shape.th;
         const uint64_t tree = jobs[i].idx >> th;
         wd[i] = {jobs[i].key, 0, (uint32_t)(tree >> 32), (uint32_t)tree,
-                 (uint32_t)(jobs[i].idx & ((1ull << th) - 1)), MODE_SIGN, 0, 0};
+                 (uint32_t)(jobs[i].idx & ((1ull << th) - 1)), 0, 0, 0};
     }
     // stage everything, then: leaves on the main stream while R / H_msg and
     // the layer-0 WOTS signatures run on the aux stream; the state update
@@ -796,13 +830,13 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
     DLeaf* dld = B.leaves_desc.need<DLeaf>(sizeof(DLeaf) * std::max<size_t>(1, ld.size()));
     DMsg* dmd = B.msg_desc.need<DMsg>(sizeof(DMsg) * count);
     uint8_t* db = B.blob.need(total);
-    DWots* dw = B.wdesc.need<DWots>(sizeof(DWots) * count);
+    DLeaf* dw = B.wdesc.need<DLeaf>(sizeof(DLeaf) * count);
     uint8_t* ska = B.sk_arena.need(sk_total);
     DBds* djobs = B.jobs.need<DBds>(sizeof(DBds) * count);
     I->h2d(dld, ld.data(), sizeof(DLeaf) * ld.size());
     I->h2d(dmd, md.data(), sizeof(DMsg) * count);
     I->h2d(db, hb.data(), total);
-    I->h2d(dw, wd.data(), sizeof(DWots) * count);
+    I->h2d(dw, wd.data(), sizeof(DLeaf) * count);
     for (int i = 0; i < count; ++i) I->h2d(ska + jd[i].sk_off, jobs[i].sk, jobs[i].shape.sk_bytes);
     I->h2d(djobs, jd.data(), sizeof(DBds) * count);
     hipEvent_t staged = I->L.ev(), front = I->L.ev();
@@ -810,10 +844,15 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
     HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
     I->L.s = I->L.s_aux;
     o->hmsg(I->L, dk, dmd, (uint32_t)count, pad, db, B.R.need((size_t)count * n), B.dig.need((size_t)count * n));
-    o->wots(I->L, dk, dw, (uint32_t)count, (const uint8_t*)B.dig.p, nullptr, B.wots0.need((size_t)count * len * n));
+    o->lat(I->L, dk, LAT_SIGN, dw, (uint32_t)count, (const uint8_t*)B.dig.p, nullptr, nullptr, 0,
+           B.lat_wots.need((size_t)count * o->lat_bytes(0)), B.wots0.need((size_t)count * len * n));
     HIPCK(hipEventRecord(front, I->L.s_aux));
     I->L.s = I->L.s_main;
-    if (!ld.empty()) o->leaf_wg(I->L, dk, dld, (uint32_t)ld.size(), false, nullptr, nullptr, nullptr, 0, leaf_arena);
+    if (!ld.empty() && ld.size() <= kLatMax)
+        o->lat(I->L, dk, LAT_LEAF, dld, (uint32_t)ld.size(), nullptr, nullptr, nullptr, 0,
+               B.lat_leaf.need(ld.size() * o->lat_bytes(0)), leaf_arena);
+    else if (!ld.empty())
+        o->leaf_wg(I->L, dk, dld, (uint32_t)ld.size(), false, nullptr, nullptr, nullptr, 0, leaf_arena);
     HIPCK(hipStreamWaitEvent(I->L.s_main, front, 0));
     o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, B.tails.need(tail_total),
                   B.scratch.need((size_t)count * 4 * n));

@@ -1,0 +1,178 @@
+// kernels_lat.h -- latency-shaped leaf, WOTS-signature and verify-layer
+// kernels (gen_leaf_wots, wots_sign, and one layer of sign_open:
+// wots_pk_from_sig + l_tree + compute_root; xmss_commons.c:16-113,
+// wots.c:117-175).
+//
+// A single leaf is a chain of dependent hashes: 15 thash_f per WOTS chain,
+// then the L-tree heights, then (verify) the auth-path walk.  But every PRF
+// key and mask, and for SHA-2 the first block of every F / H outer hash,
+// depends only on the address, never on a chain value.  So each request runs
+// in two launches:
+//   k_lat_pre  -- wide: one lane per (chain, step), per L-tree merge and per
+//                 auth level computes F::pre_f / F::pre_h (and the chain
+//                 starts, PRF_keygen) into a per-request scratch area;
+//   k_lat_fin  -- one workgroup per request: the chains run F::fin_f only
+//                 (one compression per step for SHA-2 n = 32), the L-tree
+//                 and the auth walk F::fin_h (two).
+// The hash depth of a leaf drops from about 100 compressions to about 30;
+// the work is the same.  Batches of thousands of requests use the one-launch
+// kernels of kernels.h instead, where the GPU is full either way.
+#pragma once
+#include "kernels.h"
+
+namespace xm {
+
+constexpr uint32_t LAT_LEAF = 0, LAT_SIGN = 1, LAT_VERIFY = 2;
+
+// Per-request scratch: chain starts [LEN], F pre-images [LEN][15], H
+// pre-images [LEN - 1 L-tree merges, in height order][th auth levels].
+template <class F>
+struct LatLayout {
+    using FPre = typename F::FPre;
+    using HPre = typename F::HPre;
+    static constexpr size_t align(size_t x) { return (x + 255) & ~(size_t)255; }
+    static constexpr size_t starts = 0;
+    static constexpr size_t fpre = align(sizeof(typename F::Node) * F::LEN);
+    static constexpr size_t hpre = fpre + align(sizeof(FPre) * F::LEN * 15);
+    static size_t bytes(uint32_t th) { return align(hpre + sizeof(HPre) * (F::LEN - 1 + th)); }
+};
+
+// L-tree merge t (0 .. LEN-2, height-major) -> (height, index).
+template <class F>
+__device__ __forceinline__ void ltree_merge(uint32_t t, uint32_t& height, uint32_t& index) {
+    uint32_t l = F::LEN, off = 0, h = 0;
+    while (t >= off + (l >> 1)) {
+        off += l >> 1;
+        l = (l + 1) >> 1;
+        ++h;
+    }
+    height = h;
+    index = t - off;
+}
+
+template <class F, uint32_t MODE>
+__global__ __launch_bounds__(WG) void k_lat_pre(const typename F::Key* __restrict__ keys,
+                                                 const DLeaf* __restrict__ reqs,
+                                                 const uint8_t* __restrict__ msgs, uint32_t th,
+                                                 uint8_t* __restrict__ scratch, size_t stride) {
+    using Lay = LatLayout<F>;
+    const uint32_t r = blockIdx.y;
+    const uint32_t t = blockIdx.x * WG + threadIdx.x;
+    constexpr uint32_t LEN = F::LEN;
+    const uint32_t nf = LEN * 15, nh = (MODE == LAT_SIGN) ? 0 : LEN - 1 + (MODE == LAT_VERIFY ? th : 0);
+    if (t >= LEN + nf + nh) return;
+    const DLeaf q = reqs[r];
+    const typename F::Key& k = keys[q.key];
+    uint8_t* S = scratch + (size_t)r * stride;
+    if (t < LEN) {  // chain start: PRF_keygen (wots.c:14-29)
+        if (MODE == LAT_VERIFY) return;
+        uint32_t a[8] = {q.layer, q.tree_hi, q.tree_lo, ADDR_OTS, q.leaf, t, 0, 0};
+        typename F::Node x;
+        F::prf_keygen(x, k, a);
+        reinterpret_cast<typename F::Node*>(S + Lay::starts)[t] = x;
+        return;
+    }
+    if (t < LEN + nf) {  // chain c, step s
+        const uint32_t c = (t - LEN) / 15, s = (t - LEN) % 15;
+        if (MODE != LAT_LEAF) {
+            const uint32_t len = chain_len<F>(msgs + (size_t)r * F::kN, c);
+            if (MODE == LAT_VERIFY ? s < len : s >= len) return;
+        }
+        uint32_t a[8] = {q.layer, q.tree_hi, q.tree_lo, ADDR_OTS, q.leaf, c, s, 0};
+        typename F::FPre p;
+        F::pre_f(p, k, a);
+        reinterpret_cast<typename F::FPre*>(S + Lay::fpre)[c * 15 + s] = p;
+        return;
+    }
+    const uint32_t u = t - LEN - nf;
+    uint32_t a[8];
+    if (u < LEN - 1) {  // L-tree merge (xmss_commons.c:16-49)
+        uint32_t h, i;
+        ltree_merge<F>(u, h, i);
+        const uint32_t b[8] = {q.layer, q.tree_hi, q.tree_lo, ADDR_LTREE, q.leaf, h, i, 0};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = b[j];
+    } else {  // auth level (xmss_commons.c:54-96)
+        const uint32_t i = u - (LEN - 1);
+        const uint32_t b[8] = {q.layer, q.tree_hi, q.tree_lo, ADDR_HASHTREE, 0, i, q.leaf >> (i + 1), 0};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = b[j];
+    }
+    typename F::HPre p;
+    F::pre_h(p, k, a);
+    reinterpret_cast<typename F::HPre*>(S + Lay::hpre)[u] = p;
+}
+
+// One workgroup per request, ceil(LEN/64) waves.
+//   LAT_LEAF:   out[r] = leaf (n bytes)
+//   LAT_SIGN:   out[r] = the WOTS signature (LEN x n bytes) of msgs[r]
+//   LAT_VERIFY: out[r] = the root implied by sigs[r], msgs[r] and auth[r]
+template <class F, uint32_t MODE>
+__global__ __launch_bounds__(192) void k_lat_fin(const DLeaf* __restrict__ reqs, const uint8_t* __restrict__ msgs,
+                                                  const uint8_t* __restrict__ sigs,
+                                                  const uint8_t* __restrict__ auth, uint32_t th,
+                                                  const uint8_t* __restrict__ scratch, size_t stride,
+                                                  uint8_t* __restrict__ out) {
+    using Lay = LatLayout<F>;
+    using FPre = typename F::FPre;
+    using HPre = typename F::HPre;
+    __shared__ typename F::Node pk[F::LEN];
+    const uint32_t r = blockIdx.x;
+    const uint32_t c = threadIdx.x;
+    const uint8_t* S = scratch + (size_t)r * stride;
+    const HPre* hp = reinterpret_cast<const HPre*>(S + Lay::hpre);
+    if (c < (uint32_t)F::LEN) {
+        typename F::Node x;
+        uint32_t s0 = 0, s1 = 15;
+        if (MODE == LAT_VERIFY) {
+            s0 = chain_len<F>(msgs + (size_t)r * F::kN, c);
+            F::load(x, sigs + ((size_t)r * F::LEN + c) * F::kN);
+        } else {
+            if (MODE == LAT_SIGN) s1 = chain_len<F>(msgs + (size_t)r * F::kN, c);
+            x = reinterpret_cast<const typename F::Node*>(S + Lay::starts)[c];
+        }
+        const FPre* fp = reinterpret_cast<const FPre*>(S + Lay::fpre) + c * 15;
+        // the next step's pre-image is loaded while this step hashes
+        FPre cur = fp[s0 < 15 ? s0 : 14];
+        for (uint32_t s = s0; s < s1; ++s) {
+            const FPre nxt = fp[s + 1 < 15 ? s + 1 : 14];
+            F::fin_f(x, cur, x);
+            cur = nxt;
+        }
+        if (MODE == LAT_SIGN) {
+            F::store(x, out + ((size_t)r * F::LEN + c) * F::kN);
+            return;
+        }
+        pk[c] = x;
+    }
+    if (MODE == LAT_SIGN) return;
+    __syncthreads();
+    uint32_t l = F::LEN, off = 0;
+    while (l > 1) {
+        const uint32_t parents = l >> 1;
+        typename F::Node o;
+        if (c < parents) F::fin_h(o, hp[off + c], pk[2 * c], pk[2 * c + 1]);
+        __syncthreads();
+        if (c < parents) pk[c] = o;
+        if ((l & 1) && c == 0) pk[l >> 1] = pk[l - 1];
+        __syncthreads();
+        off += parents;
+        l = (l + 1) >> 1;
+    }
+    if (c != 0) return;
+    typename F::Node node = pk[0];
+    if (MODE == LAT_VERIFY) {
+        uint32_t idx = reqs[r].leaf;
+        const uint8_t* ap = auth + (size_t)r * th * F::kN;
+        for (uint32_t i = 0; i < th; ++i) {
+            typename F::Node sib;
+            F::load(sib, ap + (size_t)i * F::kN);
+            if (idx & 1) F::fin_h(node, hp[F::LEN - 1 + i], sib, node);
+            else F::fin_h(node, hp[F::LEN - 1 + i], node, sib);
+            idx >>= 1;
+        }
+    }
+    F::store(node, out + (size_t)r * F::kN);
+}
+
+}  // namespace xm

@@ -3,7 +3,9 @@ state in sk and every signature must match the reference's
 xmss_core_fast.c byte for byte, over a key's whole life."""
 import pytest
 
-from common import dg, golden, product
+import random
+
+from common import dg, golden, oracle, product
 from ffi import Params
 
 pytestmark = pytest.mark.gpu
@@ -48,3 +50,36 @@ def test_bds_xmssmt_sha2_20_4_256_1100_signs():
 def test_bds_xmss_sha2_16_256_keygen_and_signs():
     """XMSS-SHA2_16_256 (65,536 leaves): the BDS keypair and 16 signatures."""
     _replay(product(1), golden("bds_xmss_sha2_16_256.json"), 0)
+
+
+MT_FAMILIES = ["XMSSMT-SHA2_20/4_192", "XMSSMT-SHA2_20/4_512", "XMSSMT-SHAKE_20/4_256",
+               "XMSSMT-SHAKE_20/4_512", "XMSSMT-SHAKE256_20/4_256", "XMSSMT-SHAKE256_20/4_192"]
+
+
+@pytest.mark.parametrize("name", MT_FAMILIES)
+def test_bds_every_hash_family_against_oracle(name):
+    """BDS keypair and 40 signatures of an XMSSMT-*_20/4 set of every other
+    hash family (SHA-256 n = 24, SHA-512, SHAKE128, SHAKE256 n = 24/32/64):
+    the signatures cross the layer-0 subtree switch at index 32, messages are
+    32 B - 2 KB; sk and signed message equal the oracle's BDS core after
+    every signature."""
+    lib, orc = product(1), oracle()
+    oid = orc.str_to_oid(name)
+    p = lib.params(oid, 1)
+    seed = 0x5EED00 + oid
+    lib.lib.xmss_mi355x_rng_seed(seed)
+    orc.seed(seed)
+    rc, pk, sk = lib.keypair(oid, 1, sk_init=bytes(4 + p.sk_bytes))
+    rc2, pk2, sk2 = orc.keypair(oid, 1, 1, sk_init=bytes(4 + p.sk_bytes))
+    assert rc == rc2 == 0
+    assert pk == pk2 and sk == sk2
+    rng = random.Random(oid)
+    for i in range(40):
+        m = bytes(rng.getrandbits(8) for _ in range(int(32 * 64 ** rng.random())))
+        rc, sm, sk = lib.sign(sk, m, 1)
+        rc2, sm2, sk2 = orc.sign(sk2, m, 1, 1)
+        assert rc == rc2 == 0, i
+        assert sm == sm2, "sign %d sm" % i
+        assert sk == sk2, "sign %d sk" % i
+    vr, mm = lib.sign_open(sm, pk, 1)
+    assert vr == 0 and mm == m
