@@ -42,10 +42,12 @@ struct BdsOps {
                            const unsigned char* m, uint64_t mlen) = 0;
 };
 
-// Backend that computes nothing and records which leaves were asked for.
+// Backend that computes nothing and records which leaves and which tree
+// merges (their ADRS) were asked for, in order.
 struct RecordingOps final : BdsOps {
     unsigned n;
     std::vector<LeafId> leaves;
+    std::vector<uint32_t> merges;  // 8 ADRS words per thash_h call
     explicit RecordingOps(unsigned n_) : n(n_) {}
     void bind_key(const unsigned char*, const unsigned char*, const unsigned char*) override {}
     void leaf(unsigned char* out, const LeafId& id) override;
@@ -65,9 +67,14 @@ int bds_sign(const xmss_params* p, unsigned char* sk, unsigned char* sm, unsigne
 std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* sk, unsigned long long mlen,
                                     bool in_order = false);
 
+// The merges (thash_h ADRS, 8 words each) bds_sign would hash for this sk, in
+// order, beside the leaves in order (a dry run on a copy).
+void bds_plan(const xmss_params* p, const unsigned char* sk, unsigned long long mlen, std::vector<LeafId>& leaves,
+              std::vector<uint32_t>& merges);
+
 // A signature whose traversal runs on the device (Engine::bds_sign_batch).
 struct BdsDeviceSign {
-    std::vector<uint32_t> layer, leaf;
+    std::vector<uint32_t> layer, leaf, merges;
     std::vector<uint64_t> tree;
     std::vector<unsigned char> seeds;  // SK_SEED, SK_PRF, PUB_SEED, root
     unsigned long long idx = 0;

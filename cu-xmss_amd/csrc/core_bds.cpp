@@ -331,7 +331,10 @@ void RecordingOps::leaf(unsigned char* out, const LeafId& id) {
     leaves.push_back(id);
     memset(out, 0, n);
 }
-void RecordingOps::thash_h(unsigned char* out, const unsigned char*, const uint32_t*) { memset(out, 0, n); }
+void RecordingOps::thash_h(unsigned char* out, const unsigned char*, const uint32_t* adrs) {
+    merges.insert(merges.end(), adrs, adrs + 8);
+    memset(out, 0, n);
+}
 void RecordingOps::wots_sign(unsigned char* sig, const unsigned char*, uint32_t, uint64_t, uint32_t) {
     memset(sig, 0, (size_t)(2 * n + 3) * n);
 }
@@ -479,13 +482,25 @@ std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* s
     return v;
 }
 
+void bds_plan(const xmss_params* p, const unsigned char* sk, unsigned long long mlen, std::vector<LeafId>& leaves,
+              std::vector<uint32_t>& merges) {
+    std::vector<unsigned char> skc(sk, sk + p->sk_bytes);
+    std::vector<unsigned char> smc(p->sig_bytes + mlen + 1), mc(mlen + 1, 0);
+    unsigned long long l = 0;
+    RecordingOps rec(p->n);
+    bds_sign(p, skc.data(), smc.data(), &l, mc.data(), mlen, rec);
+    leaves.swap(rec.leaves);
+    merges.swap(rec.merges);
+}
+
 bool bds_device_prepare(const xmss_params* p, unsigned char* sk, unsigned char* sm, const unsigned char* m,
                         unsigned long long mlen, BdsDeviceSign& ds) {
     const unsigned n = p->n, ib = p->index_bytes;
     unsigned long long idx = 0;
     for (unsigned i = 0; i < ib; ++i) idx |= ((unsigned long long)sk[i]) << 8 * (ib - 1 - i);
     if (idx >= (1ULL << p->full_height) - 1) return false;  // wipe / exhausted: host path
-    const std::vector<LeafId> plan = bds_plan_leaves(p, sk, mlen, true);
+    std::vector<LeafId> plan;
+    bds_plan(p, sk, mlen, plan, ds.merges);
     ds.layer.resize(plan.size());
     ds.tree.resize(plan.size());
     ds.leaf.resize(plan.size());
@@ -516,6 +531,8 @@ bool bds_device_prepare(const xmss_params* p, unsigned char* sk, unsigned char* 
     ds.job.plan_tree = ds.tree.data();
     ds.job.plan_leaf = ds.leaf.data();
     ds.job.nplan = (uint32_t)plan.size();
+    ds.job.plan_merge = ds.merges.data();
+    ds.job.nmerge = (uint32_t)(ds.merges.size() / 8);
     ds.job.R_out = sm + ib;
     ds.job.wots0_out = sm + ib + n;
     ds.job.tail_out = sm + ib + n + p->wots_sig_bytes;
@@ -676,7 +693,9 @@ static int bds_sign_api(const xmss_params* p, unsigned char* sk, unsigned char* 
     if (int rc = prepare_engine(p, &e, &fam)) return rc;
     BdsDeviceSign ds;
     if (bds_device_prepare(p, sk, sm, m, mlen, ds)) {
-        e->bds_sign_batch(fam, 1, &ds.job, false);
+        // the key context stays in the device pool across signatures
+        ds.job.key = (uint32_t)e->pool_slot(fam, {ds.job.sk_seed, ds.job.sk_prf, ds.job.pub_seed});
+        e->bds_sign_batch(fam, 1, &ds.job, true);
         bds_device_finish(p, sm, smlen, mlen, ds);
         return 0;
     }

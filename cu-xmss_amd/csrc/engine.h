@@ -116,6 +116,8 @@ public:
     // request `key` fields of the calls below index the pool of that family
     // (until the next set_keys).
     int pool_add(int fam, const KeySeeds& seeds);
+    // The slot of a key already in the pool (same seed bytes), else pool_add.
+    int pool_slot(int fam, const KeySeeds& seeds);
     void pool_use(int fam);
     size_t pool_key_bytes(int fam) const;
 
@@ -190,6 +192,8 @@ public:
         const uint64_t* plan_tree;
         const uint32_t* plan_leaf;
         uint32_t nplan;
+        const uint32_t* plan_merge;  // planned tree merges: 8 ADRS words each, in hashing order
+        uint32_t nmerge;
         uint8_t* R_out;        // n
         uint8_t* wots0_out;    // wots_sig_bytes
         uint8_t* tail_out;     // sig_bytes - index_bytes - n - wots_sig_bytes

@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <string>
+#include <unordered_map>
 
 #include "engine.h"
 #include "launch.h"
@@ -87,6 +89,7 @@ struct Engine::Impl {
     const void* kp() const { return kp_override ? kp_override : keys.p; }
     Engine::Batch* spare[2] = {nullptr, nullptr};  // reusable verify / sign batches
     DevBuf keys, seeds, desc, ends, tmp, nodes, a, b, c, d, idx, lat;
+    std::unordered_map<std::string, int> slots[F_COUNT];  // pool slot by seed bytes (pool_slot)
     int key_fam = -1;
     int nkeys = 0;
     void sync() {
@@ -748,9 +751,24 @@ void Engine::free_batch(Batch* b) {
 
 namespace xm {
 
+// Grow-only pinned host buffer (hipHostMalloc), for single-copy staging.
+struct PinnedBuf {
+    uint8_t* p = nullptr;
+    size_t cap = 0;
+    uint8_t* need(size_t bytes) {
+        bytes = std::max<size_t>(bytes, 256);
+        if (bytes > cap) {
+            if (p) HIPCK(hipHostFree(p));
+            cap = (std::max(bytes, cap + cap / 2) + 4095) & ~(size_t)4095;
+            HIPCK(hipHostMalloc((void**)&p, cap, hipHostMallocDefault));
+        }
+        return p;
+    }
+};
+
 struct BdsBufs {
-    DevBuf jobs, leaves_desc, leaf_arena, msg_desc, blob, R, dig, wdesc, wots0, sk_arena, tails, scratch, lat_leaf,
-        lat_wots;
+    DevBuf io, leaf_arena, dig, scratch, lat_leaf, lat_wots, merge_arena;
+    PinnedBuf staging;
 };
 
 void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
@@ -772,6 +790,7 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
     const void* dk = I->kp();
     // per-job layout: planned leaves, sk bytes, signature tail
     std::vector<DLeaf> ld;
+    std::vector<DThash> mg;
     std::vector<DBds> jd(count);
     std::vector<size_t> tail_bytes(count);
     size_t sk_total = 0, tail_total = 0, lds = 0;
@@ -783,6 +802,13 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
             ld.push_back({jobs[i].key, jobs[i].plan_layer[k], (uint32_t)(t >> 32), (uint32_t)t, jobs[i].plan_leaf[k],
                           0, 0, 0});
         }
+        jd[i].merge_off = mg.size();
+        for (uint32_t k = 0; k < jobs[i].nmerge; ++k) {
+            DThash t{};
+            t.key = jobs[i].key;
+            for (int w = 0; w < 8; ++w) t.adrs[w] = jobs[i].plan_merge[8 * k + w];
+            mg.push_back(t);
+        }
         tail_bytes[i] = (size_t)s.sig_bytes - s.index_bytes - n - s.wots_sig_bytes;
         jd[i].sk_off = sk_total;
         sk_total += (s.sk_bytes + 7) & ~7ull;
@@ -790,7 +816,7 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
         tail_total += tail_bytes[i];
         jd[i].idx = jobs[i].idx;
         jd[i].key = jobs[i].key;
-        jd[i].p0 = 0;
+        jd[i].nmerge = jobs[i].nmerge;
         BdsP& P = jd[i].P;
         P = {s.th, s.d, s.full_height, s.index_bytes, s.bds_k, s.wots_sig_bytes, s.sig_bytes, 0};
         P.state_bytes = (s.th + 1) * n + 4 + s.th + 1 + s.th * n + (s.th >> 1) * n + (s.th - s.bds_k) * (7 + n) +
@@ -806,15 +832,30 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
         total += (pb + jobs[i].mlen + 7) & ~(size_t)7;
     }
     total += 8;
-    std::vector<uint8_t> hb(total, 0);
-    std::vector<DLeaf> wd(count);
+    // One pinned staging area and one device arena, so that the inputs go
+    // over in one copy and the results come back in one:
+    //   [descriptors, H_msg blob | sk arena (updated in place) | R, WOTS, tails]
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_ld = 0, o_md = o_ld + al(sizeof(DLeaf) * ld.size()), o_blob = o_md + al(sizeof(DMsg) * count),
+                 o_wd = o_blob + al(total), o_jd = o_wd + al(sizeof(DLeaf) * count),
+                 o_mg = o_jd + al(sizeof(DBds) * count), o_sk = o_mg + al(sizeof(DThash) * mg.size()),
+                 o_R = o_sk + al(sk_total), o_W = o_R + al((size_t)count * n),
+                 o_T = o_W + al((size_t)count * len * n), io_bytes = o_T + al(tail_total);
+    uint8_t* h = B.staging.need(io_bytes);
+    uint8_t* dv = B.io.need(io_bytes);
+    memcpy(h + o_ld, ld.data(), sizeof(DLeaf) * ld.size());
+    memcpy(h + o_md, md.data(), sizeof(DMsg) * count);
+    memset(h + o_blob, 0, total);
+    memcpy(h + o_jd, jd.data(), sizeof(DBds) * count);
+    memcpy(h + o_mg, mg.data(), sizeof(DThash) * mg.size());
+    DLeaf* wd = reinterpret_cast<DLeaf*>(h + o_wd);
     for (int i = 0; i < count; ++i) {
-        uint8_t* p = &hb[md[i].off];
+        uint8_t* p = h + o_blob + md[i].off;
         p[pad - 1] = 2;
         memcpy(p + pad + n, jobs[i].pub_root, n);
         uint64_t v = jobs[i].idx;
-        for (int b = 0; b < 8; ++b) {
-            p[pad + 3 * n - 1 - b] = (uint8_t)(v & 0xff);
+        for (int k = 0; k < 8; ++k) {
+            p[pad + 3 * n - 1 - k] = (uint8_t)(v & 0xff);
             v >>= 8;
         }
         if (jobs[i].mlen) memcpy(p + pb, jobs[i].m, jobs[i].mlen);
@@ -822,53 +863,66 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
         const uint64_t tree = jobs[i].idx >> th;
         wd[i] = {jobs[i].key, 0, (uint32_t)(tree >> 32), (uint32_t)tree,
                  (uint32_t)(jobs[i].idx & ((1ull << th) - 1)), 0, 0, 0};
+        memcpy(h + o_sk + jd[i].sk_off, jobs[i].sk, jobs[i].shape.sk_bytes);
     }
-    // stage everything, then: leaves on the main stream while R / H_msg and
-    // the layer-0 WOTS signatures run on the aux stream; the state update
-    // waits for both
+    const DLeaf* dld = reinterpret_cast<const DLeaf*>(dv + o_ld);
+    const DMsg* dmd = reinterpret_cast<const DMsg*>(dv + o_md);
+    uint8_t* db = dv + o_blob;
+    const DLeaf* dw = reinterpret_cast<const DLeaf*>(dv + o_wd);
+    const DBds* djobs = reinterpret_cast<const DBds*>(dv + o_jd);
+    const DThash* dmg = reinterpret_cast<const DThash*>(dv + o_mg);
+    uint8_t* ska = dv + o_sk;
+    uint8_t* dR = dv + o_R;
+    uint8_t* dW = dv + o_W;
+    uint8_t* dT = dv + o_T;
     uint8_t* leaf_arena = B.leaf_arena.need(ld.size() * n);
-    DLeaf* dld = B.leaves_desc.need<DLeaf>(sizeof(DLeaf) * std::max<size_t>(1, ld.size()));
-    DMsg* dmd = B.msg_desc.need<DMsg>(sizeof(DMsg) * count);
-    uint8_t* db = B.blob.need(total);
-    DLeaf* dw = B.wdesc.need<DLeaf>(sizeof(DLeaf) * count);
-    uint8_t* ska = B.sk_arena.need(sk_total);
-    DBds* djobs = B.jobs.need<DBds>(sizeof(DBds) * count);
-    I->h2d(dld, ld.data(), sizeof(DLeaf) * ld.size());
-    I->h2d(dmd, md.data(), sizeof(DMsg) * count);
-    I->h2d(db, hb.data(), total);
-    I->h2d(dw, wd.data(), sizeof(DLeaf) * count);
-    for (int i = 0; i < count; ++i) I->h2d(ska + jd[i].sk_off, jobs[i].sk, jobs[i].shape.sk_bytes);
-    I->h2d(djobs, jd.data(), sizeof(DBds) * count);
+    void* merge_arena = B.merge_arena.need(std::max<size_t>(1, mg.size()) * o->hpre_bytes());
+    I->h2d(dv, h, o_R);
+    // then: the merges' and leaves' work on the main stream while R / H_msg
+    // and the layer-0 WOTS signatures run on the aux stream; the state update
+    // waits for both
     hipEvent_t staged = I->L.ev(), front = I->L.ev();
     HIPCK(hipEventRecord(staged, I->L.s_main));
     HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
     I->L.s = I->L.s_aux;
-    o->hmsg(I->L, dk, dmd, (uint32_t)count, pad, db, B.R.need((size_t)count * n), B.dig.need((size_t)count * n));
+    o->hmsg(I->L, dk, dmd, (uint32_t)count, pad, db, dR, B.dig.need((size_t)count * n));
     o->lat(I->L, dk, LAT_SIGN, dw, (uint32_t)count, (const uint8_t*)B.dig.p, nullptr, nullptr, 0,
-           B.lat_wots.need((size_t)count * o->lat_bytes(0)), B.wots0.need((size_t)count * len * n));
+           B.lat_wots.need((size_t)count * o->lat_bytes(0)), dW);
     HIPCK(hipEventRecord(front, I->L.s_aux));
     I->L.s = I->L.s_main;
+    if (!mg.empty()) o->pre_h(I->L, dk, dmg, (uint32_t)mg.size(), merge_arena);
     if (!ld.empty() && ld.size() <= kLatMax)
         o->lat(I->L, dk, LAT_LEAF, dld, (uint32_t)ld.size(), nullptr, nullptr, nullptr, 0,
                B.lat_leaf.need(ld.size() * o->lat_bytes(0)), leaf_arena);
     else if (!ld.empty())
         o->leaf_wg(I->L, dk, dld, (uint32_t)ld.size(), false, nullptr, nullptr, nullptr, 0, leaf_arena);
     HIPCK(hipStreamWaitEvent(I->L.s_main, front, 0));
-    o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, B.tails.need(tail_total),
+    o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, dT,
                   B.scratch.need((size_t)count * 4 * n));
-    std::vector<uint8_t> R((size_t)count * n), W((size_t)count * len * n), T(tail_total);
-    I->d2h(R.data(), B.R.p, R.size());
-    I->d2h(W.data(), B.wots0.p, W.size());
-    I->d2h(T.data(), B.tails.p, T.size());
-    for (int i = 0; i < count; ++i) I->d2h(jobs[i].sk, ska + jd[i].sk_off, jobs[i].shape.sk_bytes);
+    I->d2h(h + o_sk, ska, io_bytes - o_sk);
     I->sync();
     I->L.pool.push_back(staged);
     I->L.pool.push_back(front);
     for (int i = 0; i < count; ++i) {
-        memcpy(jobs[i].R_out, &R[(size_t)i * n], n);
-        memcpy(jobs[i].wots0_out, &W[(size_t)i * len * n], (size_t)len * n);
-        memcpy(jobs[i].tail_out, &T[jd[i].tail_off], tail_bytes[i]);
+        memcpy(jobs[i].sk, h + o_sk + jd[i].sk_off, jobs[i].shape.sk_bytes);
+        memcpy(jobs[i].R_out, h + o_R + (size_t)i * n, n);
+        memcpy(jobs[i].wots0_out, h + o_W + (size_t)i * len * n, (size_t)len * n);
+        memcpy(jobs[i].tail_out, h + o_T + jd[i].tail_off, tail_bytes[i]);
     }
+}
+
+int Engine::pool_slot(int fam, const KeySeeds& s) {
+    Impl* I = impl_;
+    const unsigned n = fam_n(fam);
+    std::string k(3 * n, '\0');
+    if (s.sk_seed) memcpy(&k[0], s.sk_seed, n);
+    if (s.sk_prf) memcpy(&k[n], s.sk_prf, n);
+    if (s.pub_seed) memcpy(&k[2 * n], s.pub_seed, n);
+    auto it = I->slots[fam].find(k);
+    if (it != I->slots[fam].end()) return it->second;
+    const int slot = pool_add(fam, s);
+    I->slots[fam].emplace(std::move(k), slot);
+    return slot;
 }
 
 }  // namespace xm

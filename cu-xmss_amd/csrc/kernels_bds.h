@@ -30,7 +30,8 @@ struct DBds {
     uint64_t tail_off;    // signature bytes after R and the layer-0 WOTS signature
     uint64_t idx;         // index being signed
     uint32_t key;         // key context
-    uint32_t p0;
+    uint32_t nmerge;      // planned merges of this job
+    uint64_t merge_off;   // first planned merge (F::HPre) in the merge arena
     BdsP P;               // parameter shape of this key
 };
 
@@ -41,13 +42,17 @@ struct BdsDev {
     const BdsP& P;
     const uint8_t* leaves;
     uint32_t cur;
+    const typename F::HPre* merges;  // the planned merges' address-only work, in hashing order
+    uint32_t mcur, nmerge;
     const typename F::Key& key;
     uint32_t tid;
 
     static constexpr uint32_t n = F::kN;
 
-    __device__ BdsDev(uint8_t* lds, uint8_t* skg, const BdsP& P_, const uint8_t* lv, const typename F::Key& k)
-        : sk(lds), sk_glob(skg), P(P_), leaves(lv), cur(0), key(k), tid(threadIdx.x) {}
+    __device__ BdsDev(uint8_t* lds, uint8_t* skg, const BdsP& P_, const uint8_t* lv, const typename F::HPre* mg,
+                      uint32_t nmg, const typename F::Key& k)
+        : sk(lds), sk_glob(skg), P(P_), leaves(lv), cur(0), merges(mg), mcur(0), nmerge(nmg), key(k),
+          tid(threadIdx.x) {}
 
     // ---- layout (xmss_core_fast.c:38-78), offsets within the states region ----
     __device__ size_t st(uint32_t i) const { return (size_t)i * P.state_bytes; }
@@ -104,19 +109,28 @@ struct BdsDev {
             }
         __syncthreads();
     }
-    // dst <- thash_h(in[0..2n)) at ADRS a (thread 0); dst may alias in
+    // dst <- thash_h(in[0..2n)) at ADRS a (thread 0); dst may alias in.  The
+    // merges come in the order the host's dry run recorded them, so the
+    // address-only part (PRF key and masks, first outer block) was computed
+    // in a wide launch before this one (F::pre_h); past the planned count the
+    // whole thash_h runs here.
     __device__ void hash_h(uint8_t* dst, const uint8_t* in, const uint32_t (&a)[8]) {
         if (tid == 0) {
             typename F::Node L, R, o;
-            uint32_t ad[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) ad[i] = a[i];
             // node buffers are not 4-byte aligned in the serialized state: gather bytes
             load_bytes(L, in);
             load_bytes(R, in + n);
-            F::thash_h(o, key, ad, L, R);
+            if (mcur < nmerge) {
+                F::fin_h(o, merges[mcur], L, R);
+            } else {
+                uint32_t ad[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) ad[i] = a[i];
+                F::thash_h(o, key, ad, L, R);
+            }
             store_bytes(o, dst);
         }
+        ++mcur;
         __syncthreads();
     }
     __device__ static void load_bytes(typename F::Node& x, const uint8_t* p) {
@@ -294,6 +308,22 @@ struct BdsDev {
     }
 };
 
+// Address-only part of a list of thash_h calls (F::pre_h), one per lane.
+template <class F>
+__global__ __launch_bounds__(WG) void k_pre_h(const typename F::Key* __restrict__ keys,
+                                               const DThash* __restrict__ reqs, uint32_t n,
+                                               typename F::HPre* __restrict__ out) {
+    const uint32_t r = blockIdx.x * WG + threadIdx.x;
+    if (r >= n) return;
+    const DThash q = reqs[r];
+    uint32_t a[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = q.adrs[i];
+    typename F::HPre p;
+    F::pre_h(p, keys[q.key], a);
+    out[r] = p;
+}
+
 // The state part of one BDS signature per workgroup (everything after R, H_msg
 // and the layer-0 WOTS signature, which run in their own launches): the auth
 // paths and lower-layer WOTS signatures go to the signature tail, then the
@@ -301,12 +331,14 @@ struct BdsDev {
 template <class F>
 __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __restrict__ keys, const DBds* __restrict__ jobs,
                              uint8_t* __restrict__ sk_arena, const uint8_t* __restrict__ leaf_arena,
+                             const typename F::HPre* __restrict__ merge_arena,
                              uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena) {
     extern __shared__ uint8_t lds_states[];
     const DBds job = jobs[blockIdx.x];
     const BdsP& P = job.P;
     uint8_t* skg = sk_arena + job.sk_off;
-    BdsDev<F> B(lds_states, skg, P, leaf_arena + job.leaf_off * F::kN, keys[job.key]);
+    BdsDev<F> B(lds_states, skg, P, leaf_arena + job.leaf_off * F::kN, merge_arena + job.merge_off, job.nmerge,
+                keys[job.key]);
     uint8_t* scratch = scratch_arena + (size_t)blockIdx.x * 4 * F::kN;
     uint8_t* tail = tails + job.tail_off;
     const uint32_t n = F::kN, th = P.th, d = P.d;

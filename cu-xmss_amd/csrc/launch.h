@@ -116,8 +116,11 @@ struct FamOps {
                          uint8_t* out) = 0;
     virtual void hmsg(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad,
                       uint8_t* blob, uint8_t* R, uint8_t* dig) = 0;
+    virtual size_t hpre_bytes() const = 0;
+    virtual void pre_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, void* out) = 0;
     virtual void bds_update(Launcher& L, const void* keys, size_t lds, const DBds* jobs, uint32_t n,
-                            uint8_t* sk_arena, const uint8_t* leaf_arena, uint8_t* tails, uint8_t* scratch) = 0;
+                            uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena, uint8_t* tails,
+                            uint8_t* scratch) = 0;
 };
 
 template <class F>
@@ -188,10 +191,15 @@ struct FamOpsT final : FamOps {
               uint8_t* R, uint8_t* dig) override {
         L.run("hmsg", k_hmsg<F>, dim3(cdiv(n, WG)), dim3(WG), (const K*)keys, reqs, n, pad, blob, R, dig);
     }
+    size_t hpre_bytes() const override { return sizeof(typename F::HPre); }
+    void pre_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, void* out) override {
+        L.run("pre_h", k_pre_h<F>, dim3(cdiv(n, WG)), dim3(WG), (const K*)keys, reqs, n, (typename F::HPre*)out);
+    }
     void bds_update(Launcher& L, const void* keys, size_t lds, const DBds* jobs, uint32_t n,
-                    uint8_t* sk_arena, const uint8_t* leaf_arena, uint8_t* tails, uint8_t* scratch) override {
+                    uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena, uint8_t* tails,
+                    uint8_t* scratch) override {
         L.run_lds("bds_update", k_bds_update<F>, dim3(n), dim3(((F::LEN + 63) / 64) * 64), lds, (const K*)keys,
-                  jobs, sk_arena, leaf_arena, tails, scratch);
+                  jobs, sk_arena, leaf_arena, (const typename F::HPre*)merge_arena, tails, scratch);
     }
 };
 
