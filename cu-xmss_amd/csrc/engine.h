@@ -203,6 +203,21 @@ public:
         } shape;
     };
     using BdsShape = BdsJob::Shape;
+
+    // Verification of signatures of keys in the device pool of `fam`
+    // (pool_slot with PUB_SEED set), any mix of parameter shapes, in one
+    // launch sequence: one staging copy in, H_msg beside the address-only work
+    // of every layer, then one verify-layer launch per XMSS^MT layer, one copy
+    // out.  ok[i] = 1 when signature i is valid.
+    struct VerifyJob {
+        uint32_t key;            // pool slot
+        unsigned th, d, index_bytes;
+        const uint8_t* pk_root;  // n bytes
+        const uint8_t* sig;      // idx || R || d x (WOTS signature || auth path)
+        const uint8_t* m;
+        uint64_t mlen;
+    };
+    void verify_many(int fam, int count, const VerifyJob* jobs, int* ok);
     // Jobs may mix parameter shapes of one hash family.
     void bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool);
 

@@ -74,6 +74,7 @@ struct DevBuf {
 };
 
 struct BdsBufs;
+struct VerifyBufs;
 
 struct KeyPool {
     void* p = nullptr;
@@ -86,6 +87,7 @@ struct Engine::Impl {
     KeyPool pool[F_COUNT];
     const void* kp_override = nullptr;  // pool in use, or null for `keys`
     BdsBufs* bds = nullptr;             // buffers of bds_sign_batch
+    VerifyBufs* ver = nullptr;          // buffers of verify_many
     const void* kp() const { return kp_override ? kp_override : keys.p; }
     Engine::Batch* spare[2] = {nullptr, nullptr};  // reusable verify / sign batches
     DevBuf keys, seeds, desc, ends, tmp, nodes, a, b, c, d, idx, lat;
@@ -311,7 +313,7 @@ void Engine::verify_layer(int fam, unsigned th, int nreq, const RootReq* reqs, c
     std::vector<DLeaf> d(nreq);
     for (int i = 0; i < nreq; ++i) {
         d[i] = {reqs[i].key, reqs[i].layer, (uint32_t)(reqs[i].tree >> 32), (uint32_t)reqs[i].tree,
-                reqs[i].leaf_idx, 0, 0, 0};
+                reqs[i].leaf_idx, th, 0, (uint32_t)(i * th)};
     }
     DLeaf* dd = I->desc.need<DLeaf>(sizeof(DLeaf) * nreq);
     uint8_t* dm = I->a.need((size_t)nreq * n);
@@ -596,7 +598,7 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
             const uint32_t leaf = (uint32_t)(idx & ((1ull << th) - 1));
             idx >>= th;
             const size_t t = (size_t)L * count + i;
-            vd[t] = {(uint32_t)i, L, (uint32_t)(idx >> 32), (uint32_t)idx, leaf, 0, 0, 0};
+            vd[t] = {(uint32_t)i, L, (uint32_t)(idx >> 32), (uint32_t)idx, leaf, th, 0, (uint32_t)(i * th)};
             memcpy(&sigs[t * len * n], p, (size_t)len * n);
             p += (size_t)len * n;
             memcpy(&auth[t * th * n], p, (size_t)th * n);
@@ -908,6 +910,120 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
         memcpy(jobs[i].R_out, h + o_R + (size_t)i * n, n);
         memcpy(jobs[i].wots0_out, h + o_W + (size_t)i * len * n, (size_t)len * n);
         memcpy(jobs[i].tail_out, h + o_T + jd[i].tail_off, tail_bytes[i]);
+    }
+}
+
+struct VerifyBufs {
+    DevBuf io, dig, roots, scratch;
+    PinnedBuf staging;
+};
+
+void Engine::verify_many(int fam, int count, const VerifyJob* jobs, int* ok) {
+    Impl* I = impl_;
+    if (!I->ver) I->ver = new VerifyBufs;
+    VerifyBufs& V = *I->ver;
+    FamOps* o = ops_of(fam);
+    const unsigned n = fam_n(fam), len = fam_len(fam), pad = fam_pad(fam);
+    pool_use(fam);
+    const void* dk = I->kp();
+    // deepest first: the requests that still have a layer L are a prefix
+    std::vector<int> ord(count);
+    for (int i = 0; i < count; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return jobs[a].d > jobs[b].d; });
+    const unsigned dmax = count ? jobs[ord[0]].d : 0;
+    std::vector<uint32_t> cnt(dmax, 0), base(dmax + 1, 0);
+    unsigned thmax = 0;
+    for (int j = 0; j < count; ++j) {
+        for (unsigned L = 0; L < jobs[ord[j]].d; ++L) ++cnt[L];
+        thmax = std::max(thmax, jobs[ord[j]].th);
+    }
+    for (unsigned L = 0; L < dmax; ++L) base[L + 1] = base[L] + cnt[L];
+    const uint32_t items = base[dmax];
+    // auth offsets (in nodes) from each layer's first auth path
+    std::vector<uint32_t> auth_at(dmax + 1, 0);
+    for (unsigned L = 0; L < dmax; ++L) {
+        uint32_t s = 0;
+        for (uint32_t j = 0; j < cnt[L]; ++j) s += jobs[ord[j]].th;
+        auth_at[L + 1] = auth_at[L] + s;
+    }
+    const size_t pb = pad + 3 * n;
+    std::vector<DMsg> md(count);
+    size_t total = 0;
+    for (int j = 0; j < count; ++j) {
+        const VerifyJob& v = jobs[ord[j]];
+        uint64_t idx = 0;
+        for (unsigned b = 0; b < v.index_bytes; ++b) idx = (idx << 8) | v.sig[b];
+        md[j] = {v.key, 0u, idx, (uint64_t)total, (uint32_t)(pb + v.mlen), 0};
+        total += (pb + v.mlen + 7) & ~(size_t)7;
+    }
+    total += 8;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_md = 0, o_blob = o_md + al(sizeof(DMsg) * count), o_it = o_blob + al(total),
+                 o_sig = o_it + al(sizeof(DLeaf) * items), o_auth = o_sig + al((size_t)items * len * n),
+                 io_bytes = o_auth + al((size_t)auth_at[dmax] * n);
+    uint8_t* h = V.staging.need(std::max(io_bytes, (size_t)items * n));
+    uint8_t* dv = V.io.need(io_bytes);
+    memcpy(h + o_md, md.data(), sizeof(DMsg) * count);
+    memset(h + o_blob, 0, total);
+    DLeaf* it = reinterpret_cast<DLeaf*>(h + o_it);
+    std::vector<uint32_t> fill(dmax, 0), afill(dmax, 0);
+    for (int j = 0; j < count; ++j) {
+        const VerifyJob& v = jobs[ord[j]];
+        const uint8_t* R = v.sig + v.index_bytes;
+        uint8_t* p = h + o_blob + md[j].off;
+        p[pad - 1] = 2;  // toByte(2, pad) || R || root || toByte(idx, n) || M (hash.c:107-120)
+        memcpy(p + pad, R, n);
+        memcpy(p + pad + n, v.pk_root, n);
+        uint64_t x = md[j].idx;
+        for (int b = 0; b < 8; ++b) {
+            p[pad + 3 * n - 1 - b] = (uint8_t)(x & 0xff);
+            x >>= 8;
+        }
+        if (v.mlen) memcpy(p + pb, v.m, v.mlen);
+        uint64_t idx = md[j].idx;
+        const uint8_t* s = R + n;
+        for (unsigned L = 0; L < v.d; ++L) {
+            const uint32_t leaf = (uint32_t)(idx & ((1ull << v.th) - 1));
+            idx >>= v.th;
+            const uint32_t t = base[L] + fill[L]++;
+            it[t] = {v.key, L, (uint32_t)(idx >> 32), (uint32_t)idx, leaf, v.th, LAT_ALL_STEPS, afill[L]};
+            memcpy(h + o_sig + (size_t)t * len * n, s, (size_t)len * n);
+            s += (size_t)len * n;
+            memcpy(h + o_auth + (size_t)(auth_at[L] + afill[L]) * n, s, (size_t)v.th * n);
+            afill[L] += v.th;
+            s += (size_t)v.th * n;
+        }
+    }
+    I->h2d(dv, h, io_bytes);
+    const DMsg* dmd = reinterpret_cast<const DMsg*>(dv + o_md);
+    const DLeaf* dit = reinterpret_cast<const DLeaf*>(dv + o_it);
+    uint8_t* roots = V.roots.need((size_t)items * n);
+    uint8_t* dig = V.dig.need((size_t)count * n);
+    uint8_t* scratch = V.scratch.need((size_t)items * o->lat_bytes(thmax));
+    // address-only work of every layer on the aux stream beside H_msg
+    hipEvent_t staged = I->L.ev(), pre = I->L.ev();
+    HIPCK(hipEventRecord(staged, I->L.s_main));
+    HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
+    I->L.s = I->L.s_aux;
+    o->lat_pre(I->L, dk, LAT_VERIFY, dit, items, nullptr, thmax, scratch);
+    HIPCK(hipEventRecord(pre, I->L.s_aux));
+    I->L.s = I->L.s_main;
+    o->hmsg(I->L, dk, dmd, (uint32_t)count, pad, dv + o_blob, nullptr, dig);
+    HIPCK(hipStreamWaitEvent(I->L.s_main, pre, 0));
+    const size_t stride = o->lat_bytes(thmax);
+    for (unsigned L = 0; L < dmax; ++L) {
+        const uint8_t* msgs = L == 0 ? dig : roots + (size_t)base[L - 1] * n;
+        o->lat_fin(I->L, LAT_VERIFY, dit + base[L], cnt[L], msgs, dv + o_sig + (size_t)base[L] * len * n,
+                   dv + o_auth + (size_t)auth_at[L] * n, thmax, scratch + (size_t)base[L] * stride,
+                   roots + (size_t)base[L] * n);
+    }
+    I->d2h(h, roots, (size_t)items * n);
+    I->sync();
+    I->L.pool.push_back(staged);
+    I->L.pool.push_back(pre);
+    for (int j = 0; j < count; ++j) {
+        const VerifyJob& v = jobs[ord[j]];
+        ok[ord[j]] = !memcmp(h + (size_t)(base[v.d - 1] + j) * n, v.pk_root, n);
     }
 }
 

@@ -23,6 +23,12 @@
 namespace xm {
 
 constexpr uint32_t LAT_LEAF = 0, LAT_SIGN = 1, LAT_VERIFY = 2;
+// Request fields beyond the address (DLeaf p0..p2) for LAT_VERIFY: p0 = the
+// tree height (auth levels) of this request, p1 bit 0 = precompute all 15
+// steps of every chain (the message is not known yet when k_lat_pre runs,
+// as for the upper XMSS^MT layers), p2 = the offset of its auth path, in
+// nodes, from the auth pointer of the k_lat_fin launch.
+constexpr uint32_t LAT_ALL_STEPS = 1;
 
 // Per-request scratch: chain starts [LEN], F pre-images [LEN][15], H
 // pre-images [LEN - 1 L-tree merges, in height order][th auth levels].
@@ -53,15 +59,15 @@ __device__ __forceinline__ void ltree_merge(uint32_t t, uint32_t& height, uint32
 template <class F, uint32_t MODE>
 __global__ __launch_bounds__(WG) void k_lat_pre(const typename F::Key* __restrict__ keys,
                                                  const DLeaf* __restrict__ reqs,
-                                                 const uint8_t* __restrict__ msgs, uint32_t th,
+                                                 const uint8_t* __restrict__ msgs,
                                                  uint8_t* __restrict__ scratch, size_t stride) {
     using Lay = LatLayout<F>;
     const uint32_t r = blockIdx.y;
     const uint32_t t = blockIdx.x * WG + threadIdx.x;
     constexpr uint32_t LEN = F::LEN;
-    const uint32_t nf = LEN * 15, nh = (MODE == LAT_SIGN) ? 0 : LEN - 1 + (MODE == LAT_VERIFY ? th : 0);
-    if (t >= LEN + nf + nh) return;
     const DLeaf q = reqs[r];
+    const uint32_t nf = LEN * 15, nh = (MODE == LAT_SIGN) ? 0 : LEN - 1 + (MODE == LAT_VERIFY ? q.p0 : 0);
+    if (t >= LEN + nf + nh) return;
     const typename F::Key& k = keys[q.key];
     uint8_t* S = scratch + (size_t)r * stride;
     if (t < LEN) {  // chain start: PRF_keygen (wots.c:14-29)
@@ -74,7 +80,7 @@ __global__ __launch_bounds__(WG) void k_lat_pre(const typename F::Key* __restric
     }
     if (t < LEN + nf) {  // chain c, step s
         const uint32_t c = (t - LEN) / 15, s = (t - LEN) % 15;
-        if (MODE != LAT_LEAF) {
+        if (MODE == LAT_SIGN || (MODE == LAT_VERIFY && !(q.p1 & LAT_ALL_STEPS))) {
             const uint32_t len = chain_len<F>(msgs + (size_t)r * F::kN, c);
             if (MODE == LAT_VERIFY ? s < len : s >= len) return;
         }
@@ -110,7 +116,7 @@ __global__ __launch_bounds__(WG) void k_lat_pre(const typename F::Key* __restric
 template <class F, uint32_t MODE>
 __global__ __launch_bounds__(192) void k_lat_fin(const DLeaf* __restrict__ reqs, const uint8_t* __restrict__ msgs,
                                                   const uint8_t* __restrict__ sigs,
-                                                  const uint8_t* __restrict__ auth, uint32_t th,
+                                                  const uint8_t* __restrict__ auth,
                                                   const uint8_t* __restrict__ scratch, size_t stride,
                                                   uint8_t* __restrict__ out) {
     using Lay = LatLayout<F>;
@@ -162,9 +168,10 @@ __global__ __launch_bounds__(192) void k_lat_fin(const DLeaf* __restrict__ reqs,
     if (c != 0) return;
     typename F::Node node = pk[0];
     if (MODE == LAT_VERIFY) {
-        uint32_t idx = reqs[r].leaf;
-        const uint8_t* ap = auth + (size_t)r * th * F::kN;
-        for (uint32_t i = 0; i < th; ++i) {
+        const DLeaf q = reqs[r];
+        uint32_t idx = q.leaf;
+        const uint8_t* ap = auth + (size_t)q.p2 * F::kN;
+        for (uint32_t i = 0; i < q.p0; ++i) {
             typename F::Node sib;
             F::load(sib, ap + (size_t)i * F::kN);
             if (idx & 1) F::fin_h(node, hp[F::LEN - 1 + i], sib, node);

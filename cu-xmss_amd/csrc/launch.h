@@ -107,11 +107,18 @@ struct FamOps {
                          const uint8_t* msgs, const uint8_t* sigs, const uint8_t* auth, uint32_t th,
                          uint8_t* out) = 0;
     // Latency-shaped leaves / WOTS signatures / verify layers (kernels_lat.h):
-    // scratch must hold n x lat_bytes(th); n <= 65535.
+    // scratch must hold n x lat_bytes(th) with th the largest tree height of
+    // the requests (0 except for verification); n <= 65535.  lat = lat_pre +
+    // lat_fin on the same requests and scratch.
     virtual size_t lat_bytes(uint32_t th) const = 0;
     virtual void lat(Launcher& L, const void* keys, uint32_t mode, const DLeaf* reqs, uint32_t n,
                      const uint8_t* msgs, const uint8_t* sigs, const uint8_t* auth, uint32_t th,
                      uint8_t* scratch, uint8_t* out) = 0;
+    virtual void lat_pre(Launcher& L, const void* keys, uint32_t mode, const DLeaf* reqs, uint32_t n,
+                         const uint8_t* msgs, uint32_t th, uint8_t* scratch) = 0;
+    virtual void lat_fin(Launcher& L, uint32_t mode, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
+                         const uint8_t* sigs, const uint8_t* auth, uint32_t th, const uint8_t* scratch,
+                         uint8_t* out) = 0;
     virtual void thash_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, const uint8_t* in,
                          uint8_t* out) = 0;
     virtual void hmsg(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad,
@@ -166,22 +173,36 @@ struct FamOpsT final : FamOps {
     }
     size_t lat_bytes(uint32_t th) const override { return LatLayout<F>::bytes(th); }
     template <uint32_t MODE>
-    void lat_t(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
-               const uint8_t* sigs, const uint8_t* auth, uint32_t th, uint8_t* scratch, uint8_t* out) {
-        static const char* pre_name[3] = {"lat_pre_leaf", "lat_pre_sign", "lat_pre_verify"};
-        static const char* fin_name[3] = {"lat_leaf", "lat_sign", "lat_verify"};
-        const size_t stride = LatLayout<F>::bytes(th);
+    void lat_pre_t(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
+                   uint32_t th, uint8_t* scratch) {
+        static const char* name[3] = {"lat_pre_leaf", "lat_pre_sign", "lat_pre_verify"};
         const uint32_t items = F::LEN * 17 - 1 + th;  // starts, 15 F per chain, L-tree merges, auth levels
-        L.run(pre_name[MODE], k_lat_pre<F, MODE>, dim3(cdiv(items, WG), n), dim3(WG), (const K*)keys, reqs, msgs,
-              th, scratch, stride);
-        L.run(fin_name[MODE], k_lat_fin<F, MODE>, dim3(n), dim3(((F::LEN + 63) / 64) * 64), reqs, msgs, sigs,
-              auth, th, (const uint8_t*)scratch, stride, out);
+        L.run(name[MODE], k_lat_pre<F, MODE>, dim3(cdiv(items, WG), n), dim3(WG), (const K*)keys, reqs, msgs,
+              scratch, LatLayout<F>::bytes(th));
+    }
+    template <uint32_t MODE>
+    void lat_fin_t(Launcher& L, const DLeaf* reqs, uint32_t n, const uint8_t* msgs, const uint8_t* sigs,
+                   const uint8_t* auth, uint32_t th, const uint8_t* scratch, uint8_t* out) {
+        static const char* name[3] = {"lat_leaf", "lat_sign", "lat_verify"};
+        L.run(name[MODE], k_lat_fin<F, MODE>, dim3(n), dim3(((F::LEN + 63) / 64) * 64), reqs, msgs, sigs, auth,
+              scratch, LatLayout<F>::bytes(th), out);
+    }
+    void lat_pre(Launcher& L, const void* keys, uint32_t mode, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
+                 uint32_t th, uint8_t* scratch) override {
+        if (mode == LAT_LEAF) lat_pre_t<LAT_LEAF>(L, keys, reqs, n, msgs, th, scratch);
+        else if (mode == LAT_SIGN) lat_pre_t<LAT_SIGN>(L, keys, reqs, n, msgs, th, scratch);
+        else lat_pre_t<LAT_VERIFY>(L, keys, reqs, n, msgs, th, scratch);
+    }
+    void lat_fin(Launcher& L, uint32_t mode, const DLeaf* reqs, uint32_t n, const uint8_t* msgs, const uint8_t* sigs,
+                 const uint8_t* auth, uint32_t th, const uint8_t* scratch, uint8_t* out) override {
+        if (mode == LAT_LEAF) lat_fin_t<LAT_LEAF>(L, reqs, n, msgs, sigs, auth, th, scratch, out);
+        else if (mode == LAT_SIGN) lat_fin_t<LAT_SIGN>(L, reqs, n, msgs, sigs, auth, th, scratch, out);
+        else lat_fin_t<LAT_VERIFY>(L, reqs, n, msgs, sigs, auth, th, scratch, out);
     }
     void lat(Launcher& L, const void* keys, uint32_t mode, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
              const uint8_t* sigs, const uint8_t* auth, uint32_t th, uint8_t* scratch, uint8_t* out) override {
-        if (mode == LAT_LEAF) lat_t<LAT_LEAF>(L, keys, reqs, n, msgs, sigs, auth, th, scratch, out);
-        else if (mode == LAT_SIGN) lat_t<LAT_SIGN>(L, keys, reqs, n, msgs, sigs, auth, th, scratch, out);
-        else lat_t<LAT_VERIFY>(L, keys, reqs, n, msgs, sigs, auth, th, scratch, out);
+        lat_pre(L, keys, mode, reqs, n, msgs, th, scratch);
+        lat_fin(L, mode, reqs, n, msgs, sigs, auth, th, scratch, out);
     }
     void thash_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, const uint8_t* in,
                  uint8_t* out) override {

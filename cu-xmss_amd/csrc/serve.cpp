@@ -76,14 +76,26 @@ struct Call {
 
 class Dispatcher {
 public:
-    Dispatcher(Engine* e, Engine* ev) : e_(e), ev_(ev), th_([this] { loop(); }) {}
+    // Two lanes, each with its own thread: verifications on the private
+    // engine `ev` (its own stream), everything else on the shared engine `e`.
+    // A lane launches what it has queued once every running request is
+    // waiting on the device (queued or executing in either lane) or 200 us
+    // after its oldest queued call, so signatures never wait for a
+    // verification group and the other way round.
+    Dispatcher(Engine* e, Engine* ev) : e_(e), ev_(ev) {
+        lanes_[0].engine = e;
+        lanes_[0].shared = true;
+        lanes_[1].engine = ev ? ev : e;
+        lanes_[1].shared = ev == nullptr;
+        for (int i = 0; i < 2; ++i) lanes_[i].th = std::thread([this, i] { loop(lanes_[i]); });
+    }
     ~Dispatcher() {
         {
             std::lock_guard<std::mutex> lk(mu_);
             stop_ = true;
         }
         cv_.notify_all();
-        th_.join();
+        for (auto& l : lanes_) l.th.join();
     }
     void enter() {
         std::lock_guard<std::mutex> lk(mu_);
@@ -97,8 +109,10 @@ public:
     // Blocks until the dispatcher has executed the call.
     void run(Call* c) {
         std::unique_lock<std::mutex> lk(mu_);
-        pending_.push_back(c);
-        if (pending_.size() == 1) first_ = Clock::now();
+        Lane& l = lanes_[c->kind == Call::VERIFY ? 1 : 0];
+        l.pending.push_back(c);
+        if (l.pending.size() == 1) l.first = Clock::now();
+        ++waiting_;
         cv_.notify_all();
         cv_done_.wait(lk, [c] { return c->done; });
     }
@@ -106,39 +120,34 @@ public:
     uint64_t calls() const { return calls_; }
 
 private:
-    void loop() {
+    struct Lane {
+        Engine* engine = nullptr;
+        bool shared = true;  // the shared engine: hold the library-wide engine lock
+        std::vector<Call*> pending;
+        Clock::time_point first;
+        std::thread th;
+    };
+
+    void loop(Lane& l) {
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
-            // every running request waiting on the device, or 200 us since the oldest call
-            cv_.wait_for(lk, std::chrono::microseconds(200), [this] {
-                return stop_ || (!pending_.empty() && (int)pending_.size() >= active_);
+            cv_.wait_for(lk, std::chrono::microseconds(200), [&] {
+                return stop_ || (!l.pending.empty() && waiting_ >= active_);
             });
-            if (stop_ && pending_.empty()) return;
-            if (pending_.empty()) continue;
-            if ((int)pending_.size() < active_ && Clock::now() - first_ < std::chrono::microseconds(200)) continue;
+            if (stop_ && l.pending.empty()) return;
+            if (l.pending.empty()) continue;
+            if (waiting_ < active_ && Clock::now() - l.first < std::chrono::microseconds(200)) continue;
             std::vector<Call*> batch;
-            batch.swap(pending_);
+            batch.swap(l.pending);
             lk.unlock();
-            execute(batch);
+            execute_on(l.engine, batch, l.shared);
             lk.lock();
             for (Call* c : batch) c->done = true;
+            waiting_ -= (int)batch.size();
             calls_ += batch.size();
             cv_done_.notify_all();
+            cv_.notify_all();
         }
-    }
-
-    // Verifications run on their own engine (stream) in a helper thread while
-    // the other kinds run on the shared engine.
-    void execute(std::vector<Call*>& batch) {
-        std::vector<Call*> ver, rest;
-        for (Call* c : batch) (c->kind == Call::VERIFY ? ver : rest).push_back(c);
-        std::thread tv;
-        if (!ver.empty()) {
-            if (ev_) tv = std::thread([this, &ver] { execute_on(ev_, ver, false); });
-            else rest.insert(rest.end(), ver.begin(), ver.end());
-        }
-        execute_on(e_, rest, true);
-        if (tv.joinable()) tv.join();
     }
 
     void execute_on(Engine* e_, std::vector<Call*>& batch, bool shared) {
@@ -212,24 +221,18 @@ private:
                     break;
                 }
                 case Call::VERIFY: {
-                    // one verify batch per parameter set
-                    std::map<std::tuple<unsigned, unsigned, unsigned>, std::vector<Call*>> by;
-                    for (Call* c : cs) by[std::make_tuple(c->p->tree_height, c->p->d, c->p->full_height)].push_back(c);
-                    for (auto& v : by) {
-                        const xmss_params* p = v.second[0]->p;
-                        std::vector<Engine::VerifyItem> it;
-                        for (Call* c : v.second) {
-                            it.push_back({c->pk, bytes_to_ull(c->sm, p->index_bytes), c->sm + p->index_bytes,
-                                          c->sm + p->index_bytes + p->n, c->sm + p->sig_bytes,
-                                          c->smlen - p->sig_bytes});
-                        }
-                        Engine::Batch* b = e_->verify_prepare(fam, p->tree_height, p->d, (int)it.size(), it.data());
-                        e_->run(b);
-                        std::vector<int> ok(it.size());
-                        e_->collect_verify(b, ok.data());
-                        e_->free_batch(b);
-                        for (size_t i = 0; i < v.second.size(); ++i) v.second[i]->ok = ok[i];
+                    // every parameter shape of the family in one launch sequence
+                    // (PUB_SEED contexts in this engine's own pool, cached by seed)
+                    std::vector<Engine::VerifyJob> jobs;
+                    for (Call* c : cs) {
+                        const xmss_params* p = c->p;
+                        const uint32_t slot = (uint32_t)e_->pool_slot(fam, {nullptr, nullptr, c->pk + p->n});
+                        jobs.push_back({slot, p->tree_height, p->d, p->index_bytes, c->pk, c->sm,
+                                        c->sm + p->sig_bytes, c->smlen - p->sig_bytes});
                     }
+                    std::vector<int> ok(jobs.size());
+                    e_->verify_many(fam, (int)jobs.size(), jobs.data(), ok.data());
+                    for (size_t i = 0; i < cs.size(); ++i) cs[i]->ok = ok[i];
                     break;
                 }
             }
@@ -241,13 +244,12 @@ private:
     Engine* ev_;
     std::mutex mu_;
     std::condition_variable cv_, cv_done_;
-    std::vector<Call*> pending_;
-    int active_ = 0;
+    Lane lanes_[2];
+    int active_ = 0;   // requests running on workers
+    int waiting_ = 0;  // calls queued or executing in either lane
     bool stop_ = false;
-    Clock::time_point first_;
     std::atomic<uint64_t> launches_{0};
     uint64_t calls_ = 0;
-    std::thread th_;
 };
 
 // BDS backend of one in-flight signature: hashing goes through the dispatcher.

@@ -39,12 +39,11 @@ int core_sign_open(const xmss_params* p, unsigned char* m, unsigned long long* m
     memcpy(pre + p->padding_len + p->n, pk, p->n);
     ull_to_bytes(pre + p->padding_len + 2 * p->n, p->n, idx);
 
-    Engine::VerifyItem it{pk, idx, sm + p->index_bytes, sm + p->index_bytes + p->n, m + p->sig_bytes, *mlen};
-    Engine::Batch* b = e->verify_prepare(fam, p->tree_height, p->d, 1, &it);
-    e->run(b);
+    // PUB_SEED's context stays in the device pool across verifications
+    const uint32_t slot = (uint32_t)e->pool_slot(fam, {nullptr, nullptr, pk + p->n});
+    Engine::VerifyJob job{slot, p->tree_height, p->d, p->index_bytes, pk, sm, m + p->sig_bytes, *mlen};
     int ok = 0;
-    e->collect_verify(b, &ok);
-    e->free_batch(b);
+    e->verify_many(fam, 1, &job, &ok);
     if (!ok) {
         memset(m, 0, *mlen);
         *mlen = 0;
