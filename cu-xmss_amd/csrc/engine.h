@@ -108,6 +108,8 @@ public:
     // for work that runs concurrently with the shared engine (the serving
     // loop's verifications).  Not shared; the caller serialises its use.
     static Engine* make_private();
+    // Hands a private engine back for reuse by a later make_private.
+    static void release_private(Engine* e);
 
     void set_keys(int fam, int nkeys, const KeySeeds* seeds);
 

@@ -137,10 +137,35 @@ Engine* Engine::get() {
     return e;
 }
 
+// Private engines are kept for reuse rather than destroyed: their streams
+// and device buffers live as long as the process, as the shared engine's do.
+static std::mutex& private_mutex() {
+    static std::mutex m;
+    return m;
+}
+static std::vector<Engine*>& private_free() {
+    static std::vector<Engine*> v;
+    return v;
+}
+
 Engine* Engine::make_private() {
+    {
+        std::lock_guard<std::mutex> lk(private_mutex());
+        if (!private_free().empty()) {
+            Engine* e = private_free().back();
+            private_free().pop_back();
+            return e;
+        }
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return nullptr;
     return new Engine();
+}
+
+void Engine::release_private(Engine* e) {
+    if (!e) return;
+    std::lock_guard<std::mutex> lk(private_mutex());
+    private_free().push_back(e);
 }
 
 void Engine::set_profiling(bool on) {

@@ -94,19 +94,18 @@ struct BdsDev {
     __device__ size_t th_node(uint32_t i, uint32_t j) const { return o_th(i, j) + 7; }
     __device__ uint32_t next_leaf(uint32_t i) const { return rd(o_next_leaf(i), 4); }
 
-    // node copies (n bytes) by thread 0, then a barrier
+    // node copies (n bytes, disjoint ranges), one byte per lane, then a barrier
     __device__ void copy(uint8_t* dst, const uint8_t* src) {
-        if (tid == 0)
-            for (uint32_t b = 0; b < n; ++b) dst[b] = src[b];
+        if (tid < n) dst[tid] = src[tid];
         __syncthreads();
     }
+    // swap of two disjoint byte ranges of the states
     __device__ void swap(size_t a, size_t b, size_t len) {
-        if (tid == 0)
-            for (size_t x = 0; x < len; ++x) {
-                const uint8_t t = sk[a + x];
-                sk[a + x] = sk[b + x];
-                sk[b + x] = t;
-            }
+        for (size_t x = tid; x < len; x += blockDim.x) {
+            const uint8_t t = sk[a + x];
+            sk[a + x] = sk[b + x];
+            sk[b + x] = t;
+        }
         __syncthreads();
     }
     // dst <- thash_h(in[0..2n)) at ADRS a (thread 0); dst may alias in.  The
@@ -115,11 +114,15 @@ struct BdsDev {
     // in a wide launch before this one (F::pre_h); past the planned count the
     // whole thash_h runs here.
     __device__ void hash_h(uint8_t* dst, const uint8_t* in, const uint32_t (&a)[8]) {
+        // node buffers are not 4-byte aligned in the serialized state: the
+        // lanes gather the 2n input bytes into an aligned buffer first
+        __shared__ alignas(16) uint8_t hb[3 * F::kN];
+        if (tid < 2 * n) hb[tid] = in[tid];
+        __syncthreads();
         if (tid == 0) {
             typename F::Node L, R, o;
-            // node buffers are not 4-byte aligned in the serialized state: gather bytes
-            load_bytes(L, in);
-            load_bytes(R, in + n);
+            F::load(L, hb);
+            F::load(R, hb + n);
             if (mcur < nmerge) {
                 F::fin_h(o, merges[mcur], L, R);
             } else {
@@ -128,9 +131,11 @@ struct BdsDev {
                 for (int i = 0; i < 8; ++i) ad[i] = a[i];
                 F::thash_h(o, key, ad, L, R);
             }
-            store_bytes(o, dst);
+            F::store(o, hb + 2 * n);
         }
         ++mcur;
+        __syncthreads();
+        if (tid < n) dst[tid] = hb[2 * n + tid];
         __syncthreads();
     }
     __device__ static void load_bytes(typename F::Node& x, const uint8_t* p) {

@@ -76,18 +76,20 @@ struct Call {
 
 class Dispatcher {
 public:
-    // Two lanes, each with its own thread: verifications on the private
-    // engine `ev` (its own stream), everything else on the shared engine `e`.
-    // A lane launches what it has queued once every running request is
-    // waiting on the device (queued or executing in either lane) or 200 us
-    // after its oldest queued call, so signatures never wait for a
-    // verification group and the other way round.
-    Dispatcher(Engine* e, Engine* ev) : e_(e), ev_(ev) {
-        lanes_[0].engine = e;
-        lanes_[0].shared = true;
-        lanes_[1].engine = ev ? ev : e;
-        lanes_[1].shared = ev == nullptr;
-        for (int i = 0; i < 2; ++i) lanes_[i].th = std::thread([this, i] { loop(lanes_[i]); });
+    // Lanes, each with its own thread and engine (HIP streams and buffers):
+    // lane 0 runs on the library's shared engine and takes every kind of call;
+    // the other signing lanes take BDS signatures, the verification lanes
+    // verifications, each on a private engine.  A lane launches what its
+    // queue holds as soon as it is idle and every running request is waiting
+    // on the device (queued or executing anywhere), or 200 us after the
+    // oldest queued call -- so a request that arrives while a group is on
+    // the GPU starts in the next idle lane instead of waiting for that group.
+    Dispatcher(Engine* shared, const std::vector<Engine*>& sign, const std::vector<Engine*>& verify) {
+        lanes_.emplace_back(new Lane{shared, true, (1u << Q_OTHER) | (1u << Q_SIGN)});
+        for (Engine* e : sign) lanes_.emplace_back(new Lane{e, false, 1u << Q_SIGN});
+        for (Engine* e : verify) lanes_.emplace_back(new Lane{e, false, 1u << Q_VERIFY});
+        if (verify.empty()) lanes_[0]->queues |= 1u << Q_VERIFY;
+        for (auto& l : lanes_) l->th = std::thread([this, p = l.get()] { loop(*p); });
     }
     ~Dispatcher() {
         {
@@ -95,7 +97,10 @@ public:
             stop_ = true;
         }
         cv_.notify_all();
-        for (auto& l : lanes_) l.th.join();
+        for (auto& l : lanes_) {
+            l->th.join();
+            if (!l->shared) Engine::release_private(l->engine);
+        }
     }
     void enter() {
         std::lock_guard<std::mutex> lk(mu_);
@@ -106,12 +111,12 @@ public:
         --active_;
         cv_.notify_all();
     }
-    // Blocks until the dispatcher has executed the call.
+    // Blocks until a lane has executed the call.
     void run(Call* c) {
         std::unique_lock<std::mutex> lk(mu_);
-        Lane& l = lanes_[c->kind == Call::VERIFY ? 1 : 0];
-        l.pending.push_back(c);
-        if (l.pending.size() == 1) l.first = Clock::now();
+        const int q = c->kind == Call::VERIFY ? Q_VERIFY : c->kind == Call::BDS ? Q_SIGN : Q_OTHER;
+        if (queue_[q].empty()) first_[q] = Clock::now();
+        queue_[q].push_back(c);
         ++waiting_;
         cv_.notify_all();
         cv_done_.wait(lk, [c] { return c->done; });
@@ -120,25 +125,36 @@ public:
     uint64_t calls() const { return calls_; }
 
 private:
+    enum { Q_SIGN = 0, Q_VERIFY = 1, Q_OTHER = 2, NQ = 3 };
     struct Lane {
-        Engine* engine = nullptr;
-        bool shared = true;  // the shared engine: hold the library-wide engine lock
-        std::vector<Call*> pending;
-        Clock::time_point first;
+        Lane(Engine* e, bool s, uint32_t q) : engine(e), shared(s), queues(q) {}
+        Engine* engine;
+        bool shared;      // the library's engine: hold the library-wide engine lock
+        uint32_t queues;  // bit mask of the queues this lane serves
         std::thread th;
     };
+
+    // The queue this lane should launch now, or -1.
+    int ready(const Lane& l) const {
+        for (int q = 0; q < NQ; ++q) {
+            if (!(l.queues & (1u << q)) || queue_[q].empty()) continue;
+            if (waiting_ >= active_ || Clock::now() - first_[q] >= std::chrono::microseconds(200)) return q;
+        }
+        return -1;
+    }
 
     void loop(Lane& l) {
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
-            cv_.wait_for(lk, std::chrono::microseconds(200), [&] {
-                return stop_ || (!l.pending.empty() && waiting_ >= active_);
-            });
-            if (stop_ && l.pending.empty()) return;
-            if (l.pending.empty()) continue;
-            if (waiting_ < active_ && Clock::now() - l.first < std::chrono::microseconds(200)) continue;
+            int q = -1;
+            cv_.wait_for(lk, std::chrono::microseconds(200), [&] { return stop_ || (q = ready(l)) >= 0; });
+            if (q < 0) q = ready(l);
+            if (q < 0) {
+                if (stop_) return;
+                continue;
+            }
             std::vector<Call*> batch;
-            batch.swap(l.pending);
+            batch.swap(queue_[q]);
             lk.unlock();
             execute_on(l.engine, batch, l.shared);
             lk.lock();
@@ -214,9 +230,14 @@ private:
                     break;
                 }
                 case Call::BDS: {
-                    // signatures with the traversal on the device: one batch for all shapes
+                    // signatures with the traversal on the device: one batch for all shapes,
+                    // key contexts from this engine's own pool (cached by seed)
                     std::vector<Engine::BdsJob> jobs;
-                    for (Call* c : cs) jobs.push_back(*c->job);
+                    for (Call* c : cs) {
+                        jobs.push_back(*c->job);
+                        Engine::BdsJob& j = jobs.back();
+                        j.key = (uint32_t)e_->pool_slot(fam, {j.sk_seed, j.sk_prf, j.pub_seed});
+                    }
                     e_->bds_sign_batch(fam, (int)jobs.size(), jobs.data(), true);
                     break;
                 }
@@ -240,13 +261,13 @@ private:
         }
     }
 
-    Engine* e_;
-    Engine* ev_;
     std::mutex mu_;
     std::condition_variable cv_, cv_done_;
-    Lane lanes_[2];
+    std::vector<std::unique_ptr<Lane>> lanes_;
+    std::vector<Call*> queue_[NQ];
+    Clock::time_point first_[NQ];
     int active_ = 0;   // requests running on workers
-    int waiting_ = 0;  // calls queued or executing in either lane
+    int waiting_ = 0;  // calls queued or executing in any lane
     bool stop_ = false;
     std::atomic<uint64_t> launches_{0};
     uint64_t calls_ = 0;
@@ -507,7 +528,13 @@ xmss_mi355x_server* xmss_mi355x_server_create(size_t hbm_budget_bytes, int worke
     xmss_mi355x_server* s = new xmss_mi355x_server;
     s->e = e;
     s->budget = hbm_budget_bytes;
-    s->disp.reset(new Dispatcher(e, Engine::make_private()));
+    // one more signing lane and two verification lanes on private engines
+    std::vector<Engine*> sign_lanes, verify_lanes;
+    for (int i = 0; i < 3; ++i) {
+        Engine* p = Engine::make_private();
+        if (p) (i == 0 ? sign_lanes : verify_lanes).push_back(p);
+    }
+    s->disp.reset(new Dispatcher(e, sign_lanes, verify_lanes));
     if (workers <= 0) workers = 256;
     for (int i = 0; i < workers; ++i) s->workers.emplace_back([s] { s->work(); });
     return s;
