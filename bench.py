@@ -201,6 +201,46 @@ def bench_plain_single(api, count):
             "verify_p50_ms": round(1000 * q(vlat, 0.5), 3), "verify_p99_ms": round(1000 * q(vlat, 0.99), 3)}
 
 
+def bench_opk(api, count):
+    """One XMSS-SHA2_10_256 key signing `count` messages (log-uniform 32 B -
+    4 KB) at consecutive indices from 0 with xmss_mi355x_sign_many (one tree
+    pass shared by all), against the same key signing one xmss_sign call per
+    message (timed on the first 32).  Returns signs/s of both."""
+    L = api.lib
+    L.xmss_mi355x_sign_many.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_size_t, C.c_void_p]
+    rng = random.Random(SEED + 19)
+    p = api.params(0x01, 0)
+    rc, pk, sk = api.seed_keypair(0x01, 0, bytes(rng.getrandbits(8) for _ in range(96)))
+    assert rc == 0
+    oid = (1).to_bytes(4, "big")
+    pk, sk = oid + pk, oid + sk
+    msgs = [bytes(rng.getrandbits(8) for _ in range(int(32 * 128 ** rng.random()))) for _ in range(count)]
+    sms = [buf(p.sig_bytes + len(m)) for m in msgs]
+    mbs = [buf(m) for m in msgs]
+    smp = (C.c_void_p * count)(*[C.cast(x, C.c_void_p) for x in sms])
+    mp = (C.c_void_p * count)(*[C.cast(x, C.c_void_p) for x in mbs])
+    mlens = (C.c_ulonglong * count)(*[len(m) for m in msgs])
+    smlens = (C.c_ulonglong * count)()
+    rets = (C.c_int * count)()
+    skb = buf(sk)
+    t0 = time.perf_counter()
+    assert L.xmss_mi355x_sign_many(0, skb, smp, smlens, mp, mlens, count, rets) == 0
+    t_many = time.perf_counter() - t0
+    assert all(rets[i] == 0 for i in range(count))
+    vr, mm = api.sign_open(raw(sms[count - 1], smlens[count - 1]), pk, 0)
+    assert vr == 0 and mm == msgs[count - 1]
+    k = min(32, count)
+    t0 = time.perf_counter()
+    s = sk
+    for m in msgs[:k]:
+        rc, _, s = api.sign(s, m, 0)
+        assert rc == 0
+    t_one = time.perf_counter() - t0
+    return {"signs": count, "signs_per_s": round(count / t_many, 1), "seconds": round(t_many, 4),
+            "one_call_each_signs_per_s": round(k / t_one, 1)}
+
+
 MIXED = [("XMSSMT-SHA2_20/2_256", 1, 0x01), ("XMSS-SHA2_10_512", 0, 0x04), ("XMSS-SHAKE_10_256", 0, 0x07)]
 
 
@@ -467,6 +507,8 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--latency-signs", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--opk", type=int, default=1023,
+                    help="consecutive signatures of one XMSS-SHA2_10_256 key through sign_many (0: skip)")
     ap.add_argument("--single", type=int, default=256,
                     help="keys signed one xmss_sign call each, beside the batch (0: skip)")
     ap.add_argument("--mixed", type=int, default=1024, help="keys per parameter set of the mixed batch (0: skip)")
@@ -534,6 +576,10 @@ def main():
     if args.batch and args.single:
         log("plain one-request-per-launch x%d ..." % args.single)
         single = bench_plain_single(api, args.single)
+    opk = None
+    if args.opk:
+        log("one key, %d consecutive signatures ..." % args.opk)
+        opk = bench_opk(api, args.opk)
     mixed = None
     if args.mixed:
         log("mixed batch x%d per set ..." % args.mixed)
@@ -576,6 +622,7 @@ def main():
         "kernels_batch_sign_ms": {k: round(v[1], 3) for k, v in bst.items()},
         "batch_last_index_rejected": n_last,
         "plain_one_per_call": single,
+        "one_key_many_signs": opk,
         "mixed_batch": mixed,
         "serving": serve,
     }

@@ -3,10 +3,12 @@
 // signature rebuilds its subtrees, here as one GPU tree build per call.
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/randombytes.h"
 #include "../../include/xmss_core.h"
+#include "../../include/xmss_mi355x.h"
 #include "core_common.h"
 
 using namespace xm;
@@ -76,6 +78,55 @@ int xmssmt_core_sign(const xmss_params* params, unsigned char* sk, unsigned char
     unsigned char* out = sm + ib;
     e->collect_sign(b, &out);
     e->free_batch(b);
+    return 0;
+}
+
+// Consecutive signatures of one key (the reference's GPU "opk" mode,
+// gpu/gpu_sign_origin.h:23-26): the same bytes and final sk as `count`
+// successive xmss[mt]_sign calls, with every subtree the signatures share
+// built once.  The last index and anything after it go through
+// xmssmt_core_sign one at a time (wiped key, then -2).
+int xmss_mi355x_sign_many(int mt, unsigned char* sk, unsigned char* const* sms, unsigned long long* smlens,
+                          const unsigned char* const* ms, const unsigned long long* mlens, size_t count,
+                          int* rets) {
+    xmss_params p;
+    const uint32_t oid = (uint32_t)bytes_to_ull(sk, XMSS_OID_LEN);
+    if ((mt ? xmssmt_parse_oid(&p, oid) : xmss_parse_oid(&p, oid)) != 0) {
+        for (size_t i = 0; i < count; ++i) rets[i] = -1;
+        return -1;
+    }
+    const unsigned n = p.n, ib = p.index_bytes;
+    const int fam = fam_of(p.func, n);
+    unsigned char* k = sk + XMSS_OID_LEN;
+    const unsigned long long idx0 = bytes_to_ull(k, ib);
+    const unsigned long long last = (1ULL << p.full_height) - 1;
+    const size_t shared = (fam >= 0 && idx0 < last) ? (size_t)std::min<unsigned long long>(count, last - idx0) : 0;
+    if (shared) {
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        Engine* e = Engine::get();
+        if (!e) {
+            for (size_t i = 0; i < count; ++i) rets[i] = XMSS_ENODEV;
+            return XMSS_ENODEV;
+        }
+        std::vector<Engine::SignItem> items(shared);
+        std::vector<unsigned char*> outs(shared);
+        for (size_t i = 0; i < shared; ++i) {
+            unsigned char* sm = sms[i];
+            memmove(sm + p.sig_bytes, ms[i], mlens[i]);
+            smlens[i] = p.sig_bytes + mlens[i];
+            ull_to_bytes(sm, ib, idx0 + i);
+            items[i] = {k + ib, k + ib + n, k + ib + 3 * n, k + ib + 2 * n, idx0 + i, sm + p.sig_bytes, mlens[i]};
+            outs[i] = sm + ib;
+            rets[i] = 0;
+        }
+        // the index is past all of them before any signature exists
+        ull_to_bytes(k, ib, idx0 + shared);
+        Engine::Batch* b = e->sign_prepare(fam, p.tree_height, p.d, (int)shared, items.data(), true);
+        e->run(b);
+        e->collect_sign(b, outs.data());
+        e->free_batch(b);
+    }
+    for (size_t i = shared; i < count; ++i) rets[i] = xmssmt_core_sign(&p, k, sms[i], &smlens[i], ms[i], mlens[i]);
     return 0;
 }
 

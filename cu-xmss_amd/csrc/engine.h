@@ -165,7 +165,9 @@ public:
         uint64_t mlen;
     };
     struct Batch;
-    Batch* sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items);
+    // one_key: all items sign with the same key (its seeds from items[0]);
+    // items whose signatures use the same (layer, tree) share one subtree build.
+    Batch* sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items, bool one_key = false);
     Batch* verify_prepare(int fam, unsigned th, unsigned d, int count, const VerifyItem* items);
     void run(Batch* b);
     // sign: per item R (n) || d x (WOTS sig || auth path) written to out[i]

@@ -474,6 +474,7 @@ struct Engine::Batch {
     int fam = 0;
     unsigned th = 0, d = 0, n = 0, len = 0;
     int count = 0;
+    int nruns = 0;  // subtrees to build (count x d, or fewer when items share trees)
     DevBuf keys, seeds, msg_desc, blob, R, dig, runs, nodes, ends, tmp, gidx, auth, wdesc, wots, vdesc,
         roots, ok, pkroot;
     size_t nodes_stride = 0;
@@ -540,7 +541,8 @@ static Engine::Batch* take_batch(Engine::Impl* I, bool sign) {
     return b ? b : new Engine::Batch;
 }
 
-Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items) {
+Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items,
+                                    bool one_key) {
     Impl* I = impl_;
     Batch* b = take_batch(I, true);
     b->sign = true;
@@ -551,34 +553,50 @@ Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count,
     b->len = fam_len(fam);
     b->count = count;
     const unsigned n = b->n;
-    std::vector<KeySeeds> ks(count);
+    // one_key: every item signs with the same key (consecutive indices), so
+    // one key context serves all and items on the same (layer, tree) share
+    // that subtree -- it is built once
+    std::vector<KeySeeds> ks(one_key ? 1 : count);
     std::vector<MsgReq> mr(count);
     for (int i = 0; i < count; ++i) {
-        ks[i] = {items[i].sk_seed, items[i].sk_prf, items[i].pub_seed};
-        mr[i] = {(uint32_t)i, 1, items[i].idx, nullptr, items[i].root, items[i].m, items[i].mlen};
+        const uint32_t key = one_key ? 0u : (uint32_t)i;
+        if (!one_key || i == 0) ks[i] = {items[i].sk_seed, items[i].sk_prf, items[i].pub_seed};
+        mr[i] = {key, 1, items[i].idx, nullptr, items[i].root, items[i].m, items[i].mlen};
     }
     stage_keys(I, b, ks);
     stage_msgs(I, b->msg_desc, b->blob, fam, count, true, mr);
-    // one subtree per (item, layer): layer i, tree idx >> (th * (i + 1))
-    std::vector<DRun> runs((size_t)count * d);
+    // one subtree per distinct (key, layer, tree): layer i, tree idx >> (th * (i + 1))
+    std::vector<DRun> runs;
+    std::map<std::pair<uint32_t, uint64_t>, uint32_t> shared;  // one_key: (layer, tree) -> run
     std::vector<DWots> wd((size_t)count * d);
     std::vector<uint64_t> gi((size_t)count * d * (th + 1));
     b->nodes_stride = (2ull << th) - 1;
     for (int i = 0; i < count; ++i) {
         uint64_t idx = items[i].idx;
+        const uint32_t key = one_key ? 0u : (uint32_t)i;
         for (unsigned L = 0; L < d; ++L) {
             const uint32_t leaf = (uint32_t)(idx & ((1ull << th) - 1));
             idx >>= th;
-            const size_t t = (size_t)L * count + i;  // layer-major
-            runs[t] = {(uint32_t)i, L, (uint32_t)(idx >> 32), (uint32_t)idx, 0, 0, 0, 0};
-            wd[t] = {(uint32_t)i, L, (uint32_t)(idx >> 32), (uint32_t)idx, leaf, MODE_SIGN, 0, 0};
+            const size_t t = (size_t)L * count + i;  // layer-major item slots
+            uint64_t run;
+            auto it = one_key ? shared.find({L, idx}) : shared.end();
+            if (it != shared.end()) {
+                run = it->second;
+            } else {
+                run = runs.size();
+                runs.push_back({key, L, (uint32_t)(idx >> 32), (uint32_t)idx, 0, 0, 0, 0});
+                if (one_key) shared.emplace(std::make_pair(L, idx), (uint32_t)run);
+            }
+            wd[t] = {key, L, (uint32_t)(idx >> 32), (uint32_t)idx, leaf, MODE_SIGN, 0, 0};
             // auth path (th nodes) then the root, gathered into auth[t]
             for (unsigned j = 0; j < th; ++j) {
-                gi[t * (th + 1) + j] = t * b->nodes_stride + ((2ull << th) - (2ull << (th - j))) + ((leaf >> j) ^ 1);
+                gi[t * (th + 1) + j] =
+                    run * b->nodes_stride + ((2ull << th) - (2ull << (th - j))) + ((leaf >> j) ^ 1);
             }
-            gi[t * (th + 1) + th] = t * b->nodes_stride + b->nodes_stride - 1;
+            gi[t * (th + 1) + th] = run * b->nodes_stride + b->nodes_stride - 1;
         }
     }
+    b->nruns = (int)runs.size();
     DRun* dr = b->runs.need<DRun>(sizeof(DRun) * runs.size());
     I->h2d(dr, runs.data(), sizeof(DRun) * runs.size());
     DWots* dw = b->wdesc.need<DWots>(sizeof(DWots) * wd.size());
@@ -587,7 +605,7 @@ Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count,
     I->h2d(dg, gi.data(), sizeof(uint64_t) * gi.size());
     b->R.need((size_t)count * n);
     b->dig.need((size_t)count * n);
-    b->nodes.need(b->nodes_stride * n * (size_t)count * d);
+    b->nodes.need(b->nodes_stride * n * (size_t)b->nruns);
     b->auth.need(gi.size() * n);
     b->wots.need((size_t)count * d * b->len * n);
     b->roots.need((size_t)count * d * n);  // layer-major roots
@@ -655,8 +673,8 @@ void Engine::run(Batch* b) {
     o->hmsg(I->L, dk, (const DMsg*)b->msg_desc.p, (uint32_t)count, fam_pad(b->fam), (uint8_t*)b->blob.p,
             (uint8_t*)b->R.p, (uint8_t*)b->dig.p);
     if (b->sign) {
-        // all subtrees (count x d) in one pipeline
-        const int nruns = count * (int)d;
+        // all subtrees in one pipeline
+        const int nruns = b->nruns;
         const uint64_t leaves = 1ull << th;
         const uint64_t per_leaf = (uint64_t)len * n;
         const uint64_t stride = b->nodes_stride;
@@ -688,7 +706,7 @@ void Engine::run(Batch* b) {
             r0 += rg;
         }
         // auth paths and roots: auth[t] = th auth nodes || root, t = layer * count + item
-        const uint32_t ng = (uint32_t)(nruns * (th + 1));
+        const uint32_t ng = (uint32_t)((size_t)count * d * (th + 1));
         I->L.run("gather", k_gather, dim3(cdiv((uint64_t)ng * (n / 4), WG)), dim3(WG), (const uint8_t*)nodes,
                  (const uint64_t*)b->gidx.p, ng, n, (uint8_t*)b->auth.p);
         // WOTS signatures layer by layer; layer L signs the root of layer L-1

@@ -73,6 +73,20 @@ void xmss_mi355x_job_free(xmss_mi355x_job *job);
 int xmss_batch_keypair(int mt, unsigned char *const *pks, unsigned char *const *sks,
                        const unsigned char *seeds, uint32_t oid, size_t count);
 
+/* ---- one key, many messages (plain core: libxmss_mi355x.so only) --------
+ * Signs ms[0..count) with consecutive indices of one OID-prefixed secret key:
+ * the signatures, return values (rets[i]) and final sk are exactly those of
+ * `count` successive xmss[mt]_sign calls, but every subtree the signatures
+ * share is built once (one tree pass for up to 2^h signatures of an XMSS
+ * key).  The counterpart of the reference's GPU one-key mode
+ * (gpu/gpu_sign_origin.h:23-26, gpu_xmss[mt]_opk_sign_origin), with the
+ * messages given separately.  Returns 0, -1 for an unknown OID, or -3
+ * without a GPU.  xmss_batch_sign_open verifies such signatures in one
+ * batch (pks[i] may all be the same key). */
+int xmss_mi355x_sign_many(int mt, unsigned char *sk, unsigned char *const *sms,
+                          unsigned long long *smlens, const unsigned char *const *ms,
+                          const unsigned long long *mlens, size_t count, int *rets);
+
 /* One-call forms of the above. */
 int xmss_batch_sign(int mt, unsigned char *const *sks, unsigned char *const *sms,
                     unsigned long long *smlens, const unsigned char *const *ms,
