@@ -250,11 +250,18 @@ def bench_mixed(api, per_set):
     """The mixed-parameter batch: per_set keys of each of XMSSMT-SHA2_20/2_256,
     XMSS-SHA2_10_512 and XMSS-SHAKE_10_256, one plain-core signature each at
     a uniform leaf with a log-uniform 32 B - 4 KB message, then verification.
-    One batch per parameter set, run back to back; the time is their sum."""
+    Timed twice: one batch per parameter set back to back (per-set rates), and
+    all 3 x per_set requests as one mixed job (each set on its own engine,
+    sharing the GPU)."""
     L = api.lib
+    L.xmss_mi355x_mixed_sign_prepare.restype = C.c_void_p
+    L.xmss_mi355x_mixed_sign_prepare.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    L.xmss_mi355x_mixed_verify_prepare.restype = C.c_void_p
+    L.xmss_mi355x_mixed_verify_prepare.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
     rng = random.Random(SEED + 11)
     res = {}
     total_sign = total_ver = 0.0
+    allk = []  # (mt, p, pk bytes, sk bytes, msg)
     for name, mt, oid in MIXED:
         p = api.params(oid, mt)
         seeds = bytes(rng.getrandbits(8) for _ in range(3 * p.n * per_set))
@@ -265,6 +272,8 @@ def bench_mixed(api, per_set):
         for i in range(per_set):
             sks[i][4:4 + p.index_bytes] = list(idxs[i].to_bytes(p.index_bytes, "big"))
         msgs = [bytes(rng.getrandbits(8) for _ in range(int(32 * 128 ** rng.random()))) for _ in range(per_set)]
+        for i in range(per_set):
+            allk.append((mt, p, raw(pks[i], 4 + p.pk_bytes), raw(sks[i], 4 + p.sk_bytes), msgs[i]))
         ms = [buf(m) for m in msgs]
         mlens = (C.c_ulonglong * per_set)(*[len(m) for m in msgs])
         sms = [buf(p.sig_bytes + len(m)) for m in msgs]
@@ -294,6 +303,38 @@ def bench_mixed(api, per_set):
         log("  %s: sign %.3f s, verify %.4f s" % (name, ts, tv))
     res["total_sign_s"] = round(total_sign, 4)
     res["total_verify_s"] = round(total_ver, 5)
+    # the same requests, shuffled, as one mixed job
+    rng.shuffle(allk)
+    k = len(allk)
+    mts = (C.c_int * k)(*[x[0] for x in allk])
+    sks = [buf(x[3]) for x in allk]
+    ms = [buf(x[4]) for x in allk]
+    mlens = (C.c_ulonglong * k)(*[len(x[4]) for x in allk])
+    sms = [buf(x[1].sig_bytes + len(x[4])) for x in allk]
+    smlens = (C.c_ulonglong * k)()
+    rets = (C.c_int * k)()
+    job = L.xmss_mi355x_mixed_sign_prepare(mts, ptrs(sks), ptrs(ms), mlens, k)
+    t0 = time.perf_counter()
+    L.xmss_mi355x_job_run(job)
+    ts = time.perf_counter() - t0
+    L.xmss_mi355x_sign_collect(job, ptrs(sms), smlens, rets)
+    L.xmss_mi355x_job_free(job)
+    assert all(rets[i] == 0 for i in range(k))
+    pks = [buf(x[2]) for x in allk]
+    smb = [buf(raw(sms[i], smlens[i])) for i in range(k)]
+    outs = [buf(smlens[i]) for i in range(k)]
+    mls = (C.c_ulonglong * k)()
+    vr = (C.c_int * k)()
+    job = L.xmss_mi355x_mixed_verify_prepare(mts, ptrs(smb), smlens, ptrs(pks), k)
+    t0 = time.perf_counter()
+    L.xmss_mi355x_job_run(job)
+    tv = time.perf_counter() - t0
+    L.xmss_mi355x_verify_collect(job, ptrs(outs), mls, vr)
+    L.xmss_mi355x_job_free(job)
+    assert all(vr[i] == 0 for i in range(k))
+    res["one_job"] = {"requests": k, "sign_s": round(ts, 4), "verify_s": round(tv, 5),
+                      "signs_per_s": round(k / ts, 1), "verifies_per_s": round(k / tv, 1)}
+    log("  one mixed job: sign %.3f s, verify %.4f s" % (ts, tv))
     return res
 
 

@@ -169,7 +169,10 @@ public:
     // items whose signatures use the same (layer, tree) share one subtree build.
     Batch* sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items, bool one_key = false);
     Batch* verify_prepare(int fam, unsigned th, unsigned d, int count, const VerifyItem* items);
-    void run(Batch* b);
+    // wait_done = false returns once the work is queued; wait() then blocks
+    // until this engine's streams are idle.
+    void run(Batch* b, bool wait_done = true);
+    void wait();
     // sign: per item R (n) || d x (WOTS sig || auth path) written to out[i]
     //       (sig_bytes - index_bytes bytes each).
     // verify: ok[i] = 1 when the recomputed root equals the pk root.

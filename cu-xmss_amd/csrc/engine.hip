@@ -669,7 +669,9 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
     return b;
 }
 
-void Engine::run(Batch* b) {
+void Engine::wait() { impl_->sync(); }
+
+void Engine::run(Batch* b, bool wait_done) {
     Impl* I = impl_;
     FamOps* o = ops_of(b->fam);
     const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
@@ -750,7 +752,7 @@ void Engine::run(Batch* b) {
                  (const uint8_t*)b->roots.p + (size_t)(d - 1) * count * n, (const uint8_t*)b->pkroot.p,
                  (uint32_t)count, n, (int*)b->ok.p);
     }
-    I->sync();
+    if (wait_done) I->sync();
 }
 
 void Engine::collect_sign(Batch* b, uint8_t* const* out) {

@@ -12,17 +12,29 @@
 
 using namespace xm;
 
-struct xmss_mi355x_job {
-    bool sign = false;
+// One parameter set of a job: its batch on one engine.  A single-set job has
+// one; a mixed job one per (XMSS / XMSS^MT, OID), each on its own engine so
+// that their launches run side by side on the GPU.
+struct JobSet {
+    int mt = 0;
+    uint32_t oid = 0;
     xmss_params p{};
     int fam = -1;
+    Engine* e = nullptr;
+    bool shared = true;        // e is the library's engine (else a private one)
+    Engine::Batch* b = nullptr;
+    std::vector<size_t> live;  // job inputs that reach the device, in batch order
+};
+
+struct xmss_mi355x_job {
+    bool sign = false;
     size_t count = 0;
     std::vector<int> rets;
-    std::vector<size_t> live;  // inputs that reach the device
-    Engine::Batch* b = nullptr;
+    std::vector<int> set;  // input -> index in sets (-1: unknown OID)
+    std::vector<JobSet> sets;
     // sign (the pointer arrays are copied; the buffers they point to must
     // stay valid until collect)
-    std::vector<unsigned char> idxbytes;
+    std::vector<unsigned char> idxbytes;  // 8 bytes per input
     std::vector<const unsigned char*> ms;
     std::vector<unsigned long long> mlens;
     // verify
@@ -83,53 +95,98 @@ size_t xmss_mi355x_kernel_stats(char* buf, size_t len) {
     return s.size() + 1;
 }
 
-xmss_mi355x_job* xmss_mi355x_sign_prepare(int mt, unsigned char* const* sks, const unsigned char* const* ms,
-                                          const unsigned long long* mlens, size_t count) {
+}  // extern "C"
+
+namespace {
+
+// The set of input i (created on first use), or -1 for an unknown OID.  With
+// `mixed` false every input must carry the first input's OID.
+int set_of(xmss_mi355x_job* j, int mt, uint32_t oid, bool mixed) {
+    for (size_t s = 0; s < j->sets.size(); ++s)
+        if (j->sets[s].mt == mt && j->sets[s].oid == oid) return (int)s;
+    if (!mixed && !j->sets.empty()) return -1;
+    JobSet js;
+    js.mt = mt;
+    js.oid = oid;
+    if (parse(mt, oid, &js.p) != 0) return -1;
+    js.fam = fam_of(js.p.func, js.p.n);
+    if (js.fam < 0) return -1;
+    j->sets.push_back(js);
+    return (int)j->sets.size() - 1;
+}
+
+// Engines for the sets that reach the device: the library's for the first,
+// private ones for the others.  Called with the engine lock held.
+bool bind_engines(xmss_mi355x_job* j) {
+    Engine* shared = Engine::get();
+    if (!shared) return false;
+    bool first = true;
+    for (JobSet& s : j->sets) {
+        if (s.live.empty()) continue;
+        s.e = first ? shared : Engine::make_private();
+        s.shared = first;
+        if (!s.e) {
+            s.e = shared;
+            s.shared = true;
+        }
+        first = false;
+    }
+    return true;
+}
+
+void fail_all(xmss_mi355x_job* j, int rc) {
+    for (JobSet& s : j->sets) {
+        for (size_t i : s.live) j->rets[i] = rc;
+        s.live.clear();
+    }
+}
+
+xmss_mi355x_job* sign_prepare(const int* mts, int mt_all, unsigned char* const* sks, const unsigned char* const* ms,
+                              const unsigned long long* mlens, size_t count, bool mixed) {
     xmss_mi355x_job* j = new xmss_mi355x_job;
     j->sign = true;
     j->count = count;
     j->ms.assign(ms, ms + count);
     j->mlens.assign(mlens, mlens + count);
     j->rets.assign(count, -1);
-    if (count == 0) return j;
-    const uint32_t oid0 = oid_of(sks[0]);
-    if (parse(mt, oid0, &j->p)) return j;
-    const xmss_params& p = j->p;
-    const unsigned n = p.n, ib = p.index_bytes;
-    j->fam = fam_of(p.func, n);
-    j->idxbytes.assign(count * ib, 0);
-    std::vector<Engine::SignItem> items;
+    j->set.assign(count, -1);
+    j->idxbytes.assign(count * 8, 0);
+    std::vector<std::vector<Engine::SignItem>> items;
     for (size_t i = 0; i < count; ++i) {
-        unsigned char* sk = sks[i];
-        if (oid_of(sk) != oid0) continue;  // one parameter set per batch
-        sk += XMSS_OID_LEN;
+        const int s = set_of(j, mts ? mts[i] : mt_all, oid_of(sks[i]), mixed);
+        if (s < 0) continue;  // unknown OID (or, single-set job, another parameter set)
+        j->set[i] = s;
+        items.resize(j->sets.size());
+        const xmss_params& p = j->sets[s].p;
+        const unsigned n = p.n, ib = p.index_bytes;
+        unsigned char* sk = sks[i] + XMSS_OID_LEN;
         const unsigned long long idx = (unsigned long)bytes_to_ull(sk, ib);
         if (index_check(&p, sk, idx)) {
             j->rets[i] = -2;
             continue;
         }
-        memcpy(&j->idxbytes[i * ib], sk, ib);
+        memcpy(&j->idxbytes[i * 8], sk, ib);
         ull_to_bytes(sk, ib, idx + 1);
-        items.push_back({sk + ib, sk + ib + n, sk + ib + 3 * n, sk + ib + 2 * n, idx, ms[i], mlens[i]});
-        j->live.push_back(i);
+        items[s].push_back({sk + ib, sk + ib + n, sk + ib + 3 * n, sk + ib + 2 * n, idx, ms[i], mlens[i]});
+        j->sets[s].live.push_back(i);
         j->rets[i] = 0;
     }
-    if (!items.empty() && j->fam >= 0) {
-        std::lock_guard<std::mutex> lk(engine_mutex());
-        Engine* e = Engine::get();
-        if (!e) {
-            for (size_t i : j->live) j->rets[i] = XMSS_ENODEV;
-            j->live.clear();
-            return j;
-        }
-        j->b = e->sign_prepare(j->fam, p.tree_height, p.d, (int)items.size(), items.data());
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    if (!bind_engines(j)) {
+        fail_all(j, XMSS_ENODEV);
+        return j;
+    }
+    for (size_t s = 0; s < j->sets.size(); ++s) {
+        JobSet& js = j->sets[s];
+        if (!js.live.empty())
+            js.b = js.e->sign_prepare(js.fam, js.p.tree_height, js.p.d, (int)items[s].size(), items[s].data());
     }
     return j;
 }
 
-xmss_mi355x_job* xmss_mi355x_verify_prepare(int mt, const unsigned char* const* sms,
-                                            const unsigned long long* smlens, const unsigned char* const* pks,
-                                            size_t count) {
+xmss_mi355x_job* verify_prepare(const int* mts, int mt_all, const unsigned char* const* sms,
+                                const unsigned long long* smlens, const unsigned char* const* pks, size_t count,
+                                bool mixed) {
     xmss_mi355x_job* j = new xmss_mi355x_job;
     j->sign = false;
     j->count = count;
@@ -137,103 +194,142 @@ xmss_mi355x_job* xmss_mi355x_verify_prepare(int mt, const unsigned char* const* 
     j->smlens.assign(smlens, smlens + count);
     j->pks.assign(pks, pks + count);
     j->rets.assign(count, -1);
-    if (count == 0) return j;
-    const uint32_t oid0 = oid_of(pks[0]);
-    if (parse(mt, oid0, &j->p)) return j;
-    const xmss_params& p = j->p;
-    const unsigned n = p.n, ib = p.index_bytes;
-    j->fam = fam_of(p.func, n);
-    std::vector<Engine::VerifyItem> items;
+    j->set.assign(count, -1);
+    std::vector<std::vector<Engine::VerifyItem>> items;
     for (size_t i = 0; i < count; ++i) {
-        if (oid_of(pks[i]) != oid0 || smlens[i] < p.sig_bytes) continue;
+        const int s = set_of(j, mts ? mts[i] : mt_all, oid_of(pks[i]), mixed);
+        if (s < 0) continue;
+        j->set[i] = s;
+        items.resize(j->sets.size());
+        const xmss_params& p = j->sets[s].p;
+        const unsigned n = p.n, ib = p.index_bytes;
+        if (smlens[i] < p.sig_bytes) continue;
         const unsigned char* sm = sms[i];
-        items.push_back({pks[i] + XMSS_OID_LEN, bytes_to_ull(sm, ib), sm + ib, sm + ib + n, sm + p.sig_bytes,
-                         smlens[i] - p.sig_bytes});
-        j->live.push_back(i);
+        items[s].push_back({pks[i] + XMSS_OID_LEN, bytes_to_ull(sm, ib), sm + ib, sm + ib + n, sm + p.sig_bytes,
+                            smlens[i] - p.sig_bytes});
+        j->sets[s].live.push_back(i);
     }
-    if (!items.empty() && j->fam >= 0) {
-        std::lock_guard<std::mutex> lk(engine_mutex());
-        Engine* e = Engine::get();
-        if (!e) {
-            for (size_t i : j->live) j->rets[i] = XMSS_ENODEV;
-            j->live.clear();
-            return j;
-        }
-        j->b = e->verify_prepare(j->fam, p.tree_height, p.d, (int)items.size(), items.data());
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    if (!bind_engines(j)) {
+        fail_all(j, XMSS_ENODEV);
+        return j;
+    }
+    for (size_t s = 0; s < j->sets.size(); ++s) {
+        JobSet& js = j->sets[s];
+        if (!js.live.empty())
+            js.b = js.e->verify_prepare(js.fam, js.p.tree_height, js.p.d, (int)items[s].size(), items[s].data());
     }
     return j;
 }
 
+}  // namespace
+
+extern "C" {
+
+xmss_mi355x_job* xmss_mi355x_sign_prepare(int mt, unsigned char* const* sks, const unsigned char* const* ms,
+                                          const unsigned long long* mlens, size_t count) {
+    return sign_prepare(nullptr, mt, sks, ms, mlens, count, false);
+}
+
+xmss_mi355x_job* xmss_mi355x_verify_prepare(int mt, const unsigned char* const* sms,
+                                            const unsigned long long* smlens, const unsigned char* const* pks,
+                                            size_t count) {
+    return verify_prepare(nullptr, mt, sms, smlens, pks, count, false);
+}
+
+xmss_mi355x_job* xmss_mi355x_mixed_sign_prepare(const int* mts, unsigned char* const* sks,
+                                                const unsigned char* const* ms, const unsigned long long* mlens,
+                                                size_t count) {
+    return sign_prepare(mts, 0, sks, ms, mlens, count, true);
+}
+
+xmss_mi355x_job* xmss_mi355x_mixed_verify_prepare(const int* mts, const unsigned char* const* sms,
+                                                  const unsigned long long* smlens,
+                                                  const unsigned char* const* pks, size_t count) {
+    return verify_prepare(mts, 0, sms, smlens, pks, count, true);
+}
+
 int xmss_mi355x_job_run(xmss_mi355x_job* j) {
-    if (!j->b) return 0;
     std::lock_guard<std::mutex> lk(engine_mutex());
-    Engine* e = Engine::get();
-    if (!e) return XMSS_ENODEV;
-    e->run(j->b);
+    // every set's launches are queued first, on its own engine's streams,
+    // so the parameter sets share the GPU; then all are waited for
+    for (JobSet& s : j->sets)
+        if (s.b) s.e->run(s.b, false);
+    for (JobSet& s : j->sets)
+        if (s.b) s.e->wait();
     return 0;
 }
 
 int xmss_mi355x_sign_collect(xmss_mi355x_job* j, unsigned char* const* sms, unsigned long long* smlens,
                              int* rets) {
-    const xmss_params& p = j->p;
-    if (j->b) {
-        std::vector<unsigned char*> outs;
-        for (size_t i : j->live) outs.push_back(sms[i] + p.index_bytes);
+    {
         std::lock_guard<std::mutex> lk(engine_mutex());
-        Engine::get()->collect_sign(j->b, outs.data());
+        for (JobSet& s : j->sets) {
+            if (!s.b) continue;
+            std::vector<unsigned char*> outs;
+            for (size_t i : s.live) outs.push_back(sms[i] + s.p.index_bytes);
+            s.e->collect_sign(s.b, outs.data());
+        }
     }
     for (size_t i = 0; i < j->count; ++i) {
         rets[i] = j->rets[i];
+        if (j->set[i] < 0) continue;
+        const xmss_params& p = j->sets[j->set[i]].p;
         if (j->rets[i] == 0 || j->rets[i] == -2) {
             // message after the signature, smlen set even for -2 (xmss_core.c:205-206)
             memmove(sms[i] + p.sig_bytes, j->ms[i], j->mlens[i]);
             smlens[i] = p.sig_bytes + j->mlens[i];
         }
-        if (j->rets[i] == 0) memcpy(sms[i], &j->idxbytes[i * p.index_bytes], p.index_bytes);
+        if (j->rets[i] == 0) memcpy(sms[i], &j->idxbytes[i * 8], p.index_bytes);
     }
     return 0;
 }
 
 int xmss_mi355x_verify_collect(xmss_mi355x_job* j, unsigned char* const* ms, unsigned long long* mlens,
                                int* rets) {
-    const xmss_params& p = j->p;
-    std::vector<int> ok(j->live.size(), 0);
-    if (j->b) {
-        std::lock_guard<std::mutex> lk(engine_mutex());
-        Engine::get()->collect_verify(j->b, ok.data());
-    }
     for (size_t i = 0; i < j->count; ++i) {
         rets[i] = j->rets[i];
         if (rets[i] == -1) mlens[i] = 0;
     }
-    for (size_t k = 0; k < j->live.size(); ++k) {
-        const size_t i = j->live[k];
-        const unsigned char* sm = j->sms[i];
-        unsigned char* m = ms[i];
-        const unsigned long long mlen = j->smlens[i] - p.sig_bytes;
-        memmove(m + p.sig_bytes, sm + p.sig_bytes, mlen);
-        unsigned char* pre = m + p.sig_bytes - p.padding_len - 3 * p.n;
-        ull_to_bytes(pre, p.padding_len, 2);
-        memcpy(pre + p.padding_len, sm + p.index_bytes, p.n);
-        memcpy(pre + p.padding_len + p.n, j->pks[i] + XMSS_OID_LEN, p.n);
-        ull_to_bytes(pre + p.padding_len + 2 * p.n, p.n, bytes_to_ull(sm, p.index_bytes));
-        if (ok[k]) {
-            memmove(m, sm + p.sig_bytes, mlen);
-            mlens[i] = mlen;
-            rets[i] = 0;
-        } else {
-            memset(m, 0, mlen);
-            mlens[i] = 0;
-            rets[i] = -1;
+    for (JobSet& s : j->sets) {
+        const xmss_params& p = s.p;
+        std::vector<int> ok(s.live.size(), 0);
+        if (s.b) {
+            std::lock_guard<std::mutex> lk(engine_mutex());
+            s.e->collect_verify(s.b, ok.data());
+        }
+        for (size_t k = 0; k < s.live.size(); ++k) {
+            const size_t i = s.live[k];
+            const unsigned char* sm = j->sms[i];
+            unsigned char* m = ms[i];
+            const unsigned long long mlen = j->smlens[i] - p.sig_bytes;
+            memmove(m + p.sig_bytes, sm + p.sig_bytes, mlen);
+            unsigned char* pre = m + p.sig_bytes - p.padding_len - 3 * p.n;
+            ull_to_bytes(pre, p.padding_len, 2);
+            memcpy(pre + p.padding_len, sm + p.index_bytes, p.n);
+            memcpy(pre + p.padding_len + p.n, j->pks[i] + XMSS_OID_LEN, p.n);
+            ull_to_bytes(pre + p.padding_len + 2 * p.n, p.n, bytes_to_ull(sm, p.index_bytes));
+            if (ok[k]) {
+                memmove(m, sm + p.sig_bytes, mlen);
+                mlens[i] = mlen;
+                rets[i] = 0;
+            } else {
+                memset(m, 0, mlen);
+                mlens[i] = 0;
+                rets[i] = -1;
+            }
         }
     }
     return 0;
 }
 
 void xmss_mi355x_job_free(xmss_mi355x_job* j) {
-    if (j->b) {
+    {
         std::lock_guard<std::mutex> lk(engine_mutex());
-        Engine::get()->free_batch(j->b);
+        for (JobSet& s : j->sets) {
+            if (s.b) s.e->free_batch(s.b);
+            if (s.e && !s.shared) Engine::release_private(s.e);
+        }
     }
     delete j;
 }

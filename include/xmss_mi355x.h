@@ -58,6 +58,16 @@ xmss_mi355x_job *xmss_mi355x_verify_prepare(int mt, const unsigned char *const *
                                             const unsigned long long *smlens,
                                             const unsigned char *const *pks,
                                             size_t count);
+/* Mixed parameter sets in one job: input i belongs to XMSS (mts[i] = 0) or
+ * XMSS^MT (mts[i] = 1) and to the parameter set its OID names; each set gets
+ * its own batch on its own engine (HIP streams), and job_run queues all of
+ * them before waiting, so the sets share the GPU.  collect and free as above. */
+xmss_mi355x_job *xmss_mi355x_mixed_sign_prepare(const int *mts, unsigned char *const *sks,
+                                                const unsigned char *const *ms,
+                                                const unsigned long long *mlens, size_t count);
+xmss_mi355x_job *xmss_mi355x_mixed_verify_prepare(const int *mts, const unsigned char *const *sms,
+                                                  const unsigned long long *smlens,
+                                                  const unsigned char *const *pks, size_t count);
 int xmss_mi355x_job_run(xmss_mi355x_job *job);
 int xmss_mi355x_sign_collect(xmss_mi355x_job *job, unsigned char *const *sms,
                              unsigned long long *smlens, int *rets);

@@ -88,3 +88,89 @@ def test_three_step_job_api_with_temporary_pointer_arrays():
     for i, it in enumerate(items):
         assert rets[i] == it["rc"]
         assert raw(sms[i], smlens[i]).hex() == it["sm"]
+
+
+MIXED_SETS = [("XMSS-SHA2_10_256", 0), ("XMSSMT-SHA2_20/4_256", 1), ("XMSS-SHAKE_10_256", 0),
+              ("XMSSMT-SHA2_20/4_512", 1), ("XMSS-SHA2_10_192", 0), ("XMSSMT-SHAKE256_20/4_192", 1)]
+
+
+def test_mixed_parameter_sets_in_one_job():
+    """One job over six parameter sets of five hash families (each set on its
+    own engine): every signature, sk and return value equals the single
+    xmss[mt]_sign call on the same key, including a last-index (wiped key)
+    and an exhausted key; the mixed verification job accepts exactly the
+    valid ones and rejects an unknown OID."""
+    import random
+    lib = product(0)
+    L = lib.lib
+    L.xmss_mi355x_mixed_sign_prepare.restype = C.c_void_p
+    L.xmss_mi355x_mixed_sign_prepare.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    L.xmss_mi355x_mixed_verify_prepare.restype = C.c_void_p
+    L.xmss_mi355x_mixed_verify_prepare.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    L.xmss_mi355x_job_run.argtypes = [C.c_void_p]
+    L.xmss_mi355x_sign_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.xmss_mi355x_verify_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.xmss_mi355x_job_free.argtypes = [C.c_void_p]
+    rng = random.Random(4242)
+    cases = []  # (mt, p, pk, sk_in, msg)
+    for name, mt in MIXED_SETS:
+        oid = lib.str_to_oid(name)
+        p = lib.params(oid, mt)
+        L.xmss_mi355x_rng_seed(oid * 7 + mt)
+        rc, pk, sk = lib.keypair(oid, mt)
+        assert rc == 0
+        top = (1 << p.full_height) - 1
+        idxs = [rng.randrange(0, top), rng.randrange(0, top)]
+        if name == "XMSS-SHA2_10_256":
+            idxs += [top, top + 1]  # last index (wiped key) and exhausted
+        for idx in idxs:
+            s = bytearray(sk)
+            s[4:4 + p.index_bytes] = idx.to_bytes(p.index_bytes, "big")
+            m = bytes(rng.getrandbits(8) for _ in range(int(32 * 64 ** rng.random())))
+            cases.append((mt, p, pk, bytes(s), m))
+    rng.shuffle(cases)
+    k = len(cases)
+    want = [lib.sign(c[3], c[4], c[0]) for c in cases]
+    mts = (C.c_int * k)(*[c[0] for c in cases])
+    sks = [buf(c[3]) for c in cases]
+    mb = [buf(c[4]) for c in cases]
+    mlens = (C.c_ulonglong * k)(*[len(c[4]) for c in cases])
+    sms = [buf(c[1].sig_bytes + len(c[4])) for c in cases]
+    smlens = (C.c_ulonglong * k)()
+    rets = (C.c_int * k)()
+    job = L.xmss_mi355x_mixed_sign_prepare(mts, _arr(sks), _arr(mb), mlens, k)
+    assert L.xmss_mi355x_job_run(job) == 0
+    assert L.xmss_mi355x_sign_collect(job, _arr(sms), smlens, rets) == 0
+    L.xmss_mi355x_job_free(job)
+    for i, (c, w) in enumerate(zip(cases, want)):
+        assert rets[i] == w[0], "item %d rc" % i
+        if w[0] == 0:
+            assert raw(sms[i], smlens[i]) == w[1], "item %d signature" % i
+        assert raw(sks[i], len(c[3])) == w[2], "item %d sk" % i
+    # verification: the signed items, plus one with an unknown OID
+    vi = [i for i in range(k) if rets[i] == 0]
+    vsm = [raw(sms[i], smlens[i]) for i in vi]
+    vpk = [cases[i][2] for i in vi]
+    vmt = [cases[i][0] for i in vi]
+    vsm.append(vsm[0])
+    vpk.append(b"\xff\xff\xff\xff" + vpk[0][4:])
+    vmt.append(vmt[0])
+    kv = len(vsm)
+    smb = [buf(s) for s in vsm]
+    outs = [buf(len(s)) for s in vsm]
+    smls = (C.c_ulonglong * kv)(*[len(s) for s in vsm])
+    mls = (C.c_ulonglong * kv)()
+    vr = (C.c_int * kv)()
+    pkb = [buf(x) for x in vpk]  # must outlive the job
+    job = L.xmss_mi355x_mixed_verify_prepare((C.c_int * kv)(*vmt), _arr(smb), smls, _arr(pkb), kv)
+    assert L.xmss_mi355x_job_run(job) == 0
+    assert L.xmss_mi355x_verify_collect(job, _arr(outs), mls, vr) == 0
+    L.xmss_mi355x_job_free(job)
+    for t, i in enumerate(vi):
+        mt, p = cases[i][0], cases[i][1]
+        idx = int.from_bytes(bytes(cases[i][3][4:4 + p.index_bytes]), "big")
+        ok = idx < (1 << p.full_height) - 1
+        assert vr[t] == (0 if ok else -1), "verify item %d" % i
+        if ok:
+            assert raw(outs[t], mls[t]) == cases[i][4]
+    assert vr[kv - 1] == -1
