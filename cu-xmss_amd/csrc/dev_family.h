@@ -419,7 +419,25 @@ struct Sha256Fam {
         sha256_iv(st);
         const uint32_t nb = (len + 9 + 63) / 64;
         const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
-        for (uint32_t b = 0; b < nb; ++b) {
+        // blocks made only of message bytes: plain loads, the next block's
+        // words in flight while this one compresses
+        const uint32_t nfull = len / 64;
+        uint32_t b = 0;
+        if (nfull) {
+            uint32_t nx[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) nx[j] = q[j];
+            for (; b < nfull; ++b) {
+                uint32_t w[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) w[j] = bswap32(nx[j]);
+                const uint32_t* qn = q + 16 * (b + 1 < nfull ? b + 1 : b);
+#pragma unroll
+                for (int j = 0; j < 16; ++j) nx[j] = qn[j];
+                sha256_compress(st, w);
+            }
+        }
+        for (; b < nb; ++b) {
             uint32_t w[16];
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
@@ -655,7 +673,25 @@ struct Sha512Fam {
         sha512_iv(st);
         const uint32_t nb = (len + 17 + 127) / 128;
         const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
-        for (uint32_t b = 0; b < nb; ++b) {
+        // blocks made only of message bytes: plain loads, the next block's
+        // words in flight while this one compresses
+        const uint32_t nfull = len / 128;
+        uint32_t b = 0;
+        if (nfull) {
+            uint32_t nx[32];
+#pragma unroll
+            for (int j = 0; j < 32; ++j) nx[j] = q[j];
+            for (; b < nfull; ++b) {
+                uint64_t w[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) w[j] = ((uint64_t)bswap32(nx[2 * j]) << 32) | bswap32(nx[2 * j + 1]);
+                const uint32_t* qn = q + 32 * (b + 1 < nfull ? b + 1 : b);
+#pragma unroll
+                for (int j = 0; j < 32; ++j) nx[j] = qn[j];
+                sha512_compress(st, w);
+            }
+        }
+        for (; b < nb; ++b) {
             uint64_t w[16];
 #pragma unroll
             for (int j = 0; j < 16; ++j) {

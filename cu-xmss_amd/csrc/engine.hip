@@ -57,6 +57,10 @@ static FamOps* ops_of(int fam) {
 // XMSS-SHA2_10_256 verifications: 2.4 M/s, against 1.3 M/s with the
 // one-workgroup-per-request kernel, whose lanes each walk a whole chain).
 constexpr size_t kLatScratch = 4ull << 30;
+// From this many requests of one layer on, a verification batch finishes
+// lane-dense (per chain, per L-tree pair, per auth walk) instead of with one
+// workgroup per request.
+constexpr int kLatDense = 256;
 static bool use_lat(FamOps* o, size_t count, uint32_t th) {
     return count > 0 && count <= 65535 && count * o->lat_bytes(th) <= kLatScratch;
 }
@@ -646,7 +650,7 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
             const uint32_t leaf = (uint32_t)(idx & ((1ull << th) - 1));
             idx >>= th;
             const size_t t = (size_t)L * count + i;
-            vd[t] = {(uint32_t)i, L, (uint32_t)(idx >> 32), (uint32_t)idx, leaf, th, 0, (uint32_t)(i * th)};
+            vd[t] = {(uint32_t)i, L, (uint32_t)(idx >> 32), (uint32_t)idx, leaf, th, LAT_ALL_STEPS, (uint32_t)(i * th)};
             memcpy(&sigs[t * len * n], p, (size_t)len * n);
             p += (size_t)len * n;
             memcpy(&auth[t * th * n], p, (size_t)th * n);
@@ -677,8 +681,26 @@ void Engine::run(Batch* b, bool wait_done) {
     const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
     const int count = b->count;
     const void* dk = b->keys.p;
+    // verification through the latency kernels: the address-only work of
+    // every layer (all 15 steps of each chain) runs on the aux stream beside
+    // H_msg, which it does not need
+    const size_t nv = (size_t)count * d;
+    const bool vlat = !b->sign && use_lat(o, nv, th);
+    uint8_t* vscratch = vlat ? I->lat.need(nv * o->lat_bytes(th)) : nullptr;
+    hipEvent_t staged = nullptr, pre = nullptr;
+    if (vlat) {
+        staged = I->L.ev();
+        pre = I->L.ev();
+        HIPCK(hipEventRecord(staged, I->L.s_main));
+        HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
+        I->L.s = I->L.s_aux;
+        o->lat_pre(I->L, dk, LAT_VERIFY, (const DLeaf*)b->vdesc.p, (uint32_t)nv, nullptr, th, vscratch);
+        HIPCK(hipEventRecord(pre, I->L.s_aux));
+        I->L.s = I->L.s_main;
+    }
     o->hmsg(I->L, dk, (const DMsg*)b->msg_desc.p, (uint32_t)count, fam_pad(b->fam), (uint8_t*)b->blob.p,
             (uint8_t*)b->R.p, (uint8_t*)b->dig.p);
+    if (vlat) HIPCK(hipStreamWaitEvent(I->L.s_main, pre, 0));
     if (b->sign) {
         // all subtrees in one pipeline
         const int nruns = b->nruns;
@@ -737,11 +759,17 @@ void Engine::run(Batch* b, bool wait_done) {
     } else {
         for (unsigned L = 0; L < d; ++L) {
             const uint8_t* msg_l = L == 0 ? (const uint8_t*)b->dig.p : (const uint8_t*)b->roots.p + (size_t)(L - 1) * count * n;
-            if (use_lat(o, count, th))
-                o->lat(I->L, dk, LAT_VERIFY, (const DLeaf*)b->vdesc.p + (size_t)L * count, (uint32_t)count, msg_l,
-                       (const uint8_t*)b->wots.p + (size_t)L * count * len * n,
-                       (const uint8_t*)b->auth.p + (size_t)L * count * th * n, th,
-                       I->lat.need((size_t)count * o->lat_bytes(th)), (uint8_t*)b->roots.p + (size_t)L * count * n);
+            const DLeaf* vd = (const DLeaf*)b->vdesc.p + (size_t)L * count;
+            const uint8_t* sig_l = (const uint8_t*)b->wots.p + (size_t)L * count * len * n;
+            const uint8_t* auth_l = (const uint8_t*)b->auth.p + (size_t)L * count * th * n;
+            uint8_t* root_l = (uint8_t*)b->roots.p + (size_t)L * count * n;
+            const uint8_t* scr_l = vscratch + (size_t)L * count * o->lat_bytes(th);
+            if (vlat && count >= kLatDense)
+                o->lat_verify_batch(I->L, vd, (uint32_t)count, msg_l, sig_l, auth_l, th, scr_l,
+                                    b->ends.need((size_t)count * len * n),
+                                    b->tmp.need((size_t)count * ((len + 1) / 2) * n), root_l);
+            else if (vlat)
+                o->lat_fin(I->L, LAT_VERIFY, vd, (uint32_t)count, msg_l, sig_l, auth_l, th, scr_l, root_l);
             else
                 o->leaf_wg(I->L, dk, (const DLeaf*)b->vdesc.p + (size_t)L * count, (uint32_t)count, true, msg_l,
                            (const uint8_t*)b->wots.p + (size_t)L * count * len * n,
@@ -751,6 +779,10 @@ void Engine::run(Batch* b, bool wait_done) {
         I->L.run("compare", k_compare, dim3(cdiv(count, WG)), dim3(WG),
                  (const uint8_t*)b->roots.p + (size_t)(d - 1) * count * n, (const uint8_t*)b->pkroot.p,
                  (uint32_t)count, n, (int*)b->ok.p);
+    }
+    if (vlat) {
+        I->L.pool.push_back(staged);
+        I->L.pool.push_back(pre);
     }
     if (wait_done) I->sync();
 }

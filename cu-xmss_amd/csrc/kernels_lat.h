@@ -182,4 +182,86 @@ __global__ __launch_bounds__(192) void k_lat_fin(const DLeaf* __restrict__ reqs,
     F::store(node, out + (size_t)r * F::kN);
 }
 
+// ---- the same finish for large verification batches, lane-dense --------
+// k_lat_fin gives each request a workgroup, whose L-tree heights and auth
+// walk leave most lanes idle; for thousands of requests the finish runs as
+// one lane per (request, chain), then per (request, L-tree pair) and height,
+// then per request for the auth walk.
+
+// WOTS chains of verify layers from their signatures: ends[r][c].
+template <class F>
+__global__ __launch_bounds__(WG) void k_latb_chains(uint32_t n, const uint8_t* __restrict__ msgs,
+                                                     const uint8_t* __restrict__ sigs,
+                                                     const uint8_t* __restrict__ scratch, size_t stride,
+                                                     uint8_t* __restrict__ ends) {
+    using FPre = typename F::FPre;
+    const uint32_t g = blockIdx.x * WG + threadIdx.x;
+    if (g >= n * (uint32_t)F::LEN) return;
+    const uint32_t r = g / F::LEN, c = g % F::LEN;
+    const uint32_t s0 = chain_len<F>(msgs + (size_t)r * F::kN, c);
+    typename F::Node x;
+    F::load(x, sigs + (size_t)g * F::kN);
+    const FPre* fp = reinterpret_cast<const FPre*>(scratch + (size_t)r * stride + LatLayout<F>::fpre) + c * 15;
+    FPre cur = fp[s0 < 15 ? s0 : 14];
+    for (uint32_t s = s0; s < 15; ++s) {
+        const FPre nxt = fp[s + 1 < 15 ? s + 1 : 14];
+        F::fin_f(x, cur, x);
+        cur = nxt;
+    }
+    F::store(x, ends + (size_t)g * F::kN);
+}
+
+// One L-tree height for every request: l nodes per request in src (src_stride
+// nodes apart) -> ceil(l/2) in dst; merge `off + i` of the request's H
+// pre-images hashes pair i; an odd last node is carried up.
+template <class F>
+__global__ __launch_bounds__(WG) void k_latb_ltree(uint32_t n, uint32_t l, uint32_t off,
+                                                    const uint8_t* __restrict__ src, uint32_t src_stride,
+                                                    uint8_t* __restrict__ dst, uint32_t dst_stride,
+                                                    const uint8_t* __restrict__ scratch, size_t stride) {
+    using HPre = typename F::HPre;
+    const uint32_t half = (l + 1) >> 1;
+    const uint32_t g = blockIdx.x * WG + threadIdx.x;
+    if (g >= n * half) return;
+    const uint32_t r = g / half, i = g % half;
+    const uint8_t* s = src + (size_t)r * src_stride * F::kN;
+    typename F::Node L, R, o;
+    F::load(L, s + (size_t)(2 * i) * F::kN);
+    if (2 * i + 1 < l) {
+        F::load(R, s + (size_t)(2 * i + 1) * F::kN);
+        const HPre* hp = reinterpret_cast<const HPre*>(scratch + (size_t)r * stride + LatLayout<F>::hpre);
+        F::fin_h(o, hp[off + i], L, R);
+    } else {
+        o = L;
+    }
+    F::store(o, dst + ((size_t)r * dst_stride + i) * F::kN);
+}
+
+// Auth walk (compute_root, xmss_commons.c:54-96) of every request, from its
+// L-tree root at leaves[r * leaf_stride].
+template <class F>
+__global__ __launch_bounds__(WG) void k_latb_auth(const DLeaf* __restrict__ reqs, uint32_t n,
+                                                   const uint8_t* __restrict__ leaves, uint32_t leaf_stride,
+                                                   const uint8_t* __restrict__ auth,
+                                                   const uint8_t* __restrict__ scratch, size_t stride,
+                                                   uint8_t* __restrict__ out) {
+    using HPre = typename F::HPre;
+    const uint32_t r = blockIdx.x * WG + threadIdx.x;
+    if (r >= n) return;
+    const DLeaf q = reqs[r];
+    const HPre* hp = reinterpret_cast<const HPre*>(scratch + (size_t)r * stride + LatLayout<F>::hpre);
+    typename F::Node node;
+    F::load(node, leaves + (size_t)r * leaf_stride * F::kN);
+    uint32_t idx = q.leaf;
+    const uint8_t* ap = auth + (size_t)q.p2 * F::kN;
+    for (uint32_t i = 0; i < q.p0; ++i) {
+        typename F::Node sib;
+        F::load(sib, ap + (size_t)i * F::kN);
+        if (idx & 1) F::fin_h(node, hp[F::LEN - 1 + i], sib, node);
+        else F::fin_h(node, hp[F::LEN - 1 + i], node, sib);
+        idx >>= 1;
+    }
+    F::store(node, out + (size_t)r * F::kN);
+}
+
 }  // namespace xm

@@ -119,6 +119,11 @@ struct FamOps {
     virtual void lat_fin(Launcher& L, uint32_t mode, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
                          const uint8_t* sigs, const uint8_t* auth, uint32_t th, const uint8_t* scratch,
                          uint8_t* out) = 0;
+    // Lane-dense finish of n verify requests (same layer) whose k_lat_pre
+    // scratch is ready: ends and tmp hold n x LEN and n x ceil(LEN/2) nodes.
+    virtual void lat_verify_batch(Launcher& L, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
+                                  const uint8_t* sigs, const uint8_t* auth, uint32_t th, const uint8_t* scratch,
+                                  uint8_t* ends, uint8_t* tmp, uint8_t* out) = 0;
     virtual void thash_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, const uint8_t* in,
                          uint8_t* out) = 0;
     virtual void hmsg(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad,
@@ -203,6 +208,26 @@ struct FamOpsT final : FamOps {
              const uint8_t* sigs, const uint8_t* auth, uint32_t th, uint8_t* scratch, uint8_t* out) override {
         lat_pre(L, keys, mode, reqs, n, msgs, th, scratch);
         lat_fin(L, mode, reqs, n, msgs, sigs, auth, th, scratch, out);
+    }
+    void lat_verify_batch(Launcher& L, const DLeaf* reqs, uint32_t n, const uint8_t* msgs, const uint8_t* sigs,
+                          const uint8_t* auth, uint32_t th, const uint8_t* scratch, uint8_t* ends, uint8_t* tmp,
+                          uint8_t* out) override {
+        const size_t stride = LatLayout<F>::bytes(th);
+        L.run("latb_chains", k_latb_chains<F>, dim3(cdiv((uint64_t)n * F::LEN, WG)), dim3(WG), n, msgs, sigs,
+              scratch, stride, ends);
+        uint32_t l = F::LEN, off = 0;
+        bool from_a = true;
+        const uint32_t sa = F::LEN, sb = (F::LEN + 1) / 2;
+        while (l > 1) {
+            L.run("latb_ltree", k_latb_ltree<F>, dim3(cdiv((uint64_t)n * ((l + 1) / 2), WG)), dim3(WG), n, l, off,
+                  (const uint8_t*)(from_a ? ends : tmp), from_a ? sa : sb, from_a ? tmp : ends, from_a ? sb : sa,
+                  scratch, stride);
+            off += l >> 1;
+            l = (l + 1) >> 1;
+            from_a = !from_a;
+        }
+        L.run("latb_auth", k_latb_auth<F>, dim3(cdiv(n, WG)), dim3(WG), reqs, n,
+              (const uint8_t*)(from_a ? ends : tmp), from_a ? sa : sb, auth, scratch, stride, out);
     }
     void thash_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, const uint8_t* in,
                  uint8_t* out) override {
