@@ -421,19 +421,26 @@ struct Sha256Fam {
         const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
         // blocks made only of message bytes: plain loads, the next block's
         // words in flight while this one compresses
+        // (p is 16-byte aligned: four 16-byte loads per block)
         const uint32_t nfull = len / 64;
         uint32_t b = 0;
         if (nfull) {
-            uint32_t nx[16];
+            const uint4* q4 = reinterpret_cast<const uint4*>(p);
+            uint4 nx[4];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) nx[j] = q[j];
+            for (int j = 0; j < 4; ++j) nx[j] = q4[j];
             for (; b < nfull; ++b) {
                 uint32_t w[16];
 #pragma unroll
-                for (int j = 0; j < 16; ++j) w[j] = bswap32(nx[j]);
-                const uint32_t* qn = q + 16 * (b + 1 < nfull ? b + 1 : b);
+                for (int j = 0; j < 4; ++j) {
+                    w[4 * j] = bswap32(nx[j].x);
+                    w[4 * j + 1] = bswap32(nx[j].y);
+                    w[4 * j + 2] = bswap32(nx[j].z);
+                    w[4 * j + 3] = bswap32(nx[j].w);
+                }
+                const uint4* qn = q4 + 4 * (b + 1 < nfull ? b + 1 : b);
 #pragma unroll
-                for (int j = 0; j < 16; ++j) nx[j] = qn[j];
+                for (int j = 0; j < 4; ++j) nx[j] = qn[j];
                 sha256_compress(st, w);
             }
         }
@@ -675,19 +682,24 @@ struct Sha512Fam {
         const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
         // blocks made only of message bytes: plain loads, the next block's
         // words in flight while this one compresses
+        // (p is 16-byte aligned: eight 16-byte loads per block)
         const uint32_t nfull = len / 128;
         uint32_t b = 0;
         if (nfull) {
-            uint32_t nx[32];
+            const uint4* q4 = reinterpret_cast<const uint4*>(p);
+            uint4 nx[8];
 #pragma unroll
-            for (int j = 0; j < 32; ++j) nx[j] = q[j];
+            for (int j = 0; j < 8; ++j) nx[j] = q4[j];
             for (; b < nfull; ++b) {
                 uint64_t w[16];
 #pragma unroll
-                for (int j = 0; j < 16; ++j) w[j] = ((uint64_t)bswap32(nx[2 * j]) << 32) | bswap32(nx[2 * j + 1]);
-                const uint32_t* qn = q + 32 * (b + 1 < nfull ? b + 1 : b);
+                for (int j = 0; j < 8; ++j) {
+                    w[2 * j] = ((uint64_t)bswap32(nx[j].x) << 32) | bswap32(nx[j].y);
+                    w[2 * j + 1] = ((uint64_t)bswap32(nx[j].z) << 32) | bswap32(nx[j].w);
+                }
+                const uint4* qn = q4 + 8 * (b + 1 < nfull ? b + 1 : b);
 #pragma unroll
-                for (int j = 0; j < 32; ++j) nx[j] = qn[j];
+                for (int j = 0; j < 8; ++j) nx[j] = qn[j];
                 sha512_compress(st, w);
             }
         }

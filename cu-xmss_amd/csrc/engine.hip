@@ -432,7 +432,7 @@ void Engine::hmsg(int fam, int nreq, const MsgReq* reqs, uint8_t* R_out, uint8_t
     size_t total = 0;
     for (int i = 0; i < nreq; ++i) {
         d[i] = {reqs[i].key, reqs[i].sign, reqs[i].idx, (uint64_t)total, (uint32_t)(pb + reqs[i].mlen), 0};
-        total += (pb + reqs[i].mlen + 7) & ~(size_t)7;
+        total += (pb + reqs[i].mlen + 15) & ~(size_t)15;
     }
     total += 8;
     // toByte(2, pad) || R || root || toByte(idx, n) || M  (hash.c:107-120)
@@ -506,7 +506,7 @@ static void stage_msgs(Engine::Impl* I, DevBuf& desc, DevBuf& blob, int fam, int
     size_t total = 0;
     for (int i = 0; i < count; ++i) {
         d[i] = {reqs[i].key, sign ? 1u : 0u, reqs[i].idx, (uint64_t)total, (uint32_t)(pb + reqs[i].mlen), 0};
-        total += (pb + reqs[i].mlen + 7) & ~(size_t)7;
+        total += (pb + reqs[i].mlen + 15) & ~(size_t)15;
     }
     total += 8;
     std::vector<uint8_t> h(total, 0);
@@ -918,7 +918,7 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
     size_t total = 0;
     for (int i = 0; i < count; ++i) {
         md[i] = {jobs[i].key, 1u, jobs[i].idx, (uint64_t)total, (uint32_t)(pb + jobs[i].mlen), 0};
-        total += (pb + jobs[i].mlen + 7) & ~(size_t)7;
+        total += (pb + jobs[i].mlen + 15) & ~(size_t)15;
     }
     total += 8;
     // One pinned staging area and one device arena, so that the inputs go
@@ -1046,7 +1046,7 @@ void Engine::verify_many(int fam, int count, const VerifyJob* jobs, int* ok) {
         uint64_t idx = 0;
         for (unsigned b = 0; b < v.index_bytes; ++b) idx = (idx << 8) | v.sig[b];
         md[j] = {v.key, 0u, idx, (uint64_t)total, (uint32_t)(pb + v.mlen), 0};
-        total += (pb + v.mlen + 7) & ~(size_t)7;
+        total += (pb + v.mlen + 15) & ~(size_t)15;
     }
     total += 8;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
