@@ -72,6 +72,7 @@ struct Call {
     // result
     unsigned char* out = nullptr;
     bool done = false;
+    std::condition_variable cv;  // the caller waits here (no wake-ups of unrelated callers)
 };
 
 class Dispatcher {
@@ -119,7 +120,7 @@ public:
         queue_[q].push_back(c);
         ++waiting_;
         cv_.notify_all();
-        cv_done_.wait(lk, [c] { return c->done; });
+        c->cv.wait(lk, [c] { return c->done; });
     }
     uint64_t launches() const { return launches_; }
     uint64_t calls() const { return calls_; }
@@ -158,10 +159,12 @@ private:
             lk.unlock();
             execute_on(l.engine, batch, l.shared);
             lk.lock();
-            for (Call* c : batch) c->done = true;
+            for (Call* c : batch) {
+                c->done = true;
+                c->cv.notify_one();
+            }
             waiting_ -= (int)batch.size();
             calls_ += batch.size();
-            cv_done_.notify_all();
             cv_.notify_all();
         }
     }
@@ -262,7 +265,7 @@ private:
     }
 
     std::mutex mu_;
-    std::condition_variable cv_, cv_done_;
+    std::condition_variable cv_;  // lanes wait here
     std::vector<std::unique_ptr<Lane>> lanes_;
     std::vector<Call*> queue_[NQ];
     Clock::time_point first_[NQ];
@@ -421,14 +424,16 @@ struct xmss_mi355x_server {
             disp->enter();
             execute(r);
             disp->leave();
+            bool more;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 reserved -= r->reserve;
                 if (r->sign) keys[r->key].busy = false;
                 r->t1 = Clock::now();
                 r->done = true;
+                more = !queue.empty();  // freed budget or key may admit queued requests
             }
-            cv_work.notify_all();
+            if (more) cv_work.notify_all();
             cv_done.notify_all();
         }
     }
