@@ -421,9 +421,12 @@ struct xmss_mi355x_server {
                 peak = std::max(peak, reserved);
                 if (r->sign) keys[r->key].busy = true;
             }
+            // the worker counts as active (more device work coming) from
+            // here until its last device call has returned
             disp->enter();
-            execute(r);
-            disp->leave();
+            bool active = true;
+            execute(r, active);
+            if (active) disp->leave();
             bool more;
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -461,7 +464,7 @@ struct xmss_mi355x_server {
         return v;
     }
 
-    void execute(Request* r) {
+    void execute(Request* r, bool& active) {
         Key& k = keys[r->key];
         const xmss_params& p = k.p;
         if (r->sign) {
@@ -477,6 +480,8 @@ struct xmss_mi355x_server {
                 call.p = &p;
                 call.job = &ds.job;
                 disp->run(&call);
+                disp->leave();
+                active = false;
                 bds_device_finish(&p, r->out.data(), &r->outlen, r->data.size(), ds);
                 r->ret = 0;
                 return;
@@ -508,6 +513,8 @@ struct xmss_mi355x_server {
             call.sm = r->data.data();
             call.smlen = smlen;
             disp->run(&call);
+            disp->leave();
+            active = false;
             const unsigned long long mlen = smlen - p.sig_bytes;
             if (call.ok) {
                 memcpy(r->out.data(), r->data.data() + p.sig_bytes, mlen);
