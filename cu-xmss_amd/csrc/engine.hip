@@ -57,6 +57,9 @@ static FamOps* ops_of(int fam) {
 // XMSS-SHA2_10_256 verifications: 2.4 M/s, against 1.3 M/s with the
 // one-workgroup-per-request kernel, whose lanes each walk a whole chain).
 constexpr size_t kLatScratch = 4ull << 30;
+// Up to this many subtrees per build, their levels use the split
+// (pre-image + finish) kernels.
+constexpr int kSplitRuns = 64;
 // From this many requests of one layer on, a verification batch finishes
 // lane-dense (per chain, per L-tree pair, per auth walk) instead of with one
 // workgroup per request.
@@ -99,7 +102,7 @@ struct Engine::Impl {
     VerifyBufs* ver = nullptr;          // buffers of verify_many
     const void* kp() const { return kp_override ? kp_override : keys.p; }
     Engine::Batch* spare[2] = {nullptr, nullptr};  // reusable verify / sign batches
-    DevBuf keys, seeds, desc, ends, tmp, nodes, a, b, c, d, idx, lat;
+    DevBuf keys, seeds, desc, ends, tmp, nodes, a, b, c, d, idx, lat, tpre;
     std::unordered_map<std::string, int> slots[F_COUNT];  // pool slot by seed bytes (pool_slot)
     int key_fam = -1;
     int nkeys = 0;
@@ -269,6 +272,21 @@ void Engine::build_trees(int fam, unsigned th, int nruns, const LeafRun* runs) {
     I->h2d(drun, d.data(), sizeof(DRun) * d.size());
     uint8_t* nodes = I->nodes.need(stride * n * (size_t)nruns);
     const void* dk = I->kp();
+    // few trees: their levels are latency-bound, so the address-only part of
+    // every internal node runs on the aux stream beside the leaves
+    const bool split = nruns <= kSplitRuns;
+    void* tpre = split ? I->tpre.need((size_t)nruns * ((1ull << th) - 1) * o->hpre_bytes()) : nullptr;
+    hipEvent_t staged = nullptr, pre = nullptr;
+    if (split) {
+        staged = I->L.ev();
+        pre = I->L.ev();
+        HIPCK(hipEventRecord(staged, I->L.s_main));
+        HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
+        I->L.s = I->L.s_aux;
+        o->pre_tree(I->L, dk, drun, nruns, th, tpre);
+        HIPCK(hipEventRecord(pre, I->L.s_aux));
+        I->L.s = I->L.s_main;
+    }
     for (int r0 = 0; r0 < nruns;) {
         int rg = std::min<int>(nruns - r0, 65535);
         uint64_t lpp = leaves;
@@ -292,10 +310,19 @@ void Engine::build_trees(int fam, unsigned th, int nruns, const LeafRun* runs) {
                 ++height;
             }
         }
-        for (unsigned lev = 0; lev < th; ++lev) o->level(I->L, dk, drun + r0, rg, th, lev, tnodes);
+        if (split) {
+            HIPCK(hipStreamWaitEvent(I->L.s_main, pre, 0));
+            o->levels_fin(I->L, rg, th, tnodes, (const uint8_t*)tpre + (size_t)r0 * ((1ull << th) - 1) * o->hpre_bytes());
+        } else {
+            for (unsigned lev = 0; lev < th; ++lev) o->level(I->L, dk, drun + r0, rg, th, lev, tnodes);
+        }
         r0 += rg;
     }
     I->sync();
+    if (split) {
+        I->L.pool.push_back(staged);
+        I->L.pool.push_back(pre);
+    }
 }
 
 void Engine::fetch_nodes(int fam, unsigned th, int nrefs, const NodeRef* refs, uint8_t* out) {
@@ -702,8 +729,21 @@ void Engine::run(Batch* b, bool wait_done) {
             (uint8_t*)b->R.p, (uint8_t*)b->dig.p);
     if (vlat) HIPCK(hipStreamWaitEvent(I->L.s_main, pre, 0));
     if (b->sign) {
-        // all subtrees in one pipeline
+        // all subtrees in one pipeline (few: levels split as in build_trees)
         const int nruns = b->nruns;
+        const bool split = nruns <= kSplitRuns;
+        void* tpre = split ? I->tpre.need((size_t)nruns * ((1ull << th) - 1) * o->hpre_bytes()) : nullptr;
+        hipEvent_t tstaged = nullptr, tdone = nullptr;
+        if (split) {
+            tstaged = I->L.ev();
+            tdone = I->L.ev();
+            HIPCK(hipEventRecord(tstaged, I->L.s_main));
+            HIPCK(hipStreamWaitEvent(I->L.s_aux, tstaged, 0));
+            I->L.s = I->L.s_aux;
+            o->pre_tree(I->L, dk, (const DRun*)b->runs.p, nruns, th, tpre);
+            HIPCK(hipEventRecord(tdone, I->L.s_aux));
+            I->L.s = I->L.s_main;
+        }
         const uint64_t leaves = 1ull << th;
         const uint64_t per_leaf = (uint64_t)len * n;
         const uint64_t stride = b->nodes_stride;
@@ -731,8 +771,18 @@ void Engine::run(Batch* b, bool wait_done) {
                     ++height;
                 }
             }
-            for (unsigned lev = 0; lev < th; ++lev) o->level(I->L, dk, drun + r0, rg, th, lev, tnodes);
+            if (split) {
+                HIPCK(hipStreamWaitEvent(I->L.s_main, tdone, 0));
+                o->levels_fin(I->L, rg, th, tnodes,
+                              (const uint8_t*)tpre + (size_t)r0 * ((1ull << th) - 1) * o->hpre_bytes());
+            } else {
+                for (unsigned lev = 0; lev < th; ++lev) o->level(I->L, dk, drun + r0, rg, th, lev, tnodes);
+            }
             r0 += rg;
+        }
+        if (split) {
+            I->L.pool.push_back(tstaged);
+            I->L.pool.push_back(tdone);
         }
         // auth paths and roots: auth[t] = th auth nodes || root, t = layer * count + item
         const uint32_t ng = (uint32_t)((size_t)count * d * (th + 1));

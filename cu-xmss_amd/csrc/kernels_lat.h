@@ -182,6 +182,77 @@ __global__ __launch_bounds__(192) void k_lat_fin(const DLeaf* __restrict__ reqs,
     F::store(node, out + (size_t)r * F::kN);
 }
 
+// ---- subtree levels of a few trees -------------------------------------
+// Internal node (level, i) of a 2^th-leaf subtree -- the parent of children
+// 2i, 2i+1 at `level` -- has pre-image slot 2^th - 2^(th - level) + i.
+__host__ __device__ inline uint64_t tree_pre_slot(uint32_t th, uint32_t level, uint32_t i) {
+    return (1ull << th) - (1ull << (th - level)) + i;
+}
+
+// Address-only part of every internal node of each run's subtree.
+template <class F>
+__global__ __launch_bounds__(WG) void k_pre_tree(const typename F::Key* __restrict__ keys,
+                                                  const DRun* __restrict__ runs, uint32_t th,
+                                                  typename F::HPre* __restrict__ out) {
+    const uint32_t r = blockIdx.y;
+    const uint64_t h = (uint64_t)blockIdx.x * WG + threadIdx.x;
+    const uint64_t per = (1ull << th) - 1;
+    if (h >= per) return;
+    uint32_t level = 0;
+    while (h >= tree_pre_slot(th, level + 1, 0)) ++level;
+    const uint32_t i = (uint32_t)(h - tree_pre_slot(th, level, 0));
+    const DRun run = runs[r];
+    uint32_t a[8] = {run.layer, run.tree_hi, run.tree_lo, ADDR_HASHTREE, 0, level, i, 0};
+    typename F::HPre p;
+    F::pre_h(p, keys[run.key], a);
+    out[(size_t)r * per + h] = p;
+}
+
+// One subtree level (parents at level + 1) from pre-images (k_level's split form).
+template <class F>
+__global__ __launch_bounds__(WG) void k_level_fin(uint32_t th, uint32_t level, uint8_t* __restrict__ nodes,
+                                                   const typename F::HPre* __restrict__ pre) {
+    const uint32_t r = blockIdx.y;
+    const uint32_t i = blockIdx.x * WG + threadIdx.x;
+    if (i >= (1u << (th - level - 1))) return;
+    const uint64_t stride = (2ull << th) - 1;
+    const uint64_t base_c = (2ull << th) - (2ull << (th - level));
+    const uint64_t base_p = (2ull << th) - (2ull << (th - level - 1));
+    uint8_t* T = nodes + (size_t)r * stride * F::kN;
+    typename F::Node L, R, o;
+    F::load(L, T + (base_c + 2 * i) * F::kN);
+    F::load(R, T + (base_c + 2 * i + 1) * F::kN);
+    F::fin_h(o, pre[(size_t)r * ((1ull << th) - 1) + tree_pre_slot(th, level, i)], L, R);
+    F::store(o, T + (base_p + i) * F::kN);
+}
+
+// The top levels from `level0` (at most 64 children there) up to the root,
+// one workgroup of 64 lanes per run, the level in between kept in LDS.
+template <class F>
+__global__ __launch_bounds__(64) void k_top_fin(uint32_t th, uint32_t level0, uint8_t* __restrict__ nodes,
+                                                 const typename F::HPre* __restrict__ pre) {
+    __shared__ typename F::Node buf[64];
+    const uint32_t r = blockIdx.x;
+    const uint32_t t = threadIdx.x;
+    const uint64_t stride = (2ull << th) - 1;
+    const typename F::HPre* hp = pre + (size_t)r * ((1ull << th) - 1);
+    uint8_t* T = nodes + (size_t)r * stride * F::kN;
+    const uint32_t m0 = 1u << (th - level0);  // children at level0
+    if (t < m0) F::load(buf[t], T + ((2ull << th) - (2ull << (th - level0)) + t) * F::kN);
+    __syncthreads();
+    for (uint32_t level = level0; level < th; ++level) {
+        const uint32_t parents = 1u << (th - level - 1);
+        typename F::Node o;
+        if (t < parents) F::fin_h(o, hp[tree_pre_slot(th, level, t)], buf[2 * t], buf[2 * t + 1]);
+        __syncthreads();
+        if (t < parents) {
+            buf[t] = o;
+            F::store(o, T + ((2ull << th) - (2ull << (th - level - 1)) + t) * F::kN);
+        }
+        __syncthreads();
+    }
+}
+
 // ---- the same finish for large verification batches, lane-dense --------
 // k_lat_fin gives each request a workgroup, whose L-tree heights and auth
 // walk leave most lanes idle; for thousands of requests the finish runs as

@@ -124,6 +124,11 @@ struct FamOps {
     virtual void lat_verify_batch(Launcher& L, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
                                   const uint8_t* sigs, const uint8_t* auth, uint32_t th, const uint8_t* scratch,
                                   uint8_t* ends, uint8_t* tmp, uint8_t* out) = 0;
+    // Subtree levels of a few runs split into address-only and finishing
+    // work: pre_tree fills 2^th - 1 pre-images per run (tree_pre_slot
+    // order), levels_fin builds levels 0 .. th-1 from them.
+    virtual void pre_tree(Launcher& L, const void* keys, const DRun* runs, int nruns, uint32_t th, void* pre) = 0;
+    virtual void levels_fin(Launcher& L, int nruns, uint32_t th, uint8_t* nodes, const void* pre) = 0;
     virtual void thash_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, const uint8_t* in,
                          uint8_t* out) = 0;
     virtual void hmsg(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad,
@@ -208,6 +213,18 @@ struct FamOpsT final : FamOps {
              const uint8_t* sigs, const uint8_t* auth, uint32_t th, uint8_t* scratch, uint8_t* out) override {
         lat_pre(L, keys, mode, reqs, n, msgs, th, scratch);
         lat_fin(L, mode, reqs, n, msgs, sigs, auth, th, scratch, out);
+    }
+    void pre_tree(Launcher& L, const void* keys, const DRun* runs, int nruns, uint32_t th, void* pre) override {
+        L.run("pre_tree", k_pre_tree<F>, dim3(cdiv((1ull << th) - 1, WG), nruns), dim3(WG), (const K*)keys, runs, th,
+              (typename F::HPre*)pre);
+    }
+    void levels_fin(Launcher& L, int nruns, uint32_t th, uint8_t* nodes, const void* pre) override {
+        const auto* hp = (const typename F::HPre*)pre;
+        const uint32_t top = th > 6 ? th - 6 : 0;  // from here on one workgroup per run
+        for (uint32_t lev = 0; lev < top; ++lev)
+            L.run("level_fin", k_level_fin<F>, dim3(cdiv(1ull << (th - lev - 1), WG), nruns), dim3(WG), th, lev,
+                  nodes, hp);
+        L.run("top_fin", k_top_fin<F>, dim3(nruns), dim3(64), th, top, nodes, hp);
     }
     void lat_verify_batch(Launcher& L, const DLeaf* reqs, uint32_t n, const uint8_t* msgs, const uint8_t* sigs,
                           const uint8_t* auth, uint32_t th, const uint8_t* scratch, uint8_t* ends, uint8_t* tmp,
