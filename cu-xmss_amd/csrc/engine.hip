@@ -85,6 +85,21 @@ struct DevBuf {
     }
 };
 
+// Grow-only pinned host buffer (hipHostMalloc), for single-copy staging.
+struct PinnedBuf {
+    uint8_t* p = nullptr;
+    size_t cap = 0;
+    uint8_t* need(size_t bytes) {
+        bytes = std::max<size_t>(bytes, 256);
+        if (bytes > cap) {
+            if (p) HIPCK(hipHostFree(p));
+            cap = (std::max(bytes, cap + cap / 2) + 4095) & ~(size_t)4095;
+            HIPCK(hipHostMalloc((void**)&p, cap, hipHostMallocDefault));
+        }
+        return p;
+    }
+};
+
 struct BdsBufs;
 struct VerifyBufs;
 
@@ -103,6 +118,7 @@ struct Engine::Impl {
     const void* kp() const { return kp_override ? kp_override : keys.p; }
     Engine::Batch* spare[2] = {nullptr, nullptr};  // reusable verify / sign batches
     DevBuf keys, seeds, desc, ends, tmp, nodes, a, b, c, d, idx, lat, tpre;
+    PinnedBuf pin;  // staging of batch inputs and results
     std::unordered_map<std::string, int> slots[F_COUNT];  // pool slot by seed bytes (pool_slot)
     int key_fam = -1;
     int nkeys = 0;
@@ -511,9 +527,24 @@ struct Engine::Batch {
     unsigned th = 0, d = 0, n = 0, len = 0;
     int count = 0;
     int nruns = 0;  // subtrees to build (count x d, or fewer when items share trees)
-    DevBuf keys, seeds, msg_desc, blob, R, dig, runs, nodes, ends, tmp, gidx, auth, wdesc, wots, vdesc,
-        roots, ok, pkroot;
+    // inputs, staged in one copy into `io`; results in `res`, copied back in one
+    DevBuf io, res, keys, dig, nodes, ends, tmp, roots;
     size_t nodes_stride = 0;
+    const uint8_t* seeds = nullptr;
+    const DMsg* md = nullptr;
+    uint8_t* blob = nullptr;
+    const DRun* runs = nullptr;
+    const DWots* wd = nullptr;
+    const uint64_t* gidx = nullptr;
+    const DLeaf* vd = nullptr;
+    const uint8_t* sigs = nullptr;     // verify: WOTS signatures, layer-major
+    const uint8_t* auth_in = nullptr;  // verify: auth paths, layer-major
+    const uint8_t* pkroot = nullptr;
+    uint8_t* R = nullptr;     // sign: R | auth records | WOTS signatures, contiguous in res
+    uint8_t* auth = nullptr;
+    uint8_t* wots = nullptr;
+    size_t res_bytes = 0;
+    int* ok = nullptr;        // verify
 };
 
 __global__ __launch_bounds__(WG) void k_compare(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b,
@@ -525,50 +556,59 @@ __global__ __launch_bounds__(WG) void k_compare(const uint8_t* __restrict__ a, c
     ok[i] = eq;
 }
 
-static void stage_msgs(Engine::Impl* I, DevBuf& desc, DevBuf& blob, int fam, int count, bool sign,
-                       const std::vector<MsgReq>& reqs) {
+// Offsets of the parts of one staging area (256-byte aligned).
+struct Layout {
+    size_t size = 0;
+    size_t add(size_t bytes) {
+        const size_t o = size;
+        size += (bytes + 255) & ~(size_t)255;
+        return o;
+    }
+};
+
+// H_msg inputs: descriptors and the prefixed messages (toByte(2) || R ||
+// root || toByte(idx) || M, R filled in on the device for signatures).
+struct MsgStage {
+    std::vector<DMsg> d;
+    size_t total = 0;
+};
+static MsgStage plan_msgs(int fam, int count, bool sign, const std::vector<MsgReq>& reqs) {
     const unsigned n = fam_n(fam), pad = fam_pad(fam);
     const size_t pb = pad + 3 * n;
-    std::vector<DMsg> d(count);
-    size_t total = 0;
+    MsgStage m;
+    m.d.resize(count);
     for (int i = 0; i < count; ++i) {
-        d[i] = {reqs[i].key, sign ? 1u : 0u, reqs[i].idx, (uint64_t)total, (uint32_t)(pb + reqs[i].mlen), 0};
-        total += (pb + reqs[i].mlen + 15) & ~(size_t)15;
+        m.d[i] = {reqs[i].key, sign ? 1u : 0u, reqs[i].idx, (uint64_t)m.total, (uint32_t)(pb + reqs[i].mlen), 0};
+        m.total += (pb + reqs[i].mlen + 15) & ~(size_t)15;
     }
-    total += 8;
-    std::vector<uint8_t> h(total, 0);
-    for (int i = 0; i < count; ++i) {
-        uint8_t* p = &h[d[i].off];
+    m.total += 16;
+    return m;
+}
+static void fill_msgs(uint8_t* h, int fam, bool sign, const std::vector<MsgReq>& reqs, const MsgStage& m) {
+    const unsigned n = fam_n(fam), pad = fam_pad(fam);
+    const size_t pb = pad + 3 * n;
+    memset(h, 0, m.total);
+    for (size_t i = 0; i < m.d.size(); ++i) {
+        uint8_t* p = h + m.d[i].off;
         p[pad - 1] = 2;
         if (!sign) memcpy(p + pad, reqs[i].R, n);
         memcpy(p + pad + n, reqs[i].root, n);
         uint64_t v = reqs[i].idx;
-        for (int b = 0; b < 8; ++b) {
-            p[pad + 3 * n - 1 - b] = (uint8_t)(v & 0xff);
+        for (int k = 0; k < 8; ++k) {
+            p[pad + 3 * n - 1 - k] = (uint8_t)(v & 0xff);
             v >>= 8;
         }
         if (reqs[i].mlen) memcpy(p + pb, reqs[i].m, reqs[i].mlen);
     }
-    DMsg* dd = desc.need<DMsg>(sizeof(DMsg) * count);
-    uint8_t* db = blob.need(total);
-    I->h2d(dd, d.data(), sizeof(DMsg) * count);
-    I->h2d(db, h.data(), total);
 }
-
-static void stage_keys(Engine::Impl* I, Engine::Batch* b, const std::vector<KeySeeds>& ks) {
-    const unsigned n = b->n;
-    std::vector<uint8_t> h((size_t)ks.size() * 3 * n, 0);
+static void fill_seeds(uint8_t* h, unsigned n, const std::vector<KeySeeds>& ks) {
+    memset(h, 0, ks.size() * 3 * n);
     for (size_t i = 0; i < ks.size(); ++i) {
-        uint8_t* d = &h[i * 3 * n];
+        uint8_t* d = h + i * 3 * n;
         if (ks[i].sk_seed) memcpy(d, ks[i].sk_seed, n);
         if (ks[i].sk_prf) memcpy(d + n, ks[i].sk_prf, n);
         if (ks[i].pub_seed) memcpy(d + 2 * n, ks[i].pub_seed, n);
     }
-    uint8_t* ds = b->seeds.need(h.size());
-    I->h2d(ds, h.data(), h.size());
-    FamOps* o = ops_of(b->fam);
-    void* dk = b->keys.need(o->key_bytes() * ks.size());
-    o->prepare(I->L, ds, (int)ks.size(), dk);
 }
 
 static Engine::Batch* take_batch(Engine::Impl* I, bool sign) {
@@ -589,6 +629,7 @@ Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count,
     b->len = fam_len(fam);
     b->count = count;
     const unsigned n = b->n;
+    FamOps* o = ops_of(fam);
     // one_key: every item signs with the same key (consecutive indices), so
     // one key context serves all and items on the same (layer, tree) share
     // that subtree -- it is built once
@@ -599,8 +640,6 @@ Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count,
         if (!one_key || i == 0) ks[i] = {items[i].sk_seed, items[i].sk_prf, items[i].pub_seed};
         mr[i] = {key, 1, items[i].idx, nullptr, items[i].root, items[i].m, items[i].mlen};
     }
-    stage_keys(I, b, ks);
-    stage_msgs(I, b->msg_desc, b->blob, fam, count, true, mr);
     // one subtree per distinct (key, layer, tree): layer i, tree idx >> (th * (i + 1))
     std::vector<DRun> runs;
     std::map<std::pair<uint32_t, uint64_t>, uint32_t> shared;  // one_key: (layer, tree) -> run
@@ -633,17 +672,37 @@ Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count,
         }
     }
     b->nruns = (int)runs.size();
-    DRun* dr = b->runs.need<DRun>(sizeof(DRun) * runs.size());
-    I->h2d(dr, runs.data(), sizeof(DRun) * runs.size());
-    DWots* dw = b->wdesc.need<DWots>(sizeof(DWots) * wd.size());
-    I->h2d(dw, wd.data(), sizeof(DWots) * wd.size());
-    uint64_t* dg = b->gidx.need<uint64_t>(sizeof(uint64_t) * gi.size());
-    I->h2d(dg, gi.data(), sizeof(uint64_t) * gi.size());
-    b->R.need((size_t)count * n);
+    const MsgStage ms = plan_msgs(fam, count, true, mr);
+    Layout in;
+    const size_t o_seeds = in.add(ks.size() * 3 * n), o_md = in.add(sizeof(DMsg) * count), o_blob = in.add(ms.total),
+                 o_runs = in.add(sizeof(DRun) * runs.size()), o_wd = in.add(sizeof(DWots) * wd.size()),
+                 o_gi = in.add(sizeof(uint64_t) * gi.size());
+    uint8_t* h = I->pin.need(in.size);
+    fill_seeds(h + o_seeds, n, ks);
+    memcpy(h + o_md, ms.d.data(), sizeof(DMsg) * count);
+    fill_msgs(h + o_blob, fam, true, mr, ms);
+    memcpy(h + o_runs, runs.data(), sizeof(DRun) * runs.size());
+    memcpy(h + o_wd, wd.data(), sizeof(DWots) * wd.size());
+    memcpy(h + o_gi, gi.data(), sizeof(uint64_t) * gi.size());
+    uint8_t* dv = b->io.need(in.size);
+    I->h2d(dv, h, in.size);
+    b->seeds = dv + o_seeds;
+    b->md = reinterpret_cast<const DMsg*>(dv + o_md);
+    b->blob = dv + o_blob;
+    b->runs = reinterpret_cast<const DRun*>(dv + o_runs);
+    b->wd = reinterpret_cast<const DWots*>(dv + o_wd);
+    b->gidx = reinterpret_cast<const uint64_t*>(dv + o_gi);
+    o->prepare(I->L, b->seeds, (int)ks.size(), b->keys.need(o->key_bytes() * ks.size()));
+    Layout out;
+    const size_t o_R = out.add((size_t)count * n), o_auth = out.add(gi.size() * n),
+                 o_w = out.add((size_t)count * d * b->len * n);
+    uint8_t* dr = b->res.need(out.size);
+    b->R = dr + o_R;
+    b->auth = dr + o_auth;
+    b->wots = dr + o_w;
+    b->res_bytes = out.size;
     b->dig.need((size_t)count * n);
     b->nodes.need(b->nodes_stride * n * (size_t)b->nruns);
-    b->auth.need(gi.size() * n);
-    b->wots.need((size_t)count * d * b->len * n);
     b->roots.need((size_t)count * d * n);  // layer-major roots
     I->sync();
     return b;
@@ -660,16 +719,23 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
     b->len = fam_len(fam);
     b->count = count;
     const unsigned n = b->n, len = b->len;
+    FamOps* o = ops_of(fam);
     std::vector<KeySeeds> ks(count);
     std::vector<MsgReq> mr(count);
     for (int i = 0; i < count; ++i) {
         ks[i] = {nullptr, nullptr, items[i].pk + n};
         mr[i] = {(uint32_t)i, 0, items[i].idx, items[i].R, items[i].pk, items[i].m, items[i].mlen};
     }
-    stage_keys(I, b, ks);
-    stage_msgs(I, b->msg_desc, b->blob, fam, count, false, mr);
-    std::vector<DLeaf> vd((size_t)count * d);
-    std::vector<uint8_t> sigs((size_t)count * d * len * n), auth((size_t)count * d * th * n), pkr((size_t)count * n);
+    const MsgStage ms = plan_msgs(fam, count, false, mr);
+    Layout in;
+    const size_t o_seeds = in.add(ks.size() * 3 * n), o_md = in.add(sizeof(DMsg) * count), o_blob = in.add(ms.total),
+                 o_vd = in.add(sizeof(DLeaf) * count * d), o_sig = in.add((size_t)count * d * len * n),
+                 o_auth = in.add((size_t)count * d * th * n), o_pk = in.add((size_t)count * n);
+    uint8_t* h = I->pin.need(in.size);
+    fill_seeds(h + o_seeds, n, ks);
+    memcpy(h + o_md, ms.d.data(), sizeof(DMsg) * count);
+    fill_msgs(h + o_blob, fam, false, mr, ms);
+    DLeaf* vd = reinterpret_cast<DLeaf*>(h + o_vd);
     for (int i = 0; i < count; ++i) {
         uint64_t idx = items[i].idx;
         const uint8_t* p = items[i].wots;
@@ -678,24 +744,27 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
             idx >>= th;
             const size_t t = (size_t)L * count + i;
             vd[t] = {(uint32_t)i, L, (uint32_t)(idx >> 32), (uint32_t)idx, leaf, th, LAT_ALL_STEPS, (uint32_t)(i * th)};
-            memcpy(&sigs[t * len * n], p, (size_t)len * n);
+            memcpy(h + o_sig + t * len * n, p, (size_t)len * n);
             p += (size_t)len * n;
-            memcpy(&auth[t * th * n], p, (size_t)th * n);
+            memcpy(h + o_auth + t * th * n, p, (size_t)th * n);
             p += (size_t)th * n;
         }
-        memcpy(&pkr[(size_t)i * n], items[i].pk, n);
+        memcpy(h + o_pk + (size_t)i * n, items[i].pk, n);
     }
-    DLeaf* dv = b->vdesc.need<DLeaf>(sizeof(DLeaf) * vd.size());
-    I->h2d(dv, vd.data(), sizeof(DLeaf) * vd.size());
-    uint8_t* dws = b->wots.need(sigs.size());
-    I->h2d(dws, sigs.data(), sigs.size());
-    uint8_t* da = b->auth.need(auth.size());
-    I->h2d(da, auth.data(), auth.size());
-    uint8_t* dp = b->pkroot.need(pkr.size());
-    I->h2d(dp, pkr.data(), pkr.size());
+    uint8_t* dv = b->io.need(in.size);
+    I->h2d(dv, h, in.size);
+    b->seeds = dv + o_seeds;
+    b->md = reinterpret_cast<const DMsg*>(dv + o_md);
+    b->blob = dv + o_blob;
+    b->vd = reinterpret_cast<const DLeaf*>(dv + o_vd);
+    b->sigs = dv + o_sig;
+    b->auth_in = dv + o_auth;
+    b->pkroot = dv + o_pk;
+    o->prepare(I->L, b->seeds, count, b->keys.need(o->key_bytes() * count));
+    b->ok = b->res.need<int>(sizeof(int) * (size_t)count);
+    b->res_bytes = sizeof(int) * (size_t)count;
     b->dig.need((size_t)count * n);
     b->roots.need((size_t)count * d * n);
-    b->ok.need(sizeof(int) * (size_t)count);
     I->sync();
     return b;
 }
@@ -721,12 +790,11 @@ void Engine::run(Batch* b, bool wait_done) {
         HIPCK(hipEventRecord(staged, I->L.s_main));
         HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
         I->L.s = I->L.s_aux;
-        o->lat_pre(I->L, dk, LAT_VERIFY, (const DLeaf*)b->vdesc.p, (uint32_t)nv, nullptr, th, vscratch);
+        o->lat_pre(I->L, dk, LAT_VERIFY, b->vd, (uint32_t)nv, nullptr, th, vscratch);
         HIPCK(hipEventRecord(pre, I->L.s_aux));
         I->L.s = I->L.s_main;
     }
-    o->hmsg(I->L, dk, (const DMsg*)b->msg_desc.p, (uint32_t)count, fam_pad(b->fam), (uint8_t*)b->blob.p,
-            (uint8_t*)b->R.p, (uint8_t*)b->dig.p);
+    o->hmsg(I->L, dk, b->md, (uint32_t)count, fam_pad(b->fam), b->blob, b->R, (uint8_t*)b->dig.p);
     if (vlat) HIPCK(hipStreamWaitEvent(I->L.s_main, pre, 0));
     if (b->sign) {
         // all subtrees in one pipeline (few: levels split as in build_trees)
@@ -740,7 +808,7 @@ void Engine::run(Batch* b, bool wait_done) {
             HIPCK(hipEventRecord(tstaged, I->L.s_main));
             HIPCK(hipStreamWaitEvent(I->L.s_aux, tstaged, 0));
             I->L.s = I->L.s_aux;
-            o->pre_tree(I->L, dk, (const DRun*)b->runs.p, nruns, th, tpre);
+            o->pre_tree(I->L, dk, b->runs, nruns, th, tpre);
             HIPCK(hipEventRecord(tdone, I->L.s_aux));
             I->L.s = I->L.s_main;
         }
@@ -748,7 +816,7 @@ void Engine::run(Batch* b, bool wait_done) {
         const uint64_t per_leaf = (uint64_t)len * n;
         const uint64_t stride = b->nodes_stride;
         uint8_t* nodes = (uint8_t*)b->nodes.p;
-        const DRun* drun = (const DRun*)b->runs.p;
+        const DRun* drun = b->runs;
         for (int r0 = 0; r0 < nruns;) {
             int rg = std::min<int>(nruns - r0, 65535);
             uint64_t lpp = leaves;
@@ -787,7 +855,7 @@ void Engine::run(Batch* b, bool wait_done) {
         // auth paths and roots: auth[t] = th auth nodes || root, t = layer * count + item
         const uint32_t ng = (uint32_t)((size_t)count * d * (th + 1));
         I->L.run("gather", k_gather, dim3(cdiv((uint64_t)ng * (n / 4), WG)), dim3(WG), (const uint8_t*)nodes,
-                 (const uint64_t*)b->gidx.p, ng, n, (uint8_t*)b->auth.p);
+                 b->gidx, ng, n, b->auth);
         // WOTS signatures layer by layer; layer L signs the root of layer L-1
         // (the digest for layer 0).  Roots sit at the end of each auth record.
         uint8_t* msgs = (uint8_t*)b->roots.p;
@@ -799,19 +867,19 @@ void Engine::run(Batch* b, bool wait_done) {
             } else {
                 // compact roots of layer L-1 into msgs[L-1]
                 uint8_t* dst = msgs + (size_t)(L - 1) * count * n;
-                HIPCK(hipMemcpy2DAsync(dst, n, (const uint8_t*)b->auth.p + ((size_t)(L - 1) * count) * rec + th * n,
+                HIPCK(hipMemcpy2DAsync(dst, n, b->auth + ((size_t)(L - 1) * count) * rec + th * n,
                                        rec, n, count, hipMemcpyDeviceToDevice, I->L.s));
                 msg_l = dst;
             }
-            o->wots(I->L, dk, (const DWots*)b->wdesc.p + (size_t)L * count, (uint32_t)count, msg_l, nullptr,
-                    (uint8_t*)b->wots.p + (size_t)L * count * len * n);
+            o->wots(I->L, dk, b->wd + (size_t)L * count, (uint32_t)count, msg_l, nullptr,
+                    b->wots + (size_t)L * count * len * n);
         }
     } else {
         for (unsigned L = 0; L < d; ++L) {
             const uint8_t* msg_l = L == 0 ? (const uint8_t*)b->dig.p : (const uint8_t*)b->roots.p + (size_t)(L - 1) * count * n;
-            const DLeaf* vd = (const DLeaf*)b->vdesc.p + (size_t)L * count;
-            const uint8_t* sig_l = (const uint8_t*)b->wots.p + (size_t)L * count * len * n;
-            const uint8_t* auth_l = (const uint8_t*)b->auth.p + (size_t)L * count * th * n;
+            const DLeaf* vd = b->vd + (size_t)L * count;
+            const uint8_t* sig_l = b->sigs + (size_t)L * count * len * n;
+            const uint8_t* auth_l = b->auth_in + (size_t)L * count * th * n;
             uint8_t* root_l = (uint8_t*)b->roots.p + (size_t)L * count * n;
             const uint8_t* scr_l = vscratch + (size_t)L * count * o->lat_bytes(th);
             if (vlat && count >= kLatDense)
@@ -821,14 +889,10 @@ void Engine::run(Batch* b, bool wait_done) {
             else if (vlat)
                 o->lat_fin(I->L, LAT_VERIFY, vd, (uint32_t)count, msg_l, sig_l, auth_l, th, scr_l, root_l);
             else
-                o->leaf_wg(I->L, dk, (const DLeaf*)b->vdesc.p + (size_t)L * count, (uint32_t)count, true, msg_l,
-                           (const uint8_t*)b->wots.p + (size_t)L * count * len * n,
-                           (const uint8_t*)b->auth.p + (size_t)L * count * th * n, th,
-                           (uint8_t*)b->roots.p + (size_t)L * count * n);
+                o->leaf_wg(I->L, dk, vd, (uint32_t)count, true, msg_l, sig_l, auth_l, th, root_l);
         }
         I->L.run("compare", k_compare, dim3(cdiv(count, WG)), dim3(WG),
-                 (const uint8_t*)b->roots.p + (size_t)(d - 1) * count * n, (const uint8_t*)b->pkroot.p,
-                 (uint32_t)count, n, (int*)b->ok.p);
+                 (const uint8_t*)b->roots.p + (size_t)(d - 1) * count * n, b->pkroot, (uint32_t)count, n, b->ok);
     }
     if (vlat) {
         I->L.pool.push_back(staged);
@@ -841,20 +905,21 @@ void Engine::collect_sign(Batch* b, uint8_t* const* out) {
     Impl* I = impl_;
     const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
     const int count = b->count;
-    std::vector<uint8_t> R((size_t)count * n), auth((size_t)count * d * (th + 1) * n), w((size_t)count * d * len * n);
-    I->d2h(R.data(), b->R.p, R.size());
-    I->d2h(auth.data(), b->auth.p, auth.size());
-    I->d2h(w.data(), b->wots.p, w.size());
+    uint8_t* h = I->pin.need(b->res_bytes);
+    I->d2h(h, b->res.p, b->res_bytes);
     I->sync();
+    const uint8_t* R = h + (b->R - (uint8_t*)b->res.p);
+    const uint8_t* auth = h + (b->auth - (uint8_t*)b->res.p);
+    const uint8_t* w = h + (b->wots - (uint8_t*)b->res.p);
     for (int i = 0; i < count; ++i) {
         uint8_t* p = out[i];
-        memcpy(p, &R[(size_t)i * n], n);
+        memcpy(p, R + (size_t)i * n, n);
         p += n;
         for (unsigned L = 0; L < d; ++L) {
             const size_t t = (size_t)L * count + i;
-            memcpy(p, &w[t * len * n], (size_t)len * n);
+            memcpy(p, w + t * len * n, (size_t)len * n);
             p += (size_t)len * n;
-            memcpy(p, &auth[t * (th + 1) * n], (size_t)th * n);
+            memcpy(p, auth + t * (th + 1) * n, (size_t)th * n);
             p += (size_t)th * n;
         }
     }
@@ -862,7 +927,7 @@ void Engine::collect_sign(Batch* b, uint8_t* const* out) {
 
 void Engine::collect_verify(Batch* b, int* ok) {
     Impl* I = impl_;
-    I->d2h(ok, b->ok.p, sizeof(int) * (size_t)b->count);
+    I->d2h(ok, b->ok, sizeof(int) * (size_t)b->count);
     I->sync();
 }
 
@@ -871,9 +936,7 @@ void Engine::free_batch(Batch* b) {
         impl_->spare[b->sign] = b;
         return;
     }
-    DevBuf* bufs[] = {&b->keys, &b->seeds, &b->msg_desc, &b->blob, &b->R, &b->dig, &b->runs, &b->nodes,
-                      &b->ends, &b->tmp, &b->gidx, &b->auth, &b->wdesc, &b->wots, &b->vdesc, &b->roots,
-                      &b->ok, &b->pkroot};
+    DevBuf* bufs[] = {&b->io, &b->res, &b->keys, &b->dig, &b->nodes, &b->ends, &b->tmp, &b->roots};
     for (DevBuf* x : bufs) {
         if (x->p) HIPCK(hipFree(x->p));
         x->p = nullptr;
@@ -885,20 +948,6 @@ void Engine::free_batch(Batch* b) {
 
 namespace xm {
 
-// Grow-only pinned host buffer (hipHostMalloc), for single-copy staging.
-struct PinnedBuf {
-    uint8_t* p = nullptr;
-    size_t cap = 0;
-    uint8_t* need(size_t bytes) {
-        bytes = std::max<size_t>(bytes, 256);
-        if (bytes > cap) {
-            if (p) HIPCK(hipHostFree(p));
-            cap = (std::max(bytes, cap + cap / 2) + 4095) & ~(size_t)4095;
-            HIPCK(hipHostMalloc((void**)&p, cap, hipHostMallocDefault));
-        }
-        return p;
-    }
-};
 
 struct BdsBufs {
     DevBuf io, leaf_arena, dig, scratch, lat_leaf, lat_wots, merge_arena;
