@@ -383,6 +383,7 @@ def bench_serve(rate, seconds, nkeys, budget):
     pending = queue.Queue()
     done_sigs = []  # (key, sm) available for verification
     lat = {"s": [], "v": []}
+    by_set = {}  # (kind, parameter set) -> latencies
     fails = [0]
 
     def collect():
@@ -398,6 +399,7 @@ def bench_serve(rate, seconds, nkeys, budget):
             if rc != 0:
                 fails[0] += 1
             lat[kind].append(ms.value)
+            by_set.setdefault((kind, SERVE_SETS[k % 3][0]), []).append(ms.value)
             if kind == "s" and rc == 0:
                 done_sigs.append((k, raw(out, olen.value)))
 
@@ -433,6 +435,9 @@ def bench_serve(rate, seconds, nkeys, budget):
             "sign_p50_ms": round(q(lat["s"], 0.5), 3), "sign_p99_ms": round(q(lat["s"], 0.99), 3),
             "verify_p50_ms": round(q(lat["v"], 0.5), 3) if lat["v"] else None,
             "verify_p99_ms": round(q(lat["v"], 0.99), 3) if lat["v"] else None,
+            "by_set_p50_p99_ms": {"%s %s" % ("sign" if kd == "s" else "verify", nm): [round(q(v, 0.5), 3),
+                                                                                   round(q(v, 0.99), 3)]
+                                  for (kd, nm), v in sorted(by_set.items())},
             "hbm_budget_bytes": budget, "peak_reserved_bytes": peak.value, "admission_waits": waits.value,
             "device_calls": calls.value, "launch_groups": launches.value, "key_setup_s": round(keygen_s, 2)}
 

@@ -39,6 +39,9 @@ using namespace xm;
 namespace {
 
 using Clock = std::chrono::steady_clock;
+// A lane waits at most this long for running requests to queue their device
+// calls before it launches what it has.
+constexpr std::chrono::microseconds kBatchWindow(200);
 
 // ---------------------------------------------------------------- dispatcher
 struct Call {
@@ -139,7 +142,7 @@ private:
     int ready(const Lane& l) const {
         for (int q = 0; q < NQ; ++q) {
             if (!(l.queues & (1u << q)) || queue_[q].empty()) continue;
-            if (waiting_ >= active_ || Clock::now() - first_[q] >= std::chrono::microseconds(200)) return q;
+            if (waiting_ >= active_ || Clock::now() - first_[q] >= kBatchWindow) return q;
         }
         return -1;
     }
@@ -148,7 +151,7 @@ private:
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
             int q = -1;
-            cv_.wait_for(lk, std::chrono::microseconds(200), [&] { return stop_ || (q = ready(l)) >= 0; });
+            cv_.wait_for(lk, kBatchWindow, [&] { return stop_ || (q = ready(l)) >= 0; });
             if (q < 0) q = ready(l);
             if (q < 0) {
                 if (stop_) return;
