@@ -111,6 +111,13 @@ public:
     // Hands a private engine back for reuse by a later make_private.
     static void release_private(Engine* e);
 
+    // Device memory that outlives calls (resident key state of the serving
+    // loop): allocation, release and a synchronous host-to-device copy on the
+    // current device.  device_alloc returns nullptr when the allocation fails.
+    static void* device_alloc(size_t bytes);
+    static void device_free(void* p);
+    static void upload(void* dst, const void* src, size_t bytes);
+
     void set_keys(int fam, int nkeys, const KeySeeds* seeds);
 
     // Device-resident key contexts that outlive a call (the serving pool):
@@ -192,6 +199,8 @@ public:
         const uint8_t* pub_root;
         uint32_t key;          // pool slot (use_pool)
         uint8_t* sk;           // OID-less sk, index already advanced
+        uint8_t* dsk;          // resident copy of the sk in HBM (index not yet advanced), or null:
+                               // the update runs on it in place and sk receives the result
         uint64_t idx;          // index being signed
         const uint8_t* m;
         uint64_t mlen;

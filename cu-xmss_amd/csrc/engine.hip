@@ -196,6 +196,20 @@ void Engine::release_private(Engine* e) {
     private_free().push_back(e);
 }
 
+void* Engine::device_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+    return p;
+}
+
+void Engine::device_free(void* p) {
+    if (p) HIPCK(hipFree(p));
+}
+
+void Engine::upload(void* dst, const void* src, size_t bytes) {
+    if (bytes) HIPCK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+
 void Engine::set_profiling(bool on) {
     prof_on_ = on;
     impl_->L.prof = on;
@@ -1005,6 +1019,8 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
         jd[i].idx = jobs[i].idx;
         jd[i].key = jobs[i].key;
         jd[i].nmerge = jobs[i].nmerge;
+        jd[i].res = reinterpret_cast<uint64_t>(jobs[i].dsk);
+        jd[i].sk_bytes = (uint32_t)s.sk_bytes;
         BdsP& P = jd[i].P;
         P = {s.th, s.d, s.full_height, s.index_bytes, s.bds_k, s.wots_sig_bytes, s.sig_bytes, 0};
         P.state_bytes = (s.th + 1) * n + 4 + s.th + 1 + s.th * n + (s.th >> 1) * n + (s.th - s.bds_k) * (7 + n) +
@@ -1051,7 +1067,7 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
         const uint64_t tree = jobs[i].idx >> th;
         wd[i] = {jobs[i].key, 0, (uint32_t)(tree >> 32), (uint32_t)tree,
                  (uint32_t)(jobs[i].idx & ((1ull << th) - 1)), 0, 0, 0};
-        memcpy(h + o_sk + jd[i].sk_off, jobs[i].sk, jobs[i].shape.sk_bytes);
+        if (!jobs[i].dsk) memcpy(h + o_sk + jd[i].sk_off, jobs[i].sk, jobs[i].shape.sk_bytes);
     }
     const DLeaf* dld = reinterpret_cast<const DLeaf*>(dv + o_ld);
     const DMsg* dmd = reinterpret_cast<const DMsg*>(dv + o_md);

@@ -33,6 +33,9 @@ struct DBds {
     uint32_t nmerge;      // planned merges of this job
     uint64_t merge_off;   // first planned merge (F::HPre) in the merge arena
     BdsP P;               // parameter shape of this key
+    uint64_t res;         // device address of the key's resident sk (index not advanced), or 0
+    uint32_t sk_bytes;    // OID-less sk size
+    uint32_t unused;
 };
 
 template <class F>
@@ -332,18 +335,16 @@ __global__ __launch_bounds__(WG) void k_pre_h(const typename F::Key* __restrict_
 // The state part of one BDS signature per workgroup (everything after R, H_msg
 // and the layer-0 WOTS signature, which run in their own launches): the auth
 // paths and lower-layer WOTS signatures go to the signature tail, then the
-// traversal updates the state.  `scratch` holds 4n bytes per workgroup.
+// traversal updates the state at skg.  `scratch` holds 4n bytes per workgroup.
 template <class F>
-__global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __restrict__ keys, const DBds* __restrict__ jobs,
-                             uint8_t* __restrict__ sk_arena, const uint8_t* __restrict__ leaf_arena,
-                             const typename F::HPre* __restrict__ merge_arena,
-                             uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena) {
+__device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint8_t* skg,
+                               const uint8_t* __restrict__ leaf_arena,
+                               const typename F::HPre* __restrict__ merge_arena, uint8_t* __restrict__ tails,
+                               uint8_t* __restrict__ scratch_arena) {
     extern __shared__ uint8_t lds_states[];
-    const DBds job = jobs[blockIdx.x];
     const BdsP& P = job.P;
-    uint8_t* skg = sk_arena + job.sk_off;
     BdsDev<F> B(lds_states, skg, P, leaf_arena + job.leaf_off * F::kN, merge_arena + job.merge_off, job.nmerge,
-                keys[job.key]);
+                key);
     uint8_t* scratch = scratch_arena + (size_t)blockIdx.x * 4 * F::kN;
     uint8_t* tail = tails + job.tail_off;
     const uint32_t n = F::kN, th = P.th, d = P.d;
@@ -421,6 +422,35 @@ __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __res
             needswap_upto = (int)i;
             for (uint32_t j = 0; j < th - P.bds_k; ++j) B.wr(B.o_th(i, j) + 6, 1, 1);
         }
+    }
+}
+
+// One signature's state update per workgroup.  The sk is staged in the sk
+// arena; for a key whose state is resident in HBM (job.res), the arena copy
+// is filled from it first, with the index advanced as the signature requires
+// (idx + 1, big-endian), and the updated bytes go back to it at the end, so
+// the arena holds the new sk for the host's copy either way.
+template <class F>
+__global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __restrict__ keys, const DBds* __restrict__ jobs,
+                             uint8_t* __restrict__ sk_arena, const uint8_t* __restrict__ leaf_arena,
+                             const typename F::HPre* __restrict__ merge_arena,
+                             uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena) {
+    const DBds job = jobs[blockIdx.x];
+    uint8_t* skg = sk_arena + job.sk_off;
+    uint8_t* res = reinterpret_cast<uint8_t*>(job.res);
+    if (res) {
+        for (uint32_t b = threadIdx.x; b < job.sk_bytes; b += blockDim.x) skg[b] = res[b];
+        __syncthreads();
+        if (threadIdx.x < job.P.index_bytes) {
+            const uint32_t sh = 8 * (job.P.index_bytes - 1 - threadIdx.x);
+            skg[threadIdx.x] = sh < 64 ? (uint8_t)((job.idx + 1) >> sh) : 0;
+        }
+        __syncthreads();
+    }
+    bds_update_one<F>(keys[job.key], job, skg, leaf_arena, merge_arena, tails, scratch_arena);
+    if (res) {
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < job.sk_bytes; b += blockDim.x) res[b] = skg[b];
     }
 }
 

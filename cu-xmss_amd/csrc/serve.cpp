@@ -11,11 +11,18 @@
 // others are on the GPU joins the next launch.  Signatures of one key run in
 // arrival order (its sk is a sequential state); other keys are independent.
 //
-// Key contexts (the PRF midstates of SK_SEED, SK_PRF, PUB_SEED) live in an
-// HBM pool for the lifetime of the key.  Admission: every request reserves its
-// device working set (message, signature, planned leaves, and the key context
-// the first time its key is placed) against a fixed HBM budget and waits in
-// FIFO order while the budget is exhausted.
+// Key state is resident in HBM for the lifetime of the key: the key context
+// (the PRF midstates of SK_SEED, SK_PRF, PUB_SEED) in the device key pool and
+// the serialized sk (index, seeds, BDS states, XMSS^MT WOTS signatures) in a
+// buffer of its own, which the state-update kernel advances in place.  The
+// server keeps a byte-identical host mirror of every sk (each signature's
+// result copy brings the updated bytes back): the host plans the next
+// signature's leaves from it, serves xmss_mi355x_server_get_sk from it, and
+// runs the last-index / exhausted path on it (then uploads it again).
+// Admission: adding a key reserves its resident bytes; every request
+// reserves its device working set (message, signature, planned leaves)
+// against the same fixed HBM budget and waits in FIFO order while the budget
+// is exhausted.
 #include <string.h>
 
 #include <atomic>
@@ -363,8 +370,10 @@ struct Key {
     int mt = 0;
     xmss_params p{};
     int fam = -1;
-    std::vector<unsigned char> sk, pk;  // OID-prefixed
+    std::vector<unsigned char> sk, pk;  // OID-prefixed; sk is the host mirror of dsk
+    unsigned char* dsk = nullptr;       // the resident OID-less sk in HBM
     int slot = -1;                      // pool slot of the key's own seeds
+    std::vector<unsigned char> seeds;   // SK_SEED || SK_PRF || PUB_SEED the slot was made from
     bool busy = false;
 };
 
@@ -392,18 +401,21 @@ struct xmss_mi355x_server {
     std::deque<Request*> queue;  // arrival order
     std::map<uint64_t, Request*> reqs;
     uint64_t next_id = 1;
-    size_t budget = 0, reserved = 0, peak = 0;
+    // resident: key contexts and sks of the added keys; transient: the
+    // working sets of running requests; peak: of their sum
+    size_t budget = 0, resident = 0, transient = 0, peak = 0;
     uint64_t admission_waits = 0;
     bool stop = false;
     std::vector<std::thread> workers;
 
     // The request to run next: the oldest one whose key is free, if the
-    // budget admits it (FIFO: a request that does not fit holds back the rest).
+    // budget admits it (FIFO: a request that does not fit holds back the rest;
+    // with no request running, the oldest runs whatever its size).
     Request* pick() {
         for (auto it = queue.begin(); it != queue.end(); ++it) {
             Request* r = *it;
             if (r->sign && keys[r->key].busy) continue;
-            if (reserved + r->reserve > budget && reserved > 0) {
+            if (resident + transient + r->reserve > budget && transient > 0) {
                 ++admission_waits;
                 return nullptr;
             }
@@ -420,8 +432,8 @@ struct xmss_mi355x_server {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] { return stop || (r = pick()) != nullptr; });
                 if (!r) return;
-                reserved += r->reserve;
-                peak = std::max(peak, reserved);
+                transient += r->reserve;
+                peak = std::max(peak, resident + transient);
                 if (r->sign) keys[r->key].busy = true;
             }
             // the worker counts as active (more device work coming) from
@@ -433,7 +445,7 @@ struct xmss_mi355x_server {
             bool more;
             {
                 std::lock_guard<std::mutex> lk(mu);
-                reserved -= r->reserve;
+                transient -= r->reserve;
                 if (r->sign) keys[r->key].busy = false;
                 r->t1 = Clock::now();
                 r->done = true;
@@ -444,18 +456,16 @@ struct xmss_mi355x_server {
         }
     }
 
+    // The pool slot for the seeds a signature asks for: the key's own slot,
+    // or -- for the last index, which signs with the wiped (zero) seeds -- the
+    // pool's slot of those seeds.  Compared with the seeds the key's slot was
+    // made from, not with the sk, which the signature has already wiped.
     int slot_for(Key& k, const unsigned char* a, const unsigned char* b, const unsigned char* c) {
-        const unsigned n = k.p.n, ib = k.p.index_bytes;
-        const unsigned char* s = k.pk.data();  // unused; seeds come from sk
-        (void)s;
+        const unsigned n = k.p.n;
         std::lock_guard<std::mutex> elk(engine_mutex());
-        if (k.slot >= 0 && !memcmp(a, seeds_of(k).data(), n) && !memcmp(b, seeds_of(k).data() + n, n) &&
-            !memcmp(c, seeds_of(k).data() + 2 * n, n))
+        if (k.slot >= 0 && !memcmp(a, &k.seeds[0], n) && !memcmp(b, &k.seeds[n], n) && !memcmp(c, &k.seeds[2 * n], n))
             return k.slot;
-        (void)ib;
-        // a wiped key (last index) signs with zero seeds: a context of its own
-        KeySeeds ks{a, b, c};
-        return e->pool_add(k.fam, ks);
+        return e->pool_slot(k.fam, {a, b, c});
     }
     std::vector<unsigned char> seeds_of(const Key& k) {
         const unsigned n = k.p.n, ib = k.p.index_bytes;
@@ -477,6 +487,7 @@ struct xmss_mi355x_server {
             BdsDeviceSign ds;
             if (bds_device_prepare(&p, sk, r->out.data(), r->data.data(), r->data.size(), ds)) {
                 ds.job.key = (uint32_t)k.slot;
+                ds.job.dsk = k.dsk;
                 Call call;
                 call.kind = Call::BDS;
                 call.fam = k.fam;
@@ -500,6 +511,7 @@ struct xmss_mi355x_server {
             };
             ops.plan = bds_plan_leaves(&p, sk, r->data.size());
             r->ret = bds_sign(&p, sk, r->out.data(), &r->outlen, r->data.data(), r->data.size(), ops);
+            Engine::upload(k.dsk, sk, p.sk_bytes);  // the mirror changed on the host
         } else {
             const unsigned long long smlen = r->data.size();
             r->out.assign(smlen + 1, 0);
@@ -564,17 +576,29 @@ int xmss_mi355x_server_add_key(xmss_mi355x_server* s, int mt, const unsigned cha
     if (k.fam < 0) return -1;
     k.sk.assign(sk, sk + XMSS_OID_LEN + k.p.sk_bytes);
     k.pk.assign(pk, pk + XMSS_OID_LEN + k.p.pk_bytes);
-    std::vector<unsigned char> seeds = s->seeds_of(k);
+    const size_t res_bytes = s->e->pool_key_bytes(k.fam) + k.p.sk_bytes;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->resident + res_bytes > s->budget) return -1;  // the budget cannot hold the key
+        s->resident += res_bytes;
+        s->peak = std::max(s->peak, s->resident + s->transient);
+    }
+    k.dsk = static_cast<unsigned char*>(Engine::device_alloc(k.p.sk_bytes));
+    if (!k.dsk) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->resident -= res_bytes;
+        return -1;
+    }
+    Engine::upload(k.dsk, k.sk.data() + XMSS_OID_LEN, k.p.sk_bytes);
+    k.seeds = s->seeds_of(k);
     const unsigned n = k.p.n;
-    KeySeeds ks{&seeds[0], &seeds[n], &seeds[2 * n]};
+    KeySeeds ks{&k.seeds[0], &k.seeds[n], &k.seeds[2 * n]};
     {
         std::lock_guard<std::mutex> elk(engine_mutex());
         k.slot = s->e->pool_add(k.fam, ks);
     }
     std::lock_guard<std::mutex> lk(s->mu);
     s->keys.push_back(std::move(k));
-    s->reserved += s->e->pool_key_bytes(s->keys.back().fam);  // resident key context
-    s->peak = std::max(s->peak, s->reserved);
     return (int)s->keys.size() - 1;
 }
 
@@ -650,6 +674,7 @@ void xmss_mi355x_server_destroy(xmss_mi355x_server* s) {
     for (auto& t : s->workers) t.join();
     s->disp.reset();
     for (auto& kv : s->reqs) delete kv.second;
+    for (Key& k : s->keys) Engine::device_free(k.dsk);
     delete s;
 }
 

@@ -111,13 +111,17 @@ int xmss_batch_sign_open(int mt, unsigned char *const *ms, unsigned long long *m
  * libxmss_mi355x_fast writes it) and public key; the server owns the key from
  * then on.  Signatures of one key run in submission order and update its sk
  * exactly as xmss[mt]_sign would; requests of different keys share GPU
- * launches.  Every request reserves its device working set against
- * hbm_budget_bytes (key contexts stay resident) and waits while the budget is
+ * launches.  Each key's state (key context and serialized sk) stays resident
+ * in HBM and is advanced there; the server keeps a byte-identical host copy.
+ * Resident keys count against hbm_budget_bytes, and every request reserves
+ * its device working set against what is left and waits while the budget is
  * used up.  Results come back with their latency (submission to completion). */
 typedef struct xmss_mi355x_server xmss_mi355x_server;
 
 xmss_mi355x_server *xmss_mi355x_server_create(size_t hbm_budget_bytes, int workers);
-/* Returns the key id, or -1 for an unknown OID. */
+/* Returns the key id, or -1 for an unknown OID, when the budget left by the
+ * keys already added cannot hold the key's resident state, or when that HBM
+ * cannot be allocated. */
 int xmss_mi355x_server_add_key(xmss_mi355x_server *s, int mt, const unsigned char *sk,
                                const unsigned char *pk);
 /* Return request ids (0 for an unknown key). */

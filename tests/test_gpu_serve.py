@@ -56,7 +56,9 @@ def _alone(lib, keys, msgs):
     return out
 
 
-@pytest.mark.parametrize("budget", [1 << 30, 40000])
+# the small budget holds the six keys' resident state (53,320 sk bytes plus
+# key contexts) and about 45 KB of request working sets
+@pytest.mark.parametrize("budget", [1 << 30, 100000])
 def test_serve_mix_equals_alone(budget):
     lib = product(1)
     L = lib.lib
@@ -107,4 +109,78 @@ def test_serve_mix_equals_alone(budget):
     L.xmss_mi355x_server_destroy(srv)
     assert calls.value > launches.value  # calls of different keys shared launches
     if budget < 1 << 20:
-        assert waits.value > 0 and peak.value <= budget + 20000
+        assert waits.value > 0 and peak.value <= budget + 30000
+
+
+def test_serve_key_to_its_end():
+    """A resident XMSS-SHA2_10_256 key signed through the server across its
+    last indices: 1020-1022 take the device path, 1023 (the wiped key, still
+    returning 0) and the exhausted calls after it (-2) take the host path,
+    which re-uploads the resident state; signatures, return codes and the
+    final sk equal the key signing alone."""
+    lib = product(1)
+    L = lib.lib
+    _server_api(L)
+    lib.lib.xmss_mi355x_rng_seed(4242)
+    p = lib.params(0x01, 0)
+    rc, pk, sk = lib.keypair(0x01, 0, sk_init=bytes(4 + p.sk_bytes))
+    assert rc == 0
+    for i in range(1020):
+        rc, _, sk = lib.sign(sk, b"%d" % i, 0)
+        assert rc == 0
+    rng = random.Random(5)
+    msgs = [bytes(rng.getrandbits(8) for _ in range(40 + 7 * j)) for j in range(6)]
+    want, ska = [], sk
+    for m in msgs:
+        rc, sm, ska = lib.sign(ska, m, 0)
+        want.append((rc, sm))
+    assert [w[0] for w in want] == [0, 0, 0, 0, -2, -2]
+
+    srv = L.xmss_mi355x_server_create(1 << 24, 8)
+    assert srv
+    k = L.xmss_mi355x_server_add_key(srv, 0, buf(sk), buf(pk))
+    assert k == 0
+    for j, m in enumerate(msgs):
+        rid = L.xmss_mi355x_server_sign(srv, k, m, len(m))
+        out = buf(p.sig_bytes + len(m))
+        olen = C.c_ulonglong(0)
+        rc = L.xmss_mi355x_server_result(srv, rid, out, C.byref(olen), None)
+        assert rc == want[j][0], "sign %d" % j
+        if rc == 0:
+            assert raw(out, olen.value) == want[j][1], "sign %d" % j
+        if j == 2:  # device path so far: the mirror matches the key signed alone
+            skb = buf(4 + p.sk_bytes)
+            assert L.xmss_mi355x_server_get_sk(srv, k, skb) == 0
+            _, _, sk3 = lib.sign(sk, msgs[0], 0)
+            _, _, sk3 = lib.sign(sk3, msgs[1], 0)
+            _, _, sk3 = lib.sign(sk3, msgs[2], 0)
+            assert raw(skb, 4 + p.sk_bytes) == sk3
+    skb = buf(4 + p.sk_bytes)
+    assert L.xmss_mi355x_server_get_sk(srv, k, skb) == 0
+    assert raw(skb, 4 + p.sk_bytes) == ska
+    L.xmss_mi355x_server_destroy(srv)
+
+
+def test_serve_budget_counts_resident_keys():
+    """Adding keys reserves their resident state (key context and sk bytes)
+    against the HBM budget before any request runs; a key the budget cannot
+    hold is refused."""
+    lib = product(1)
+    L = lib.lib
+    _server_api(L)
+    keys = _keys(lib)
+    srv = L.xmss_mi355x_server_create(1 << 30, 4)
+    assert srv
+    for (mt, oid, p, pk, sk) in keys:
+        assert L.xmss_mi355x_server_add_key(srv, mt, buf(sk), buf(pk)) >= 0
+    launches, calls, peak, waits = C.c_uint64(), C.c_uint64(), C.c_size_t(), C.c_uint64()
+    L.xmss_mi355x_server_stats(srv, C.byref(launches), C.byref(calls), C.byref(peak), C.byref(waits))
+    L.xmss_mi355x_server_destroy(srv)
+    assert peak.value >= sum(p.sk_bytes for (_, _, p, _, _) in keys)
+    assert calls.value == 0
+    # a budget just below the six keys' resident bytes refuses the last key
+    srv = L.xmss_mi355x_server_create(peak.value - 1, 4)
+    assert srv
+    got = [L.xmss_mi355x_server_add_key(srv, mt, buf(sk), buf(pk)) for (mt, oid, p, pk, sk) in keys]
+    L.xmss_mi355x_server_destroy(srv)
+    assert got == [0, 1, 2, 3, 4, -1]
