@@ -152,9 +152,15 @@ def bench_batch(api, count):
     mls = (C.c_ulonglong * count)()
     vr = (C.c_int * count)()
     job = L.xmss_mi355x_verify_prepare(0, ptrs(smb), smls, ptrs(pks), count)
-    t0 = time.perf_counter()
+    # the same staged job runs once untimed (first launches load the kernels),
+    # then three timed runs: the median
     L.xmss_mi355x_job_run(job)
-    ver_t = time.perf_counter() - t0
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        L.xmss_mi355x_job_run(job)
+        runs.append(time.perf_counter() - t0)
+    ver_t = sorted(runs)[1]
     L.xmss_mi355x_verify_collect(job, ptrs(outs), mls, vr)
     L.xmss_mi355x_job_free(job)
     # leaf 1023 is the last index: the reference signs it with the wiped key,

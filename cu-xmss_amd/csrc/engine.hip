@@ -559,6 +559,7 @@ struct Engine::Batch {
     uint8_t* wots = nullptr;
     size_t res_bytes = 0;
     int* ok = nullptr;        // verify
+    std::vector<int> perm;    // verify: staged position -> the caller's index (longest message first)
 };
 
 __global__ __launch_bounds__(WG) void k_compare(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b,
@@ -734,11 +735,18 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
     b->count = count;
     const unsigned n = b->n, len = b->len;
     FamOps* o = ops_of(fam);
+    // staged longest message first, so that run() can start the rest while
+    // the longest messages are still being hashed
+    std::vector<int>& ord = b->perm;
+    ord.resize(count);
+    for (int i = 0; i < count; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return items[x].mlen > items[y].mlen; });
     std::vector<KeySeeds> ks(count);
     std::vector<MsgReq> mr(count);
     for (int i = 0; i < count; ++i) {
-        ks[i] = {nullptr, nullptr, items[i].pk + n};
-        mr[i] = {(uint32_t)i, 0, items[i].idx, items[i].R, items[i].pk, items[i].m, items[i].mlen};
+        const VerifyItem& it = items[ord[i]];
+        ks[i] = {nullptr, nullptr, it.pk + n};
+        mr[i] = {(uint32_t)i, 0, it.idx, it.R, it.pk, it.m, it.mlen};
     }
     const MsgStage ms = plan_msgs(fam, count, false, mr);
     Layout in;
@@ -751,8 +759,9 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
     fill_msgs(h + o_blob, fam, false, mr, ms);
     DLeaf* vd = reinterpret_cast<DLeaf*>(h + o_vd);
     for (int i = 0; i < count; ++i) {
-        uint64_t idx = items[i].idx;
-        const uint8_t* p = items[i].wots;
+        const VerifyItem& it = items[ord[i]];
+        uint64_t idx = it.idx;
+        const uint8_t* p = it.wots;
         for (unsigned L = 0; L < d; ++L) {
             const uint32_t leaf = (uint32_t)(idx & ((1ull << th) - 1));
             idx >>= th;
@@ -763,7 +772,7 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
             memcpy(h + o_auth + t * th * n, p, (size_t)th * n);
             p += (size_t)th * n;
         }
-        memcpy(h + o_pk + (size_t)i * n, items[i].pk, n);
+        memcpy(h + o_pk + (size_t)i * n, it.pk, n);
     }
     uint8_t* dv = b->io.need(in.size);
     I->h2d(dv, h, in.size);
@@ -783,6 +792,76 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
     return b;
 }
 
+// Large verification batches (staged longest message first).  The requests
+// split into two groups: the longest-message quarter on the aux stream and
+// the rest on the main stream, each running H_msg -> per layer: F pre-images
+// of only the steps its signatures need -> chains -> L-tree -> auth walk.
+// The long messages' H_msg (up to ~70 serial compressions) thus overlaps the
+// other group's hashing.  The H pre-images (L-tree merges, auth levels:
+// address only) of every layer and request run once, on the main stream
+// after the short messages' first F pre-images; both groups wait for them
+// before their chains.
+static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* vscratch) {
+    const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
+    const uint32_t count = (uint32_t)b->count;
+    const void* dk = b->keys.p;
+    const size_t lb = o->lat_bytes(th);
+    uint8_t* dig = (uint8_t*)b->dig.p;
+    uint8_t* roots = (uint8_t*)b->roots.p;
+    uint8_t* ends = b->ends.need((size_t)count * len * n);
+    uint8_t* tmp = b->tmp.need((size_t)count * ((len + 1) / 2) * n);
+    hipEvent_t staged = I->L.ev(), hpre = I->L.ev(), aux_done = I->L.ev();
+    HIPCK(hipEventRecord(staged, I->L.s_main));
+    HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
+    // group 0: the longest-message quarter (aux); group 1: the rest (main).
+    // Launches are issued stage by stage, alternating the groups, so that
+    // neither stream waits for the host to issue the other's launches; the
+    // long messages' H_msg starts first, the H pre-images follow the short
+    // ones' on the main stream.
+    const uint32_t cut = count / 4;
+    const uint32_t lo[2] = {0, cut}, cnt[2] = {cut, count - cut};
+    hipStream_t st[2] = {I->L.s_aux, I->L.s_main};
+    for (int g : {0, 1}) {
+        if (!cnt[g]) continue;
+        I->L.s = st[g];
+        o->hmsg(I->L, dk, b->md + lo[g], cnt[g], fam_pad(b->fam), b->blob, b->R, dig + (size_t)lo[g] * n);
+    }
+    for (unsigned L = 0; L < d; ++L) {
+        for (int stage = 0; stage < 2; ++stage) {
+            if (L == 0 && stage == 1) {
+                // the H pre-images, after the short messages' F pre-images
+                I->L.s = I->L.s_main;
+                o->lat_pre_h(I->L, dk, b->vd, count * d, th, vscratch);
+                HIPCK(hipEventRecord(hpre, I->L.s_main));
+                HIPCK(hipStreamWaitEvent(I->L.s_aux, hpre, 0));
+            }
+            for (int g : {1, 0}) {
+                const uint32_t a = lo[g], m = cnt[g];
+                if (!m) continue;
+                I->L.s = st[g];
+                const size_t t = (size_t)L * count + a;  // first request of the group in this layer
+                const uint8_t* msg = L == 0 ? dig + (size_t)a * n : roots + ((size_t)(L - 1) * count + a) * n;
+                const DLeaf* vd = b->vd + t;
+                uint8_t* scr = vscratch + t * lb;
+                if (stage == 0)
+                    o->pre_need(I->L, dk, vd, m, msg, th, scr);
+                else
+                    o->lat_verify_batch(I->L, vd, m, msg, b->sigs + t * len * n,
+                                        b->auth_in + (size_t)L * count * th * n, th, scr, ends + (size_t)a * len * n,
+                                        tmp + (size_t)a * ((len + 1) / 2) * n, roots + t * n);
+            }
+        }
+    }
+    HIPCK(hipEventRecord(aux_done, I->L.s_aux));
+    I->L.s = I->L.s_main;
+    HIPCK(hipStreamWaitEvent(I->L.s_main, aux_done, 0));
+    I->L.run("compare", k_compare, dim3(cdiv(count, WG)), dim3(WG), (const uint8_t*)roots + (size_t)(d - 1) * count * n,
+             b->pkroot, count, n, b->ok);
+    I->L.pool.push_back(staged);
+    I->L.pool.push_back(hpre);
+    I->L.pool.push_back(aux_done);
+}
+
 void Engine::wait() { impl_->sync(); }
 
 void Engine::run(Batch* b, bool wait_done) {
@@ -797,6 +876,11 @@ void Engine::run(Batch* b, bool wait_done) {
     const size_t nv = (size_t)count * d;
     const bool vlat = !b->sign && use_lat(o, nv, th);
     uint8_t* vscratch = vlat ? I->lat.need(nv * o->lat_bytes(th)) : nullptr;
+    if (vlat && count >= kLatDense) {
+        verify_dense(I, o, b, vscratch);
+        if (wait_done) I->sync();
+        return;
+    }
     hipEvent_t staged = nullptr, pre = nullptr;
     if (vlat) {
         staged = I->L.ev();
@@ -941,8 +1025,10 @@ void Engine::collect_sign(Batch* b, uint8_t* const* out) {
 
 void Engine::collect_verify(Batch* b, int* ok) {
     Impl* I = impl_;
-    I->d2h(ok, b->ok, sizeof(int) * (size_t)b->count);
+    int* h = reinterpret_cast<int*>(I->pin.need(sizeof(int) * (size_t)b->count));
+    I->d2h(h, b->ok, sizeof(int) * (size_t)b->count);
     I->sync();
+    for (int i = 0; i < b->count; ++i) ok[b->perm[i]] = h[i];
 }
 
 void Engine::free_batch(Batch* b) {

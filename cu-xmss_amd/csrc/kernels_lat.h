@@ -56,6 +56,28 @@ __device__ __forceinline__ void ltree_merge(uint32_t t, uint32_t& height, uint32
     index = t - off;
 }
 
+// The H pre-image u of a request: L-tree merge u (u < LEN - 1), else auth level u - (LEN - 1).
+template <class F>
+__device__ __forceinline__ void lat_pre_h_item(const typename F::Key& k, const DLeaf& q, uint32_t u, uint8_t* S) {
+    constexpr uint32_t LEN = F::LEN;
+    uint32_t a[8];
+    if (u < LEN - 1) {  // L-tree merge (xmss_commons.c:16-49)
+        uint32_t h, i;
+        ltree_merge<F>(u, h, i);
+        const uint32_t b[8] = {q.layer, q.tree_hi, q.tree_lo, ADDR_LTREE, q.leaf, h, i, 0};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = b[j];
+    } else {  // auth level (xmss_commons.c:54-96)
+        const uint32_t i = u - (LEN - 1);
+        const uint32_t b[8] = {q.layer, q.tree_hi, q.tree_lo, ADDR_HASHTREE, 0, i, q.leaf >> (i + 1), 0};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = b[j];
+    }
+    typename F::HPre p;
+    F::pre_h(p, k, a);
+    reinterpret_cast<typename F::HPre*>(S + LatLayout<F>::hpre)[u] = p;
+}
+
 template <class F, uint32_t MODE>
 __global__ __launch_bounds__(WG) void k_lat_pre(const typename F::Key* __restrict__ keys,
                                                  const DLeaf* __restrict__ reqs,
@@ -90,23 +112,22 @@ __global__ __launch_bounds__(WG) void k_lat_pre(const typename F::Key* __restric
         reinterpret_cast<typename F::FPre*>(S + Lay::fpre)[c * 15 + s] = p;
         return;
     }
-    const uint32_t u = t - LEN - nf;
-    uint32_t a[8];
-    if (u < LEN - 1) {  // L-tree merge (xmss_commons.c:16-49)
-        uint32_t h, i;
-        ltree_merge<F>(u, h, i);
-        const uint32_t b[8] = {q.layer, q.tree_hi, q.tree_lo, ADDR_LTREE, q.leaf, h, i, 0};
-#pragma unroll
-        for (int j = 0; j < 8; ++j) a[j] = b[j];
-    } else {  // auth level (xmss_commons.c:54-96)
-        const uint32_t i = u - (LEN - 1);
-        const uint32_t b[8] = {q.layer, q.tree_hi, q.tree_lo, ADDR_HASHTREE, 0, i, q.leaf >> (i + 1), 0};
-#pragma unroll
-        for (int j = 0; j < 8; ++j) a[j] = b[j];
-    }
-    typename F::HPre p;
-    F::pre_h(p, k, a);
-    reinterpret_cast<typename F::HPre*>(S + Lay::hpre)[u] = p;
+    lat_pre_h_item<F>(k, q, t - LEN - nf, S);
+}
+
+// Only the H pre-images of LAT_VERIFY requests that all have tree height th
+// (their F pre-images then come from k_pre_need, once the message is known),
+// flat over (request, merge) items.
+template <class F>
+__global__ __launch_bounds__(WG) void k_lat_pre_h(const typename F::Key* __restrict__ keys,
+                                                   const DLeaf* __restrict__ reqs, uint32_t n, uint32_t th,
+                                                   uint8_t* __restrict__ scratch, size_t stride) {
+    const uint32_t nh = F::LEN - 1 + th;
+    const uint32_t t = blockIdx.x * WG + threadIdx.x;
+    if (t >= n * nh) return;
+    const uint32_t r = t / nh;
+    const DLeaf q = reqs[r];
+    lat_pre_h_item<F>(keys[q.key], q, t - r * nh, scratch + (size_t)r * stride);
 }
 
 // One workgroup per request, ceil(LEN/64) waves.
@@ -250,6 +271,51 @@ __global__ __launch_bounds__(64) void k_top_fin(uint32_t th, uint32_t level0, ui
             F::store(o, T + ((2ull << th) - (2ull << (th - level - 1)) + t) * F::kN);
         }
         __syncthreads();
+    }
+}
+
+// F pre-images of the chain steps a verification needs (steps len_c .. 14 of
+// chain c, len_c the message's base-w digit), once the message is known: one
+// workgroup per request walks the needed (chain, step) items densely (about
+// half of the 15 x LEN), so no lane idles on a step the chain skips.
+template <class F>
+__global__ __launch_bounds__(256) void k_pre_need(const typename F::Key* __restrict__ keys,
+                                                   const DLeaf* __restrict__ reqs,
+                                                   const uint8_t* __restrict__ msgs,
+                                                   uint8_t* __restrict__ scratch, size_t stride) {
+    constexpr uint32_t LEN = F::LEN;
+    __shared__ uint32_t first[LEN + 1];  // first needed item of each chain; first[LEN] = all
+    __shared__ uint32_t lens[LEN];
+    const uint32_t r = blockIdx.x;
+    const DLeaf q = reqs[r];
+    const uint8_t* msg = msgs + (size_t)r * F::kN;
+    for (uint32_t c = threadIdx.x; c < LEN; c += blockDim.x) lens[c] = chain_len<F>(msg, c);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t c = 0; c < LEN; ++c) {
+            first[c] = acc;
+            acc += 15 - lens[c];
+        }
+        first[LEN] = acc;
+    }
+    __syncthreads();
+    const typename F::Key& k = keys[q.key];
+    auto* fp = reinterpret_cast<typename F::FPre*>(scratch + (size_t)r * stride + LatLayout<F>::fpre);
+    const uint32_t total = first[LEN];
+    for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
+        // the chain holding item t: first[lo] <= t < first[lo + 1]
+        uint32_t lo = 0, hi = LEN;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (first[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t s = lens[lo] + (t - first[lo]);
+        uint32_t a[8] = {q.layer, q.tree_hi, q.tree_lo, ADDR_OTS, q.leaf, lo, s, 0};
+        typename F::FPre p;
+        F::pre_f(p, k, a);
+        fp[lo * 15 + s] = p;
     }
 }
 

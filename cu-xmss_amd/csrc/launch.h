@@ -121,6 +121,13 @@ struct FamOps {
                          uint8_t* out) = 0;
     // Lane-dense finish of n verify requests (same layer) whose k_lat_pre
     // scratch is ready: ends and tmp hold n x LEN and n x ceil(LEN/2) nodes.
+    // For LAT_VERIFY requests of tree height th: lat_pre_h computes only the
+    // H pre-images (address only), pre_need the F pre-images of only the
+    // chain steps their signatures need, from their messages.
+    virtual void lat_pre_h(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, uint32_t th,
+                           uint8_t* scratch) = 0;
+    virtual void pre_need(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
+                          uint32_t th, uint8_t* scratch) = 0;
     virtual void lat_verify_batch(Launcher& L, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
                                   const uint8_t* sigs, const uint8_t* auth, uint32_t th, const uint8_t* scratch,
                                   uint8_t* ends, uint8_t* tmp, uint8_t* out) = 0;
@@ -202,6 +209,16 @@ struct FamOpsT final : FamOps {
         if (mode == LAT_LEAF) lat_pre_t<LAT_LEAF>(L, keys, reqs, n, msgs, th, scratch);
         else if (mode == LAT_SIGN) lat_pre_t<LAT_SIGN>(L, keys, reqs, n, msgs, th, scratch);
         else lat_pre_t<LAT_VERIFY>(L, keys, reqs, n, msgs, th, scratch);
+    }
+    void lat_pre_h(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, uint32_t th,
+                   uint8_t* scratch) override {
+        L.run("lat_pre_h", k_lat_pre_h<F>, dim3(cdiv((uint64_t)n * (F::LEN - 1 + th), WG)), dim3(WG),
+              (const K*)keys, reqs, n, th, scratch, LatLayout<F>::bytes(th));
+    }
+    void pre_need(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, const uint8_t* msgs, uint32_t th,
+                  uint8_t* scratch) override {
+        L.run("pre_need", k_pre_need<F>, dim3(n), dim3(256), (const K*)keys, reqs, msgs, scratch,
+              LatLayout<F>::bytes(th));
     }
     void lat_fin(Launcher& L, uint32_t mode, const DLeaf* reqs, uint32_t n, const uint8_t* msgs, const uint8_t* sigs,
                  const uint8_t* auth, uint32_t th, const uint8_t* scratch, uint8_t* out) override {

@@ -174,3 +174,68 @@ def test_mixed_parameter_sets_in_one_job():
         if ok:
             assert raw(outs[t], mls[t]) == cases[i][4]
     assert vr[kv - 1] == -1
+
+
+@pytest.mark.parametrize("name,mt", [("XMSS-SHA2_10_256", 0), ("XMSSMT-SHA2_20/4_256", 1),
+                                     ("XMSS-SHA2_10_512", 0)])
+def test_large_verify_batch_accepts_exactly_the_valid(name, mt):
+    """A verification batch of 300 requests (the dense path: requests staged
+    longest message first, in two groups): valid signatures of 32 B - 4 KB
+    messages in shuffled order, and ones with a flipped byte in R, in a WOTS
+    signature, in an auth path, in an upper layer or in the message; every
+    verdict lands on its own request and equals the single sign_open call."""
+    import random
+    from test_gpu_opk import _sign_many
+    lib = product(0)
+    L = lib.lib
+    oid = lib.str_to_oid(name)
+    p = lib.params(oid, mt)
+    L.xmss_mi355x_rng_seed(oid + 17 * mt)
+    rc, pk, sk = lib.keypair(oid, mt)
+    assert rc == 0
+    rng = random.Random(oid * 3 + mt)
+    k = 300
+    msgs = [bytes(rng.getrandbits(8) for _ in range(int(32 * 128 ** rng.random()))) for _ in range(k)]
+    out, _ = _sign_many(lib, sk, msgs, mt, p)
+    sms = []
+    ib, n, wb = p.index_bytes, p.n, p.wots_sig_bytes
+    layer = wb + p.tree_height * n
+    for i, (rc, sm) in enumerate(out):
+        assert rc == 0
+        sm = bytearray(sm)
+        kind = i % 6
+        if kind == 1:
+            sm[ib + rng.randrange(n)] ^= 1                           # R
+        elif kind == 2:
+            sm[ib + n + rng.randrange(wb)] ^= 0x40                   # layer-0 WOTS signature
+        elif kind == 3:
+            sm[ib + n + wb + rng.randrange(p.tree_height * n)] ^= 8  # layer-0 auth path
+        elif kind == 4:
+            sm[ib + n + layer * (p.d - 1) + rng.randrange(layer)] ^= 2  # top layer
+        elif kind == 5 and len(msgs[i]):
+            sm[p.sig_bytes + rng.randrange(len(msgs[i]))] ^= 1      # message
+        sms.append(bytes(sm))
+    order = list(range(k))
+    rng.shuffle(order)
+    sms = [sms[i] for i in order]
+    want = []
+    for i, s in zip(order, sms):
+        ok = i % 6 == 0 or (i % 6 == 5 and not msgs[i])
+        want.append(0 if ok else -1)
+    pkb = [buf(pk) for _ in range(k)]
+    smb = [buf(s) for s in sms]
+    outs = [buf(len(s)) for s in sms]
+    smls = (C.c_ulonglong * k)(*[len(s) for s in sms])
+    mls = (C.c_ulonglong * k)()
+    vr = (C.c_int * k)()
+    L.xmss_batch_sign_open.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_size_t]
+    assert L.xmss_batch_sign_open(mt, _arr(outs), mls, _arr(smb), smls, _arr(pkb), vr, k) == 0
+    for j in range(k):
+        assert vr[j] == want[j], "request %d (signature %d, kind %d)" % (j, order[j], order[j] % 6)
+        if want[j] == 0:
+            assert raw(outs[j], mls[j]) == msgs[order[j]]
+    # a few against the one-call API
+    for j in range(0, k, 37):
+        rc, m = lib.sign_open(sms[j], pk, mt)
+        assert rc == vr[j]
