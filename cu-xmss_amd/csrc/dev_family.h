@@ -412,13 +412,12 @@ struct Sha256Fam {
         prf8(out, k.ms_prfk, k.prfk, in);
     }
 
-    // SHA-256 of len bytes at p (p 4-byte aligned, readable up to the next
-    // multiple of 4), output truncated to n bytes.
+    // SHA-256 of len bytes at p (p 16-byte aligned, readable kMsgSlack bytes
+    // past the end), output truncated to n bytes.
     __device__ static void hash_bytes(Node& out, const uint8_t* p, uint32_t len) {
         uint32_t st[8];
         sha256_iv(st);
         const uint32_t nb = (len + 9 + 63) / 64;
-        const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
         // blocks made only of message bytes: plain loads, the next block's
         // words in flight while this one compresses
         // (p is 16-byte aligned: four 16-byte loads per block)
@@ -444,24 +443,29 @@ struct Sha256Fam {
                 sha256_compress(st, w);
             }
         }
+        // the last one or two blocks: whole-block loads (the bytes past the
+        // message are readable, kMsgSlack), then the padding in registers
         for (; b < nb; ++b) {
+            const uint4* qb = reinterpret_cast<const uint4*>(p + 64 * (size_t)b);
+            uint4 t[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) t[j] = qb[j];
             uint32_t w[16];
 #pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                w[4 * j] = t[j].x;
+                w[4 * j + 1] = t[j].y;
+                w[4 * j + 2] = t[j].z;
+                w[4 * j + 3] = t[j].w;
+            }
+#pragma unroll
             for (int j = 0; j < 16; ++j) {
-                const uint32_t o = 64 * b + 4 * j;
-                uint32_t v;
-                if (o + 4 <= len) {
-                    v = bswap32(q[o / 4]);
-                } else if (o < len) {
-                    const uint32_t keep = len - o;  // 1..3 bytes
-                    v = bswap32(q[o / 4]) & (0xFFFFFFFFu << (32 - 8 * keep));
-                    v |= 0x80u << (24 - 8 * keep);
-                } else if (o == len) {
-                    v = 0x80000000u;
-                } else {
-                    v = 0;
-                }
-                w[j] = v;
+                const int rem = (int)len - (int)(64 * b + 4 * j);  // message bytes from this word on
+                const uint32_t v = bswap32(w[j]);
+                if (rem >= 4) w[j] = v;
+                else if (rem > 0) w[j] = (v & (0xFFFFFFFFu << (32 - 8 * rem))) | (0x80u << (24 - 8 * rem));
+                else if (rem == 0) w[j] = 0x80000000u;
+                else w[j] = 0;
             }
             if (b == nb - 1) {
                 w[14] = 0;
@@ -679,7 +683,6 @@ struct Sha512Fam {
         uint64_t st[8];
         sha512_iv(st);
         const uint32_t nb = (len + 17 + 127) / 128;
-        const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
         // blocks made only of message bytes: plain loads, the next block's
         // words in flight while this one compresses
         // (p is 16-byte aligned: eight 16-byte loads per block)
@@ -703,25 +706,32 @@ struct Sha512Fam {
                 sha512_compress(st, w);
             }
         }
+        // the last one or two blocks: whole-block loads (the bytes past the
+        // message are readable, kMsgSlack), then the padding in registers
         for (; b < nb; ++b) {
+            const uint4* qb = reinterpret_cast<const uint4*>(p + 128 * (size_t)b);
+            uint32_t raw[32];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint4 t = qb[j];
+                raw[4 * j] = t.x;
+                raw[4 * j + 1] = t.y;
+                raw[4 * j + 2] = t.z;
+                raw[4 * j + 3] = t.w;
+            }
             uint64_t w[16];
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 uint64_t v = 0;
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {  // two 32-bit halves, big-endian
-                    const uint32_t o = 128 * b + 8 * j + 4 * h;
+                    const int rem = (int)len - (int)(128 * b + 8 * j + 4 * h);
+                    const uint32_t x = bswap32(raw[2 * j + h]);
                     uint32_t u;
-                    if (o + 4 <= len) {
-                        u = bswap32(q[o / 4]);
-                    } else if (o < len) {
-                        const uint32_t keep = len - o;
-                        u = (bswap32(q[o / 4]) & (0xFFFFFFFFu << (32 - 8 * keep))) | (0x80u << (24 - 8 * keep));
-                    } else if (o == len) {
-                        u = 0x80000000u;
-                    } else {
-                        u = 0;
-                    }
+                    if (rem >= 4) u = x;
+                    else if (rem > 0) u = (x & (0xFFFFFFFFu << (32 - 8 * rem))) | (0x80u << (24 - 8 * rem));
+                    else if (rem == 0) u = 0x80000000u;
+                    else u = 0;
                     v |= (uint64_t)u << (h ? 0 : 32);
                 }
                 w[j] = v;
@@ -920,21 +930,32 @@ struct ShakeFam {
 #pragma unroll
         for (int i = 0; i < 25; ++i) s[i] = 0;
         const uint32_t nb = len / RATE + 1;
-        const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+        // whole-block 8-byte loads (p is 16-byte aligned and RATE a multiple
+        // of 8; the bytes past the message are readable, kMsgSlack), the next
+        // block's words in flight while this one is permuted; the padding of
+        // the last block is applied in registers
+        const uint2* q = reinterpret_cast<const uint2*>(p);
+        uint2 nx[RATE / 8];
+#pragma unroll
+        for (int j = 0; j < RATE / 8; ++j) nx[j] = q[j];
         for (uint32_t b = 0; b < nb; ++b) {
+            uint2 cur[RATE / 8];
+#pragma unroll
+            for (int j = 0; j < RATE / 8; ++j) cur[j] = nx[j];
+            const uint2* qn = q + (RATE / 8) * (b + 1 < nb ? b + 1 : b);
+#pragma unroll
+            for (int j = 0; j < RATE / 8; ++j) nx[j] = qn[j];
+            const bool lastb = b + 1 == nb;
 #pragma unroll
             for (int j = 0; j < RATE / 8; ++j) {
-                uint32_t lohi[2];
+                uint32_t lohi[2] = {cur[j].x, cur[j].y};
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    const uint32_t o = b * RATE + 8 * j + 4 * h;
-                    uint32_t v;
-                    if (o + 4 <= len) v = q[o / 4];
-                    else if (o < len) v = q[o / 4] & (0xFFFFFFFFu >> (32 - 8 * (len - o)));
-                    else v = 0;
-                    if (o <= len && len < o + 4) v ^= 0x1Fu << (8 * (len - o));
-                    const uint32_t last = nb * RATE - 1;
-                    if (o <= last && last < o + 4) v ^= 0x80u << (8 * (last - o));
+                    const int rem = (int)len - (int)(b * RATE + 8 * j + 4 * h);
+                    uint32_t v = lohi[h];
+                    if (rem < 4) v = rem > 0 ? v & (0xFFFFFFFFu >> (32 - 8 * rem)) : 0;
+                    if (rem >= 0 && rem < 4) v ^= 0x1Fu << (8 * rem);
+                    if (lastb && 8 * j + 4 * h == RATE - 4) v ^= 0x80000000u;
                     lohi[h] = v;
                 }
                 s[j] ^= ((uint64_t)lohi[1] << 32) | lohi[0];

@@ -491,7 +491,7 @@ void Engine::hmsg(int fam, int nreq, const MsgReq* reqs, uint8_t* R_out, uint8_t
         d[i] = {reqs[i].key, reqs[i].sign, reqs[i].idx, (uint64_t)total, (uint32_t)(pb + reqs[i].mlen), 0};
         total += (pb + reqs[i].mlen + 15) & ~(size_t)15;
     }
-    total += 8;
+    total += kMsgSlack;
     // toByte(2, pad) || R || root || toByte(idx, n) || M  (hash.c:107-120)
     std::vector<uint8_t> blob(total, 0);
     for (int i = 0; i < nreq; ++i) {
@@ -596,7 +596,7 @@ static MsgStage plan_msgs(int fam, int count, bool sign, const std::vector<MsgRe
         m.d[i] = {reqs[i].key, sign ? 1u : 0u, reqs[i].idx, (uint64_t)m.total, (uint32_t)(pb + reqs[i].mlen), 0};
         m.total += (pb + reqs[i].mlen + 15) & ~(size_t)15;
     }
-    m.total += 16;
+    m.total += kMsgSlack;
     return m;
 }
 static void fill_msgs(uint8_t* h, int fam, bool sign, const std::vector<MsgReq>& reqs, const MsgStage& m) {
@@ -793,14 +793,16 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
 }
 
 // Large verification batches (staged longest message first).  The requests
-// split into two groups: the longest-message quarter on the aux stream and
-// the rest on the main stream, each running H_msg -> per layer: F pre-images
-// of only the steps its signatures need -> chains -> L-tree -> auth walk.
-// The long messages' H_msg (up to ~70 serial compressions) thus overlaps the
-// other group's hashing.  The H pre-images (L-tree merges, auth levels:
-// address only) of every layer and request run once, on the main stream
-// after the short messages' first F pre-images; both groups wait for them
-// before their chains.
+// split into two groups by message length, each on its own stream: the
+// longest quarter (aux) and the rest (main).  Each group runs H_msg -> per
+// layer: F pre-images of only the chain steps its signatures need -> chains
+// (sorted by length within each workgroup) -> L-tree -> auth walk.  H_msg is
+// serial per message and fills few lanes, so the short messages' pipeline
+// starts while the long ones are still being hashed.  The H pre-images of
+// every layer and request (L-tree merges and auth levels: address only) run
+// once, on the main stream after its F pre-images; the aux stream waits for
+// them before its chains.  (Measured on one MI355X at 4096 requests: three
+// groups, or the H pre-images right after H_msg, were 3-7% slower.)
 static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* vscratch) {
     const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
     const uint32_t count = (uint32_t)b->count;
@@ -810,33 +812,29 @@ static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* 
     uint8_t* roots = (uint8_t*)b->roots.p;
     uint8_t* ends = b->ends.need((size_t)count * len * n);
     uint8_t* tmp = b->tmp.need((size_t)count * ((len + 1) / 2) * n);
-    hipEvent_t staged = I->L.ev(), hpre = I->L.ev(), aux_done = I->L.ev();
+    constexpr int G = 2;
+    const uint32_t lo[G] = {0, count / 4}, hi[G] = {count / 4, count};
+    hipStream_t st[G] = {I->L.s_aux, I->L.s_main};
+    hipEvent_t staged = I->L.ev(), hpre = I->L.ev(), done[G - 1] = {I->L.ev()};
     HIPCK(hipEventRecord(staged, I->L.s_main));
-    HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
-    // group 0: the longest-message quarter (aux); group 1: the rest (main).
-    // Launches are issued stage by stage, alternating the groups, so that
-    // neither stream waits for the host to issue the other's launches; the
-    // long messages' H_msg starts first, the H pre-images follow the short
-    // ones' on the main stream.
-    const uint32_t cut = count / 4;
-    const uint32_t lo[2] = {0, cut}, cnt[2] = {cut, count - cut};
-    hipStream_t st[2] = {I->L.s_aux, I->L.s_main};
-    for (int g : {0, 1}) {
-        if (!cnt[g]) continue;
+    for (int g = 0; g < G - 1; ++g) HIPCK(hipStreamWaitEvent(st[g], staged, 0));
+    // Launches are issued stage by stage across the groups, so that no stream
+    // waits for the host to issue the others' launches.
+    for (int g = 0; g < G; ++g) {
+        if (hi[g] == lo[g]) continue;
         I->L.s = st[g];
-        o->hmsg(I->L, dk, b->md + lo[g], cnt[g], fam_pad(b->fam), b->blob, b->R, dig + (size_t)lo[g] * n);
+        o->hmsg(I->L, dk, b->md + lo[g], hi[g] - lo[g], fam_pad(b->fam), b->blob, b->R, dig + (size_t)lo[g] * n);
     }
     for (unsigned L = 0; L < d; ++L) {
         for (int stage = 0; stage < 2; ++stage) {
             if (L == 0 && stage == 1) {
-                // the H pre-images, after the short messages' F pre-images
                 I->L.s = I->L.s_main;
                 o->lat_pre_h(I->L, dk, b->vd, count * d, th, vscratch);
                 HIPCK(hipEventRecord(hpre, I->L.s_main));
-                HIPCK(hipStreamWaitEvent(I->L.s_aux, hpre, 0));
+                for (int g = 0; g < G - 1; ++g) HIPCK(hipStreamWaitEvent(st[g], hpre, 0));
             }
-            for (int g : {1, 0}) {
-                const uint32_t a = lo[g], m = cnt[g];
+            for (int g = G - 1; g >= 0; --g) {
+                const uint32_t a = lo[g], m = hi[g] - lo[g];
                 if (!m) continue;
                 I->L.s = st[g];
                 const size_t t = (size_t)L * count + a;  // first request of the group in this layer
@@ -852,14 +850,16 @@ static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* 
             }
         }
     }
-    HIPCK(hipEventRecord(aux_done, I->L.s_aux));
+    for (int g = 0; g < G - 1; ++g) {
+        HIPCK(hipEventRecord(done[g], st[g]));
+        HIPCK(hipStreamWaitEvent(I->L.s_main, done[g], 0));
+    }
     I->L.s = I->L.s_main;
-    HIPCK(hipStreamWaitEvent(I->L.s_main, aux_done, 0));
     I->L.run("compare", k_compare, dim3(cdiv(count, WG)), dim3(WG), (const uint8_t*)roots + (size_t)(d - 1) * count * n,
              b->pkroot, count, n, b->ok);
     I->L.pool.push_back(staged);
     I->L.pool.push_back(hpre);
-    I->L.pool.push_back(aux_done);
+    for (hipEvent_t e : done) I->L.pool.push_back(e);
 }
 
 void Engine::wait() { impl_->sync(); }
@@ -1121,7 +1121,7 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
         md[i] = {jobs[i].key, 1u, jobs[i].idx, (uint64_t)total, (uint32_t)(pb + jobs[i].mlen), 0};
         total += (pb + jobs[i].mlen + 15) & ~(size_t)15;
     }
-    total += 8;
+    total += kMsgSlack;
     // One pinned staging area and one device arena, so that the inputs go
     // over in one copy and the results come back in one:
     //   [descriptors, H_msg blob | sk arena (updated in place) | R, WOTS, tails]
@@ -1249,7 +1249,7 @@ void Engine::verify_many(int fam, int count, const VerifyJob* jobs, int* ok) {
         md[j] = {v.key, 0u, idx, (uint64_t)total, (uint32_t)(pb + v.mlen), 0};
         total += (pb + v.mlen + 15) & ~(size_t)15;
     }
-    total += 8;
+    total += kMsgSlack;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t o_md = 0, o_blob = o_md + al(sizeof(DMsg) * count), o_it = o_blob + al(total),
                  o_sig = o_it + al(sizeof(DLeaf) * items), o_auth = o_sig + al((size_t)items * len * n),

@@ -33,6 +33,10 @@ struct DThash { uint32_t key, adrs[8], p[7]; };
 struct DMsg { uint32_t key, sign; uint64_t idx; uint64_t off; uint32_t len, p0; };
 
 constexpr uint32_t ADDR_OTS = 0, ADDR_LTREE = 1, ADDR_HASHTREE = 2;
+// hash_bytes reads whole blocks: every message blob a kernel hashes is
+// followed by at least this many readable bytes (at most one SHAKE128 block
+// of 168 bytes is read past a message's end).
+constexpr size_t kMsgSlack = 256;
 constexpr uint32_t MODE_PKGEN = 0, MODE_SIGN = 1, MODE_VERIFY = 2;
 constexpr int WG = 64;
 

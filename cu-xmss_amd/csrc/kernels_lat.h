@@ -325,17 +325,46 @@ __global__ __launch_bounds__(256) void k_pre_need(const typename F::Key* __restr
 // one lane per (request, chain), then per (request, L-tree pair) and height,
 // then per request for the auth walk.
 
-// WOTS chains of verify layers from their signatures: ends[r][c].
+// WOTS chains of verify layers from their signatures: ends[r][c].  A chain
+// runs 15 - d steps (d its message digit, uniform over 0..15), so a wave of
+// arbitrary chains runs as long as its longest one, twice the average.  Each
+// workgroup therefore takes 256 consecutive chains, sorts them by step count
+// in LDS (counting sort) and hands them to its lanes in that order: each wave
+// gets chains of about equal length.  Odd workgroups hand them out in
+// reverse, so that the long waves do not all land on the same SIMD.
+constexpr uint32_t LATB_CHAINS_WG = 256;
 template <class F>
-__global__ __launch_bounds__(WG) void k_latb_chains(uint32_t n, const uint8_t* __restrict__ msgs,
-                                                     const uint8_t* __restrict__ sigs,
-                                                     const uint8_t* __restrict__ scratch, size_t stride,
-                                                     uint8_t* __restrict__ ends) {
+__global__ __launch_bounds__(LATB_CHAINS_WG) void k_latb_chains(uint32_t n, const uint8_t* __restrict__ msgs,
+                                                                 const uint8_t* __restrict__ sigs,
+                                                                 const uint8_t* __restrict__ scratch, size_t stride,
+                                                                 uint8_t* __restrict__ ends) {
     using FPre = typename F::FPre;
-    const uint32_t g = blockIdx.x * WG + threadIdx.x;
-    if (g >= n * (uint32_t)F::LEN) return;
+    __shared__ uint32_t hist[16];
+    __shared__ uint32_t order[LATB_CHAINS_WG];
+    __shared__ uint8_t first[LATB_CHAINS_WG];
+    const uint32_t total = n * (uint32_t)F::LEN;
+    const uint32_t base = blockIdx.x * LATB_CHAINS_WG;
+    const uint32_t tid = threadIdx.x;
+    // first step of this thread's own chain (15: none, also past the end)
+    uint32_t d0 = 15;
+    if (base + tid < total) {
+        const uint32_t r0 = (base + tid) / F::LEN, c0 = (base + tid) % F::LEN;
+        d0 = chain_len<F>(msgs + (size_t)r0 * F::kN, c0);
+    }
+    first[tid] = (uint8_t)d0;
+    if (tid < 16) hist[tid] = 0;
+    __syncthreads();
+    const uint32_t rank = atomicAdd(&hist[d0], 1u);
+    __syncthreads();
+    uint32_t pos = rank;  // longest first: after every chain that starts earlier
+    for (uint32_t b = 0; b < d0; ++b) pos += hist[b];
+    order[pos] = tid;
+    __syncthreads();
+    const uint32_t mine = order[(blockIdx.x & 1) ? LATB_CHAINS_WG - 1 - tid : tid];
+    const uint32_t g = base + mine;
+    if (g >= total) return;
     const uint32_t r = g / F::LEN, c = g % F::LEN;
-    const uint32_t s0 = chain_len<F>(msgs + (size_t)r * F::kN, c);
+    const uint32_t s0 = first[mine];
     typename F::Node x;
     F::load(x, sigs + (size_t)g * F::kN);
     const FPre* fp = reinterpret_cast<const FPre*>(scratch + (size_t)r * stride + LatLayout<F>::fpre) + c * 15;

@@ -247,8 +247,8 @@ struct FamOpsT final : FamOps {
                           const uint8_t* auth, uint32_t th, const uint8_t* scratch, uint8_t* ends, uint8_t* tmp,
                           uint8_t* out) override {
         const size_t stride = LatLayout<F>::bytes(th);
-        L.run("latb_chains", k_latb_chains<F>, dim3(cdiv((uint64_t)n * F::LEN, WG)), dim3(WG), n, msgs, sigs,
-              scratch, stride, ends);
+        L.run("latb_chains", k_latb_chains<F>, dim3(cdiv((uint64_t)n * F::LEN, LATB_CHAINS_WG)),
+              dim3(LATB_CHAINS_WG), n, msgs, sigs, scratch, stride, ends);
         uint32_t l = F::LEN, off = 0;
         bool from_a = true;
         const uint32_t sa = F::LEN, sb = (F::LEN + 1) / 2;
