@@ -1054,6 +1054,11 @@ struct BdsBufs {
     PinnedBuf staging;
 };
 
+static void bds_two_streams(Engine::Impl* I, FamOps* o, BdsBufs& B, const void* dk, int count, unsigned n,
+                            unsigned pad, const DMsg* dmd, uint8_t* db, uint8_t* dR, const DLeaf* dw, uint8_t* dW,
+                            const DLeaf* dld, size_t nl, const DThash* dmg, size_t nmg, uint8_t* leaf_arena,
+                            void* merge_arena, size_t lds, const DBds* djobs, uint8_t* ska, uint8_t* dT);
+
 void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
     constexpr int kChunk = 8192;  // keeps every launch's grid within its limits
     if (count > kChunk && use_pool) {
@@ -1138,6 +1143,11 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
     memset(h + o_blob, 0, total);
     memcpy(h + o_jd, jd.data(), sizeof(DBds) * count);
     memcpy(h + o_mg, mg.data(), sizeof(DThash) * mg.size());
+    // one stream, three launches (bds_lat) unless the grids would be too large
+    const size_t nl = ld.size();
+    const size_t xrows = (mg.size() + count) / WG + 2;  // bounds its message and merge rows
+    const bool one_stream = use_lat(o, nl + count, 0) && nl + count + xrows <= 65535;
+    const uint32_t all_steps = one_stream ? LAT_ALL_STEPS : 0;  // WOTS pre-images before the digest exists
     DLeaf* wd = reinterpret_cast<DLeaf*>(h + o_wd);
     for (int i = 0; i < count; ++i) {
         uint8_t* p = h + o_blob + md[i].off;
@@ -1152,7 +1162,7 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
         const unsigned th = jobs[i].shape.th;
         const uint64_t tree = jobs[i].idx >> th;
         wd[i] = {jobs[i].key, 0, (uint32_t)(tree >> 32), (uint32_t)tree,
-                 (uint32_t)(jobs[i].idx & ((1ull << th) - 1)), 0, 0, 0};
+                 (uint32_t)(jobs[i].idx & ((1ull << th) - 1)), 0, all_steps, 0};
         if (!jobs[i].dsk) memcpy(h + o_sk + jd[i].sk_off, jobs[i].sk, jobs[i].shape.sk_bytes);
     }
     const DLeaf* dld = reinterpret_cast<const DLeaf*>(dv + o_ld);
@@ -1168,37 +1178,59 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
     uint8_t* leaf_arena = B.leaf_arena.need(ld.size() * n);
     void* merge_arena = B.merge_arena.need(std::max<size_t>(1, mg.size()) * o->hpre_bytes());
     I->h2d(dv, h, o_R);
-    // then: the merges' and leaves' work on the main stream while R / H_msg
-    // and the layer-0 WOTS signatures run on the aux stream; the state update
-    // waits for both
-    hipEvent_t staged = I->L.ev(), front = I->L.ev();
-    HIPCK(hipEventRecord(staged, I->L.s_main));
-    HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
-    I->L.s = I->L.s_aux;
-    o->hmsg(I->L, dk, dmd, (uint32_t)count, pad, db, dR, B.dig.need((size_t)count * n));
-    o->lat(I->L, dk, LAT_SIGN, dw, (uint32_t)count, (const uint8_t*)B.dig.p, nullptr, nullptr, 0,
-           B.lat_wots.need((size_t)count * o->lat_bytes(0)), dW);
-    HIPCK(hipEventRecord(front, I->L.s_aux));
-    I->L.s = I->L.s_main;
-    if (!mg.empty()) o->pre_h(I->L, dk, dmg, (uint32_t)mg.size(), merge_arena);
-    if (use_lat(o, ld.size(), 0))
-        o->lat(I->L, dk, LAT_LEAF, dld, (uint32_t)ld.size(), nullptr, nullptr, nullptr, 0,
-               B.lat_leaf.need(ld.size() * o->lat_bytes(0)), leaf_arena);
-    else if (!ld.empty())
-        o->leaf_wg(I->L, dk, dld, (uint32_t)ld.size(), false, nullptr, nullptr, nullptr, 0, leaf_arena);
-    HIPCK(hipStreamWaitEvent(I->L.s_main, front, 0));
-    o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, dT,
-                  B.scratch.need((size_t)count * 4 * n));
+    if (one_stream) {
+        // three launches on the main stream: R / H_msg with the address-only
+        // work of the planned leaves, the layer-0 WOTS signatures and the
+        // merges, then the leaves' and signatures' finishing work, then the
+        // state update
+        o->bds_lat(I->L, dk, dmd, pad, db, dR, B.dig.need((size_t)count * n), dld, (uint32_t)nl, dw, (uint32_t)count,
+                   B.lat_leaf.need((nl + count) * o->lat_bytes(0)), dmg, (uint32_t)mg.size(), merge_arena,
+                   leaf_arena, dW);
+        o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, dT,
+                      B.scratch.need((size_t)count * 4 * n));
+    } else {
+        bds_two_streams(I, o, B, dk, count, n, pad, dmd, db, dR, dw, dW, dld, nl, dmg, mg.size(), leaf_arena,
+                        merge_arena, lds, djobs, ska, dT);
+    }
     I->d2h(h + o_sk, ska, io_bytes - o_sk);
     I->sync();
-    I->L.pool.push_back(staged);
-    I->L.pool.push_back(front);
     for (int i = 0; i < count; ++i) {
         memcpy(jobs[i].sk, h + o_sk + jd[i].sk_off, jobs[i].shape.sk_bytes);
         memcpy(jobs[i].R_out, h + o_R + (size_t)i * n, n);
         memcpy(jobs[i].wots0_out, h + o_W + (size_t)i * len * n, (size_t)len * n);
         memcpy(jobs[i].tail_out, h + o_T + jd[i].tail_off, tail_bytes[i]);
     }
+}
+
+// The same launches for batches too large for one bds_lat pair: the planned
+// leaves (the longest dependent chain) on the main stream, issued first; R /
+// H_msg, the layer-0 WOTS signatures and the merges' address-only work on the
+// aux stream; the state update waits for both.
+static void bds_two_streams(Engine::Impl* I, FamOps* o, BdsBufs& B, const void* dk, int count, unsigned n,
+                            unsigned pad, const DMsg* dmd, uint8_t* db, uint8_t* dR, const DLeaf* dw, uint8_t* dW,
+                            const DLeaf* dld, size_t nl, const DThash* dmg, size_t nmg, uint8_t* leaf_arena,
+                            void* merge_arena, size_t lds, const DBds* djobs, uint8_t* ska, uint8_t* dT) {
+    hipEvent_t staged = I->L.ev(), front = I->L.ev();
+    HIPCK(hipEventRecord(staged, I->L.s_main));
+    I->L.s = I->L.s_main;
+    if (use_lat(o, nl, 0))
+        o->lat(I->L, dk, LAT_LEAF, dld, (uint32_t)nl, nullptr, nullptr, nullptr, 0,
+               B.lat_leaf.need(nl * o->lat_bytes(0)), leaf_arena);
+    else if (nl)
+        o->leaf_wg(I->L, dk, dld, (uint32_t)nl, false, nullptr, nullptr, nullptr, 0, leaf_arena);
+    HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
+    I->L.s = I->L.s_aux;
+    o->hmsg(I->L, dk, dmd, (uint32_t)count, pad, db, dR, B.dig.need((size_t)count * n));
+    o->lat(I->L, dk, LAT_SIGN, dw, (uint32_t)count, (const uint8_t*)B.dig.p, nullptr, nullptr, 0,
+           B.lat_wots.need((size_t)count * o->lat_bytes(0)), dW);
+    if (nmg) o->pre_h(I->L, dk, dmg, (uint32_t)nmg, merge_arena);
+    HIPCK(hipEventRecord(front, I->L.s_aux));
+    I->L.s = I->L.s_main;
+    HIPCK(hipStreamWaitEvent(I->L.s_main, front, 0));
+    o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, dT,
+                  B.scratch.need((size_t)count * 4 * n));
+    I->L.pool.push_back(staged);
+    I->L.pool.push_back(front);
 }
 
 struct VerifyBufs {

@@ -292,12 +292,9 @@ __global__ __launch_bounds__(WG) void k_thash_h(const typename F::Key* __restric
 // R (sign requests) and H_msg over the prefixed message in `blob`
 // (hash.c:107-120 with R = PRF(SK_PRF, toByte(idx, 32)), xmss_core.c:242-248).
 template <class F>
-__global__ __launch_bounds__(WG) void k_hmsg(const typename F::Key* __restrict__ keys,
-                                              const DMsg* __restrict__ reqs, uint32_t n, uint32_t pad,
-                                              uint8_t* __restrict__ blob, uint8_t* __restrict__ R_out,
-                                              uint8_t* __restrict__ dig_out) {
-    const uint32_t r = blockIdx.x * WG + threadIdx.x;
-    if (r >= n) return;
+__device__ __forceinline__ void hmsg_one(const typename F::Key* __restrict__ keys, const DMsg* __restrict__ reqs,
+                                         uint32_t r, uint32_t pad, uint8_t* __restrict__ blob,
+                                         uint8_t* __restrict__ R_out, uint8_t* __restrict__ dig_out) {
     const DMsg q = reqs[r];
     uint8_t* p = blob + q.off;
     if (q.sign) {
@@ -309,6 +306,16 @@ __global__ __launch_bounds__(WG) void k_hmsg(const typename F::Key* __restrict__
     typename F::Node d;
     F::hash_bytes(d, p, q.len);
     F::store(d, dig_out + (size_t)r * F::kN);
+}
+
+template <class F>
+__global__ __launch_bounds__(WG) void k_hmsg(const typename F::Key* __restrict__ keys,
+                                              const DMsg* __restrict__ reqs, uint32_t n, uint32_t pad,
+                                              uint8_t* __restrict__ blob, uint8_t* __restrict__ R_out,
+                                              uint8_t* __restrict__ dig_out) {
+    const uint32_t r = blockIdx.x * WG + threadIdx.x;
+    if (r >= n) return;
+    hmsg_one<F>(keys, reqs, r, pad, blob, R_out, dig_out);
 }
 
 __global__ __launch_bounds__(WG) void k_gather(const uint8_t* __restrict__ src,

@@ -16,6 +16,7 @@
 // lower-layer WOTS signatures stay in HBM.
 #pragma once
 #include "kernels.h"
+#include "kernels_lat.h"
 
 namespace xm {
 
@@ -330,6 +331,66 @@ __global__ __launch_bounds__(WG) void k_pre_h(const typename F::Key* __restrict_
     typename F::HPre p;
     F::pre_h(p, keys[q.key], a);
     out[r] = p;
+}
+
+// The hashing of BDS signatures before their state update, in two launches
+// on one stream (so that a signature costs three launches and no
+// cross-stream waits: the host's launch calls, as much as the GPU, bound a
+// single signature's latency):
+//   k_bds_pre: R and H_msg (the first rows, one message per lane: the
+//              longest-running lanes start first), the address-only work of
+//              the planned leaves (LAT_LEAF rows), of the layer-0 WOTS
+//              signatures (LAT_SIGN rows, all 15 steps of every chain, since
+//              the digests are computed in the same launch) and of the
+//              planned tree merges (the last rows, one merge per lane);
+//   k_bds_fin: one workgroup per planned leaf, then one per WOTS signature.
+// Both kinds of requests share one scratch area (rows nleaf .. for the
+// signatures).
+template <class F>
+__global__ __launch_bounds__(WG) void k_bds_pre(const typename F::Key* __restrict__ keys,
+                                                 const DMsg* __restrict__ msgs, uint32_t pad,
+                                                 uint8_t* __restrict__ blob, uint8_t* __restrict__ R_out,
+                                                 uint8_t* __restrict__ digests, uint32_t hrows,
+                                                 const DLeaf* __restrict__ leaves, uint32_t nleaf,
+                                                 const DLeaf* __restrict__ sigs, uint32_t nsign,
+                                                 uint8_t* __restrict__ scratch, size_t stride,
+                                                 const DThash* __restrict__ merges, uint32_t nmerge,
+                                                 typename F::HPre* __restrict__ merge_out) {
+    const uint32_t t = blockIdx.x * WG + threadIdx.x;
+    if (blockIdx.y < hrows) {
+        const uint32_t i = blockIdx.y * gridDim.x * WG + t;
+        if (i < nsign) hmsg_one<F>(keys, msgs, i, pad, blob, R_out, digests);
+        return;
+    }
+    const uint32_t y = blockIdx.y - hrows;
+    if (y < nleaf) {
+        lat_pre_one<F, LAT_LEAF>(keys, leaves, y, t, nullptr, scratch, stride);
+    } else if (y < nleaf + nsign) {
+        lat_pre_one<F, LAT_SIGN>(keys, sigs, y - nleaf, t, nullptr, scratch + (size_t)nleaf * stride, stride);
+    } else {
+        const uint32_t i = (y - nleaf - nsign) * gridDim.x * WG + t;
+        if (i >= nmerge) return;
+        const DThash q = merges[i];
+        uint32_t a[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = q.adrs[j];
+        typename F::HPre p;
+        F::pre_h(p, keys[q.key], a);
+        merge_out[i] = p;
+    }
+}
+
+template <class F>
+__global__ __launch_bounds__(192) void k_bds_fin(const DLeaf* __restrict__ leaves, uint32_t nleaf,
+                                                  const DLeaf* __restrict__ sigs, const uint8_t* __restrict__ digests,
+                                                  const uint8_t* __restrict__ scratch, size_t stride,
+                                                  uint8_t* __restrict__ leaf_out, uint8_t* __restrict__ sig_out) {
+    __shared__ typename F::Node pk[F::LEN];
+    if (blockIdx.x < nleaf)
+        lat_fin_one<F, LAT_LEAF>(blockIdx.x, pk, leaves, nullptr, nullptr, nullptr, scratch, stride, leaf_out);
+    else
+        lat_fin_one<F, LAT_SIGN>(blockIdx.x - nleaf, pk, sigs, digests, nullptr, nullptr,
+                                 scratch + (size_t)nleaf * stride, stride, sig_out);
 }
 
 // The state part of one BDS signature per workgroup (everything after R, H_msg

@@ -27,7 +27,8 @@ constexpr uint32_t LAT_LEAF = 0, LAT_SIGN = 1, LAT_VERIFY = 2;
 // tree height (auth levels) of this request, p1 bit 0 = precompute all 15
 // steps of every chain (the message is not known yet when k_lat_pre runs,
 // as for the upper XMSS^MT layers), p2 = the offset of its auth path, in
-// nodes, from the auth pointer of the k_lat_fin launch.
+// nodes, from the auth pointer of the k_lat_fin launch.  LAT_SIGN requests
+// use p1 bit 0 the same way (then k_lat_pre reads no message).
 constexpr uint32_t LAT_ALL_STEPS = 1;
 
 // Per-request scratch: chain starts [LEN], F pre-images [LEN][15], H
@@ -78,14 +79,13 @@ __device__ __forceinline__ void lat_pre_h_item(const typename F::Key& k, const D
     reinterpret_cast<typename F::HPre*>(S + LatLayout<F>::hpre)[u] = p;
 }
 
+// Item t of request r (one lane each).
 template <class F, uint32_t MODE>
-__global__ __launch_bounds__(WG) void k_lat_pre(const typename F::Key* __restrict__ keys,
-                                                 const DLeaf* __restrict__ reqs,
-                                                 const uint8_t* __restrict__ msgs,
-                                                 uint8_t* __restrict__ scratch, size_t stride) {
+__device__ __forceinline__ void lat_pre_one(const typename F::Key* __restrict__ keys,
+                                            const DLeaf* __restrict__ reqs, uint32_t r, uint32_t t,
+                                            const uint8_t* __restrict__ msgs, uint8_t* __restrict__ scratch,
+                                            size_t stride) {
     using Lay = LatLayout<F>;
-    const uint32_t r = blockIdx.y;
-    const uint32_t t = blockIdx.x * WG + threadIdx.x;
     constexpr uint32_t LEN = F::LEN;
     const DLeaf q = reqs[r];
     const uint32_t nf = LEN * 15, nh = (MODE == LAT_SIGN) ? 0 : LEN - 1 + (MODE == LAT_VERIFY ? q.p0 : 0);
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(WG) void k_lat_pre(const typename F::Key* __restric
     }
     if (t < LEN + nf) {  // chain c, step s
         const uint32_t c = (t - LEN) / 15, s = (t - LEN) % 15;
-        if (MODE == LAT_SIGN || (MODE == LAT_VERIFY && !(q.p1 & LAT_ALL_STEPS))) {
+        if (!(q.p1 & LAT_ALL_STEPS) && (MODE == LAT_SIGN || MODE == LAT_VERIFY)) {
             const uint32_t len = chain_len<F>(msgs + (size_t)r * F::kN, c);
             if (MODE == LAT_VERIFY ? s < len : s >= len) return;
         }
@@ -113,6 +113,14 @@ __global__ __launch_bounds__(WG) void k_lat_pre(const typename F::Key* __restric
         return;
     }
     lat_pre_h_item<F>(k, q, t - LEN - nf, S);
+}
+
+template <class F, uint32_t MODE>
+__global__ __launch_bounds__(WG) void k_lat_pre(const typename F::Key* __restrict__ keys,
+                                                 const DLeaf* __restrict__ reqs,
+                                                 const uint8_t* __restrict__ msgs,
+                                                 uint8_t* __restrict__ scratch, size_t stride) {
+    lat_pre_one<F, MODE>(keys, reqs, blockIdx.y, blockIdx.x * WG + threadIdx.x, msgs, scratch, stride);
 }
 
 // Only the H pre-images of LAT_VERIFY requests that all have tree height th
@@ -134,17 +142,15 @@ __global__ __launch_bounds__(WG) void k_lat_pre_h(const typename F::Key* __restr
 //   LAT_LEAF:   out[r] = leaf (n bytes)
 //   LAT_SIGN:   out[r] = the WOTS signature (LEN x n bytes) of msgs[r]
 //   LAT_VERIFY: out[r] = the root implied by sigs[r], msgs[r] and auth[r]
+// Request r on the calling workgroup; pk: LEN nodes of LDS.
 template <class F, uint32_t MODE>
-__global__ __launch_bounds__(192) void k_lat_fin(const DLeaf* __restrict__ reqs, const uint8_t* __restrict__ msgs,
-                                                  const uint8_t* __restrict__ sigs,
-                                                  const uint8_t* __restrict__ auth,
-                                                  const uint8_t* __restrict__ scratch, size_t stride,
-                                                  uint8_t* __restrict__ out) {
+__device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, const DLeaf* __restrict__ reqs,
+                                            const uint8_t* __restrict__ msgs, const uint8_t* __restrict__ sigs,
+                                            const uint8_t* __restrict__ auth, const uint8_t* __restrict__ scratch,
+                                            size_t stride, uint8_t* __restrict__ out) {
     using Lay = LatLayout<F>;
     using FPre = typename F::FPre;
     using HPre = typename F::HPre;
-    __shared__ typename F::Node pk[F::LEN];
-    const uint32_t r = blockIdx.x;
     const uint32_t c = threadIdx.x;
     const uint8_t* S = scratch + (size_t)r * stride;
     const HPre* hp = reinterpret_cast<const HPre*>(S + Lay::hpre);
@@ -201,6 +207,16 @@ __global__ __launch_bounds__(192) void k_lat_fin(const DLeaf* __restrict__ reqs,
         }
     }
     F::store(node, out + (size_t)r * F::kN);
+}
+
+template <class F, uint32_t MODE>
+__global__ __launch_bounds__(192) void k_lat_fin(const DLeaf* __restrict__ reqs, const uint8_t* __restrict__ msgs,
+                                                  const uint8_t* __restrict__ sigs,
+                                                  const uint8_t* __restrict__ auth,
+                                                  const uint8_t* __restrict__ scratch, size_t stride,
+                                                  uint8_t* __restrict__ out) {
+    __shared__ typename F::Node pk[F::LEN];
+    lat_fin_one<F, MODE>(blockIdx.x, pk, reqs, msgs, sigs, auth, scratch, stride, out);
 }
 
 // ---- subtree levels of a few trees -------------------------------------
