@@ -1239,7 +1239,11 @@ struct VerifyBufs {
 };
 
 void Engine::verify_many(int fam, int count, const VerifyJob* jobs, int* ok) {
-    constexpr int kChunk = 8192;  // keeps every launch's grid within its limits
+    // chunks keep every launch's grid within its limits (the pre-image launch
+    // has one row per (request, layer) plus one per 64 messages)
+    unsigned dall = 1;
+    for (int j = 0; j < count; ++j) dall = std::max(dall, jobs[j].d);
+    const int kChunk = std::min(8192, (int)(65000 / (dall + 1)));
     if (count > kChunk) {
         for (int i = 0; i < count; i += kChunk) verify_many(fam, std::min(kChunk, count - i), jobs + i, ok + i);
         return;
@@ -1256,6 +1260,7 @@ void Engine::verify_many(int fam, int count, const VerifyJob* jobs, int* ok) {
     for (int i = 0; i < count; ++i) ord[i] = i;
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return jobs[a].d > jobs[b].d; });
     const unsigned dmax = count ? jobs[ord[0]].d : 0;
+    if (dmax > 16) fatal("verify_many: more than 16 layers");
     std::vector<uint32_t> cnt(dmax, 0), base(dmax + 1, 0);
     unsigned thmax = 0;
     for (int j = 0; j < count; ++j) {
@@ -1311,7 +1316,8 @@ void Engine::verify_many(int fam, int count, const VerifyJob* jobs, int* ok) {
             const uint32_t leaf = (uint32_t)(idx & ((1ull << v.th) - 1));
             idx >>= v.th;
             const uint32_t t = base[L] + fill[L]++;
-            it[t] = {v.key, L, (uint32_t)(idx >> 32), (uint32_t)idx, leaf, v.th, LAT_ALL_STEPS, afill[L]};
+            it[t] = {v.key, L, (uint32_t)(idx >> 32), (uint32_t)idx, leaf, v.th, LAT_ALL_STEPS,
+                     auth_at[L] + afill[L]};
             memcpy(h + o_sig + (size_t)t * len * n, s, (size_t)len * n);
             s += (size_t)len * n;
             memcpy(h + o_auth + (size_t)(auth_at[L] + afill[L]) * n, s, (size_t)v.th * n);
@@ -1325,27 +1331,13 @@ void Engine::verify_many(int fam, int count, const VerifyJob* jobs, int* ok) {
     uint8_t* roots = V.roots.need((size_t)items * n);
     uint8_t* dig = V.dig.need((size_t)count * n);
     uint8_t* scratch = V.scratch.need((size_t)items * o->lat_bytes(thmax));
-    // address-only work of every layer on the aux stream beside H_msg
-    hipEvent_t staged = I->L.ev(), pre = I->L.ev();
-    HIPCK(hipEventRecord(staged, I->L.s_main));
-    HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
-    I->L.s = I->L.s_aux;
-    o->lat_pre(I->L, dk, LAT_VERIFY, dit, items, nullptr, thmax, scratch);
-    HIPCK(hipEventRecord(pre, I->L.s_aux));
-    I->L.s = I->L.s_main;
-    o->hmsg(I->L, dk, dmd, (uint32_t)count, pad, dv + o_blob, nullptr, dig);
-    HIPCK(hipStreamWaitEvent(I->L.s_main, pre, 0));
-    const size_t stride = o->lat_bytes(thmax);
-    for (unsigned L = 0; L < dmax; ++L) {
-        const uint8_t* msgs = L == 0 ? dig : roots + (size_t)base[L - 1] * n;
-        o->lat_fin(I->L, LAT_VERIFY, dit + base[L], cnt[L], msgs, dv + o_sig + (size_t)base[L] * len * n,
-                   dv + o_auth + (size_t)auth_at[L] * n, thmax, scratch + (size_t)base[L] * stride,
-                   roots + (size_t)base[L] * n);
-    }
+    // two launches on one stream: H_msg with the address-only work of every
+    // layer, then every layer of each request in one workgroup
+    o->verify_pre(I->L, dk, dmd, (uint32_t)count, pad, dv + o_blob, dig, dit, items, thmax, scratch);
+    o->verify_layers(I->L, dit, (uint32_t)count, base.data(), dmax, dig, dv + o_sig, dv + o_auth, thmax, scratch,
+                     roots);
     I->d2h(h, roots, (size_t)items * n);
     I->sync();
-    I->L.pool.push_back(staged);
-    I->L.pool.push_back(pre);
     for (int j = 0; j < count; ++j) {
         const VerifyJob& v = jobs[ord[j]];
         ok[ord[j]] = !memcmp(h + (size_t)(base[v.d - 1] + j) * n, v.pk_root, n);

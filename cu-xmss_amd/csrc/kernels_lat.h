@@ -123,6 +123,25 @@ __global__ __launch_bounds__(WG) void k_lat_pre(const typename F::Key* __restric
     lat_pre_one<F, MODE>(keys, reqs, blockIdx.y, blockIdx.x * WG + threadIdx.x, msgs, scratch, stride);
 }
 
+// H_msg of nmsg verification requests (the first rows, one message per lane:
+// the longest-running lanes start first) and the address-only work of their
+// verify layers (LAT_VERIFY requests with LAT_ALL_STEPS: the upper layers'
+// messages are not known yet) in one launch.
+template <class F>
+__global__ __launch_bounds__(WG) void k_ver_pre(const typename F::Key* __restrict__ keys,
+                                                 const DMsg* __restrict__ msgs, uint32_t nmsg, uint32_t pad,
+                                                 uint8_t* __restrict__ blob, uint8_t* __restrict__ digests,
+                                                 uint32_t hrows, const DLeaf* __restrict__ reqs,
+                                                 uint8_t* __restrict__ scratch, size_t stride) {
+    const uint32_t t = blockIdx.x * WG + threadIdx.x;
+    if (blockIdx.y < hrows) {
+        const uint32_t i = blockIdx.y * gridDim.x * WG + t;
+        if (i < nmsg) hmsg_one<F>(keys, msgs, i, pad, blob, nullptr, digests);
+        return;
+    }
+    lat_pre_one<F, LAT_VERIFY>(keys, reqs, blockIdx.y - hrows, t, nullptr, scratch, stride);
+}
+
 // Only the H pre-images of LAT_VERIFY requests that all have tree height th
 // (their F pre-images then come from k_pre_need, once the message is known),
 // flat over (request, merge) items.
@@ -217,6 +236,34 @@ __global__ __launch_bounds__(192) void k_lat_fin(const DLeaf* __restrict__ reqs,
                                                   uint8_t* __restrict__ out) {
     __shared__ typename F::Node pk[F::LEN];
     lat_fin_one<F, MODE>(blockIdx.x, pk, reqs, msgs, sigs, auth, scratch, stride, out);
+}
+
+// Every verify layer of a request in one workgroup, bottom up (one launch
+// for any XMSS^MT depth): request j has layer L (item base.b[L] + j of reqs,
+// sigs and scratch) while j < base.b[L + 1] - base.b[L]; layer 0 verifies
+// digests[j], layer L the root layer L - 1 left in roots.  The roots of every
+// item are written to roots; auth offsets (DLeaf p2) count from auth.
+struct LayerBase {
+    uint32_t b[17];
+};
+template <class F>
+__global__ __launch_bounds__(192) void k_lat_verify_layers(const DLeaf* __restrict__ reqs, LayerBase base,
+                                                            uint32_t dmax, const uint8_t* __restrict__ digests,
+                                                            const uint8_t* __restrict__ sigs,
+                                                            const uint8_t* __restrict__ auth,
+                                                            const uint8_t* __restrict__ scratch, size_t stride,
+                                                            uint8_t* __restrict__ roots) {
+    __shared__ typename F::Node pk[F::LEN];
+    const uint32_t j = blockIdx.x;
+    for (uint32_t L = 0; L < dmax && j < base.b[L + 1] - base.b[L]; ++L) {
+        const uint32_t t = base.b[L] + j;
+        // the message of item t is at msgs + t * n: the digest, or the root of item base.b[L - 1] + j
+        const uint8_t* msgs = L == 0 ? digests - (size_t)base.b[0] * F::kN
+                                     : roots - (size_t)(base.b[L] - base.b[L - 1]) * F::kN;
+        lat_fin_one<F, LAT_VERIFY>(t, pk, reqs, msgs, sigs, auth, scratch, stride, roots);
+        __threadfence_block();
+        __syncthreads();
+    }
 }
 
 // ---- subtree levels of a few trees -------------------------------------

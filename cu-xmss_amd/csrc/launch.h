@@ -116,6 +116,17 @@ struct FamOps {
                      uint8_t* scratch, uint8_t* out) = 0;
     virtual void lat_pre(Launcher& L, const void* keys, uint32_t mode, const DLeaf* reqs, uint32_t n,
                          const uint8_t* msgs, uint32_t th, uint8_t* scratch) = 0;
+    // H_msg of nmsg messages (digests) with lat_pre(LAT_VERIFY) of n requests
+    // (all of them LAT_ALL_STEPS) in one launch; n + ceil(nmsg / 64) <= 65535.
+    virtual void verify_pre(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad,
+                            uint8_t* blob, uint8_t* digests, const DLeaf* reqs, uint32_t n, uint32_t th,
+                            uint8_t* scratch) = 0;
+    // Every verify layer of n requests (request j: items base[L] + j for
+    // L < its depth, requests sorted deepest first; dmax <= 16), one
+    // workgroup per request; DLeaf p2 = auth offset from auth.
+    virtual void verify_layers(Launcher& L, const DLeaf* reqs, uint32_t n, const uint32_t* base, uint32_t dmax,
+                               const uint8_t* digests, const uint8_t* sigs, const uint8_t* auth, uint32_t th,
+                               const uint8_t* scratch, uint8_t* roots) = 0;
     virtual void lat_fin(Launcher& L, uint32_t mode, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
                          const uint8_t* sigs, const uint8_t* auth, uint32_t th, const uint8_t* scratch,
                          uint8_t* out) = 0;
@@ -218,6 +229,21 @@ struct FamOpsT final : FamOps {
         if (mode == LAT_LEAF) lat_pre_t<LAT_LEAF>(L, keys, reqs, n, msgs, th, scratch);
         else if (mode == LAT_SIGN) lat_pre_t<LAT_SIGN>(L, keys, reqs, n, msgs, th, scratch);
         else lat_pre_t<LAT_VERIFY>(L, keys, reqs, n, msgs, th, scratch);
+    }
+    void verify_pre(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad, uint8_t* blob,
+                    uint8_t* digests, const DLeaf* reqs, uint32_t n, uint32_t th, uint8_t* scratch) override {
+        const uint32_t xb = cdiv(F::LEN * 17 - 1 + th, WG);
+        const uint32_t hrows = cdiv(nmsg, (uint64_t)xb * WG);
+        L.run("verify_pre", k_ver_pre<F>, dim3(xb, hrows + n), dim3(WG), (const K*)keys, msgs, nmsg, pad, blob,
+              digests, hrows, reqs, scratch, LatLayout<F>::bytes(th));
+    }
+    void verify_layers(Launcher& L, const DLeaf* reqs, uint32_t n, const uint32_t* base, uint32_t dmax,
+                       const uint8_t* digests, const uint8_t* sigs, const uint8_t* auth, uint32_t th,
+                       const uint8_t* scratch, uint8_t* roots) override {
+        LayerBase lb{};
+        for (uint32_t i = 0; i <= dmax && i < 17; ++i) lb.b[i] = base[i];
+        L.run("verify_layers", k_lat_verify_layers<F>, dim3(n), dim3(((F::LEN + 63) / 64) * 64), reqs, lb, dmax,
+              digests, sigs, auth, scratch, LatLayout<F>::bytes(th), roots);
     }
     void lat_pre_h(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, uint32_t th,
                    uint8_t* scratch) override {
