@@ -176,8 +176,14 @@ public:
     // items whose signatures use the same (layer, tree) share one subtree build.
     Batch* sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items, bool one_key = false);
     Batch* verify_prepare(int fam, unsigned th, unsigned d, int count, const VerifyItem* items);
+    // New inputs for a verification batch of the same shape (count items of
+    // its parameter set), staged into its buffers; its captured launch graph
+    // stays valid unless the messages no longer fit (then it is dropped).
+    void verify_refill(Batch* b, const VerifyItem* items);
     // wait_done = false returns once the work is queued; wait() then blocks
-    // until this engine's streams are idle.
+    // until this engine's streams are idle.  A verification batch's launches
+    // are captured into a HIP graph on its second run and replayed from then
+    // on (not while per-kernel timing is on).
     void run(Batch* b, bool wait_done = true);
     void wait();
     // sign: per item R (n) || d x (WOTS sig || auth path) written to out[i]

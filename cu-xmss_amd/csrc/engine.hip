@@ -560,6 +560,14 @@ struct Engine::Batch {
     size_t res_bytes = 0;
     int* ok = nullptr;        // verify
     std::vector<int> perm;    // verify: staged position -> the caller's index (longest message first)
+    DevBuf lat;               // verify: pre-image scratch of the latency kernels
+    hipGraphExec_t graph = nullptr;  // verify: the captured launches of run()
+    int nrun = 0;                    // verify: runs since staging
+    void drop_graph() {
+        if (graph) HIPCK(hipGraphExecDestroy(graph));
+        graph = nullptr;
+        nrun = 0;
+    }
 };
 
 __global__ __launch_bounds__(WG) void k_compare(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b,
@@ -629,6 +637,7 @@ static void fill_seeds(uint8_t* h, unsigned n, const std::vector<KeySeeds>& ks) 
 static Engine::Batch* take_batch(Engine::Impl* I, bool sign) {
     Engine::Batch* b = I->spare[sign];
     I->spare[sign] = nullptr;
+    if (b) b->drop_graph();
     return b ? b : new Engine::Batch;
 }
 
@@ -723,18 +732,13 @@ Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count,
     return b;
 }
 
-Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int count, const VerifyItem* items) {
-    Impl* I = impl_;
-    Batch* b = take_batch(I, false);
-    b->sign = false;
-    b->fam = fam;
-    b->th = th;
-    b->d = d;
-    b->n = fam_n(fam);
-    b->len = fam_len(fam);
-    b->count = count;
-    const unsigned n = b->n, len = b->len;
-    FamOps* o = ops_of(fam);
+// Stages a verification batch's inputs: key contexts, messages, signatures.
+// The message blob comes last, so every other device offset depends only on
+// the batch's shape (count, d, th, n) and a refill of the same shape keeps
+// the pointers a captured graph uses, unless the io buffer has to grow.
+static void stage_verify(Engine::Impl* I, FamOps* o, Engine::Batch* b, const Engine::VerifyItem* items) {
+    const int count = b->count;
+    const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
     // staged longest message first, so that run() can start the rest while
     // the longest messages are still being hashed
     std::vector<int>& ord = b->perm;
@@ -744,22 +748,23 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
     std::vector<KeySeeds> ks(count);
     std::vector<MsgReq> mr(count);
     for (int i = 0; i < count; ++i) {
-        const VerifyItem& it = items[ord[i]];
+        const Engine::VerifyItem& it = items[ord[i]];
         ks[i] = {nullptr, nullptr, it.pk + n};
         mr[i] = {(uint32_t)i, 0, it.idx, it.R, it.pk, it.m, it.mlen};
     }
-    const MsgStage ms = plan_msgs(fam, count, false, mr);
+    const MsgStage ms = plan_msgs(b->fam, count, false, mr);
     Layout in;
-    const size_t o_seeds = in.add(ks.size() * 3 * n), o_md = in.add(sizeof(DMsg) * count), o_blob = in.add(ms.total),
+    const size_t o_seeds = in.add(ks.size() * 3 * n), o_md = in.add(sizeof(DMsg) * count),
                  o_vd = in.add(sizeof(DLeaf) * count * d), o_sig = in.add((size_t)count * d * len * n),
-                 o_auth = in.add((size_t)count * d * th * n), o_pk = in.add((size_t)count * n);
+                 o_auth = in.add((size_t)count * d * th * n), o_pk = in.add((size_t)count * n),
+                 o_blob = in.add(ms.total);
     uint8_t* h = I->pin.need(in.size);
     fill_seeds(h + o_seeds, n, ks);
     memcpy(h + o_md, ms.d.data(), sizeof(DMsg) * count);
-    fill_msgs(h + o_blob, fam, false, mr, ms);
+    fill_msgs(h + o_blob, b->fam, false, mr, ms);
     DLeaf* vd = reinterpret_cast<DLeaf*>(h + o_vd);
     for (int i = 0; i < count; ++i) {
-        const VerifyItem& it = items[ord[i]];
+        const Engine::VerifyItem& it = items[ord[i]];
         uint64_t idx = it.idx;
         const uint8_t* p = it.wots;
         for (unsigned L = 0; L < d; ++L) {
@@ -774,7 +779,9 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
         }
         memcpy(h + o_pk + (size_t)i * n, it.pk, n);
     }
+    uint8_t* const before = (uint8_t*)b->io.p;
     uint8_t* dv = b->io.need(in.size);
+    if (before && dv != before) b->drop_graph();  // the buffer moved: the captured pointers are stale
     I->h2d(dv, h, in.size);
     b->seeds = dv + o_seeds;
     b->md = reinterpret_cast<const DMsg*>(dv + o_md);
@@ -784,13 +791,29 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
     b->auth_in = dv + o_auth;
     b->pkroot = dv + o_pk;
     o->prepare(I->L, b->seeds, count, b->keys.need(o->key_bytes() * count));
+    I->sync();
+}
+
+Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int count, const VerifyItem* items) {
+    Impl* I = impl_;
+    Batch* b = take_batch(I, false);
+    b->sign = false;
+    b->fam = fam;
+    b->th = th;
+    b->d = d;
+    b->n = fam_n(fam);
+    b->len = fam_len(fam);
+    b->count = count;
+    FamOps* o = ops_of(fam);
     b->ok = b->res.need<int>(sizeof(int) * (size_t)count);
     b->res_bytes = sizeof(int) * (size_t)count;
-    b->dig.need((size_t)count * n);
-    b->roots.need((size_t)count * d * n);
-    I->sync();
+    b->dig.need((size_t)count * b->n);
+    b->roots.need((size_t)count * d * b->n);
+    stage_verify(I, o, b, items);
     return b;
 }
+
+void Engine::verify_refill(Batch* b, const VerifyItem* items) { stage_verify(impl_, ops_of(b->fam), b, items); }
 
 // Large verification batches (staged longest message first).  The requests
 // split into two groups by message length, each on its own stream: the
@@ -864,21 +887,18 @@ static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* 
 
 void Engine::wait() { impl_->sync(); }
 
-void Engine::run(Batch* b, bool wait_done) {
-    Impl* I = impl_;
-    FamOps* o = ops_of(b->fam);
+// The launches of one verification batch: verify_dense for large batches,
+// else H_msg beside the address-only work of every layer (all 15 steps of
+// each chain, aux stream), then one finishing launch per layer.
+static void verify_launches(Engine::Impl* I, FamOps* o, Engine::Batch* b) {
     const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
     const int count = b->count;
     const void* dk = b->keys.p;
-    // verification through the latency kernels: the address-only work of
-    // every layer (all 15 steps of each chain) runs on the aux stream beside
-    // H_msg, which it does not need
     const size_t nv = (size_t)count * d;
-    const bool vlat = !b->sign && use_lat(o, nv, th);
-    uint8_t* vscratch = vlat ? I->lat.need(nv * o->lat_bytes(th)) : nullptr;
+    const bool vlat = use_lat(o, nv, th);
+    uint8_t* vscratch = vlat ? b->lat.need(nv * o->lat_bytes(th)) : nullptr;
     if (vlat && count >= kLatDense) {
         verify_dense(I, o, b, vscratch);
-        if (wait_done) I->sync();
         return;
     }
     hipEvent_t staged = nullptr, pre = nullptr;
@@ -894,7 +914,63 @@ void Engine::run(Batch* b, bool wait_done) {
     }
     o->hmsg(I->L, dk, b->md, (uint32_t)count, fam_pad(b->fam), b->blob, b->R, (uint8_t*)b->dig.p);
     if (vlat) HIPCK(hipStreamWaitEvent(I->L.s_main, pre, 0));
-    if (b->sign) {
+    for (unsigned L = 0; L < d; ++L) {
+        const uint8_t* msg_l = L == 0 ? (const uint8_t*)b->dig.p : (const uint8_t*)b->roots.p + (size_t)(L - 1) * count * n;
+        const DLeaf* vd = b->vd + (size_t)L * count;
+        const uint8_t* sig_l = b->sigs + (size_t)L * count * len * n;
+        const uint8_t* auth_l = b->auth_in + (size_t)L * count * th * n;
+        uint8_t* root_l = (uint8_t*)b->roots.p + (size_t)L * count * n;
+        if (vlat)
+            o->lat_fin(I->L, LAT_VERIFY, vd, (uint32_t)count, msg_l, sig_l, auth_l, th,
+                       vscratch + (size_t)L * count * o->lat_bytes(th), root_l);
+        else
+            o->leaf_wg(I->L, dk, vd, (uint32_t)count, true, msg_l, sig_l, auth_l, th, root_l);
+    }
+    I->L.run("compare", k_compare, dim3(cdiv(count, WG)), dim3(WG),
+             (const uint8_t*)b->roots.p + (size_t)(d - 1) * count * n, b->pkroot, (uint32_t)count, n, b->ok);
+    if (vlat) {
+        I->L.pool.push_back(staged);
+        I->L.pool.push_back(pre);
+    }
+}
+
+// A verification batch: launched directly on its first run (which sizes
+// every buffer), captured into a HIP graph on its second and replayed from
+// then on -- one launch call instead of tens.  The capture holds no
+// allocation, memset or host copy; with per-kernel timing on, the launches
+// stay direct.
+static void run_verify(Engine::Impl* I, Engine::Batch* b) {
+    FamOps* o = ops_of(b->fam);
+    if (b->graph && !I->L.prof) {
+        HIPCK(hipGraphLaunch(b->graph, I->L.s_main));
+        return;
+    }
+    if (I->L.prof || b->nrun++ < 1) {
+        verify_launches(I, o, b);
+        return;
+    }
+    hipGraph_t g;
+    HIPCK(hipStreamBeginCapture(I->L.s_main, hipStreamCaptureModeRelaxed));
+    verify_launches(I, o, b);
+    HIPCK(hipStreamEndCapture(I->L.s_main, &g));
+    HIPCK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+    HIPCK(hipGraphDestroy(g));
+    HIPCK(hipGraphLaunch(b->graph, I->L.s_main));
+}
+
+void Engine::run(Batch* b, bool wait_done) {
+    Impl* I = impl_;
+    if (!b->sign) {
+        run_verify(I, b);
+        if (wait_done) I->sync();
+        return;
+    }
+    FamOps* o = ops_of(b->fam);
+    const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
+    const int count = b->count;
+    const void* dk = b->keys.p;
+    o->hmsg(I->L, dk, b->md, (uint32_t)count, fam_pad(b->fam), b->blob, b->R, (uint8_t*)b->dig.p);
+    {
         // all subtrees in one pipeline (few: levels split as in build_trees)
         const int nruns = b->nruns;
         const bool split = nruns <= kSplitRuns;
@@ -972,29 +1048,6 @@ void Engine::run(Batch* b, bool wait_done) {
             o->wots(I->L, dk, b->wd + (size_t)L * count, (uint32_t)count, msg_l, nullptr,
                     b->wots + (size_t)L * count * len * n);
         }
-    } else {
-        for (unsigned L = 0; L < d; ++L) {
-            const uint8_t* msg_l = L == 0 ? (const uint8_t*)b->dig.p : (const uint8_t*)b->roots.p + (size_t)(L - 1) * count * n;
-            const DLeaf* vd = b->vd + (size_t)L * count;
-            const uint8_t* sig_l = b->sigs + (size_t)L * count * len * n;
-            const uint8_t* auth_l = b->auth_in + (size_t)L * count * th * n;
-            uint8_t* root_l = (uint8_t*)b->roots.p + (size_t)L * count * n;
-            const uint8_t* scr_l = vscratch + (size_t)L * count * o->lat_bytes(th);
-            if (vlat && count >= kLatDense)
-                o->lat_verify_batch(I->L, vd, (uint32_t)count, msg_l, sig_l, auth_l, th, scr_l,
-                                    b->ends.need((size_t)count * len * n),
-                                    b->tmp.need((size_t)count * ((len + 1) / 2) * n), root_l);
-            else if (vlat)
-                o->lat_fin(I->L, LAT_VERIFY, vd, (uint32_t)count, msg_l, sig_l, auth_l, th, scr_l, root_l);
-            else
-                o->leaf_wg(I->L, dk, vd, (uint32_t)count, true, msg_l, sig_l, auth_l, th, root_l);
-        }
-        I->L.run("compare", k_compare, dim3(cdiv(count, WG)), dim3(WG),
-                 (const uint8_t*)b->roots.p + (size_t)(d - 1) * count * n, b->pkroot, (uint32_t)count, n, b->ok);
-    }
-    if (vlat) {
-        I->L.pool.push_back(staged);
-        I->L.pool.push_back(pre);
     }
     if (wait_done) I->sync();
 }
@@ -1036,7 +1089,8 @@ void Engine::free_batch(Batch* b) {
         impl_->spare[b->sign] = b;
         return;
     }
-    DevBuf* bufs[] = {&b->io, &b->res, &b->keys, &b->dig, &b->nodes, &b->ends, &b->tmp, &b->roots};
+    b->drop_graph();
+    DevBuf* bufs[] = {&b->io, &b->res, &b->keys, &b->dig, &b->nodes, &b->ends, &b->tmp, &b->roots, &b->lat};
     for (DevBuf* x : bufs) {
         if (x->p) HIPCK(hipFree(x->p));
         x->p = nullptr;

@@ -41,6 +41,7 @@ struct xmss_mi355x_job {
     std::vector<const unsigned char*> sms;
     std::vector<unsigned long long> smlens;
     std::vector<const unsigned char*> pks;
+    std::vector<uint32_t> oids;  // of each input's pk, for refills
 };
 
 namespace {
@@ -195,8 +196,10 @@ xmss_mi355x_job* verify_prepare(const int* mts, int mt_all, const unsigned char*
     j->pks.assign(pks, pks + count);
     j->rets.assign(count, -1);
     j->set.assign(count, -1);
+    j->oids.resize(count);
     std::vector<std::vector<Engine::VerifyItem>> items;
     for (size_t i = 0; i < count; ++i) {
+        j->oids[i] = oid_of(pks[i]);
         const int s = set_of(j, mts ? mts[i] : mt_all, oid_of(pks[i]), mixed);
         if (s < 0) continue;
         j->set[i] = s;
@@ -247,6 +250,37 @@ xmss_mi355x_job* xmss_mi355x_mixed_verify_prepare(const int* mts, const unsigned
                                                   const unsigned long long* smlens,
                                                   const unsigned char* const* pks, size_t count) {
     return verify_prepare(mts, 0, sms, smlens, pks, count, true);
+}
+
+int xmss_mi355x_verify_refill(xmss_mi355x_job* j, const unsigned char* const* sms,
+                              const unsigned long long* smlens, const unsigned char* const* pks) {
+    if (!j || j->sign) return -1;
+    // the same shape: each input keeps its public key's OID and whether it
+    // reaches the device (a signed message at least sig_bytes long)
+    std::vector<std::vector<Engine::VerifyItem>> items(j->sets.size());
+    std::vector<size_t> at(j->sets.size(), 0);
+    for (size_t i = 0; i < j->count; ++i) {
+        if (oid_of(pks[i]) != j->oids[i]) return -1;
+        const int s = j->set[i];
+        if (s < 0) continue;
+        JobSet& js = j->sets[s];
+        const xmss_params& p = js.p;
+        const bool live = at[s] < js.live.size() && js.live[at[s]] == i;
+        if (live != (smlens[i] >= p.sig_bytes)) return -1;
+        if (!live) continue;
+        ++at[s];
+        const unsigned n = p.n, ib = p.index_bytes;
+        const unsigned char* sm = sms[i];
+        items[s].push_back({pks[i] + XMSS_OID_LEN, bytes_to_ull(sm, ib), sm + ib, sm + ib + n, sm + p.sig_bytes,
+                            smlens[i] - p.sig_bytes});
+    }
+    j->sms.assign(sms, sms + j->count);
+    j->smlens.assign(smlens, smlens + j->count);
+    j->pks.assign(pks, pks + j->count);
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    for (size_t s = 0; s < j->sets.size(); ++s)
+        if (j->sets[s].b) j->sets[s].e->verify_refill(j->sets[s].b, items[s].data());
+    return 0;
 }
 
 int xmss_mi355x_job_run(xmss_mi355x_job* j) {

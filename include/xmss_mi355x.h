@@ -68,6 +68,16 @@ xmss_mi355x_job *xmss_mi355x_mixed_sign_prepare(const int *mts, unsigned char *c
 xmss_mi355x_job *xmss_mi355x_mixed_verify_prepare(const int *mts, const unsigned char *const *sms,
                                                   const unsigned long long *smlens,
                                                   const unsigned char *const *pks, size_t count);
+/* New inputs for a verification job, of its shape: the same count, each
+ * input's public key of the same parameter set (OID) as before and each
+ * signed message again at least sig_bytes long (or again shorter).  The job
+ * keeps its device buffers and, once run twice, the HIP graph its launches
+ * were captured into, so a stream of same-shape batches replays one graph
+ * per batch.  Returns 0, or -1 (job unchanged) when the shape differs. */
+int xmss_mi355x_verify_refill(xmss_mi355x_job *job, const unsigned char *const *sms,
+                              const unsigned long long *smlens, const unsigned char *const *pks);
+/* Runs a job on the GPU (no host traffic).  A verification job's launches
+ * are captured into a HIP graph on its second run and replayed afterwards. */
 int xmss_mi355x_job_run(xmss_mi355x_job *job);
 int xmss_mi355x_sign_collect(xmss_mi355x_job *job, unsigned char *const *sms,
                              unsigned long long *smlens, int *rets);

@@ -239,3 +239,83 @@ def test_large_verify_batch_accepts_exactly_the_valid(name, mt):
     for j in range(0, k, 37):
         rc, m = lib.sign_open(sms[j], pk, mt)
         assert rc == vr[j]
+
+
+def _verify_job_api(L):
+    L.xmss_mi355x_verify_prepare.restype = C.c_void_p
+    L.xmss_mi355x_verify_prepare.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    L.xmss_mi355x_verify_refill.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.xmss_mi355x_job_run.argtypes = [C.c_void_p]
+    L.xmss_mi355x_verify_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.xmss_mi355x_job_free.argtypes = [C.c_void_p]
+
+
+@pytest.mark.parametrize("k", [300, 40])  # the large-batch path and the smaller one
+def test_verify_job_replays_and_refills(k):
+    """A verification job run five times (direct launches, then captured into
+    a HIP graph and replayed), with the host collecting and checking between
+    runs, and then refilled twice with new inputs of the same shape (another
+    order and other corrupted signatures): every run's verdicts equal the
+    single sign_open calls.  A refill whose public keys change parameter set
+    is refused."""
+    import random
+    from test_gpu_opk import _sign_many
+    lib = product(0)
+    L = lib.lib
+    _verify_job_api(L)
+    oid, mt = 0x01, 0
+    p = lib.params(oid, mt)
+    L.xmss_mi355x_rng_seed(7 + k)
+    rc, pk, sk = lib.keypair(oid, mt)
+    assert rc == 0
+    rng = random.Random(k)
+    msgs = [bytes(rng.getrandbits(8) for _ in range(int(32 * 128 ** rng.random()))) for _ in range(k)]
+    out, _ = _sign_many(lib, sk, msgs, mt, p)
+    good = [sm for rc, sm in out]
+
+    def variant(seed):
+        r = random.Random(seed)
+        order = list(range(k))
+        r.shuffle(order)
+        sms = []
+        for i in order:
+            sm = bytearray(good[i])
+            if r.random() < 0.3:
+                sm[r.randrange(p.index_bytes, p.sig_bytes)] ^= 1 << r.randrange(8)
+            sms.append(bytes(sm))
+        want = [lib.sign_open(s, pk, mt)[0] for s in sms]
+        return sms, want
+
+    pkb = [buf(pk) for _ in range(k)]
+
+    def check(job, sms, want, label):
+        outs = [buf(len(s)) for s in sms]
+        mls = (C.c_ulonglong * k)()
+        vr = (C.c_int * k)()
+        assert L.xmss_mi355x_verify_collect(job, _arr(outs), mls, vr) == 0
+        assert [vr[j] for j in range(k)] == want, label
+        assert 0 in want and -1 in want
+
+    sms, want = variant(1)
+    smb = [buf(s) for s in sms]
+    job = L.xmss_mi355x_verify_prepare(mt, _arr(smb), (C.c_ulonglong * k)(*[len(s) for s in sms]), _arr(pkb), k)
+    for run in range(5):
+        assert L.xmss_mi355x_job_run(job) == 0
+        check(job, sms, want, "run %d" % run)
+    for seed in (2, 3):
+        sms2, want2 = variant(seed)
+        smb2 = [buf(s) for s in sms2]
+        assert L.xmss_mi355x_verify_refill(job, _arr(smb2), (C.c_ulonglong * k)(*[len(s) for s in sms2]),
+                                           _arr(pkb)) == 0
+        for run in range(2):
+            assert L.xmss_mi355x_job_run(job) == 0
+            check(job, sms2, want2, "refill %d run %d" % (seed, run))
+    # another parameter set at position 0: refused, the job unchanged
+    other = bytearray(pk)
+    other[3] = 0x02
+    bad = [buf(bytes(other))] + pkb[1:]
+    assert L.xmss_mi355x_verify_refill(job, _arr(smb2), (C.c_ulonglong * k)(*[len(s) for s in sms2]),
+                                       _arr(bad)) == -1
+    assert L.xmss_mi355x_job_run(job) == 0
+    check(job, sms2, want2, "after a refused refill")
+    L.xmss_mi355x_job_free(job)
