@@ -695,7 +695,7 @@ static int bds_sign_api(const xmss_params* p, unsigned char* sk, unsigned char* 
     if (bds_device_prepare(p, sk, sm, m, mlen, ds)) {
         // the key context stays in the device pool across signatures
         ds.job.key = (uint32_t)e->pool_slot(fam, {ds.job.sk_seed, ds.job.sk_prf, ds.job.pub_seed});
-        e->bds_sign_batch(fam, 1, &ds.job, true);
+        e->bds_sign_batch(fam, 1, &ds.job);
         bds_device_finish(p, sm, smlen, mlen, ds);
         return 0;
     }

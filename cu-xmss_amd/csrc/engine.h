@@ -194,16 +194,17 @@ public:
     void free_batch(Batch* b);
 
     // ---- BDS signatures with the traversal on the device ----
-    // For each job: the planned leaves (in the order the signature consumes
-    // them), R and H_msg, the layer-0 WOTS signature and the state update of
-    // the serialized sk run in four launches on one stream; sk is updated in
-    // place.  Keys: set from the seeds, or pool slots when use_pool.
+    // For each job: R and H_msg, the planned leaves (in the order the
+    // signature consumes them), the layer-0 WOTS signature and the state
+    // update of the serialized sk, in three launches on one stream per chunk
+    // of jobs; sk (or its resident copy dsk) is updated in place.  key is
+    // the job's slot in the device key pool.
     struct BdsJob {
         const uint8_t* sk_seed;
         const uint8_t* sk_prf;
         const uint8_t* pub_seed;
         const uint8_t* pub_root;
-        uint32_t key;          // pool slot (use_pool)
+        uint32_t key;          // pool slot
         uint8_t* sk;           // OID-less sk, index already advanced
         uint8_t* dsk;          // resident copy of the sk in HBM (index not yet advanced), or null:
                                // the update runs on it in place and sk receives the result
@@ -240,8 +241,9 @@ public:
         uint64_t mlen;
     };
     void verify_many(int fam, int count, const VerifyJob* jobs, int* ok);
-    // Jobs may mix parameter shapes of one hash family.
-    void bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool);
+    // Jobs may mix parameter shapes of one hash family; their keys are pool
+    // slots of that family (BdsJob::key).
+    void bds_sign_batch(int fam, int count, BdsJob* jobs);
 
     // Per-kernel HIP-event timing (off by default).
     void set_profiling(bool on);
@@ -263,6 +265,7 @@ public:
 
 private:
     Engine();
+    void bds_sign_chunk(int fam, int count, BdsJob* jobs);
     void* stream_ = nullptr;
     int dev_ = 0;
     bool prof_on_ = false;

@@ -1104,36 +1104,40 @@ namespace xm {
 
 
 struct BdsBufs {
-    DevBuf io, leaf_arena, dig, scratch, lat_leaf, lat_wots, merge_arena;
+    DevBuf io, leaf_arena, dig, scratch, lat_leaf, merge_arena;
     PinnedBuf staging;
 };
 
-static void bds_two_streams(Engine::Impl* I, FamOps* o, BdsBufs& B, const void* dk, int count, unsigned n,
-                            unsigned pad, const DMsg* dmd, uint8_t* db, uint8_t* dR, const DLeaf* dw, uint8_t* dW,
-                            const DLeaf* dld, size_t nl, const DThash* dmg, size_t nmg, uint8_t* leaf_arena,
-                            void* merge_arena, size_t lds, const DBds* djobs, uint8_t* ska, uint8_t* dT);
 
-void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
-    constexpr int kChunk = 8192;  // keeps every launch's grid within its limits
-    if (count > kChunk && use_pool) {
-        for (int i = 0; i < count; i += kChunk) bds_sign_batch(fam, std::min(kChunk, count - i), jobs + i, true);
-        return;
+void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs) {
+    // chunks that fit one three-launch sequence: the pre-image launch's rows
+    // (planned leaves, signatures, message and merge rows) within the grid
+    // limit and the latency kernels' scratch within kLatScratch
+    const size_t lb = ops_of(fam)->lat_bytes(0);
+    int start = 0;
+    size_t rows = 0, extra = 0;
+    for (int i = 0; i <= count; ++i) {
+        const size_t r = i < count ? jobs[i].nplan + 1 : 0, x = i < count ? jobs[i].nmerge + 1 : 0;
+        const size_t rows2 = rows + r, extra2 = extra + x;
+        if (i == count || (i > start && (rows2 + extra2 / WG + 2 > 65535 || rows2 * lb > kLatScratch))) {
+            if (i > start) bds_sign_chunk(fam, i - start, jobs + start);
+            start = i;
+            rows = r;
+            extra = x;
+        } else {
+            rows = rows2;
+            extra = extra2;
+        }
     }
+}
+
+void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     Impl* I = impl_;
     if (!I->bds) I->bds = new BdsBufs;
     BdsBufs& B = *I->bds;
     FamOps* o = ops_of(fam);
     const unsigned n = fam_n(fam), len = fam_len(fam), pad = fam_pad(fam);
-    if (!use_pool) {
-        std::vector<KeySeeds> ks(count);
-        for (int i = 0; i < count; ++i) {
-            ks[i] = {jobs[i].sk_seed, jobs[i].sk_prf, jobs[i].pub_seed};
-            jobs[i].key = (uint32_t)i;
-        }
-        set_keys(fam, count, ks.data());
-    } else {
-        pool_use(fam);
-    }
+    pool_use(fam);
     const void* dk = I->kp();
     // per-job layout: planned leaves, sk bytes, signature tail
     std::vector<DLeaf> ld;
@@ -1197,11 +1201,9 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
     memset(h + o_blob, 0, total);
     memcpy(h + o_jd, jd.data(), sizeof(DBds) * count);
     memcpy(h + o_mg, mg.data(), sizeof(DThash) * mg.size());
-    // one stream, three launches (bds_lat) unless the grids would be too large
     const size_t nl = ld.size();
-    const size_t xrows = (mg.size() + count) / WG + 2;  // bounds its message and merge rows
-    const bool one_stream = use_lat(o, nl + count, 0) && nl + count + xrows <= 65535;
-    const uint32_t all_steps = one_stream ? LAT_ALL_STEPS : 0;  // WOTS pre-images before the digest exists
+    if (nl + count + (mg.size() + count) / WG + 2 > 65535 || !use_lat(o, nl + count, 0))
+        fatal("bds_sign_chunk: batch too large for one launch sequence");
     DLeaf* wd = reinterpret_cast<DLeaf*>(h + o_wd);
     for (int i = 0; i < count; ++i) {
         uint8_t* p = h + o_blob + md[i].off;
@@ -1216,7 +1218,7 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
         const unsigned th = jobs[i].shape.th;
         const uint64_t tree = jobs[i].idx >> th;
         wd[i] = {jobs[i].key, 0, (uint32_t)(tree >> 32), (uint32_t)tree,
-                 (uint32_t)(jobs[i].idx & ((1ull << th) - 1)), 0, all_steps, 0};
+                 (uint32_t)(jobs[i].idx & ((1ull << th) - 1)), 0, LAT_ALL_STEPS, 0};
         if (!jobs[i].dsk) memcpy(h + o_sk + jd[i].sk_off, jobs[i].sk, jobs[i].shape.sk_bytes);
     }
     const DLeaf* dld = reinterpret_cast<const DLeaf*>(dv + o_ld);
@@ -1232,20 +1234,15 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
     uint8_t* leaf_arena = B.leaf_arena.need(ld.size() * n);
     void* merge_arena = B.merge_arena.need(std::max<size_t>(1, mg.size()) * o->hpre_bytes());
     I->h2d(dv, h, o_R);
-    if (one_stream) {
-        // three launches on the main stream: R / H_msg with the address-only
-        // work of the planned leaves, the layer-0 WOTS signatures and the
-        // merges, then the leaves' and signatures' finishing work, then the
-        // state update
-        o->bds_lat(I->L, dk, dmd, pad, db, dR, B.dig.need((size_t)count * n), dld, (uint32_t)nl, dw, (uint32_t)count,
-                   B.lat_leaf.need((nl + count) * o->lat_bytes(0)), dmg, (uint32_t)mg.size(), merge_arena,
-                   leaf_arena, dW);
-        o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, dT,
-                      B.scratch.need((size_t)count * 4 * n));
-    } else {
-        bds_two_streams(I, o, B, dk, count, n, pad, dmd, db, dR, dw, dW, dld, nl, dmg, mg.size(), leaf_arena,
-                        merge_arena, lds, djobs, ska, dT);
-    }
+    // three launches on the main stream: R / H_msg with the address-only work
+    // of the planned leaves, the layer-0 WOTS signatures (all steps: the
+    // digests are computed in the same launch) and the merges, then the
+    // leaves' and signatures' finishing work, then the state update
+    o->bds_lat(I->L, dk, dmd, pad, db, dR, B.dig.need((size_t)count * n), dld, (uint32_t)nl, dw, (uint32_t)count,
+               B.lat_leaf.need((nl + count) * o->lat_bytes(0)), dmg, (uint32_t)mg.size(), merge_arena, leaf_arena,
+               dW);
+    o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, dT,
+                  B.scratch.need((size_t)count * 4 * n));
     I->d2h(h + o_sk, ska, io_bytes - o_sk);
     I->sync();
     for (int i = 0; i < count; ++i) {
@@ -1256,36 +1253,6 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs, bool use_pool) {
     }
 }
 
-// The same launches for batches too large for one bds_lat pair: the planned
-// leaves (the longest dependent chain) on the main stream, issued first; R /
-// H_msg, the layer-0 WOTS signatures and the merges' address-only work on the
-// aux stream; the state update waits for both.
-static void bds_two_streams(Engine::Impl* I, FamOps* o, BdsBufs& B, const void* dk, int count, unsigned n,
-                            unsigned pad, const DMsg* dmd, uint8_t* db, uint8_t* dR, const DLeaf* dw, uint8_t* dW,
-                            const DLeaf* dld, size_t nl, const DThash* dmg, size_t nmg, uint8_t* leaf_arena,
-                            void* merge_arena, size_t lds, const DBds* djobs, uint8_t* ska, uint8_t* dT) {
-    hipEvent_t staged = I->L.ev(), front = I->L.ev();
-    HIPCK(hipEventRecord(staged, I->L.s_main));
-    I->L.s = I->L.s_main;
-    if (use_lat(o, nl, 0))
-        o->lat(I->L, dk, LAT_LEAF, dld, (uint32_t)nl, nullptr, nullptr, nullptr, 0,
-               B.lat_leaf.need(nl * o->lat_bytes(0)), leaf_arena);
-    else if (nl)
-        o->leaf_wg(I->L, dk, dld, (uint32_t)nl, false, nullptr, nullptr, nullptr, 0, leaf_arena);
-    HIPCK(hipStreamWaitEvent(I->L.s_aux, staged, 0));
-    I->L.s = I->L.s_aux;
-    o->hmsg(I->L, dk, dmd, (uint32_t)count, pad, db, dR, B.dig.need((size_t)count * n));
-    o->lat(I->L, dk, LAT_SIGN, dw, (uint32_t)count, (const uint8_t*)B.dig.p, nullptr, nullptr, 0,
-           B.lat_wots.need((size_t)count * o->lat_bytes(0)), dW);
-    if (nmg) o->pre_h(I->L, dk, dmg, (uint32_t)nmg, merge_arena);
-    HIPCK(hipEventRecord(front, I->L.s_aux));
-    I->L.s = I->L.s_main;
-    HIPCK(hipStreamWaitEvent(I->L.s_main, front, 0));
-    o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, dT,
-                  B.scratch.need((size_t)count * 4 * n));
-    I->L.pool.push_back(staged);
-    I->L.pool.push_back(front);
-}
 
 struct VerifyBufs {
     DevBuf io, dig, roots, scratch;

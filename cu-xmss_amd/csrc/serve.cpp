@@ -251,7 +251,7 @@ private:
                         Engine::BdsJob& j = jobs.back();
                         j.key = (uint32_t)e_->pool_slot(fam, {j.sk_seed, j.sk_prf, j.pub_seed});
                     }
-                    e_->bds_sign_batch(fam, (int)jobs.size(), jobs.data(), true);
+                    e_->bds_sign_batch(fam, (int)jobs.size(), jobs.data());
                     break;
                 }
                 case Call::VERIFY: {
