@@ -649,7 +649,7 @@ def main():
         bkeygen_t, sign_run, sign_total, (ver_t, ver_refill_t), bst, n_last = bench_batch(api, args.batch)
     rate = lambda t: None if not t else round(args.batch / t, 2)
     single = None
-    if args.batch and args.single:
+    if args.single:
         log("plain one-request-per-launch x%d ..." % args.single)
         single = bench_plain_single(api, args.single)
     opk = None

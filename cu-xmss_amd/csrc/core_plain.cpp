@@ -73,8 +73,10 @@ int xmssmt_core_sign(const xmss_params* params, unsigned char* sk, unsigned char
     // the index is advanced before the signature exists (xmss_core.c:235-239)
     ull_to_bytes(sk, ib, idx + 1);
     Engine::SignItem it{sk + ib, sk + ib + n, sk + ib + 3 * n, sk + ib + 2 * n, idx, sm + params->sig_bytes, mlen};
-    Engine::Batch* b = e->sign_prepare(fam, params->tree_height, params->d, 1, &it);
-    e->run(b);
+    // one stream, one wait: the staging copy, the launches and the result
+    // copy are ordered on it, and collect waits for the last
+    Engine::Batch* b = e->sign_prepare(fam, params->tree_height, params->d, 1, &it, false, false);
+    e->run(b, false);
     unsigned char* out = sm + ib;
     e->collect_sign(b, &out);
     e->free_batch(b);
@@ -121,8 +123,8 @@ int xmss_mi355x_sign_many(int mt, unsigned char* sk, unsigned char* const* sms, 
         }
         // the index is past all of them before any signature exists
         ull_to_bytes(k, ib, idx0 + shared);
-        Engine::Batch* b = e->sign_prepare(fam, p.tree_height, p.d, (int)shared, items.data(), true);
-        e->run(b);
+        Engine::Batch* b = e->sign_prepare(fam, p.tree_height, p.d, (int)shared, items.data(), true, false);
+        e->run(b, false);
         e->collect_sign(b, outs.data());
         e->free_batch(b);
     }

@@ -174,7 +174,11 @@ public:
     struct Batch;
     // one_key: all items sign with the same key (its seeds from items[0]);
     // items whose signatures use the same (layer, tree) share one subtree build.
-    Batch* sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items, bool one_key = false);
+    // wait = false returns before the staging copy has completed: the next
+    // prepare on this engine must then come after a run + collect (whose
+    // copies are ordered after it on the same stream).
+    Batch* sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items, bool one_key = false,
+                        bool wait = true);
     Batch* verify_prepare(int fam, unsigned th, unsigned d, int count, const VerifyItem* items);
     // New inputs for a verification batch of the same shape (count items of
     // its parameter set), staged into its buffers; its captured launch graph

@@ -64,6 +64,9 @@ constexpr int kSplitRuns = 64;
 // lane-dense (per chain, per L-tree pair, per auth walk) instead of with one
 // workgroup per request.
 constexpr int kLatDense = 256;
+// Up to this many leaves per signing batch, its subtrees' leaves come from
+// the latency kernels (the throughput kernels would leave most SIMDs idle).
+constexpr uint64_t kLatLeaves = 16384;
 static bool use_lat(FamOps* o, size_t count, uint32_t th) {
     return count > 0 && count <= 65535 && count * o->lat_bytes(th) <= kLatScratch;
 }
@@ -92,7 +95,10 @@ struct PinnedBuf {
     uint8_t* need(size_t bytes) {
         bytes = std::max<size_t>(bytes, 256);
         if (bytes > cap) {
-            if (p) HIPCK(hipHostFree(p));
+            if (p) {
+                HIPCK(hipDeviceSynchronize());  // no copy may still read or write the old buffer
+                HIPCK(hipHostFree(p));
+            }
             cap = (std::max(bytes, cap + cap / 2) + 4095) & ~(size_t)4095;
             HIPCK(hipHostMalloc((void**)&p, cap, hipHostMallocDefault));
         }
@@ -560,7 +566,8 @@ struct Engine::Batch {
     size_t res_bytes = 0;
     int* ok = nullptr;        // verify
     std::vector<int> perm;    // verify: staged position -> the caller's index (longest message first)
-    DevBuf lat;               // verify: pre-image scratch of the latency kernels
+    DevBuf lat;               // pre-image scratch of the latency kernels
+    DevBuf lreq;              // sign: request descriptors of the latency kernels
     hipGraphExec_t graph = nullptr;  // verify: the captured launches of run()
     int nrun = 0;                    // verify: runs since staging
     void drop_graph() {
@@ -569,6 +576,23 @@ struct Engine::Batch {
         nrun = 0;
     }
 };
+
+// Descriptors of the latency kernels from those of the throughput path:
+// every leaf of nruns subtrees of height th (run-major); WOTS signatures.
+__global__ __launch_bounds__(WG) void k_runs_to_leaves(const DRun* __restrict__ runs, uint32_t nruns, uint32_t th,
+                                                       DLeaf* __restrict__ out) {
+    const uint64_t g = (uint64_t)blockIdx.x * WG + threadIdx.x;
+    if (g >= ((uint64_t)nruns << th)) return;
+    const DRun q = runs[g >> th];
+    out[g] = {q.key, q.layer, q.tree_hi, q.tree_lo, q.leaf0 + (uint32_t)(g & ((1ull << th) - 1)), 0, 0, 0};
+}
+__global__ __launch_bounds__(WG) void k_wots_to_leaves(const DWots* __restrict__ w, uint32_t n,
+                                                       DLeaf* __restrict__ out) {
+    const uint32_t g = blockIdx.x * WG + threadIdx.x;
+    if (g >= n) return;
+    const DWots q = w[g];
+    out[g] = {q.key, q.layer, q.tree_hi, q.tree_lo, q.ots, 0, 0, 0};
+}
 
 __global__ __launch_bounds__(WG) void k_compare(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b,
                                                 uint32_t n, uint32_t nbytes, int* __restrict__ ok) {
@@ -642,7 +666,7 @@ static Engine::Batch* take_batch(Engine::Impl* I, bool sign) {
 }
 
 Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items,
-                                    bool one_key) {
+                                    bool one_key, bool wait) {
     Impl* I = impl_;
     Batch* b = take_batch(I, true);
     b->sign = true;
@@ -728,7 +752,7 @@ Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count,
     b->dig.need((size_t)count * n);
     b->nodes.need(b->nodes_stride * n * (size_t)b->nruns);
     b->roots.need((size_t)count * d * n);  // layer-major roots
-    I->sync();
+    if (wait) I->sync();
     return b;
 }
 
@@ -969,29 +993,44 @@ void Engine::run(Batch* b, bool wait_done) {
     const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
     const int count = b->count;
     const void* dk = b->keys.p;
-    o->hmsg(I->L, dk, b->md, (uint32_t)count, fam_pad(b->fam), b->blob, b->R, (uint8_t*)b->dig.p);
     {
-        // all subtrees in one pipeline (few: levels split as in build_trees)
+        // all subtrees in one pipeline (few: levels split as in build_trees);
+        // R / H_msg (needed only by the WOTS signatures) and the levels'
+        // address-only work on the aux stream beside the leaves
         const int nruns = b->nruns;
         const bool split = nruns <= kSplitRuns;
         void* tpre = split ? I->tpre.need((size_t)nruns * ((1ull << th) - 1) * o->hpre_bytes()) : nullptr;
-        hipEvent_t tstaged = nullptr, tdone = nullptr;
-        if (split) {
-            tstaged = I->L.ev();
-            tdone = I->L.ev();
-            HIPCK(hipEventRecord(tstaged, I->L.s_main));
-            HIPCK(hipStreamWaitEvent(I->L.s_aux, tstaged, 0));
-            I->L.s = I->L.s_aux;
-            o->pre_tree(I->L, dk, b->runs, nruns, th, tpre);
-            HIPCK(hipEventRecord(tdone, I->L.s_aux));
-            I->L.s = I->L.s_main;
-        }
+        hipEvent_t tstaged = I->L.ev(), tdone = I->L.ev();
+        HIPCK(hipEventRecord(tstaged, I->L.s_main));
+        HIPCK(hipStreamWaitEvent(I->L.s_aux, tstaged, 0));
+        I->L.s = I->L.s_aux;
+        o->hmsg(I->L, dk, b->md, (uint32_t)count, fam_pad(b->fam), b->blob, b->R, (uint8_t*)b->dig.p);
+        if (split) o->pre_tree(I->L, dk, b->runs, nruns, th, tpre);
+        HIPCK(hipEventRecord(tdone, I->L.s_aux));
+        I->L.s = I->L.s_main;
         const uint64_t leaves = 1ull << th;
         const uint64_t per_leaf = (uint64_t)len * n;
         const uint64_t stride = b->nodes_stride;
         uint8_t* nodes = (uint8_t*)b->nodes.p;
         const DRun* drun = b->runs;
-        for (int r0 = 0; r0 < nruns;) {
+        // few leaves (a single signature's trees): the latency kernels -- one
+        // wide launch of address-only work, then the chains at one
+        // compression per step and the L-trees through LDS -- instead of
+        // k_chains at about one wave per SIMD and seven L-tree launches
+        const uint64_t nlv = (uint64_t)nruns * leaves;
+        const bool lat_leaves = split && nlv <= kLatLeaves && use_lat(o, nlv, 0);
+        if (lat_leaves) {
+            DLeaf* dl = b->lreq.need<DLeaf>(nlv * sizeof(DLeaf));
+            I->L.run("runs_to_leaves", k_runs_to_leaves, dim3(cdiv(nlv, WG)), dim3(WG), drun, (uint32_t)nruns, th, dl);
+            uint8_t* lv = b->tmp.need(nlv * n);
+            o->lat_leaves_dense(I->L, dk, dl, (uint32_t)nlv, b->lat.need(nlv * o->lat_bytes(0)),
+                                b->ends.need(nlv * len * n), lv);
+            HIPCK(hipMemcpy2DAsync(nodes, stride * n, lv, leaves * n, leaves * n, nruns, hipMemcpyDeviceToDevice,
+                                   I->L.s));
+            HIPCK(hipStreamWaitEvent(I->L.s_main, tdone, 0));
+            o->levels_fin(I->L, nruns, th, nodes, tpre);
+        }
+        for (int r0 = lat_leaves ? nruns : 0; r0 < nruns;) {
             int rg = std::min<int>(nruns - r0, 65535);
             uint64_t lpp = leaves;
             while ((uint64_t)rg * lpp * per_leaf > kEndsBudget && lpp > 64) lpp >>= 1;
@@ -1022,10 +1061,9 @@ void Engine::run(Batch* b, bool wait_done) {
             }
             r0 += rg;
         }
-        if (split) {
-            I->L.pool.push_back(tstaged);
-            I->L.pool.push_back(tdone);
-        }
+        HIPCK(hipStreamWaitEvent(I->L.s_main, tdone, 0));  // the digests
+        I->L.pool.push_back(tstaged);
+        I->L.pool.push_back(tdone);
         // auth paths and roots: auth[t] = th auth nodes || root, t = layer * count + item
         const uint32_t ng = (uint32_t)((size_t)count * d * (th + 1));
         I->L.run("gather", k_gather, dim3(cdiv((uint64_t)ng * (n / 4), WG)), dim3(WG), (const uint8_t*)nodes,
@@ -1034,6 +1072,15 @@ void Engine::run(Batch* b, bool wait_done) {
         // (the digest for layer 0).  Roots sit at the end of each auth record.
         uint8_t* msgs = (uint8_t*)b->roots.p;
         const size_t rec = (size_t)(th + 1) * n;
+        // few requests: the latency kernels (address-only work wide, then one
+        // compression per chain step), else one lane per chain (k_wots)
+        const bool lat_wots = count <= kLatDense && use_lat(o, count, 0);
+        DLeaf* dw = nullptr;
+        if (lat_wots) {
+            dw = b->lreq.need<DLeaf>((size_t)d * count * sizeof(DLeaf));
+            I->L.run("wots_to_leaves", k_wots_to_leaves, dim3(cdiv((uint64_t)d * count, WG)), dim3(WG), b->wd,
+                     (uint32_t)(d * count), dw);
+        }
         for (unsigned L = 0; L < d; ++L) {
             const uint8_t* msg_l;
             if (L == 0) {
@@ -1045,8 +1092,12 @@ void Engine::run(Batch* b, bool wait_done) {
                                        rec, n, count, hipMemcpyDeviceToDevice, I->L.s));
                 msg_l = dst;
             }
-            o->wots(I->L, dk, b->wd + (size_t)L * count, (uint32_t)count, msg_l, nullptr,
-                    b->wots + (size_t)L * count * len * n);
+            if (lat_wots)
+                o->lat(I->L, dk, LAT_SIGN, dw + (size_t)L * count, (uint32_t)count, msg_l, nullptr, nullptr, 0,
+                       b->lat.need((size_t)count * o->lat_bytes(0)), b->wots + (size_t)L * count * len * n);
+            else
+                o->wots(I->L, dk, b->wd + (size_t)L * count, (uint32_t)count, msg_l, nullptr,
+                        b->wots + (size_t)L * count * len * n);
         }
     }
     if (wait_done) I->sync();
@@ -1090,7 +1141,7 @@ void Engine::free_batch(Batch* b) {
         return;
     }
     b->drop_graph();
-    DevBuf* bufs[] = {&b->io, &b->res, &b->keys, &b->dig, &b->nodes, &b->ends, &b->tmp, &b->roots, &b->lat};
+    DevBuf* bufs[] = {&b->io, &b->res, &b->keys, &b->dig, &b->nodes, &b->ends, &b->tmp, &b->roots, &b->lat, &b->lreq};
     for (DevBuf* x : bufs) {
         if (x->p) HIPCK(hipFree(x->p));
         x->p = nullptr;

@@ -161,6 +161,25 @@ __global__ __launch_bounds__(WG) void k_lat_pre_h(const typename F::Key* __restr
 //   LAT_LEAF:   out[r] = leaf (n bytes)
 //   LAT_SIGN:   out[r] = the WOTS signature (LEN x n bytes) of msgs[r]
 //   LAT_VERIFY: out[r] = the root implied by sigs[r], msgs[r] and auth[r]
+// L-tree (xmss_commons.c:16-49) of the LEN nodes in pk (LDS), height by
+// height from H pre-images hp; thread c hashes pair c (at least LEN / 2
+// threads); the root ends in pk[0].
+template <class F>
+__device__ __forceinline__ void ltree_lds(typename F::Node* pk, const typename F::HPre* hp, uint32_t c) {
+    uint32_t l = F::LEN, off = 0;
+    while (l > 1) {
+        const uint32_t parents = l >> 1;
+        typename F::Node o;
+        if (c < parents) F::fin_h(o, hp[off + c], pk[2 * c], pk[2 * c + 1]);
+        __syncthreads();
+        if (c < parents) pk[c] = o;
+        if ((l & 1) && c == 0) pk[l >> 1] = pk[l - 1];
+        __syncthreads();
+        off += parents;
+        l = (l + 1) >> 1;
+    }
+}
+
 // Request r on the calling workgroup; pk: LEN nodes of LDS.
 template <class F, uint32_t MODE>
 __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, const DLeaf* __restrict__ reqs,
@@ -199,18 +218,7 @@ __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, co
     }
     if (MODE == LAT_SIGN) return;
     __syncthreads();
-    uint32_t l = F::LEN, off = 0;
-    while (l > 1) {
-        const uint32_t parents = l >> 1;
-        typename F::Node o;
-        if (c < parents) F::fin_h(o, hp[off + c], pk[2 * c], pk[2 * c + 1]);
-        __syncthreads();
-        if (c < parents) pk[c] = o;
-        if ((l & 1) && c == 0) pk[l >> 1] = pk[l - 1];
-        __syncthreads();
-        off += parents;
-        l = (l + 1) >> 1;
-    }
+    ltree_lds<F>(pk, hp, c);
     if (c != 0) return;
     typename F::Node node = pk[0];
     if (MODE == LAT_VERIFY) {
@@ -265,6 +273,14 @@ __global__ __launch_bounds__(192) void k_lat_verify_layers(const DLeaf* __restri
         __syncthreads();
     }
 }
+
+// Request descriptors of the latency kernels from those of the throughput
+// path: every leaf of nruns subtrees of height th (run-major), and WOTS
+// signature requests (defined in engine.hip).
+__global__ __launch_bounds__(WG) void k_runs_to_leaves(const DRun* __restrict__ runs, uint32_t nruns, uint32_t th,
+                                                       DLeaf* __restrict__ out);
+__global__ __launch_bounds__(WG) void k_wots_to_leaves(const DWots* __restrict__ w, uint32_t n,
+                                                       DLeaf* __restrict__ out);
 
 // ---- subtree levels of a few trees -------------------------------------
 // Internal node (level, i) of a 2^th-leaf subtree -- the parent of children
@@ -380,6 +396,49 @@ __global__ __launch_bounds__(256) void k_pre_need(const typename F::Key* __restr
         F::pre_f(p, k, a);
         fp[lo * 15 + s] = p;
     }
+}
+
+// ---- LAT_LEAF finish for hundreds of leaves --------------------------
+// k_lat_fin gives each leaf a workgroup of ceil(LEN/64) waves, the last one
+// nearly empty; with many leaves the finish runs instead as one lane per
+// chain (all 15 steps from the pre-images, full waves) and then one workgroup
+// of ceil(LEN/2 / 64) waves per leaf for the L-tree through LDS.
+template <class F>
+__global__ __launch_bounds__(WG) void k_lat_chains_all(uint32_t n, const uint8_t* __restrict__ scratch, size_t stride,
+                                                        uint8_t* __restrict__ ends) {
+    using FPre = typename F::FPre;
+    const uint32_t g = blockIdx.x * WG + threadIdx.x;
+    if (g >= n * (uint32_t)F::LEN) return;
+    const uint32_t r = g / F::LEN, c = g % F::LEN;
+    const uint8_t* S = scratch + (size_t)r * stride;
+    typename F::Node x = reinterpret_cast<const typename F::Node*>(S + LatLayout<F>::starts)[c];
+    const FPre* fp = reinterpret_cast<const FPre*>(S + LatLayout<F>::fpre) + c * 15;
+    FPre cur = fp[0];
+    for (uint32_t s = 0; s < 15; ++s) {
+        const FPre nxt = fp[s + 1 < 15 ? s + 1 : 14];
+        F::fin_f(x, cur, x);
+        cur = nxt;
+    }
+    F::store(x, ends + (size_t)g * F::kN);
+}
+
+template <class F>
+struct LtreeThreads {
+    static constexpr uint32_t value = ((F::LEN / 2 + 63) / 64) * 64;
+};
+
+template <class F>
+__global__ __launch_bounds__(LtreeThreads<F>::value) void k_lat_ltree(const uint8_t* __restrict__ ends,
+                                                                      const uint8_t* __restrict__ scratch,
+                                                                      size_t stride, uint8_t* __restrict__ out) {
+    __shared__ typename F::Node pk[F::LEN];
+    const uint32_t r = blockIdx.x;
+    for (uint32_t c = threadIdx.x; c < (uint32_t)F::LEN; c += blockDim.x)
+        F::load(pk[c], ends + ((size_t)r * F::LEN + c) * F::kN);
+    __syncthreads();
+    ltree_lds<F>(pk, reinterpret_cast<const typename F::HPre*>(scratch + (size_t)r * stride + LatLayout<F>::hpre),
+                 threadIdx.x);
+    if (threadIdx.x == 0) F::store(pk[0], out + (size_t)r * F::kN);
 }
 
 // ---- the same finish for large verification batches, lane-dense --------

@@ -127,6 +127,10 @@ struct FamOps {
     virtual void verify_layers(Launcher& L, const DLeaf* reqs, uint32_t n, const uint32_t* base, uint32_t dmax,
                                const uint8_t* digests, const uint8_t* sigs, const uint8_t* auth, uint32_t th,
                                const uint8_t* scratch, uint8_t* roots) = 0;
+    // LAT_LEAF for many requests: lat_pre, then one lane per chain and one
+    // workgroup per L-tree (ends: n x LEN nodes of scratch) -> out[r] = leaf.
+    virtual void lat_leaves_dense(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, uint8_t* scratch,
+                                  uint8_t* ends, uint8_t* out) = 0;
     virtual void lat_fin(Launcher& L, uint32_t mode, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
                          const uint8_t* sigs, const uint8_t* auth, uint32_t th, const uint8_t* scratch,
                          uint8_t* out) = 0;
@@ -244,6 +248,15 @@ struct FamOpsT final : FamOps {
         for (uint32_t i = 0; i <= dmax && i < 17; ++i) lb.b[i] = base[i];
         L.run("verify_layers", k_lat_verify_layers<F>, dim3(n), dim3(((F::LEN + 63) / 64) * 64), reqs, lb, dmax,
               digests, sigs, auth, scratch, LatLayout<F>::bytes(th), roots);
+    }
+    void lat_leaves_dense(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, uint8_t* scratch,
+                          uint8_t* ends, uint8_t* out) override {
+        const size_t stride = LatLayout<F>::bytes(0);
+        lat_pre_t<LAT_LEAF>(L, keys, reqs, n, nullptr, 0, scratch);
+        L.run("lat_chains_all", k_lat_chains_all<F>, dim3(cdiv((uint64_t)n * F::LEN, WG)), dim3(WG), n,
+              (const uint8_t*)scratch, stride, ends);
+        L.run("lat_ltree", k_lat_ltree<F>, dim3(n), dim3(LtreeThreads<F>::value), (const uint8_t*)ends,
+              (const uint8_t*)scratch, stride, out);
     }
     void lat_pre_h(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, uint32_t th,
                    uint8_t* scratch) override {

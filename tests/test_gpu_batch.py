@@ -162,18 +162,38 @@ def test_mixed_parameter_sets_in_one_job():
     mls = (C.c_ulonglong * kv)()
     vr = (C.c_int * kv)()
     pkb = [buf(x) for x in vpk]  # must outlive the job
+    L.xmss_mi355x_verify_refill.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     job = L.xmss_mi355x_mixed_verify_prepare((C.c_int * kv)(*vmt), _arr(smb), smls, _arr(pkb), kv)
-    assert L.xmss_mi355x_job_run(job) == 0
-    assert L.xmss_mi355x_verify_collect(job, _arr(outs), mls, vr) == 0
+
+    def check(flipped):
+        for t, i in enumerate(vi):
+            mt, p = cases[i][0], cases[i][1]
+            idx = int.from_bytes(bytes(cases[i][3][4:4 + p.index_bytes]), "big")
+            ok = idx < (1 << p.full_height) - 1 and t not in flipped
+            assert vr[t] == (0 if ok else -1), "verify item %d" % i
+            if ok:
+                assert raw(outs[t], mls[t]) == cases[i][4]
+        assert vr[kv - 1] == -1
+
+    # three runs (the second and third from each set's captured graph) ...
+    for _ in range(3):
+        assert L.xmss_mi355x_job_run(job) == 0
+        assert L.xmss_mi355x_verify_collect(job, _arr(outs), mls, vr) == 0
+        check(set())
+    # ... then a refill of the same shape with two signatures corrupted, twice
+    flipped = {0, kv - 2}
+    vsm2 = list(vsm)
+    for t in flipped:
+        b2 = bytearray(vsm2[t])
+        b2[cases[vi[t]][1].index_bytes + 5] ^= 4
+        vsm2[t] = bytes(b2)
+    smb2 = [buf(s) for s in vsm2]
+    assert L.xmss_mi355x_verify_refill(job, _arr(smb2), smls, _arr(pkb)) == 0
+    for _ in range(2):
+        assert L.xmss_mi355x_job_run(job) == 0
+        assert L.xmss_mi355x_verify_collect(job, _arr(outs), mls, vr) == 0
+        check(flipped)
     L.xmss_mi355x_job_free(job)
-    for t, i in enumerate(vi):
-        mt, p = cases[i][0], cases[i][1]
-        idx = int.from_bytes(bytes(cases[i][3][4:4 + p.index_bytes]), "big")
-        ok = idx < (1 << p.full_height) - 1
-        assert vr[t] == (0 if ok else -1), "verify item %d" % i
-        if ok:
-            assert raw(outs[t], mls[t]) == cases[i][4]
-    assert vr[kv - 1] == -1
 
 
 @pytest.mark.parametrize("name,mt", [("XMSS-SHA2_10_256", 0), ("XMSSMT-SHA2_20/4_256", 1),
