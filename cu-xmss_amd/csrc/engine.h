@@ -10,7 +10,9 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace xm {
@@ -98,6 +100,28 @@ struct MsgReq {
     const uint8_t* m;
     uint64_t mlen;
 };
+
+// Runs f(i) for every i in [0, n) on up to eight host threads (this one
+// included) when there are at least `grain` items per thread; the items must
+// touch disjoint memory.  Used for the per-request staging copies of large
+// batches.
+template <class Fn>
+void parallel_for(size_t n, size_t grain, Fn&& f) {
+    const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+    const size_t t = std::min<size_t>(std::min<size_t>(8, hw), n / std::max<size_t>(grain, 1));
+    if (t <= 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    const size_t chunk = (n + t - 1) / t;
+    std::vector<std::thread> th;
+    for (size_t k = 1; k < t; ++k)
+        th.emplace_back([&f, k, chunk, n] {
+            for (size_t i = k * chunk; i < std::min(n, (k + 1) * chunk); ++i) f(i);
+        });
+    for (size_t i = 0; i < std::min(n, chunk); ++i) f(i);
+    for (std::thread& x : th) x.join();
+}
 
 class Engine {
 public:

@@ -634,19 +634,27 @@ static MsgStage plan_msgs(int fam, int count, bool sign, const std::vector<MsgRe
 static void fill_msgs(uint8_t* h, int fam, bool sign, const std::vector<MsgReq>& reqs, const MsgStage& m) {
     const unsigned n = fam_n(fam), pad = fam_pad(fam);
     const size_t pb = pad + 3 * n;
-    memset(h, 0, m.total);
-    for (size_t i = 0; i < m.d.size(); ++i) {
+    const size_t cnt = m.d.size();
+    // each message's region (up to the next one, or to the end with the
+    // slack) is written whole: prefix, message, zeros
+    parallel_for(cnt, 512, [&](size_t i) {
         uint8_t* p = h + m.d[i].off;
+        const size_t end = i + 1 < cnt ? m.d[i + 1].off : m.total;
+        memset(p, 0, pad);
         p[pad - 1] = 2;
-        if (!sign) memcpy(p + pad, reqs[i].R, n);
+        if (sign) memset(p + pad, 0, n);
+        else memcpy(p + pad, reqs[i].R, n);
         memcpy(p + pad + n, reqs[i].root, n);
         uint64_t v = reqs[i].idx;
+        memset(p + pad + 2 * n, 0, n);
         for (int k = 0; k < 8; ++k) {
             p[pad + 3 * n - 1 - k] = (uint8_t)(v & 0xff);
             v >>= 8;
         }
         if (reqs[i].mlen) memcpy(p + pb, reqs[i].m, reqs[i].mlen);
-    }
+        memset(p + pb + reqs[i].mlen, 0, end - m.d[i].off - pb - reqs[i].mlen);
+    });
+    if (!cnt) memset(h, 0, m.total);
 }
 static void fill_seeds(uint8_t* h, unsigned n, const std::vector<KeySeeds>& ks) {
     memset(h, 0, ks.size() * 3 * n);
@@ -787,7 +795,7 @@ static void stage_verify(Engine::Impl* I, FamOps* o, Engine::Batch* b, const Eng
     memcpy(h + o_md, ms.d.data(), sizeof(DMsg) * count);
     fill_msgs(h + o_blob, b->fam, false, mr, ms);
     DLeaf* vd = reinterpret_cast<DLeaf*>(h + o_vd);
-    for (int i = 0; i < count; ++i) {
+    parallel_for(count, 512, [&](size_t i) {
         const Engine::VerifyItem& it = items[ord[i]];
         uint64_t idx = it.idx;
         const uint8_t* p = it.wots;
@@ -801,8 +809,8 @@ static void stage_verify(Engine::Impl* I, FamOps* o, Engine::Batch* b, const Eng
             memcpy(h + o_auth + t * th * n, p, (size_t)th * n);
             p += (size_t)th * n;
         }
-        memcpy(h + o_pk + (size_t)i * n, it.pk, n);
-    }
+        memcpy(h + o_pk + i * n, it.pk, n);
+    });
     uint8_t* const before = (uint8_t*)b->io.p;
     uint8_t* dv = b->io.need(in.size);
     if (before && dv != before) b->drop_graph();  // the buffer moved: the captured pointers are stale
@@ -1113,9 +1121,9 @@ void Engine::collect_sign(Batch* b, uint8_t* const* out) {
     const uint8_t* R = h + (b->R - (uint8_t*)b->res.p);
     const uint8_t* auth = h + (b->auth - (uint8_t*)b->res.p);
     const uint8_t* w = h + (b->wots - (uint8_t*)b->res.p);
-    for (int i = 0; i < count; ++i) {
+    parallel_for(count, 512, [&](size_t i) {
         uint8_t* p = out[i];
-        memcpy(p, R + (size_t)i * n, n);
+        memcpy(p, R + i * n, n);
         p += n;
         for (unsigned L = 0; L < d; ++L) {
             const size_t t = (size_t)L * count + i;
@@ -1124,7 +1132,7 @@ void Engine::collect_sign(Batch* b, uint8_t* const* out) {
             memcpy(p, auth + t * (th + 1) * n, (size_t)th * n);
             p += (size_t)th * n;
         }
-    }
+    });
 }
 
 void Engine::collect_verify(Batch* b, int* ok) {

@@ -305,9 +305,9 @@ int xmss_mi355x_sign_collect(xmss_mi355x_job* j, unsigned char* const* sms, unsi
             s.e->collect_sign(s.b, outs.data());
         }
     }
-    for (size_t i = 0; i < j->count; ++i) {
+    parallel_for(j->count, 512, [&](size_t i) {
         rets[i] = j->rets[i];
-        if (j->set[i] < 0) continue;
+        if (j->set[i] < 0) return;
         const xmss_params& p = j->sets[j->set[i]].p;
         if (j->rets[i] == 0 || j->rets[i] == -2) {
             // message after the signature, smlen set even for -2 (xmss_core.c:205-206)
@@ -315,7 +315,7 @@ int xmss_mi355x_sign_collect(xmss_mi355x_job* j, unsigned char* const* sms, unsi
             smlens[i] = p.sig_bytes + j->mlens[i];
         }
         if (j->rets[i] == 0) memcpy(sms[i], &j->idxbytes[i * 8], p.index_bytes);
-    }
+    });
     return 0;
 }
 
@@ -332,7 +332,7 @@ int xmss_mi355x_verify_collect(xmss_mi355x_job* j, unsigned char* const* ms, uns
             std::lock_guard<std::mutex> lk(engine_mutex());
             s.e->collect_verify(s.b, ok.data());
         }
-        for (size_t k = 0; k < s.live.size(); ++k) {
+        parallel_for(s.live.size(), 512, [&](size_t k) {
             const size_t i = s.live[k];
             const unsigned char* sm = j->sms[i];
             unsigned char* m = ms[i];
@@ -352,7 +352,7 @@ int xmss_mi355x_verify_collect(xmss_mi355x_job* j, unsigned char* const* ms, uns
                 mlens[i] = 0;
                 rets[i] = -1;
             }
-        }
+        });
     }
     return 0;
 }
