@@ -317,22 +317,6 @@ struct BdsDev {
     }
 };
 
-// Address-only part of a list of thash_h calls (F::pre_h), one per lane.
-template <class F>
-__global__ __launch_bounds__(WG) void k_pre_h(const typename F::Key* __restrict__ keys,
-                                               const DThash* __restrict__ reqs, uint32_t n,
-                                               typename F::HPre* __restrict__ out) {
-    const uint32_t r = blockIdx.x * WG + threadIdx.x;
-    if (r >= n) return;
-    const DThash q = reqs[r];
-    uint32_t a[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) a[i] = q.adrs[i];
-    typename F::HPre p;
-    F::pre_h(p, keys[q.key], a);
-    out[r] = p;
-}
-
 // The hashing of BDS signatures before their state update, in two launches
 // on one stream (so that a signature costs three launches and no
 // cross-stream waits: the host's launch calls, as much as the GPU, bound a

@@ -156,7 +156,6 @@ struct FamOps {
     virtual void hmsg(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad,
                       uint8_t* blob, uint8_t* R, uint8_t* dig) = 0;
     virtual size_t hpre_bytes() const = 0;
-    virtual void pre_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, void* out) = 0;
     // BDS signatures: R and H_msg of the nsign messages, the planned leaves
     // (LAT_LEAF requests) and the layer-0 WOTS signatures (LAT_SIGN requests
     // with LAT_ALL_STEPS) in one pre-image launch, which also does the
@@ -320,9 +319,6 @@ struct FamOpsT final : FamOps {
         L.run("hmsg", k_hmsg<F>, dim3(cdiv(n, WG)), dim3(WG), (const K*)keys, reqs, n, pad, blob, R, dig);
     }
     size_t hpre_bytes() const override { return sizeof(typename F::HPre); }
-    void pre_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, void* out) override {
-        L.run("pre_h", k_pre_h<F>, dim3(cdiv(n, WG)), dim3(WG), (const K*)keys, reqs, n, (typename F::HPre*)out);
-    }
     void bds_lat(Launcher& L, const void* keys, const DMsg* msgs, uint32_t pad, uint8_t* blob, uint8_t* R,
                  uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs, uint32_t nsign,
                  uint8_t* scratch, const DThash* merges, uint32_t nmerge, void* merge_out, uint8_t* leaf_out,
