@@ -54,6 +54,8 @@ def load_product(core):
     L.xmss_mi355x_sign_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.xmss_mi355x_verify_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.xmss_mi355x_verify_refill.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.xmss_mi355x_job_launch.argtypes = [C.c_void_p]
+    L.xmss_mi355x_job_wait.argtypes = [C.c_void_p]
     L.xmss_mi355x_job_free.argtypes = [C.c_void_p]
     L.xmss_batch_keypair.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_char_p, C.c_uint32, C.c_size_t]
     return api
@@ -184,6 +186,11 @@ def bench_batch(api, count):
     L.xmss_mi355x_verify_collect(job, outp2, mls2, vr2)
     ver_refill_t = time.perf_counter() - t0
     assert [vr2[j] for j in range(count)] == [vr[i] for i in perm]
+    # the same stream through two jobs: each batch is staged while the other
+    # job's launched work runs, and collected after the next one is launched
+    stream_t = bench_verify_stream(L, job, count, (ptrs(smb), smls, ptrs(pks), ptrs(outs)),
+                                   (smp2, smls2, pkp2, outp2), [vr[i] for i in range(count)],
+                                   [vr[i] for i in perm])
     L.xmss_mi355x_job_free(job)
     # leaf 1023 is the last index: the reference signs it with the wiped key,
     # so exactly those signatures must be rejected (xmss_core.c:222-231)
@@ -192,7 +199,44 @@ def bench_batch(api, count):
         assert vr[i] == want, (i, idxs[i], vr[i])
         if want == 0:
             assert raw(outs[i], mls[i]) == msgs[i]
-    return keygen_t, sign_run, sign_total, (ver_t, ver_refill_t), st, sum(1 for i in idxs if i == 1023)
+    return keygen_t, sign_run, sign_total, (ver_t, ver_refill_t, stream_t), st, sum(1 for i in idxs if i == 1023)
+
+
+def bench_verify_stream(L, job_a, count, in_a, in_b, want_a, want_b, nbatch=16):
+    """Seconds per batch of a stream of `nbatch` same-shape verification
+    batches (the two input sets alternating) through two jobs:
+    launch(A); loop { refill(B); collect(A); launch(B); swap }.  Host staging
+    and the PCIe copy of one batch overlap the GPU work of the other; every
+    batch's verdicts are checked after the timed region."""
+    job_b = L.xmss_mi355x_verify_prepare(0, in_b[0], in_b[1], in_b[2], count)
+    L.xmss_mi355x_job_run(job_b)  # sizes the buffers, then captures the graph
+    L.xmss_mi355x_job_run(job_b)
+    jobs, holds = [job_a, job_b], [None, None]
+    sets = [in_a, in_b]
+    mls = [(C.c_ulonglong * count)() for _ in range(nbatch)]
+    vrs = [(C.c_int * count)() for _ in range(nbatch)]
+    t0 = time.perf_counter()
+    cur = 0
+    src = sets[0]
+    assert L.xmss_mi355x_verify_refill(jobs[0], src[0], src[1], src[2]) == 0
+    holds[0] = (0, src)
+    L.xmss_mi355x_job_launch(jobs[0])
+    for i in range(1, nbatch):
+        nxt = 1 - cur
+        src = sets[i % 2]
+        assert L.xmss_mi355x_verify_refill(jobs[nxt], src[0], src[1], src[2]) == 0
+        holds[nxt] = (i, src)
+        j, hs = holds[cur]
+        L.xmss_mi355x_verify_collect(jobs[cur], hs[3], mls[j], vrs[j])
+        L.xmss_mi355x_job_launch(jobs[nxt])
+        cur = nxt
+    j, hs = holds[cur]
+    L.xmss_mi355x_verify_collect(jobs[cur], hs[3], mls[j], vrs[j])
+    t = time.perf_counter() - t0
+    L.xmss_mi355x_job_free(job_b)
+    for i in range(nbatch):
+        assert [vrs[i][j] for j in range(count)] == (want_a if i % 2 == 0 else want_b), "stream batch %d" % i
+    return t / nbatch
 
 
 def bench_plain_single(api, count):
@@ -642,11 +686,11 @@ def main():
     total_comps = 298582010
     keygen_frac = total_comps * OPS_PER_SHA256 / (ms_per_step / 1000) / 1e12 / VALU_PEAK_TOPS
 
-    bkeygen_t = sign_run = sign_total = ver_t = ver_refill_t = None
+    bkeygen_t = sign_run = sign_total = ver_t = ver_refill_t = ver_stream_t = None
     bst, n_last = {}, None
     if args.batch:
         log("batch keygen/sign/verify x%d ..." % args.batch)
-        bkeygen_t, sign_run, sign_total, (ver_t, ver_refill_t), bst, n_last = bench_batch(api, args.batch)
+        bkeygen_t, sign_run, sign_total, (ver_t, ver_refill_t, ver_stream_t), bst, n_last = bench_batch(api, args.batch)
     rate = lambda t: None if not t else round(args.batch / t, 2)
     single = None
     if args.single:
@@ -690,6 +734,7 @@ def main():
         "signs_per_s_batch4096_incl_pcie": rate(sign_total),
         "verifies_per_s_batch4096": rate(ver_t),
         "verifies_per_s_batch4096_refill_incl_pcie": rate(ver_refill_t),
+        "verifies_per_s_batch4096_stream_incl_pcie": rate(ver_stream_t),
         "p50_sign_ms": None if p50 is None else round(1000 * p50, 3),
         "p99_sign_ms": None if p99 is None else round(1000 * p99, 3),
         "p50_verify_ms": None if v50 is None else round(1000 * v50, 3),

@@ -10,7 +10,13 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <sys/types.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -101,6 +107,66 @@ struct MsgReq {
     uint64_t mlen;
 };
 
+// Host worker threads for parallel_for, started on first use and kept for
+// the process (a forked child starts its own).  One parallel_for at a time
+// uses them; a concurrent caller starts threads of its own.
+class HostPool {
+public:
+    static HostPool* get(size_t workers) {
+        static std::mutex mu;
+        static HostPool* pool = nullptr;
+        std::lock_guard<std::mutex> lk(mu);
+        if (!pool || pool->pid_ != getpid()) pool = new HostPool(workers);  // a fork's pool is not reused
+        return pool;
+    }
+    size_t workers() const { return nw_; }
+    bool try_acquire() { return busy_.try_lock(); }
+    void release() { busy_.unlock(); }
+    // Runs f(k) for k in [0, t), t <= workers() + 1: f(0) on the caller, the
+    // rest on the workers; returns when all are done.
+    void run(size_t t, const std::function<void(size_t)>& f) {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            task_ = &f;
+            ntask_ = t;
+            pending_ = t - 1;
+            ++gen_;
+        }
+        go_.notify_all();
+        f(0);
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+
+private:
+    explicit HostPool(size_t w) : nw_(w), pid_(getpid()) {
+        for (size_t k = 1; k <= w; ++k) std::thread([this, k] { loop(k); }).detach();
+    }
+    void loop(size_t k) {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(size_t)>* t;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                go_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (k >= ntask_) continue;
+                t = task_;
+            }
+            (*t)(k);
+            std::lock_guard<std::mutex> lk(m_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    const size_t nw_;
+    const pid_t pid_;
+    std::mutex busy_, m_;
+    std::condition_variable go_, done_;
+    const std::function<void(size_t)>* task_ = nullptr;
+    size_t ntask_ = 0, pending_ = 0;
+    uint64_t gen_ = 0;
+};
+
 // Runs f(i) for every i in [0, n) on up to eight host threads (this one
 // included) when there are at least `grain` items per thread; the items must
 // touch disjoint memory.  Used for the per-request staging copies of large
@@ -114,12 +180,18 @@ void parallel_for(size_t n, size_t grain, Fn&& f) {
         return;
     }
     const size_t chunk = (n + t - 1) / t;
+    const std::function<void(size_t)> part = [&f, chunk, n](size_t k) {
+        for (size_t i = k * chunk; i < std::min(n, (k + 1) * chunk); ++i) f(i);
+    };
+    HostPool* pool = HostPool::get(std::min<size_t>(8, hw) - 1);
+    if (t <= pool->workers() + 1 && pool->try_acquire()) {
+        pool->run(t, part);
+        pool->release();
+        return;
+    }
     std::vector<std::thread> th;
-    for (size_t k = 1; k < t; ++k)
-        th.emplace_back([&f, k, chunk, n] {
-            for (size_t i = k * chunk; i < std::min(n, (k + 1) * chunk); ++i) f(i);
-        });
-    for (size_t i = 0; i < std::min(n, chunk); ++i) f(i);
+    for (size_t k = 1; k < t; ++k) th.emplace_back(part, k);
+    part(0);
     for (std::thread& x : th) x.join();
 }
 
@@ -214,6 +286,13 @@ public:
     // on (not while per-kernel timing is on).
     void run(Batch* b, bool wait_done = true);
     void wait();
+    // launch() queues a batch's work and returns; wait_batch() blocks until
+    // that work is done (no lock is needed for it).  A launched batch's
+    // refill, collect and free wait for it by themselves; its staging and
+    // result copies run on a copy stream, so a verification batch can be
+    // staged and another collected while it runs.
+    void launch(Batch* b);
+    void wait_batch(Batch* b);
     // sign: per item R (n) || d x (WOTS sig || auth path) written to out[i]
     //       (sig_bytes - index_bytes bytes each).
     // verify: ok[i] = 1 when the recomputed root equals the pk root.

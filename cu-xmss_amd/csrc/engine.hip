@@ -124,7 +124,9 @@ struct Engine::Impl {
     const void* kp() const { return kp_override ? kp_override : keys.p; }
     Engine::Batch* spare[2] = {nullptr, nullptr};  // reusable verify / sign batches
     DevBuf keys, seeds, desc, ends, tmp, nodes, a, b, c, d, idx, lat, tpre;
-    PinnedBuf pin;  // staging of batch inputs and results
+    // copies of launched batches: a batch's inputs go in and its results come
+    // out here while other batches' kernels run on s_main / s_aux
+    hipStream_t s_copy = nullptr;
     std::unordered_map<std::string, int> slots[F_COUNT];  // pool slot by seed bytes (pool_slot)
     int key_fam = -1;
     int nkeys = 0;
@@ -148,6 +150,7 @@ Engine::Engine() : impl_(new Impl) {
     HIPCK(hipGetDevice(&dev_));
     HIPCK(hipStreamCreateWithFlags(&impl_->L.s_main, hipStreamNonBlocking));
     HIPCK(hipStreamCreateWithFlags(&impl_->L.s_aux, hipStreamNonBlocking));
+    HIPCK(hipStreamCreateWithFlags(&impl_->s_copy, hipStreamNonBlocking));
     impl_->L.s = impl_->L.s_main;
     stream_ = impl_->L.s;
 }
@@ -570,6 +573,12 @@ struct Engine::Batch {
     DevBuf lreq;              // sign: request descriptors of the latency kernels
     hipGraphExec_t graph = nullptr;  // verify: the captured launches of run()
     int nrun = 0;                    // verify: runs since staging
+    PinnedBuf pin;                   // host staging of this batch's inputs and results
+    hipEvent_t done = nullptr;       // recorded after the work of launch()
+    bool inflight = false;           // launched and not yet waited for
+    bool evented = false;            // launched through launch(): results leave on the copy stream
+    hipEvent_t staged = nullptr;     // verify: recorded after the staging copy on the copy stream
+    bool copying = false;            // verify: that copy may still be reading `pin`
     void drop_graph() {
         if (graph) HIPCK(hipGraphExecDestroy(graph));
         graph = nullptr;
@@ -666,6 +675,18 @@ static void fill_seeds(uint8_t* h, unsigned n, const std::vector<KeySeeds>& ks) 
     }
 }
 
+// Blocks until the work of a launched batch is done (no-op otherwise).
+static void wait_batch(Engine::Batch* b) {
+    if (b->inflight) {
+        HIPCK(hipEventSynchronize(b->done));
+        b->inflight = false;
+    }
+    if (b->copying) {  // staged and never run
+        HIPCK(hipEventSynchronize(b->staged));
+        b->copying = false;
+    }
+}
+
 static Engine::Batch* take_batch(Engine::Impl* I, bool sign) {
     Engine::Batch* b = I->spare[sign];
     I->spare[sign] = nullptr;
@@ -733,7 +754,7 @@ Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count,
     const size_t o_seeds = in.add(ks.size() * 3 * n), o_md = in.add(sizeof(DMsg) * count), o_blob = in.add(ms.total),
                  o_runs = in.add(sizeof(DRun) * runs.size()), o_wd = in.add(sizeof(DWots) * wd.size()),
                  o_gi = in.add(sizeof(uint64_t) * gi.size());
-    uint8_t* h = I->pin.need(in.size);
+    uint8_t* h = b->pin.need(in.size);
     fill_seeds(h + o_seeds, n, ks);
     memcpy(h + o_md, ms.d.data(), sizeof(DMsg) * count);
     fill_msgs(h + o_blob, fam, true, mr, ms);
@@ -769,14 +790,22 @@ Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count,
 // the batch's shape (count, d, th, n) and a refill of the same shape keeps
 // the pointers a captured graph uses, unless the io buffer has to grow.
 static void stage_verify(Engine::Impl* I, FamOps* o, Engine::Batch* b, const Engine::VerifyItem* items) {
+    wait_batch(b);  // a launched batch still reads its inputs
     const int count = b->count;
     const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
     // staged longest message first, so that run() can start the rest while
-    // the longest messages are still being hashed
+    // the longest messages are still being hashed: a stable counting sort by
+    // 64-byte blocks (H_msg's cost), messages of 4 MB and more in one class
     std::vector<int>& ord = b->perm;
     ord.resize(count);
-    for (int i = 0; i < count; ++i) ord[i] = i;
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return items[x].mlen > items[y].mlen; });
+    {
+        constexpr uint64_t kClasses = 1u << 16;
+        std::vector<uint32_t> at(kClasses + 1, 0);
+        auto cls = [&](int i) { return (size_t)(kClasses - 1 - std::min<uint64_t>(items[i].mlen / 64, kClasses - 1)); };
+        for (int i = 0; i < count; ++i) ++at[cls(i) + 1];
+        for (size_t c = 0; c < kClasses; ++c) at[c + 1] += at[c];
+        for (int i = 0; i < count; ++i) ord[at[cls(i)]++] = i;
+    }
     std::vector<KeySeeds> ks(count);
     std::vector<MsgReq> mr(count);
     for (int i = 0; i < count; ++i) {
@@ -790,7 +819,7 @@ static void stage_verify(Engine::Impl* I, FamOps* o, Engine::Batch* b, const Eng
                  o_vd = in.add(sizeof(DLeaf) * count * d), o_sig = in.add((size_t)count * d * len * n),
                  o_auth = in.add((size_t)count * d * th * n), o_pk = in.add((size_t)count * n),
                  o_blob = in.add(ms.total);
-    uint8_t* h = I->pin.need(in.size);
+    uint8_t* h = b->pin.need(in.size);
     fill_seeds(h + o_seeds, n, ks);
     memcpy(h + o_md, ms.d.data(), sizeof(DMsg) * count);
     fill_msgs(h + o_blob, b->fam, false, mr, ms);
@@ -814,7 +843,9 @@ static void stage_verify(Engine::Impl* I, FamOps* o, Engine::Batch* b, const Eng
     uint8_t* const before = (uint8_t*)b->io.p;
     uint8_t* dv = b->io.need(in.size);
     if (before && dv != before) b->drop_graph();  // the buffer moved: the captured pointers are stale
-    I->h2d(dv, h, in.size);
+    // The copy and the key contexts go on the copy stream, so that staging
+    // this batch overlaps the kernels of batches launched before it.
+    HIPCK(hipMemcpyAsync(dv, h, in.size, hipMemcpyHostToDevice, I->s_copy));
     b->seeds = dv + o_seeds;
     b->md = reinterpret_cast<const DMsg*>(dv + o_md);
     b->blob = dv + o_blob;
@@ -822,8 +853,18 @@ static void stage_verify(Engine::Impl* I, FamOps* o, Engine::Batch* b, const Eng
     b->sigs = dv + o_sig;
     b->auth_in = dv + o_auth;
     b->pkroot = dv + o_pk;
-    o->prepare(I->L, b->seeds, count, b->keys.need(o->key_bytes() * count));
-    I->sync();
+    void* keys = b->keys.need(o->key_bytes() * count);
+    I->L.s = I->s_copy;
+    o->prepare(I->L, b->seeds, count, keys);
+    I->L.s = I->L.s_main;
+    // Returns without waiting for the copy: the batch's run waits for it on
+    // the device, and the next refill, collect or free of this batch on the
+    // host (they reuse `pin`).  With per-kernel timing on, the copy stream is
+    // drained here so that its timings can be read at the next wait.
+    if (!b->staged) HIPCK(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
+    HIPCK(hipEventRecord(b->staged, I->s_copy));
+    b->copying = true;
+    if (I->L.prof) wait_batch(b);
 }
 
 Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int count, const VerifyItem* items) {
@@ -919,6 +960,17 @@ static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* 
 
 void Engine::wait() { impl_->sync(); }
 
+void Engine::launch(Batch* b) {
+    wait_batch(b);
+    run(b, false);
+    if (!b->done) HIPCK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+    // both paths end on s_main with the aux stream joined into it
+    HIPCK(hipEventRecord(b->done, impl_->L.s_main));
+    b->inflight = b->evented = true;
+}
+
+void Engine::wait_batch(Batch* b) { xm::wait_batch(b); }
+
 // The launches of one verification batch: verify_dense for large batches,
 // else H_msg beside the address-only work of every layer (all 15 steps of
 // each chain, aux stream), then one finishing launch per layer.
@@ -973,6 +1025,10 @@ static void verify_launches(Engine::Impl* I, FamOps* o, Engine::Batch* b) {
 // stay direct.
 static void run_verify(Engine::Impl* I, Engine::Batch* b) {
     FamOps* o = ops_of(b->fam);
+    if (b->copying) {  // the inputs are staged on the copy stream (outside any capture)
+        HIPCK(hipStreamWaitEvent(I->L.s_main, b->staged, 0));
+        b->copying = false;
+    }
     if (b->graph && !I->L.prof) {
         HIPCK(hipGraphLaunch(b->graph, I->L.s_main));
         return;
@@ -1111,13 +1167,26 @@ void Engine::run(Batch* b, bool wait_done) {
     if (wait_done) I->sync();
 }
 
+// Results of a batch run with run() leave on its own stream, ordered after
+// its kernels (one wait); those of a launched batch on the copy stream once
+// it is done, so that batches launched after it keep the compute streams.
+static void fetch_results(Engine::Impl* I, Engine::Batch* b, void* h, const void* src, size_t bytes) {
+    if (b->evented) {
+        wait_batch(b);
+        HIPCK(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, I->s_copy));
+        HIPCK(hipStreamSynchronize(I->s_copy));
+    } else {
+        I->d2h(h, src, bytes);
+        I->sync();
+    }
+}
+
 void Engine::collect_sign(Batch* b, uint8_t* const* out) {
     Impl* I = impl_;
     const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
     const int count = b->count;
-    uint8_t* h = I->pin.need(b->res_bytes);
-    I->d2h(h, b->res.p, b->res_bytes);
-    I->sync();
+    uint8_t* h = b->pin.need(b->res_bytes);
+    fetch_results(I, b, h, b->res.p, b->res_bytes);
     const uint8_t* R = h + (b->R - (uint8_t*)b->res.p);
     const uint8_t* auth = h + (b->auth - (uint8_t*)b->res.p);
     const uint8_t* w = h + (b->wots - (uint8_t*)b->res.p);
@@ -1137,18 +1206,22 @@ void Engine::collect_sign(Batch* b, uint8_t* const* out) {
 
 void Engine::collect_verify(Batch* b, int* ok) {
     Impl* I = impl_;
-    int* h = reinterpret_cast<int*>(I->pin.need(sizeof(int) * (size_t)b->count));
-    I->d2h(h, b->ok, sizeof(int) * (size_t)b->count);
-    I->sync();
+    int* h = reinterpret_cast<int*>(b->pin.need(sizeof(int) * (size_t)b->count));
+    fetch_results(I, b, h, b->ok, sizeof(int) * (size_t)b->count);
     for (int i = 0; i < b->count; ++i) ok[b->perm[i]] = h[i];
 }
 
 void Engine::free_batch(Batch* b) {
+    wait_batch(b);
+    b->evented = false;
     if (!impl_->spare[b->sign]) {  // keep one of each kind: its buffers only grow
         impl_->spare[b->sign] = b;
         return;
     }
     b->drop_graph();
+    if (b->done) HIPCK(hipEventDestroy(b->done));
+    if (b->staged) HIPCK(hipEventDestroy(b->staged));
+    if (b->pin.p) HIPCK(hipHostFree(b->pin.p));
     DevBuf* bufs[] = {&b->io, &b->res, &b->keys, &b->dig, &b->nodes, &b->ends, &b->tmp, &b->roots, &b->lat, &b->lreq};
     for (DevBuf* x : bufs) {
         if (x->p) HIPCK(hipFree(x->p));

@@ -283,15 +283,35 @@ int xmss_mi355x_verify_refill(xmss_mi355x_job* j, const unsigned char* const* sm
     return 0;
 }
 
-int xmss_mi355x_job_run(xmss_mi355x_job* j) {
+int xmss_mi355x_job_launch(xmss_mi355x_job* j) {
+    if (!j) return -1;
     std::lock_guard<std::mutex> lk(engine_mutex());
-    // every set's launches are queued first, on its own engine's streams,
-    // so the parameter sets share the GPU; then all are waited for
+    // every set's launches are queued on its own engine's streams, so the
+    // parameter sets share the GPU
     for (JobSet& s : j->sets)
-        if (s.b) s.e->run(s.b, false);
-    for (JobSet& s : j->sets)
-        if (s.b) s.e->wait();
+        if (s.b) s.e->launch(s.b);
     return 0;
+}
+
+int xmss_mi355x_job_wait(xmss_mi355x_job* j) {
+    if (!j) return -1;
+    bool prof = false;
+    for (JobSet& s : j->sets) {
+        if (!s.b) continue;
+        s.e->wait_batch(s.b);  // without the engine lock: other callers go on meanwhile
+        prof = prof || s.e->profiling();
+    }
+    if (prof) {  // per-kernel timings are read once the engines are idle
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        for (JobSet& s : j->sets)
+            if (s.b) s.e->wait();
+    }
+    return 0;
+}
+
+int xmss_mi355x_job_run(xmss_mi355x_job* j) {
+    const int rc = xmss_mi355x_job_launch(j);
+    return rc ? rc : xmss_mi355x_job_wait(j);
 }
 
 int xmss_mi355x_sign_collect(xmss_mi355x_job* j, unsigned char* const* sms, unsigned long long* smlens,

@@ -77,8 +77,20 @@ xmss_mi355x_job *xmss_mi355x_mixed_verify_prepare(const int *mts, const unsigned
 int xmss_mi355x_verify_refill(xmss_mi355x_job *job, const unsigned char *const *sms,
                               const unsigned long long *smlens, const unsigned char *const *pks);
 /* Runs a job on the GPU (no host traffic).  A verification job's launches
- * are captured into a HIP graph on its second run and replayed afterwards. */
+ * are captured into a HIP graph on its second run and replayed afterwards.
+ * job_run = job_launch + job_wait. */
 int xmss_mi355x_job_run(xmss_mi355x_job *job);
+/* job_launch queues a job's device work and returns at once; job_wait
+ * blocks until it is done.  Refill, collect and free of a launched job wait
+ * for it by themselves.  A verification job's input copy (prepare, refill)
+ * and a launched job's result copy use a copy stream of their own, and
+ * refill returns once the inputs are in pinned host memory, so with two jobs
+ * a stream of batches overlaps host staging and the PCIe copies with the GPU
+ * work:  launch(A); loop { refill(B); collect(A); launch(B); swap(A, B); }
+ * (A signing job's input copy keeps the compute stream.)  Each returns 0,
+ * or -1 for a null job. */
+int xmss_mi355x_job_launch(xmss_mi355x_job *job);
+int xmss_mi355x_job_wait(xmss_mi355x_job *job);
 int xmss_mi355x_sign_collect(xmss_mi355x_job *job, unsigned char *const *sms,
                              unsigned long long *smlens, int *rets);
 /* ms[i] must hold smlens[i] bytes, as for xmss_sign_open. */

@@ -339,3 +339,151 @@ def test_verify_job_replays_and_refills(k):
     assert L.xmss_mi355x_job_run(job) == 0
     check(job, sms2, want2, "after a refused refill")
     L.xmss_mi355x_job_free(job)
+
+
+@pytest.mark.parametrize("k", [300, 40])
+def test_verify_jobs_pipelined(k):
+    """Two verification jobs alternating over a stream of six batches, each
+    batch staged (refill) while the other job's launched work runs, and
+    launched before or after the other job is collected:
+    launch(A); loop { refill(B); launch(B); collect(A); swap } and
+    launch(A); loop { refill(B); collect(A); launch(B); swap } in turn.
+    Every batch's verdicts equal the single sign_open calls.  A refill of a
+    job still in flight, a launch with no wait and a free of a job in flight
+    wait for its work by themselves."""
+    import random
+    from test_gpu_opk import _sign_many
+    lib = product(0)
+    L = lib.lib
+    _verify_job_api(L)
+    L.xmss_mi355x_job_launch.argtypes = [C.c_void_p]
+    L.xmss_mi355x_job_wait.argtypes = [C.c_void_p]
+    oid, mt = 0x01, 0
+    p = lib.params(oid, mt)
+    L.xmss_mi355x_rng_seed(11 + k)
+    rc, pk, sk = lib.keypair(oid, mt)
+    assert rc == 0
+    rng = random.Random(100 + k)
+    msgs = [bytes(rng.getrandbits(8) for _ in range(int(32 * 128 ** rng.random()))) for _ in range(k)]
+    out, _ = _sign_many(lib, sk, msgs, mt, p)
+    good = [sm for rc, sm in out]
+    pkb = [buf(pk) for _ in range(k)]
+
+    def variant(seed):
+        r = random.Random(seed)
+        order = list(range(k))
+        r.shuffle(order)
+        sms = []
+        for i in order:
+            sm = bytearray(good[i])
+            if r.random() < 0.3:
+                sm[r.randrange(p.index_bytes, p.sig_bytes)] ^= 1 << r.randrange(8)
+            sms.append(bytes(sm))
+        want = [lib.sign_open(s, pk, mt)[0] for s in sms]
+        return [buf(s) for s in sms], (C.c_ulonglong * k)(*[len(s) for s in sms]), want
+
+    def collect(job, batch, label):
+        smb, smls, want = batch
+        outs = [buf(smls[j]) for j in range(k)]
+        mls = (C.c_ulonglong * k)()
+        vr = (C.c_int * k)()
+        assert L.xmss_mi355x_verify_collect(job, _arr(outs), mls, vr) == 0
+        assert [vr[j] for j in range(k)] == want, label
+        for j in range(k):
+            if want[j] == 0:
+                assert raw(outs[j], mls[j]) == bytes(smb[j][p.sig_bytes:smls[j]]), label
+        assert 0 in want and -1 in want
+
+    batches = [variant(s) for s in range(1, 7)]
+    jobs = [L.xmss_mi355x_verify_prepare(mt, _arr(b[0]), b[1], _arr(pkb), k) for b in batches[:2]]
+    held = [batches[0], batches[1]]  # the inputs each job holds
+    cur = 0
+    assert L.xmss_mi355x_job_launch(jobs[cur]) == 0
+    for i in range(1, len(batches)):
+        nxt = 1 - cur
+        b = batches[i]
+        assert L.xmss_mi355x_verify_refill(jobs[nxt], _arr(b[0]), b[1], _arr(pkb)) == 0
+        held[nxt] = b
+        if i % 2:
+            assert L.xmss_mi355x_job_launch(jobs[nxt]) == 0
+            collect(jobs[cur], held[cur], "batch %d" % (i - 1))
+        else:
+            collect(jobs[cur], held[cur], "batch %d" % (i - 1))
+            assert L.xmss_mi355x_job_launch(jobs[nxt]) == 0
+        cur = nxt
+    assert L.xmss_mi355x_job_wait(jobs[cur]) == 0
+    collect(jobs[cur], held[cur], "last batch")
+    # refill while in flight: waits, then stages the new inputs
+    assert L.xmss_mi355x_job_launch(jobs[0]) == 0
+    b = batches[2]
+    assert L.xmss_mi355x_verify_refill(jobs[0], _arr(b[0]), b[1], _arr(pkb)) == 0
+    assert L.xmss_mi355x_job_run(jobs[0]) == 0
+    collect(jobs[0], b, "refilled in flight")
+    assert L.xmss_mi355x_job_launch(jobs[1]) == 0
+    L.xmss_mi355x_job_free(jobs[1])  # in flight
+    L.xmss_mi355x_job_free(jobs[0])
+
+
+def test_sign_job_launched_beside_a_verify_job():
+    """A signing job and a verification job launched together and collected
+    in the other order: the signatures equal those of job_run."""
+    import random
+    lib = product(0)
+    L = lib.lib
+    _verify_job_api(L)
+    L.xmss_mi355x_job_launch.argtypes = [C.c_void_p]
+    L.xmss_mi355x_job_wait.argtypes = [C.c_void_p]
+    L.xmss_mi355x_sign_prepare.restype = C.c_void_p
+    L.xmss_mi355x_sign_prepare.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    L.xmss_mi355x_sign_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    oid, mt = 0x01, 0
+    p = lib.params(oid, mt)
+    rng = random.Random(77)
+    k = 6
+    keys = []
+    for i in range(k):
+        L.xmss_mi355x_rng_seed(500 + i)
+        rc, pk, sk = lib.keypair(oid, mt)
+        assert rc == 0
+        sk = sk[:4] + (rng.randrange(1000)).to_bytes(4, "big") + sk[8:]
+        keys.append((pk, sk))
+    msgs = [bytes(rng.getrandbits(8) for _ in range(rng.randrange(1, 3000))) for _ in range(k)]
+
+    def sign_job(launch_only):
+        skb = [buf(sk) for _, sk in keys]
+        msb = [buf(m) for m in msgs]
+        mls = (C.c_ulonglong * k)(*[len(m) for m in msgs])
+        job = L.xmss_mi355x_sign_prepare(mt, _arr(skb), _arr(msb), mls, k)
+        return job, skb, msb, mls
+
+    def sign_collect(job, skb):
+        sms = [buf(p.sig_bytes + len(m)) for m in msgs]
+        smls = (C.c_ulonglong * k)()
+        rets = (C.c_int * k)()
+        assert L.xmss_mi355x_sign_collect(job, _arr(sms), smls, rets) == 0
+        assert all(rets[i] == 0 for i in range(k))
+        return [raw(sms[i], smls[i]) for i in range(k)], [raw(skb[i], 4 + p.sk_bytes) for i in range(k)]
+
+    job, skb, _m1, _l1 = sign_job(False)
+    assert L.xmss_mi355x_job_run(job) == 0
+    want_sm, want_sk = sign_collect(job, skb)
+    L.xmss_mi355x_job_free(job)
+    # a verification job of those signatures, and the same signing job again
+    vsmb = [buf(s) for s in want_sm]
+    vpkb = [buf(pk) for pk, _ in keys]
+    vjob = L.xmss_mi355x_verify_prepare(mt, _arr(vsmb), (C.c_ulonglong * k)(*[len(s) for s in want_sm]),
+                                        _arr(vpkb), k)
+    sjob, skb2, _m2, _l2 = sign_job(True)
+    assert L.xmss_mi355x_job_launch(sjob) == 0
+    assert L.xmss_mi355x_job_launch(vjob) == 0
+    outs = [buf(len(s)) for s in want_sm]
+    mls = (C.c_ulonglong * k)()
+    vr = (C.c_int * k)()
+    assert L.xmss_mi355x_verify_collect(vjob, _arr(outs), mls, vr) == 0
+    assert [vr[i] for i in range(k)] == [0] * k
+    assert [raw(outs[i], mls[i]) for i in range(k)] == msgs
+    assert L.xmss_mi355x_job_wait(sjob) == 0
+    got_sm, got_sk = sign_collect(sjob, skb2)
+    assert got_sm == want_sm and got_sk == want_sk
+    L.xmss_mi355x_job_free(sjob)
+    L.xmss_mi355x_job_free(vjob)
