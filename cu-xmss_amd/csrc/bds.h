@@ -43,11 +43,17 @@ struct BdsOps {
 };
 
 // Backend that computes nothing and records which leaves and which tree
-// merges (their ADRS) were asked for, in order.
+// merges (their ADRS) were asked for, in order, and what each merge hashes.
+// Each leaf and merge result it hands back is a tag (a fixed n-5 byte
+// pattern, a kind byte, a 4-byte index), so that the inputs of a later
+// merge name their sources: a tag is a leaf or an earlier merge, any other
+// bytes are node values the sk itself holds (kept as literals).
 struct RecordingOps final : BdsOps {
     unsigned n;
     std::vector<LeafId> leaves;
-    std::vector<uint32_t> merges;  // 8 ADRS words per thash_h call
+    std::vector<uint32_t> merges;     // 8 ADRS words per thash_h call
+    std::vector<uint32_t> dag;        // 3 words per merge: left source, right source, level
+    std::vector<unsigned char> lits;  // n bytes per literal source
     explicit RecordingOps(unsigned n_) : n(n_) {}
     void bind_key(const unsigned char*, const unsigned char*, const unsigned char*) override {}
     void leaf(unsigned char* out, const LeafId& id) override;
@@ -69,12 +75,17 @@ std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* s
 
 // The merges (thash_h ADRS, 8 words each) bds_sign would hash for this sk, in
 // order, beside the leaves in order (a dry run on a copy).
+// A source is kind | index (Engine::BdsJob::plan_dag); the level of a merge is
+// 1 + the deepest level among the merges it depends on (leaves and literals
+// are level 0), so that merges of one level can be hashed side by side.
 void bds_plan(const xmss_params* p, const unsigned char* sk, unsigned long long mlen, std::vector<LeafId>& leaves,
-              std::vector<uint32_t>& merges);
+              std::vector<uint32_t>& merges, std::vector<uint32_t>* dag = nullptr,
+              std::vector<unsigned char>* lits = nullptr);
 
 // A signature whose traversal runs on the device (Engine::bds_sign_batch).
 struct BdsDeviceSign {
-    std::vector<uint32_t> layer, leaf, merges;
+    std::vector<uint32_t> layer, leaf, merges, dag;
+    std::vector<unsigned char> lits;
     std::vector<uint64_t> tree;
     std::vector<unsigned char> seeds;  // SK_SEED, SK_PRF, PUB_SEED, root
     unsigned long long idx = 0;

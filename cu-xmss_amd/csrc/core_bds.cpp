@@ -327,13 +327,51 @@ int prepare_engine(const xmss_params* p, Engine** e, int* fam) {
 
 namespace xm {
 
-void RecordingOps::leaf(unsigned char* out, const LeafId& id) {
-    leaves.push_back(id);
-    memset(out, 0, n);
+// The tag pattern: bytes no hash output or unwritten sk region is expected
+// to hold (a chance of 2^-(8(n-5)) per node value).
+static const unsigned char* tag_pattern() {
+    static unsigned char t[64];
+    static bool init = false;
+    if (!init) {
+        uint64_t x = 0x5852a7e3c11d94b5ull;  // fixed: the tags never leave the dry run
+        for (unsigned char& b : t) {
+            x = x * 6364136223846793005ull + 1442695040888963407ull;
+            b = (unsigned char)(x >> 56) | 0x01;
+        }
+        init = true;
+    }
+    return t;
 }
-void RecordingOps::thash_h(unsigned char* out, const unsigned char*, const uint32_t* adrs) {
+static void write_tag(unsigned char* out, unsigned n, uint32_t kind, size_t index) {
+    memcpy(out, tag_pattern(), n - 5);
+    out[n - 5] = (unsigned char)(kind >> 28);
+    for (int b = 0; b < 4; ++b) out[n - 4 + b] = (unsigned char)(index >> (24 - 8 * b));
+}
+
+void RecordingOps::leaf(unsigned char* out, const LeafId& id) {
+    write_tag(out, n, Engine::DAG_LEAF, leaves.size());
+    leaves.push_back(id);
+}
+void RecordingOps::thash_h(unsigned char* out, const unsigned char* in, const uint32_t* adrs) {
+    uint32_t src[2], lev = 1;
+    for (int h = 0; h < 2; ++h) {
+        const unsigned char* x = in + h * n;
+        const uint32_t kind = (uint32_t)x[n - 5] << 28;
+        const uint32_t idx = ((uint32_t)x[n - 4] << 24) | ((uint32_t)x[n - 3] << 16) | ((uint32_t)x[n - 2] << 8) | x[n - 1];
+        if (memcmp(x, tag_pattern(), n - 5) == 0 && (kind == Engine::DAG_LEAF || kind == Engine::DAG_MERGE)) {
+            src[h] = kind | idx;
+            if (kind == Engine::DAG_MERGE) lev = std::max(lev, dag[3 * idx + 2] + 1);
+        } else {
+            src[h] = Engine::DAG_LIT | (uint32_t)(lits.size() / n);
+            lits.insert(lits.end(), x, x + n);
+        }
+    }
+    const size_t k = merges.size() / 8;
     merges.insert(merges.end(), adrs, adrs + 8);
-    memset(out, 0, n);
+    dag.push_back(src[0]);
+    dag.push_back(src[1]);
+    dag.push_back(lev);
+    write_tag(out, n, Engine::DAG_MERGE, k);
 }
 void RecordingOps::wots_sign(unsigned char* sig, const unsigned char*, uint32_t, uint64_t, uint32_t) {
     memset(sig, 0, (size_t)(2 * n + 3) * n);
@@ -483,7 +521,7 @@ std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* s
 }
 
 void bds_plan(const xmss_params* p, const unsigned char* sk, unsigned long long mlen, std::vector<LeafId>& leaves,
-              std::vector<uint32_t>& merges) {
+              std::vector<uint32_t>& merges, std::vector<uint32_t>* dag, std::vector<unsigned char>* lits) {
     std::vector<unsigned char> skc(sk, sk + p->sk_bytes);
     std::vector<unsigned char> smc(p->sig_bytes + mlen + 1), mc(mlen + 1, 0);
     unsigned long long l = 0;
@@ -491,6 +529,8 @@ void bds_plan(const xmss_params* p, const unsigned char* sk, unsigned long long 
     bds_sign(p, skc.data(), smc.data(), &l, mc.data(), mlen, rec);
     leaves.swap(rec.leaves);
     merges.swap(rec.merges);
+    if (dag) dag->swap(rec.dag);
+    if (lits) lits->swap(rec.lits);
 }
 
 bool bds_device_prepare(const xmss_params* p, unsigned char* sk, unsigned char* sm, const unsigned char* m,
@@ -500,7 +540,7 @@ bool bds_device_prepare(const xmss_params* p, unsigned char* sk, unsigned char* 
     for (unsigned i = 0; i < ib; ++i) idx |= ((unsigned long long)sk[i]) << 8 * (ib - 1 - i);
     if (idx >= (1ULL << p->full_height) - 1) return false;  // wipe / exhausted: host path
     std::vector<LeafId> plan;
-    bds_plan(p, sk, mlen, plan, ds.merges);
+    bds_plan(p, sk, mlen, plan, ds.merges, &ds.dag, &ds.lits);
     ds.layer.resize(plan.size());
     ds.tree.resize(plan.size());
     ds.leaf.resize(plan.size());
@@ -533,6 +573,9 @@ bool bds_device_prepare(const xmss_params* p, unsigned char* sk, unsigned char* 
     ds.job.nplan = (uint32_t)plan.size();
     ds.job.plan_merge = ds.merges.data();
     ds.job.nmerge = (uint32_t)(ds.merges.size() / 8);
+    ds.job.plan_dag = ds.dag.data();
+    ds.job.plan_lit = ds.lits.data();
+    ds.job.nlit = (uint32_t)(ds.lits.size() / n);
     ds.job.R_out = sm + ib;
     ds.job.wots0_out = sm + ib + n;
     ds.job.tail_out = sm + ib + n + p->wots_sig_bytes;

@@ -324,6 +324,14 @@ public:
         uint32_t nplan;
         const uint32_t* plan_merge;  // planned tree merges: 8 ADRS words each, in hashing order
         uint32_t nmerge;
+        // What each planned merge hashes, or null (merges hashed one after
+        // another as the state machine reaches them): 3 words per merge --
+        // the sources of its left and right halves (DAG_LIT | index into
+        // plan_lit, DAG_LEAF | index of a planned leaf, DAG_MERGE | index of
+        // an earlier merge) and its level (1 + the deepest merge it needs).
+        const uint32_t* plan_dag;
+        const uint8_t* plan_lit;     // n bytes per literal (node values the sk holds)
+        uint32_t nlit;
         uint8_t* R_out;        // n
         uint8_t* wots0_out;    // wots_sig_bytes
         uint8_t* tail_out;     // sig_bytes - index_bytes - n - wots_sig_bytes
@@ -333,6 +341,7 @@ public:
         } shape;
     };
     using BdsShape = BdsJob::Shape;
+    enum : uint32_t { DAG_LIT = 0u << 28, DAG_LEAF = 1u << 28, DAG_MERGE = 2u << 28, DAG_INDEX = (1u << 28) - 1 };
 
     // Verification of signatures of keys in the device pool of `fam`
     // (pool_slot with PUB_SEED set), any mix of parameter shapes, in one

@@ -67,6 +67,11 @@ constexpr int kLatDense = 256;
 // Up to this many leaves per signing batch, its subtrees' leaves come from
 // the latency kernels (the throughput kernels would leave most SIMDs idle).
 constexpr uint64_t kLatLeaves = 16384;
+// Dynamic LDS of k_bds_update: the serialized states plus, when they fit in
+// this much, the values of the planned merges.
+constexpr size_t kBdsLds = 64 * 1024;
+static_assert(Engine::DAG_LEAF == DAG_LEAF && Engine::DAG_MERGE == DAG_MERGE && Engine::DAG_INDEX == DAG_INDEX,
+              "merge source codes of the host plan and the kernel");
 static bool use_lat(FamOps* o, size_t count, uint32_t th) {
     return count > 0 && count <= 65535 && count * o->lat_bytes(th) <= kLatScratch;
 }
@@ -1274,6 +1279,8 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     // per-job layout: planned leaves, sk bytes, signature tail
     std::vector<DLeaf> ld;
     std::vector<DThash> mg;
+    std::vector<uint32_t> dag;  // 3 words per merge, parallel to mg
+    std::vector<uint8_t> lits;
     std::vector<DBds> jd(count);
     std::vector<size_t> tail_bytes(count);
     size_t sk_total = 0, tail_total = 0, lds = 0;
@@ -1306,7 +1313,20 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
         P = {s.th, s.d, s.full_height, s.index_bytes, s.bds_k, s.wots_sig_bytes, s.sig_bytes, 0};
         P.state_bytes = (s.th + 1) * n + 4 + s.th + 1 + s.th * n + (s.th >> 1) * n + (s.th - s.bds_k) * (7 + n) +
                         ((1u << s.bds_k) - s.bds_k - 1) * n + 4;
-        lds = std::max(lds, (size_t)(2 * s.d - 1) * P.state_bytes);
+        const size_t region = (size_t)(2 * s.d - 1) * P.state_bytes;
+        // the merge DAG: its values go to LDS after the states, if they fit
+        const size_t with_dag = ((region + 15) & ~(size_t)15) + (size_t)jobs[i].nmerge * n;
+        jd[i].maxlev = 0;
+        jd[i].lit_off = lits.size() / n;
+        if (jobs[i].plan_dag && jobs[i].nmerge && with_dag <= kBdsLds) {
+            for (uint32_t k = 0; k < jobs[i].nmerge; ++k)
+                jd[i].maxlev = std::max(jd[i].maxlev, jobs[i].plan_dag[3 * k + 2]);
+            dag.insert(dag.end(), jobs[i].plan_dag, jobs[i].plan_dag + 3 * jobs[i].nmerge);
+            lits.insert(lits.end(), jobs[i].plan_lit, jobs[i].plan_lit + (size_t)jobs[i].nlit * n);
+        } else {
+            dag.resize(dag.size() + 3 * jobs[i].nmerge, 0);
+        }
+        lds = std::max(lds, jd[i].maxlev ? with_dag : region);
     }
     // H_msg inputs
     const size_t pb = pad + 3 * n;
@@ -1323,7 +1343,8 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t o_ld = 0, o_md = o_ld + al(sizeof(DLeaf) * ld.size()), o_blob = o_md + al(sizeof(DMsg) * count),
                  o_wd = o_blob + al(total), o_jd = o_wd + al(sizeof(DLeaf) * count),
-                 o_mg = o_jd + al(sizeof(DBds) * count), o_sk = o_mg + al(sizeof(DThash) * mg.size()),
+                 o_mg = o_jd + al(sizeof(DBds) * count), o_dag = o_mg + al(sizeof(DThash) * mg.size()),
+                 o_lit = o_dag + al(sizeof(uint32_t) * dag.size()), o_sk = o_lit + al(lits.size()),
                  o_R = o_sk + al(sk_total), o_W = o_R + al((size_t)count * n),
                  o_T = o_W + al((size_t)count * len * n), io_bytes = o_T + al(tail_total);
     uint8_t* h = B.staging.need(io_bytes);
@@ -1333,6 +1354,8 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     memset(h + o_blob, 0, total);
     memcpy(h + o_jd, jd.data(), sizeof(DBds) * count);
     memcpy(h + o_mg, mg.data(), sizeof(DThash) * mg.size());
+    memcpy(h + o_dag, dag.data(), sizeof(uint32_t) * dag.size());
+    memcpy(h + o_lit, lits.data(), lits.size());
     const size_t nl = ld.size();
     if (nl + count + (mg.size() + count) / WG + 2 > 65535 || !use_lat(o, nl + count, 0))
         fatal("bds_sign_chunk: batch too large for one launch sequence");
@@ -1359,6 +1382,8 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     const DLeaf* dw = reinterpret_cast<const DLeaf*>(dv + o_wd);
     const DBds* djobs = reinterpret_cast<const DBds*>(dv + o_jd);
     const DThash* dmg = reinterpret_cast<const DThash*>(dv + o_mg);
+    const uint32_t* ddag = reinterpret_cast<const uint32_t*>(dv + o_dag);
+    const uint8_t* dlit = dv + o_lit;
     uint8_t* ska = dv + o_sk;
     uint8_t* dR = dv + o_R;
     uint8_t* dW = dv + o_W;
@@ -1373,7 +1398,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     o->bds_lat(I->L, dk, dmd, pad, db, dR, B.dig.need((size_t)count * n), dld, (uint32_t)nl, dw, (uint32_t)count,
                B.lat_leaf.need((nl + count) * o->lat_bytes(0)), dmg, (uint32_t)mg.size(), merge_arena, leaf_arena,
                dW);
-    o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, dT,
+    o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, ddag, dlit, dT,
                   B.scratch.need((size_t)count * 4 * n));
     I->d2h(h + o_sk, ska, io_bytes - o_sk);
     I->sync();

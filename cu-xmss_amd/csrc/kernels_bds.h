@@ -36,8 +36,12 @@ struct DBds {
     BdsP P;               // parameter shape of this key
     uint64_t res;         // device address of the key's resident sk (index not advanced), or 0
     uint32_t sk_bytes;    // OID-less sk size
-    uint32_t unused;
+    uint32_t maxlev;      // levels of the merge DAG (0: merges hashed as the state machine reaches them)
+    uint64_t lit_off;     // first literal (n bytes each) in the literal arena
 };
+
+// Sources of a planned merge's halves (kind in the top 4 bits, index below).
+enum : uint32_t { DAG_LIT = 0u << 28, DAG_LEAF = 1u << 28, DAG_MERGE = 2u << 28, DAG_INDEX = (1u << 28) - 1 };
 
 template <class F>
 struct BdsDev {
@@ -48,6 +52,7 @@ struct BdsDev {
     uint32_t cur;
     const typename F::HPre* merges;  // the planned merges' address-only work, in hashing order
     uint32_t mcur, nmerge;
+    const uint8_t* mval = nullptr;   // the planned merges' values, when computed up front (LDS)
     const typename F::Key& key;
     uint32_t tid;
 
@@ -118,6 +123,11 @@ struct BdsDev {
     // in a wide launch before this one (F::pre_h); past the planned count the
     // whole thash_h runs here.
     __device__ void hash_h(uint8_t* dst, const uint8_t* in, const uint32_t (&a)[8]) {
+        if (mval && mcur < nmerge) {  // computed before the state machine ran
+            copy(dst, mval + (size_t)mcur * n);
+            ++mcur;
+            return;
+        }
         // node buffers are not 4-byte aligned in the serialized state: the
         // lanes gather the 2n input bytes into an aligned buffer first
         __shared__ alignas(16) uint8_t hb[3 * F::kN];
@@ -382,11 +392,20 @@ __global__ __launch_bounds__(192) void k_bds_fin(const DLeaf* __restrict__ leave
 // paths and lower-layer WOTS signatures go to the signature tail, then the
 // traversal updates the state at skg.  `scratch` holds 4n bytes per workgroup.
 template <class F>
+__device__ void dag_load(typename F::Node& x, uint32_t src, const uint8_t* lits, const uint8_t* leaves,
+                         const uint8_t* mval) {
+    const uint32_t i = src & DAG_INDEX;
+    const uint8_t* p = (src & ~DAG_INDEX) == DAG_LIT ? lits : (src & ~DAG_INDEX) == DAG_LEAF ? leaves : mval;
+    F::load(x, p + (size_t)i * F::kN);
+}
+
+template <class F>
 __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint8_t* skg,
                                const uint8_t* __restrict__ leaf_arena,
-                               const typename F::HPre* __restrict__ merge_arena, uint8_t* __restrict__ tails,
+                               const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
+                               const uint8_t* __restrict__ lit_arena, uint8_t* __restrict__ tails,
                                uint8_t* __restrict__ scratch_arena) {
-    extern __shared__ uint8_t lds_states[];
+    extern __shared__ __align__(16) uint8_t lds_states[];
     const BdsP& P = job.P;
     BdsDev<F> B(lds_states, skg, P, leaf_arena + job.leaf_off * F::kN, merge_arena + job.merge_off, job.nmerge,
                 key);
@@ -397,6 +416,31 @@ __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint
     const uint32_t region = (2 * d - 1) * P.state_bytes;
     // stage the serialized states in LDS; written back when the update is done
     for (uint32_t b = threadIdx.x; b < region; b += blockDim.x) lds_states[b] = skg[B.states_base() + b];
+    // The planned merges whose sources the host's dry run resolved (node
+    // values of the sk, planned leaves, earlier merges) are hashed here level
+    // by level, one lane per merge, into LDS after the states; the state
+    // machine below then copies merge k where it would hash it.  Only merges
+    // of one chain depend on each other, so this takes a few merge latencies
+    // instead of one per merge.
+    if (job.maxlev) {
+        uint8_t* mv = lds_states + ((region + 15) & ~15u);
+        const uint32_t* q0 = dag + 3 * job.merge_off;
+        const uint8_t* lits = lit_arena + job.lit_off * n;
+        const typename F::HPre* pre = merge_arena + job.merge_off;
+        for (uint32_t lev = 1; lev <= job.maxlev; ++lev) {
+            for (uint32_t m = threadIdx.x; m < job.nmerge; m += blockDim.x) {
+                const uint32_t* q = q0 + 3 * m;
+                if (q[2] != lev) continue;
+                typename F::Node L, R, o;
+                dag_load<F>(L, q[0], lits, B.leaves, mv);
+                dag_load<F>(R, q[1], lits, B.leaves, mv);
+                F::fin_h(o, pre[m], L, R);
+                F::store(o, mv + (size_t)m * n);
+            }
+            __syncthreads();
+        }
+        B.mval = mv;
+    }
     __syncthreads();
     struct WriteBack {
         uint8_t* l;
@@ -478,7 +522,8 @@ __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint
 template <class F>
 __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __restrict__ keys, const DBds* __restrict__ jobs,
                              uint8_t* __restrict__ sk_arena, const uint8_t* __restrict__ leaf_arena,
-                             const typename F::HPre* __restrict__ merge_arena,
+                             const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
+                             const uint8_t* __restrict__ lit_arena,
                              uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena) {
     const DBds job = jobs[blockIdx.x];
     uint8_t* skg = sk_arena + job.sk_off;
@@ -492,7 +537,7 @@ __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __res
         }
         __syncthreads();
     }
-    bds_update_one<F>(keys[job.key], job, skg, leaf_arena, merge_arena, tails, scratch_arena);
+    bds_update_one<F>(keys[job.key], job, skg, leaf_arena, merge_arena, dag, lit_arena, tails, scratch_arena);
     if (res) {
         __syncthreads();
         for (uint32_t b = threadIdx.x; b < job.sk_bytes; b += blockDim.x) res[b] = skg[b];

@@ -165,9 +165,12 @@ struct FamOps {
                          uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs, uint32_t nsign,
                          uint8_t* scratch, const DThash* merges, uint32_t nmerge, void* merge_out,
                          uint8_t* leaf_out, uint8_t* sig_out) = 0;
+    // The state updates, one workgroup per job; dag / lits: the planned
+    // merges' sources (3 words per merge, parallel to merge_arena) and the
+    // literal node values (jobs with maxlev 0 hash their merges one by one).
     virtual void bds_update(Launcher& L, const void* keys, size_t lds, const DBds* jobs, uint32_t n,
-                            uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena, uint8_t* tails,
-                            uint8_t* scratch) = 0;
+                            uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena,
+                            const uint32_t* dag, const uint8_t* lits, uint8_t* tails, uint8_t* scratch) = 0;
 };
 
 template <class F>
@@ -333,10 +336,10 @@ struct FamOpsT final : FamOps {
               digests, (const uint8_t*)scratch, stride, leaf_out, sig_out);
     }
     void bds_update(Launcher& L, const void* keys, size_t lds, const DBds* jobs, uint32_t n,
-                    uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena, uint8_t* tails,
-                    uint8_t* scratch) override {
+                    uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena, const uint32_t* dag,
+                    const uint8_t* lits, uint8_t* tails, uint8_t* scratch) override {
         L.run_lds("bds_update", k_bds_update<F>, dim3(n), dim3(((F::LEN + 63) / 64) * 64), lds, (const K*)keys,
-                  jobs, sk_arena, leaf_arena, (const typename F::HPre*)merge_arena, tails, scratch);
+                  jobs, sk_arena, leaf_arena, (const typename F::HPre*)merge_arena, dag, lits, tails, scratch);
     }
 };
 
