@@ -54,11 +54,12 @@ struct RecordingOps final : BdsOps {
     std::vector<uint32_t> merges;     // 8 ADRS words per thash_h call
     std::vector<uint32_t> dag;        // 3 words per merge: left source, right source, level
     std::vector<unsigned char> lits;  // n bytes per literal source
+    std::vector<uint32_t> wots_up;    // 4 words per upper-layer WOTS signature: layer, tree hi, tree lo, ots
     explicit RecordingOps(unsigned n_) : n(n_) {}
     void bind_key(const unsigned char*, const unsigned char*, const unsigned char*) override {}
     void leaf(unsigned char* out, const LeafId& id) override;
     void thash_h(unsigned char* out, const unsigned char*, const uint32_t*) override;
-    void wots_sign(unsigned char* sig, const unsigned char*, uint32_t, uint64_t, uint32_t) override;
+    void wots_sign(unsigned char* sig, const unsigned char*, uint32_t layer, uint64_t tree, uint32_t ots) override;
     void hmsg_sign(unsigned char* R, unsigned char* digest, uint64_t, const unsigned char*, const unsigned char*,
                    uint64_t) override;
 };
@@ -80,11 +81,11 @@ std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* s
 // are level 0), so that merges of one level can be hashed side by side.
 void bds_plan(const xmss_params* p, const unsigned char* sk, unsigned long long mlen, std::vector<LeafId>& leaves,
               std::vector<uint32_t>& merges, std::vector<uint32_t>* dag = nullptr,
-              std::vector<unsigned char>* lits = nullptr);
+              std::vector<unsigned char>* lits = nullptr, std::vector<uint32_t>* wots_up = nullptr);
 
 // A signature whose traversal runs on the device (Engine::bds_sign_batch).
 struct BdsDeviceSign {
-    std::vector<uint32_t> layer, leaf, merges, dag;
+    std::vector<uint32_t> layer, leaf, merges, dag, wots_up;
     std::vector<unsigned char> lits;
     std::vector<uint64_t> tree;
     std::vector<unsigned char> seeds;  // SK_SEED, SK_PRF, PUB_SEED, root

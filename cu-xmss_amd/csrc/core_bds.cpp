@@ -373,7 +373,11 @@ void RecordingOps::thash_h(unsigned char* out, const unsigned char* in, const ui
     dag.push_back(lev);
     write_tag(out, n, Engine::DAG_MERGE, k);
 }
-void RecordingOps::wots_sign(unsigned char* sig, const unsigned char*, uint32_t, uint64_t, uint32_t) {
+void RecordingOps::wots_sign(unsigned char* sig, const unsigned char*, uint32_t layer, uint64_t tree, uint32_t ots) {
+    if (layer > 0) {  // an XMSS^MT subtree root signed by the layer above
+        const uint32_t w[4] = {layer, (uint32_t)(tree >> 32), (uint32_t)tree, ots};
+        wots_up.insert(wots_up.end(), w, w + 4);
+    }
     memset(sig, 0, (size_t)(2 * n + 3) * n);
 }
 void RecordingOps::hmsg_sign(unsigned char* R, unsigned char* digest, uint64_t, const unsigned char*,
@@ -521,7 +525,8 @@ std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* s
 }
 
 void bds_plan(const xmss_params* p, const unsigned char* sk, unsigned long long mlen, std::vector<LeafId>& leaves,
-              std::vector<uint32_t>& merges, std::vector<uint32_t>* dag, std::vector<unsigned char>* lits) {
+              std::vector<uint32_t>& merges, std::vector<uint32_t>* dag, std::vector<unsigned char>* lits,
+              std::vector<uint32_t>* wots_up) {
     std::vector<unsigned char> skc(sk, sk + p->sk_bytes);
     std::vector<unsigned char> smc(p->sig_bytes + mlen + 1), mc(mlen + 1, 0);
     unsigned long long l = 0;
@@ -531,6 +536,7 @@ void bds_plan(const xmss_params* p, const unsigned char* sk, unsigned long long 
     merges.swap(rec.merges);
     if (dag) dag->swap(rec.dag);
     if (lits) lits->swap(rec.lits);
+    if (wots_up) wots_up->swap(rec.wots_up);
 }
 
 bool bds_device_prepare(const xmss_params* p, unsigned char* sk, unsigned char* sm, const unsigned char* m,
@@ -540,7 +546,7 @@ bool bds_device_prepare(const xmss_params* p, unsigned char* sk, unsigned char* 
     for (unsigned i = 0; i < ib; ++i) idx |= ((unsigned long long)sk[i]) << 8 * (ib - 1 - i);
     if (idx >= (1ULL << p->full_height) - 1) return false;  // wipe / exhausted: host path
     std::vector<LeafId> plan;
-    bds_plan(p, sk, mlen, plan, ds.merges, &ds.dag, &ds.lits);
+    bds_plan(p, sk, mlen, plan, ds.merges, &ds.dag, &ds.lits, &ds.wots_up);
     ds.layer.resize(plan.size());
     ds.tree.resize(plan.size());
     ds.leaf.resize(plan.size());
@@ -576,6 +582,8 @@ bool bds_device_prepare(const xmss_params* p, unsigned char* sk, unsigned char* 
     ds.job.plan_dag = ds.dag.data();
     ds.job.plan_lit = ds.lits.data();
     ds.job.nlit = (uint32_t)(ds.lits.size() / n);
+    ds.job.plan_wots = ds.wots_up.data();
+    ds.job.nwots = (uint32_t)(ds.wots_up.size() / 4);
     ds.job.R_out = sm + ib;
     ds.job.wots0_out = sm + ib + n;
     ds.job.tail_out = sm + ib + n + p->wots_sig_bytes;

@@ -332,6 +332,11 @@ public:
         const uint32_t* plan_dag;
         const uint8_t* plan_lit;     // n bytes per literal (node values the sk holds)
         uint32_t nlit;
+        // XMSS^MT subtree roots the update signs (layer >= 1), 4 words each:
+        // layer, tree hi, tree lo, ots -- their address-only work is done in
+        // the pre-image launch; null: whole chains in the update
+        const uint32_t* plan_wots;
+        uint32_t nwots;
         uint8_t* R_out;        // n
         uint8_t* wots0_out;    // wots_sig_bytes
         uint8_t* tail_out;     // sig_bytes - index_bytes - n - wots_sig_bytes

@@ -1254,7 +1254,8 @@ void Engine::bds_sign_batch(int fam, int count, BdsJob* jobs) {
     int start = 0;
     size_t rows = 0, extra = 0;
     for (int i = 0; i <= count; ++i) {
-        const size_t r = i < count ? jobs[i].nplan + 1 : 0, x = i < count ? jobs[i].nmerge + 1 : 0;
+        const size_t r = i < count ? jobs[i].nplan + 1 + (jobs[i].plan_wots ? jobs[i].nwots : 0) : 0,
+                     x = i < count ? jobs[i].nmerge + 1 : 0;
         const size_t rows2 = rows + r, extra2 = extra + x;
         if (i == count || (i > start && (rows2 + extra2 / WG + 2 > 65535 || rows2 * lb > kLatScratch))) {
             if (i > start) bds_sign_chunk(fam, i - start, jobs + start);
@@ -1281,6 +1282,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     std::vector<DThash> mg;
     std::vector<uint32_t> dag;  // 3 words per merge, parallel to mg
     std::vector<uint8_t> lits;
+    std::vector<DLeaf> up;      // upper-layer WOTS signatures of the state updates (pre-images only)
     std::vector<DBds> jd(count);
     std::vector<size_t> tail_bytes(count);
     size_t sk_total = 0, tail_total = 0, lds = 0;
@@ -1327,6 +1329,12 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
             dag.resize(dag.size() + 3 * jobs[i].nmerge, 0);
         }
         lds = std::max(lds, jd[i].maxlev ? with_dag : region);
+        jd[i].wots_row = (uint32_t)up.size();
+        jd[i].nwots = jobs[i].plan_wots ? jobs[i].nwots : 0;
+        for (uint32_t k = 0; k < jd[i].nwots; ++k) {
+            const uint32_t* w = jobs[i].plan_wots + 4 * k;
+            up.push_back({jobs[i].key, w[0], w[1], w[2], w[3], 0, LAT_ALL_STEPS, 0});
+        }
     }
     // H_msg inputs
     const size_t pb = pad + 3 * n;
@@ -1342,7 +1350,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     //   [descriptors, H_msg blob | sk arena (updated in place) | R, WOTS, tails]
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t o_ld = 0, o_md = o_ld + al(sizeof(DLeaf) * ld.size()), o_blob = o_md + al(sizeof(DMsg) * count),
-                 o_wd = o_blob + al(total), o_jd = o_wd + al(sizeof(DLeaf) * count),
+                 o_wd = o_blob + al(total), o_jd = o_wd + al(sizeof(DLeaf) * (count + up.size())),
                  o_mg = o_jd + al(sizeof(DBds) * count), o_dag = o_mg + al(sizeof(DThash) * mg.size()),
                  o_lit = o_dag + al(sizeof(uint32_t) * dag.size()), o_sk = o_lit + al(lits.size()),
                  o_R = o_sk + al(sk_total), o_W = o_R + al((size_t)count * n),
@@ -1356,8 +1364,8 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     memcpy(h + o_mg, mg.data(), sizeof(DThash) * mg.size());
     memcpy(h + o_dag, dag.data(), sizeof(uint32_t) * dag.size());
     memcpy(h + o_lit, lits.data(), lits.size());
-    const size_t nl = ld.size();
-    if (nl + count + (mg.size() + count) / WG + 2 > 65535 || !use_lat(o, nl + count, 0))
+    const size_t nl = ld.size(), nw = count + up.size();
+    if (nl + nw + (mg.size() + count) / WG + 2 > 65535 || !use_lat(o, nl + nw, 0))
         fatal("bds_sign_chunk: batch too large for one launch sequence");
     DLeaf* wd = reinterpret_cast<DLeaf*>(h + o_wd);
     for (int i = 0; i < count; ++i) {
@@ -1376,6 +1384,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
                  (uint32_t)(jobs[i].idx & ((1ull << th) - 1)), 0, LAT_ALL_STEPS, 0};
         if (!jobs[i].dsk) memcpy(h + o_sk + jd[i].sk_off, jobs[i].sk, jobs[i].shape.sk_bytes);
     }
+    if (!up.empty()) memcpy(wd + count, up.data(), sizeof(DLeaf) * up.size());
     const DLeaf* dld = reinterpret_cast<const DLeaf*>(dv + o_ld);
     const DMsg* dmd = reinterpret_cast<const DMsg*>(dv + o_md);
     uint8_t* db = dv + o_blob;
@@ -1395,11 +1404,13 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     // of the planned leaves, the layer-0 WOTS signatures (all steps: the
     // digests are computed in the same launch) and the merges, then the
     // leaves' and signatures' finishing work, then the state update
-    o->bds_lat(I->L, dk, dmd, pad, db, dR, B.dig.need((size_t)count * n), dld, (uint32_t)nl, dw, (uint32_t)count,
-               B.lat_leaf.need((nl + count) * o->lat_bytes(0)), dmg, (uint32_t)mg.size(), merge_arena, leaf_arena,
-               dW);
-    o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, ddag, dlit, dT,
-                  B.scratch.need((size_t)count * 4 * n));
+    // scratch rows: the planned leaves, the layer-0 WOTS signatures, then the
+    // upper-layer WOTS signatures of the state updates
+    uint8_t* lat = B.lat_leaf.need((nl + nw) * o->lat_bytes(0));
+    o->bds_lat(I->L, dk, dmd, (uint32_t)count, pad, db, dR, B.dig.need((size_t)count * n), dld, (uint32_t)nl, dw,
+               (uint32_t)nw, lat, dmg, (uint32_t)mg.size(), merge_arena, leaf_arena, dW);
+    o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, ddag, dlit,
+                  lat + (nl + count) * o->lat_bytes(0), dT, B.scratch.need((size_t)count * 4 * n));
     I->d2h(h + o_sk, ska, io_bytes - o_sk);
     I->sync();
     for (int i = 0; i < count; ++i) {

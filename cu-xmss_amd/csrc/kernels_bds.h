@@ -38,6 +38,8 @@ struct DBds {
     uint32_t sk_bytes;    // OID-less sk size
     uint32_t maxlev;      // levels of the merge DAG (0: merges hashed as the state machine reaches them)
     uint64_t lit_off;     // first literal (n bytes each) in the literal arena
+    uint32_t wots_row;    // first pre-image row of its upper-layer WOTS signatures
+    uint32_t nwots;       // planned upper-layer WOTS signatures (beyond: whole chains)
 };
 
 // Sources of a planned merge's halves (kind in the top 4 bits, index below).
@@ -53,6 +55,9 @@ struct BdsDev {
     const typename F::HPre* merges;  // the planned merges' address-only work, in hashing order
     uint32_t mcur, nmerge;
     const uint8_t* mval = nullptr;   // the planned merges' values, when computed up front (LDS)
+    const uint8_t* wpre = nullptr;   // pre-image rows of the planned upper-layer WOTS signatures
+    size_t wstride = 0;
+    uint32_t wcur = 0, nwots = 0;
     const typename F::Key& key;
     uint32_t tid;
 
@@ -316,6 +321,22 @@ struct BdsDev {
         // copy the message to an aligned scratch first (it may move during the swap)
         if (tid < n) scratch[tid] = msg[tid];
         __syncthreads();
+        if (wcur < nwots) {
+            // planned: chain starts and every step's address-only work come
+            // from the pre-image launch (LAT_SIGN, all steps), so each step
+            // is one compression
+            const uint8_t* S = wpre + (size_t)wcur * wstride;
+            ++wcur;
+            if (tid < (uint32_t)F::LEN) {
+                typename F::Node x = reinterpret_cast<const typename F::Node*>(S + LatLayout<F>::starts)[tid];
+                const typename F::FPre* fp = reinterpret_cast<const typename F::FPre*>(S + LatLayout<F>::fpre) + tid * 15;
+                const uint32_t steps = chain_len<F>(scratch, tid);
+                for (uint32_t s = 0; s < steps; ++s) F::fin_f(x, fp[s], x);
+                store_bytes(x, sig + (size_t)tid * n);
+            }
+            __syncthreads();
+            return;
+        }
         if (tid < (uint32_t)F::LEN) {
             uint32_t a[8] = {layer, (uint32_t)(tree >> 32), (uint32_t)tree, 0, ots, tid, 0, 0};
             typename F::Node x;
@@ -344,7 +365,7 @@ template <class F>
 __global__ __launch_bounds__(WG) void k_bds_pre(const typename F::Key* __restrict__ keys,
                                                  const DMsg* __restrict__ msgs, uint32_t pad,
                                                  uint8_t* __restrict__ blob, uint8_t* __restrict__ R_out,
-                                                 uint8_t* __restrict__ digests, uint32_t hrows,
+                                                 uint8_t* __restrict__ digests, uint32_t hrows, uint32_t nmsg,
                                                  const DLeaf* __restrict__ leaves, uint32_t nleaf,
                                                  const DLeaf* __restrict__ sigs, uint32_t nsign,
                                                  uint8_t* __restrict__ scratch, size_t stride,
@@ -353,7 +374,7 @@ __global__ __launch_bounds__(WG) void k_bds_pre(const typename F::Key* __restric
     const uint32_t t = blockIdx.x * WG + threadIdx.x;
     if (blockIdx.y < hrows) {
         const uint32_t i = blockIdx.y * gridDim.x * WG + t;
-        if (i < nsign) hmsg_one<F>(keys, msgs, i, pad, blob, R_out, digests);
+        if (i < nmsg) hmsg_one<F>(keys, msgs, i, pad, blob, R_out, digests);
         return;
     }
     const uint32_t y = blockIdx.y - hrows;
@@ -403,12 +424,15 @@ template <class F>
 __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint8_t* skg,
                                const uint8_t* __restrict__ leaf_arena,
                                const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
-                               const uint8_t* __restrict__ lit_arena, uint8_t* __restrict__ tails,
-                               uint8_t* __restrict__ scratch_arena) {
+                               const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
+                               uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena) {
     extern __shared__ __align__(16) uint8_t lds_states[];
     const BdsP& P = job.P;
     BdsDev<F> B(lds_states, skg, P, leaf_arena + job.leaf_off * F::kN, merge_arena + job.merge_off, job.nmerge,
                 key);
+    B.wpre = wpre + (size_t)job.wots_row * wstride;
+    B.wstride = wstride;
+    B.nwots = job.nwots;
     uint8_t* scratch = scratch_arena + (size_t)blockIdx.x * 4 * F::kN;
     uint8_t* tail = tails + job.tail_off;
     const uint32_t n = F::kN, th = P.th, d = P.d;
@@ -523,7 +547,7 @@ template <class F>
 __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __restrict__ keys, const DBds* __restrict__ jobs,
                              uint8_t* __restrict__ sk_arena, const uint8_t* __restrict__ leaf_arena,
                              const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
-                             const uint8_t* __restrict__ lit_arena,
+                             const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
                              uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena) {
     const DBds job = jobs[blockIdx.x];
     uint8_t* skg = sk_arena + job.sk_off;
@@ -537,7 +561,8 @@ __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __res
         }
         __syncthreads();
     }
-    bds_update_one<F>(keys[job.key], job, skg, leaf_arena, merge_arena, dag, lit_arena, tails, scratch_arena);
+    bds_update_one<F>(keys[job.key], job, skg, leaf_arena, merge_arena, dag, lit_arena, wpre, wstride, tails,
+                      scratch_arena);
     if (res) {
         __syncthreads();
         for (uint32_t b = threadIdx.x; b < job.sk_bytes; b += blockDim.x) res[b] = skg[b];
