@@ -202,38 +202,36 @@ def bench_batch(api, count):
     return keygen_t, sign_run, sign_total, (ver_t, ver_refill_t, stream_t), st, sum(1 for i in idxs if i == 1023)
 
 
-def bench_verify_stream(L, job_a, count, in_a, in_b, want_a, want_b, nbatch=16):
+def bench_verify_stream(L, job_a, count, in_a, in_b, want_a, want_b, nbatch=16, njobs=3):
     """Seconds per batch of a stream of `nbatch` same-shape verification
-    batches (the two input sets alternating) through two jobs:
-    launch(A); loop { refill(B); collect(A); launch(B); swap }.  Host staging
-    and the PCIe copy of one batch overlap the GPU work of the other; every
+    batches (the two input sets alternating) through `njobs` jobs in
+    rotation: batch i is refilled into job i mod njobs, then batch
+    i - njobs + 1 is collected and batch i launched.  Host staging and the
+    PCIe copy of one batch overlap the GPU work of the others, and with three
+    jobs the GPU has the next batch queued while the host collects; every
     batch's verdicts are checked after the timed region."""
-    job_b = L.xmss_mi355x_verify_prepare(0, in_b[0], in_b[1], in_b[2], count)
-    L.xmss_mi355x_job_run(job_b)  # sizes the buffers, then captures the graph
-    L.xmss_mi355x_job_run(job_b)
-    jobs, holds = [job_a, job_b], [None, None]
+    jobs = [job_a]
+    for _ in range(njobs - 1):
+        j = L.xmss_mi355x_verify_prepare(0, in_b[0], in_b[1], in_b[2], count)
+        L.xmss_mi355x_job_run(j)  # sizes the buffers, then captures the graph
+        L.xmss_mi355x_job_run(j)
+        jobs.append(j)
     sets = [in_a, in_b]
     mls = [(C.c_ulonglong * count)() for _ in range(nbatch)]
     vrs = [(C.c_int * count)() for _ in range(nbatch)]
     t0 = time.perf_counter()
-    cur = 0
-    src = sets[0]
-    assert L.xmss_mi355x_verify_refill(jobs[0], src[0], src[1], src[2]) == 0
-    holds[0] = (0, src)
-    L.xmss_mi355x_job_launch(jobs[0])
-    for i in range(1, nbatch):
-        nxt = 1 - cur
-        src = sets[i % 2]
-        assert L.xmss_mi355x_verify_refill(jobs[nxt], src[0], src[1], src[2]) == 0
-        holds[nxt] = (i, src)
-        j, hs = holds[cur]
-        L.xmss_mi355x_verify_collect(jobs[cur], hs[3], mls[j], vrs[j])
-        L.xmss_mi355x_job_launch(jobs[nxt])
-        cur = nxt
-    j, hs = holds[cur]
-    L.xmss_mi355x_verify_collect(jobs[cur], hs[3], mls[j], vrs[j])
+    for i in range(nbatch + njobs - 1):
+        if i < nbatch:
+            src = sets[i % 2]
+            assert L.xmss_mi355x_verify_refill(jobs[i % njobs], src[0], src[1], src[2]) == 0
+        k = i - (njobs - 1)
+        if k >= 0:
+            L.xmss_mi355x_verify_collect(jobs[k % njobs], sets[k % 2][3], mls[k], vrs[k])
+        if i < nbatch:
+            L.xmss_mi355x_job_launch(jobs[i % njobs])
     t = time.perf_counter() - t0
-    L.xmss_mi355x_job_free(job_b)
+    for j in jobs[1:]:
+        L.xmss_mi355x_job_free(j)
     for i in range(nbatch):
         assert [vrs[i][j] for j in range(count)] == (want_a if i % 2 == 0 else want_b), "stream batch %d" % i
     return t / nbatch

@@ -87,8 +87,9 @@ int xmss_mi355x_job_run(xmss_mi355x_job *job);
  * refill returns once the inputs are in pinned host memory, so with two jobs
  * a stream of batches overlaps host staging and the PCIe copies with the GPU
  * work:  launch(A); loop { refill(B); collect(A); launch(B); swap(A, B); }
- * (A signing job's input copy keeps the compute stream.)  Each returns 0,
- * or -1 for a null job. */
+ * and with three jobs in rotation the GPU also has the next batch queued
+ * while the host collects.  (A signing job's input copy keeps the compute
+ * stream.)  Each returns 0, or -1 for a null job. */
 int xmss_mi355x_job_launch(xmss_mi355x_job *job);
 int xmss_mi355x_job_wait(xmss_mi355x_job *job);
 int xmss_mi355x_sign_collect(xmss_mi355x_job *job, unsigned char *const *sms,
