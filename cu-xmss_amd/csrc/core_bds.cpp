@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <vector>
 
@@ -330,17 +331,17 @@ namespace xm {
 // The tag pattern: bytes no hash output or unwritten sk region is expected
 // to hold (a chance of 2^-(8(n-5)) per node value).
 static const unsigned char* tag_pattern() {
-    static unsigned char t[64];
-    static bool init = false;
-    if (!init) {
+    // dry runs run on many threads at once (serving workers): initialized once, thread-safe
+    static const std::array<unsigned char, 64> t = [] {
+        std::array<unsigned char, 64> a{};
         uint64_t x = 0x5852a7e3c11d94b5ull;  // fixed: the tags never leave the dry run
-        for (unsigned char& b : t) {
+        for (unsigned char& b : a) {
             x = x * 6364136223846793005ull + 1442695040888963407ull;
             b = (unsigned char)(x >> 56) | 0x01;
         }
-        init = true;
-    }
-    return t;
+        return a;
+    }();
+    return t.data();
 }
 static void write_tag(unsigned char* out, unsigned n, uint32_t kind, size_t index) {
     memcpy(out, tag_pattern(), n - 5);
