@@ -428,6 +428,7 @@ struct xmss_mi355x_server {
     void work() {
         for (;;) {
             Request* r = nullptr;
+            bool others;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] { return stop || (r = pick()) != nullptr; });
@@ -435,7 +436,13 @@ struct xmss_mi355x_server {
                 transient += r->reserve;
                 peak = std::max(peak, resident + transient);
                 if (r->sign) keys[r->key].busy = true;
+                others = !queue.empty();
             }
+            // Wake-ups go one worker at a time: a worker that takes a request
+            // while others wait wakes the next one, which takes the next
+            // admissible request or goes back to sleep -- instead of every
+            // idle worker contending for the lock at once.
+            if (others) cv_work.notify_one();
             // the worker counts as active (more device work coming) from
             // here until its last device call has returned
             disp->enter();
@@ -451,7 +458,7 @@ struct xmss_mi355x_server {
                 r->done = true;
                 more = !queue.empty();  // freed budget or key may admit queued requests
             }
-            if (more) cv_work.notify_all();
+            if (more) cv_work.notify_one();
             cv_done.notify_all();
         }
     }
