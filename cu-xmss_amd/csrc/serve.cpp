@@ -388,6 +388,7 @@ struct Request {
     size_t reserve = 0;
     Clock::time_point t0, t1;
     bool done = false;
+    std::condition_variable cv;  // its result's waiter (no wake-ups of the others)
 };
 
 }  // namespace
@@ -396,7 +397,9 @@ struct xmss_mi355x_server {
     Engine* e = nullptr;
     std::unique_ptr<Dispatcher> disp;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work;
+    std::condition_variable cv_key;  // get_sk waiters: a key became free
+    int key_waiters = 0;
     std::vector<Key> keys;
     std::deque<Request*> queue;  // arrival order
     std::map<uint64_t, Request*> reqs;
@@ -449,17 +452,19 @@ struct xmss_mi355x_server {
             bool active = true;
             execute(r, active);
             if (active) disp->leave();
-            bool more;
+            bool more, key_freed;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 transient -= r->reserve;
                 if (r->sign) keys[r->key].busy = false;
                 r->t1 = Clock::now();
                 r->done = true;
+                r->cv.notify_all();
                 more = !queue.empty();  // freed budget or key may admit queued requests
+                key_freed = r->sign && key_waiters > 0;
             }
             if (more) cv_work.notify_one();
-            cv_done.notify_all();
+            if (key_freed) cv_key.notify_all();
         }
     }
 
@@ -645,7 +650,7 @@ int xmss_mi355x_server_result(xmss_mi355x_server* s, uint64_t id, unsigned char*
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return -4;
     Request* r = it->second;
-    s->cv_done.wait(lk, [r] { return r->done; });
+    r->cv.wait(lk, [r] { return r->done; });
     if (out) memcpy(out, r->out.data(), r->outlen);
     if (outlen) *outlen = r->outlen;
     if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t1 - r->t0).count();
@@ -658,7 +663,9 @@ int xmss_mi355x_server_result(xmss_mi355x_server* s, uint64_t id, unsigned char*
 int xmss_mi355x_server_get_sk(xmss_mi355x_server* s, int key, unsigned char* sk) {
     std::unique_lock<std::mutex> lk(s->mu);
     if (key < 0 || key >= (int)s->keys.size()) return -1;
-    s->cv_done.wait(lk, [&] { return !s->keys[key].busy; });
+    ++s->key_waiters;
+    s->cv_key.wait(lk, [&] { return !s->keys[key].busy; });
+    --s->key_waiters;
     memcpy(sk, s->keys[key].sk.data(), s->keys[key].sk.size());
     return 0;
 }
