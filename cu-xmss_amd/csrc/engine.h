@@ -1,11 +1,13 @@
 // engine.h -- host-side interface of the device engine (internal to the
 // library; the public C ABI lives in include/).
 //
-// The engine owns one HIP stream on the current device and a grow-only
-// workspace.  Every call is synchronous: inputs are host byte arrays, outputs
-// are written back to host byte arrays before the call returns.  The cores
-// (core_plain.cpp, core_bds.cpp, batch.cpp) express all hashing through these
-// calls; no hash is ever computed on the host.
+// The engine owns three HIP streams on the current device (two for kernels,
+// one for the copies of launched batches) and grow-only workspaces.  Calls
+// are synchronous -- inputs are host byte arrays, outputs are written back
+// to host byte arrays before the call returns -- except the batch calls
+// below, whose launch / wait split lets a caller overlap host work with the
+// GPU.  The cores (core_plain.cpp, core_bds.cpp, ext.cpp, serve.cpp) express
+// all hashing through these calls; no hash is ever computed on the host.
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
