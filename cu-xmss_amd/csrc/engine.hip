@@ -64,6 +64,10 @@ constexpr int kSplitRuns = 64;
 // lane-dense (per chain, per L-tree pair, per auth walk) instead of with one
 // workgroup per request.
 constexpr int kLatDense = 256;
+// Up to this many requests per layer, a signing batch's WOTS signatures come
+// from the latency kernels (address-only work of the needed steps wide, then
+// one compression per step) instead of one lane per whole chain (k_wots).
+constexpr int kLatWots = 256;
 // Up to this many leaves per signing batch, its subtrees' leaves come from
 // the latency kernels (the throughput kernels would leave most SIMDs idle).
 constexpr uint64_t kLatLeaves = 16384;
@@ -1143,7 +1147,7 @@ void Engine::run(Batch* b, bool wait_done) {
         const size_t rec = (size_t)(th + 1) * n;
         // few requests: the latency kernels (address-only work wide, then one
         // compression per chain step), else one lane per chain (k_wots)
-        const bool lat_wots = count <= kLatDense && use_lat(o, count, 0);
+        const bool lat_wots = count <= kLatWots && use_lat(o, count, 0);
         DLeaf* dw = nullptr;
         if (lat_wots) {
             dw = b->lreq.need<DLeaf>((size_t)d * count * sizeof(DLeaf));
