@@ -68,6 +68,10 @@ constexpr int kLatDense = 256;
 // from the latency kernels (address-only work of the needed steps wide, then
 // one compression per step) instead of one lane per whole chain (k_wots).
 constexpr int kLatWots = 256;
+// Up to this many XMSS^MT requests per verify_many call, the upper layers'
+// chains are hashed speculatively for every start digit (16x their work)
+// beside layer 0, taking the chains off each upper layer's serial path.
+constexpr int kSpecVerify = 64;
 // Up to this many leaves per signing batch, its subtrees' leaves come from
 // the latency kernels (the throughput kernels would leave most SIMDs idle).
 constexpr uint64_t kLatLeaves = 16384;
@@ -1427,7 +1431,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
 
 
 struct VerifyBufs {
-    DevBuf io, dig, roots, scratch;
+    DevBuf io, dig, roots, scratch, spec;
     PinnedBuf staging;
 };
 
@@ -1525,10 +1529,22 @@ void Engine::verify_many(int fam, int count, const VerifyJob* jobs, int* ok) {
     uint8_t* dig = V.dig.need((size_t)count * n);
     uint8_t* scratch = V.scratch.need((size_t)items * o->lat_bytes(thmax));
     // two launches on one stream: H_msg with the address-only work of every
-    // layer, then every layer of each request in one workgroup
+    // layer, then every layer of each request in one workgroup.  For few
+    // XMSS^MT requests, three: the second walks layer 0 beside the
+    // speculative chains of the upper layers (every start digit), the third
+    // the upper layers, each without its chains.
     o->verify_pre(I->L, dk, dmd, (uint32_t)count, pad, dv + o_blob, dig, dit, items, thmax, scratch);
-    o->verify_layers(I->L, dit, (uint32_t)count, base.data(), dmax, dig, dv + o_sig, dv + o_auth, thmax, scratch,
-                     roots);
+    if (dmax > 1 && count <= kSpecVerify) {
+        const uint32_t nspec = (uint32_t)(items - base[1]);
+        uint8_t* spec = V.spec.need((size_t)nspec * len * 16 * n);
+        o->verify_layers(I->L, dit, (uint32_t)count, base.data(), dmax, 0, 1, dig, dv + o_sig, dv + o_auth, thmax,
+                         scratch, roots, spec, nspec);
+        o->verify_layers(I->L, dit, (uint32_t)count, base.data(), dmax, 1, dmax, dig, dv + o_sig, dv + o_auth,
+                         thmax, scratch, roots, spec, 0);
+    } else {
+        o->verify_layers(I->L, dit, (uint32_t)count, base.data(), dmax, 0, dmax, dig, dv + o_sig, dv + o_auth,
+                         thmax, scratch, roots, nullptr, 0);
+    }
     I->d2h(h, roots, (size_t)items * n);
     I->sync();
     for (int j = 0; j < count; ++j) {

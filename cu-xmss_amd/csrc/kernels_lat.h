@@ -180,12 +180,15 @@ __device__ __forceinline__ void ltree_lds(typename F::Node* pk, const typename F
     }
 }
 
-// Request r on the calling workgroup; pk: LEN nodes of LDS.
+// Request r on the calling workgroup; pk: LEN nodes of LDS.  LAT_VERIFY
+// items r >= spec_first take their chain ends from spec (16 per chain, by
+// start digit: k_lat_verify_layers' speculative chains) instead of hashing.
 template <class F, uint32_t MODE>
 __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, const DLeaf* __restrict__ reqs,
                                             const uint8_t* __restrict__ msgs, const uint8_t* __restrict__ sigs,
                                             const uint8_t* __restrict__ auth, const uint8_t* __restrict__ scratch,
-                                            size_t stride, uint8_t* __restrict__ out) {
+                                            size_t stride, uint8_t* __restrict__ out,
+                                            const uint8_t* __restrict__ spec = nullptr, uint32_t spec_first = 0) {
     using Lay = LatLayout<F>;
     using FPre = typename F::FPre;
     using HPre = typename F::HPre;
@@ -197,7 +200,12 @@ __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, co
         uint32_t s0 = 0, s1 = 15;
         if (MODE == LAT_VERIFY) {
             s0 = chain_len<F>(msgs + (size_t)r * F::kN, c);
-            F::load(x, sigs + ((size_t)r * F::LEN + c) * F::kN);
+            if (spec && r >= spec_first) {
+                F::load(x, spec + (((size_t)(r - spec_first) * F::LEN + c) * 16 + s0) * F::kN);
+                s0 = 15;  // already at the chain's end
+            } else {
+                F::load(x, sigs + ((size_t)r * F::LEN + c) * F::kN);
+            }
         } else {
             if (MODE == LAT_SIGN) s1 = chain_len<F>(msgs + (size_t)r * F::kN, c);
             x = reinterpret_cast<const typename F::Node*>(S + Lay::starts)[c];
@@ -254,21 +262,46 @@ __global__ __launch_bounds__(192) void k_lat_fin(const DLeaf* __restrict__ reqs,
 struct LayerBase {
     uint32_t b[17];
 };
+//
+// Layers L0 .. L1-1 are walked here (the layers below were walked by an
+// earlier launch and left their roots).  Speculation for few XMSS^MT
+// requests: an upper layer's chain lengths follow the root of the layer
+// below, but the chain from each possible start digit d (16 of them) to the
+// end does not, so blocks past the nwalk walkers hash all of them for the
+// nspec items from spec_first on (one lane per item, chain and digit, into
+// spec) while the walkers verify layer 0; a second launch then walks the
+// upper layers with their chain ends looked up (L-tree and auth path only).
 template <class F>
 __global__ __launch_bounds__(192) void k_lat_verify_layers(const DLeaf* __restrict__ reqs, LayerBase base,
-                                                            uint32_t dmax, const uint8_t* __restrict__ digests,
+                                                            uint32_t L0, uint32_t L1,
+                                                            const uint8_t* __restrict__ digests,
                                                             const uint8_t* __restrict__ sigs,
                                                             const uint8_t* __restrict__ auth,
                                                             const uint8_t* __restrict__ scratch, size_t stride,
-                                                            uint8_t* __restrict__ roots) {
+                                                            uint8_t* __restrict__ roots, uint32_t nwalk,
+                                                            uint8_t* __restrict__ spec, uint32_t spec_first,
+                                                            uint32_t nspec) {
+    using FPre = typename F::FPre;
     __shared__ typename F::Node pk[F::LEN];
+    if (blockIdx.x >= nwalk) {
+        const uint32_t g = (blockIdx.x - nwalk) * blockDim.x + threadIdx.x;
+        if (g >= nspec * F::LEN * 16) return;
+        const uint32_t t = spec_first + g / (F::LEN * 16), c = (g / 16) % F::LEN, d = g % 16;
+        typename F::Node x;
+        F::load(x, sigs + ((size_t)t * F::LEN + c) * F::kN);
+        const FPre* fp = reinterpret_cast<const FPre*>(scratch + (size_t)t * stride + LatLayout<F>::fpre) + c * 15;
+        for (uint32_t s = d; s < 15; ++s) F::fin_f(x, fp[s], x);
+        F::store(x, spec + (size_t)g * F::kN);
+        return;
+    }
     const uint32_t j = blockIdx.x;
-    for (uint32_t L = 0; L < dmax && j < base.b[L + 1] - base.b[L]; ++L) {
+    for (uint32_t L = L0; L < L1 && j < base.b[L + 1] - base.b[L]; ++L) {
         const uint32_t t = base.b[L] + j;
         // the message of item t is at msgs + t * n: the digest, or the root of item base.b[L - 1] + j
         const uint8_t* msgs = L == 0 ? digests - (size_t)base.b[0] * F::kN
                                      : roots - (size_t)(base.b[L] - base.b[L - 1]) * F::kN;
-        lat_fin_one<F, LAT_VERIFY>(t, pk, reqs, msgs, sigs, auth, scratch, stride, roots);
+        lat_fin_one<F, LAT_VERIFY>(t, pk, reqs, msgs, sigs, auth, scratch, stride, roots,
+                                   nspec ? nullptr : spec, spec_first);
         __threadfence_block();
         __syncthreads();
     }

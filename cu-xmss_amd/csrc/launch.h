@@ -124,9 +124,14 @@ struct FamOps {
     // Every verify layer of n requests (request j: items base[L] + j for
     // L < its depth, requests sorted deepest first; dmax <= 16), one
     // workgroup per request; DLeaf p2 = auth offset from auth.
+    // Layers L0 .. L1-1 of n requests (items layer-major from base[]); with
+    // nspec > 0 it also hashes the speculative chain ends of the nspec items
+    // of layers >= 1 into spec, and with nspec = 0 and spec set, those items
+    // take their chain ends from spec.
     virtual void verify_layers(Launcher& L, const DLeaf* reqs, uint32_t n, const uint32_t* base, uint32_t dmax,
-                               const uint8_t* digests, const uint8_t* sigs, const uint8_t* auth, uint32_t th,
-                               const uint8_t* scratch, uint8_t* roots) = 0;
+                               uint32_t L0, uint32_t L1, const uint8_t* digests, const uint8_t* sigs,
+                               const uint8_t* auth, uint32_t th, const uint8_t* scratch, uint8_t* roots,
+                               uint8_t* spec, uint32_t nspec) = 0;
     // LAT_LEAF for many requests: lat_pre, then one lane per chain and one
     // workgroup per L-tree (ends: n x LEN nodes of scratch) -> out[r] = leaf.
     virtual void lat_leaves_dense(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, uint8_t* scratch,
@@ -248,12 +253,15 @@ struct FamOpsT final : FamOps {
               digests, hrows, reqs, scratch, LatLayout<F>::bytes(th));
     }
     void verify_layers(Launcher& L, const DLeaf* reqs, uint32_t n, const uint32_t* base, uint32_t dmax,
-                       const uint8_t* digests, const uint8_t* sigs, const uint8_t* auth, uint32_t th,
-                       const uint8_t* scratch, uint8_t* roots) override {
+                       uint32_t L0, uint32_t L1, const uint8_t* digests, const uint8_t* sigs, const uint8_t* auth,
+                       uint32_t th, const uint8_t* scratch, uint8_t* roots, uint8_t* spec,
+                       uint32_t nspec) override {
         LayerBase lb{};
         for (uint32_t i = 0; i <= dmax && i < 17; ++i) lb.b[i] = base[i];
-        L.run("verify_layers", k_lat_verify_layers<F>, dim3(n), dim3(((F::LEN + 63) / 64) * 64), reqs, lb, dmax,
-              digests, sigs, auth, scratch, LatLayout<F>::bytes(th), roots);
+        const uint32_t threads = ((F::LEN + 63) / 64) * 64;
+        const uint32_t sblocks = cdiv((uint64_t)nspec * F::LEN * 16, threads);
+        L.run("verify_layers", k_lat_verify_layers<F>, dim3(n + sblocks), dim3(threads), reqs, lb, L0, L1, digests,
+              sigs, auth, scratch, LatLayout<F>::bytes(th), roots, n, spec, dmax > 1 ? base[1] : 0u, nspec);
     }
     void lat_leaves_dense(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, uint8_t* scratch,
                           uint8_t* ends, uint8_t* out) override {

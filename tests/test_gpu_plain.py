@@ -92,3 +92,39 @@ def test_seed_keypair_matches_oracle_small_trees():
         _, pk_o, sk_o = o.seed_keypair(oid, 1, seed)
         _, pk_g, sk_g = lib.seed_keypair(oid, 1, seed)
         assert pk_g == pk_o and sk_g == sk_o, hex(oid)
+
+
+@pytest.mark.parametrize("name", ["XMSSMT-SHA2_20/4_256", "XMSSMT-SHA2_60/12_256", "XMSSMT-SHAKE_40/8_256"])
+def test_mt_verify_each_layer_corrupted(name):
+    """One-call XMSS^MT verification (upper layers' chains hashed for every
+    start digit beside layer 0, then looked up): a valid signature verifies;
+    one flipped byte in any layer's WOTS signature or auth path, or in R,
+    is rejected -- each verdict and output equal to the oracle's."""
+    import random
+    o = oracle()
+    lib = product(0)
+    oid = lib.str_to_oid(name)
+    p = lib.params(oid, 1)
+    lib.lib.xmss_mi355x_rng_seed(oid * 7 + 3)
+    rc, pk, sk = lib.keypair(oid, 1)
+    assert rc == 0
+    rng = random.Random(oid)
+    skx = bytearray(sk)
+    idx = rng.randrange(1 << p.full_height)
+    skx[4:4 + p.index_bytes] = idx.to_bytes(p.index_bytes, "big")
+    m = bytes(rng.getrandbits(8) for _ in range(77))
+    rc, sm, _ = lib.sign(bytes(skx), m, 1)
+    assert rc == 0
+    assert lib.sign_open(sm, pk, 1) == (0, m) == o.sign_open(sm, pk, 1)
+    ib, n, wb = p.index_bytes, p.n, p.wots_sig_bytes
+    layer = wb + p.tree_height * n
+    cases = [ib + rng.randrange(n)]  # R
+    for L in range(p.d):
+        cases.append(ib + n + L * layer + rng.randrange(wb))                               # WOTS signature
+        cases.append(ib + n + L * layer + wb + rng.randrange(p.tree_height * n))          # auth path
+    for pos in cases:
+        bad = bytearray(sm)
+        bad[pos] ^= 1 << rng.randrange(8)
+        got = lib.sign_open(bytes(bad), pk, 1)
+        assert got[0] == -1, (name, pos)
+        assert got == o.sign_open(bytes(bad), pk, 1), (name, pos)
