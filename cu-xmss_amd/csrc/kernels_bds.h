@@ -348,18 +348,22 @@ struct BdsDev {
     }
 };
 
-// The hashing of BDS signatures before their state update, in two launches
-// on one stream (so that a signature costs three launches and no
-// cross-stream waits: the host's launch calls, as much as the GPU, bound a
-// single signature's latency):
-//   k_bds_pre: R and H_msg (the first rows, one message per lane: the
-//              longest-running lanes start first), the address-only work of
-//              the planned leaves (LAT_LEAF rows), of the layer-0 WOTS
-//              signatures (LAT_SIGN rows, all 15 steps of every chain, since
-//              the digests are computed in the same launch) and of the
-//              planned tree merges (the last rows, one merge per lane);
-//   k_bds_fin: one workgroup per planned leaf, then one per WOTS signature.
-// Both kinds of requests share one scratch area (rows nleaf .. for the
+// The hashing of BDS signatures, in three launches on one stream (so that a
+// signature costs three launches and no cross-stream waits: the host's
+// launch calls, as much as the GPU, bound a single signature's latency):
+//   k_bds_pre:    the address-only work of the planned leaves (LAT_LEAF
+//                 rows), of the layer-0 WOTS signatures (LAT_SIGN rows, all 15
+//                 steps of every chain) and of the planned tree merges (one
+//                 merge per lane);
+//   k_bds_fin:    one workgroup per planned leaf;
+//   k_bds_update: the state update of each signature.
+// R / H_msg (one message per lane) and the layer-0 WOTS signatures (one
+// workgroup each) go one of two ways.  Short messages: H_msg in the first
+// rows of k_bds_pre and the signatures beside the leaves in k_bds_fin.  Long
+// messages (H_msg a long serial chain of compressions): H_msg beside the
+// leaves in k_bds_fin and the signatures beside the state updates in
+// k_bds_update, so that H_msg overlaps the leaves.
+// The leaves and signatures share one scratch area (rows nleaf .. for the
 // signatures).
 template <class F>
 __global__ __launch_bounds__(WG) void k_bds_pre(const typename F::Key* __restrict__ keys,
@@ -395,17 +399,26 @@ __global__ __launch_bounds__(WG) void k_bds_pre(const typename F::Key* __restric
     }
 }
 
+// Blocks: nleaf leaves, then nsig WOTS signatures (short messages), then
+// ceil(nmsg / blockDim) blocks of H_msg lanes (long messages).
 template <class F>
-__global__ __launch_bounds__(192) void k_bds_fin(const DLeaf* __restrict__ leaves, uint32_t nleaf,
-                                                  const DLeaf* __restrict__ sigs, const uint8_t* __restrict__ digests,
+__global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restrict__ keys,
+                                                  const DMsg* __restrict__ msgs, uint32_t nmsg, uint32_t pad,
+                                                  uint8_t* __restrict__ blob, uint8_t* __restrict__ R_out,
+                                                  uint8_t* __restrict__ digests, const DLeaf* __restrict__ leaves,
+                                                  uint32_t nleaf, const DLeaf* __restrict__ sigs, uint32_t nsig,
                                                   const uint8_t* __restrict__ scratch, size_t stride,
                                                   uint8_t* __restrict__ leaf_out, uint8_t* __restrict__ sig_out) {
     __shared__ typename F::Node pk[F::LEN];
-    if (blockIdx.x < nleaf)
+    if (blockIdx.x < nleaf) {
         lat_fin_one<F, LAT_LEAF>(blockIdx.x, pk, leaves, nullptr, nullptr, nullptr, scratch, stride, leaf_out);
-    else
+    } else if (blockIdx.x < nleaf + nsig) {
         lat_fin_one<F, LAT_SIGN>(blockIdx.x - nleaf, pk, sigs, digests, nullptr, nullptr,
                                  scratch + (size_t)nleaf * stride, stride, sig_out);
+    } else {
+        const uint32_t i = (blockIdx.x - nleaf - nsig) * blockDim.x + threadIdx.x;
+        if (i < nmsg) hmsg_one<F>(keys, msgs, i, pad, blob, R_out, digests);
+    }
 }
 
 // The state part of one BDS signature per workgroup (everything after R, H_msg
@@ -548,7 +561,14 @@ __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __res
                              uint8_t* __restrict__ sk_arena, const uint8_t* __restrict__ leaf_arena,
                              const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
                              const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
-                             uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena) {
+                             uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena, uint32_t njobs,
+                             const DLeaf* __restrict__ sigs, const uint8_t* __restrict__ digests,
+                             const uint8_t* __restrict__ sign_scratch, uint8_t* __restrict__ sig_out) {
+    if (blockIdx.x >= njobs) {  // long messages: the layer-0 WOTS signature of job blockIdx.x - njobs
+        lat_fin_one<F, LAT_SIGN>(blockIdx.x - njobs, nullptr, sigs, digests, nullptr, nullptr, sign_scratch, wstride,
+                                 sig_out);
+        return;
+    }
     const DBds job = jobs[blockIdx.x];
     uint8_t* skg = sk_arena + job.sk_off;
     uint8_t* res = reinterpret_cast<uint8_t*>(job.res);

@@ -164,22 +164,29 @@ struct FamOps {
     // BDS signatures: R and H_msg of the nsign messages, the planned leaves
     // (LAT_LEAF requests) and the layer-0 WOTS signatures (LAT_SIGN requests
     // with LAT_ALL_STEPS) in one pre-image launch, which also does the
-    // planned merges' address-only work, and one finishing launch.  The
-    // first nmsg of the nsign signature requests sign the H_msg digests and
-    // are finished here; the rest (the upper-layer roots the state update
-    // signs) only get their pre-images.  scratch holds (nleaf + nsign) x
-    // lat_bytes(0).
+    // planned merges' address-only work, and one launch finishing the leaves
+    // beside R / H_msg of the nmsg messages.  The nsign signature requests
+    // (the first nmsg sign the digests, the rest are the upper-layer roots
+    // the state update signs) only get their pre-images here; the state
+    // update launch finishes the first nmsg.  scratch holds (nleaf + nsign)
+    // x lat_bytes(0).
+    // long: R / H_msg beside the leaves and the nmsg layer-0 signatures
+    // left to bds_update (else H_msg first and the signatures beside the
+    // leaves, into sig_out).
     virtual void bds_lat(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad, uint8_t* blob,
                          uint8_t* R, uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs,
                          uint32_t nsign, uint8_t* scratch, const DThash* merges, uint32_t nmerge, void* merge_out,
-                         uint8_t* leaf_out, uint8_t* sig_out) = 0;
+                         uint8_t* leaf_out, uint8_t* sig_out, bool long_msgs) = 0;
     // The state updates, one workgroup per job; dag / lits: the planned
     // merges' sources (3 words per merge, parallel to merge_arena) and the
     // literal node values (jobs with maxlev 0 hash their merges one by one).
+    // ... and, with sigs set (long messages), beside them the n layer-0 WOTS
+    // signatures (sigs / digests, pre-images at sign_scratch) into sig_out.
     virtual void bds_update(Launcher& L, const void* keys, size_t lds, const DBds* jobs, uint32_t n,
                             uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena,
                             const uint32_t* dag, const uint8_t* lits, const uint8_t* wpre, uint8_t* tails,
-                            uint8_t* scratch) = 0;
+                            uint8_t* scratch, const DLeaf* sigs, const uint8_t* digests,
+                            const uint8_t* sign_scratch, uint8_t* sig_out) = 0;
 };
 
 template <class F>
@@ -337,22 +344,27 @@ struct FamOpsT final : FamOps {
     void bds_lat(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad, uint8_t* blob,
                  uint8_t* R, uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs, uint32_t nsign,
                  uint8_t* scratch, const DThash* merges, uint32_t nmerge, void* merge_out, uint8_t* leaf_out,
-                 uint8_t* sig_out) override {
+                 uint8_t* sig_out, bool long_msgs) override {
         const size_t stride = LatLayout<F>::bytes(0);
         const uint32_t xb = cdiv(F::LEN * 17 - 1, WG);  // LAT_LEAF / LAT_SIGN items per request
-        const uint32_t hrows = cdiv(nmsg, (uint64_t)xb * WG), mrows = cdiv(nmerge, (uint64_t)xb * WG);
+        const uint32_t mrows = cdiv(nmerge, (uint64_t)xb * WG);
+        const uint32_t hrows = long_msgs ? 0 : cdiv(nmsg, (uint64_t)xb * WG);
+        const uint32_t threads = ((F::LEN + 63) / 64) * 64;
         L.run("bds_pre", k_bds_pre<F>, dim3(xb, hrows + nleaf + nsign + mrows), dim3(WG), (const K*)keys, msgs, pad,
               blob, R, digests, hrows, nmsg, leaves, nleaf, sigs, nsign, scratch, stride, merges, nmerge,
               (typename F::HPre*)merge_out);
-        L.run("bds_fin", k_bds_fin<F>, dim3(nleaf + nmsg), dim3(((F::LEN + 63) / 64) * 64), leaves, nleaf, sigs,
-              digests, (const uint8_t*)scratch, stride, leaf_out, sig_out);
+        const uint32_t nsig = long_msgs ? 0 : nmsg, nlate = long_msgs ? nmsg : 0;
+        L.run("bds_fin", k_bds_fin<F>, dim3(nleaf + nsig + cdiv(nlate, threads)), dim3(threads), (const K*)keys,
+              msgs, nlate, pad, blob, R, digests, leaves, nleaf, sigs, nsig, (const uint8_t*)scratch, stride,
+              leaf_out, sig_out);
     }
     void bds_update(Launcher& L, const void* keys, size_t lds, const DBds* jobs, uint32_t n,
                     uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena, const uint32_t* dag,
-                    const uint8_t* lits, const uint8_t* wpre, uint8_t* tails, uint8_t* scratch) override {
-        L.run_lds("bds_update", k_bds_update<F>, dim3(n), dim3(((F::LEN + 63) / 64) * 64), lds, (const K*)keys,
+                    const uint8_t* lits, const uint8_t* wpre, uint8_t* tails, uint8_t* scratch, const DLeaf* sigs,
+                    const uint8_t* digests, const uint8_t* sign_scratch, uint8_t* sig_out) override {
+        L.run_lds("bds_update", k_bds_update<F>, dim3(sigs ? 2 * n : n), dim3(((F::LEN + 63) / 64) * 64), lds, (const K*)keys,
                   jobs, sk_arena, leaf_arena, (const typename F::HPre*)merge_arena, dag, lits, wpre,
-                  LatLayout<F>::bytes(0), tails, scratch);
+                  LatLayout<F>::bytes(0), tails, scratch, n, sigs, digests, sign_scratch, sig_out);
     }
 };
 
