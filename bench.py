@@ -512,8 +512,9 @@ def bench_serve(rate, seconds, nkeys, budget):
             "device_calls": calls.value, "launch_groups": launches.value, "key_setup_s": round(keygen_s, 2)}
 
 
-def bench_latency(nsigns):
-    """Sequential BDS signatures of one XMSS-SHA2_16_256 key (fast core)."""
+def bench_latency(nsigns, fixed_len=None):
+    """Sequential BDS signatures of one XMSS-SHA2_16_256 key (fast core),
+    messages log-uniform 32 B - 4 KB (or all fixed_len bytes)."""
     api = load_product(1)
     p = api.params(0x02, 0)
     rc, pk, sk = api.keypair(0x02, 0, sk_init=bytes(4 + p.sk_bytes))
@@ -522,7 +523,8 @@ def bench_latency(nsigns):
     lat = []
     last = None
     for _ in range(nsigns):
-        m = bytes(rng.getrandbits(8) for _ in range(32))
+        mlen = fixed_len if fixed_len else int(32 * 128 ** rng.random())
+        m = bytes(rng.getrandbits(8) for _ in range(mlen))
         t0 = time.perf_counter()
         rc, sm, sk = api.sign(sk, m, 0)
         lat.append(time.perf_counter() - t0)
@@ -708,6 +710,7 @@ def main():
         serve = bench_serve(args.serve_rate, args.serve_seconds, 256, args.serve_budget)
     log("latency (BDS) x%d ..." % args.latency_signs)
     p50, p99, v50 = bench_latency(args.latency_signs) if args.latency_signs else (None, None, None)
+    p50s, p99s, _ = bench_latency(args.latency_signs, 32) if args.latency_signs else (None, None, None)
     cpu = None if args.no_cpu else cpu_baseline()
 
     out = {
@@ -724,7 +727,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic (seeded keys and messages)",
         "config": {"workload": "XMSS-SHA2_16_256 keygen (65,536 leaves); batch 4096 x XMSS-SHA2_10_256 "
-                               "plain sign + verify; 1024 sequential BDS signs",
+                               "plain sign + verify; 1024 sequential BDS signs (messages log-uniform 32 B - 4 KB)",
                    "parallelism": "single GPU"},
         "keygen_nominal_frac_of_spec": round(keygen_frac, 4),
         "keygens_per_s_batch4096": rate(bkeygen_t),
@@ -736,6 +739,8 @@ def main():
         "p50_sign_ms": None if p50 is None else round(1000 * p50, 3),
         "p99_sign_ms": None if p99 is None else round(1000 * p99, 3),
         "p50_verify_ms": None if v50 is None else round(1000 * v50, 3),
+        "bds_sign_32B_messages_ms": None if p50s is None else {"p50": round(1000 * p50s, 3),
+                                                                "p99": round(1000 * p99s, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_keygen_ms": {k: round(v[1], 4) for k, v in kst.items()},
