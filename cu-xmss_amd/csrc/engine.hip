@@ -586,6 +586,7 @@ struct Engine::Batch {
     std::vector<int> perm;    // verify: staged position -> the caller's index (longest message first)
     DevBuf lat;               // pre-image scratch of the latency kernels
     DevBuf lreq;              // sign: request descriptors of the latency kernels
+    DevBuf wlat, wreq;        // sign: the same for the WOTS signatures (layer-major)
     hipGraphExec_t graph = nullptr;  // verify: the captured launches of run()
     int nrun = 0;                    // verify: runs since staging
     PinnedBuf pin;                   // host staging of this batch's inputs and results
@@ -1072,6 +1073,21 @@ void Engine::run(Batch* b, bool wait_done) {
     const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
     const int count = b->count;
     const void* dk = b->keys.p;
+    // WOTS signatures of layer L (on the current stream): few requests with
+    // the latency kernels (address-only work wide, then one compression per
+    // chain step; scratch per layer), else one lane per chain (k_wots)
+    const bool lat_wots = count <= kLatWots && use_lat(o, count, 0);
+    DLeaf* dw = nullptr;
+    uint8_t* wlat = lat_wots ? b->wlat.need((size_t)d * count * o->lat_bytes(0)) : nullptr;
+    auto sign_layer = [&](unsigned L, const uint8_t* msg_l) {
+        if (lat_wots)
+            o->lat(I->L, dk, LAT_SIGN, dw + (size_t)L * count, (uint32_t)count, msg_l, nullptr, nullptr, 0,
+                   wlat + (size_t)L * count * o->lat_bytes(0), b->wots + (size_t)L * count * len * n);
+        else
+            o->wots(I->L, dk, b->wd + (size_t)L * count, (uint32_t)count, msg_l, nullptr,
+                    b->wots + (size_t)L * count * len * n);
+    };
+    hipEvent_t wdone = nullptr;
     {
         // all subtrees in one pipeline (few: levels split as in build_trees);
         // R / H_msg (needed only by the WOTS signatures) and the levels'
@@ -1080,12 +1096,22 @@ void Engine::run(Batch* b, bool wait_done) {
         const bool split = nruns <= kSplitRuns;
         void* tpre = split ? I->tpre.need((size_t)nruns * ((1ull << th) - 1) * o->hpre_bytes()) : nullptr;
         hipEvent_t tstaged = I->L.ev(), tdone = I->L.ev();
+        wdone = I->L.ev();
         HIPCK(hipEventRecord(tstaged, I->L.s_main));
         HIPCK(hipStreamWaitEvent(I->L.s_aux, tstaged, 0));
         I->L.s = I->L.s_aux;
         o->hmsg(I->L, dk, b->md, (uint32_t)count, fam_pad(b->fam), b->blob, b->R, (uint8_t*)b->dig.p);
         if (split) o->pre_tree(I->L, dk, b->runs, nruns, th, tpre);
+        if (lat_wots) {
+            dw = b->wreq.need<DLeaf>((size_t)d * count * sizeof(DLeaf));
+            I->L.run("wots_to_leaves", k_wots_to_leaves, dim3(cdiv((uint64_t)d * count, WG)), dim3(WG), b->wd,
+                     (uint32_t)(d * count), dw);
+        }
         HIPCK(hipEventRecord(tdone, I->L.s_aux));
+        // the layer-0 WOTS signatures sign the digests, not a subtree root:
+        // on the aux stream beside the leaves
+        sign_layer(0, (const uint8_t*)b->dig.p);
+        HIPCK(hipEventRecord(wdone, I->L.s_aux));
         I->L.s = I->L.s_main;
         const uint64_t leaves = 1ull << th;
         const uint64_t per_leaf = (uint64_t)len * n;
@@ -1147,38 +1173,20 @@ void Engine::run(Batch* b, bool wait_done) {
         const uint32_t ng = (uint32_t)((size_t)count * d * (th + 1));
         I->L.run("gather", k_gather, dim3(cdiv((uint64_t)ng * (n / 4), WG)), dim3(WG), (const uint8_t*)nodes,
                  b->gidx, ng, n, b->auth);
-        // WOTS signatures layer by layer; layer L signs the root of layer L-1
-        // (the digest for layer 0).  Roots sit at the end of each auth record.
+        // WOTS signatures of the upper layers: layer L signs the root of layer
+        // L-1.  Roots sit at the end of each auth record.
         uint8_t* msgs = (uint8_t*)b->roots.p;
         const size_t rec = (size_t)(th + 1) * n;
-        // few requests: the latency kernels (address-only work wide, then one
-        // compression per chain step), else one lane per chain (k_wots)
-        const bool lat_wots = count <= kLatWots && use_lat(o, count, 0);
-        DLeaf* dw = nullptr;
-        if (lat_wots) {
-            dw = b->lreq.need<DLeaf>((size_t)d * count * sizeof(DLeaf));
-            I->L.run("wots_to_leaves", k_wots_to_leaves, dim3(cdiv((uint64_t)d * count, WG)), dim3(WG), b->wd,
-                     (uint32_t)(d * count), dw);
-        }
-        for (unsigned L = 0; L < d; ++L) {
-            const uint8_t* msg_l;
-            if (L == 0) {
-                msg_l = (const uint8_t*)b->dig.p;
-            } else {
-                // compact roots of layer L-1 into msgs[L-1]
-                uint8_t* dst = msgs + (size_t)(L - 1) * count * n;
-                HIPCK(hipMemcpy2DAsync(dst, n, b->auth + ((size_t)(L - 1) * count) * rec + th * n,
-                                       rec, n, count, hipMemcpyDeviceToDevice, I->L.s));
-                msg_l = dst;
-            }
-            if (lat_wots)
-                o->lat(I->L, dk, LAT_SIGN, dw + (size_t)L * count, (uint32_t)count, msg_l, nullptr, nullptr, 0,
-                       b->lat.need((size_t)count * o->lat_bytes(0)), b->wots + (size_t)L * count * len * n);
-            else
-                o->wots(I->L, dk, b->wd + (size_t)L * count, (uint32_t)count, msg_l, nullptr,
-                        b->wots + (size_t)L * count * len * n);
+        for (unsigned L = 1; L < d; ++L) {
+            // compact roots of layer L-1 into msgs[L-1]
+            uint8_t* dst = msgs + (size_t)(L - 1) * count * n;
+            HIPCK(hipMemcpy2DAsync(dst, n, b->auth + ((size_t)(L - 1) * count) * rec + th * n, rec, n, count,
+                                   hipMemcpyDeviceToDevice, I->L.s));
+            sign_layer(L, dst);
         }
     }
+    HIPCK(hipStreamWaitEvent(I->L.s_main, wdone, 0));  // the layer-0 signatures
+    I->L.pool.push_back(wdone);
     if (wait_done) I->sync();
 }
 
@@ -1237,7 +1245,8 @@ void Engine::free_batch(Batch* b) {
     if (b->done) HIPCK(hipEventDestroy(b->done));
     if (b->staged) HIPCK(hipEventDestroy(b->staged));
     if (b->pin.p) HIPCK(hipHostFree(b->pin.p));
-    DevBuf* bufs[] = {&b->io, &b->res, &b->keys, &b->dig, &b->nodes, &b->ends, &b->tmp, &b->roots, &b->lat, &b->lreq};
+    DevBuf* bufs[] = {&b->io,    &b->res, &b->keys, &b->dig,  &b->nodes, &b->ends,
+                      &b->tmp,   &b->roots, &b->lat, &b->lreq, &b->wlat,  &b->wreq};
     for (DevBuf* x : bufs) {
         if (x->p) HIPCK(hipFree(x->p));
         x->p = nullptr;
