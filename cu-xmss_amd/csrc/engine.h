@@ -291,8 +291,8 @@ public:
     // launch() queues a batch's work and returns; wait_batch() blocks until
     // that work is done (no lock is needed for it).  A launched batch's
     // refill, collect and free wait for it by themselves; its staging and
-    // result copies run on a copy stream, so a verification batch can be
-    // staged and another collected while it runs.
+    // result copies run on streams of their own (inputs, results), so a
+    // verification batch can be staged and another collected while it runs.
     void launch(Batch* b);
     void wait_batch(Batch* b);
     // sign: per item R (n) || d x (WOTS sig || auth path) written to out[i]

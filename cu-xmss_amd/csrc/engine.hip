@@ -139,9 +139,12 @@ struct Engine::Impl {
     const void* kp() const { return kp_override ? kp_override : keys.p; }
     Engine::Batch* spare[2] = {nullptr, nullptr};  // reusable verify / sign batches
     DevBuf keys, seeds, desc, ends, tmp, nodes, a, b, c, d, idx, lat, tpre;
-    // copies of launched batches: a batch's inputs go in and its results come
-    // out here while other batches' kernels run on s_main / s_aux
+    // input copies of verification batches: a batch's inputs go in here while
+    // other batches' kernels run on s_main / s_aux
     hipStream_t s_copy = nullptr;
+    // result copies of launched batches: apart from s_copy, so that a
+    // collect does not wait behind the next batch's input copy
+    hipStream_t s_back = nullptr;
     std::unordered_map<std::string, int> slots[F_COUNT];  // pool slot by seed bytes (pool_slot)
     int key_fam = -1;
     int nkeys = 0;
@@ -166,6 +169,7 @@ Engine::Engine() : impl_(new Impl) {
     HIPCK(hipStreamCreateWithFlags(&impl_->L.s_main, hipStreamNonBlocking));
     HIPCK(hipStreamCreateWithFlags(&impl_->L.s_aux, hipStreamNonBlocking));
     HIPCK(hipStreamCreateWithFlags(&impl_->s_copy, hipStreamNonBlocking));
+    HIPCK(hipStreamCreateWithFlags(&impl_->s_back, hipStreamNonBlocking));
     impl_->L.s = impl_->L.s_main;
     stream_ = impl_->L.s;
 }
@@ -1191,13 +1195,14 @@ void Engine::run(Batch* b, bool wait_done) {
 }
 
 // Results of a batch run with run() leave on its own stream, ordered after
-// its kernels (one wait); those of a launched batch on the copy stream once
-// it is done, so that batches launched after it keep the compute streams.
+// its kernels (one wait); those of a launched batch on the result stream
+// once it is done, so that batches launched after it keep the compute
+// streams and the next batch's input copy does not delay them.
 static void fetch_results(Engine::Impl* I, Engine::Batch* b, void* h, const void* src, size_t bytes) {
     if (b->evented) {
         wait_batch(b);
-        HIPCK(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, I->s_copy));
-        HIPCK(hipStreamSynchronize(I->s_copy));
+        HIPCK(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, I->s_back));
+        HIPCK(hipStreamSynchronize(I->s_back));
     } else {
         I->d2h(h, src, bytes);
         I->sync();
