@@ -566,6 +566,41 @@ def cpu_baseline():
             "sample": "%d XMSS-SHA2_10_256 keygens (%.2f s each) on 1 core, x%.2f compressions" % (n, dt, scale)}
 
 
+def cpu_sign_worker():
+    """One plain-core XMSS-SHA2_10_256 signature by the reference C code
+    (oracle/_ref) at a random leaf, in a process of its own: prints seconds.
+    The sk's root field is arbitrary bytes: signing time does not depend on
+    it (the signature rebuilds the whole tree either way)."""
+    ref = load_ref(0)
+    p = ref.params(0x01, 0)
+    rng = random.Random(os.getpid())
+    sk = (1).to_bytes(4, "big") + rng.randrange(1023).to_bytes(4, "big") + \
+        bytes(rng.getrandbits(8) for _ in range(4 * p.n))
+    m = bytes(rng.getrandbits(8) for _ in range(int(32 * 128 ** rng.random())))
+    t0 = time.perf_counter()
+    rc, _, _ = ref.sign(sk, m, 0)
+    assert rc == 0
+    print("%.6f" % (time.perf_counter() - t0), flush=True)
+
+
+def cpu_batch_baseline(cores):
+    """The batch-sign workload (XMSS-SHA2_10_256 plain signatures of
+    independent keys) on the reference C code across `cores` host processes,
+    one signature each, started as child processes (no GPU use in them)."""
+    import subprocess
+    if load_ref(0) is None:
+        return None
+    t0 = time.perf_counter()
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-sign-worker"],
+                              stdout=subprocess.PIPE, text=True) for _ in range(cores)]
+    each = [float(pr.communicate()[0].strip().splitlines()[-1]) for pr in procs]
+    wall = time.perf_counter() - t0
+    return {"value": round(cores / wall, 3), "unit": "XMSS-SHA2_10_256 plain signs/s", "cores": cores,
+            "kind": "reference",
+            "sample": "%d signatures, one per process (%.2f s each on average), %.2f s wall" % (
+                cores, sum(each) / len(each), wall)}
+
+
 def copy_calibration(nbytes):
     """One plain device-to-device copy of nbytes (reads nbytes, writes nbytes):
     the reference point for HBM byte counters."""
@@ -627,6 +662,7 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--latency-signs", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sign-worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--opk", type=int, default=1023,
                     help="consecutive signatures of one XMSS-SHA2_10_256 key through sign_many (0: skip)")
     ap.add_argument("--single", type=int, default=256,
@@ -638,6 +674,9 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="MiB of one device-to-device copy, run before the keygens (HBM counter calibration)")
     args = ap.parse_args()
+    if args.cpu_sign_worker:
+        cpu_sign_worker()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: keys are independent and shard by key id with no "
               "communication, so no multi-GPU run is defined here (use --gpus 1).", file=sys.stderr)
@@ -712,6 +751,7 @@ def main():
     p50, p99, v50 = bench_latency(args.latency_signs) if args.latency_signs else (None, None, None)
     p50s, p99s, _ = bench_latency(args.latency_signs, 32) if args.latency_signs else (None, None, None)
     cpu = None if args.no_cpu else cpu_baseline()
+    cpu_batch = None if args.no_cpu else cpu_batch_baseline(min(16, os.cpu_count() or 1))
 
     out = {
         "metric": METRIC,
@@ -743,6 +783,7 @@ def main():
                                                                 "p99": round(1000 * p99s, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_baseline_batch_sign": cpu_batch,
         "kernels_keygen_ms": {k: round(v[1], 4) for k, v in kst.items()},
         "kernels_batch_sign_ms": {k: round(v[1], 3) for k, v in bst.items()},
         "batch_last_index_rejected": n_last,
