@@ -290,8 +290,8 @@ public:
     void wait();
     // launch() queues a batch's work and returns; wait_batch() blocks until
     // that work is done (no lock is needed for it).  A launched batch's
-    // refill, collect and free wait for it by themselves; its staging and
-    // result copies run on streams of their own (inputs, results), so a
+    // refill, collect and free wait for it by themselves; its input copy runs
+    // on a copy stream and its result copy on the aux stream, so a
     // verification batch can be staged and another collected while it runs.
     void launch(Batch* b);
     void wait_batch(Batch* b);

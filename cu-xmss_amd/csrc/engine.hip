@@ -142,14 +142,6 @@ struct Engine::Impl {
     // input copies of verification batches: a batch's inputs go in here while
     // other batches' kernels run on s_main / s_aux
     hipStream_t s_copy = nullptr;
-    // result copies of launched batches: apart from s_copy, so that a
-    // collect does not wait behind the next batch's input copy.  Created on
-    // first use: HIP maps a process's streams onto a few hardware queues
-    // (four by default) in creation order, and with three streams per engine
-    // the main streams of engines made one after another (the serving lanes)
-    // land on different queues, so their kernels do not serialize (measured:
-    // a fourth stream in every engine raised serving sign p99 0.55 -> 1.44 ms).
-    hipStream_t s_back = nullptr;
     std::unordered_map<std::string, int> slots[F_COUNT];  // pool slot by seed bytes (pool_slot)
     int key_fam = -1;
     int nkeys = 0;
@@ -1199,15 +1191,19 @@ void Engine::run(Batch* b, bool wait_done) {
 }
 
 // Results of a batch run with run() leave on its own stream, ordered after
-// its kernels (one wait); those of a launched batch on the result stream
-// once it is done, so that batches launched after it keep the compute
-// streams and the next batch's input copy does not delay them.
+// its kernels (one wait); those of a launched batch, once it is done, on the
+// aux stream: batches launched after it replay their graphs from the main
+// stream, and the next batch's input copy on the copy stream does not delay
+// them.  (No fourth stream: HIP maps a process's streams onto a few hardware
+// queues -- four by default -- in creation order, and with three streams per
+// engine the main streams of engines made one after another, such as the
+// serving lanes, land on different queues; a fourth stream in every engine
+// put them all on one and serialized their kernels.)
 static void fetch_results(Engine::Impl* I, Engine::Batch* b, void* h, const void* src, size_t bytes) {
     if (b->evented) {
         wait_batch(b);
-        if (!I->s_back) HIPCK(hipStreamCreateWithFlags(&I->s_back, hipStreamNonBlocking));
-        HIPCK(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, I->s_back));
-        HIPCK(hipStreamSynchronize(I->s_back));
+        HIPCK(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, I->L.s_aux));
+        HIPCK(hipStreamSynchronize(I->L.s_aux));
     } else {
         I->d2h(h, src, bytes);
         I->sync();

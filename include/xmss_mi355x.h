@@ -83,7 +83,7 @@ int xmss_mi355x_job_run(xmss_mi355x_job *job);
 /* job_launch queues a job's device work and returns at once; job_wait
  * blocks until it is done.  Refill, collect and free of a launched job wait
  * for it by themselves.  A verification job's input copy (prepare, refill)
- * and a launched job's result copy use copy streams of their own, and
+ * and a launched job's result copy run apart from the launched work, and
  * refill returns once the inputs are in pinned host memory, so with two jobs
  * a stream of batches overlaps host staging and the PCIe copies with the GPU
  * work:  launch(A); loop { refill(B); collect(A); launch(B); swap(A, B); }
