@@ -316,6 +316,7 @@ int xmss_mi355x_job_run(xmss_mi355x_job* j) {
 
 int xmss_mi355x_sign_collect(xmss_mi355x_job* j, unsigned char* const* sms, unsigned long long* smlens,
                              int* rets) {
+    if (!j || !j->sign) return -1;
     {
         std::lock_guard<std::mutex> lk(engine_mutex());
         for (JobSet& s : j->sets) {
@@ -341,6 +342,7 @@ int xmss_mi355x_sign_collect(xmss_mi355x_job* j, unsigned char* const* sms, unsi
 
 int xmss_mi355x_verify_collect(xmss_mi355x_job* j, unsigned char* const* ms, unsigned long long* mlens,
                                int* rets) {
+    if (!j || j->sign) return -1;
     for (size_t i = 0; i < j->count; ++i) {
         rets[i] = j->rets[i];
         if (rets[i] == -1) mlens[i] = 0;
@@ -378,6 +380,7 @@ int xmss_mi355x_verify_collect(xmss_mi355x_job* j, unsigned char* const* ms, uns
 }
 
 void xmss_mi355x_job_free(xmss_mi355x_job* j) {
+    if (!j) return;
     {
         std::lock_guard<std::mutex> lk(engine_mutex());
         for (JobSet& s : j->sets) {

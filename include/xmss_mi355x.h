@@ -92,6 +92,8 @@ int xmss_mi355x_job_run(xmss_mi355x_job *job);
  * stream.)  Each returns 0, or -1 for a null job. */
 int xmss_mi355x_job_launch(xmss_mi355x_job *job);
 int xmss_mi355x_job_wait(xmss_mi355x_job *job);
+/* Collect returns 0, or -1 for a null job or one of the other kind (sign /
+ * verify); free ignores a null job. */
 int xmss_mi355x_sign_collect(xmss_mi355x_job *job, unsigned char *const *sms,
                              unsigned long long *smlens, int *rets);
 /* ms[i] must hold smlens[i] bytes, as for xmss_sign_open. */

@@ -48,3 +48,20 @@ def test_no_cpu_fallback_without_gpu():
         pytest.skip("a GPU is present")
     pk, sk = buf(100), buf(200)
     assert so.xmss_keypair(pk, sk, 1) == -3
+
+
+def test_job_calls_reject_null_and_wrong_kind():
+    """The job API's null-job and wrong-kind checks run before any device
+    work, so they hold without a GPU."""
+    import ctypes as C
+    import os
+    from ffi import PRODUCT_DIR
+    lib = C.CDLL(os.path.join(PRODUCT_DIR, "libxmss_mi355x.so"))
+    for nm in ("xmss_mi355x_job_launch", "xmss_mi355x_job_wait", "xmss_mi355x_job_run"):
+        getattr(lib, nm).argtypes = [C.c_void_p]
+        assert getattr(lib, nm)(None) == -1, nm
+    for nm in ("xmss_mi355x_sign_collect", "xmss_mi355x_verify_collect", "xmss_mi355x_verify_refill"):
+        getattr(lib, nm).argtypes = [C.c_void_p] * 4
+        assert getattr(lib, nm)(None, None, None, None) == -1, nm
+    lib.xmss_mi355x_job_free.argtypes = [C.c_void_p]
+    lib.xmss_mi355x_job_free(None)
