@@ -555,10 +555,10 @@ def cpu_baseline():
     else:
         run = lambda: ref.seed_keypair(0x01, 0, bytes(range(96)))
     n, t0 = 0, time.perf_counter()
-    while n < 2 or time.perf_counter() - t0 < 8.0:
+    while n < 2 or time.perf_counter() - t0 < 12.0:  # about 10 s of CPU work
         run()
         n += 1
-        if n >= 6:
+        if n >= 10:
             break
     dt = (time.perf_counter() - t0) / n
     scale = 298582010 / 4665338
