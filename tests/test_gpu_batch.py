@@ -4,7 +4,7 @@ import ctypes as C
 
 import pytest
 
-from common import golden, product
+from common import golden, oracle, product
 from ffi import buf, raw
 
 pytestmark = pytest.mark.gpu
@@ -487,3 +487,77 @@ def test_sign_job_launched_beside_a_verify_job():
     assert got_sm == want_sm and got_sk == want_sk
     L.xmss_mi355x_job_free(sjob)
     L.xmss_mi355x_job_free(vjob)
+
+
+def test_empty_and_all_rejected_batches():
+    """Batches that reach no device work: zero inputs (every batch entry point
+    returns 0), and a batch whose keys are all exhausted or of an unknown OID
+    (each ret and sk as xmss_sign leaves them: -2 with the key wiped, -1 with
+    the sk untouched)."""
+    lib = product(0)
+    o = oracle()
+    L = lib.lib
+    vp = C.c_void_p
+    L.xmss_batch_sign.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, C.c_size_t]
+    L.xmss_batch_sign_open.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, C.c_size_t]
+    L.xmss_batch_keypair.argtypes = [C.c_int, vp, vp, vp, C.c_uint32, C.c_size_t]
+    L.xmss_mi355x_sign_many.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_size_t, vp]
+    for nm in ("xmss_mi355x_sign_prepare", "xmss_mi355x_verify_prepare"):
+        getattr(L, nm).restype = vp
+        getattr(L, nm).argtypes = [C.c_int, vp, vp, vp, C.c_size_t]
+    for nm in ("xmss_mi355x_mixed_sign_prepare", "xmss_mi355x_mixed_verify_prepare"):
+        getattr(L, nm).restype = vp
+        getattr(L, nm).argtypes = [vp, vp, vp, vp, C.c_size_t]
+    L.xmss_mi355x_job_run.argtypes = [vp]
+    L.xmss_mi355x_sign_collect.argtypes = [vp, vp, vp, vp]
+    L.xmss_mi355x_verify_collect.argtypes = [vp, vp, vp, vp]
+    L.xmss_mi355x_job_free.argtypes = [vp]
+    e = _arr([])
+    z = (C.c_ulonglong * 1)()
+    r = (C.c_int * 1)()
+    assert L.xmss_batch_sign(0, e, e, z, e, z, r, 0) == 0
+    assert L.xmss_batch_sign_open(1, e, z, e, z, e, r, 0) == 0
+    assert L.xmss_batch_keypair(0, e, e, None, 0x00000001, 0) == 0
+    sk1 = buf((1).to_bytes(4, "big") + bytes(132))
+    assert L.xmss_mi355x_sign_many(0, sk1, e, z, e, z, 0, r) == 0
+    for prep, collect in (("xmss_mi355x_sign_prepare", "xmss_mi355x_sign_collect"),
+                          ("xmss_mi355x_verify_prepare", "xmss_mi355x_verify_collect")):
+        job = getattr(L, prep)(0, e, z, e, 0)
+        assert job
+        assert L.xmss_mi355x_job_run(job) == 0
+        assert getattr(L, collect)(job, e, z, r) == 0
+        L.xmss_mi355x_job_free(job)
+    mts = (C.c_int * 1)()
+    job = L.xmss_mi355x_mixed_verify_prepare(mts, e, z, e, 0)
+    assert job and L.xmss_mi355x_job_run(job) == 0
+    L.xmss_mi355x_job_free(job)
+
+    # keys past their last index (XMSS-SHA2_10_256: 1024, 2^32 - 1) and an unknown OID
+    sk = (1).to_bytes(4, "big") + bytes((37 * i + 11) & 255 for i in range(132))
+    cases = []
+    for idx in (1024, 0xFFFFFFFF):
+        s = bytearray(sk)
+        s[4:8] = idx.to_bytes(4, "big")
+        cases.append(bytes(s))
+    bad = bytearray(sk)
+    bad[0:4] = (0x7FFF).to_bytes(4, "big")
+    cases.append(bytes(bad))
+    k = len(cases)
+    sks = [buf(s) for s in cases]
+    msgs = [bytes([i]) * 32 for i in range(k)]
+    mbufs = [buf(m) for m in msgs]
+    sms = [buf(2500 + 32) for _ in range(k)]
+    smlens = (C.c_ulonglong * k)()
+    mlens = (C.c_ulonglong * k)(*[32] * k)
+    rets = (C.c_int * k)()
+    mts = (C.c_int * k)()
+    job = L.xmss_mi355x_mixed_sign_prepare(mts, _arr(sks), _arr(mbufs), mlens, k)
+    assert job and L.xmss_mi355x_job_run(job) == 0
+    assert L.xmss_mi355x_sign_collect(job, _arr(sms), smlens, rets) == 0
+    L.xmss_mi355x_job_free(job)
+    for i in range(2):
+        rc, _, sko = o.sign(cases[i], msgs[i], 0, 0)
+        assert rc == -2 and rets[i] == -2
+        assert raw(sks[i], len(cases[i])) == sko
+    assert rets[2] == -1
+    assert raw(sks[2], len(cases[2])) == cases[2]
