@@ -670,90 +670,114 @@ unsigned long long xmss_xmssmt_core_sk_bytes(const xmss_params* p) {
 // SK_SEED = SK_PRF = 0x02.., PUB_SEED = 0x04.. (:678-687).
 int xmss_core_keypair(const xmss_params* params, unsigned char* pk, unsigned char* sk) {
     const xmss_params* p = params;
-    std::lock_guard<std::mutex> lk(engine_mutex());
-    Engine* e;
-    int fam;
-    if (int rc = prepare_engine(p, &e, &fam)) return rc;
-    std::vector<TreehashInst> ths(p->tree_height - p->bds_k + 1);
-    BdsState st;
-    load_states(p, &st, ths.data(), nullptr, sk);
-    st.stackoffset = 0;
-    st.next_leaf = 0;
-    sk[0] = sk[1] = sk[2] = sk[3] = 0;
-    memset(sk + p->index_bytes, 2, 2 * p->n);
-    memset(sk + p->index_bytes + 3 * p->n, 4, p->n);
-    memcpy(pk + p->n, sk + p->index_bytes + 3 * p->n, p->n);
-    KeySeeds ks{sk + p->index_bytes, sk + p->index_bytes + p->n, sk + p->index_bytes + 3 * p->n};
-    e->set_keys(fam, 1, &ks);
-    uint32_t layer = 0;
-    BdsState* sp = &st;
-    treehash_init_many(p, e, fam, 1, &layer, &sp, pk);
-    memcpy(sk + p->index_bytes + 2 * p->n, pk, p->n);
-    store_states(p, sk, &st);
-    return 0;
+    if (fam_of(p->func, p->n) < 0) return -1;
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        Engine* e;
+        int fam;
+        if (int rc = prepare_engine(p, &e, &fam)) return rc;
+        std::vector<TreehashInst> ths(p->tree_height - p->bds_k + 1);
+        BdsState st;
+        load_states(p, &st, ths.data(), nullptr, sk);
+        st.stackoffset = 0;
+        st.next_leaf = 0;
+        sk[0] = sk[1] = sk[2] = sk[3] = 0;
+        memset(sk + p->index_bytes, 2, 2 * p->n);
+        memset(sk + p->index_bytes + 3 * p->n, 4, p->n);
+        memcpy(pk + p->n, sk + p->index_bytes + 3 * p->n, p->n);
+        KeySeeds ks{sk + p->index_bytes, sk + p->index_bytes + p->n, sk + p->index_bytes + 3 * p->n};
+        e->set_keys(fam, 1, &ks);
+        uint32_t layer = 0;
+        BdsState* sp = &st;
+        treehash_init_many(p, e, fam, 1, &layer, &sp, pk);
+        memcpy(sk + p->index_bytes + 2 * p->n, pk, p->n);
+        store_states(p, sk, &st);
+        e->scrub_keys();
+        return 0;
+    });
 }
 
 // xmss_core_fast.c:858-908
 int xmssmt_core_keypair(const xmss_params* params, unsigned char* pk, unsigned char* sk) {
     const xmss_params* p = params;
-    const unsigned n = p->n, d = p->d;
-    std::lock_guard<std::mutex> lk(engine_mutex());
-    Engine* e;
-    int fam;
-    if (int rc = prepare_engine(p, &e, &fam)) return rc;
-    const unsigned nth = p->tree_height - p->bds_k;
-    std::vector<BdsState> st(2 * d - 1);
-    std::vector<TreehashInst> ths((2 * d - 1) * nth + 1);
-    unsigned char* wots_sigs;
-    load_states(p, st.data(), ths.data(), &wots_sigs, sk);
-    for (unsigned i = 0; i < 2 * d - 1; ++i) {
-        st[i].stackoffset = 0;
-        st[i].next_leaf = 0;
-    }
-    memset(sk, 0, p->index_bytes);
-    randombytes(sk + p->index_bytes, 2 * n);
-    randombytes(sk + p->index_bytes + 3 * n, n);
-    memcpy(pk + n, sk + p->index_bytes + 3 * n, n);
-    KeySeeds ks{sk + p->index_bytes, sk + p->index_bytes + n, sk + p->index_bytes + 3 * n};
-    e->set_keys(fam, 1, &ks);
-    // the d layer-0..d-1 trees (tree address 0) are independent: one build
-    std::vector<uint32_t> layers(d);
-    std::vector<BdsState*> sps(d);
-    for (unsigned i = 0; i < d; ++i) {
-        layers[i] = i;
-        sps[i] = &st[i];
-    }
-    std::vector<unsigned char> roots((size_t)d * n);
-    treehash_init_many(p, e, fam, d, layers.data(), sps.data(), roots.data());
-    // layer i+1 signs the root of layer i (leaf 0 of tree 0)
-    if (d > 1) {
-        std::vector<WotsReq> w(d - 1);
-        for (unsigned i = 0; i + 1 < d; ++i) w[i] = {0, i + 1, 0, 0, WOTS_SIGN};
-        e->wots(fam, (int)(d - 1), w.data(), roots.data(), nullptr, wots_sigs);
-    }
-    memcpy(pk, &roots[(size_t)(d - 1) * n], n);
-    memcpy(sk + p->index_bytes + 2 * n, pk, n);
-    store_states(p, sk, st.data());
-    return 0;
+    if (fam_of(p->func, p->n) < 0) return -1;
+    return guarded([&] {
+        const unsigned n = p->n, d = p->d;
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        Engine* e;
+        int fam;
+        if (int rc = prepare_engine(p, &e, &fam)) return rc;
+        const unsigned nth = p->tree_height - p->bds_k;
+        std::vector<BdsState> st(2 * d - 1);
+        std::vector<TreehashInst> ths((2 * d - 1) * nth + 1);
+        unsigned char* wots_sigs;
+        load_states(p, st.data(), ths.data(), &wots_sigs, sk);
+        for (unsigned i = 0; i < 2 * d - 1; ++i) {
+            st[i].stackoffset = 0;
+            st[i].next_leaf = 0;
+        }
+        memset(sk, 0, p->index_bytes);
+        randombytes(sk + p->index_bytes, 2 * n);
+        randombytes(sk + p->index_bytes + 3 * n, n);
+        memcpy(pk + n, sk + p->index_bytes + 3 * n, n);
+        KeySeeds ks{sk + p->index_bytes, sk + p->index_bytes + n, sk + p->index_bytes + 3 * n};
+        e->set_keys(fam, 1, &ks);
+        // the d layer-0..d-1 trees (tree address 0) are independent: one build
+        std::vector<uint32_t> layers(d);
+        std::vector<BdsState*> sps(d);
+        for (unsigned i = 0; i < d; ++i) {
+            layers[i] = i;
+            sps[i] = &st[i];
+        }
+        std::vector<unsigned char> roots((size_t)d * n);
+        treehash_init_many(p, e, fam, d, layers.data(), sps.data(), roots.data());
+        // layer i+1 signs the root of layer i (leaf 0 of tree 0)
+        if (d > 1) {
+            std::vector<WotsReq> w(d - 1);
+            for (unsigned i = 0; i + 1 < d; ++i) w[i] = {0, i + 1, 0, 0, WOTS_SIGN};
+            e->wots(fam, (int)(d - 1), w.data(), roots.data(), nullptr, wots_sigs);
+        }
+        memcpy(pk, &roots[(size_t)(d - 1) * n], n);
+        memcpy(sk + p->index_bytes + 2 * n, pk, n);
+        store_states(p, sk, st.data());
+        e->scrub_keys();
+        return 0;
+    });
 }
 
 static int bds_sign_api(const xmss_params* p, unsigned char* sk, unsigned char* sm, unsigned long long* smlen,
                         const unsigned char* m, unsigned long long mlen) {
-    std::lock_guard<std::mutex> lk(engine_mutex());
-    Engine* e;
-    int fam;
-    if (int rc = prepare_engine(p, &e, &fam)) return rc;
-    BdsDeviceSign ds;
-    if (bds_device_prepare(p, sk, sm, m, mlen, ds)) {
-        // the key context stays in the device pool across signatures
-        ds.job.key = (uint32_t)e->pool_slot(fam, {ds.job.sk_seed, ds.job.sk_prf, ds.job.pub_seed});
-        e->bds_sign_batch(fam, 1, &ds.job);
-        bds_device_finish(p, sm, smlen, mlen, ds);
-        return 0;
-    }
-    // last index (wiped key) or exhausted key: no traversal work, host control
-    DirectOps ops(e, fam, p->n, bds_plan_leaves(p, sk, mlen, false));
-    return bds_sign(p, sk, sm, smlen, m, mlen, ops);
+    if (fam_of(p->func, p->n) < 0 || mlen > kMaxMsgBytes) return -1;
+    // a device failure leaves the sk (index and traversal state) as it was:
+    // no signature left the call
+    std::vector<unsigned char> before(sk, sk + p->sk_bytes);
+    const int rc = guarded([&] {
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        Engine* e;
+        int fam;
+        if (int rc = prepare_engine(p, &e, &fam)) return rc;
+        BdsDeviceSign ds;
+        if (bds_device_prepare(p, sk, sm, m, mlen, ds)) {
+            // the key context stays in the device pool across signatures
+            e->pool_begin();
+            ds.job.key = (uint32_t)e->pool_slot(fam, {ds.job.sk_seed, ds.job.sk_prf, ds.job.pub_seed});
+            e->bds_sign_batch(fam, 1, &ds.job);
+            bds_device_finish(p, sm, smlen, mlen, ds);
+            return 0;
+        }
+        // last index (the key is wiped now) or exhausted key: the pool keeps no
+        // context of the wiped seeds; no traversal work, host control
+        const unsigned n = p->n, ib = p->index_bytes;
+        e->pool_forget(fam, {sk + ib, sk + ib + n, sk + ib + 3 * n});
+        DirectOps ops(e, fam, n, bds_plan_leaves(p, sk, mlen, false));
+        const int r = bds_sign(p, sk, sm, smlen, m, mlen, ops);
+        e->scrub_keys();
+        return r;
+    });
+    if (rc == XMSS_ENODEV) memcpy(sk, before.data(), before.size());
+    volatile unsigned char* v = before.data();
+    for (size_t i = 0; i < before.size(); ++i) v[i] = 0;
+    return rc;
 }
 
 // xmss_core_fast.c:710-851

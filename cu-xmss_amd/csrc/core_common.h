@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <new>
 
 #include "../../include/params.h"
 #include "engine.h"
@@ -41,8 +42,34 @@ inline int index_check(const xmss_params* p, unsigned char* sk, unsigned long lo
 // device (one stream); callers on other threads wait here.
 std::mutex& engine_mutex();
 
-// Return code when no GPU is usable (the library has no CPU path).
+// Return code when no GPU is usable (the library has no CPU path) or a device
+// call failed.
 constexpr int XMSS_ENODEV = -3;
+
+// Longest message the device path hashes: an H_msg descriptor holds the
+// prefixed message length in 32 bits (toByte(2, pad) || R || root ||
+// toByte(idx, n) || M, plus a block of padding).  Longer messages are
+// rejected with -1 before any byte is moved and with the sk unchanged,
+// where the reference would hash them (cuxmss/hash.c:107-120).
+constexpr unsigned long long kMaxMsgBytes = 0xFFFFF000ull;
+
+// Runs f (an entry point's body) and turns a device failure into -3: the
+// engine of the current device is brought back to a consistent state and
+// the caller's process goes on.
+template <class Fn>
+int guarded(Fn&& f) {
+    try {
+        return f();
+    } catch (const DeviceError&) {
+    } catch (const std::bad_alloc&) {
+    }
+    try {
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        if (Engine* e = Engine::get()) e->recover();
+    } catch (...) {
+    }
+    return XMSS_ENODEV;
+}
 
 // Shared verification (xmss_commons.c:136-215).
 int core_sign_open(const xmss_params* p, unsigned char* m, unsigned long long* mlen, const unsigned char* sm,

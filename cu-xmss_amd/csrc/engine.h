@@ -19,6 +19,7 @@
 #include <condition_variable>
 #include <functional>
 #include <mutex>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -36,6 +37,18 @@ enum Fam : int {
     F_SHAKE256_64,
     F_COUNT
 };
+
+// A failed HIP call, an exhausted device allocation or a request the launch
+// sequences cannot hold: thrown inside the library and returned as -3 at the
+// C ABI (core_common.h, guarded), never an abort of the caller's process.
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// Device bytes held by the library (workspaces, key pools, resident serving
+// state) and the limit on them (0: none); set_device_limit returns the old one.
+size_t device_bytes();
+size_t set_device_limit(size_t bytes);
 
 // VALU issue-rate probe (probe.hip): lane-ops/s and kernel ms of one launch
 // of instruction mix `kind` (0: SHA-256 chain step; 1-5: single classes).
@@ -211,20 +224,36 @@ public:
 
     // Device memory that outlives calls (resident key state of the serving
     // loop): allocation, release and a synchronous host-to-device copy on the
-    // current device.  device_alloc returns nullptr when the allocation fails.
+    // current device.  device_alloc returns nullptr when the allocation fails
+    // or would pass the library's device limit.
     static void* device_alloc(size_t bytes);
-    static void device_free(void* p);
+    static void device_free(void* p, size_t bytes);
     static void upload(void* dst, const void* src, size_t bytes);
 
-    void set_keys(int fam, int nkeys, const KeySeeds* seeds);
+    // After a DeviceError: back to a consistent launcher state (main stream,
+    // no pending timings), streams drained.
+    void recover();
 
-    // Device-resident key contexts that outlive a call (the serving pool):
-    // pool_add prepares one key and returns its slot; after pool_use(fam) the
-    // request `key` fields of the calls below index the pool of that family
-    // (until the next set_keys).
+    void set_keys(int fam, int nkeys, const KeySeeds* seeds);
+    // Overwrites the key contexts and seeds of the last set_keys with zeros
+    // (queued on the engine's stream).
+    void scrub_keys();
+
+    // Device-resident key contexts that outlive a call: after pool_use(fam)
+    // the request `key` fields of the calls below index the pool of that
+    // family (until the next set_keys).  Slots are found by a keyed digest
+    // of the seeds; no seed bytes are kept on the host.
+    // pool_add: a pinned slot (lives until pool_release);
     int pool_add(int fam, const KeySeeds& seeds);
-    // The slot of a key already in the pool (same seed bytes), else pool_add.
+    void pool_release(int fam, int slot);
+    // pool_slot: the slot of these seeds, else a cached slot -- at most
+    // kPoolCached of them per family, the least recently used one of an
+    // earlier call (pool_begin) evicted and zeroed first;
     int pool_slot(int fam, const KeySeeds& seeds);
+    void pool_begin();
+    // pool_forget: drops the cached slot of these seeds (a wiped key).
+    void pool_forget(int fam, const KeySeeds& seeds);
+    size_t pool_cached(int fam) const;
     void pool_use(int fam);
     size_t pool_key_bytes(int fam) const;
 

@@ -4,8 +4,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
+#include <random>
+#include <array>
 #include <string>
 #include <unordered_map>
 
@@ -15,10 +18,52 @@
 namespace xm {
 
 [[noreturn]] void fatal(const char* what) {
-    fprintf(stderr, "xmss_mi355x: fatal: %s\n", what);
+    fprintf(stderr, "xmss_mi355x: %s\n", what);
     fflush(stderr);
-    abort();
+    throw DeviceError(what);
 }
+
+[[noreturn]] void hip_failed(hipError_t e, const char* file, int line, const char* what) {
+    char msg[512];
+    snprintf(msg, sizeof msg, "HIP error %s at %s:%d (%s)", hipGetErrorString(e), file, line, what);
+    fprintf(stderr, "xmss_mi355x: %s\n", msg);
+    fflush(stderr);
+    throw DeviceError(msg);
+}
+
+// Bytes of device memory the library holds, and the limit on them.
+static std::atomic<size_t> g_dev_bytes{0};
+static std::atomic<size_t> g_dev_limit{0};
+
+void* dev_malloc(size_t bytes) {
+    const size_t lim = g_dev_limit.load();
+    const size_t now = g_dev_bytes.fetch_add(bytes) + bytes;
+    if (lim && now > lim) {
+        g_dev_bytes.fetch_sub(bytes);
+        char msg[160];
+        snprintf(msg, sizeof msg, "device allocation of %zu bytes beyond the library's limit of %zu", bytes, lim);
+        fatal(msg);
+    }
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        g_dev_bytes.fetch_sub(bytes);
+        (void)hipGetLastError();  // an out-of-memory error is not sticky: clear it
+        char msg[160];
+        snprintf(msg, sizeof msg, "hipMalloc of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+        fatal(msg);
+    }
+    return p;
+}
+
+void dev_free(void* p, size_t bytes) {
+    if (!p) return;
+    g_dev_bytes.fetch_sub(bytes);
+    HIPCK(hipFree(p));
+}
+
+size_t device_bytes() { return g_dev_bytes.load(); }
+size_t set_device_limit(size_t bytes) { return g_dev_limit.exchange(bytes); }
 
 int fam_of(unsigned func, unsigned n) {
     // the (func, n) pairs core_hash accepts (cuxmss/hash.c:36-63)
@@ -93,13 +138,25 @@ struct DevBuf {
     T* need(size_t bytes) {
         bytes = std::max<size_t>(bytes, 64);
         if (bytes > cap) {
-            if (p) HIPCK(hipFree(p));
+            release();
             size_t c = std::max(bytes, cap + cap / 2);
             c = (c + 4095) & ~(size_t)4095;
-            HIPCK(hipMalloc(&p, c));
+            try {
+                p = dev_malloc(c);
+            } catch (const DeviceError&) {  // the 1.5x growth did not fit: try the exact size
+                c = (bytes + 4095) & ~(size_t)4095;
+                p = dev_malloc(c);
+            }
             cap = c;
         }
         return (T*)p;
+    }
+    void release() {
+        void* q = p;
+        const size_t c = cap;
+        p = nullptr;
+        cap = 0;
+        dev_free(q, c);
     }
 };
 
@@ -124,10 +181,80 @@ struct PinnedBuf {
 struct BdsBufs;
 struct VerifyBufs;
 
+// Keyed 128-bit digest of a key's seed bytes: the pool finds a key's slot by
+// it, so no seed bytes are kept on the host.  Two SipHash-2-4 values
+// (Aumasson and Bernstein, 2012) under independent random per-process keys.
+struct Digest {
+    uint64_t a = 0, b = 0;
+    bool operator==(const Digest& o) const { return a == o.a && b == o.b; }
+};
+struct DigestHash {
+    size_t operator()(const Digest& d) const { return (size_t)(d.a ^ (d.b * 0x9E3779B97F4A7C15ull)); }
+};
+[[maybe_unused]] static inline uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+static uint64_t siphash24(uint64_t k0, uint64_t k1, const uint8_t* m, size_t len) {
+    uint64_t v0 = k0 ^ 0x736f6d6570736575ull, v1 = k1 ^ 0x646f72616e646f6dull;
+    uint64_t v2 = k0 ^ 0x6c7967656e657261ull, v3 = k1 ^ 0x7465646279746573ull;
+    auto round = [&] {
+        v0 += v1; v1 = rotl64(v1, 13); v1 ^= v0; v0 = rotl64(v0, 32);
+        v2 += v3; v3 = rotl64(v3, 16); v3 ^= v2;
+        v0 += v3; v3 = rotl64(v3, 21); v3 ^= v0;
+        v2 += v1; v1 = rotl64(v1, 17); v1 ^= v2; v2 = rotl64(v2, 32);
+    };
+    size_t i = 0;
+    for (; i + 8 <= len; i += 8) {
+        uint64_t w = 0;
+        for (int b = 7; b >= 0; --b) w = (w << 8) | m[i + b];
+        v3 ^= w;
+        round();
+        round();
+        v0 ^= w;
+    }
+    uint64_t last = (uint64_t)(len & 0xff) << 56;
+    for (size_t b = 0; i + b < len; ++b) last |= (uint64_t)m[i + b] << (8 * b);
+    v3 ^= last;
+    round();
+    round();
+    v0 ^= last;
+    v2 ^= 0xff;
+    for (int r = 0; r < 4; ++r) round();
+    return v0 ^ v1 ^ v2 ^ v3;
+}
+static Digest seed_digest(unsigned n, const KeySeeds& s) {
+    static const std::array<uint64_t, 4> key = [] {
+        std::random_device rd;
+        std::array<uint64_t, 4> k;
+        for (uint64_t& x : k) x = ((uint64_t)rd() << 32) ^ rd();
+        return k;
+    }();
+    uint8_t b[3 * 64] = {0};
+    if (s.sk_seed) memcpy(b, s.sk_seed, n);
+    if (s.sk_prf) memcpy(b + n, s.sk_prf, n);
+    if (s.pub_seed) memcpy(b + 2 * n, s.pub_seed, n);
+    Digest d{siphash24(key[0], key[1], b, 3 * n), siphash24(key[2], key[3], b, 3 * n)};
+    volatile uint8_t* vb = b;
+    for (unsigned i = 0; i < 3 * n; ++i) vb[i] = 0;
+    return d;
+}
+
+// Device-resident key contexts of one hash family.  Pinned slots (pool_add:
+// the serving loop's keys) live until released; cached slots (pool_slot: keys
+// met by one-call signatures and verifications) are evicted least recently
+// used beyond kPoolCached, never within the call that uses them (pool_begin
+// starts a call).  A freed slot's context is overwritten with zeros.
+constexpr size_t kPoolCached = 1024;
 struct KeyPool {
     void* p = nullptr;
-    size_t cap = 0;  // keys
-    int n = 0;
+    size_t cap = 0;  // slots allocated
+    struct Slot {
+        Digest d;
+        uint64_t used = 0;  // epoch of the last use
+        bool live = false, pinned = false;
+    };
+    std::vector<Slot> slot;
+    std::vector<int> free_slots;
+    std::unordered_map<Digest, int, DigestHash> by_digest;
+    size_t cached = 0;
 };
 
 struct Engine::Impl {
@@ -142,7 +269,7 @@ struct Engine::Impl {
     // input copies of verification batches: a batch's inputs go in here while
     // other batches' kernels run on s_main / s_aux
     hipStream_t s_copy = nullptr;
-    std::unordered_map<std::string, int> slots[F_COUNT];  // pool slot by seed bytes (pool_slot)
+    uint64_t epoch = 1;  // pool_begin: slots used since then are not evicted
     int key_fam = -1;
     int nkeys = 0;
     void sync() {
@@ -181,10 +308,15 @@ Engine* Engine::get() {
         return nullptr;
     }
     int dev = 0;
-    HIPCK(hipGetDevice(&dev));
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     auto it = engines.find(dev);
     if (it != engines.end()) return it->second;
-    Engine* e = new Engine();
+    Engine* e = nullptr;
+    try {
+        e = new Engine();
+    } catch (const DeviceError&) {
+        return nullptr;
+    }
     engines[dev] = e;
     return e;
 }
@@ -211,7 +343,11 @@ Engine* Engine::make_private() {
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return nullptr;
-    return new Engine();
+    try {
+        return new Engine();
+    } catch (const DeviceError&) {
+        return nullptr;
+    }
 }
 
 void Engine::release_private(Engine* e) {
@@ -221,17 +357,55 @@ void Engine::release_private(Engine* e) {
 }
 
 void* Engine::device_alloc(size_t bytes) {
-    void* p = nullptr;
-    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
-    return p;
+    try {
+        return dev_malloc(bytes);
+    } catch (const DeviceError&) {
+        return nullptr;
+    }
 }
 
-void Engine::device_free(void* p) {
-    if (p) HIPCK(hipFree(p));
-}
+void Engine::device_free(void* p, size_t bytes) { dev_free(p, bytes); }
 
 void Engine::upload(void* dst, const void* src, size_t bytes) {
     if (bytes) HIPCK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+
+__global__ __launch_bounds__(WG) void k_fill32(uint32_t* __restrict__ p, uint64_t words, uint32_t v) {
+    const uint64_t g = (uint64_t)blockIdx.x * WG + threadIdx.x;
+    if (g < words) p[g] = v;
+}
+
+// Zeros `bytes` (a multiple of 4) at p on the launcher's stream: a kernel
+// rather than hipMemsetAsync, so that a caller may capture it in a graph.
+static void zero_async(Launcher& L, void* p, size_t bytes) {
+    const uint64_t words = bytes / 4;
+    for (uint64_t w0 = 0; w0 < words; w0 += (uint64_t)WG * 65535) {
+        const uint64_t w = std::min<uint64_t>(words - w0, (uint64_t)WG * 65535);
+        L.run("fill", k_fill32, dim3(cdiv(w, WG)), dim3(WG), (uint32_t*)p + w0, w, 0u);
+    }
+}
+
+void Engine::recover() {
+    // after an exception: the launcher back on the main stream, pending
+    // timings dropped, the streams drained (errors ignored: the next call
+    // reports a device that stays broken)
+    Impl* I = impl_;
+    I->L.s = I->L.s_main;
+    for (auto& p : I->L.pending) {
+        I->L.pool.push_back(p.e0);
+        I->L.pool.push_back(p.e1);
+    }
+    I->L.pending.clear();
+    (void)hipStreamSynchronize(I->L.s_main);
+    (void)hipStreamSynchronize(I->L.s_aux);
+    (void)hipStreamSynchronize(I->s_copy);
+    (void)hipGetLastError();
+}
+
+void Engine::scrub_keys() {
+    Impl* I = impl_;
+    if (I->keys.p) zero_async(I->L, I->keys.p, I->keys.cap);
+    if (I->seeds.p) zero_async(I->L, I->seeds.p, I->seeds.cap);
 }
 
 void Engine::set_profiling(bool on) {
@@ -263,37 +437,98 @@ void Engine::set_keys(int fam, int nkeys, const KeySeeds* seeds) {
     I->key_fam = fam;
     I->nkeys = nkeys;
     I->kp_override = nullptr;
+    HIPCK(hipStreamSynchronize(I->L.s));  // the copy has read h: clear it
+    volatile uint8_t* vh = h.data();
+    for (size_t i = 0; i < h.size(); ++i) vh[i] = 0;
+}
+
+// A free slot of the pool (grown when none is left), prepared for `s`.
+static int pool_take(Engine::Impl* I, int fam, const KeySeeds& s, bool pinned, const Digest& d) {
+    FamOps* o = ops_of(fam);
+    KeyPool& P = I->pool[fam];
+    const size_t kb = o->key_bytes();
+    int slot;
+    if (!P.free_slots.empty()) {
+        slot = P.free_slots.back();
+        P.free_slots.pop_back();
+    } else {
+        if (P.slot.size() == P.cap) {  // grow, keeping the prepared contexts
+            const size_t cap = std::max<size_t>(64, P.cap * 2);
+            void* np = dev_malloc(cap * kb);
+            if (P.p) {
+                HIPCK(hipMemcpyAsync(np, P.p, P.slot.size() * kb, hipMemcpyDeviceToDevice, I->L.s));
+                HIPCK(hipStreamSynchronize(I->L.s));
+                dev_free(P.p, P.cap * kb);
+            }
+            P.p = np;
+            P.cap = cap;
+            if (I->key_fam == fam && I->kp_override) I->kp_override = P.p;
+        }
+        slot = (int)P.slot.size();
+        P.slot.emplace_back();
+    }
+    const unsigned n = fam_n(fam);
+    uint8_t h[3 * 64] = {0};
+    if (s.sk_seed) memcpy(&h[0], s.sk_seed, n);
+    if (s.sk_prf) memcpy(&h[n], s.sk_prf, n);
+    if (s.pub_seed) memcpy(&h[2 * n], s.pub_seed, n);
+    uint8_t* ds = I->seeds.need(3 * n);
+    I->h2d(ds, h, 3 * n);
+    o->prepare(I->L, ds, 1, (uint8_t*)P.p + (size_t)slot * kb);
+    zero_async(I->L, ds, ((3 * n + 3) / 4) * 4);
+    I->sync();
+    volatile uint8_t* vh = h;
+    for (unsigned i = 0; i < 3 * n; ++i) vh[i] = 0;
+    KeyPool::Slot& S = P.slot[slot];
+    S.d = d;
+    S.used = I->epoch;
+    S.live = true;
+    S.pinned = pinned;
+    if (!pinned) ++P.cached;
+    return slot;
+}
+
+// Frees a slot: its context is zeroed and its digest forgotten.
+static void pool_drop(Engine::Impl* I, int fam, int slot) {
+    KeyPool& P = I->pool[fam];
+    KeyPool::Slot& S = P.slot[slot];
+    if (!S.live) return;
+    auto it = P.by_digest.find(S.d);
+    if (it != P.by_digest.end() && it->second == slot) P.by_digest.erase(it);
+    if (!S.pinned) --P.cached;
+    const size_t kb = ops_of(fam)->key_bytes();
+    zero_async(I->L, (uint8_t*)P.p + (size_t)slot * kb, kb);
+    S = KeyPool::Slot{};
+    P.free_slots.push_back(slot);
 }
 
 int Engine::pool_add(int fam, const KeySeeds& s) {
     Impl* I = impl_;
-    FamOps* o = ops_of(fam);
-    KeyPool& P = I->pool[fam];
-    const size_t kb = o->key_bytes();
-    if ((size_t)P.n == P.cap) {  // grow, keeping the prepared contexts
-        const size_t cap = std::max<size_t>(64, P.cap * 2);
-        void* np = nullptr;
-        HIPCK(hipMalloc(&np, cap * kb));
-        if (P.p) {
-            HIPCK(hipMemcpyAsync(np, P.p, (size_t)P.n * kb, hipMemcpyDeviceToDevice, I->L.s));
-            HIPCK(hipStreamSynchronize(I->L.s));
-            HIPCK(hipFree(P.p));
-        }
-        P.p = np;
-        P.cap = cap;
-        if (I->key_fam == fam && I->kp_override) I->kp_override = P.p;
-    }
-    const unsigned n = fam_n(fam);
-    std::vector<uint8_t> h(3 * n, 0);
-    if (s.sk_seed) memcpy(&h[0], s.sk_seed, n);
-    if (s.sk_prf) memcpy(&h[n], s.sk_prf, n);
-    if (s.pub_seed) memcpy(&h[2 * n], s.pub_seed, n);
-    uint8_t* ds = I->seeds.need(h.size());
-    I->h2d(ds, h.data(), h.size());
-    o->prepare(I->L, ds, 1, (uint8_t*)P.p + (size_t)P.n * kb);
-    I->sync();
-    return P.n++;
+    const Digest d = seed_digest(fam_n(fam), s);
+    const int slot = pool_take(I, fam, s, true, d);
+    I->pool[fam].by_digest[d] = slot;
+    return slot;
 }
+
+void Engine::pool_release(int fam, int slot) {
+    Impl* I = impl_;
+    if (slot < 0 || (size_t)slot >= I->pool[fam].slot.size()) return;
+    pool_drop(I, fam, slot);
+    I->sync();
+}
+
+void Engine::pool_forget(int fam, const KeySeeds& s) {
+    Impl* I = impl_;
+    KeyPool& P = I->pool[fam];
+    auto it = P.by_digest.find(seed_digest(fam_n(fam), s));
+    if (it == P.by_digest.end() || P.slot[it->second].pinned) return;
+    pool_drop(I, fam, it->second);
+    I->sync();
+}
+
+void Engine::pool_begin() { ++impl_->epoch; }
+
+size_t Engine::pool_cached(int fam) const { return impl_->pool[fam].cached; }
 
 void Engine::pool_use(int fam) {
     impl_->key_fam = fam;
@@ -590,6 +825,7 @@ struct Engine::Batch {
     hipGraphExec_t graph = nullptr;  // verify: the captured launches of run()
     int nrun = 0;                    // verify: runs since staging
     PinnedBuf pin;                   // host staging of this batch's inputs and results
+    size_t seed_off = 0, seed_bytes = 0;  // sign: the seeds' region in `pin` and at `seeds`
     hipEvent_t done = nullptr;       // recorded after the work of launch()
     bool inflight = false;           // launched and not yet waited for
     bool evented = false;            // launched through launch(): results leave on the copy stream
@@ -710,10 +946,25 @@ static Engine::Batch* take_batch(Engine::Impl* I, bool sign) {
     return b ? b : new Engine::Batch;
 }
 
+// Frees a batch whose preparation failed (the exception goes on).
+struct BatchGuard {
+    Engine* e;
+    Engine::Batch* b;
+    bool ok = false;
+    ~BatchGuard() {
+        if (ok || !b) return;
+        try {
+            e->free_batch(b);
+        } catch (const DeviceError&) {
+        }
+    }
+};
+
 Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items,
                                     bool one_key, bool wait) {
     Impl* I = impl_;
     Batch* b = take_batch(I, true);
+    BatchGuard guard{this, b};
     b->sign = true;
     b->fam = fam;
     b->th = th;
@@ -772,6 +1023,8 @@ Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count,
                  o_gi = in.add(sizeof(uint64_t) * gi.size());
     uint8_t* h = b->pin.need(in.size);
     fill_seeds(h + o_seeds, n, ks);
+    b->seed_off = o_seeds;
+    b->seed_bytes = ks.size() * 3 * n;
     memcpy(h + o_md, ms.d.data(), sizeof(DMsg) * count);
     fill_msgs(h + o_blob, fam, true, mr, ms);
     memcpy(h + o_runs, runs.data(), sizeof(DRun) * runs.size());
@@ -798,6 +1051,7 @@ Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count,
     b->nodes.need(b->nodes_stride * n * (size_t)b->nruns);
     b->roots.need((size_t)count * d * n);  // layer-major roots
     if (wait) I->sync();
+    guard.ok = true;
     return b;
 }
 
@@ -886,6 +1140,7 @@ static void stage_verify(Engine::Impl* I, FamOps* o, Engine::Batch* b, const Eng
 Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int count, const VerifyItem* items) {
     Impl* I = impl_;
     Batch* b = take_batch(I, false);
+    BatchGuard guard{this, b};
     b->sign = false;
     b->fam = fam;
     b->th = th;
@@ -899,6 +1154,7 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
     b->dig.need((size_t)count * b->n);
     b->roots.need((size_t)count * d * b->n);
     stage_verify(I, o, b, items);
+    guard.ok = true;
     return b;
 }
 
@@ -1243,6 +1499,19 @@ void Engine::collect_verify(Batch* b, int* ok) {
 void Engine::free_batch(Batch* b) {
     wait_batch(b);
     b->evented = false;
+    if (b->sign) {
+        // a signing batch's seeds (staging, device copy) and key contexts are
+        // overwritten with zeros before its buffers are kept or freed
+        Impl* I = impl_;
+        if (b->seed_bytes) {
+            volatile uint8_t* v = b->pin.p + b->seed_off;
+            for (size_t i = 0; i < b->seed_bytes; ++i) v[i] = 0;
+            zero_async(I->L, (void*)b->seeds, (b->seed_bytes + 3) & ~(size_t)3);
+        }
+        if (b->keys.p) zero_async(I->L, b->keys.p, b->keys.cap);
+        b->seed_bytes = 0;
+        I->sync();
+    }
     if (!impl_->spare[b->sign]) {  // keep one of each kind: its buffers only grow
         impl_->spare[b->sign] = b;
         return;
@@ -1253,10 +1522,7 @@ void Engine::free_batch(Batch* b) {
     if (b->pin.p) HIPCK(hipHostFree(b->pin.p));
     DevBuf* bufs[] = {&b->io,    &b->res, &b->keys, &b->dig,  &b->nodes, &b->ends,
                       &b->tmp,   &b->roots, &b->lat, &b->lreq, &b->wlat,  &b->wreq};
-    for (DevBuf* x : bufs) {
-        if (x->p) HIPCK(hipFree(x->p));
-        x->p = nullptr;
-    }
+    for (DevBuf* x : bufs) x->release();
     delete b;
 }
 
@@ -1583,15 +1849,24 @@ void Engine::verify_many(int fam, int count, const VerifyJob* jobs, int* ok) {
 
 int Engine::pool_slot(int fam, const KeySeeds& s) {
     Impl* I = impl_;
-    const unsigned n = fam_n(fam);
-    std::string k(3 * n, '\0');
-    if (s.sk_seed) memcpy(&k[0], s.sk_seed, n);
-    if (s.sk_prf) memcpy(&k[n], s.sk_prf, n);
-    if (s.pub_seed) memcpy(&k[2 * n], s.pub_seed, n);
-    auto it = I->slots[fam].find(k);
-    if (it != I->slots[fam].end()) return it->second;
-    const int slot = pool_add(fam, s);
-    I->slots[fam].emplace(std::move(k), slot);
+    KeyPool& P = I->pool[fam];
+    const Digest d = seed_digest(fam_n(fam), s);
+    auto it = P.by_digest.find(d);
+    if (it != P.by_digest.end()) {
+        P.slot[it->second].used = I->epoch;
+        return it->second;
+    }
+    if (P.cached >= kPoolCached) {  // evict the least recently used cached slot of an earlier call
+        int victim = -1;
+        for (size_t k = 0; k < P.slot.size(); ++k) {
+            const KeyPool::Slot& S = P.slot[k];
+            if (S.live && !S.pinned && S.used < I->epoch && (victim < 0 || S.used < P.slot[victim].used))
+                victim = (int)k;
+        }
+        if (victim >= 0) pool_drop(I, fam, victim);
+    }
+    const int slot = pool_take(I, fam, s, false, d);
+    P.by_digest[d] = slot;
     return slot;
 }
 

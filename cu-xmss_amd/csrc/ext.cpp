@@ -24,10 +24,12 @@ struct JobSet {
     bool shared = true;        // e is the library's engine (else a private one)
     Engine::Batch* b = nullptr;
     std::vector<size_t> live;  // job inputs that reach the device, in batch order
+    std::vector<int> ok;       // verify: verdicts of the live inputs (collect)
 };
 
 struct xmss_mi355x_job {
     bool sign = false;
+    bool failed = false;  // a device call failed: every input that reached the device returns -3
     size_t count = 0;
     std::vector<int> rets;
     std::vector<int> set;  // input -> index in sets (-1: unknown OID)
@@ -45,10 +47,17 @@ struct xmss_mi355x_job {
 };
 
 namespace {
+int batch_keypair(int mt, unsigned char* const* pks, unsigned char* const* sks, const unsigned char* seeds,
+                  uint32_t oid, size_t count, const xmss_params& p, int fam);
 uint32_t oid_of(const unsigned char* b) {
     return ((uint32_t)b[0] << 24) | ((uint32_t)b[1] << 16) | ((uint32_t)b[2] << 8) | b[3];
 }
 int parse(int mt, uint32_t oid, xmss_params* p) { return mt ? xmssmt_parse_oid(p, oid) : xmss_parse_oid(p, oid); }
+// A signed message the device path verifies: at least a signature, and a
+// message no longer than kMaxMsgBytes (others are rejected with -1).
+bool verifiable(const xmss_params& p, unsigned long long smlen) {
+    return smlen >= p.sig_bytes && smlen - p.sig_bytes <= kMaxMsgBytes;
+}
 }  // namespace
 
 extern "C" {
@@ -63,10 +72,16 @@ int xmss_mi355x_valu_peak(unsigned blocks, unsigned iters, double* lane_ops_per_
 }
 
 int xmss_mi355x_valu_probe(int kind, unsigned blocks, unsigned iters, double* lane_ops_per_s, double* ms) {
-    std::lock_guard<std::mutex> lk(engine_mutex());
-    if (!Engine::get()) return XMSS_ENODEV;
-    return valu_probe(kind, blocks, iters, lane_ops_per_s, ms);
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        if (!Engine::get()) return XMSS_ENODEV;
+        return valu_probe(kind, blocks, iters, lane_ops_per_s, ms);
+    });
 }
+
+size_t xmss_mi355x_set_device_limit(size_t bytes) { return set_device_limit(bytes); }
+
+size_t xmss_mi355x_device_bytes(void) { return device_bytes(); }
 
 void xmss_mi355x_profile(int on) {
     std::lock_guard<std::mutex> lk(engine_mutex());
@@ -142,6 +157,23 @@ void fail_all(xmss_mi355x_job* j, int rc) {
     }
 }
 
+// After a device failure: every set's batch is dropped (its engine brought
+// back to a consistent state) and every input that reached the device
+// returns -3.  Called with the engine lock held.
+void fail_job(xmss_mi355x_job* j) {
+    for (JobSet& s : j->sets) {
+        if (!s.e) continue;
+        try {
+            s.e->recover();
+            if (s.b) s.e->free_batch(s.b);
+        } catch (...) {
+        }
+        s.b = nullptr;
+    }
+    fail_all(j, XMSS_ENODEV);
+    j->failed = true;
+}
+
 xmss_mi355x_job* sign_prepare(const int* mts, int mt_all, unsigned char* const* sks, const unsigned char* const* ms,
                               const unsigned long long* mlens, size_t count, bool mixed) {
     xmss_mi355x_job* j = new xmss_mi355x_job;
@@ -161,6 +193,7 @@ xmss_mi355x_job* sign_prepare(const int* mts, int mt_all, unsigned char* const* 
         const xmss_params& p = j->sets[s].p;
         const unsigned n = p.n, ib = p.index_bytes;
         unsigned char* sk = sks[i] + XMSS_OID_LEN;
+        if (mlens[i] > kMaxMsgBytes) continue;  // -1, sk unchanged
         const unsigned long long idx = (unsigned long)bytes_to_ull(sk, ib);
         if (index_check(&p, sk, idx)) {
             j->rets[i] = -2;
@@ -173,14 +206,18 @@ xmss_mi355x_job* sign_prepare(const int* mts, int mt_all, unsigned char* const* 
         j->rets[i] = 0;
     }
     std::lock_guard<std::mutex> lk(engine_mutex());
-    if (!bind_engines(j)) {
-        fail_all(j, XMSS_ENODEV);
-        return j;
-    }
-    for (size_t s = 0; s < j->sets.size(); ++s) {
-        JobSet& js = j->sets[s];
-        if (!js.live.empty())
-            js.b = js.e->sign_prepare(js.fam, js.p.tree_height, js.p.d, (int)items[s].size(), items[s].data());
+    try {
+        if (!bind_engines(j)) {
+            fail_all(j, XMSS_ENODEV);
+            return j;
+        }
+        for (size_t s = 0; s < j->sets.size(); ++s) {
+            JobSet& js = j->sets[s];
+            if (!js.live.empty())
+                js.b = js.e->sign_prepare(js.fam, js.p.tree_height, js.p.d, (int)items[s].size(), items[s].data());
+        }
+    } catch (const DeviceError&) {
+        fail_job(j);
     }
     return j;
 }
@@ -206,21 +243,25 @@ xmss_mi355x_job* verify_prepare(const int* mts, int mt_all, const unsigned char*
         items.resize(j->sets.size());
         const xmss_params& p = j->sets[s].p;
         const unsigned n = p.n, ib = p.index_bytes;
-        if (smlens[i] < p.sig_bytes) continue;
+        if (!verifiable(p, smlens[i])) continue;
         const unsigned char* sm = sms[i];
         items[s].push_back({pks[i] + XMSS_OID_LEN, bytes_to_ull(sm, ib), sm + ib, sm + ib + n, sm + p.sig_bytes,
                             smlens[i] - p.sig_bytes});
         j->sets[s].live.push_back(i);
     }
     std::lock_guard<std::mutex> lk(engine_mutex());
-    if (!bind_engines(j)) {
-        fail_all(j, XMSS_ENODEV);
-        return j;
-    }
-    for (size_t s = 0; s < j->sets.size(); ++s) {
-        JobSet& js = j->sets[s];
-        if (!js.live.empty())
-            js.b = js.e->verify_prepare(js.fam, js.p.tree_height, js.p.d, (int)items[s].size(), items[s].data());
+    try {
+        if (!bind_engines(j)) {
+            fail_all(j, XMSS_ENODEV);
+            return j;
+        }
+        for (size_t s = 0; s < j->sets.size(); ++s) {
+            JobSet& js = j->sets[s];
+            if (!js.live.empty())
+                js.b = js.e->verify_prepare(js.fam, js.p.tree_height, js.p.d, (int)items[s].size(), items[s].data());
+        }
+    } catch (const DeviceError&) {
+        fail_job(j);
     }
     return j;
 }
@@ -255,6 +296,7 @@ xmss_mi355x_job* xmss_mi355x_mixed_verify_prepare(const int* mts, const unsigned
 int xmss_mi355x_verify_refill(xmss_mi355x_job* j, const unsigned char* const* sms,
                               const unsigned long long* smlens, const unsigned char* const* pks) {
     if (!j || j->sign) return -1;
+    if (j->failed) return XMSS_ENODEV;
     // the same shape: each input keeps its public key's OID and whether it
     // reaches the device (a signed message at least sig_bytes long)
     std::vector<std::vector<Engine::VerifyItem>> items(j->sets.size());
@@ -266,7 +308,7 @@ int xmss_mi355x_verify_refill(xmss_mi355x_job* j, const unsigned char* const* sm
         JobSet& js = j->sets[s];
         const xmss_params& p = js.p;
         const bool live = at[s] < js.live.size() && js.live[at[s]] == i;
-        if (live != (smlens[i] >= p.sig_bytes)) return -1;
+        if (live != verifiable(p, smlens[i])) return -1;
         if (!live) continue;
         ++at[s];
         const unsigned n = p.n, ib = p.index_bytes;
@@ -278,33 +320,51 @@ int xmss_mi355x_verify_refill(xmss_mi355x_job* j, const unsigned char* const* sm
     j->smlens.assign(smlens, smlens + j->count);
     j->pks.assign(pks, pks + j->count);
     std::lock_guard<std::mutex> lk(engine_mutex());
-    for (size_t s = 0; s < j->sets.size(); ++s)
-        if (j->sets[s].b) j->sets[s].e->verify_refill(j->sets[s].b, items[s].data());
+    try {
+        for (size_t s = 0; s < j->sets.size(); ++s)
+            if (j->sets[s].b) j->sets[s].e->verify_refill(j->sets[s].b, items[s].data());
+    } catch (const DeviceError&) {
+        fail_job(j);
+        return XMSS_ENODEV;
+    }
     return 0;
 }
 
 int xmss_mi355x_job_launch(xmss_mi355x_job* j) {
     if (!j) return -1;
+    if (j->failed) return XMSS_ENODEV;
     std::lock_guard<std::mutex> lk(engine_mutex());
-    // every set's launches are queued on its own engine's streams, so the
-    // parameter sets share the GPU
-    for (JobSet& s : j->sets)
-        if (s.b) s.e->launch(s.b);
+    try {
+        // every set's launches are queued on its own engine's streams, so the
+        // parameter sets share the GPU
+        for (JobSet& s : j->sets)
+            if (s.b) s.e->launch(s.b);
+    } catch (const DeviceError&) {
+        fail_job(j);
+        return XMSS_ENODEV;
+    }
     return 0;
 }
 
 int xmss_mi355x_job_wait(xmss_mi355x_job* j) {
     if (!j) return -1;
-    bool prof = false;
-    for (JobSet& s : j->sets) {
-        if (!s.b) continue;
-        s.e->wait_batch(s.b);  // without the engine lock: other callers go on meanwhile
-        prof = prof || s.e->profiling();
-    }
-    if (prof) {  // per-kernel timings are read once the engines are idle
+    if (j->failed) return XMSS_ENODEV;
+    try {
+        bool prof = false;
+        for (JobSet& s : j->sets) {
+            if (!s.b) continue;
+            s.e->wait_batch(s.b);  // without the engine lock: other callers go on meanwhile
+            prof = prof || s.e->profiling();
+        }
+        if (prof) {  // per-kernel timings are read once the engines are idle
+            std::lock_guard<std::mutex> lk(engine_mutex());
+            for (JobSet& s : j->sets)
+                if (s.b) s.e->wait();
+        }
+    } catch (const DeviceError&) {
         std::lock_guard<std::mutex> lk(engine_mutex());
-        for (JobSet& s : j->sets)
-            if (s.b) s.e->wait();
+        fail_job(j);
+        return XMSS_ENODEV;
     }
     return 0;
 }
@@ -319,11 +379,15 @@ int xmss_mi355x_sign_collect(xmss_mi355x_job* j, unsigned char* const* sms, unsi
     if (!j || !j->sign) return -1;
     {
         std::lock_guard<std::mutex> lk(engine_mutex());
-        for (JobSet& s : j->sets) {
-            if (!s.b) continue;
-            std::vector<unsigned char*> outs;
-            for (size_t i : s.live) outs.push_back(sms[i] + s.p.index_bytes);
-            s.e->collect_sign(s.b, outs.data());
+        try {
+            for (JobSet& s : j->sets) {
+                if (!s.b) continue;
+                std::vector<unsigned char*> outs;
+                for (size_t i : s.live) outs.push_back(sms[i] + s.p.index_bytes);
+                s.e->collect_sign(s.b, outs.data());
+            }
+        } catch (const DeviceError&) {
+            fail_job(j);
         }
     }
     parallel_for(j->count, 512, [&](size_t i) {
@@ -343,17 +407,24 @@ int xmss_mi355x_sign_collect(xmss_mi355x_job* j, unsigned char* const* sms, unsi
 int xmss_mi355x_verify_collect(xmss_mi355x_job* j, unsigned char* const* ms, unsigned long long* mlens,
                                int* rets) {
     if (!j || j->sign) return -1;
+    if (!j->failed) {
+        std::lock_guard<std::mutex> lk(engine_mutex());
+        try {
+            for (JobSet& s : j->sets) {
+                s.ok.assign(s.live.size(), 0);
+                if (s.b) s.e->collect_verify(s.b, s.ok.data());
+            }
+        } catch (const DeviceError&) {
+            fail_job(j);
+        }
+    }
     for (size_t i = 0; i < j->count; ++i) {
         rets[i] = j->rets[i];
-        if (rets[i] == -1) mlens[i] = 0;
+        if (rets[i] != 0) mlens[i] = 0;
     }
     for (JobSet& s : j->sets) {
         const xmss_params& p = s.p;
-        std::vector<int> ok(s.live.size(), 0);
-        if (s.b) {
-            std::lock_guard<std::mutex> lk(engine_mutex());
-            s.e->collect_verify(s.b, ok.data());
-        }
+        const std::vector<int>& ok = s.ok;
         parallel_for(s.live.size(), 512, [&](size_t k) {
             const size_t i = s.live[k];
             const unsigned char* sm = j->sms[i];
@@ -384,7 +455,11 @@ void xmss_mi355x_job_free(xmss_mi355x_job* j) {
     {
         std::lock_guard<std::mutex> lk(engine_mutex());
         for (JobSet& s : j->sets) {
-            if (s.b) s.e->free_batch(s.b);
+            try {
+                if (s.b) s.e->free_batch(s.b);
+            } catch (const DeviceError&) {
+                s.e->recover();
+            }
             if (s.e && !s.shared) Engine::release_private(s.e);
         }
     }
@@ -395,10 +470,20 @@ int xmss_batch_keypair(int mt, unsigned char* const* pks, unsigned char* const* 
                        uint32_t oid, size_t count) {
     xmss_params p;
     if (parse(mt, oid, &p)) return -1;
-    const unsigned n = p.n, ib = p.index_bytes;
-    const int fam = fam_of(p.func, n);
+    const int fam = fam_of(p.func, p.n);
     if (fam < 0) return -1;
     if (count == 0) return 0;
+    return guarded([&] { return batch_keypair(mt, pks, sks, seeds, oid, count, p, fam); });
+}
+
+}  // extern "C"
+
+namespace {
+
+int batch_keypair(int mt, unsigned char* const* pks, unsigned char* const* sks, const unsigned char* seeds,
+                  uint32_t oid, size_t count, const xmss_params& p, int fam) {
+    (void)mt;
+    const unsigned n = p.n, ib = p.index_bytes;
     std::lock_guard<std::mutex> lk(engine_mutex());
     Engine* e = Engine::get();
     if (!e) return XMSS_ENODEV;
@@ -430,26 +515,33 @@ int xmss_batch_keypair(int mt, unsigned char* const* pks, unsigned char* const* 
         memcpy(pk, &roots[i * n], n);
         memcpy(sk + ib + 2 * n, pk, n);
     }
+    e->scrub_keys();
     return 0;
 }
+
+}  // namespace
+
+extern "C" {
 
 int xmss_batch_sign(int mt, unsigned char* const* sks, unsigned char* const* sms, unsigned long long* smlens,
                     const unsigned char* const* ms, const unsigned long long* mlens, int* rets, size_t count) {
     xmss_mi355x_job* j = xmss_mi355x_sign_prepare(mt, sks, ms, mlens, count);
-    int rc = xmss_mi355x_job_run(j);
-    if (rc == 0) rc = xmss_mi355x_sign_collect(j, sms, smlens, rets);
+    // collect also after a failed run: it reports -3 for the inputs that
+    // reached the device
+    const int rc = xmss_mi355x_job_run(j);
+    const int rc2 = xmss_mi355x_sign_collect(j, sms, smlens, rets);
     xmss_mi355x_job_free(j);
-    return rc;
+    return rc ? rc : rc2;
 }
 
 int xmss_batch_sign_open(int mt, unsigned char* const* ms, unsigned long long* mlens,
                          const unsigned char* const* sms, const unsigned long long* smlens,
                          const unsigned char* const* pks, int* rets, size_t count) {
     xmss_mi355x_job* j = xmss_mi355x_verify_prepare(mt, sms, smlens, pks, count);
-    int rc = xmss_mi355x_job_run(j);
-    if (rc == 0) rc = xmss_mi355x_verify_collect(j, ms, mlens, rets);
+    const int rc = xmss_mi355x_job_run(j);
+    const int rc2 = xmss_mi355x_verify_collect(j, ms, mlens, rets);
     xmss_mi355x_job_free(j);
-    return rc;
+    return rc ? rc : rc2;
 }
 
 }  // extern "C"

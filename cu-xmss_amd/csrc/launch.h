@@ -6,9 +6,11 @@
 #include <stdlib.h>
 
 #include <map>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "engine.h"
 #include "kernels.h"
 #include "kernels_bds.h"
 #include "kernels_lat.h"
@@ -16,17 +18,19 @@
 namespace xm {
 
 [[noreturn]] void fatal(const char* what);
+[[noreturn]] void hip_failed(hipError_t e, const char* file, int line, const char* what);
 
-#define HIPCK(x)                                                                  \
-    do {                                                                          \
-        hipError_t e_ = (x);                                                      \
-        if (e_ != hipSuccess) {                                                   \
-            fprintf(stderr, "xmss_mi355x: HIP error %s at %s:%d (%s)\n",          \
-                    hipGetErrorString(e_), __FILE__, __LINE__, #x);               \
-            fflush(stderr);                                                       \
-            abort();                                                              \
-        }                                                                         \
+#define HIPCK(x)                                                  \
+    do {                                                          \
+        hipError_t e_ = (x);                                      \
+        if (e_ != hipSuccess) hip_failed(e_, __FILE__, __LINE__, #x); \
     } while (0)
+
+// Device memory of the library's workspaces, key pools and resident serving
+// state goes through these: they count the bytes held and refuse an
+// allocation beyond the limit set by xmss_mi355x_set_device_limit (0: none).
+void* dev_malloc(size_t bytes);
+void dev_free(void* p, size_t bytes);
 
 struct Launcher {
     hipStream_t s = nullptr;

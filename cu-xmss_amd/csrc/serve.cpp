@@ -82,6 +82,7 @@ struct Call {
     // result
     unsigned char* out = nullptr;
     bool done = false;
+    bool failed = false;  // the lane's device work failed (DeviceError)
     std::condition_variable cv;  // the caller waits here (no wake-ups of unrelated callers)
 };
 
@@ -167,7 +168,19 @@ private:
             std::vector<Call*> batch;
             batch.swap(queue_[q]);
             lk.unlock();
-            execute_on(l.engine, batch, l.shared);
+            try {
+                execute_on(l.engine, batch, l.shared);
+            } catch (const DeviceError&) {
+                // every call of the group fails; the lane's engine is made
+                // consistent again for the next group
+                for (Call* c : batch) c->failed = true;
+                std::unique_lock<std::mutex> elk(engine_mutex(), std::defer_lock);
+                if (l.shared) elk.lock();
+                try {
+                    l.engine->recover();
+                } catch (...) {
+                }
+            }
             lk.lock();
             for (Call* c : batch) {
                 c->done = true;
@@ -182,6 +195,7 @@ private:
     void execute_on(Engine* e_, std::vector<Call*>& batch, bool shared) {
         std::unique_lock<std::mutex> elk(engine_mutex(), std::defer_lock);
         if (shared) elk.lock();
+        e_->pool_begin();  // the group's key contexts stay in the pool until it is done
         std::map<std::pair<int, int>, std::vector<Call*>> groups;  // (fam, kind)
         for (Call* c : batch) groups[{c->fam, (int)c->kind}].push_back(c);
         for (auto& g : groups) {
@@ -306,7 +320,7 @@ struct ServeOps final : BdsOps {
         call.ids = plan;
         std::vector<unsigned char> out(plan.size() * n);
         call.out = out.data();
-        d->run(&call);
+        run(call);
         for (size_t i = 0; i < plan.size(); ++i)
             cache[plan[i]] = std::vector<unsigned char>(out.begin() + i * n, out.begin() + (i + 1) * n);
     }
@@ -319,7 +333,7 @@ struct ServeOps final : BdsOps {
             call.slot = slot;
             call.ids = {id};
             call.out = out;
-            d->run(&call);
+            run(call);
             return;
         }
         memcpy(out, it->second.data(), n);
@@ -333,7 +347,7 @@ struct ServeOps final : BdsOps {
         call.in = in;
         std::vector<unsigned char> tmp(n);
         call.out = tmp.data();
-        d->run(&call);
+        run(call);
         memcpy(out, tmp.data(), n);
     }
     void wots_sign(unsigned char* sig, const unsigned char* msg, uint32_t layer, uint64_t tree,
@@ -348,7 +362,7 @@ struct ServeOps final : BdsOps {
         std::vector<unsigned char> m(msg, msg + n);
         call.msg = m.data();
         call.out = sig;
-        d->run(&call);
+        run(call);
     }
     void hmsg_sign(unsigned char* R, unsigned char* digest, uint64_t idx, const unsigned char* root,
                    const unsigned char* m, uint64_t mlen) override {
@@ -362,7 +376,11 @@ struct ServeOps final : BdsOps {
         call.mlen = mlen;
         call.out = R;
         call.out2 = digest;
+        run(call);
+    }
+    void run(Call& call) {
         d->run(&call);
+        if (call.failed) throw DeviceError("serving lane: device call failed");
     }
 };
 
@@ -375,12 +393,14 @@ struct Key {
     int slot = -1;                      // pool slot of the key's own seeds
     std::vector<unsigned char> seeds;   // SK_SEED || SK_PRF || PUB_SEED the slot was made from
     bool busy = false;
+    bool failed = false;  // a device call of one of its signatures failed: its requests return -3
 };
 
 struct Request {
     uint64_t id = 0;
     bool sign = true;
     int key = 0;
+    Key* k = nullptr;  // the key (stable address; taken under the server lock at submission)
     std::vector<unsigned char> data;  // sign: message; verify: sm
     std::vector<unsigned char> out;   // sign: sm; verify: m
     unsigned long long outlen = 0;
@@ -400,7 +420,9 @@ struct xmss_mi355x_server {
     std::condition_variable cv_work;
     std::condition_variable cv_key;  // get_sk waiters: a key became free
     int key_waiters = 0;
-    std::vector<Key> keys;
+    // keys live on the heap, so that a key added while requests run never
+    // moves one that a worker or a lane is using
+    std::vector<std::unique_ptr<Key>> keys;
     std::deque<Request*> queue;  // arrival order
     std::map<uint64_t, Request*> reqs;
     uint64_t next_id = 1;
@@ -417,7 +439,7 @@ struct xmss_mi355x_server {
     Request* pick() {
         for (auto it = queue.begin(); it != queue.end(); ++it) {
             Request* r = *it;
-            if (r->sign && keys[r->key].busy) continue;
+            if (r->sign && r->k->busy) continue;
             if (resident + transient + r->reserve > budget && transient > 0) {
                 ++admission_waits;
                 return nullptr;
@@ -438,7 +460,7 @@ struct xmss_mi355x_server {
                 if (!r) return;
                 transient += r->reserve;
                 peak = std::max(peak, resident + transient);
-                if (r->sign) keys[r->key].busy = true;
+                if (r->sign) r->k->busy = true;
                 others = !queue.empty();
             }
             // Wake-ups go one worker at a time: a worker that takes a request
@@ -456,7 +478,7 @@ struct xmss_mi355x_server {
             {
                 std::lock_guard<std::mutex> lk(mu);
                 transient -= r->reserve;
-                if (r->sign) keys[r->key].busy = false;
+                if (r->sign) r->k->busy = false;
                 r->t1 = Clock::now();
                 r->done = true;
                 r->cv.notify_all();
@@ -490,44 +512,73 @@ struct xmss_mi355x_server {
     }
 
     void execute(Request* r, bool& active) {
-        Key& k = keys[r->key];
+        Key& k = *r->k;
         const xmss_params& p = k.p;
         if (r->sign) {
             // the key's sk is owned by this request while busy
             unsigned char* sk = k.sk.data() + XMSS_OID_LEN;
             r->out.assign(p.sig_bytes + r->data.size(), 0);
-            BdsDeviceSign ds;
-            if (bds_device_prepare(&p, sk, r->out.data(), r->data.data(), r->data.size(), ds)) {
-                ds.job.key = (uint32_t)k.slot;
-                ds.job.dsk = k.dsk;
-                Call call;
-                call.kind = Call::BDS;
-                call.fam = k.fam;
-                call.p = &p;
-                call.job = &ds.job;
-                disp->run(&call);
-                disp->leave();
-                active = false;
-                bds_device_finish(&p, r->out.data(), &r->outlen, r->data.size(), ds);
-                r->ret = 0;
+            if (k.failed || r->data.size() > kMaxMsgBytes) {
+                r->ret = k.failed ? XMSS_ENODEV : -1;
+                r->outlen = 0;
                 return;
             }
-            // last index or exhausted key: host control, hashing through the dispatcher
-            ServeOps ops;
-            ops.d = disp.get();
-            ops.fam = k.fam;
-            ops.n = p.n;
-            ops.slot = k.slot;
-            ops.slot_for = [&](const unsigned char* a, const unsigned char* b, const unsigned char* c) {
-                return slot_for(k, a, b, c);
-            };
-            ops.plan = bds_plan_leaves(&p, sk, r->data.size());
-            r->ret = bds_sign(&p, sk, r->out.data(), &r->outlen, r->data.data(), r->data.size(), ops);
-            Engine::upload(k.dsk, sk, p.sk_bytes);  // the mirror changed on the host
+            // a failed device call leaves the mirror as it was before this
+            // signature, and the key failed (its device copy is unknown)
+            std::vector<unsigned char> before(k.sk);
+            BdsDeviceSign ds;
+            bool ok = true;
+            try {
+                if (bds_device_prepare(&p, sk, r->out.data(), r->data.data(), r->data.size(), ds)) {
+                    ds.job.key = (uint32_t)k.slot;
+                    ds.job.dsk = k.dsk;
+                    Call call;
+                    call.kind = Call::BDS;
+                    call.fam = k.fam;
+                    call.p = &p;
+                    call.job = &ds.job;
+                    disp->run(&call);
+                    disp->leave();
+                    active = false;
+                    if (call.failed) throw DeviceError("serving lane: signature failed");
+                    bds_device_finish(&p, r->out.data(), &r->outlen, r->data.size(), ds);
+                    r->ret = 0;
+                    return;
+                }
+                // last index or exhausted key: host control, hashing through the dispatcher
+                ServeOps ops;
+                ops.d = disp.get();
+                ops.fam = k.fam;
+                ops.n = p.n;
+                ops.slot = k.slot;
+                ops.slot_for = [&](const unsigned char* a, const unsigned char* b, const unsigned char* c) {
+                    return slot_for(k, a, b, c);
+                };
+                ops.plan = bds_plan_leaves(&p, sk, r->data.size());
+                r->ret = bds_sign(&p, sk, r->out.data(), &r->outlen, r->data.data(), r->data.size(), ops);
+                Engine::upload(k.dsk, sk, p.sk_bytes);  // the mirror changed on the host
+                if (r->ret == 0 && k.slot >= 0) {
+                    // the last index was signed and the sk wiped: the pool keeps
+                    // no context of the old seeds
+                    std::lock_guard<std::mutex> elk(engine_mutex());
+                    e->pool_release(k.fam, k.slot);
+                    k.slot = -1;
+                }
+            } catch (const DeviceError&) {
+                ok = false;
+            }
+            if (!ok) {
+                k.sk = before;
+                k.failed = true;
+                r->ret = XMSS_ENODEV;
+                r->outlen = 0;
+            }
+            volatile unsigned char* v = before.data();
+            for (size_t i = 0; i < before.size(); ++i) v[i] = 0;
         } else {
             const unsigned long long smlen = r->data.size();
             r->out.assign(smlen + 1, 0);
-            if (smlen < p.sig_bytes) {
+            if (smlen < p.sig_bytes || smlen - p.sig_bytes > kMaxMsgBytes) {
                 r->ret = -1;
                 r->outlen = 0;
                 return;
@@ -543,7 +594,10 @@ struct xmss_mi355x_server {
             disp->leave();
             active = false;
             const unsigned long long mlen = smlen - p.sig_bytes;
-            if (call.ok) {
+            if (call.failed) {
+                r->outlen = 0;
+                r->ret = XMSS_ENODEV;
+            } else if (call.ok) {
                 memcpy(r->out.data(), r->data.data() + p.sig_bytes, mlen);
                 r->outlen = mlen;
                 r->ret = 0;
@@ -580,7 +634,8 @@ xmss_mi355x_server* xmss_mi355x_server_create(size_t hbm_budget_bytes, int worke
 }
 
 int xmss_mi355x_server_add_key(xmss_mi355x_server* s, int mt, const unsigned char* sk, const unsigned char* pk) {
-    Key k;
+    std::unique_ptr<Key> kp(new Key);
+    Key& k = *kp;
     k.mt = mt;
     const uint32_t oid = ((uint32_t)pk[0] << 24) | ((uint32_t)pk[1] << 16) | ((uint32_t)pk[2] << 8) | pk[3];
     if (mt ? xmssmt_parse_oid(&k.p, oid) : xmss_parse_oid(&k.p, oid)) return -1;
@@ -596,21 +651,27 @@ int xmss_mi355x_server_add_key(xmss_mi355x_server* s, int mt, const unsigned cha
         s->peak = std::max(s->peak, s->resident + s->transient);
     }
     k.dsk = static_cast<unsigned char*>(Engine::device_alloc(k.p.sk_bytes));
-    if (!k.dsk) {
+    bool ok = k.dsk != nullptr;
+    if (ok) {
+        try {
+            Engine::upload(k.dsk, k.sk.data() + XMSS_OID_LEN, k.p.sk_bytes);
+            k.seeds = s->seeds_of(k);
+            const unsigned n = k.p.n;
+            KeySeeds ks{&k.seeds[0], &k.seeds[n], &k.seeds[2 * n]};
+            std::lock_guard<std::mutex> elk(engine_mutex());
+            k.slot = s->e->pool_add(k.fam, ks);
+        } catch (const DeviceError&) {
+            ok = false;
+        }
+    }
+    if (!ok) {
+        Engine::device_free(k.dsk, k.p.sk_bytes);
         std::lock_guard<std::mutex> lk(s->mu);
         s->resident -= res_bytes;
         return -1;
     }
-    Engine::upload(k.dsk, k.sk.data() + XMSS_OID_LEN, k.p.sk_bytes);
-    k.seeds = s->seeds_of(k);
-    const unsigned n = k.p.n;
-    KeySeeds ks{&k.seeds[0], &k.seeds[n], &k.seeds[2 * n]};
-    {
-        std::lock_guard<std::mutex> elk(engine_mutex());
-        k.slot = s->e->pool_add(k.fam, ks);
-    }
     std::lock_guard<std::mutex> lk(s->mu);
-    s->keys.push_back(std::move(k));
+    s->keys.push_back(std::move(kp));
     return (int)s->keys.size() - 1;
 }
 
@@ -625,7 +686,8 @@ static uint64_t submit(xmss_mi355x_server* s, bool sign, int key, const unsigned
         delete r;
         return 0;
     }
-    const xmss_params& p = s->keys[key].p;
+    r->k = s->keys[key].get();
+    const xmss_params& p = r->k->p;
     // device working set: message + signature staging, planned leaves, H_msg prefix
     r->reserve = (size_t)p.sig_bytes + len + (size_t)(p.tree_height + 1) * p.n * 2 + 4 * p.n;
     r->id = s->next_id++;
@@ -663,10 +725,11 @@ int xmss_mi355x_server_result(xmss_mi355x_server* s, uint64_t id, unsigned char*
 int xmss_mi355x_server_get_sk(xmss_mi355x_server* s, int key, unsigned char* sk) {
     std::unique_lock<std::mutex> lk(s->mu);
     if (key < 0 || key >= (int)s->keys.size()) return -1;
+    Key* k = s->keys[key].get();
     ++s->key_waiters;
-    s->cv_key.wait(lk, [&] { return !s->keys[key].busy; });
+    s->cv_key.wait(lk, [&] { return !k->busy; });
     --s->key_waiters;
-    memcpy(sk, s->keys[key].sk.data(), s->keys[key].sk.size());
+    memcpy(sk, k->sk.data(), k->sk.size());
     return 0;
 }
 
@@ -688,7 +751,14 @@ void xmss_mi355x_server_destroy(xmss_mi355x_server* s) {
     for (auto& t : s->workers) t.join();
     s->disp.reset();
     for (auto& kv : s->reqs) delete kv.second;
-    for (Key& k : s->keys) Engine::device_free(k.dsk);
+    for (auto& k : s->keys) {
+        Engine::device_free(k->dsk, k->p.sk_bytes);
+        try {
+            std::lock_guard<std::mutex> elk(engine_mutex());
+            if (k->slot >= 0) s->e->pool_release(k->fam, k->slot);
+        } catch (const DeviceError&) {
+        }
+    }
     delete s;
 }
 

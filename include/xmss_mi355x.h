@@ -24,6 +24,22 @@ void xmss_mi355x_rng_urandom(void);
  * returns -3). */
 int xmss_mi355x_device_ok(void);
 
+/* Error behaviour of every hashing entry point (the reference functions of
+ * xmss.h / xmss_core.h included): a failed HIP call or a device allocation the
+ * GPU (or the limit below) cannot hold returns -3 and leaves the process
+ * running.  A one-call signature that fails that way leaves its sk as it was
+ * (no signature left the call); a batch job's inputs that reached the device
+ * return -3, and their keys' indices stay advanced.  Messages longer than
+ * 0xFFFFF000 bytes are refused with -1 before anything is written (sign: sk
+ * unchanged; verify: *mlen = 0); the reference would hash them.
+ *
+ * Device memory: xmss_mi355x_device_bytes() is what the library holds now
+ * (workspaces, key pools, resident serving state); set_device_limit caps it
+ * (0 = no cap, the default) and returns the previous cap.  An allocation
+ * beyond the cap fails as above. */
+size_t xmss_mi355x_set_device_limit(size_t bytes);
+size_t xmss_mi355x_device_bytes(void);
+
 /* Per-kernel timing with HIP events on the library's stream. */
 void xmss_mi355x_profile(int on);
 void xmss_mi355x_reset_stats(void);
@@ -89,7 +105,9 @@ int xmss_mi355x_job_run(xmss_mi355x_job *job);
  * work:  launch(A); loop { refill(B); collect(A); launch(B); swap(A, B); }
  * and with three jobs in rotation the GPU also has the next batch queued
  * while the host collects.  (A signing job's input copy keeps the compute
- * stream.)  Each returns 0, or -1 for a null job. */
+ * stream.)  Each returns 0, -1 for a null job, or -3 when a device call
+ * failed: the job is then failed, and collect reports -3 for every input that
+ * reached the device (refill, too, returns -3 for a failed job). */
 int xmss_mi355x_job_launch(xmss_mi355x_job *job);
 int xmss_mi355x_job_wait(xmss_mi355x_job *job);
 /* Collect returns 0, or -1 for a null job or one of the other kind (sign /
