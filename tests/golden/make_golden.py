@@ -22,6 +22,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 from ffi import Params, addr, buf, load_ref, raw, ull, u8p, REF_DIR  # noqa: E402
+import workloads as W  # noqa: E402
 
 SEED = 0x584D5353  # "XMSS"
 
@@ -266,6 +267,220 @@ def bds_xmssmt(ref_fast, name, nsigns, fname):
                   "pk": h(pk), "sk": h(sk0), "signs": rows})
 
 
+# ---- fixtures of the batch, mixed-set, serving and whole-life workloads ----
+# These run many reference signatures (about a second each for a plain-core
+# signature of one 1024-leaf tree), so they fan out over worker processes;
+# each worker loads its own handle of the reference build.
+
+_REF = {}
+
+
+def _ref(core):
+    if core not in _REF:
+        _REF[core] = load_ref(core)
+    return _REF[core]
+
+
+def _pool_map(fn, args):
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(min(8, os.cpu_count() or 1)) as pool:
+        return pool.map(fn, args, chunksize=1)
+
+
+def _seed_keypair_task(a):
+    oid, mt, seed3 = a
+    ref = _ref(0)
+    rc, pk, sk = ref.seed_keypair(oid, mt, seed3)
+    assert rc == 0
+    return pk
+
+
+def _plain_sign_task(a):
+    sk, m, mt, pk = a
+    ref = _ref(0)
+    rc, sm, sko = ref.sign(sk, m, mt)
+    row = {"rc": rc, "sk_out": dg(sko)}
+    if rc == 0:
+        row["sm"] = dg(sm)
+        row["verify"] = ref.sign_open(sm, pk, mt)[0]
+    return row
+
+
+def plain_batch_large(count=320):
+    """Independent XMSS-SHA2_10_256 keys from seeds (one keypair per key via
+    xmssmt_core_seed_keypair), each signing one log-uniform 32 B - 4 KB
+    message at a random index in [0, 1024] (the first four keys at 1023, 1024,
+    1022 and 0).  Shape of the 4096-key batch, sized for the reference's CPU
+    time; it reaches the throughput kernels of a GPU signing batch."""
+    oid, mt = 0x01, 0
+    ref = _ref(0)
+    p = ref.params(oid, mt)
+    kseed, mseed = W.SEED + 600, W.SEED + 601
+    seeds = [W.key_seed(kseed, k, p.n) for k in range(count)]
+    roots = _pool_map(_seed_keypair_task, [(oid, mt, s) for s in seeds])
+    roots = [r[:p.n] for r in roots]
+    idx = W.batch_indices(kseed, count, (1 << p.full_height) - 1)
+    tasks = []
+    for k in range(count):
+        sk = W.plain_sk(oid, p, idx[k], seeds[k], roots[k])
+        pk = W.with_oid(oid, roots[k] + seeds[k][2 * p.n:])
+        tasks.append((sk, W.message(mseed, k), mt, pk))
+    rows = _pool_map(_plain_sign_task, tasks)
+    items = [dict(root=h(roots[k]), idx=idx[k], **rows[k]) for k in range(count)]
+    write("plain_batch_large.json", {"oid": oid, "mt": mt, "count": count, "key_seed": kseed, "msg_seed": mseed,
+                                     "items": items})
+
+
+MIXED4 = [("XMSSMT-SHA2_20/2_256", 1), ("XMSS-SHA2_10_512", 0), ("XMSS-SHAKE_10_256", 0)]
+
+
+def mixed_sets(requests=260, keys=8):
+    """The three parameter sets of the mixed workload: per set `keys` seeded
+    keys and `requests` plain signatures, request i by key i % keys at a
+    uniform index in [0, 2^h - 2] with a log-uniform 32 B - 4 KB message."""
+    ref = _ref(0)
+    res = {}
+    for s, (name, mt) in enumerate(MIXED4):
+        t = time.time()
+        oid = ref.str_to_oid(name)
+        p = ref.params(oid, mt)
+        kseed, mseed = W.SEED + 700 + 10 * s, W.SEED + 701 + 10 * s
+        seeds = [W.key_seed(kseed, k, p.n) for k in range(keys)]
+        roots = [r[:p.n] for r in _pool_map(_seed_keypair_task, [(oid, mt, x) for x in seeds])]
+        rng = random.Random(kseed)
+        idx = [rng.randrange(0, (1 << p.full_height) - 1) for _ in range(requests)]
+        tasks = []
+        for i in range(requests):
+            k = i % keys
+            sk = W.plain_sk(oid, p, idx[i], seeds[k], roots[k])
+            pk = W.with_oid(oid, roots[k] + seeds[k][2 * p.n:])
+            tasks.append((sk, W.message(mseed, i), mt, pk))
+        rows = _pool_map(_plain_sign_task, tasks)
+        res[name] = {"oid": oid, "mt": mt, "keys": keys, "key_seed": kseed, "msg_seed": mseed,
+                     "roots": [h(r) for r in roots], "items": [dict(idx=idx[i], **rows[i]) for i in range(requests)]}
+        print("  ", name, "%.1fs" % (time.time() - t), flush=True)
+    write("mixed_sets.json", res)
+
+
+def _bds_seq_task(a):
+    """BDS signatures of one key in order: digests of every sm and sk."""
+    sk, mt, mseed, count = a
+    ref = _ref(1)
+    rows = []
+    for i in range(count):
+        rc, sm, sk = ref.sign(sk, W.message(mseed, i), mt)
+        rows.append({"rc": rc, "sm": dg(sm), "sk": dg(sk)})
+    return rows
+
+
+SERVE5 = [("XMSS-SHA2_16_256", 0), ("XMSSMT-SHA2_20/4_256", 1), ("XMSSMT-SHA2_40/4_256", 1)]
+
+
+def serve_sets(keys=3, count=24, long_count=1040):
+    """The three BDS parameter sets of the serving workload: per set `keys`
+    keys (XMSS: the BDS keypair's fixed seeds, so one key with separate
+    message streams; XMSS^MT: seeded keypairs), each signing `count`
+    log-uniform 32 B - 4 KB messages in order, the last XMSS^MT key of each
+    set `long_count` (past its first layer-0 subtree switch at 1024)."""
+    ref_fast = _ref(1)
+    res = {}
+    for s, (name, mt) in enumerate(SERVE5):
+        t = time.time()
+        oid = ref_fast.str_to_oid(name)
+        p = ref_fast.params(oid, mt)
+        tasks, keyrows = [], []
+        pk0 = sk0 = None
+        for k in range(keys):
+            rseed = W.SEED + 800 + 10 * s + k
+            if mt or pk0 is None:
+                ref_fast.seed(rseed)
+                rc, pk0, sk0 = ref_fast.keypair(oid, mt, sk_init=bytes(4 + p.sk_bytes))
+                assert rc == 0
+            mseed = W.SEED + 900 + 10 * s + k
+            cnt = long_count if (mt and k == keys - 1) else count
+            tasks.append((sk0, mt, mseed, cnt))
+            keyrows.append({"rng_seed": rseed, "msg_seed": mseed, "pk": h(pk0), "sk": dg(sk0), "count": cnt})
+        rows = _pool_map(_bds_seq_task, tasks)
+        for kr, r in zip(keyrows, rows):
+            kr["signs"] = r
+        res[name] = {"oid": oid, "mt": mt, "keys": keyrows}
+        print("  ", name, "%.1fs" % (time.time() - t), flush=True)
+    write("serve_sets.json", res)
+
+
+def nonzero_sk_keypairs():
+    """BDS keypairs written into caller buffers that are not zero: the
+    reference leaves some state fields unwritten (xmss_core_fast.c:656-701),
+    so they keep the caller's bytes; then four signatures."""
+    ref_fast = _ref(1)
+    res = {}
+    for name, mt in (("XMSS-SHA2_10_256", 0), ("XMSSMT-SHA2_20/4_256", 1)):
+        oid = ref_fast.str_to_oid(name)
+        p = ref_fast.params(oid, mt)
+        init = random.Random(W.SEED + 950 + mt).randbytes(4 + p.sk_bytes)
+        ref_fast.seed(W.SEED + 951 + mt)
+        rc, pk, sk = ref_fast.keypair(oid, mt, sk_init=init)
+        assert rc == 0
+        sk0 = sk
+        rows = []
+        for i in range(4):
+            m = W.fixed_message(W.SEED + 952, i)
+            rc, sm, sk = ref_fast.sign(sk, m, mt)
+            rows.append({"rc": rc, "sm": dg(sm), "sk": dg(sk)})
+        res[name] = {"oid": oid, "mt": mt, "rng_seed": W.SEED + 951 + mt, "init_seed": W.SEED + 950 + mt,
+                     "pk": h(pk), "sk": h(sk0), "signs": rows}
+    write("nonzero_sk_keypairs.json", res)
+
+
+def whole_life_samples():
+    """XMSSMT-SHA2_20/2_256 (cuxmss/test/xmss_max_signatures.c:35-102): a
+    seeded key and one random 32-byte message, as that test draws them
+    (randombytes(m) before the keypair); reference signatures at sampled
+    indices, the last index (wiped key, returns 0) and the two -2 calls after
+    it, with the sk after each.  Also the plain-core signature of the last
+    index of the XMSS-SHA2_10_256 BDS life fixture's key with its message 1023
+    (the BDS core signs that index differently)."""
+    ref = _ref(0)
+    name, mt = "XMSSMT-SHA2_20/2_256", 1
+    oid = ref.str_to_oid(name)
+    p = ref.params(oid, mt)
+    ref.seed(W.SEED + 960)
+    m = buf(32)
+    ref.lib.randombytes.argtypes = [u8p, ull]
+    ref.lib.randombytes(m, 32)
+    msg = raw(m, 32)
+    rc, pk, sk = ref.keypair(oid, mt)
+    assert rc == 0
+    top = (1 << p.full_height) - 1
+    samples = [0, 1, 2, 1023, 1024, 1025, 524287, 524288, top - 1]
+    tasks = []
+    for i in samples:
+        skx = bytearray(sk)
+        skx[4:4 + p.index_bytes] = i.to_bytes(p.index_bytes, "big")
+        tasks.append((bytes(skx), msg, mt, pk))
+    rows = _pool_map(_plain_sign_task, tasks)
+    skx = bytearray(sk)
+    skx[4:4 + p.index_bytes] = top.to_bytes(p.index_bytes, "big")
+    tail = []
+    cur = bytes(skx)
+    for _ in range(3):
+        rc, sm, cur = ref.sign(cur, msg, mt)
+        tail.append({"rc": rc, "sm": dg(sm) if rc == 0 else None, "sk_out": h(cur)})
+    # the BDS life fixture's key in plain-core form at its last index
+    bds = json.load(open(os.path.join(HERE, "bds_xmss_sha2_10_256.json")))
+    p10 = ref.params(0x01, 0)
+    gen = W.bds10_messages()
+    msgs = [next(gen) for _ in range(1024)]
+    pk10 = bytes.fromhex(bds["pk"])
+    seed3 = b"\x02" * 64 + b"\x04" * 32
+    sk10 = W.plain_sk(0x01, p10, 1023, seed3, pk10[4:4 + 32])
+    rc, sm, sko = ref.sign(sk10, msgs[1023], 0)
+    write("whole_life.json", {"name": name, "oid": oid, "rng_seed": W.SEED + 960, "msg": h(msg), "pk": h(pk),
+                              "sk": h(sk), "samples": [dict(idx=i, **r) for i, r in zip(samples, rows)],
+                              "last": tail,
+                              "bds10_last_plain": {"rc": rc, "sm": dg(sm), "sk_out": h(sko)}})
+
+
 def main():
     big = "--big" in sys.argv
     only = [a for a in sys.argv[1:] if not a.startswith("--")]
@@ -280,6 +495,12 @@ def main():
         "batch": lambda: plain_batch(ref),
         "bds10": lambda: bds_xmss(ref_fast, 0x01, 1025, "bds_xmss_sha2_10_256.json", full=(0, 1, 1022, 1023)),
         "bdsmt": lambda: bds_xmssmt(ref_fast, "XMSSMT-SHA2_20/4_256", 1100, "bds_xmssmt_sha2_20_4_256.json"),
+        "bdsmt40": lambda: bds_xmssmt(ref_fast, "XMSSMT-SHA2_40/4_256", 1100, "bds_xmssmt_sha2_40_4_256.json"),
+        "batch320": plain_batch_large,
+        "mixed": mixed_sets,
+        "serve": serve_sets,
+        "nonzero": nonzero_sk_keypairs,
+        "life": whole_life_samples,
     }
     if big:
         jobs["bds16"] = lambda: bds_xmss(ref_fast, 0x02, 16, "bds_xmss_sha2_16_256.json", full=(0,))

@@ -47,6 +47,37 @@ def test_bds_xmssmt_sha2_20_4_256_1100_signs():
     _replay(product(1), golden("bds_xmssmt_sha2_20_4_256.json"), 1)
 
 
+def test_bds_xmssmt_sha2_40_4_256_1100_signs():
+    """XMSSMT-SHA2_40/4_256 (four layers of 2^10-leaf subtrees, the served
+    set of the largest height): seeded keypair, 1100 signatures across the
+    first layer-0 subtree switch."""
+    _replay(product(1), golden("bds_xmssmt_sha2_40_4_256.json"), 1)
+
+
+@pytest.mark.parametrize("name", ["XMSS-SHA2_10_256", "XMSSMT-SHA2_20/4_256"])
+def test_bds_keypair_into_a_buffer_that_is_not_zero(name):
+    """The BDS keypair leaves some state fields unwritten
+    (xmss_core_fast.c:656-701): they keep the caller's bytes, as the
+    reference's do; four signatures after it equal the reference's too."""
+    import workloads as W
+    g = golden("nonzero_sk_keypairs.json")[name]
+    lib = product(1)
+    mt = g["mt"]
+    p = lib.params(g["oid"], mt)
+    init = random.Random(g["init_seed"]).randbytes(4 + p.sk_bytes)
+    assert any(init[4:])
+    lib.lib.xmss_mi355x_rng_seed(g["rng_seed"])
+    rc, pk, sk = lib.keypair(g["oid"], mt, sk_init=init)
+    assert rc == 0
+    assert pk.hex() == g["pk"]
+    assert sk.hex() == g["sk"]
+    for i, row in enumerate(g["signs"]):
+        rc, sm, sk = lib.sign(sk, W.fixed_message(W.SEED + 952, i), mt)
+        assert rc == row["rc"]
+        assert dg(sm) == row["sm"], "sign %d" % i
+        assert dg(sk) == row["sk"], "sign %d sk" % i
+
+
 def test_bds_xmss_sha2_16_256_keygen_and_signs():
     """XMSS-SHA2_16_256 (65,536 leaves): the BDS keypair and 16 signatures."""
     _replay(product(1), golden("bds_xmss_sha2_16_256.json"), 0)

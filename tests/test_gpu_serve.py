@@ -184,3 +184,92 @@ def test_serve_budget_counts_resident_keys():
     got = [L.xmss_mi355x_server_add_key(srv, mt, buf(sk), buf(pk)) for (mt, oid, p, pk, sk) in keys]
     L.xmss_mi355x_server_destroy(srv)
     assert got == [0, 1, 2, 3, 4, -1]
+
+
+def test_serve_config5_sets_equal_reference():
+    """The serving workload's three parameter sets -- XMSS-SHA2_16_256,
+    XMSSMT-SHA2_20/4_256, XMSSMT-SHA2_40/4_256 -- three keys each, signing
+    log-uniform 32 B - 4 KB messages in order (one key of each XMSS^MT set
+    past its first layer-0 subtree switch), interleaved with verifications.
+    Every served signature and every final sk equals the reference's BDS core
+    (tests/golden/serve_sets.json, written by xmss_core_fast.c).  The last
+    key of each set is added while the others' requests are in flight."""
+    import workloads as W
+    from common import dg, golden
+    g = golden("serve_sets.json")
+    lib = product(1)
+    L = lib.lib
+    _server_api(L)
+    keys = []  # (mt, p, pk, sk, row)
+    for name, s in g.items():
+        mt, oid = s["mt"], s["oid"]
+        p = lib.params(oid, mt)
+        for row in s["keys"]:
+            lib.lib.xmss_mi355x_rng_seed(row["rng_seed"])
+            rc, pk, sk = lib.keypair(oid, mt, sk_init=bytes(4 + p.sk_bytes))
+            assert rc == 0
+            assert pk.hex() == row["pk"] and dg(sk) == row["sk"], name
+            keys.append((mt, p, pk, sk, row))
+    srv = L.xmss_mi355x_server_create(1 << 30, 64)
+    assert srv
+    late = [i for i in range(len(keys)) if i % 3 == 2]
+    kid = {}
+    for i, (mt, p, pk, sk, row) in enumerate(keys):
+        if i not in late:
+            kid[i] = L.xmss_mi355x_server_add_key(srv, mt, buf(sk), buf(pk))
+            assert kid[i] >= 0
+    reqs = []
+
+    def submit(i, j):
+        m = W.message(keys[i][4]["msg_seed"], j)
+        reqs.append((i, j, m, L.xmss_mi355x_server_sign(srv, kid[i], m, len(m))))
+
+    early = [i for i in kid]
+    for j in range(8):
+        for i in early:
+            submit(i, j)
+    for i in late:  # added while the requests above run
+        mt, p, pk, sk, row = keys[i]
+        kid[i] = L.xmss_mi355x_server_add_key(srv, mt, buf(sk), buf(pk))
+        assert kid[i] >= 0
+    for i in late:
+        for j in range(8):
+            submit(i, j)
+    jmax = max(k[4]["count"] for k in keys)
+    for j in range(8, jmax):
+        for i in range(len(keys)):
+            if j < keys[i][4]["count"]:
+                submit(i, j)
+    done = {}
+    for i, j, m, rid in reqs:
+        p = keys[i][1]
+        out = buf(p.sig_bytes + len(m))
+        olen = C.c_ulonglong(0)
+        rc = L.xmss_mi355x_server_result(srv, rid, out, C.byref(olen), None)
+        row = keys[i][4]["signs"][j]
+        assert rc == row["rc"] == 0, "key %d sign %d" % (i, j)
+        sm = raw(out, olen.value)
+        assert dg(sm) == row["sm"], "key %d sign %d" % (i, j)
+        done[(i, j)] = sm
+    # verifications of served signatures (and one corrupted), as requests
+    vreqs = []
+    for i in range(len(keys)):
+        for j in (0, 5, keys[i][4]["count"] - 1):
+            sm = done[(i, j)]
+            vreqs.append((i, j, True, L.xmss_mi355x_server_verify(srv, kid[i], sm, len(sm))))
+        bad = bytearray(done[(i, 3)])
+        bad[keys[i][1].index_bytes + 9] ^= 2
+        vreqs.append((i, 3, False, L.xmss_mi355x_server_verify(srv, kid[i], bytes(bad), len(bad))))
+    for i, j, ok, rid in vreqs:
+        out = buf(len(done[(i, j)]))
+        olen = C.c_ulonglong(0)
+        rc = L.xmss_mi355x_server_result(srv, rid, out, C.byref(olen), None)
+        if ok:
+            assert rc == 0 and raw(out, olen.value) == W.message(keys[i][4]["msg_seed"], j)
+        else:
+            assert rc == -1 and olen.value == 0
+    for i, (mt, p, pk, sk, row) in enumerate(keys):
+        skb = buf(4 + p.sk_bytes)
+        assert L.xmss_mi355x_server_get_sk(srv, kid[i], skb) == 0
+        assert dg(raw(skb, 4 + p.sk_bytes)) == row["signs"][-1]["sk"], "key %d final sk" % i
+    L.xmss_mi355x_server_destroy(srv)
