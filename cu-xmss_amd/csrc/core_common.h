@@ -71,8 +71,15 @@ int guarded(Fn&& f) {
     return XMSS_ENODEV;
 }
 
-// Shared verification (xmss_commons.c:136-215).
+// Return code of the allocation-free entry point for a request larger than
+// its static device scratch.
+constexpr int XMSS_ETOOBIG = -4;
+
+[[noreturn]] void fatal(const char* what);
+
+// Shared verification (xmss_commons.c:136-215); no_alloc: the
+// allocation-free form (xmss_mi355x_sign_open_static).
 int core_sign_open(const xmss_params* p, unsigned char* m, unsigned long long* mlen, const unsigned char* sm,
-                   unsigned long long smlen, const unsigned char* pk);
+                   unsigned long long smlen, const unsigned char* pk, bool no_alloc);
 
 }  // namespace xm

@@ -393,6 +393,13 @@ public:
         uint64_t mlen;
     };
     void verify_many(int fam, int count, const VerifyJob* jobs, int* ok);
+    // One verification with no device allocation: every buffer lives in a
+    // fixed device array of static_capacity() bytes per GPU (pk_root must
+    // point at root || PUB_SEED).  Returns 0 (ok set), or -4 when the request
+    // needs more than the array holds (static_bytes).
+    int verify_static(int fam, const VerifyJob& job, int* ok);
+    static size_t static_capacity();
+    static size_t static_bytes(int fam, unsigned th, unsigned d, uint64_t mlen);
     // Jobs may mix parameter shapes of one hash family; their keys are pool
     // slots of that family (BdsJob::key).
     void bds_sign_batch(int fam, int count, BdsJob* jobs);

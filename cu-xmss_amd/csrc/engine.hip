@@ -257,8 +257,19 @@ struct KeyPool {
     size_t cached = 0;
 };
 
+// The allocation-free verification's working memory (verify_static): one
+// file-scope device array per GPU, holding every buffer of one call -- staged
+// inputs, key context, pre-images, digests, roots.  Its address comes from
+// hipGetSymbolAddress once per device (Engine::get); host code uses only that
+// address, never the array's name.
+constexpr size_t kStaticScratch = 16u << 20;
+namespace noalloc {
+static __device__ __align__(256) uint8_t scratch[kStaticScratch];
+}
+
 struct Engine::Impl {
     Launcher L;
+    void* static_scratch = nullptr;  // this device's noalloc::scratch
     KeyPool pool[F_COUNT];
     const void* kp_override = nullptr;  // pool in use, or null for `keys`
     BdsBufs* bds = nullptr;             // buffers of bds_sign_batch
@@ -314,6 +325,11 @@ Engine* Engine::get() {
     Engine* e = nullptr;
     try {
         e = new Engine();
+        // set-up, once per device with the device current: the address of
+        // this device's copy of the allocation-free path's scratch array
+        void* ptr = nullptr;
+        HIPCK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc::scratch)));
+        e->impl_->static_scratch = ptr;
     } catch (const DeviceError&) {
         return nullptr;
     }
@@ -1845,6 +1861,111 @@ void Engine::verify_many(int fam, int count, const VerifyJob* jobs, int* ok) {
         const VerifyJob& v = jobs[ord[j]];
         ok[ord[j]] = !memcmp(h + (size_t)(base[v.d - 1] + j) * n, v.pk_root, n);
     }
+}
+
+// Layout of one verification in the static scratch (256-byte aligned parts).
+struct StaticLayout {
+    size_t keys, seed, md, it, blob, sig, auth, roots, dig, lat, spec, total;
+};
+static StaticLayout static_layout(FamOps* o, int fam, unsigned th, unsigned d, uint64_t mlen) {
+    const unsigned n = fam_n(fam), len = fam_len(fam), pad = fam_pad(fam);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    StaticLayout t;
+    t.keys = 0;
+    t.seed = al(o->key_bytes());
+    t.md = t.seed + al(3 * n);
+    t.it = t.md + al(sizeof(DMsg));
+    t.blob = t.it + al(sizeof(DLeaf) * d);
+    t.sig = t.blob + al(((pad + 3 * n + mlen + 15) & ~(uint64_t)15) + kMsgSlack);
+    t.auth = t.sig + al((size_t)d * len * n);
+    t.roots = t.auth + al((size_t)d * th * n);
+    t.dig = t.roots + al((size_t)d * n);
+    t.lat = t.dig + al(n);
+    t.spec = t.lat + al((size_t)d * o->lat_bytes(th));
+    t.total = t.spec + (d > 1 ? al((size_t)(d - 1) * len * 16 * n) : 0);
+    return t;
+}
+
+size_t Engine::static_capacity() { return kStaticScratch; }
+
+size_t Engine::static_bytes(int fam, unsigned th, unsigned d, uint64_t mlen) {
+    return static_layout(ops_of(fam), fam, th, d, mlen).total;
+}
+
+// One verification (the verify_many sequence for a single request) with all
+// of its device memory in the static scratch: the inputs are copied from the
+// caller's (pageable) buffers, the key context is prepared into the scratch
+// instead of the key pool, and no allocation is made.  The blob region is
+// zeroed first by a kernel (the H_msg padding reads past the message).
+int Engine::verify_static(int fam, const VerifyJob& v, int* ok) {
+    Impl* I = impl_;
+    FamOps* o = ops_of(fam);
+    const unsigned n = fam_n(fam), len = fam_len(fam), pad = fam_pad(fam);
+    const unsigned d = v.d, th = v.th;
+    const StaticLayout t = static_layout(o, fam, th, d, v.mlen);
+    if (t.total > kStaticScratch || d > 16) return -4;
+    uint8_t* S = (uint8_t*)I->static_scratch;
+    if (!S) fatal("verify_static: no static scratch on this device");
+    I->L.s = I->L.s_main;
+    const size_t pb = pad + 3 * n;
+    zero_async(I->L, S + t.blob, t.sig - t.blob);
+    // one header copy: PUB_SEED, the H_msg descriptor, the layer items and
+    // the H_msg prefix (toByte(2, pad) || R || root || toByte(idx, n))
+    uint8_t hdr[4096];
+    const size_t hb = t.blob + pb - t.seed;
+    if (hb > sizeof hdr) fatal("verify_static: header too large");
+    memset(hdr, 0, hb);
+    auto at = [&](size_t off) { return hdr + (off - t.seed); };  // the header byte of a scratch offset
+    memcpy(at(t.seed) + 2 * n, v.pk_root + n, n);  // PUB_SEED: pk_root points at root || PUB_SEED
+    uint64_t idx = 0;
+    for (unsigned b = 0; b < v.index_bytes; ++b) idx = (idx << 8) | v.sig[b];
+    const DMsg md{0u, 0u, idx, 0u, (uint32_t)(pb + v.mlen), 0u};
+    memcpy(at(t.md), &md, sizeof md);
+    uint64_t x = idx;
+    for (unsigned L = 0; L < d; ++L) {
+        const uint32_t leaf = (uint32_t)(x & ((1ull << th) - 1));
+        x >>= th;
+        const DLeaf q{0u, L, (uint32_t)(x >> 32), (uint32_t)x, leaf, th, LAT_ALL_STEPS, L * th};
+        memcpy(at(t.it) + (size_t)L * sizeof(DLeaf), &q, sizeof q);
+    }
+    uint8_t* pre = at(t.blob);
+    pre[pad - 1] = 2;
+    memcpy(pre + pad, v.sig + v.index_bytes, n);
+    memcpy(pre + pad + n, v.pk_root, n);
+    x = idx;
+    for (int b = 0; b < 8; ++b) {
+        pre[pad + 3 * n - 1 - b] = (uint8_t)(x & 0xff);
+        x >>= 8;
+    }
+    I->h2d(S + t.seed, hdr, hb);
+    I->h2d(S + t.blob + pb, v.m, v.mlen);
+    // WOTS signatures and auth paths of the d layers, straight out of sm
+    const uint8_t* l0 = v.sig + v.index_bytes + n;
+    const size_t lb = (size_t)(len + th) * n;
+    HIPCK(hipMemcpy2DAsync(S + t.sig, (size_t)len * n, l0, lb, (size_t)len * n, d, hipMemcpyHostToDevice, I->L.s));
+    if (th)
+        HIPCK(hipMemcpy2DAsync(S + t.auth, (size_t)th * n, l0 + (size_t)len * n, lb, (size_t)th * n, d,
+                               hipMemcpyHostToDevice, I->L.s));
+    o->prepare(I->L, S + t.seed, 1, S + t.keys);
+    uint32_t base[17] = {0};
+    for (unsigned L = 0; L <= d; ++L) base[L] = L;
+    o->verify_pre(I->L, S + t.keys, (const DMsg*)(S + t.md), 1, pad, S + t.blob, S + t.dig,
+                  (const DLeaf*)(S + t.it), d, th, S + t.lat);
+    const DLeaf* it = (const DLeaf*)(S + t.it);
+    if (d > 1) {
+        o->verify_layers(I->L, it, 1, base, d, 0, 1, S + t.dig, S + t.sig, S + t.auth, th, S + t.lat, S + t.roots,
+                         S + t.spec, d - 1);
+        o->verify_layers(I->L, it, 1, base, d, 1, d, S + t.dig, S + t.sig, S + t.auth, th, S + t.lat, S + t.roots,
+                         S + t.spec, 0);
+    } else {
+        o->verify_layers(I->L, it, 1, base, d, 0, d, S + t.dig, S + t.sig, S + t.auth, th, S + t.lat, S + t.roots,
+                         nullptr, 0);
+    }
+    uint8_t root[64];
+    I->d2h(root, S + t.roots + (size_t)(d - 1) * n, n);
+    I->sync();
+    *ok = !memcmp(root, v.pk_root, n);
+    return 0;
 }
 
 int Engine::pool_slot(int fam, const KeySeeds& s) {

@@ -40,6 +40,22 @@ int xmss_mi355x_device_ok(void);
 size_t xmss_mi355x_set_device_limit(size_t bytes);
 size_t xmss_mi355x_device_bytes(void);
 
+/* Verification that allocates no device memory.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Same
+ * arguments, return values and m / *mlen behaviour as xmss_sign_open
+ * (mt = 0) or xmssmt_sign_open (mt = 1), computed by the same kernels, but
+ * every device buffer of the call -- staged inputs, key context, pre-images,
+ * digests, roots -- lives in one fixed 16 MiB device array per GPU that the
+ * library holds until the process ends; the inputs are copied from the
+ * caller's buffers as they are.  A request needing more than the array holds
+ * (messages of about 15 MB and more) returns -4 (with *mlen = 0 and a message
+ * on stderr).  Calls are serialized by the library; the array keeps the last
+ * call's contents, and the blob region is cleared before each use. */
+int xmss_mi355x_sign_open_static(int mt, unsigned char *m, unsigned long long *mlen,
+                                 const unsigned char *sm, unsigned long long smlen,
+                                 const unsigned char *pk);
+
 /* Per-kernel timing with HIP events on the library's stream. */
 void xmss_mi355x_profile(int on);
 void xmss_mi355x_reset_stats(void);
