@@ -44,6 +44,11 @@ struct LatLayout {
     static size_t bytes(uint32_t th) { return align(hpre + sizeof(HPre) * (F::LEN - 1 + th)); }
 };
 
+// A compiler barrier for memory operations: loads issued before it stay
+// before it, so a prefetch cannot be sunk next to its first use (the
+// hardware waits for a load only where its registers are read).
+__device__ __forceinline__ void keep_loads_ahead() { asm volatile("" ::: "memory"); }
+
 // L-tree merge t (0 .. LEN-2, height-major) -> (height, index).
 template <class F>
 __device__ __forceinline__ void ltree_merge(uint32_t t, uint32_t& height, uint32_t& index) {
@@ -489,9 +494,11 @@ __global__ __launch_bounds__(LtreeThreads<F>::value) void k_lat_ltree(const uint
 // reverse, so that the long waves do not all land on the same SIMD.
 constexpr uint32_t LATB_CHAINS_WG = 256;
 template <class F>
+// (scratch is not __restrict__: restrict-qualified loads count as invariant
+// and would move past keep_loads_ahead)
 __global__ __launch_bounds__(LATB_CHAINS_WG) void k_latb_chains(uint32_t n, const uint8_t* __restrict__ msgs,
                                                                  const uint8_t* __restrict__ sigs,
-                                                                 const uint8_t* __restrict__ scratch, size_t stride,
+                                                                 const uint8_t* scratch, size_t stride,
                                                                  uint8_t* __restrict__ ends) {
     using FPre = typename F::FPre;
     __shared__ uint32_t hist[16];
@@ -523,11 +530,17 @@ __global__ __launch_bounds__(LATB_CHAINS_WG) void k_latb_chains(uint32_t n, cons
     typename F::Node x;
     F::load(x, sigs + (size_t)g * F::kN);
     const FPre* fp = reinterpret_cast<const FPre*>(scratch + (size_t)r * stride + LatLayout<F>::fpre) + c * 15;
-    FPre cur = fp[s0 < 15 ? s0 : 14];
-    for (uint32_t s = s0; s < 15; ++s) {
-        const FPre nxt = fp[s + 1 < 15 ? s + 1 : 14];
-        F::fin_f(x, cur, x);
-        cur = nxt;
+    // two pre-image registers in turn: the next step's pre-image is loaded
+    // before the current step hashes and is waited for only at its use
+    FPre a = fp[s0 < 15 ? s0 : 14], b;
+    for (uint32_t s = s0; s < 15; s += 2) {
+        b = fp[s + 1 < 15 ? s + 1 : 14];
+        keep_loads_ahead();
+        F::fin_f(x, a, x);
+        if (s + 1 >= 15) break;
+        a = fp[s + 2 < 15 ? s + 2 : 14];
+        keep_loads_ahead();
+        F::fin_f(x, b, x);
     }
     F::store(x, ends + (size_t)g * F::kN);
 }
@@ -563,8 +576,7 @@ __global__ __launch_bounds__(WG) void k_latb_ltree(uint32_t n, uint32_t l, uint3
 template <class F>
 __global__ __launch_bounds__(WG) void k_latb_auth(const DLeaf* __restrict__ reqs, uint32_t n,
                                                    const uint8_t* __restrict__ leaves, uint32_t leaf_stride,
-                                                   const uint8_t* __restrict__ auth,
-                                                   const uint8_t* __restrict__ scratch, size_t stride,
+                                                   const uint8_t* auth, const uint8_t* scratch, size_t stride,
                                                    uint8_t* __restrict__ out) {
     using HPre = typename F::HPre;
     const uint32_t r = blockIdx.x * WG + threadIdx.x;
@@ -575,12 +587,24 @@ __global__ __launch_bounds__(WG) void k_latb_auth(const DLeaf* __restrict__ reqs
     F::load(node, leaves + (size_t)r * leaf_stride * F::kN);
     uint32_t idx = q.leaf;
     const uint8_t* ap = auth + (size_t)q.p2 * F::kN;
-    for (uint32_t i = 0; i < q.p0; ++i) {
-        typename F::Node sib;
-        F::load(sib, ap + (size_t)i * F::kN);
-        if (idx & 1) F::fin_h(node, hp[F::LEN - 1 + i], sib, node);
-        else F::fin_h(node, hp[F::LEN - 1 + i], node, sib);
+    // each level's sibling and pre-image are loaded one level ahead
+    const uint32_t levels = q.p0;
+    typename F::Node sib, sib_next;
+    HPre h, h_next;
+    if (levels) {
+        F::load(sib, ap);
+        h = hp[F::LEN - 1];
+    }
+    for (uint32_t i = 0; i < levels; ++i) {
+        const uint32_t j = i + 1 < levels ? i + 1 : i;
+        F::load(sib_next, ap + (size_t)j * F::kN);
+        h_next = hp[F::LEN - 1 + j];
+        keep_loads_ahead();
+        if (idx & 1) F::fin_h(node, h, sib, node);
+        else F::fin_h(node, h, node, sib);
         idx >>= 1;
+        sib = sib_next;
+        h = h_next;
     }
     F::store(node, out + (size_t)r * F::kN);
 }

@@ -116,6 +116,9 @@ int xo_sign_open(const xo_params *p, unsigned char *m, unsigned long long *mlen,
 void xo_treehash(const xo_params *p, unsigned char *root, unsigned char *auth_path,
                  const unsigned char *sk_seed, const unsigned char *pub_seed,
                  uint32_t leaf_idx, const uint32_t subtree_addr[8]);
+void xo_subtree_root(const xo_params *p, unsigned char *root, const unsigned char *sk_seed,
+                     const unsigned char *pub_seed, uint32_t leaf0, unsigned height,
+                     const uint32_t subtree_addr[8]);
 int xo_plain_seed_keypair(const xo_params *p, unsigned char *pk, unsigned char *sk,
                           const unsigned char *seed);
 int xo_plain_keypair(const xo_params *p, unsigned char *pk, unsigned char *sk);

@@ -645,6 +645,46 @@ void xo_treehash(const xo_params *p, unsigned char *root, unsigned char *auth_pa
     free(stack);
 }
 
+/* The root of the 2^height leaves leaf0 .. leaf0 + 2^height - 1 (leaf0 a
+ * multiple of 2^height) of the subtree at subtree_addr: treehash
+ * (xmss_core.c:19-86) over that aligned range, with the same node
+ * addresses the whole tree's treehash gives them.  Lets a CPU keygen split
+ * its leaves across processes; with leaf0 = 0 and height = tree_height it is
+ * xo_treehash's root. */
+void xo_subtree_root(const xo_params *p, unsigned char *root, const unsigned char *sk_seed,
+                     const unsigned char *pub_seed, uint32_t leaf0, unsigned height,
+                     const uint32_t subtree_addr[8])
+{
+    unsigned char *stack = (unsigned char *)malloc((height + 1) * p->n);
+    unsigned int heights[64];
+    unsigned int offset = 0;
+    uint32_t idx;
+    uint32_t ots_addr[8] = {0}, ltree_addr[8] = {0}, node_addr[8] = {0};
+
+    copy_subtree(ots_addr, subtree_addr);
+    copy_subtree(ltree_addr, subtree_addr);
+    copy_subtree(node_addr, subtree_addr);
+    ots_addr[3] = ADDR_OTS;
+    ltree_addr[3] = ADDR_LTREE;
+    node_addr[3] = ADDR_HASHTREE;
+    for (idx = leaf0; idx < leaf0 + (1u << height); idx++) {
+        ltree_addr[4] = idx;
+        ots_addr[4] = idx;
+        xo_gen_leaf_wots(p, stack + offset * p->n, sk_seed, pub_seed, ltree_addr, ots_addr);
+        offset++;
+        heights[offset - 1] = 0;
+        while (offset >= 2 && heights[offset - 1] == heights[offset - 2]) {
+            node_addr[5] = heights[offset - 1];
+            node_addr[6] = idx >> (heights[offset - 1] + 1);
+            xo_thash_h(p, stack + (offset - 2) * p->n, stack + (offset - 2) * p->n, pub_seed, node_addr);
+            offset--;
+            heights[offset - 1]++;
+        }
+    }
+    memcpy(root, stack, p->n);
+    free(stack);
+}
+
 /* xmssmt_core_seed_keypair: xmss_core.c:134-161 */
 int xo_plain_seed_keypair(const xo_params *p, unsigned char *pk, unsigned char *sk,
                           const unsigned char *seed)

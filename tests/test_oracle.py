@@ -178,6 +178,15 @@ def test_treehash_against_brute_force_tiny_trees():
             assert raw(root, 32) == levels[-1][0]
             want = b"".join(levels[h][(leaf_idx >> h) ^ 1] for h in range(fh))
             assert raw(auth, 32 * fh) == want
+        # every aligned subtree's root (the CPU baseline's split keygen) is
+        # the node of the directly built tree above it
+        L.xo_subtree_root.argtypes = [C.POINTER(Params), C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                      C.c_uint, C.c_void_p]
+        for h in range(fh + 1):
+            for i in range(1 << (fh - h)):
+                out = buf(32)
+                L.xo_subtree_root(C.byref(p), out, buf(ss), buf(ps), i << h, h, addr([0] * 8))
+                assert raw(out, 32) == levels[h][i], (fh, h, i)
 
 
 # ---------------------------------------------------------------- plain core
