@@ -761,7 +761,7 @@ def main():
     ap.add_argument("--mixed", type=int, default=1024, help="keys per parameter set of the mixed batch (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4000.0, help="offered requests/s of the serving run (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
-    ap.add_argument("--serve-budget", type=int, default=4 << 20, help="HBM budget of the serving run, bytes")
+    ap.add_argument("--serve-budget", type=int, default=64 << 20, help="HBM budget of the serving run, bytes")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="MiB of one device-to-device copy, run before the keygens (HBM counter calibration)")
     args = ap.parse_args()

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <condition_variable>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -49,6 +50,43 @@ struct DeviceError : std::runtime_error {
 // state) and the limit on them (0: none); set_device_limit returns the old one.
 size_t device_bytes();
 size_t set_device_limit(size_t bytes);
+
+// One device allocation carved up by a thread-safe first-fit allocator
+// (256-byte granules): the serving loop's whole device memory, so that its
+// HBM budget bounds real memory.  alloc returns nullptr when no free range
+// is large enough.
+class DeviceArena {
+public:
+    explicit DeviceArena(size_t bytes);  // throws DeviceError when the device cannot hold it
+    // An arena over memory the caller owns (a block of another arena).
+    DeviceArena(void* base, size_t bytes);
+    ~DeviceArena();
+    void* alloc(size_t bytes);
+    void release(void* p);
+    bool owns(const void* p) const;
+    void* base() const { return base_; }
+    size_t size() const { return size_; }
+    size_t in_use() const;
+    size_t peak() const;
+
+private:
+    uint8_t* base_ = nullptr;
+    size_t size_ = 0;
+    bool owner_ = true;
+    mutable std::mutex mu_;
+    std::map<size_t, size_t> free_;  // offset -> length
+    std::map<size_t, size_t> used_;  // offset -> length
+    size_t in_use_ = 0, peak_ = 0;
+};
+
+// While an ArenaScope is alive, the device allocations this thread's engine
+// calls make come from the arena (exact sizes, no spare growth) instead of
+// hipMalloc, and the arena counts them; frees return them.
+struct ArenaScope {
+    explicit ArenaScope(DeviceArena* a);
+    ~ArenaScope();
+    DeviceArena* prev;
+};
 
 // VALU issue-rate probe (probe.hip): lane-ops/s and kernel ms of one launch
 // of instruction mix `kind` (0: SHA-256 chain step; 1-5: single classes).
@@ -256,6 +294,23 @@ public:
     size_t pool_cached(int fam) const;
     void pool_use(int fam);
     size_t pool_key_bytes(int fam) const;
+    static size_t pool_key_bytes_of(int fam);
+    // Synchronous device-to-device copy / zero fill (outside any stream order).
+    static void copy_device(void* dst, const void* src, size_t bytes);
+    static void clear_device(void* p, size_t bytes);
+    // Frees every device buffer the engine holds -- workspaces, spare batches,
+    // key pools -- and zeroes the pools first (a private engine handed to a
+    // server, whose device memory must be its arena alone).
+    void trim();
+    // Key contexts of `fam` kept by the caller (the serving loop's key
+    // tables): the request `key` fields index `table` until the next
+    // set_keys or pool_use.  prepare_keys writes the contexts of `nkeys`
+    // keys (3n seed bytes each) into `table` at slot `first` (synchronous).
+    void use_keys(int fam, const void* table);
+    void prepare_keys(int fam, const KeySeeds* seeds, int nkeys, void* table, int first);
+    // Frees every workspace buffer of the engine (not its key pool or
+    // batches): under an ArenaScope, nothing is held between calls.
+    void release_workspace();
 
     // Full subtrees of height th, one per run (leaf0 must be 0).  All node
     // layers stay on the device for fetch_nodes until the next call.
@@ -403,6 +458,11 @@ public:
     // Jobs may mix parameter shapes of one hash family; their keys are pool
     // slots of that family (BdsJob::key).
     void bds_sign_batch(int fam, int count, BdsJob* jobs);
+    // Device bytes one BDS signature / one verification takes in
+    // bds_sign_batch / verify_many (its share of a launch group, rounding
+    // included), by the formulas those calls allocate with.
+    static size_t bds_bytes(int fam, const BdsJob& job);
+    static size_t verify_bytes(int fam, unsigned th, unsigned d, uint64_t mlen);
 
     // Per-kernel HIP-event timing (off by default).
     void set_profiling(bool on);

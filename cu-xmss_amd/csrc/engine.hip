@@ -35,7 +35,86 @@ namespace xm {
 static std::atomic<size_t> g_dev_bytes{0};
 static std::atomic<size_t> g_dev_limit{0};
 
+// The arena this thread's device allocations come from (ArenaScope), if any.
+static thread_local DeviceArena* t_arena = nullptr;
+
+ArenaScope::ArenaScope(DeviceArena* a) : prev(t_arena) { t_arena = a; }
+ArenaScope::~ArenaScope() { t_arena = prev; }
+
+DeviceArena::DeviceArena(size_t bytes) {
+    size_ = (std::max<size_t>(bytes, 256) + 255) & ~(size_t)255;
+    base_ = (uint8_t*)dev_malloc(size_);  // counted once, as a whole
+    free_[0] = size_;
+}
+DeviceArena::DeviceArena(void* base, size_t bytes) : base_((uint8_t*)base), size_(bytes & ~(size_t)255), owner_(false) {
+    free_[0] = size_;
+}
+DeviceArena::~DeviceArena() {
+    if (!owner_) return;
+    try {
+        dev_free(base_, size_);
+    } catch (const DeviceError&) {
+    }
+}
+void* DeviceArena::alloc(size_t bytes) {
+    bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        const size_t off = it->first, len = it->second;
+        free_.erase(it);
+        if (len > bytes) free_[off + bytes] = len - bytes;
+        used_[off] = bytes;
+        in_use_ += bytes;
+        peak_ = std::max(peak_, in_use_);
+        return base_ + off;
+    }
+    return nullptr;
+}
+void DeviceArena::release(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    const size_t off = (size_t)((uint8_t*)p - base_);
+    auto u = used_.find(off);
+    if (u == used_.end()) return;
+    size_t start = off, len = u->second;
+    in_use_ -= len;
+    used_.erase(u);
+    auto nx = free_.lower_bound(start);
+    if (nx != free_.end() && nx->first == start + len) {  // merge with the next free range
+        len += nx->second;
+        nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {  // and with the previous one
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == start) {
+            start = pv->first;
+            len += pv->second;
+            free_.erase(pv);
+        }
+    }
+    free_[start] = len;
+}
+bool DeviceArena::owns(const void* p) const { return p >= base_ && p < base_ + size_; }
+size_t DeviceArena::in_use() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    return in_use_;
+}
+size_t DeviceArena::peak() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    return peak_;
+}
+
 void* dev_malloc(size_t bytes) {
+    if (DeviceArena* a = t_arena) {
+        void* p = a->alloc(bytes);
+        if (!p) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "%zu device bytes do not fit in the serving budget's free space", bytes);
+            fatal(msg);
+        }
+        return p;
+    }
     const size_t lim = g_dev_limit.load();
     const size_t now = g_dev_bytes.fetch_add(bytes) + bytes;
     if (lim && now > lim) {
@@ -58,6 +137,10 @@ void* dev_malloc(size_t bytes) {
 
 void dev_free(void* p, size_t bytes) {
     if (!p) return;
+    if (t_arena && t_arena->owns(p)) {
+        t_arena->release(p);
+        return;
+    }
     g_dev_bytes.fetch_sub(bytes);
     HIPCK(hipFree(p));
 }
@@ -137,6 +220,12 @@ struct DevBuf {
     template <class T = uint8_t>
     T* need(size_t bytes) {
         bytes = std::max<size_t>(bytes, 64);
+        if (bytes > cap && t_arena) {  // arena: exact sizes, freed after each call
+            release();
+            cap = (bytes + 255) & ~(size_t)255;
+            p = dev_malloc(cap);
+            return (T*)p;
+        }
         if (bytes > cap) {
             release();
             size_t c = std::max(bytes, cap + cap / 2);
@@ -180,6 +269,7 @@ struct PinnedBuf {
 
 struct BdsBufs;
 struct VerifyBufs;
+void destroy_batch(Engine::Batch* b);
 
 // Keyed 128-bit digest of a key's seed bytes: the pool finds a key's slot by
 // it, so no seed bytes are kept on the host.  Two SipHash-2-4 values
@@ -281,6 +371,7 @@ struct Engine::Impl {
     // other batches' kernels run on s_main / s_aux
     hipStream_t s_copy = nullptr;
     uint64_t epoch = 1;  // pool_begin: slots used since then are not evicted
+    bool external = false;  // kp_override is a caller's key table (use_keys)
     int key_fam = -1;
     int nkeys = 0;
     void sync() {
@@ -453,6 +544,7 @@ void Engine::set_keys(int fam, int nkeys, const KeySeeds* seeds) {
     I->key_fam = fam;
     I->nkeys = nkeys;
     I->kp_override = nullptr;
+    I->external = false;
     HIPCK(hipStreamSynchronize(I->L.s));  // the copy has read h: clear it
     volatile uint8_t* vh = h.data();
     for (size_t i = 0; i < h.size(); ++i) vh[i] = 0;
@@ -549,9 +641,54 @@ size_t Engine::pool_cached(int fam) const { return impl_->pool[fam].cached; }
 void Engine::pool_use(int fam) {
     impl_->key_fam = fam;
     impl_->kp_override = impl_->pool[fam].p;
+    impl_->external = false;
+}
+
+// The keys of a launch sequence of `fam`: the caller's table when one was
+// given for this family (use_keys), else the engine's pool.
+static void keys_for(Engine* e, Engine::Impl* I, int fam) {
+    if (!(I->external && I->key_fam == fam)) e->pool_use(fam);
 }
 
 size_t Engine::pool_key_bytes(int fam) const { return ops_of(fam)->key_bytes(); }
+size_t Engine::pool_key_bytes_of(int fam) { return ops_of(fam)->key_bytes(); }
+
+void Engine::copy_device(void* dst, const void* src, size_t bytes) {
+    if (bytes) HIPCK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+}
+
+void Engine::clear_device(void* p, size_t bytes) {
+    if (p && bytes) {
+        HIPCK(hipMemset(p, 0, bytes));
+        HIPCK(hipDeviceSynchronize());
+    }
+}
+
+void Engine::use_keys(int fam, const void* table) {
+    impl_->key_fam = fam;
+    impl_->kp_override = table;
+    impl_->external = true;
+}
+
+void Engine::prepare_keys(int fam, const KeySeeds* seeds, int nkeys, void* table, int first) {
+    Impl* I = impl_;
+    FamOps* o = ops_of(fam);
+    const unsigned n = fam_n(fam);
+    std::vector<uint8_t> h((size_t)nkeys * 3 * n, 0);
+    for (int i = 0; i < nkeys; ++i) {
+        uint8_t* d = &h[(size_t)i * 3 * n];
+        if (seeds[i].sk_seed) memcpy(d, seeds[i].sk_seed, n);
+        if (seeds[i].sk_prf) memcpy(d + n, seeds[i].sk_prf, n);
+        if (seeds[i].pub_seed) memcpy(d + 2 * n, seeds[i].pub_seed, n);
+    }
+    uint8_t* ds = I->seeds.need(h.size());
+    I->h2d(ds, h.data(), h.size());
+    o->prepare(I->L, ds, nkeys, (uint8_t*)table + (size_t)first * o->key_bytes());
+    zero_async(I->L, ds, (h.size() + 3) & ~(size_t)3);
+    I->sync();
+    volatile uint8_t* vh = h.data();
+    for (size_t i = 0; i < h.size(); ++i) vh[i] = 0;
+}
 
 static std::vector<DRun> pack_runs(int n, const LeafRun* runs) {
     std::vector<DRun> v(n);
@@ -1532,6 +1669,10 @@ void Engine::free_batch(Batch* b) {
         impl_->spare[b->sign] = b;
         return;
     }
+    destroy_batch(b);
+}
+
+void destroy_batch(Engine::Batch* b) {
     b->drop_graph();
     if (b->done) HIPCK(hipEventDestroy(b->done));
     if (b->staged) HIPCK(hipEventDestroy(b->staged));
@@ -1583,7 +1724,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     BdsBufs& B = *I->bds;
     FamOps* o = ops_of(fam);
     const unsigned n = fam_n(fam), len = fam_len(fam), pad = fam_pad(fam);
-    pool_use(fam);
+    keys_for(this, I, fam);
     const void* dk = I->kp();
     // per-job layout: planned leaves, sk bytes, signature tail
     std::vector<DLeaf> ld;
@@ -1760,7 +1901,7 @@ void Engine::verify_many(int fam, int count, const VerifyJob* jobs, int* ok) {
     VerifyBufs& V = *I->ver;
     FamOps* o = ops_of(fam);
     const unsigned n = fam_n(fam), len = fam_len(fam), pad = fam_pad(fam);
-    pool_use(fam);
+    keys_for(this, I, fam);
     const void* dk = I->kp();
     // deepest first: the requests that still have a layer L are a prefix
     std::vector<int> ord(count);
@@ -1966,6 +2107,70 @@ int Engine::verify_static(int fam, const VerifyJob& v, int* ok) {
     I->sync();
     *ok = !memcmp(root, v.pk_root, n);
     return 0;
+}
+
+void Engine::release_workspace() {
+    Impl* I = impl_;
+    DevBuf* bufs[] = {&I->keys, &I->seeds, &I->desc, &I->ends, &I->tmp, &I->nodes, &I->a,
+                      &I->b,    &I->c,     &I->d,    &I->idx,  &I->lat, &I->tpre};
+    for (DevBuf* x : bufs) x->release();
+    if (I->bds) {
+        DevBuf* b[] = {&I->bds->io, &I->bds->leaf_arena, &I->bds->dig, &I->bds->scratch, &I->bds->lat_leaf,
+                       &I->bds->merge_arena};
+        for (DevBuf* x : b) x->release();
+    }
+    if (I->ver) {
+        DevBuf* v[] = {&I->ver->io, &I->ver->dig, &I->ver->roots, &I->ver->scratch, &I->ver->spec};
+        for (DevBuf* x : v) x->release();
+    }
+    // the keys of set_keys are gone, and a caller's table is not kept
+    I->kp_override = nullptr;
+    I->external = false;
+    I->key_fam = -1;
+}
+
+void Engine::trim() {
+    Impl* I = impl_;
+    release_workspace();
+    for (Batch*& b : I->spare) {
+        if (b) destroy_batch(b);  // a spare signing batch was scrubbed when it was freed
+        b = nullptr;
+    }
+    for (int f = 0; f < F_COUNT; ++f) {
+        KeyPool& P = I->pool[f];
+        if (!P.p) continue;
+        clear_device(P.p, P.cap * ops_of(f)->key_bytes());
+        dev_free(P.p, P.cap * ops_of(f)->key_bytes());
+        P = KeyPool{};
+    }
+}
+
+// A request's share of a launch group's buffers: its own parts, plus a
+// fixed allowance for the group's 256-byte region rounding, the message
+// slack and the minimum sizes (at most 8 KB per group, so one per request
+// covers it).
+static constexpr size_t kGroupAllowance = 8192;
+
+size_t Engine::bds_bytes(int fam, const BdsJob& j) {
+    FamOps* o = ops_of(fam);
+    const size_t n = fam_n(fam), len = fam_len(fam), pb = fam_pad(fam) + 3 * n;
+    const BdsShape& s = j.shape;
+    const size_t nw = 1 + (j.plan_wots ? j.nwots : 0);
+    const size_t io = sizeof(DLeaf) * j.nplan + sizeof(DMsg) + ((pb + j.mlen + 15) & ~(size_t)15) +
+                      sizeof(DLeaf) * nw + sizeof(DBds) + sizeof(DThash) * j.nmerge + 12 * (size_t)j.nmerge +
+                      n * j.nlit + ((s.sk_bytes + 7) & ~7ull) + n + len * n +
+                      ((size_t)s.sig_bytes - s.index_bytes - n - s.wots_sig_bytes);
+    return io + j.nplan * n + (size_t)j.nmerge * o->hpre_bytes() + (j.nplan + nw) * o->lat_bytes(0) + 5 * n +
+           kGroupAllowance;
+}
+
+size_t Engine::verify_bytes(int fam, unsigned th, unsigned d, uint64_t mlen) {
+    FamOps* o = ops_of(fam);
+    const size_t n = fam_n(fam), len = fam_len(fam), pb = fam_pad(fam) + 3 * n;
+    const size_t io = sizeof(DMsg) + ((pb + mlen + 15) & ~(uint64_t)15) + sizeof(DLeaf) * d + (size_t)d * len * n +
+                      (size_t)d * th * n;
+    const size_t spec = d > 1 ? (size_t)(d - 1) * len * 16 * n : 0;
+    return io + (size_t)d * n + n + (size_t)d * o->lat_bytes(th) + spec + kGroupAllowance;
 }
 
 int Engine::pool_slot(int fam, const KeySeeds& s) {

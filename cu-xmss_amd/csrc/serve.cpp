@@ -11,18 +11,24 @@
 // others are on the GPU joins the next launch.  Signatures of one key run in
 // arrival order (its sk is a sequential state); other keys are independent.
 //
-// Key state is resident in HBM for the lifetime of the key: the key context
-// (the PRF midstates of SK_SEED, SK_PRF, PUB_SEED) in the device key pool and
-// the serialized sk (index, seeds, BDS states, XMSS^MT WOTS signatures) in a
-// buffer of its own, which the state-update kernel advances in place.  The
-// server keeps a byte-identical host mirror of every sk (each signature's
-// result copy brings the updated bytes back): the host plans the next
-// signature's leaves from it, serves xmss_mi355x_server_get_sk from it, and
-// runs the last-index / exhausted path on it (then uploads it again).
-// Admission: adding a key reserves its resident bytes; every request
-// reserves its device working set (message, signature, planned leaves)
-// against the same fixed HBM budget and waits in FIFO order while the budget
-// is exhausted.
+// Device memory: the server holds exactly its HBM budget, as one arena
+// (DeviceArena) allocated at creation, and nothing else.  In it live the key
+// contexts (the PRF midstates of SK_SEED, SK_PRF, PUB_SEED: one table per
+// hash family, indexed by the key's slot), each key's serialized sk (index,
+// seeds, BDS states, XMSS^MT WOTS signatures), which the state-update kernel
+// advances in place, and the buffers of every launch group: the lanes' engines
+// allocate them inside the arena, exactly sized, while they execute a group,
+// and free them after it.  The server keeps a byte-identical host mirror of
+// every sk (each signature's result copy brings the updated bytes back): the
+// host plans the next signature's leaves from it, serves
+// xmss_mi355x_server_get_sk from it, and runs the last-index / exhausted path
+// on it (then uploads it again).
+// Admission: adding a key reserves its resident bytes; every request reserves
+// the device bytes its launches will allocate (its share of a launch group,
+// from its signature's leaf and merge plan or its verification's shape, by the
+// engine's own sizing) and waits in FIFO order until the budget left by the
+// keys and the running requests holds it.  A request larger than the budget
+// left by the keys returns -3.
 #include <string.h>
 
 #include <atomic>
@@ -33,6 +39,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <thread>
 #include <vector>
 
@@ -49,13 +56,30 @@ using Clock = std::chrono::steady_clock;
 // A lane waits at most this long for running requests to queue their device
 // calls before it launches what it has.
 constexpr std::chrono::microseconds kBatchWindow(200);
+// The largest subtree height of any parameter set (XMSS-*_20_*).
+constexpr unsigned kMaxTreeHeight = 20;
+// Device bytes of one key context's preparation (its seeds staged in a 256-byte granule).
+constexpr size_t kPrepareBytes = 4096;
+
+// Key contexts of the server's keys: one table per hash family in the arena.
+// Lanes hold `mu` shared while a launch group runs; a table that grows is
+// moved under it held exclusively.
+struct KeyTables {
+    std::shared_mutex mu;
+    struct T {
+        void* p = nullptr;
+        size_t cap = 0;  // slots
+        int n = 0;
+        int zero = -1;  // slot of the all-zero seeds (the wiped key of a last-index signature)
+    } t[F_COUNT];
+};
 
 // ---------------------------------------------------------------- dispatcher
 struct Call {
-    enum Kind { LEAVES, THASH, WOTS, HMSG, VERIFY, BDS } kind;
+    enum Kind { LEAVES, THASH, WOTS, HMSG, VERIFY, BDS, PREPARE } kind;
     Engine::BdsJob* job = nullptr;  // BDS
     int fam = 0;
-    int slot = 0;
+    int slot = 0;                   // key table slot
     unsigned n = 0;
     // LEAVES
     std::vector<LeafId> ids;
@@ -79,6 +103,10 @@ struct Call {
     const unsigned char* sm = nullptr;
     uint64_t smlen = 0;
     int ok = 0;
+    // PREPARE: the key context of `seeds` into table slot `slot`
+    KeySeeds seeds{};
+    // device bytes the call's launches take (its request's reservation)
+    size_t bytes = 0;
     // result
     unsigned char* out = nullptr;
     bool done = false;
@@ -88,18 +116,20 @@ struct Call {
 
 class Dispatcher {
 public:
-    // Lanes, each with its own thread and engine (HIP streams and buffers):
-    // lane 0 runs on the library's shared engine and takes every kind of call;
-    // the other signing lanes take BDS signatures, the verification lanes
-    // verifications, each on a private engine.  A lane launches what its
-    // queue holds as soon as it is idle and every running request is waiting
-    // on the device (queued or executing anywhere), or 200 us after the
-    // oldest queued call -- so a request that arrives while a group is on
-    // the GPU starts in the next idle lane instead of waiting for that group.
-    Dispatcher(Engine* shared, const std::vector<Engine*>& sign, const std::vector<Engine*>& verify) {
-        lanes_.emplace_back(new Lane{shared, true, (1u << Q_OTHER) | (1u << Q_SIGN)});
-        for (Engine* e : sign) lanes_.emplace_back(new Lane{e, false, 1u << Q_SIGN});
-        for (Engine* e : verify) lanes_.emplace_back(new Lane{e, false, 1u << Q_VERIFY});
+    // Lanes, each with its own thread and private engine (HIP streams): lane 0
+    // takes every kind of call, the other signing lanes BDS signatures, the
+    // verification lanes verifications.  A lane launches what its queue holds
+    // as soon as it is idle and every running request is waiting on the device
+    // (queued or executing anywhere), or 200 us after the oldest queued call --
+    // so a request that arrives while a group is on the GPU starts in the next
+    // idle lane instead of waiting for that group.  Every group runs with its
+    // buffers in the arena and the keys of the key tables.
+    Dispatcher(Engine* first, const std::vector<Engine*>& sign, const std::vector<Engine*>& verify,
+               DeviceArena* arena, KeyTables* kt)
+        : arena_(arena), kt_(kt) {
+        lanes_.emplace_back(new Lane{first, (1u << Q_OTHER) | (1u << Q_SIGN)});
+        for (Engine* e : sign) lanes_.emplace_back(new Lane{e, 1u << Q_SIGN});
+        for (Engine* e : verify) lanes_.emplace_back(new Lane{e, 1u << Q_VERIFY});
         if (verify.empty()) lanes_[0]->queues |= 1u << Q_VERIFY;
         for (auto& l : lanes_) l->th = std::thread([this, p = l.get()] { loop(*p); });
     }
@@ -111,7 +141,7 @@ public:
         cv_.notify_all();
         for (auto& l : lanes_) {
             l->th.join();
-            if (!l->shared) Engine::release_private(l->engine);
+            Engine::release_private(l->engine);
         }
     }
     void enter() {
@@ -139,9 +169,8 @@ public:
 private:
     enum { Q_SIGN = 0, Q_VERIFY = 1, Q_OTHER = 2, NQ = 3 };
     struct Lane {
-        Lane(Engine* e, bool s, uint32_t q) : engine(e), shared(s), queues(q) {}
+        Lane(Engine* e, uint32_t q) : engine(e), queues(q) {}
         Engine* engine;
-        bool shared;      // the library's engine: hold the library-wide engine lock
         uint32_t queues;  // bit mask of the queues this lane serves
         std::thread th;
     };
@@ -168,18 +197,42 @@ private:
             std::vector<Call*> batch;
             batch.swap(queue_[q]);
             lk.unlock();
-            try {
-                execute_on(l.engine, batch, l.shared);
-            } catch (const DeviceError&) {
-                // every call of the group fails; the lane's engine is made
-                // consistent again for the next group
+            // one contiguous block of the arena for everything this launch
+            // takes: the sum of its calls' reservations (admission has left
+            // that much of the budget free; while another lane's block splits
+            // the free space, wait for it to come back)
+            size_t need = 0;
+            for (Call* c : batch) need += c->bytes;
+            void* blk = nullptr;
+            {
+                std::unique_lock<std::mutex> ak(blk_mu_);
+                while (!(blk = arena_->alloc(need)) && blocks_ > 0) blk_cv_.wait(ak);
+                if (blk) ++blocks_;
+            }
+            if (!blk) {
                 for (Call* c : batch) c->failed = true;
-                std::unique_lock<std::mutex> elk(engine_mutex(), std::defer_lock);
-                if (l.shared) elk.lock();
+            } else {
+                DeviceArena block(blk, (need + 255) & ~(size_t)255);
+                ArenaScope scope(&block);
                 try {
-                    l.engine->recover();
+                    execute_on(l.engine, batch);
+                } catch (const DeviceError&) {
+                    // every call of the group fails; the lane's engine is made
+                    // consistent again for the next group
+                    for (Call* c : batch) c->failed = true;
+                    try {
+                        l.engine->recover();
+                    } catch (...) {
+                    }
+                }
+                try {
+                    l.engine->release_workspace();  // nothing held between groups
                 } catch (...) {
                 }
+                std::lock_guard<std::mutex> ak(blk_mu_);
+                arena_->release(blk);
+                --blocks_;
+                blk_cv_.notify_all();
             }
             lk.lock();
             for (Call* c : batch) {
@@ -192,19 +245,24 @@ private:
         }
     }
 
-    void execute_on(Engine* e_, std::vector<Call*>& batch, bool shared) {
-        std::unique_lock<std::mutex> elk(engine_mutex(), std::defer_lock);
-        if (shared) elk.lock();
-        e_->pool_begin();  // the group's key contexts stay in the pool until it is done
-        std::map<std::pair<int, int>, std::vector<Call*>> groups;  // (fam, kind)
+    void execute_on(Engine* e_, std::vector<Call*>& batch) {
+        std::shared_lock<std::shared_mutex> tl(kt_->mu);
+        // (fam, kind): verifications of every shape of a family share a launch
+        // sequence (their reservations count the scratch stride of the
+        // largest tree height)
+        std::map<std::pair<int, int>, std::vector<Call*>> groups;
         for (Call* c : batch) groups[{c->fam, (int)c->kind}].push_back(c);
         for (auto& g : groups) {
             const int fam = g.first.first;
             std::vector<Call*>& cs = g.second;
             const unsigned n = fam_n(fam);
+            e_->use_keys(fam, kt_->t[fam].p);
             switch ((Call::Kind)g.first.second) {
+                case Call::PREPARE: {
+                    for (Call* c : cs) e_->prepare_keys(fam, &c->seeds, 1, kt_->t[fam].p, c->slot);
+                    break;
+                }
                 case Call::LEAVES: {
-                    e_->pool_use(fam);
                     std::vector<LeafRun> runs;
                     for (Call* c : cs)
                         for (const LeafId& id : c->ids) runs.push_back({(uint32_t)c->slot, id.layer, id.tree, id.leaf});
@@ -218,7 +276,6 @@ private:
                     break;
                 }
                 case Call::THASH: {
-                    e_->pool_use(fam);
                     std::vector<ThashReq> q(cs.size());
                     std::vector<unsigned char> in(cs.size() * 2 * n), out(cs.size() * n);
                     for (size_t i = 0; i < cs.size(); ++i) {
@@ -231,7 +288,6 @@ private:
                     break;
                 }
                 case Call::WOTS: {
-                    e_->pool_use(fam);
                     const size_t sb = (size_t)fam_len(fam) * n;
                     std::vector<WotsReq> q(cs.size());
                     std::vector<unsigned char> msgs(cs.size() * n), out(cs.size() * sb);
@@ -244,7 +300,6 @@ private:
                     break;
                 }
                 case Call::HMSG: {
-                    e_->pool_use(fam);
                     std::vector<MsgReq> q(cs.size());
                     std::vector<unsigned char> R(cs.size() * n), D(cs.size() * n);
                     for (size_t i = 0; i < cs.size(); ++i)
@@ -257,25 +312,20 @@ private:
                     break;
                 }
                 case Call::BDS: {
-                    // signatures with the traversal on the device: one batch for all shapes,
-                    // key contexts from this engine's own pool (cached by seed)
+                    // signatures with the traversal on the device: one batch for all
+                    // shapes of the family, key contexts from the key table
                     std::vector<Engine::BdsJob> jobs;
-                    for (Call* c : cs) {
-                        jobs.push_back(*c->job);
-                        Engine::BdsJob& j = jobs.back();
-                        j.key = (uint32_t)e_->pool_slot(fam, {j.sk_seed, j.sk_prf, j.pub_seed});
-                    }
+                    for (Call* c : cs) jobs.push_back(*c->job);
                     e_->bds_sign_batch(fam, (int)jobs.size(), jobs.data());
                     break;
                 }
                 case Call::VERIFY: {
-                    // every parameter shape of the family in one launch sequence
-                    // (PUB_SEED contexts in this engine's own pool, cached by seed)
+                    // every key of one shape in one launch sequence (PUB_SEED's
+                    // context from the key's table slot)
                     std::vector<Engine::VerifyJob> jobs;
                     for (Call* c : cs) {
                         const xmss_params* p = c->p;
-                        const uint32_t slot = (uint32_t)e_->pool_slot(fam, {nullptr, nullptr, c->pk + p->n});
-                        jobs.push_back({slot, p->tree_height, p->d, p->index_bytes, c->pk, c->sm,
+                        jobs.push_back({(uint32_t)c->slot, p->tree_height, p->d, p->index_bytes, c->pk, c->sm,
                                         c->sm + p->sig_bytes, c->smlen - p->sig_bytes});
                     }
                     std::vector<int> ok(jobs.size());
@@ -284,10 +334,16 @@ private:
                     break;
                 }
             }
+            e_->release_workspace();  // the next group reuses the block
             ++launches_;
         }
     }
 
+    DeviceArena* arena_;
+    KeyTables* kt_;
+    std::mutex blk_mu_;              // the lanes' blocks of the arena
+    std::condition_variable blk_cv_;  // a block came back
+    int blocks_ = 0;                 // blocks out
     std::mutex mu_;
     std::condition_variable cv_;  // lanes wait here
     std::vector<std::unique_ptr<Lane>> lanes_;
@@ -306,6 +362,7 @@ struct ServeOps final : BdsOps {
     int fam;
     unsigned n;
     int slot;
+    size_t bytes;  // the request's reservation (one call of it at a time)
     std::function<int(const unsigned char*, const unsigned char*, const unsigned char*)> slot_for;
     std::vector<LeafId> plan;
     std::map<LeafId, std::vector<unsigned char>> cache;
@@ -379,6 +436,7 @@ struct ServeOps final : BdsOps {
         run(call);
     }
     void run(Call& call) {
+        call.bytes = bytes;
         d->run(&call);
         if (call.failed) throw DeviceError("serving lane: device call failed");
     }
@@ -389,9 +447,9 @@ struct Key {
     xmss_params p{};
     int fam = -1;
     std::vector<unsigned char> sk, pk;  // OID-prefixed; sk is the host mirror of dsk
-    unsigned char* dsk = nullptr;       // the resident OID-less sk in HBM
-    int slot = -1;                      // pool slot of the key's own seeds
-    std::vector<unsigned char> seeds;   // SK_SEED || SK_PRF || PUB_SEED the slot was made from
+    unsigned char* dsk = nullptr;       // the resident OID-less sk in the arena
+    int slot = -1;                      // key table slot of the key's own seeds
+    bool wiped = false;                 // the slot holds PUB_SEED only (the key signed its last index)
     bool busy = false;
     bool failed = false;  // a device call of one of its signatures failed: its requests return -3
 };
@@ -405,7 +463,9 @@ struct Request {
     std::vector<unsigned char> out;   // sign: sm; verify: m
     unsigned long long outlen = 0;
     int ret = 0;
-    size_t reserve = 0;
+    size_t reserve = 0;   // device bytes its launches take (sized when it is first considered)
+    bool sized = false;
+    bool too_big = false;  // more than the budget left by the keys: fails with -3
     Clock::time_point t0, t1;
     bool done = false;
     std::condition_variable cv;  // its result's waiter (no wake-ups of the others)
@@ -414,9 +474,11 @@ struct Request {
 }  // namespace
 
 struct xmss_mi355x_server {
-    Engine* e = nullptr;
+    std::unique_ptr<DeviceArena> arena;
+    KeyTables tables;
     std::unique_ptr<Dispatcher> disp;
     std::mutex mu;
+    std::mutex add_mu;  // one add_key at a time (table growth, slot assignment)
     std::condition_variable cv_work;
     std::condition_variable cv_key;  // get_sk waiters: a key became free
     int key_waiters = 0;
@@ -426,21 +488,58 @@ struct xmss_mi355x_server {
     std::deque<Request*> queue;  // arrival order
     std::map<uint64_t, Request*> reqs;
     uint64_t next_id = 1;
-    // resident: key contexts and sks of the added keys; transient: the
-    // working sets of running requests; peak: of their sum
-    size_t budget = 0, resident = 0, transient = 0, peak = 0;
+    // resident: arena bytes of the key tables and the keys' sks; transient:
+    // the reservations of running requests
+    size_t budget = 0, resident = 0, transient = 0;
     uint64_t admission_waits = 0;
     bool stop = false;
     std::vector<std::thread> workers;
 
+    // Device bytes a request's launches take: a signature's from its leaf and
+    // merge plan (the dry run of its BDS update on the key's current sk), a
+    // verification's from its shape and message length.
+    size_t reservation(const Request* r) const {
+        const Key& k = *r->k;
+        const xmss_params& p = k.p;
+        if (!r->sign) {
+            const unsigned long long smlen = r->data.size();
+            if (smlen < p.sig_bytes || smlen - p.sig_bytes > kMaxMsgBytes) return 0;
+            // a launch sequence of several shapes strides its scratch by the
+            // largest tree height among them: count the largest of any set
+            return Engine::verify_bytes(k.fam, kMaxTreeHeight, p.d, smlen - p.sig_bytes);
+        }
+        if (r->data.size() > kMaxMsgBytes || k.failed) return 0;
+        const unsigned char* sk = k.sk.data() + XMSS_OID_LEN;
+        std::vector<LeafId> leaves;
+        std::vector<uint32_t> merges, dag, wots;
+        std::vector<unsigned char> lits;
+        bds_plan(&p, sk, r->data.size(), leaves, merges, &dag, &lits, &wots);
+        Engine::BdsJob j{};
+        j.nplan = (uint32_t)leaves.size();
+        j.nmerge = (uint32_t)(merges.size() / 8);
+        j.nlit = (uint32_t)(lits.size() / p.n);
+        j.plan_wots = wots.data();
+        // the last index's host path signs again (a WOTS signature per layer)
+        j.nwots = (uint32_t)(wots.size() / 4) + p.d;
+        j.mlen = r->data.size();
+        j.shape = bds_shape(&p);
+        return Engine::bds_bytes(k.fam, j);
+    }
+
     // The request to run next: the oldest one whose key is free, if the
-    // budget admits it (FIFO: a request that does not fit holds back the rest;
-    // with no request running, the oldest runs whatever its size).
+    // budget admits it (FIFO: a request that does not fit holds back the rest).
     Request* pick() {
         for (auto it = queue.begin(); it != queue.end(); ++it) {
             Request* r = *it;
             if (r->sign && r->k->busy) continue;
-            if (resident + transient + r->reserve > budget && transient > 0) {
+            if (!r->sized) {  // the key is free: its sk is the one the request will sign with
+                r->reserve = reservation(r);
+                r->sized = true;
+            }
+            if (r->reserve > budget - std::min(budget, resident)) {
+                r->too_big = true;
+                r->reserve = 0;
+            } else if (resident + transient + r->reserve > budget) {
                 ++admission_waits;
                 return nullptr;
             }
@@ -459,7 +558,6 @@ struct xmss_mi355x_server {
                 cv_work.wait(lk, [&] { return stop || (r = pick()) != nullptr; });
                 if (!r) return;
                 transient += r->reserve;
-                peak = std::max(peak, resident + transient);
                 if (r->sign) r->k->busy = true;
                 others = !queue.empty();
             }
@@ -468,12 +566,17 @@ struct xmss_mi355x_server {
             // admissible request or goes back to sleep -- instead of every
             // idle worker contending for the lock at once.
             if (others) cv_work.notify_one();
-            // the worker counts as active (more device work coming) from
-            // here until its last device call has returned
-            disp->enter();
-            bool active = true;
-            execute(r, active);
-            if (active) disp->leave();
+            if (r->too_big) {
+                r->ret = XMSS_ENODEV;
+                r->outlen = 0;
+            } else {
+                // the worker counts as active (more device work coming) from
+                // here until its last device call has returned
+                disp->enter();
+                bool active = true;
+                execute(r, active);
+                if (active) disp->leave();
+            }
             bool more, key_freed;
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -490,25 +593,28 @@ struct xmss_mi355x_server {
         }
     }
 
-    // The pool slot for the seeds a signature asks for: the key's own slot,
-    // or -- for the last index, which signs with the wiped (zero) seeds -- the
-    // pool's slot of those seeds.  Compared with the seeds the key's slot was
-    // made from, not with the sk, which the signature has already wiped.
+    // Writes a key context into a table slot through lane 0 (synchronous).
+    void prepare_slot(int fam, int slot, const KeySeeds& ks) {
+        Call call;
+        call.kind = Call::PREPARE;
+        call.fam = fam;
+        call.slot = slot;
+        call.seeds = ks;
+        call.bytes = kPrepareBytes;
+        disp->run(&call);
+        if (call.failed) throw DeviceError("serving: key context preparation failed");
+    }
+
+    // The table slot for the seeds a signature asks for: the key's own slot,
+    // or -- for the last index, which signs with the wiped sk -- the family's
+    // slot of the all-zero seeds.
     int slot_for(Key& k, const unsigned char* a, const unsigned char* b, const unsigned char* c) {
         const unsigned n = k.p.n;
-        std::lock_guard<std::mutex> elk(engine_mutex());
-        if (k.slot >= 0 && !memcmp(a, &k.seeds[0], n) && !memcmp(b, &k.seeds[n], n) && !memcmp(c, &k.seeds[2 * n], n))
-            return k.slot;
-        return e->pool_slot(k.fam, {a, b, c});
-    }
-    std::vector<unsigned char> seeds_of(const Key& k) {
-        const unsigned n = k.p.n, ib = k.p.index_bytes;
-        const unsigned char* sk = k.sk.data() + XMSS_OID_LEN;
-        std::vector<unsigned char> v(3 * n);
-        memcpy(&v[0], sk + ib, n);
-        memcpy(&v[n], sk + ib + n, n);
-        memcpy(&v[2 * n], sk + ib + 3 * n, n);
-        return v;
+        bool zero = true;
+        for (unsigned i = 0; i < n; ++i) zero = zero && !a[i] && !b[i] && !c[i];
+        if (zero) return tables.t[k.fam].zero;
+        if (!k.wiped) return k.slot;
+        throw DeviceError("serving: seeds of a wiped key");
     }
 
     void execute(Request* r, bool& active) {
@@ -537,6 +643,7 @@ struct xmss_mi355x_server {
                     call.fam = k.fam;
                     call.p = &p;
                     call.job = &ds.job;
+                    call.bytes = r->reserve;
                     disp->run(&call);
                     disp->leave();
                     active = false;
@@ -546,23 +653,25 @@ struct xmss_mi355x_server {
                     return;
                 }
                 // last index or exhausted key: host control, hashing through the dispatcher
+                const bool last = !k.wiped;
                 ServeOps ops;
                 ops.d = disp.get();
                 ops.fam = k.fam;
                 ops.n = p.n;
                 ops.slot = k.slot;
+                ops.bytes = r->reserve;
                 ops.slot_for = [&](const unsigned char* a, const unsigned char* b, const unsigned char* c) {
                     return slot_for(k, a, b, c);
                 };
                 ops.plan = bds_plan_leaves(&p, sk, r->data.size());
                 r->ret = bds_sign(&p, sk, r->out.data(), &r->outlen, r->data.data(), r->data.size(), ops);
                 Engine::upload(k.dsk, sk, p.sk_bytes);  // the mirror changed on the host
-                if (r->ret == 0 && k.slot >= 0) {
-                    // the last index was signed and the sk wiped: the pool keeps
-                    // no context of the old seeds
-                    std::lock_guard<std::mutex> elk(engine_mutex());
-                    e->pool_release(k.fam, k.slot);
-                    k.slot = -1;
+                if (last && r->ret == 0) {
+                    // the last index was signed and the sk wiped: the key's slot
+                    // keeps only PUB_SEED's context, for its verifications
+                    std::vector<unsigned char> pub(k.pk.begin() + XMSS_OID_LEN + p.n, k.pk.end());
+                    prepare_slot(k.fam, k.slot, {nullptr, nullptr, pub.data()});
+                    k.wiped = true;
                 }
             } catch (const DeviceError&) {
                 ok = false;
@@ -586,6 +695,8 @@ struct xmss_mi355x_server {
             Call call;
             call.kind = Call::VERIFY;
             call.fam = k.fam;
+            call.slot = k.slot;
+            call.bytes = r->reserve;
             call.p = &p;
             call.pk = k.pk.data() + XMSS_OID_LEN;
             call.sm = r->data.data();
@@ -612,25 +723,43 @@ struct xmss_mi355x_server {
 extern "C" {
 
 xmss_mi355x_server* xmss_mi355x_server_create(size_t hbm_budget_bytes, int workers) {
-    Engine* e;
     {
         std::lock_guard<std::mutex> lk(engine_mutex());
-        e = Engine::get();
+        if (!Engine::get()) return nullptr;
     }
-    if (!e) return nullptr;
-    xmss_mi355x_server* s = new xmss_mi355x_server;
-    s->e = e;
-    s->budget = hbm_budget_bytes;
-    // one more signing lane and two verification lanes on private engines
-    std::vector<Engine*> sign_lanes, verify_lanes;
-    for (int i = 0; i < 3; ++i) {
+    std::unique_ptr<xmss_mi355x_server> s(new xmss_mi355x_server);
+    s->budget = (hbm_budget_bytes + 255) & ~(size_t)255;
+    try {
+        s->arena.reset(new DeviceArena(s->budget));
+    } catch (const DeviceError&) {
+        return nullptr;  // the device (or the library's limit) cannot hold the budget
+    }
+    // four lanes on private engines: every kind of call, one more signing
+    // lane, two verification lanes; each is stripped of any buffer an earlier
+    // user left, so that the server's device memory is its arena alone
+    std::vector<Engine*> eng;
+    for (int i = 0; i < 4; ++i) {
         Engine* p = Engine::make_private();
-        if (p) (i == 0 ? sign_lanes : verify_lanes).push_back(p);
+        bool ok = p != nullptr;
+        if (ok) {
+            try {
+                p->trim();
+            } catch (const DeviceError&) {
+                ok = false;
+            }
+        }
+        if (!ok) {
+            if (p) Engine::release_private(p);
+            for (Engine* e : eng) Engine::release_private(e);
+            return nullptr;
+        }
+        eng.push_back(p);
     }
-    s->disp.reset(new Dispatcher(e, sign_lanes, verify_lanes));
+    s->disp.reset(new Dispatcher(eng[0], {eng[1]}, {eng[2], eng[3]}, s->arena.get(), &s->tables));
     if (workers <= 0) workers = 256;
-    for (int i = 0; i < workers; ++i) s->workers.emplace_back([s] { s->work(); });
-    return s;
+    xmss_mi355x_server* sp = s.release();
+    for (int i = 0; i < workers; ++i) sp->workers.emplace_back([sp] { sp->work(); });
+    return sp;
 }
 
 int xmss_mi355x_server_add_key(xmss_mi355x_server* s, int mt, const unsigned char* sk, const unsigned char* pk) {
@@ -643,32 +772,50 @@ int xmss_mi355x_server_add_key(xmss_mi355x_server* s, int mt, const unsigned cha
     if (k.fam < 0) return -1;
     k.sk.assign(sk, sk + XMSS_OID_LEN + k.p.sk_bytes);
     k.pk.assign(pk, pk + XMSS_OID_LEN + k.p.pk_bytes);
-    const size_t res_bytes = s->e->pool_key_bytes(k.fam) + k.p.sk_bytes;
+    std::lock_guard<std::mutex> alk(s->add_mu);
+    DeviceArena* A = s->arena.get();
+    KeyTables::T& T = s->tables.t[k.fam];
+    const size_t kb = Engine::pool_key_bytes_of(k.fam);
+    const bool first = T.zero < 0;
+    const int need = T.n + (first ? 2 : 1);
+    const size_t sk_res = (k.p.sk_bytes + 255) & ~(size_t)255;
     {
+        // the resident bytes: the sk, and the table's growth
+        size_t cap = T.cap, grow = 0;
+        while ((size_t)need > cap) cap = std::max<size_t>(16, 2 * cap);
+        if (cap > T.cap) grow = ((cap * kb + 255) & ~(size_t)255) - ((T.cap * kb + 255) & ~(size_t)255);
         std::lock_guard<std::mutex> lk(s->mu);
-        if (s->resident + res_bytes > s->budget) return -1;  // the budget cannot hold the key
-        s->resident += res_bytes;
-        s->peak = std::max(s->peak, s->resident + s->transient);
-    }
-    k.dsk = static_cast<unsigned char*>(Engine::device_alloc(k.p.sk_bytes));
-    bool ok = k.dsk != nullptr;
-    if (ok) {
-        try {
-            Engine::upload(k.dsk, k.sk.data() + XMSS_OID_LEN, k.p.sk_bytes);
-            k.seeds = s->seeds_of(k);
-            const unsigned n = k.p.n;
-            KeySeeds ks{&k.seeds[0], &k.seeds[n], &k.seeds[2 * n]};
-            std::lock_guard<std::mutex> elk(engine_mutex());
-            k.slot = s->e->pool_add(k.fam, ks);
-        } catch (const DeviceError&) {
-            ok = false;
+        if (s->resident + grow + sk_res > s->budget) return -1;  // the budget cannot hold the key
+        if (cap > T.cap) {
+            std::unique_lock<std::shared_mutex> tl(s->tables.mu);  // no group runs meanwhile
+            void* np = A->alloc(cap * kb);
+            if (!np) return -1;
+            try {
+                if (T.p) Engine::copy_device(np, T.p, (size_t)T.n * kb);
+            } catch (const DeviceError&) {
+                A->release(np);
+                return -1;
+            }
+            A->release(T.p);
+            T.p = np;
+            T.cap = cap;
+            s->resident += grow;
         }
+        k.dsk = static_cast<unsigned char*>(A->alloc(k.p.sk_bytes));
+        if (!k.dsk) return -1;
+        s->resident += sk_res;
+        if (first) T.zero = T.n++;
+        k.slot = T.n++;
     }
-    if (!ok) {
-        Engine::device_free(k.dsk, k.p.sk_bytes);
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->resident -= res_bytes;
-        return -1;
+    const unsigned n = k.p.n;
+    const unsigned char* s0 = k.sk.data() + XMSS_OID_LEN + k.p.index_bytes;
+    try {
+        Engine::upload(k.dsk, k.sk.data() + XMSS_OID_LEN, k.p.sk_bytes);
+        if (first) s->prepare_slot(k.fam, T.zero, {nullptr, nullptr, nullptr});
+        // SK_SEED, SK_PRF, PUB_SEED of the sk (idx || SK_SEED || SK_PRF || root || PUB_SEED)
+        s->prepare_slot(k.fam, k.slot, {s0, s0 + n, s0 + 3 * n});
+    } catch (const DeviceError&) {
+        return -1;  // its arena bytes stay reserved; the key is not added
     }
     std::lock_guard<std::mutex> lk(s->mu);
     s->keys.push_back(std::move(kp));
@@ -687,9 +834,6 @@ static uint64_t submit(xmss_mi355x_server* s, bool sign, int key, const unsigned
         return 0;
     }
     r->k = s->keys[key].get();
-    const xmss_params& p = r->k->p;
-    // device working set: message + signature staging, planned leaves, H_msg prefix
-    r->reserve = (size_t)p.sig_bytes + len + (size_t)(p.tree_height + 1) * p.n * 2 + 4 * p.n;
     r->id = s->next_id++;
     s->reqs[r->id] = r;
     s->queue.push_back(r);
@@ -738,8 +882,17 @@ void xmss_mi355x_server_stats(xmss_mi355x_server* s, uint64_t* launches, uint64_
     std::lock_guard<std::mutex> lk(s->mu);
     if (launches) *launches = s->disp->launches();
     if (calls) *calls = s->disp->calls();
-    if (peak_bytes) *peak_bytes = s->peak;
+    if (peak_bytes) *peak_bytes = s->arena->peak();
     if (admission_waits) *admission_waits = s->admission_waits;
+}
+
+void xmss_mi355x_server_memory(xmss_mi355x_server* s, size_t* budget, size_t* resident, size_t* in_use,
+                               size_t* peak) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (budget) *budget = s->arena->size();
+    if (resident) *resident = s->resident;
+    if (in_use) *in_use = s->arena->in_use();
+    if (peak) *peak = s->arena->peak();
 }
 
 void xmss_mi355x_server_destroy(xmss_mi355x_server* s) {
@@ -751,15 +904,12 @@ void xmss_mi355x_server_destroy(xmss_mi355x_server* s) {
     for (auto& t : s->workers) t.join();
     s->disp.reset();
     for (auto& kv : s->reqs) delete kv.second;
-    for (auto& k : s->keys) {
-        Engine::device_free(k->dsk, k->p.sk_bytes);
-        try {
-            std::lock_guard<std::mutex> elk(engine_mutex());
-            if (k->slot >= 0) s->e->pool_release(k->fam, k->slot);
-        } catch (const DeviceError&) {
-        }
+    // no key context or sk outlives the server in device memory
+    try {
+        Engine::clear_device(s->arena->base(), s->arena->size());
+    } catch (const DeviceError&) {
     }
-    delete s;
+    delete s;  // the arena goes with it
 }
 
 }  // extern "C"

@@ -172,11 +172,16 @@ int xmss_batch_sign_open(int mt, unsigned char *const *ms, unsigned long long *m
  * exactly as xmss[mt]_sign would; requests of different keys share GPU
  * launches.  Each key's state (key context and serialized sk) stays resident
  * in HBM and is advanced there; the server keeps a byte-identical host copy.
- * Resident keys count against hbm_budget_bytes, and every request reserves
- * its device working set against what is left and waits while the budget is
- * used up.  Results come back with their latency (submission to completion). */
+ * The server's device memory is exactly hbm_budget_bytes: one allocation made
+ * at creation, holding the keys' contexts and sks and every launch's buffers.
+ * Resident keys count against it, and every request reserves the device bytes
+ * its launches take against what is left and waits (first in, first out)
+ * while the budget is used up; a request larger than the budget left by the
+ * keys returns -3.  Results come back with their latency (submission to
+ * completion). */
 typedef struct xmss_mi355x_server xmss_mi355x_server;
 
+/* Returns NULL without a GPU or when the budget cannot be allocated. */
 xmss_mi355x_server *xmss_mi355x_server_create(size_t hbm_budget_bytes, int workers);
 /* Returns the key id, or -1 for an unknown OID, when the budget left by the
  * keys already added cannot hold the key's resident state, or when that HBM
@@ -194,8 +199,14 @@ int xmss_mi355x_server_result(xmss_mi355x_server *s, uint64_t id, unsigned char 
                               unsigned long long *outlen, double *latency_ms);
 /* Current secret key of a key id (waits for its running signature). */
 int xmss_mi355x_server_get_sk(xmss_mi355x_server *s, int key, unsigned char *sk);
+/* peak_bytes: the most device memory the server's arena has had in use
+ * (resident keys and launch buffers together), never more than the budget. */
 void xmss_mi355x_server_stats(xmss_mi355x_server *s, uint64_t *launches, uint64_t *calls,
                               size_t *peak_bytes, uint64_t *admission_waits);
+/* Device memory of the server: its budget (the one arena it allocated), the
+ * bytes its keys hold, the bytes in use now and the most in use so far. */
+void xmss_mi355x_server_memory(xmss_mi355x_server *s, size_t *budget, size_t *resident,
+                               size_t *in_use, size_t *peak);
 void xmss_mi355x_server_destroy(xmss_mi355x_server *s);
 
 #ifdef __cplusplus

@@ -30,6 +30,42 @@ def _server_api(L):
     L.xmss_mi355x_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                            C.POINTER(C.c_size_t), C.POINTER(C.c_uint64)]
     L.xmss_mi355x_server_destroy.argtypes = [C.c_void_p]
+    L.xmss_mi355x_server_memory.argtypes = [C.c_void_p] + [C.POINTER(C.c_size_t)] * 4
+    L.xmss_mi355x_device_bytes.restype = C.c_size_t
+
+
+def _memory(L, srv):
+    b, r, u, p = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_size_t()
+    L.xmss_mi355x_server_memory(srv, C.byref(b), C.byref(r), C.byref(u), C.byref(p))
+    return b.value, r.value, u.value, p.value
+
+
+def _tight_budget(lib, keys, msgs):
+    """The keys' resident bytes plus 1.5 x the largest working set of a
+    single request of theirs (each key's first messages signed, and one
+    signature verified, alone on a probe server), so that two large requests
+    never fit at once."""
+    L = lib.lib
+    srv = L.xmss_mi355x_server_create(1 << 30, 4)
+    assert srv
+    for (mt, oid, p, pk, sk) in keys:
+        assert L.xmss_mi355x_server_add_key(srv, mt, buf(sk), buf(pk)) >= 0
+    _, resident, _, _ = _memory(L, srv)
+    most = 0
+    for k, (mt, oid, p, pk, sk) in enumerate(keys):
+        for m in msgs[k][:2]:
+            rid = L.xmss_mi355x_server_sign(srv, k, m, len(m))
+            out, olen = buf(p.sig_bytes + len(m)), C.c_ulonglong(0)
+            assert L.xmss_mi355x_server_result(srv, rid, out, C.byref(olen), None) == 0
+            _, _, _, peak = _memory(L, srv)
+            most = max(most, peak - resident)
+        sm = raw(out, olen.value)
+        rid = L.xmss_mi355x_server_verify(srv, k, sm, len(sm))
+        assert L.xmss_mi355x_server_result(srv, rid, buf(len(sm)), None, None) == 0
+        _, _, _, peak = _memory(L, srv)
+        most = max(most, peak - resident)
+    L.xmss_mi355x_server_destroy(srv)
+    return resident + most + most // 2
 
 
 def _keys(lib):
@@ -56,10 +92,10 @@ def _alone(lib, keys, msgs):
     return out
 
 
-# the small budget holds the six keys' resident state (53,320 sk bytes plus
-# key contexts) and about 45 KB of request working sets
-@pytest.mark.parametrize("budget", [1 << 30, 100000])
-def test_serve_mix_equals_alone(budget):
+# the tight budget holds the six keys' resident state and one and a half
+# times the largest single request: requests wait for each other's memory
+@pytest.mark.parametrize("tight", [False, True])
+def test_serve_mix_equals_alone(tight):
     lib = product(1)
     L = lib.lib
     _server_api(L)
@@ -69,9 +105,11 @@ def test_serve_mix_equals_alone(budget):
     msgs = [[bytes(rng.getrandbits(8) for _ in range(int(32 * 128 ** rng.random()))) for _ in range(nsig)]
             for _ in keys]
     want = _alone(lib, keys, msgs)
+    budget = _tight_budget(lib, keys, msgs) if tight else 1 << 30
 
     srv = L.xmss_mi355x_server_create(budget, 64)
     assert srv
+    held = L.xmss_mi355x_device_bytes()  # the library's device memory, the arena included
     ids = [L.xmss_mi355x_server_add_key(srv, mt, buf(sk), buf(pk)) for (mt, oid, p, pk, sk) in keys]
     assert ids == list(range(len(keys)))
     # interleave: round-robin over keys, with verifications of signatures
@@ -106,10 +144,14 @@ def test_serve_mix_equals_alone(budget):
         assert raw(skb, 4 + p.sk_bytes) == want[k][1], "key %d final sk" % k
     launches, calls, peak, waits = C.c_uint64(), C.c_uint64(), C.c_size_t(), C.c_uint64()
     L.xmss_mi355x_server_stats(srv, C.byref(launches), C.byref(calls), C.byref(peak), C.byref(waits))
+    size, resident, in_use, peak2 = _memory(L, srv)
+    # no device memory outside the arena grew while the server ran
+    assert L.xmss_mi355x_device_bytes() == held
     L.xmss_mi355x_server_destroy(srv)
     assert calls.value > launches.value  # calls of different keys shared launches
-    if budget < 1 << 20:
-        assert waits.value > 0 and peak.value <= budget + 30000
+    assert size >= budget and size - budget < 256 and peak.value == peak2 <= size and in_use == resident
+    if tight:
+        assert waits.value > 0
 
 
 def test_serve_key_to_its_end():
@@ -162,8 +204,8 @@ def test_serve_key_to_its_end():
 
 
 def test_serve_budget_counts_resident_keys():
-    """Adding keys reserves their resident state (key context and sk bytes)
-    against the HBM budget before any request runs; a key the budget cannot
+    """Adding keys reserves their resident state (key contexts and sk bytes)
+    in the server's arena before any request runs; a key the budget cannot
     hold is refused."""
     lib = product(1)
     L = lib.lib
@@ -171,15 +213,17 @@ def test_serve_budget_counts_resident_keys():
     keys = _keys(lib)
     srv = L.xmss_mi355x_server_create(1 << 30, 4)
     assert srv
+    res = []
     for (mt, oid, p, pk, sk) in keys:
         assert L.xmss_mi355x_server_add_key(srv, mt, buf(sk), buf(pk)) >= 0
+        res.append(_memory(L, srv)[1])
     launches, calls, peak, waits = C.c_uint64(), C.c_uint64(), C.c_size_t(), C.c_uint64()
     L.xmss_mi355x_server_stats(srv, C.byref(launches), C.byref(calls), C.byref(peak), C.byref(waits))
     L.xmss_mi355x_server_destroy(srv)
-    assert peak.value >= sum(p.sk_bytes for (_, _, p, _, _) in keys)
-    assert calls.value == 0
+    assert res[-1] >= sum(p.sk_bytes for (_, _, p, _, _) in keys)
+    assert calls.value == 8  # a key context each, and the zero-seed context of each of the two families
     # a budget just below the six keys' resident bytes refuses the last key
-    srv = L.xmss_mi355x_server_create(peak.value - 1, 4)
+    srv = L.xmss_mi355x_server_create(res[-1] - 256, 4)
     assert srv
     got = [L.xmss_mi355x_server_add_key(srv, mt, buf(sk), buf(pk)) for (mt, oid, p, pk, sk) in keys]
     L.xmss_mi355x_server_destroy(srv)
