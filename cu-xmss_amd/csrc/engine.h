@@ -366,6 +366,11 @@ public:
     // its parameter set), staged into its buffers; its captured launch graph
     // stays valid unless the messages no longer fit (then it is dropped).
     void verify_refill(Batch* b, const VerifyItem* items);
+    // A verification batch whose launches all go on the engine's main stream
+    // (default: its large batches split into two groups on two streams).
+    // Batches of several engines then run side by side on as many hardware
+    // queues instead of contending for them (mixed jobs).
+    void set_one_stream(Batch* b, bool on);
     // wait_done = false returns once the work is queued; wait() then blocks
     // until this engine's streams are idle.  A verification batch's launches
     // are captured into a HIP graph on its second run and replayed from then

@@ -979,6 +979,7 @@ struct Engine::Batch {
     int nrun = 0;                    // verify: runs since staging
     PinnedBuf pin;                   // host staging of this batch's inputs and results
     size_t seed_off = 0, seed_bytes = 0;  // sign: the seeds' region in `pin` and at `seeds`
+    bool one_stream = false;         // verify: all launches on the main stream (set_one_stream)
     hipEvent_t done = nullptr;       // recorded after the work of launch()
     bool inflight = false;           // launched and not yet waited for
     bool evented = false;            // launched through launch(): results leave on the copy stream
@@ -1295,6 +1296,7 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
     Batch* b = take_batch(I, false);
     BatchGuard guard{this, b};
     b->sign = false;
+    b->one_stream = false;
     b->fam = fam;
     b->th = th;
     b->d = d;
@@ -1313,6 +1315,11 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
 
 void Engine::verify_refill(Batch* b, const VerifyItem* items) { stage_verify(impl_, ops_of(b->fam), b, items); }
 
+void Engine::set_one_stream(Batch* b, bool on) {
+    if (b->one_stream != on) b->drop_graph();  // the captured launches follow the stream layout
+    b->one_stream = on;
+}
+
 // Large verification batches (staged longest message first).  The requests
 // split into two groups by message length, each on its own stream: the
 // longest quarter (aux) and the rest (main).  Each group runs H_msg -> per
@@ -1323,7 +1330,9 @@ void Engine::verify_refill(Batch* b, const VerifyItem* items) { stage_verify(imp
 // every layer and request (L-tree merges and auth levels: address only) run
 // once, on the main stream after its F pre-images; the aux stream waits for
 // them before its chains.  (Measured on one MI355X at 4096 requests: three
-// groups, or the H pre-images right after H_msg, were 3-7% slower.)
+// groups, or the H pre-images right after H_msg, were 3-7% slower; so were
+// all 15 steps' F pre-images of the long group on a third stream during its
+// H_msg, 5-8% slower with a long group of 1/4, 1/6 or 1/8 of the batch.)
 static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* vscratch) {
     const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
     const uint32_t count = (uint32_t)b->count;
@@ -1335,7 +1344,9 @@ static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* 
     uint8_t* tmp = b->tmp.need((size_t)count * ((len + 1) / 2) * n);
     constexpr int G = 2;
     const uint32_t lo[G] = {0, count / 4}, hi[G] = {count / 4, count};
-    hipStream_t st[G] = {I->L.s_aux, I->L.s_main};
+    // one stream: the batch shares the GPU with other engines' batches (a
+    // mixed job), which take the other hardware queues
+    hipStream_t st[G] = {b->one_stream ? I->L.s_main : I->L.s_aux, I->L.s_main};
     hipEvent_t staged = I->L.ev(), hpre = I->L.ev(), done[G - 1] = {I->L.ev()};
     HIPCK(hipEventRecord(staged, I->L.s_main));
     for (int g = 0; g < G - 1; ++g) HIPCK(hipStreamWaitEvent(st[g], staged, 0));
