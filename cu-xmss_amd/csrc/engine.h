@@ -495,4 +495,12 @@ private:
     bool prof_on_ = false;
 };
 
+// Autotuner of the verification-batch schedule (engine.hip): forces a plan
+// for jobs run from now on (plan >= 0; -1: tuned per shape, the default);
+// returns the previous setting.  verify_plans lists the candidate plans;
+// tuned_plans describes the settled shapes, one line each.
+int set_verify_plan(int plan);
+size_t verify_plans(uint32_t* out, size_t max);
+std::string tuned_plans();
+
 }  // namespace xm

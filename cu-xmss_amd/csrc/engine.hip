@@ -192,6 +192,20 @@ constexpr int kSplitRuns = 64;
 // lane-dense (per chain, per L-tree pair, per auth walk) instead of with one
 // workgroup per request.
 constexpr int kLatDense = 256;
+// The schedule of a large verification batch (verify_dense), a bit set:
+// VPLAN_PRIO            the long-message group's kernels at wave priority 2
+// VPLAN_PRE_H_FIRST     the H pre-images on a third stream beside H_msg
+//                       (else on the main stream after its F pre-images)
+// VPLAN_PRIO_HMSG       every H_msg at wave priority 2
+// tail (bits 2-3)       0: one launch per L-tree height, then the auth walks;
+//                       1: the L-tree through LDS in one launch, then the auth
+//                       walks; 2: L-tree and auth walk in one launch
+// div (bits 4-7)        the long group is the longest 1/div of the batch (0: 4)
+constexpr uint32_t VPLAN_PRIO = 1, VPLAN_PRE_H_FIRST = 2, VPLAN_PRIO_HMSG = 256;
+static inline uint32_t vplan_tail(uint32_t p) { return (p >> 2) & 3; }
+static inline uint32_t vplan_div(uint32_t p) { return ((p >> 4) & 15) ? ((p >> 4) & 15) : 4; }
+constexpr uint32_t kVplanDefault = 1u << 2;  // tail 1, two groups, no priorities
+uint32_t verify_plan_at_prepare();  // the plan a new verification batch starts with
 // Up to this many requests per layer, a signing batch's WOTS signatures come
 // from the latency kernels (address-only work of the needed steps wide, then
 // one compression per step) instead of one lane per whole chain (k_wots).
@@ -980,6 +994,8 @@ struct Engine::Batch {
     PinnedBuf pin;                   // host staging of this batch's inputs and results
     size_t seed_off = 0, seed_bytes = 0;  // sign: the seeds' region in `pin` and at `seeds`
     bool one_stream = false;         // verify: all launches on the main stream (set_one_stream)
+    uint32_t plan = 0;               // verify: the schedule of verify_dense (VPLAN_* bits)
+    bool tuned = false;              // verify: plan settled (forced, cached or tuned)
     hipEvent_t done = nullptr;       // recorded after the work of launch()
     bool inflight = false;           // launched and not yet waited for
     bool evented = false;            // launched through launch(): results leave on the copy stream
@@ -1297,6 +1313,8 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
     BatchGuard guard{this, b};
     b->sign = false;
     b->one_stream = false;
+    b->plan = verify_plan_at_prepare();
+    b->tuned = false;
     b->fam = fam;
     b->th = th;
     b->d = d;
@@ -1316,55 +1334,175 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
 void Engine::verify_refill(Batch* b, const VerifyItem* items) { stage_verify(impl_, ops_of(b->fam), b, items); }
 
 void Engine::set_one_stream(Batch* b, bool on) {
-    if (b->one_stream != on) b->drop_graph();  // the captured launches follow the stream layout
+    if (b->one_stream != on) {
+        b->drop_graph();  // the captured launches follow the stream layout
+        b->tuned = false;  // and the best plan may differ
+    }
     b->one_stream = on;
+}
+
+// ---- autotuner of the verification schedule ---------------------------
+// The best plan of verify_dense depends on the parameter set (chain count,
+// tree height, layers), the batch size and what shares the GPU (one-stream
+// batches of mixed jobs).  A job of a shape not seen before tries every
+// candidate plan on its second run: each is captured into a HIP graph and
+// replayed three times on the job's own staged inputs (every replay is a
+// whole, correct verification), the last two timed with HIP events on the
+// stream the graph is launched on.  The fastest is kept -- the default unless
+// another beats it by more than 2% -- for the job and for every later job of
+// that shape in the process, and appended to the file XMSS_MI355X_TUNE_FILE
+// names, from which later processes start.  XMSS_MI355X_TUNE=0 turns tuning
+// off (the default plan).
+namespace {
+const uint32_t kVerifyCands[] = {
+    kVplanDefault,                      // L-tree through LDS, then the auth walks
+    0,                                  // one launch per L-tree height
+    2u << 2,                            // L-tree and auth walk in one launch
+    kVplanDefault | (8u << 4),          // the long group is the longest 1/8
+    kVplanDefault | VPLAN_PRIO_HMSG,    // H_msg at wave priority 2
+    kVplanDefault | VPLAN_PRE_H_FIRST,  // H pre-images beside H_msg
+};
+constexpr size_t kNumVerifyCands = sizeof(kVerifyCands) / sizeof(kVerifyCands[0]);
+
+struct VerifyTuner {
+    std::mutex mu;
+    std::map<uint64_t, uint32_t> best;  // shape key -> plan
+    int forced = -1;                    // >= 0: every job uses this plan
+    bool off = false;
+    bool loaded = false;
+    std::string file;
+
+    void load_locked() {
+        if (loaded) return;
+        loaded = true;
+        const char* t = getenv("XMSS_MI355X_TUNE");
+        off = t && t[0] == '0';
+        const char* f = getenv("XMSS_MI355X_TUNE_FILE");
+        if (!f || !*f) return;
+        file = f;
+        FILE* fp = fopen(f, "r");
+        if (!fp) return;
+        unsigned long long key;
+        unsigned plan;
+        char kind[16];
+        while (fscanf(fp, "%15s %llx %u", kind, &key, &plan) == 3)
+            if (!strcmp(kind, "verify")) best[key] = plan;
+        fclose(fp);
+    }
+    void record_locked(uint64_t key, uint32_t plan) {
+        best[key] = plan;
+        if (file.empty()) return;
+        if (FILE* fp = fopen(file.c_str(), "a")) {
+            fprintf(fp, "verify %llx %u\n", (unsigned long long)key, plan);
+            fclose(fp);
+        }
+    }
+};
+VerifyTuner& vtuner() {
+    static VerifyTuner* t = new VerifyTuner;  // lives as long as the process (engines do)
+    return *t;
+}
+uint64_t verify_key(const Engine::Batch* b) {
+    uint32_t lg = 0;
+    while ((2u << lg) <= (uint32_t)b->count) ++lg;  // floor(log2(count))
+    return (uint64_t)b->fam | (uint64_t)b->th << 8 | (uint64_t)b->d << 16 | (uint64_t)lg << 24 |
+           (uint64_t)(b->one_stream ? 1 : 0) << 32;
+}
+}  // namespace
+
+uint32_t verify_plan_at_prepare() {
+    VerifyTuner& t = vtuner();
+    std::lock_guard<std::mutex> lk(t.mu);
+    t.load_locked();
+    return t.forced >= 0 ? (uint32_t)t.forced : kVplanDefault;
+}
+
+int set_verify_plan(int plan) {
+    VerifyTuner& t = vtuner();
+    std::lock_guard<std::mutex> lk(t.mu);
+    t.load_locked();
+    const int prev = t.forced;
+    t.forced = plan < 0 ? -1 : plan;
+    return prev;
+}
+
+size_t verify_plans(uint32_t* out, size_t max) {
+    for (size_t i = 0; i < kNumVerifyCands && i < max; ++i) out[i] = kVerifyCands[i];
+    return kNumVerifyCands;
+}
+
+std::string tuned_plans() {
+    VerifyTuner& t = vtuner();
+    std::lock_guard<std::mutex> lk(t.mu);
+    t.load_locked();
+    std::string s;
+    char line[96];
+    for (auto& kv : t.best) {
+        snprintf(line, sizeof line, "verify fam=%u th=%u d=%u log2count=%u one_stream=%u plan=%u\n",
+                 (unsigned)(kv.first & 255), (unsigned)(kv.first >> 8 & 255), (unsigned)(kv.first >> 16 & 255),
+                 (unsigned)(kv.first >> 24 & 255), (unsigned)(kv.first >> 32 & 1), kv.second);
+        s += line;
+    }
+    return s;
 }
 
 // Large verification batches (staged longest message first).  The requests
 // split into two groups by message length, each on its own stream: the
-// longest quarter (aux) and the rest (main).  Each group runs H_msg -> per
+// longest 1/div (aux) and the rest (main).  Each group runs H_msg -> per
 // layer: F pre-images of only the chain steps its signatures need -> chains
 // (sorted by length within each workgroup) -> L-tree -> auth walk.  H_msg is
 // serial per message and fills few lanes, so the short messages' pipeline
 // starts while the long ones are still being hashed.  The H pre-images of
 // every layer and request (L-tree merges and auth levels: address only) run
-// once, on the main stream after its F pre-images; the aux stream waits for
-// them before its chains.  (Measured on one MI355X at 4096 requests: three
-// groups, or the H pre-images right after H_msg, were 3-7% slower; so were
-// all 15 steps' F pre-images of the long group on a third stream during its
-// H_msg, 5-8% slower with a long group of 1/4, 1/6 or 1/8 of the batch.)
+// once; only the L-trees wait for them.  The plan (VerifyPlan bits) chooses
+// where they run, the tail kernels, the split and the wave priorities.
 static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* vscratch) {
     const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
     const uint32_t count = (uint32_t)b->count;
     const void* dk = b->keys.p;
     const size_t lb = o->lat_bytes(th);
+    const uint32_t plan = b->plan;
+    const uint32_t div = vplan_div(plan), tail = vplan_tail(plan);
+    const uint32_t prio_long = (plan & VPLAN_PRIO) ? 1u : 0u, prio_hmsg = (plan & VPLAN_PRIO_HMSG) ? 1u : 0u;
+    const bool hfirst = (plan & VPLAN_PRE_H_FIRST) && !b->one_stream;
     uint8_t* dig = (uint8_t*)b->dig.p;
     uint8_t* roots = (uint8_t*)b->roots.p;
     uint8_t* ends = b->ends.need((size_t)count * len * n);
     uint8_t* tmp = b->tmp.need((size_t)count * ((len + 1) / 2) * n);
     constexpr int G = 2;
-    const uint32_t lo[G] = {0, count / 4}, hi[G] = {count / 4, count};
+    const uint32_t lo[G] = {0, count / div}, hi[G] = {count / div, count};
+    const uint32_t prio[G] = {prio_long, 0};
     // one stream: the batch shares the GPU with other engines' batches (a
     // mixed job), which take the other hardware queues
     hipStream_t st[G] = {b->one_stream ? I->L.s_main : I->L.s_aux, I->L.s_main};
+    hipStream_t sh = hfirst ? I->s_copy : I->L.s_main;  // the H pre-images
     hipEvent_t staged = I->L.ev(), hpre = I->L.ev(), done[G - 1] = {I->L.ev()};
     HIPCK(hipEventRecord(staged, I->L.s_main));
     for (int g = 0; g < G - 1; ++g) HIPCK(hipStreamWaitEvent(st[g], staged, 0));
+    if (hfirst) {  // beside H_msg, on a third stream
+        HIPCK(hipStreamWaitEvent(sh, staged, 0));
+        I->L.s = sh;
+        o->lat_pre_h(I->L, dk, b->vd, count * d, th, vscratch);
+        HIPCK(hipEventRecord(hpre, sh));
+    }
     // Launches are issued stage by stage across the groups, so that no stream
     // waits for the host to issue the others' launches.
     for (int g = 0; g < G; ++g) {
         if (hi[g] == lo[g]) continue;
         I->L.s = st[g];
-        o->hmsg(I->L, dk, b->md + lo[g], hi[g] - lo[g], fam_pad(b->fam), b->blob, b->R, dig + (size_t)lo[g] * n);
+        o->hmsg(I->L, dk, b->md + lo[g], hi[g] - lo[g], fam_pad(b->fam), b->blob, b->R, dig + (size_t)lo[g] * n,
+                prio_hmsg | prio[g]);
     }
     for (unsigned L = 0; L < d; ++L) {
-        for (int stage = 0; stage < 2; ++stage) {
-            if (L == 0 && stage == 1) {
+        for (int stage = 0; stage < 3; ++stage) {
+            if (L == 0 && stage == 1 && !hfirst) {
                 I->L.s = I->L.s_main;
                 o->lat_pre_h(I->L, dk, b->vd, count * d, th, vscratch);
                 HIPCK(hipEventRecord(hpre, I->L.s_main));
-                for (int g = 0; g < G - 1; ++g) HIPCK(hipStreamWaitEvent(st[g], hpre, 0));
             }
+            if (L == 0 && stage == 2)  // the L-trees need the H pre-images
+                for (int g = 0; g < G; ++g)
+                    if (st[g] != sh) HIPCK(hipStreamWaitEvent(st[g], hpre, 0));
             for (int g = G - 1; g >= 0; --g) {
                 const uint32_t a = lo[g], m = hi[g] - lo[g];
                 if (!m) continue;
@@ -1373,12 +1511,14 @@ static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* 
                 const uint8_t* msg = L == 0 ? dig + (size_t)a * n : roots + ((size_t)(L - 1) * count + a) * n;
                 const DLeaf* vd = b->vd + t;
                 uint8_t* scr = vscratch + t * lb;
+                uint8_t* e = ends + (size_t)a * len * n;
                 if (stage == 0)
-                    o->pre_need(I->L, dk, vd, m, msg, th, scr);
+                    o->pre_need(I->L, dk, vd, m, msg, th, scr, prio[g]);
+                else if (stage == 1)
+                    o->latb_chains(I->L, m, msg, b->sigs + t * len * n, scr, lb, e, prio[g]);
                 else
-                    o->lat_verify_batch(I->L, vd, m, msg, b->sigs + t * len * n,
-                                        b->auth_in + (size_t)L * count * th * n, th, scr, ends + (size_t)a * len * n,
-                                        tmp + (size_t)a * ((len + 1) / 2) * n, roots + t * n);
+                    o->latb_tail(I->L, vd, m, b->auth_in + (size_t)L * count * th * n, scr, lb, e,
+                                 tmp + (size_t)a * ((len + 1) / 2) * n, roots + t * n, tail, prio[g]);
             }
         }
     }
@@ -1454,6 +1594,100 @@ static void verify_launches(Engine::Impl* I, FamOps* o, Engine::Batch* b) {
     }
 }
 
+static hipGraphExec_t capture_verify(Engine::Impl* I, FamOps* o, Engine::Batch* b) {
+    hipGraph_t g;
+    hipGraphExec_t x = nullptr;
+    HIPCK(hipStreamBeginCapture(I->L.s_main, hipStreamCaptureModeRelaxed));
+    verify_launches(I, o, b);
+    HIPCK(hipStreamEndCapture(I->L.s_main, &g));
+    const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCK(e);
+    return x;
+}
+
+// Settles b->plan before the batch's graph is captured: the forced plan, the
+// one already found for this shape, or -- for a batch on the lane-dense path
+// -- the fastest candidate, timed here (b->graph is then that candidate's
+// graph, whose last replay left the batch's results in place).
+static void settle_plan(Engine::Impl* I, FamOps* o, Engine::Batch* b) {
+    b->tuned = true;
+    VerifyTuner& t = vtuner();
+    const uint64_t key = verify_key(b);
+    {
+        std::lock_guard<std::mutex> lk(t.mu);
+        t.load_locked();
+        if (t.forced >= 0) {
+            b->plan = (uint32_t)t.forced;
+            return;
+        }
+        auto it = t.best.find(key);
+        if (it != t.best.end()) {
+            b->plan = it->second;
+            return;
+        }
+        if (t.off) {
+            b->plan = kVplanDefault;
+            return;
+        }
+    }
+    const size_t nv = (size_t)b->count * b->d;
+    if (!(use_lat(o, nv, b->th) && b->count >= kLatDense)) {  // the plan does not apply
+        b->plan = kVplanDefault;
+        return;
+    }
+    hipEvent_t e0 = I->L.ev(), e1 = I->L.ev();
+    float best_ms = 0, default_ms = 0;
+    uint32_t best_plan = kVplanDefault;
+    hipGraphExec_t best_graph = nullptr;
+    try {
+        for (size_t c = 0; c < kNumVerifyCands; ++c) {
+            b->plan = kVerifyCands[c];
+            hipGraphExec_t x = capture_verify(I, o, b);
+            float ms = 0;
+            for (int rep = 0; rep < 3; ++rep) {
+                HIPCK(hipEventRecord(e0, I->L.s_main));
+                HIPCK(hipGraphLaunch(x, I->L.s_main));
+                HIPCK(hipEventRecord(e1, I->L.s_main));
+                HIPCK(hipEventSynchronize(e1));
+                float r = 0;
+                HIPCK(hipEventElapsedTime(&r, e0, e1));
+                if (rep == 1 || (rep == 2 && r < ms)) ms = r;
+            }
+            if (kVerifyCands[c] == kVplanDefault) default_ms = ms;
+            if (!best_graph || ms < best_ms) {
+                if (best_graph) (void)hipGraphExecDestroy(best_graph);
+                best_graph = x, best_ms = ms, best_plan = kVerifyCands[c];
+            } else {
+                (void)hipGraphExecDestroy(x);
+            }
+        }
+    } catch (...) {
+        if (best_graph) (void)hipGraphExecDestroy(best_graph);
+        I->L.pool.push_back(e0);
+        I->L.pool.push_back(e1);
+        b->plan = kVplanDefault;
+        throw;
+    }
+    I->L.pool.push_back(e0);
+    I->L.pool.push_back(e1);
+    if (best_plan != kVplanDefault && best_ms > 0.98f * default_ms) {
+        // not clearly faster: keep the default (its results are replayed below)
+        (void)hipGraphExecDestroy(best_graph);
+        best_graph = nullptr;
+        best_plan = kVplanDefault;
+    }
+    b->plan = best_plan;
+    if (best_graph) {
+        // the last replay was the last candidate's: replay the kept graph so
+        // that this run's results come from it like every later run's
+        b->graph = best_graph;
+        HIPCK(hipGraphLaunch(b->graph, I->L.s_main));
+    }
+    std::lock_guard<std::mutex> lk(t.mu);
+    t.record_locked(key, best_plan);
+}
+
 // A verification batch: launched directly on its first run (which sizes
 // every buffer), captured into a HIP graph on its second and replayed from
 // then on -- one launch call instead of tens.  The capture holds no
@@ -1473,12 +1707,11 @@ static void run_verify(Engine::Impl* I, Engine::Batch* b) {
         verify_launches(I, o, b);
         return;
     }
-    hipGraph_t g;
-    HIPCK(hipStreamBeginCapture(I->L.s_main, hipStreamCaptureModeRelaxed));
-    verify_launches(I, o, b);
-    HIPCK(hipStreamEndCapture(I->L.s_main, &g));
-    HIPCK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
-    HIPCK(hipGraphDestroy(g));
+    if (!b->tuned) settle_plan(I, o, b);
+    if (b->graph) {  // tuning left the best candidate's graph (and its results)
+        return;
+    }
+    b->graph = capture_verify(I, o, b);
     HIPCK(hipGraphLaunch(b->graph, I->L.s_main));
 }
 

@@ -93,6 +93,25 @@ void xmss_mi355x_reset_stats(void) {
     if (Engine* e = Engine::get()) e->reset_stats();
 }
 
+int xmss_mi355x_set_verify_plan(int plan) { return xm::set_verify_plan(plan); }
+
+size_t xmss_mi355x_verify_plans(unsigned* plans, size_t max) {
+    std::vector<uint32_t> v(16);
+    const size_t n = xm::verify_plans(v.data(), v.size());
+    for (size_t i = 0; i < n && i < max && plans; ++i) plans[i] = v[i];
+    return n;
+}
+
+size_t xmss_mi355x_tuned_plans(char* buf, size_t len) {
+    const std::string s = xm::tuned_plans();
+    if (buf && len) {
+        const size_t m = std::min(len - 1, s.size());
+        memcpy(buf, s.data(), m);
+        buf[m] = 0;
+    }
+    return s.size() + 1;
+}
+
 size_t xmss_mi355x_kernel_stats(char* buf, size_t len) {
     std::lock_guard<std::mutex> lk(engine_mutex());
     std::string s;

@@ -40,6 +40,14 @@ constexpr size_t kMsgSlack = 256;
 constexpr uint32_t MODE_PKGEN = 0, MODE_SIGN = 1, MODE_VERIFY = 2;
 constexpr int WG = 64;
 
+// Issue priority of the calling wave on its SIMD (s_setprio): kernels on a
+// batch's critical path run at 2 so that, sharing a SIMD with the waves of
+// wider work launched beside them, they are not slowed by it.  0 leaves the
+// default.
+__device__ __forceinline__ void wave_priority(uint32_t prio) {
+    if (prio) __builtin_amdgcn_s_setprio(2);
+}
+
 template <class F>
 __global__ __launch_bounds__(WG) void k_prepare(const uint8_t* __restrict__ seeds, int nkeys,
                                                  typename F::Key* __restrict__ keys) {
@@ -312,9 +320,10 @@ template <class F>
 __global__ __launch_bounds__(WG) void k_hmsg(const typename F::Key* __restrict__ keys,
                                               const DMsg* __restrict__ reqs, uint32_t n, uint32_t pad,
                                               uint8_t* __restrict__ blob, uint8_t* __restrict__ R_out,
-                                              uint8_t* __restrict__ dig_out) {
+                                              uint8_t* __restrict__ dig_out, uint32_t prio) {
     const uint32_t r = blockIdx.x * WG + threadIdx.x;
     if (r >= n) return;
+    wave_priority(prio);
     hmsg_one<F>(keys, reqs, r, pad, blob, R_out, dig_out);
 }
 

@@ -153,7 +153,8 @@ __global__ __launch_bounds__(WG) void k_ver_pre(const typename F::Key* __restric
 template <class F>
 __global__ __launch_bounds__(WG) void k_lat_pre_h(const typename F::Key* __restrict__ keys,
                                                    const DLeaf* __restrict__ reqs, uint32_t n, uint32_t th,
-                                                   uint8_t* __restrict__ scratch, size_t stride) {
+                                                   uint8_t* __restrict__ scratch, size_t stride, uint32_t prio) {
+    wave_priority(prio);
     const uint32_t nh = F::LEN - 1 + th;
     const uint32_t t = blockIdx.x * WG + threadIdx.x;
     if (t >= n * nh) return;
@@ -399,10 +400,11 @@ template <class F>
 __global__ __launch_bounds__(256) void k_pre_need(const typename F::Key* __restrict__ keys,
                                                    const DLeaf* __restrict__ reqs,
                                                    const uint8_t* __restrict__ msgs,
-                                                   uint8_t* __restrict__ scratch, size_t stride) {
+                                                   uint8_t* __restrict__ scratch, size_t stride, uint32_t prio) {
     constexpr uint32_t LEN = F::LEN;
     __shared__ uint32_t first[LEN + 1];  // first needed item of each chain; first[LEN] = all
     __shared__ uint32_t lens[LEN];
+    wave_priority(prio);
     const uint32_t r = blockIdx.x;
     const DLeaf q = reqs[r];
     const uint8_t* msg = msgs + (size_t)r * F::kN;
@@ -499,9 +501,10 @@ template <class F>
 __global__ __launch_bounds__(LATB_CHAINS_WG) void k_latb_chains(uint32_t n, const uint8_t* __restrict__ msgs,
                                                                  const uint8_t* __restrict__ sigs,
                                                                  const uint8_t* scratch, size_t stride,
-                                                                 uint8_t* __restrict__ ends) {
+                                                                 uint8_t* __restrict__ ends, uint32_t prio) {
     using FPre = typename F::FPre;
     __shared__ uint32_t hist[16];
+    wave_priority(prio);
     __shared__ uint32_t order[LATB_CHAINS_WG];
     __shared__ uint8_t first[LATB_CHAINS_WG];
     const uint32_t total = n * (uint32_t)F::LEN;
@@ -552,8 +555,10 @@ template <class F>
 __global__ __launch_bounds__(WG) void k_latb_ltree(uint32_t n, uint32_t l, uint32_t off,
                                                     const uint8_t* __restrict__ src, uint32_t src_stride,
                                                     uint8_t* __restrict__ dst, uint32_t dst_stride,
-                                                    const uint8_t* __restrict__ scratch, size_t stride) {
+                                                    const uint8_t* __restrict__ scratch, size_t stride,
+                                                    uint32_t prio) {
     using HPre = typename F::HPre;
+    wave_priority(prio);
     const uint32_t half = (l + 1) >> 1;
     const uint32_t g = blockIdx.x * WG + threadIdx.x;
     if (g >= n * half) return;
@@ -571,14 +576,79 @@ __global__ __launch_bounds__(WG) void k_latb_ltree(uint32_t n, uint32_t l, uint3
     F::store(o, dst + ((size_t)r * dst_stride + i) * F::kN);
 }
 
+// The whole L-tree (xmss_commons.c:16-49) of LtreeBatch<F>::R verification
+// requests per workgroup through LDS, in one launch instead of one per
+// height: the chain ends are loaded once, and at each height the threads are
+// packed densely over (request, pair), so that the waves of the shrinking
+// heights drop out instead of idling.  With AUTH the first R lanes then walk
+// their requests' auth paths (compute_root, xmss_commons.c:54-96) and write
+// the roots; without, they write the L-tree roots (leaves) for k_latb_auth.
+template <class F>
+struct LtreeBatch {
+    static constexpr uint32_t THREADS = 256;
+    static constexpr uint32_t R = THREADS / (F::LEN / 2);  // every first-height pair has a lane
+};
+
+template <class F, bool AUTH>
+__global__ __launch_bounds__(256) void k_latb_tree(const DLeaf* __restrict__ reqs, uint32_t n,
+                                                   const uint8_t* __restrict__ ends, const uint8_t* auth,
+                                                   const uint8_t* scratch, size_t stride,
+                                                   uint8_t* __restrict__ out, uint32_t prio) {
+    using Node = typename F::Node;
+    using HPre = typename F::HPre;
+    constexpr uint32_t LEN = F::LEN, R = LtreeBatch<F>::R, T = LtreeBatch<F>::THREADS;
+    __shared__ Node pk[R * LEN];
+    wave_priority(prio);
+    const uint32_t r0 = blockIdx.x * R;
+    const uint32_t nr = n - r0 < R ? n - r0 : R;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < nr * LEN; i += T) F::load(pk[i], ends + ((size_t)r0 * LEN + i) * F::kN);
+    __syncthreads();
+    uint32_t l = LEN, off = 0;
+    while (l > 1) {
+        const uint32_t parents = l >> 1;
+        const uint32_t j = t / parents, c = t % parents;  // request j of the group, pair c
+        const bool act = j < nr;
+        Node o;
+        if (act) {
+            const HPre* hp = reinterpret_cast<const HPre*>(scratch + (size_t)(r0 + j) * stride + LatLayout<F>::hpre);
+            F::fin_h(o, hp[off + c], pk[j * LEN + 2 * c], pk[j * LEN + 2 * c + 1]);
+        }
+        __syncthreads();
+        if (act) pk[j * LEN + c] = o;
+        if ((l & 1) && t < nr) pk[t * LEN + parents] = pk[t * LEN + l - 1];  // odd last node carried up
+        __syncthreads();
+        off += parents;
+        l = (l + 1) >> 1;
+    }
+    if (t >= nr) return;
+    const uint32_t r = r0 + t;
+    Node node = pk[t * LEN];
+    if (AUTH) {
+        const DLeaf q = reqs[r];
+        const HPre* hp = reinterpret_cast<const HPre*>(scratch + (size_t)r * stride + LatLayout<F>::hpre);
+        uint32_t idx = q.leaf;
+        const uint8_t* ap = auth + (size_t)q.p2 * F::kN;
+        for (uint32_t i = 0; i < q.p0; ++i) {
+            Node sib;
+            F::load(sib, ap + (size_t)i * F::kN);
+            if (idx & 1) F::fin_h(node, hp[LEN - 1 + i], sib, node);
+            else F::fin_h(node, hp[LEN - 1 + i], node, sib);
+            idx >>= 1;
+        }
+    }
+    F::store(node, out + (size_t)r * F::kN);
+}
+
 // Auth walk (compute_root, xmss_commons.c:54-96) of every request, from its
 // L-tree root at leaves[r * leaf_stride].
 template <class F>
 __global__ __launch_bounds__(WG) void k_latb_auth(const DLeaf* __restrict__ reqs, uint32_t n,
                                                    const uint8_t* __restrict__ leaves, uint32_t leaf_stride,
                                                    const uint8_t* auth, const uint8_t* scratch, size_t stride,
-                                                   uint8_t* __restrict__ out) {
+                                                   uint8_t* __restrict__ out, uint32_t prio) {
     using HPre = typename F::HPre;
+    wave_priority(prio);
     const uint32_t r = blockIdx.x * WG + threadIdx.x;
     if (r >= n) return;
     const DLeaf q = reqs[r];

@@ -149,12 +149,19 @@ struct FamOps {
     // H pre-images (address only), pre_need the F pre-images of only the
     // chain steps their signatures need, from their messages.
     virtual void lat_pre_h(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, uint32_t th,
-                           uint8_t* scratch) = 0;
+                           uint8_t* scratch, uint32_t prio = 0) = 0;
     virtual void pre_need(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
-                          uint32_t th, uint8_t* scratch) = 0;
-    virtual void lat_verify_batch(Launcher& L, const DLeaf* reqs, uint32_t n, const uint8_t* msgs,
-                                  const uint8_t* sigs, const uint8_t* auth, uint32_t th, const uint8_t* scratch,
-                                  uint8_t* ends, uint8_t* tmp, uint8_t* out) = 0;
+                          uint32_t th, uint8_t* scratch, uint32_t prio = 0) = 0;
+    // Large verification batches, lane-dense: the chains (ends[r][c]) and
+    // then the L-tree and auth walk (roots in out).  tail: 0 = one launch per
+    // L-tree height, then the auth walks; 1 = the whole L-tree through LDS in
+    // one launch, then the auth walks; 2 = L-tree and auth walk in one
+    // launch.  prio: the waves' issue priority (wave_priority).
+    virtual void latb_chains(Launcher& L, uint32_t n, const uint8_t* msgs, const uint8_t* sigs,
+                             const uint8_t* scratch, size_t stride, uint8_t* ends, uint32_t prio) = 0;
+    virtual void latb_tail(Launcher& L, const DLeaf* reqs, uint32_t n, const uint8_t* auth, const uint8_t* scratch,
+                           size_t stride, uint8_t* ends, uint8_t* tmp, uint8_t* out, uint32_t tail,
+                           uint32_t prio) = 0;
     // Subtree levels of a few runs split into address-only and finishing
     // work: pre_tree fills 2^th - 1 pre-images per run (tree_pre_slot
     // order), levels_fin builds levels 0 .. th-1 from them.
@@ -163,7 +170,7 @@ struct FamOps {
     virtual void thash_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, const uint8_t* in,
                          uint8_t* out) = 0;
     virtual void hmsg(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad,
-                      uint8_t* blob, uint8_t* R, uint8_t* dig) = 0;
+                      uint8_t* blob, uint8_t* R, uint8_t* dig, uint32_t prio = 0) = 0;
     virtual size_t hpre_bytes() const = 0;
     // BDS signatures: R and H_msg of the nsign messages, the planned leaves
     // (LAT_LEAF requests) and the layer-0 WOTS signatures (LAT_SIGN requests
@@ -284,14 +291,14 @@ struct FamOpsT final : FamOps {
               (const uint8_t*)scratch, stride, out);
     }
     void lat_pre_h(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, uint32_t th,
-                   uint8_t* scratch) override {
+                   uint8_t* scratch, uint32_t prio = 0) override {
         L.run("lat_pre_h", k_lat_pre_h<F>, dim3(cdiv((uint64_t)n * (F::LEN - 1 + th), WG)), dim3(WG),
-              (const K*)keys, reqs, n, th, scratch, LatLayout<F>::bytes(th));
+              (const K*)keys, reqs, n, th, scratch, LatLayout<F>::bytes(th), prio);
     }
     void pre_need(Launcher& L, const void* keys, const DLeaf* reqs, uint32_t n, const uint8_t* msgs, uint32_t th,
-                  uint8_t* scratch) override {
+                  uint8_t* scratch, uint32_t prio = 0) override {
         L.run("pre_need", k_pre_need<F>, dim3(n), dim3(256), (const K*)keys, reqs, msgs, scratch,
-              LatLayout<F>::bytes(th));
+              LatLayout<F>::bytes(th), prio);
     }
     void lat_fin(Launcher& L, uint32_t mode, const DLeaf* reqs, uint32_t n, const uint8_t* msgs, const uint8_t* sigs,
                  const uint8_t* auth, uint32_t th, const uint8_t* scratch, uint8_t* out) override {
@@ -316,33 +323,47 @@ struct FamOpsT final : FamOps {
                   nodes, hp);
         L.run("top_fin", k_top_fin<F>, dim3(nruns), dim3(64), th, top, nodes, hp);
     }
-    void lat_verify_batch(Launcher& L, const DLeaf* reqs, uint32_t n, const uint8_t* msgs, const uint8_t* sigs,
-                          const uint8_t* auth, uint32_t th, const uint8_t* scratch, uint8_t* ends, uint8_t* tmp,
-                          uint8_t* out) override {
-        const size_t stride = LatLayout<F>::bytes(th);
+    void latb_chains(Launcher& L, uint32_t n, const uint8_t* msgs, const uint8_t* sigs, const uint8_t* scratch,
+                     size_t stride, uint8_t* ends, uint32_t prio) override {
         L.run("latb_chains", k_latb_chains<F>, dim3(cdiv((uint64_t)n * F::LEN, LATB_CHAINS_WG)),
-              dim3(LATB_CHAINS_WG), n, msgs, sigs, scratch, stride, ends);
+              dim3(LATB_CHAINS_WG), n, msgs, sigs, scratch, stride, ends, prio);
+    }
+    void latb_tail(Launcher& L, const DLeaf* reqs, uint32_t n, const uint8_t* auth, const uint8_t* scratch,
+                   size_t stride, uint8_t* ends, uint8_t* tmp, uint8_t* out, uint32_t tail, uint32_t prio) override {
+        if (tail != 0) {
+            const dim3 grid(cdiv(n, LtreeBatch<F>::R)), block(LtreeBatch<F>::THREADS);
+            if (tail == 2) {
+                L.run("latb_tree_auth", k_latb_tree<F, true>, grid, block, reqs, n, (const uint8_t*)ends, auth,
+                      scratch, stride, out, prio);
+                return;
+            }
+            L.run("latb_tree", k_latb_tree<F, false>, grid, block, reqs, n, (const uint8_t*)ends, auth, scratch,
+                  stride, tmp, prio);
+            L.run("latb_auth", k_latb_auth<F>, dim3(cdiv(n, WG)), dim3(WG), reqs, n, (const uint8_t*)tmp, 1u, auth,
+                  scratch, stride, out, prio);
+            return;
+        }
         uint32_t l = F::LEN, off = 0;
         bool from_a = true;
         const uint32_t sa = F::LEN, sb = (F::LEN + 1) / 2;
         while (l > 1) {
             L.run("latb_ltree", k_latb_ltree<F>, dim3(cdiv((uint64_t)n * ((l + 1) / 2), WG)), dim3(WG), n, l, off,
                   (const uint8_t*)(from_a ? ends : tmp), from_a ? sa : sb, from_a ? tmp : ends, from_a ? sb : sa,
-                  scratch, stride);
+                  scratch, stride, prio);
             off += l >> 1;
             l = (l + 1) >> 1;
             from_a = !from_a;
         }
         L.run("latb_auth", k_latb_auth<F>, dim3(cdiv(n, WG)), dim3(WG), reqs, n,
-              (const uint8_t*)(from_a ? ends : tmp), from_a ? sa : sb, auth, scratch, stride, out);
+              (const uint8_t*)(from_a ? ends : tmp), from_a ? sa : sb, auth, scratch, stride, out, prio);
     }
     void thash_h(Launcher& L, const void* keys, const DThash* reqs, uint32_t n, const uint8_t* in,
                  uint8_t* out) override {
         L.run("thash_h", k_thash_h<F>, dim3(cdiv(n, WG)), dim3(WG), (const K*)keys, reqs, n, in, out);
     }
     void hmsg(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad, uint8_t* blob,
-              uint8_t* R, uint8_t* dig) override {
-        L.run("hmsg", k_hmsg<F>, dim3(cdiv(n, WG)), dim3(WG), (const K*)keys, reqs, n, pad, blob, R, dig);
+              uint8_t* R, uint8_t* dig, uint32_t prio = 0) override {
+        L.run("hmsg", k_hmsg<F>, dim3(cdiv(n, WG)), dim3(WG), (const K*)keys, reqs, n, pad, blob, R, dig, prio);
     }
     size_t hpre_bytes() const override { return sizeof(typename F::HPre); }
     void bds_lat(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad, uint8_t* blob,

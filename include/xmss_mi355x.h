@@ -112,6 +112,25 @@ int xmss_mi355x_verify_refill(xmss_mi355x_job *job, const unsigned char *const *
  * are captured into a HIP graph on its second run and replayed afterwards.
  * job_run = job_launch + job_wait. */
 int xmss_mi355x_job_run(xmss_mi355x_job *job);
+/* Autotuner of the verification-batch schedule.  How a large verification
+ * batch is laid onto the GPU (how its requests split into groups by message
+ * length, which launches finish the L-trees and auth paths, wave priorities,
+ * where the address-only work runs) is one of a few candidate plans; the
+ * best depends on the parameter set, the batch size and whether the batch
+ * shares the GPU with other sets.  A job of a shape not seen before times
+ * every candidate on its second run -- each captured into a HIP graph and
+ * replayed on the job's own inputs, every replay a whole verification -- and
+ * keeps the fastest for itself and every later job of that shape (and, when
+ * XMSS_MI355X_TUNE_FILE names a file, for later processes; XMSS_MI355X_TUNE=0
+ * keeps the default plan).  No counterpart in the reference, whose launch
+ * shapes are compile-time constants (cuxmss/gpu/gpu_verify.cu:393-502).
+ * set_verify_plan forces one plan for jobs run from now on (plan >= 0) or
+ * returns to tuning (-1) and returns the previous setting; verify_plans
+ * writes up to max candidate plans and returns their number; tuned_plans
+ * writes one line per settled shape and returns the bytes needed. */
+int xmss_mi355x_set_verify_plan(int plan);
+size_t xmss_mi355x_verify_plans(unsigned *plans, size_t max);
+size_t xmss_mi355x_tuned_plans(char *buf, size_t len);
 /* job_launch queues a job's device work and returns at once; job_wait
  * blocks until it is done.  Refill, collect and free of a launched job wait
  * for it by themselves.  A verification job's input copy (prepare, refill)
