@@ -1609,7 +1609,7 @@ static hipGraphExec_t capture_verify(Engine::Impl* I, FamOps* o, Engine::Batch* 
 // Settles b->plan before the batch's graph is captured: the forced plan, the
 // one already found for this shape, or -- for a batch on the lane-dense path
 // -- the fastest candidate, timed here (b->graph is then that candidate's
-// graph, whose last replay left the batch's results in place).
+// graph, replayed once more so that its results are the run's).
 static void settle_plan(Engine::Impl* I, FamOps* o, Engine::Batch* b) {
     b->tuned = true;
     VerifyTuner& t = vtuner();
@@ -1672,7 +1672,8 @@ static void settle_plan(Engine::Impl* I, FamOps* o, Engine::Batch* b) {
     I->L.pool.push_back(e0);
     I->L.pool.push_back(e1);
     if (best_plan != kVplanDefault && best_ms > 0.98f * default_ms) {
-        // not clearly faster: keep the default (its results are replayed below)
+        // not clearly faster: keep the default, captured by the caller (every
+        // candidate's last replay was waited for: the graph is idle)
         (void)hipGraphExecDestroy(best_graph);
         best_graph = nullptr;
         best_plan = kVplanDefault;

@@ -2,8 +2,9 @@
 three sets XMSSMT-SHA2_20/2_256 (SHA-256), XMSS-SHA2_10_512 (SHA-512, 131
 chains) and XMSS-SHAKE_10_256 (SHAKE128), 260 plain-core signatures each, in
 ONE mixed job (more than 256 requests per set: the throughput kernels), then
-one mixed verification job over all of them
-(tests/golden/mixed_sets.json, written by the reference's xmss_core.c)."""
+one mixed verification job over all of them, run directly, then as one HIP
+graph over the three sets, replayed and refilled (tests/golden/mixed_sets.json,
+written by the reference's xmss_core.c)."""
 import ctypes as C
 import random
 
@@ -88,12 +89,34 @@ def test_mixed_job_of_the_three_sets_equals_reference():
     mls = (C.c_ulonglong * k)()
     vr = (C.c_int * k)()
     job = L.xmss_mi355x_mixed_verify_prepare(mts, _arr(smb), smls, _arr(pkb), k)
-    assert L.xmss_mi355x_job_run(job) == 0
-    assert L.xmss_mi355x_verify_collect(job, _arr(outs), mls, vr) == 0
-    L.xmss_mi355x_job_free(job)
-    for i, c in enumerate(cases):
-        if i in flipped:
-            assert vr[i] == -1 and mls[i] == 0
-        else:
-            assert vr[i] == c[6]["verify"] == 0, "verify item %d" % i
-            assert raw(outs[i], mls[i]) == c[4]
+    L.xmss_mi355x_verify_refill.argtypes = [C.c_void_p] * 4
+    # runs: direct launches, the sets captured into one HIP graph, two replays;
+    # then a refill with the flips moved to other requests and two replays
+    flips = [flipped]
+    try:
+        for run in range(6):
+            if run == 4:
+                vsm = list(out)
+                moved = set()
+                for t in flipped:
+                    u = (t + 1) % k
+                    b = bytearray(vsm[u])
+                    b[cases[u][2].index_bytes + cases[u][2].n + 7] ^= 4
+                    vsm[u] = bytes(b)
+                    moved.add(u)
+                smb = [buf(s) for s in vsm]
+                smls = (C.c_ulonglong * k)(*[len(s) for s in vsm])
+                outs = [buf(len(s)) for s in vsm]
+                assert L.xmss_mi355x_verify_refill(job, _arr(smb), smls, _arr(pkb)) == 0
+                flips.append(moved)
+            assert L.xmss_mi355x_job_run(job) == 0
+            assert L.xmss_mi355x_verify_collect(job, _arr(outs), mls, vr) == 0
+            bad = flips[-1]
+            for i, c in enumerate(cases):
+                if i in bad:
+                    assert vr[i] == -1 and mls[i] == 0, "run %d item %d accepted" % (run, i)
+                else:
+                    assert vr[i] == c[6]["verify"] == 0, "run %d verify item %d" % (run, i)
+                    assert raw(outs[i], mls[i]) == c[4]
+    finally:
+        L.xmss_mi355x_job_free(job)
