@@ -361,6 +361,10 @@ public:
     // copies are ordered after it on the same stream).
     Batch* sign_prepare(int fam, unsigned th, unsigned d, int count, const SignItem* items, bool one_key = false,
                         bool wait = true);
+    // A signing batch of the job API: its run sizes every workspace in a pass
+    // without launches, then captures its launches into one HIP graph and
+    // launches that (replayed if the batch runs again).
+    void set_graphed(Batch* b, bool on);
     Batch* verify_prepare(int fam, unsigned th, unsigned d, int count, const VerifyItem* items);
     // New inputs for a verification batch of the same shape (count items of
     // its parameter set), staged into its buffers; its captured launch graph

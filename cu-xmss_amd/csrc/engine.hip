@@ -989,11 +989,12 @@ struct Engine::Batch {
     DevBuf lat;               // pre-image scratch of the latency kernels
     DevBuf lreq;              // sign: request descriptors of the latency kernels
     DevBuf wlat, wreq;        // sign: the same for the WOTS signatures (layer-major)
-    hipGraphExec_t graph = nullptr;  // verify: the captured launches of run()
+    hipGraphExec_t graph = nullptr;  // the captured launches of run() (verify; graphed sign)
     int nrun = 0;                    // verify: runs since staging
     PinnedBuf pin;                   // host staging of this batch's inputs and results
     size_t seed_off = 0, seed_bytes = 0;  // sign: the seeds' region in `pin` and at `seeds`
     bool one_stream = false;         // verify: all launches on the main stream (set_one_stream)
+    bool graphed = false;            // sign: run() captures its launches into `graph`
     uint32_t plan = 0;               // verify: the schedule of verify_dense (VPLAN_* bits)
     bool tuned = false;              // verify: plan settled (forced, cached or tuned)
     hipEvent_t done = nullptr;       // recorded after the work of launch()
@@ -1134,6 +1135,7 @@ Engine::Batch* Engine::sign_prepare(int fam, unsigned th, unsigned d, int count,
                                     bool one_key, bool wait) {
     Impl* I = impl_;
     Batch* b = take_batch(I, true);
+    b->graphed = false;
     BatchGuard guard{this, b};
     b->sign = true;
     b->fam = fam;
@@ -1716,6 +1718,13 @@ static void run_verify(Engine::Impl* I, Engine::Batch* b) {
     HIPCK(hipGraphLaunch(b->graph, I->L.s_main));
 }
 
+void Engine::set_graphed(Batch* b, bool on) {
+    if (b->graphed != on) b->drop_graph();
+    b->graphed = on;
+}
+
+static void sign_launches(Engine::Impl* I, Engine::Batch* b);
+
 void Engine::run(Batch* b, bool wait_done) {
     Impl* I = impl_;
     if (!b->sign) {
@@ -1723,6 +1732,39 @@ void Engine::run(Batch* b, bool wait_done) {
         if (wait_done) I->sync();
         return;
     }
+    if (b->graphed && !I->L.prof) {
+        if (!b->graph) {
+            I->L.dry = true;  // sizes every workspace: the capture allocates nothing
+            try {
+                sign_launches(I, b);
+            } catch (...) {
+                I->L.dry = false;
+                throw;
+            }
+            I->L.dry = false;
+            hipGraph_t g = nullptr;
+            HIPCK(hipStreamBeginCapture(I->L.s_main, hipStreamCaptureModeRelaxed));
+            try {
+                sign_launches(I, b);
+            } catch (...) {
+                if (hipStreamEndCapture(I->L.s_main, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPCK(hipStreamEndCapture(I->L.s_main, &g));
+            const hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCK(e);
+        }
+        HIPCK(hipGraphLaunch(b->graph, I->L.s_main));
+    } else {
+        sign_launches(I, b);
+    }
+    if (wait_done) I->sync();
+}
+
+// The launches of a signing batch (on the main stream, forking to the aux
+// stream and joining it again); with L.dry only the workspaces are sized.
+static void sign_launches(Engine::Impl* I, Engine::Batch* b) {
     FamOps* o = ops_of(b->fam);
     const unsigned n = b->n, len = b->len, th = b->th, d = b->d;
     const int count = b->count;
@@ -1784,8 +1826,9 @@ void Engine::run(Batch* b, bool wait_done) {
             uint8_t* lv = b->tmp.need(nlv * n);
             o->lat_leaves_dense(I->L, dk, dl, (uint32_t)nlv, b->lat.need(nlv * o->lat_bytes(0)),
                                 b->ends.need(nlv * len * n), lv);
-            HIPCK(hipMemcpy2DAsync(nodes, stride * n, lv, leaves * n, leaves * n, nruns, hipMemcpyDeviceToDevice,
-                                   I->L.s));
+            if (!I->L.dry)
+                HIPCK(hipMemcpy2DAsync(nodes, stride * n, lv, leaves * n, leaves * n, nruns,
+                                       hipMemcpyDeviceToDevice, I->L.s));
             HIPCK(hipStreamWaitEvent(I->L.s_main, tdone, 0));
             o->levels_fin(I->L, nruns, th, nodes, tpre);
         }
@@ -1834,14 +1877,14 @@ void Engine::run(Batch* b, bool wait_done) {
         for (unsigned L = 1; L < d; ++L) {
             // compact roots of layer L-1 into msgs[L-1]
             uint8_t* dst = msgs + (size_t)(L - 1) * count * n;
-            HIPCK(hipMemcpy2DAsync(dst, n, b->auth + ((size_t)(L - 1) * count) * rec + th * n, rec, n, count,
-                                   hipMemcpyDeviceToDevice, I->L.s));
+            if (!I->L.dry)
+                HIPCK(hipMemcpy2DAsync(dst, n, b->auth + ((size_t)(L - 1) * count) * rec + th * n, rec, n, count,
+                                       hipMemcpyDeviceToDevice, I->L.s));
             sign_layer(L, dst);
         }
     }
     HIPCK(hipStreamWaitEvent(I->L.s_main, wdone, 0));  // the layer-0 signatures
     I->L.pool.push_back(wdone);
-    if (wait_done) I->sync();
 }
 
 // Results of a batch run with run() leave on its own stream, ordered after

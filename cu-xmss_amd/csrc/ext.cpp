@@ -232,8 +232,9 @@ xmss_mi355x_job* sign_prepare(const int* mts, int mt_all, unsigned char* const* 
         }
         for (size_t s = 0; s < j->sets.size(); ++s) {
             JobSet& js = j->sets[s];
-            if (!js.live.empty())
-                js.b = js.e->sign_prepare(js.fam, js.p.tree_height, js.p.d, (int)items[s].size(), items[s].data());
+            if (js.live.empty()) continue;
+            js.b = js.e->sign_prepare(js.fam, js.p.tree_height, js.p.d, (int)items[s].size(), items[s].data());
+            js.e->set_graphed(js.b, true);  // its launches go to the GPU as one HIP graph
         }
     } catch (const DeviceError&) {
         fail_job(j);

@@ -36,6 +36,9 @@ struct Launcher {
     hipStream_t s = nullptr;
     hipStream_t s_main = nullptr, s_aux = nullptr;  // s is one of the two
     bool prof = false;
+    // A sizing pass: launches are skipped (the caller skips its copies too),
+    // so that a launch sequence sizes every workspace before it is captured.
+    bool dry = false;
     struct Pending {
         const char* name;
         hipEvent_t e0, e1;
@@ -62,7 +65,7 @@ struct Launcher {
     }
     template <typename K, typename... Args>
     void run_lds(const char* name, K kernel, dim3 grid, dim3 block, size_t lds, Args... args) {
-        if (grid.x == 0 || grid.y == 0) return;
+        if (grid.x == 0 || grid.y == 0 || dry) return;
         hipEvent_t e0 = nullptr;
         if (prof) {
             e0 = ev();

@@ -127,3 +127,46 @@ def test_batch_sign_320_keys_equals_reference_on_the_throughput_kernels():
     for j, (k, _) in enumerate(signed):
         if want[j] == 0:
             assert raw(outs[j], mls[j]) == msgs[k]
+
+
+def test_batch_sign_320_keys_as_one_graph_equals_reference():
+    """The same batch without per-kernel timing: the job's launches are sized
+    in a pass without launches, captured into one HIP graph and launched as
+    that; a second run replays the graph and gives the same bytes."""
+    g = golden("plain_batch_large.json")
+    lib = product(0)
+    L = lib.lib
+    oid = g["oid"]
+    p = lib.params(oid, 0)
+    seeds, msgs = _inputs(g, p)
+    items = g["items"]
+    count = len(items)
+    roots = [bytes.fromhex(it["root"]) for it in items]
+    sk_in = [W.plain_sk(oid, p, it["idx"], seeds[k], roots[k]) for k, it in enumerate(items)]
+    sks = [buf(s) for s in sk_in]
+    mbufs = [buf(m) for m in msgs]
+    mlens = (C.c_ulonglong * count)(*[len(m) for m in msgs])
+    L.xmss_mi355x_sign_prepare.restype = C.c_void_p
+    L.xmss_mi355x_sign_prepare.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    L.xmss_mi355x_job_run.argtypes = [C.c_void_p]
+    L.xmss_mi355x_sign_collect.argtypes = [C.c_void_p] * 4
+    L.xmss_mi355x_job_free.argtypes = [C.c_void_p]
+    job = L.xmss_mi355x_sign_prepare(0, _arr(sks), _arr(mbufs), mlens, count)
+    try:
+        outs = []
+        for run in range(2):
+            assert L.xmss_mi355x_job_run(job) == 0
+            sms = [buf(p.sig_bytes + len(m)) for m in msgs]
+            smlens = (C.c_ulonglong * count)()
+            rets = (C.c_int * count)()
+            assert L.xmss_mi355x_sign_collect(job, _arr(sms), smlens, rets) == 0
+            outs.append([raw(sms[k], smlens[k]) for k in range(count)])
+            for k, it in enumerate(items):
+                assert rets[k] == it["rc"], "run %d key %d rc" % (run, k)
+                if it["rc"] == 0:
+                    assert dg(outs[-1][k]) == it["sm"], "run %d key %d (idx %d) signature" % (run, k, it["idx"])
+        assert outs[0] == outs[1]
+        for k, it in enumerate(items):
+            assert dg(raw(sks[k], len(sk_in[k]))) == it["sk_out"], "key %d sk" % k
+    finally:
+        L.xmss_mi355x_job_free(job)
