@@ -64,6 +64,17 @@ def load_product(core):
     return api
 
 
+def tuned_plans(L):
+    """The verification schedules the library's autotuner settled on, one
+    line per shape (parameter family, tree height, layers, batch size)."""
+    L.xmss_mi355x_tuned_plans.argtypes = [C.c_char_p, C.c_size_t]
+    L.xmss_mi355x_tuned_plans.restype = C.c_size_t
+    n = L.xmss_mi355x_tuned_plans(None, 0)
+    b = C.create_string_buffer(n + 1)
+    L.xmss_mi355x_tuned_plans(b, n + 1)
+    return b.value.decode().splitlines()
+
+
 def kernel_stats(L):
     n = L.xmss_mi355x_kernel_stats(None, 0)
     b = C.create_string_buffer(n + 1)
@@ -901,6 +912,7 @@ def main():
         "one_key_many_signs": opk,
         "mixed_batch": mixed,
         "serving": serve,
+        "verify_tuned_plans": tuned_plans(api.lib),
     }
     print(json.dumps(out), flush=True)
 

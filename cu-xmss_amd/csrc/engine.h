@@ -375,6 +375,13 @@ public:
     // Batches of several engines then run side by side on as many hardware
     // queues instead of contending for them (mixed jobs).
     void set_one_stream(Batch* b, bool on);
+    // The stream of this engine a verification batch's launches go to (0:
+    // main, 1: aux, 2: the copy stream).  The three streams of an engine are
+    // created one after another, so HIP gives them three different hardware
+    // queues: the sets of a mixed job, each kept to one stream (set_one_stream)
+    // on one engine, run side by side whatever streams the process created
+    // before.
+    void set_stream(Batch* b, int sel);
     // wait_done = false returns once the work is queued; wait() then blocks
     // until this engine's streams are idle.  A verification batch's launches
     // are captured into a HIP graph on its second run and replayed from then

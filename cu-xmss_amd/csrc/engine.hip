@@ -995,6 +995,7 @@ struct Engine::Batch {
     size_t seed_off = 0, seed_bytes = 0;  // sign: the seeds' region in `pin` and at `seeds`
     bool one_stream = false;         // verify: all launches on the main stream (set_one_stream)
     bool graphed = false;            // sign: run() captures its launches into `graph`
+    int sel = 0;                     // verify: the engine stream its launches go to (set_stream)
     uint32_t plan = 0;               // verify: the schedule of verify_dense (VPLAN_* bits)
     bool tuned = false;              // verify: plan settled (forced, cached or tuned)
     hipEvent_t done = nullptr;       // recorded after the work of launch()
@@ -1335,6 +1336,28 @@ Engine::Batch* Engine::verify_prepare(int fam, unsigned th, unsigned d, int coun
 
 void Engine::verify_refill(Batch* b, const VerifyItem* items) { stage_verify(impl_, ops_of(b->fam), b, items); }
 
+void Engine::set_stream(Batch* b, int sel) {
+    sel = sel < 0 || sel > 2 ? 0 : sel;
+    if (b->sel != sel) b->drop_graph();  // the graph is launched where it was captured
+    b->sel = sel;
+}
+
+// While alive, the engine's main stream is the one a verification batch was
+// given (set_stream); the launches, the capture and the completion event of
+// the batch follow it.
+struct StreamSel {
+    Engine::Impl* I;
+    hipStream_t main, cur;
+    StreamSel(Engine::Impl* impl, const Engine::Batch* b) : I(impl), main(impl->L.s_main), cur(impl->L.s) {
+        if (b->sign || b->sel == 0) return;
+        I->L.s_main = I->L.s = b->sel == 1 ? I->L.s_aux : I->s_copy;
+    }
+    ~StreamSel() {
+        I->L.s_main = main;
+        I->L.s = cur;
+    }
+};
+
 void Engine::set_one_stream(Batch* b, bool on) {
     if (b->one_stream != on) {
         b->drop_graph();  // the captured launches follow the stream layout
@@ -1540,6 +1563,7 @@ void Engine::wait() { impl_->sync(); }
 
 void Engine::launch(Batch* b) {
     wait_batch(b);
+    StreamSel sel(impl_, b);
     run(b, false);
     if (!b->done) HIPCK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
     // both paths end on s_main with the aux stream joined into it
@@ -1728,6 +1752,7 @@ static void sign_launches(Engine::Impl* I, Engine::Batch* b);
 void Engine::run(Batch* b, bool wait_done) {
     Impl* I = impl_;
     if (!b->sign) {
+        StreamSel sel(I, b);
         run_verify(I, b);
         if (wait_done) I->sync();
         return;

@@ -13,8 +13,9 @@
 using namespace xm;
 
 // One parameter set of a job: its batch on one engine.  A single-set job has
-// one; a mixed job one per (XMSS / XMSS^MT, OID), each on its own engine so
-// that their launches run side by side on the GPU.
+// one; a mixed job one per (XMSS / XMSS^MT, OID), so that their launches run
+// side by side on the GPU: a signing job's sets each on an engine of its own,
+// a verification job's on the three streams of the library's engine.
 struct JobSet {
     int mt = 0;
     uint32_t oid = 0;
@@ -151,15 +152,16 @@ int set_of(xmss_mi355x_job* j, int mt, uint32_t oid, bool mixed) {
 }
 
 // Engines for the sets that reach the device: the library's for the first,
-// private ones for the others.  Called with the engine lock held.
-bool bind_engines(xmss_mi355x_job* j) {
+// private ones for the others (one_engine: the library's for all).  Called
+// with the engine lock held.
+bool bind_engines(xmss_mi355x_job* j, bool one_engine = false) {
     Engine* shared = Engine::get();
     if (!shared) return false;
     bool first = true;
     for (JobSet& s : j->sets) {
         if (s.live.empty()) continue;
-        s.e = first ? shared : Engine::make_private();
-        s.shared = first;
+        s.e = first || one_engine ? shared : Engine::make_private();
+        s.shared = first || one_engine;
         if (!s.e) {
             s.e = shared;
             s.shared = true;
@@ -271,22 +273,29 @@ xmss_mi355x_job* verify_prepare(const int* mts, int mt_all, const unsigned char*
     }
     std::lock_guard<std::mutex> lk(engine_mutex());
     try {
-        if (!bind_engines(j)) {
+        // verification: every set on the library's engine, each on its own
+        // stream of it (below)
+        if (!bind_engines(j, true)) {
             fail_all(j, XMSS_ENODEV);
             return j;
         }
-        size_t live_sets = 0;
+        size_t live_sets = 0, k = 0;
         for (JobSet& js : j->sets) live_sets += !js.live.empty();
         for (size_t s = 0; s < j->sets.size(); ++s) {
             JobSet& js = j->sets[s];
             if (js.live.empty()) continue;
             js.b = js.e->verify_prepare(js.fam, js.p.tree_height, js.p.d, (int)items[s].size(), items[s].data());
-            // several sets: each keeps to its engine's main stream, so that the
-            // sets take fewer of the process's hardware queues (four by
-            // default) and run side by side (measured, 3 x 1024 requests:
-            // 2.20 ms against 2.6 ms with two streams per set; 2.65 ms for the
-            // sets one after another)
-            if (live_sets > 1) js.e->set_one_stream(js.b, true);
+            // several sets: each keeps to one stream of the engine, the sets
+            // on its three streams in turn, which HIP puts on three of the
+            // process's hardware queues (four by default), so that they run
+            // side by side.  (Sets on engines of their own had their main
+            // streams on queues that depended on how many streams the process
+            // had created before -- HIP's graph launches create some -- and two
+            // sets sharing a queue ran one after another.)
+            if (live_sets > 1) {
+                js.e->set_one_stream(js.b, true);
+                js.e->set_stream(js.b, (int)(k++ % 3));
+            }
         }
     } catch (const DeviceError&) {
         fail_job(j);
