@@ -128,7 +128,7 @@ def batch_inputs(count, rng):
     return seeds, idxs, msgs
 
 
-def bench_batch(api, count):
+def bench_batch(api, count, stream_batches=16):
     """Batch keygen, then one plain-core signature per key, then verification
     of all of them.  Returns timings, kernel stats and the verify results."""
     L = api.lib
@@ -202,9 +202,11 @@ def bench_batch(api, count):
     assert [vr2[j] for j in range(count)] == [vr[i] for i in perm]
     # the same stream through two jobs: each batch is staged while the other
     # job's launched work runs, and collected after the next one is launched
-    stream_t = bench_verify_stream(L, job, count, (ptrs(smb), smls, ptrs(pks), ptrs(outs)),
-                                   (smp2, smls2, pkp2, outp2), [vr[i] for i in range(count)],
-                                   [vr[i] for i in perm])
+    stream_t = None
+    if stream_batches:
+        stream_t = bench_verify_stream(L, job, count, (ptrs(smb), smls, ptrs(pks), ptrs(outs)),
+                                       (smp2, smls2, pkp2, outp2), [vr[i] for i in range(count)],
+                                       [vr[i] for i in perm], nbatch=stream_batches)
     L.xmss_mi355x_job_free(job)
     # leaf 1023 is the last index: the reference signs it with the wiped key,
     # so exactly those signatures must be rejected (xmss_core.c:222-231)
@@ -779,6 +781,8 @@ def main():
                     help="consecutive signatures of one XMSS-SHA2_10_256 key through sign_many (0: skip)")
     ap.add_argument("--single", type=int, default=256,
                     help="keys signed one xmss_sign call each, beside the batch (0: skip)")
+    ap.add_argument("--stream-batches", type=int, default=16,
+                    help="batches of the PCIe-inclusive verification stream (0: skip)")
     ap.add_argument("--mixed", type=int, default=1024, help="keys per parameter set of the mixed batch (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4000.0, help="offered requests/s of the serving run (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
@@ -847,7 +851,7 @@ def main():
     if args.batch:
         log("batch keygen/sign/verify x%d ..." % args.batch)
         (bkeygen_t, sign_run, sign_total, (ver_t, ver_refill_t, ver_stream_t), bst, n_last,
-         bnom) = bench_batch(api, args.batch)
+         bnom) = bench_batch(api, args.batch, args.stream_batches)
     rate = lambda t: None if not t else round(args.batch / t, 2)
     single = None
     if args.single:
