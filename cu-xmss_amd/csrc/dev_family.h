@@ -32,6 +32,7 @@ template <int N>
 struct Sha256Fam {
     static constexpr int kN = N;
     static constexpr bool kChainPrf = (N == 32);  // chain_prf_init / chain_prf_step available
+    static constexpr bool kPairHmsg = true;       // H_msg by two waves (k_hmsg_pair)
     static constexpr int kChainWaves = 1;         // no occupancy request for k_chains
     static constexpr int NW = N / 4;         // node words (big-endian values)
     static constexpr int LEN = 8 * N / 4 + 3;  // wots_len for w = 16
@@ -485,6 +486,7 @@ struct Sha256Fam {
 struct Sha512Fam {
     static constexpr int kN = 64;
     static constexpr bool kChainPrf = false;
+    static constexpr bool kPairHmsg = false;
     static constexpr int kChainWaves = 4;  // k_chains within 128 VGPRs
     static constexpr int NW = 8;
     static constexpr int LEN = 8 * 64 / 4 + 3;  // 131
@@ -756,6 +758,7 @@ template <int RATE, int N>
 struct ShakeFam {
     static constexpr int kN = N;
     static constexpr bool kChainPrf = false;
+    static constexpr bool kPairHmsg = false;
     static constexpr int kChainWaves = 1;
     static constexpr int NW = N / 4;
     static constexpr int LEN = 8 * N / 4 + 3;

@@ -87,6 +87,72 @@ __device__ __forceinline__ void sha256_rounds(uint32_t v[8], uint32_t w[16]) {
     v[0] = a; v[1] = b; v[2] = c; v[3] = d; v[4] = e; v[5] = f; v[6] = g; v[7] = h;
 }
 
+// Block b of a len-byte string at p (p 16-byte aligned, readable kMsgSlack
+// bytes past the end): sha256_block_load issues the four 16-byte loads,
+// sha256_block_words turns them into the 16 message words (big-endian
+// values) with the FIPS 180-4 padding of the last one or two blocks.
+__device__ __forceinline__ void sha256_block_load(uint4 (&raw)[4], const uint8_t* p, uint32_t b) {
+    const uint4* qb = reinterpret_cast<const uint4*>(p + 64 * (size_t)b);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) raw[j] = qb[j];
+}
+
+__device__ __forceinline__ void sha256_block_words(uint32_t w[16], const uint4 (&raw)[4], uint32_t len, uint32_t b) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        w[4 * j] = bswap32(raw[j].x);
+        w[4 * j + 1] = bswap32(raw[j].y);
+        w[4 * j + 2] = bswap32(raw[j].z);
+        w[4 * j + 3] = bswap32(raw[j].w);
+    }
+    if (64 * (b + 1) <= len) return;  // message bytes only
+    const int nb = (int)((len + 9 + 63) / 64);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int rem = (int)len - (int)(64 * b + 4 * j);  // message bytes from this word on
+        if (rem >= 4) continue;
+        if (rem > 0) w[j] = (w[j] & (0xFFFFFFFFu << (32 - 8 * rem))) | (0x80u << (24 - 8 * rem));
+        else if (rem == 0) w[j] = 0x80000000u;
+        else w[j] = 0;
+    }
+    if ((int)b == nb - 1) {
+        w[14] = 0;
+        w[15] = len * 8u;
+    }
+}
+
+// The 64 round inputs W[t] + K[t] of one block from its 16 message words:
+// out(t, value) receives them in order (the message schedule and the
+// constant, everything of a compression that does not depend on the state).
+template <class Out>
+__device__ __forceinline__ void sha256_schedule_k(uint32_t w[16], Out&& out) {
+#pragma unroll
+    for (int t = 0; t < 64; ++t) {
+        if (t >= 16) w[t & 15] = sha256_sched(w, t);
+        out(t, w[t & 15] + Sha256Consts::K[t]);
+    }
+}
+
+// The rounds of one compression from precomputed W[t] + K[t] (wk(t)):
+// st <- st + F(st, w), with no schedule work on this thread.
+template <class In>
+__device__ __forceinline__ void sha256_compress_wk(uint32_t st[8], In&& wk) {
+    uint32_t a = st[0], b = st[1], c = st[2], d = st[3];
+    uint32_t e = st[4], f = st[5], g = st[6], h = st[7];
+#pragma unroll
+    for (int t = 0; t < 64; ++t) {
+        const uint32_t S1 = xor3(rotr32(e, 6), rotr32(e, 11), rotr32(e, 25));
+        const uint32_t ch = (e & f) ^ (~e & g);
+        const uint32_t t1 = h + S1 + ch + wk(t);
+        const uint32_t S0 = xor3(rotr32(a, 2), rotr32(a, 13), rotr32(a, 22));
+        const uint32_t t2 = S0 + maj3(a, b, c);
+        h = g; g = f; f = e; e = d + t1;
+        d = c; c = b; b = a; a = t1 + t2;
+    }
+    st[0] += a; st[1] += b; st[2] += c; st[3] += d;
+    st[4] += e; st[5] += f; st[6] += g; st[7] += h;
+}
+
 // One compression: st <- st + F(st, w).  w is clobbered (rolling schedule).
 __device__ __forceinline__ void sha256_compress(uint32_t st[8], uint32_t w[16]) {
     uint32_t v[8];

@@ -1515,8 +1515,8 @@ static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* 
     for (int g = 0; g < G; ++g) {
         if (hi[g] == lo[g]) continue;
         I->L.s = st[g];
-        o->hmsg(I->L, dk, b->md + lo[g], hi[g] - lo[g], fam_pad(b->fam), b->blob, b->R, dig + (size_t)lo[g] * n,
-                prio_hmsg | prio[g]);
+        o->hmsg_verify(I->L, dk, b->md + lo[g], hi[g] - lo[g], fam_pad(b->fam), b->blob, dig + (size_t)lo[g] * n,
+                       prio_hmsg | prio[g]);
     }
     for (unsigned L = 0; L < d; ++L) {
         for (int stage = 0; stage < 3; ++stage) {

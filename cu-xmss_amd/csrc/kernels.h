@@ -48,6 +48,12 @@ __device__ __forceinline__ void wave_priority(uint32_t prio) {
     if (prio) __builtin_amdgcn_s_setprio(2);
 }
 
+// A compiler barrier for memory operations: loads issued before it stay
+// before it, so a prefetch cannot be sunk next to its first use (the
+// hardware waits for a load only where its registers are read).  Loads
+// through __restrict__ pointers count as invariant and may still move.
+__device__ __forceinline__ void keep_loads_ahead() { asm volatile("" ::: "memory"); }
+
 template <class F>
 __global__ __launch_bounds__(WG) void k_prepare(const uint8_t* __restrict__ seeds, int nkeys,
                                                  typename F::Key* __restrict__ keys) {
@@ -325,6 +331,93 @@ __global__ __launch_bounds__(WG) void k_hmsg(const typename F::Key* __restrict__
     if (r >= n) return;
     wave_priority(prio);
     hmsg_one<F>(keys, reqs, r, pad, blob, R_out, dig_out);
+}
+
+// H_msg of up to 64 requests [first, min(first + 64, n)) by the two waves of
+// a 128-thread workgroup, for the SHA-256 families.  H_msg is a chain of
+// compressions, one per 64 bytes, but the message schedule of every block
+// depends only on the message: wave 0 computes block j+1's W[t] + K[t] into
+// LDS (its message words loaded one block ahead) while wave 1 runs the 64
+// rounds of block j from LDS.  The serial path per block drops from a whole
+// compression (about 1400 VALU instructions) to its rounds (about 900).
+// Sign requests get R = PRF(SK_PRF, idx) first (wave 0, written into the
+// prefix as hmsg_one does).  Every thread of the workgroup must call it.
+struct HmsgPairLds {
+    uint32_t wk[2][64][64];  // [buffer][round][message]: one word per lane, conflict-free
+    uint32_t nbmax;
+};
+
+template <class F>
+__device__ __forceinline__ void hmsg_pair_block(const typename F::Key* __restrict__ keys,
+                                                const DMsg* __restrict__ reqs, uint32_t first, uint32_t n,
+                                                uint32_t pad, uint8_t* blob, uint8_t* __restrict__ R_out,
+                                                uint8_t* __restrict__ dig_out, HmsgPairLds& sh) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t r = first + lane;
+    uint32_t len = 0, nb = 0;
+    uint8_t* p = nullptr;
+    if (r < n) {
+        const DMsg q = reqs[r];
+        p = blob + q.off;
+        len = q.len;
+        nb = (len + 9 + 63) / 64;
+        if (q.sign && wave == 0) {  // R, then the prefix holds it (xmss_core.c:242-248)
+            typename F::Node R;
+            F::prf_idx(R, keys[q.key], q.idx);
+            F::store(R, p + pad);
+            F::store(R, R_out + (size_t)r * F::kN);
+        }
+    }
+    if (threadIdx.x == 0) sh.nbmax = 0;
+    __syncthreads();
+    if (wave == 0) atomicMax(&sh.nbmax, nb);
+    __syncthreads();
+    const uint32_t NB = sh.nbmax;
+    uint4 raw[4];
+    auto schedule = [&](uint32_t blk) {
+        uint32_t w[16];
+        sha256_block_words(w, raw, len, blk);
+        if (blk + 1 < nb) sha256_block_load(raw, p, blk + 1);
+        keep_loads_ahead();
+        sha256_schedule_k(w, [&](int t, uint32_t v) { sh.wk[blk & 1][t][lane] = v; });
+    };
+    uint32_t st[8];
+    sha256_iv(st);
+    if (wave == 0 && nb) {
+        sha256_block_load(raw, p, 0);
+        schedule(0);
+    }
+    __syncthreads();
+    for (uint32_t blk = 0; blk < NB; ++blk) {
+        if (wave == 0) {
+            if (blk + 1 < nb) schedule(blk + 1);
+        } else if (blk < nb) {
+            // all 64 round inputs into registers first: the LDS reads are in
+            // flight together instead of one per round
+            uint32_t x[64];
+#pragma unroll
+            for (int t = 0; t < 64; ++t) x[t] = sh.wk[blk & 1][t][lane];
+            sha256_compress_wk(st, [&](int t) { return x[t]; });
+        }
+        __syncthreads();
+    }
+    if (wave == 1 && r < n) {
+        typename F::Node d;
+#pragma unroll
+        for (int i = 0; i < F::NW; ++i) d.w[i] = st[i];
+        F::store(d, dig_out + (size_t)r * F::kN);
+    }
+}
+
+// H_msg of n requests, 64 per 128-thread workgroup (SHA-256 families).
+template <class F>
+__global__ __launch_bounds__(128) void k_hmsg_pair(const typename F::Key* __restrict__ keys,
+                                                   const DMsg* __restrict__ reqs, uint32_t n, uint32_t pad,
+                                                   uint8_t* blob, uint8_t* __restrict__ R_out,
+                                                   uint8_t* __restrict__ dig_out, uint32_t prio) {
+    __shared__ HmsgPairLds sh;
+    wave_priority(prio);
+    hmsg_pair_block<F>(keys, reqs, blockIdx.x * 64, n, pad, blob, R_out, dig_out, sh);
 }
 
 __global__ __launch_bounds__(WG) void k_gather(const uint8_t* __restrict__ src,

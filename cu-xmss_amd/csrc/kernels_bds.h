@@ -399,8 +399,27 @@ __global__ __launch_bounds__(WG) void k_bds_pre(const typename F::Key* __restric
     }
 }
 
+// Whether k_bds_fin's H_msg blocks run two waves per 64 messages
+// (hmsg_pair_block): SHA-256 families whose leaf workgroups have two waves.
+template <class F>
+struct BdsFinPair {
+    static constexpr bool value = F::kPairHmsg && ((F::LEN + 63) / 64) * 64 == 128;
+};
+template <class F, bool PAIR = BdsFinPair<F>::value>
+struct BdsFinLds {
+    union {
+        typename F::Node pk[F::LEN];
+        HmsgPairLds pair;
+    };
+};
+template <class F>
+struct BdsFinLds<F, false> {
+    typename F::Node pk[F::LEN];
+};
+
 // Blocks: nleaf leaves, then nsig WOTS signatures (short messages), then
-// ceil(nmsg / blockDim) blocks of H_msg lanes (long messages).
+// the H_msg blocks (long messages): 64 messages each by two waves
+// (BdsFinPair), else one message per lane.
 template <class F>
 __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restrict__ keys,
                                                   const DMsg* __restrict__ msgs, uint32_t nmsg, uint32_t pad,
@@ -409,12 +428,14 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
                                                   uint32_t nleaf, const DLeaf* __restrict__ sigs, uint32_t nsig,
                                                   const uint8_t* __restrict__ scratch, size_t stride,
                                                   uint8_t* __restrict__ leaf_out, uint8_t* __restrict__ sig_out) {
-    __shared__ typename F::Node pk[F::LEN];
+    __shared__ BdsFinLds<F> sh;
     if (blockIdx.x < nleaf) {
-        lat_fin_one<F, LAT_LEAF>(blockIdx.x, pk, leaves, nullptr, nullptr, nullptr, scratch, stride, leaf_out);
+        lat_fin_one<F, LAT_LEAF>(blockIdx.x, sh.pk, leaves, nullptr, nullptr, nullptr, scratch, stride, leaf_out);
     } else if (blockIdx.x < nleaf + nsig) {
-        lat_fin_one<F, LAT_SIGN>(blockIdx.x - nleaf, pk, sigs, digests, nullptr, nullptr,
+        lat_fin_one<F, LAT_SIGN>(blockIdx.x - nleaf, sh.pk, sigs, digests, nullptr, nullptr,
                                  scratch + (size_t)nleaf * stride, stride, sig_out);
+    } else if constexpr (BdsFinPair<F>::value) {
+        hmsg_pair_block<F>(keys, msgs, (blockIdx.x - nleaf - nsig) * 64, nmsg, pad, blob, R_out, digests, sh.pair);
     } else {
         const uint32_t i = (blockIdx.x - nleaf - nsig) * blockDim.x + threadIdx.x;
         if (i < nmsg) hmsg_one<F>(keys, msgs, i, pad, blob, R_out, digests);

@@ -44,10 +44,6 @@ struct LatLayout {
     static size_t bytes(uint32_t th) { return align(hpre + sizeof(HPre) * (F::LEN - 1 + th)); }
 };
 
-// A compiler barrier for memory operations: loads issued before it stay
-// before it, so a prefetch cannot be sunk next to its first use (the
-// hardware waits for a load only where its registers are read).
-__device__ __forceinline__ void keep_loads_ahead() { asm volatile("" ::: "memory"); }
 
 // L-tree merge t (0 .. LEN-2, height-major) -> (height, index).
 template <class F>

@@ -174,6 +174,10 @@ struct FamOps {
                          uint8_t* out) = 0;
     virtual void hmsg(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad,
                       uint8_t* blob, uint8_t* R, uint8_t* dig, uint32_t prio = 0) = 0;
+    // H_msg of verification requests only (no R to compute): for the SHA-256
+    // families two waves per 64 messages (k_hmsg_pair), else k_hmsg.
+    virtual void hmsg_verify(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad,
+                             uint8_t* blob, uint8_t* dig, uint32_t prio = 0) = 0;
     virtual size_t hpre_bytes() const = 0;
     // BDS signatures: R and H_msg of the nsign messages, the planned leaves
     // (LAT_LEAF requests) and the layer-0 WOTS signatures (LAT_SIGN requests
@@ -368,6 +372,14 @@ struct FamOpsT final : FamOps {
               uint8_t* R, uint8_t* dig, uint32_t prio = 0) override {
         L.run("hmsg", k_hmsg<F>, dim3(cdiv(n, WG)), dim3(WG), (const K*)keys, reqs, n, pad, blob, R, dig, prio);
     }
+    void hmsg_verify(Launcher& L, const void* keys, const DMsg* reqs, uint32_t n, uint32_t pad, uint8_t* blob,
+                     uint8_t* dig, uint32_t prio = 0) override {
+        if constexpr (F::kPairHmsg)
+            L.run("hmsg_pair", k_hmsg_pair<F>, dim3(cdiv(n, 64)), dim3(128), (const K*)keys, reqs, n, pad, blob,
+                  (uint8_t*)nullptr, dig, prio);
+        else
+            hmsg(L, keys, reqs, n, pad, blob, nullptr, dig, prio);
+    }
     size_t hpre_bytes() const override { return sizeof(typename F::HPre); }
     void bds_lat(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad, uint8_t* blob,
                  uint8_t* R, uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs, uint32_t nsign,
@@ -382,7 +394,9 @@ struct FamOpsT final : FamOps {
               blob, R, digests, hrows, nmsg, leaves, nleaf, sigs, nsign, scratch, stride, merges, nmerge,
               (typename F::HPre*)merge_out);
         const uint32_t nsig = long_msgs ? 0 : nmsg, nlate = long_msgs ? nmsg : 0;
-        L.run("bds_fin", k_bds_fin<F>, dim3(nleaf + nsig + cdiv(nlate, threads)), dim3(threads), (const K*)keys,
+        // late H_msg: 64 messages per workgroup by its two waves (k_bds_fin)
+        const uint32_t per = BdsFinPair<F>::value ? 64 : threads;
+        L.run("bds_fin", k_bds_fin<F>, dim3(nleaf + nsig + cdiv(nlate, per)), dim3(threads), (const K*)keys,
               msgs, nlate, pad, blob, R, digests, leaves, nleaf, sigs, nsig, (const uint8_t*)scratch, stride,
               leaf_out, sig_out);
     }
