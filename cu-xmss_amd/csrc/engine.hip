@@ -201,10 +201,13 @@ constexpr int kLatDense = 256;
 //                       1: the L-tree through LDS in one launch, then the auth
 //                       walks; 2: L-tree and auth walk in one launch
 // div (bits 4-7)        the long group is the longest 1/div of the batch (0: 4)
-constexpr uint32_t VPLAN_PRIO = 1, VPLAN_PRE_H_FIRST = 2, VPLAN_PRIO_HMSG = 256;
+// VPLAN_SHORT_FIRST     the short messages' group launched first: the stream
+//                       that starts first starts tens of us earlier, and the
+//                       short group holds three quarters of the work
+constexpr uint32_t VPLAN_PRIO = 1, VPLAN_PRE_H_FIRST = 2, VPLAN_PRIO_HMSG = 256, VPLAN_SHORT_FIRST = 512;
 static inline uint32_t vplan_tail(uint32_t p) { return (p >> 2) & 3; }
 static inline uint32_t vplan_div(uint32_t p) { return ((p >> 4) & 15) ? ((p >> 4) & 15) : 4; }
-constexpr uint32_t kVplanDefault = 1u << 2;  // tail 1, two groups, no priorities
+constexpr uint32_t kVplanDefault = (1u << 2) | 512u;  // tail 1, two groups, the short group first
 uint32_t verify_plan_at_prepare();  // the plan a new verification batch starts with
 // Up to this many requests per layer, a signing batch's WOTS signatures come
 // from the latency kernels (address-only work of the needed steps wide, then
@@ -1380,11 +1383,11 @@ void Engine::set_one_stream(Batch* b, bool on) {
 // off (the default plan).
 namespace {
 const uint32_t kVerifyCands[] = {
-    kVplanDefault,                      // L-tree through LDS, then the auth walks
-    0,                                  // one launch per L-tree height
-    2u << 2,                            // L-tree and auth walk in one launch
-    kVplanDefault | (8u << 4),          // the long group is the longest 1/8
+    kVplanDefault,                      // L-tree through LDS, then the auth walks; short group first
     kVplanDefault | VPLAN_PRIO_HMSG,    // H_msg at wave priority 2
+    1u << 2,                            // the long group first
+    VPLAN_SHORT_FIRST,                  // one launch per L-tree height
+    kVplanDefault | (8u << 4),          // the long group is the longest 1/8
     kVplanDefault | VPLAN_PRE_H_FIRST,  // H pre-images beside H_msg
 };
 constexpr size_t kNumVerifyCands = sizeof(kVerifyCands) / sizeof(kVerifyCands[0]);
@@ -1512,7 +1515,8 @@ static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* 
     }
     // Launches are issued stage by stage across the groups, so that no stream
     // waits for the host to issue the others' launches.
-    for (int g = 0; g < G; ++g) {
+    for (int k = 0; k < G; ++k) {
+        const int g = (plan & VPLAN_SHORT_FIRST) ? G - 1 - k : k;
         if (hi[g] == lo[g]) continue;
         I->L.s = st[g];
         o->hmsg_verify(I->L, dk, b->md + lo[g], hi[g] - lo[g], fam_pad(b->fam), b->blob, dig + (size_t)lo[g] * n,
