@@ -160,3 +160,64 @@ def test_tuned_plan_gives_the_same_verdicts(signed):
     lines = [x for x in text.value.decode().splitlines() if "th=10 d=2 log2count=8 one_stream=0" in x]
     assert len(lines) == 1, text.value
     assert int(lines[0].split("plan=")[1]) in list(plans)
+
+
+_CHILD = r'''
+import ctypes as C, os, random, sys
+sys.path.insert(0, sys.argv[1])
+from common import product
+from ffi import buf
+L = product(0).lib
+L.xmss_mi355x_tuned_plans.argtypes = [C.c_char_p, C.c_size_t]
+L.xmss_mi355x_tuned_plans.restype = C.c_size_t
+def plans():
+    n = L.xmss_mi355x_tuned_plans(None, 0)
+    b = C.create_string_buffer(n)
+    L.xmss_mi355x_tuned_plans(b, n)
+    return b.value.decode()
+if sys.argv[2] == "run":
+    # 300 well-formed but forged XMSS-SHA2_10_256 signed messages: every one
+    # is rejected, and the batch still takes the lane-dense path and its tuning
+    L.xmss_mi355x_verify_prepare.restype = C.c_void_p
+    L.xmss_mi355x_verify_prepare.argtypes = [C.c_int] + [C.c_void_p] * 3 + [C.c_size_t]
+    L.xmss_mi355x_job_run.argtypes = [C.c_void_p]
+    L.xmss_mi355x_verify_collect.argtypes = [C.c_void_p] * 4
+    L.xmss_mi355x_job_free.argtypes = [C.c_void_p]
+    rng = random.Random(9)
+    k = 300
+    sms = [bytes([0, 0, 1, rng.randrange(256)]) + bytes(rng.getrandbits(8) for _ in range(2496 + 40)) for _ in range(k)]
+    pks = [b"\x00\x00\x00\x01" + bytes(rng.getrandbits(8) for _ in range(64)) for _ in range(k)]
+    arr = lambda xs: (C.c_void_p * k)(*[C.cast(x, C.c_void_p) for x in xs])
+    smb, pkb = [buf(s) for s in sms], [buf(p) for p in pks]
+    job = L.xmss_mi355x_verify_prepare(0, arr(smb), (C.c_ulonglong * k)(*map(len, sms)), arr(pkb), k)
+    for _ in range(3):
+        assert L.xmss_mi355x_job_run(job) == 0
+    outs = [buf(len(s)) for s in sms]
+    mls = (C.c_ulonglong * k)()
+    vr = (C.c_int * k)()
+    assert L.xmss_mi355x_verify_collect(job, arr(outs), mls, vr) == 0
+    L.xmss_mi355x_job_free(job)
+    assert all(v == -1 for v in vr)
+print(plans(), end="")
+'''
+
+
+def test_tuned_plans_persist_in_the_tune_file(tmp_path):
+    """A process that tunes a new shape appends its decision to
+    XMSS_MI355X_TUNE_FILE; the next process reads it at start (before any job)."""
+    import os
+    import subprocess
+    import sys
+    tests = os.path.dirname(os.path.abspath(__file__))
+    env = dict(os.environ, XMSS_MI355X_TUNE_FILE=str(tmp_path / "tune.txt"))
+    env.pop("XMSS_MI355X_TUNE", None)
+    first = subprocess.run([sys.executable, "-c", _CHILD, tests, "run"], env=env, capture_output=True, text=True,
+                           timeout=240)
+    assert first.returncode == 0, first.stderr[-2000:]
+    line = [x for x in first.stdout.splitlines() if "fam=1 th=10 d=1 log2count=8 one_stream=0" in x]
+    assert len(line) == 1, first.stdout
+    assert (tmp_path / "tune.txt").read_text().startswith("verify ")
+    second = subprocess.run([sys.executable, "-c", _CHILD, tests, "read"], env=env, capture_output=True, text=True,
+                            timeout=240)
+    assert second.returncode == 0, second.stderr[-2000:]
+    assert line[0] in second.stdout.splitlines(), second.stdout
