@@ -391,46 +391,60 @@ __global__ __launch_bounds__(64) void k_top_fin(uint32_t th, uint32_t level0, ui
 // F pre-images of the chain steps a verification needs (steps len_c .. 14 of
 // chain c, len_c the message's base-w digit), once the message is known: one
 // workgroup per request walks the needed (chain, step) items densely (about
-// half of the 15 x LEN), so no lane idles on a step the chain skips.
+// half of the 15 x LEN), so no lane idles on a step the chain skips.  Item
+// offsets come from a prefix sum across the lanes (one thread per chain) and
+// an item -> chain map in LDS.
 template <class F>
 __global__ __launch_bounds__(256) void k_pre_need(const typename F::Key* __restrict__ keys,
                                                    const DLeaf* __restrict__ reqs,
                                                    const uint8_t* __restrict__ msgs,
                                                    uint8_t* __restrict__ scratch, size_t stride, uint32_t prio) {
     constexpr uint32_t LEN = F::LEN;
+    static_assert(LEN <= 256, "one thread per chain");
     __shared__ uint32_t first[LEN + 1];  // first needed item of each chain; first[LEN] = all
     __shared__ uint32_t lens[LEN];
+    __shared__ uint32_t wsum[4];        // needed steps per wave of chains
+    __shared__ uint8_t owner[15 * LEN];  // item -> its chain
     wave_priority(prio);
     const uint32_t r = blockIdx.x;
     const DLeaf q = reqs[r];
     const uint8_t* msg = msgs + (size_t)r * F::kN;
-    for (uint32_t c = threadIdx.x; c < LEN; c += blockDim.x) lens[c] = chain_len<F>(msg, c);
+    // the items' offsets: an exclusive prefix sum of the needed steps over the
+    // chains, scanned across the lanes of each wave (shuffles), then the
+    // earlier waves' totals added
+    const uint32_t c = threadIdx.x, lane = c & 63;
+    uint32_t need = 0;
+    if (c < LEN) {
+        lens[c] = chain_len<F>(msg, c);
+        need = 15 - lens[c];
+    }
+    uint32_t incl = need;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    if (lane == 63) wsum[c >> 6] = incl;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (uint32_t c = 0; c < LEN; ++c) {
-            first[c] = acc;
-            acc += 15 - lens[c];
-        }
-        first[LEN] = acc;
+    uint32_t base = 0;
+    for (uint32_t w = 0; w < (c >> 6); ++w) base += wsum[w];
+    if (c < LEN) {
+        const uint32_t f = base + incl - need;
+        first[c] = f;
+        for (uint32_t i = 0; i < need; ++i) owner[f + i] = (uint8_t)c;
+        if (c == LEN - 1) first[LEN] = base + incl;
     }
     __syncthreads();
     const typename F::Key& k = keys[q.key];
     auto* fp = reinterpret_cast<typename F::FPre*>(scratch + (size_t)r * stride + LatLayout<F>::fpre);
     const uint32_t total = first[LEN];
     for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
-        // the chain holding item t: first[lo] <= t < first[lo + 1]
-        uint32_t lo = 0, hi = LEN;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (first[mid] <= t) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t s = lens[lo] + (t - first[lo]);
-        uint32_t a[8] = {q.layer, q.tree_hi, q.tree_lo, ADDR_OTS, q.leaf, lo, s, 0};
+        const uint32_t ch = owner[t];
+        const uint32_t s = lens[ch] + (t - first[ch]);
+        uint32_t a[8] = {q.layer, q.tree_hi, q.tree_lo, ADDR_OTS, q.leaf, ch, s, 0};
         typename F::FPre p;
         F::pre_f(p, k, a);
-        fp[lo * 15 + s] = p;
+        fp[ch * 15 + s] = p;
     }
 }
 
