@@ -204,7 +204,9 @@ constexpr int kLatDense = 256;
 // VPLAN_SHORT_FIRST     the short messages' group launched first: the stream
 //                       that starts first starts tens of us earlier, and the
 //                       short group holds three quarters of the work
-constexpr uint32_t VPLAN_PRIO = 1, VPLAN_PRE_H_FIRST = 2, VPLAN_PRIO_HMSG = 256, VPLAN_SHORT_FIRST = 512;
+// VPLAN_PAIRED          two chains of complementary length per lane (k_latb_chains2)
+constexpr uint32_t VPLAN_PRIO = 1, VPLAN_PRE_H_FIRST = 2, VPLAN_PRIO_HMSG = 256, VPLAN_SHORT_FIRST = 512,
+                   VPLAN_PAIRED = 1024;
 static inline uint32_t vplan_tail(uint32_t p) { return (p >> 2) & 3; }
 static inline uint32_t vplan_div(uint32_t p) { return ((p >> 4) & 15) ? ((p >> 4) & 15) : 4; }
 constexpr uint32_t kVplanDefault = (1u << 2) | 512u;  // tail 1, two groups, the short group first
@@ -1387,7 +1389,7 @@ const uint32_t kVerifyCands[] = {
     kVplanDefault | VPLAN_PRIO_HMSG,    // H_msg at wave priority 2
     1u << 2,                            // the long group first
     VPLAN_SHORT_FIRST,                  // one launch per L-tree height
-    kVplanDefault | (8u << 4),          // the long group is the longest 1/8
+    kVplanDefault | VPLAN_PAIRED,       // two chains of complementary length per lane
     kVplanDefault | VPLAN_PRE_H_FIRST,  // H pre-images beside H_msg
 };
 constexpr size_t kNumVerifyCands = sizeof(kVerifyCands) / sizeof(kVerifyCands[0]);
@@ -1544,7 +1546,8 @@ static void verify_dense(Engine::Impl* I, FamOps* o, Engine::Batch* b, uint8_t* 
                 if (stage == 0)
                     o->pre_need(I->L, dk, vd, m, msg, th, scr, prio[g]);
                 else if (stage == 1)
-                    o->latb_chains(I->L, m, msg, b->sigs + t * len * n, scr, lb, e, prio[g]);
+                    o->latb_chains(I->L, m, msg, b->sigs + t * len * n, scr, lb, e, prio[g],
+                                   (plan & VPLAN_PAIRED) != 0);
                 else
                     o->latb_tail(I->L, vd, m, b->auth_in + (size_t)L * count * th * n, scr, lb, e,
                                  tmp + (size_t)a * ((len + 1) / 2) * n, roots + t * n, tail, prio[g]);

@@ -558,6 +558,73 @@ __global__ __launch_bounds__(LATB_CHAINS_WG) void k_latb_chains(uint32_t n, cons
     F::store(x, ends + (size_t)g * F::kN);
 }
 
+// The same chains, two per lane: each workgroup takes 2 x LATB_CHAINS_WG
+// consecutive chains, counting-sorts them by step count and gives lane i the
+// i-th longest and the i-th shortest, walked one after the other: every lane
+// walks about 15 steps, and half as many waves as one chain per lane carry
+// the same steps.
+template <class F>
+__device__ __forceinline__ void latb_walk(typename F::Node& x, const typename F::FPre* fp, uint32_t s0) {
+    using FPre = typename F::FPre;
+    FPre a = fp[s0 < 15 ? s0 : 14], b;
+    for (uint32_t s = s0; s < 15; s += 2) {
+        b = fp[s + 1 < 15 ? s + 1 : 14];
+        keep_loads_ahead();
+        F::fin_f(x, a, x);
+        if (s + 1 >= 15) break;
+        a = fp[s + 2 < 15 ? s + 2 : 14];
+        keep_loads_ahead();
+        F::fin_f(x, b, x);
+    }
+}
+
+template <class F>
+__global__ __launch_bounds__(LATB_CHAINS_WG) void k_latb_chains2(uint32_t n, const uint8_t* __restrict__ msgs,
+                                                                  const uint8_t* __restrict__ sigs,
+                                                                  const uint8_t* scratch, size_t stride,
+                                                                  uint8_t* __restrict__ ends, uint32_t prio) {
+    using FPre = typename F::FPre;
+    constexpr uint32_t W = LATB_CHAINS_WG, C2 = 2 * LATB_CHAINS_WG;
+    __shared__ uint32_t hist[16];
+    __shared__ uint16_t order[C2];
+    __shared__ uint8_t first[C2];
+    wave_priority(prio);
+    const uint32_t total = n * (uint32_t)F::LEN;
+    const uint32_t base = blockIdx.x * C2;
+    const uint32_t tid = threadIdx.x;
+    if (tid < 16) hist[tid] = 0;
+    __syncthreads();
+    uint32_t d[2], rank[2];
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t k = tid + h * W;
+        d[h] = 15;  // past the end: no steps
+        if (base + k < total) {
+            const uint32_t r = (base + k) / F::LEN, c = (base + k) % F::LEN;
+            d[h] = chain_len<F>(msgs + (size_t)r * F::kN, c);
+        }
+        first[k] = (uint8_t)d[h];
+        rank[h] = atomicAdd(&hist[d[h]], 1u);
+    }
+    __syncthreads();
+    for (uint32_t h = 0; h < 2; ++h) {
+        uint32_t pos = rank[h];  // longest first
+        for (uint32_t b = 0; b < d[h]; ++b) pos += hist[b];
+        order[pos] = (uint16_t)(tid + h * W);
+    }
+    __syncthreads();
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t k = order[h ? C2 - 1 - tid : tid];
+        const uint32_t g = base + k;
+        if (g >= total) continue;
+        const uint32_t r = g / F::LEN, c = g % F::LEN;
+        typename F::Node x;
+        F::load(x, sigs + (size_t)g * F::kN);
+        latb_walk<F>(x, reinterpret_cast<const FPre*>(scratch + (size_t)r * stride + LatLayout<F>::fpre) + c * 15,
+                     first[k]);
+        F::store(x, ends + (size_t)g * F::kN);
+    }
+}
+
 // One L-tree height for every request: l nodes per request in src (src_stride
 // nodes apart) -> ceil(l/2) in dst; merge `off + i` of the request's H
 // pre-images hashes pair i; an odd last node is carried up.

@@ -161,7 +161,8 @@ struct FamOps {
     // one launch, then the auth walks; 2 = L-tree and auth walk in one
     // launch.  prio: the waves' issue priority (wave_priority).
     virtual void latb_chains(Launcher& L, uint32_t n, const uint8_t* msgs, const uint8_t* sigs,
-                             const uint8_t* scratch, size_t stride, uint8_t* ends, uint32_t prio) = 0;
+                             const uint8_t* scratch, size_t stride, uint8_t* ends, uint32_t prio,
+                             bool paired = false) = 0;
     virtual void latb_tail(Launcher& L, const DLeaf* reqs, uint32_t n, const uint8_t* auth, const uint8_t* scratch,
                            size_t stride, uint8_t* ends, uint8_t* tmp, uint8_t* out, uint32_t tail,
                            uint32_t prio) = 0;
@@ -331,9 +332,13 @@ struct FamOpsT final : FamOps {
         L.run("top_fin", k_top_fin<F>, dim3(nruns), dim3(64), th, top, nodes, hp);
     }
     void latb_chains(Launcher& L, uint32_t n, const uint8_t* msgs, const uint8_t* sigs, const uint8_t* scratch,
-                     size_t stride, uint8_t* ends, uint32_t prio) override {
-        L.run("latb_chains", k_latb_chains<F>, dim3(cdiv((uint64_t)n * F::LEN, LATB_CHAINS_WG)),
-              dim3(LATB_CHAINS_WG), n, msgs, sigs, scratch, stride, ends, prio);
+                     size_t stride, uint8_t* ends, uint32_t prio, bool paired = false) override {
+        if (paired)
+            L.run("latb_chains2", k_latb_chains2<F>, dim3(cdiv((uint64_t)n * F::LEN, 2 * LATB_CHAINS_WG)),
+                  dim3(LATB_CHAINS_WG), n, msgs, sigs, scratch, stride, ends, prio);
+        else
+            L.run("latb_chains", k_latb_chains<F>, dim3(cdiv((uint64_t)n * F::LEN, LATB_CHAINS_WG)),
+                  dim3(LATB_CHAINS_WG), n, msgs, sigs, scratch, stride, ends, prio);
     }
     void latb_tail(Launcher& L, const DLeaf* reqs, uint32_t n, const uint8_t* auth, const uint8_t* scratch,
                    size_t stride, uint8_t* ends, uint8_t* tmp, uint8_t* out, uint32_t tail, uint32_t prio) override {
