@@ -353,7 +353,13 @@ __device__ __forceinline__ void hmsg_pair_block(const typename F::Key* __restric
                                                 uint32_t pad, uint8_t* blob, uint8_t* __restrict__ R_out,
                                                 uint8_t* __restrict__ dig_out, HmsgPairLds& sh) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t r = first + lane;
+    // Lanes past the last request hash a copy of request lane % count: with
+    // few messages (a single BDS signature's) the waves run at the speed they
+    // reach with every lane active, measured 1.5x the speed with one active
+    // lane; only the lanes of the requests themselves write.
+    const uint32_t count = first < n ? (n - first < 64 ? n - first : 64) : 0;
+    const bool own = lane < count;
+    const uint32_t r = count ? first + lane % count : n;
     uint32_t len = 0, nb = 0;
     uint8_t* p = nullptr;
     if (r < n) {
@@ -361,7 +367,7 @@ __device__ __forceinline__ void hmsg_pair_block(const typename F::Key* __restric
         p = blob + q.off;
         len = q.len;
         nb = (len + 9 + 63) / 64;
-        if (q.sign && wave == 0) {  // R, then the prefix holds it (xmss_core.c:242-248)
+        if (q.sign && wave == 0 && own) {  // R, then the prefix holds it (xmss_core.c:242-248)
             typename F::Node R;
             F::prf_idx(R, keys[q.key], q.idx);
             F::store(R, p + pad);
@@ -369,7 +375,7 @@ __device__ __forceinline__ void hmsg_pair_block(const typename F::Key* __restric
         }
     }
     if (threadIdx.x == 0) sh.nbmax = 0;
-    __syncthreads();
+    __syncthreads();  // (also orders the R stores before any lane's loads of the prefix)
     if (wave == 0) atomicMax(&sh.nbmax, nb);
     __syncthreads();
     const uint32_t NB = sh.nbmax;
@@ -401,7 +407,7 @@ __device__ __forceinline__ void hmsg_pair_block(const typename F::Key* __restric
         }
         __syncthreads();
     }
-    if (wave == 1 && r < n) {
+    if (wave == 1 && own) {
         typename F::Node d;
 #pragma unroll
         for (int i = 0; i < F::NW; ++i) d.w[i] = st[i];
