@@ -172,7 +172,12 @@ __device__ __forceinline__ void ltree_lds(typename F::Node* pk, const typename F
     while (l > 1) {
         const uint32_t parents = l >> 1;
         typename F::Node o;
-        if (c < parents) F::fin_h(o, hp[off + c], pk[2 * c], pk[2 * c + 1]);
+        // the idle lanes of a wave that holds merges hash a copy of one of
+        // them (mirror_lanes); only the merge's own lane writes below
+        if ((c & ~63u) < parents) {
+            const uint32_t m = c < parents ? c : c % parents;
+            F::fin_h(o, hp[off + m], pk[2 * m], pk[2 * m + 1]);
+        }
         __syncthreads();
         if (c < parents) pk[c] = o;
         if ((l & 1) && c == 0) pk[l >> 1] = pk[l - 1];
@@ -194,10 +199,13 @@ __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, co
     using Lay = LatLayout<F>;
     using FPre = typename F::FPre;
     using HPre = typename F::HPre;
-    const uint32_t c = threadIdx.x;
+    const uint32_t own = threadIdx.x;
     const uint8_t* S = scratch + (size_t)r * stride;
     const HPre* hp = reinterpret_cast<const HPre*>(S + Lay::hpre);
-    if (c < (uint32_t)F::LEN) {
+    // lanes past the last chain in the last chain's wave walk a copy of a
+    // chain (mirror_lanes); only the chain's own lane stores it
+    if ((own & ~63u) < (uint32_t)F::LEN) {
+        const uint32_t c = own < (uint32_t)F::LEN ? own : own % F::LEN;
         typename F::Node x;
         uint32_t s0 = 0, s1 = 15;
         if (MODE == LAT_VERIFY) {
@@ -221,15 +229,16 @@ __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, co
             cur = nxt;
         }
         if (MODE == LAT_SIGN) {
-            F::store(x, out + ((size_t)r * F::LEN + c) * F::kN);
+            if (own == c) F::store(x, out + ((size_t)r * F::LEN + c) * F::kN);
             return;
         }
-        pk[c] = x;
+        if (own == c) pk[c] = x;
     }
     if (MODE == LAT_SIGN) return;
     __syncthreads();
-    ltree_lds<F>(pk, hp, c);
-    if (c != 0) return;
+    ltree_lds<F>(pk, hp, own);
+    // the auth walk: lane 0, with the rest of its wave walking copies
+    if (own >= 64) return;
     typename F::Node node = pk[0];
     if (MODE == LAT_VERIFY) {
         const DLeaf q = reqs[r];
@@ -243,7 +252,7 @@ __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, co
             idx >>= 1;
         }
     }
-    F::store(node, out + (size_t)r * F::kN);
+    if (own == 0) F::store(node, out + (size_t)r * F::kN);
 }
 
 template <class F, uint32_t MODE>
