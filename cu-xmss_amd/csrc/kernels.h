@@ -48,6 +48,19 @@ __device__ __forceinline__ void wave_priority(uint32_t prio) {
     if (prio) __builtin_amdgcn_s_setprio(2);
 }
 
+// Lane mirroring.  A wave whose first lanes hold the last of n serial work
+// items (a chain walk, an H_msg, a tree merge) and whose other lanes have
+// none runs measurably slower and less evenly than a full wave running the
+// same instructions, so the idle lanes redo an item of their wave, and only
+// the item's own lane (i < n) writes the result.  i is the lane's item; the
+// items of a wave are 64 consecutive indices starting at a multiple of 64.
+// Returns the item to compute, or n when the whole wave is past the end.
+__device__ __forceinline__ uint32_t mirror_lanes(uint32_t i, uint32_t n) {
+    if (i < n) return i;
+    const uint32_t base = i & ~63u;
+    return base < n ? base + (i - base) % (n - base) : n;
+}
+
 // A compiler barrier for memory operations: loads issued before it stay
 // before it, so a prefetch cannot be sunk next to its first use (the
 // hardware waits for a load only where its registers are read).  Loads
@@ -308,18 +321,33 @@ __global__ __launch_bounds__(WG) void k_thash_h(const typename F::Key* __restric
 template <class F>
 __device__ __forceinline__ void hmsg_one(const typename F::Key* __restrict__ keys, const DMsg* __restrict__ reqs,
                                          uint32_t r, uint32_t pad, uint8_t* __restrict__ blob,
-                                         uint8_t* __restrict__ R_out, uint8_t* __restrict__ dig_out) {
+                                         uint8_t* __restrict__ R_out, uint8_t* __restrict__ dig_out,
+                                         bool own = true) {
     const DMsg q = reqs[r];
     uint8_t* p = blob + q.off;
     if (q.sign) {
         typename F::Node R;
         F::prf_idx(R, keys[q.key], q.idx);
-        F::store(R, p + pad);
-        F::store(R, R_out + (size_t)r * F::kN);
+        if (own) {
+            F::store(R, p + pad);
+            F::store(R, R_out + (size_t)r * F::kN);
+        }
     }
     typename F::Node d;
     F::hash_bytes(d, p, q.len);
-    F::store(d, dig_out + (size_t)r * F::kN);
+    if (own) F::store(d, dig_out + (size_t)r * F::kN);
+}
+
+// hmsg_one for item i of n, with the idle lanes of the last items' wave
+// mirroring them (mirror_lanes).  A mirroring lane may read the message
+// before its owner has written R into it: its digest is discarded.
+template <class F>
+__device__ __forceinline__ void hmsg_mirrored(const typename F::Key* __restrict__ keys,
+                                              const DMsg* __restrict__ reqs, uint32_t i, uint32_t n,
+                                              uint32_t pad, uint8_t* __restrict__ blob,
+                                              uint8_t* __restrict__ R_out, uint8_t* __restrict__ dig_out) {
+    const uint32_t r = mirror_lanes(i, n);
+    if (r < n) hmsg_one<F>(keys, reqs, r, pad, blob, R_out, dig_out, r == i);
 }
 
 template <class F>
@@ -327,10 +355,8 @@ __global__ __launch_bounds__(WG) void k_hmsg(const typename F::Key* __restrict__
                                               const DMsg* __restrict__ reqs, uint32_t n, uint32_t pad,
                                               uint8_t* __restrict__ blob, uint8_t* __restrict__ R_out,
                                               uint8_t* __restrict__ dig_out, uint32_t prio) {
-    const uint32_t r = blockIdx.x * WG + threadIdx.x;
-    if (r >= n) return;
     wave_priority(prio);
-    hmsg_one<F>(keys, reqs, r, pad, blob, R_out, dig_out);
+    hmsg_mirrored<F>(keys, reqs, blockIdx.x * WG + threadIdx.x, n, pad, blob, R_out, dig_out);
 }
 
 // H_msg of up to 64 requests [first, min(first + 64, n)) by the two waves of

@@ -138,7 +138,7 @@ struct BdsDev {
         __shared__ alignas(16) uint8_t hb[3 * F::kN];
         if (tid < 2 * n) hb[tid] = in[tid];
         __syncthreads();
-        if (tid == 0) {
+        if (tid < 64) {  // thread 0's wave, every lane on the same hash (mirror_lanes)
             typename F::Node L, R, o;
             F::load(L, hb);
             F::load(R, hb + n);
@@ -150,7 +150,7 @@ struct BdsDev {
                 for (int i = 0; i < 8; ++i) ad[i] = a[i];
                 F::thash_h(o, key, ad, L, R);
             }
-            F::store(o, hb + 2 * n);
+            if (tid == 0) F::store(o, hb + 2 * n);
         }
         ++mcur;
         __syncthreads();
@@ -327,22 +327,26 @@ struct BdsDev {
             // is one compression
             const uint8_t* S = wpre + (size_t)wcur * wstride;
             ++wcur;
-            if (tid < (uint32_t)F::LEN) {
-                typename F::Node x = reinterpret_cast<const typename F::Node*>(S + LatLayout<F>::starts)[tid];
-                const typename F::FPre* fp = reinterpret_cast<const typename F::FPre*>(S + LatLayout<F>::fpre) + tid * 15;
-                const uint32_t steps = chain_len<F>(scratch, tid);
+            // chain c on lane c; the lanes past the last chain in its wave
+            // walk a copy (mirror_lanes)
+            const uint32_t c = mirror_lanes(tid, F::LEN);
+            if (c < (uint32_t)F::LEN) {
+                typename F::Node x = reinterpret_cast<const typename F::Node*>(S + LatLayout<F>::starts)[c];
+                const typename F::FPre* fp = reinterpret_cast<const typename F::FPre*>(S + LatLayout<F>::fpre) + c * 15;
+                const uint32_t steps = chain_len<F>(scratch, c);
                 for (uint32_t s = 0; s < steps; ++s) F::fin_f(x, fp[s], x);
-                store_bytes(x, sig + (size_t)tid * n);
+                if (c == tid) store_bytes(x, sig + (size_t)c * n);
             }
             __syncthreads();
             return;
         }
-        if (tid < (uint32_t)F::LEN) {
-            uint32_t a[8] = {layer, (uint32_t)(tree >> 32), (uint32_t)tree, 0, ots, tid, 0, 0};
+        const uint32_t c = mirror_lanes(tid, F::LEN);
+        if (c < (uint32_t)F::LEN) {
+            uint32_t a[8] = {layer, (uint32_t)(tree >> 32), (uint32_t)tree, 0, ots, c, 0, 0};
             typename F::Node x;
             F::prf_keygen(x, key, a);
-            run_chain<F>(x, key, a, 0, chain_len<F>(scratch, tid));
-            store_bytes(x, sig + (size_t)tid * n);
+            run_chain<F>(x, key, a, 0, chain_len<F>(scratch, c));
+            if (c == tid) store_bytes(x, sig + (size_t)c * n);
         }
         __syncthreads();
     }
@@ -377,8 +381,7 @@ __global__ __launch_bounds__(WG) void k_bds_pre(const typename F::Key* __restric
                                                  typename F::HPre* __restrict__ merge_out) {
     const uint32_t t = blockIdx.x * WG + threadIdx.x;
     if (blockIdx.y < hrows) {
-        const uint32_t i = blockIdx.y * gridDim.x * WG + t;
-        if (i < nmsg) hmsg_one<F>(keys, msgs, i, pad, blob, R_out, digests);
+        hmsg_mirrored<F>(keys, msgs, blockIdx.y * gridDim.x * WG + t, nmsg, pad, blob, R_out, digests);
         return;
     }
     const uint32_t y = blockIdx.y - hrows;
@@ -437,8 +440,8 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
     } else if constexpr (BdsFinPair<F>::value) {
         hmsg_pair_block<F>(keys, msgs, (blockIdx.x - nleaf - nsig) * 64, nmsg, pad, blob, R_out, digests, sh.pair);
     } else {
-        const uint32_t i = (blockIdx.x - nleaf - nsig) * blockDim.x + threadIdx.x;
-        if (i < nmsg) hmsg_one<F>(keys, msgs, i, pad, blob, R_out, digests);
+        hmsg_mirrored<F>(keys, msgs, (blockIdx.x - nleaf - nsig) * blockDim.x + threadIdx.x, nmsg, pad, blob,
+                         R_out, digests);
     }
 }
 
@@ -486,14 +489,22 @@ __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint
         const uint8_t* lits = lit_arena + job.lit_off * n;
         const typename F::HPre* pre = merge_arena + job.merge_off;
         for (uint32_t lev = 1; lev <= job.maxlev; ++lev) {
-            for (uint32_t m = threadIdx.x; m < job.nmerge; m += blockDim.x) {
-                const uint32_t* q = q0 + 3 * m;
-                if (q[2] != lev) continue;
+            for (uint32_t m0 = 0; m0 < job.nmerge; m0 += blockDim.x) {
+                const uint32_t m = m0 + threadIdx.x;
+                const bool mine = m < job.nmerge && q0[3 * m + 2] == lev;
+                const uint64_t act = __ballot(mine);
+                if (act == 0) continue;
+                // a lane with no merge of this level hashes a copy of the
+                // next one in its wave, cyclically (mirror_lanes)
+                const uint32_t lane = threadIdx.x & 63;
+                const uint64_t rot = lane ? (act >> lane) | (act << (64 - lane)) : act;
+                const uint32_t src = m - lane + ((lane + (uint32_t)__builtin_ctzll(rot)) & 63);
+                const uint32_t* q = q0 + 3 * src;
                 typename F::Node L, R, o;
                 dag_load<F>(L, q[0], lits, B.leaves, mv);
                 dag_load<F>(R, q[1], lits, B.leaves, mv);
-                F::fin_h(o, pre[m], L, R);
-                F::store(o, mv + (size_t)m * n);
+                F::fin_h(o, pre[src], L, R);
+                if (mine) F::store(o, mv + (size_t)m * n);
             }
             __syncthreads();
         }

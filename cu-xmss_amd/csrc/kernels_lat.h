@@ -136,8 +136,7 @@ __global__ __launch_bounds__(WG) void k_ver_pre(const typename F::Key* __restric
                                                  uint8_t* __restrict__ scratch, size_t stride) {
     const uint32_t t = blockIdx.x * WG + threadIdx.x;
     if (blockIdx.y < hrows) {
-        const uint32_t i = blockIdx.y * gridDim.x * WG + t;
-        if (i < nmsg) hmsg_one<F>(keys, msgs, i, pad, blob, nullptr, digests);
+        hmsg_mirrored<F>(keys, msgs, blockIdx.y * gridDim.x * WG + t, nmsg, pad, blob, nullptr, digests);
         return;
     }
     lat_pre_one<F, LAT_VERIFY>(keys, reqs, blockIdx.y - hrows, t, nullptr, scratch, stride);
@@ -174,10 +173,8 @@ __device__ __forceinline__ void ltree_lds(typename F::Node* pk, const typename F
         typename F::Node o;
         // the idle lanes of a wave that holds merges hash a copy of one of
         // them (mirror_lanes); only the merge's own lane writes below
-        if ((c & ~63u) < parents) {
-            const uint32_t m = c < parents ? c : c % parents;
-            F::fin_h(o, hp[off + m], pk[2 * m], pk[2 * m + 1]);
-        }
+        const uint32_t m = mirror_lanes(c, parents);
+        if (m < parents) F::fin_h(o, hp[off + m], pk[2 * m], pk[2 * m + 1]);
         __syncthreads();
         if (c < parents) pk[c] = o;
         if ((l & 1) && c == 0) pk[l >> 1] = pk[l - 1];
@@ -204,8 +201,8 @@ __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, co
     const HPre* hp = reinterpret_cast<const HPre*>(S + Lay::hpre);
     // lanes past the last chain in the last chain's wave walk a copy of a
     // chain (mirror_lanes); only the chain's own lane stores it
-    if ((own & ~63u) < (uint32_t)F::LEN) {
-        const uint32_t c = own < (uint32_t)F::LEN ? own : own % F::LEN;
+    const uint32_t c = mirror_lanes(own, F::LEN);
+    if (c < (uint32_t)F::LEN) {
         typename F::Node x;
         uint32_t s0 = 0, s1 = 15;
         if (MODE == LAT_VERIFY) {
@@ -387,7 +384,8 @@ __global__ __launch_bounds__(64) void k_top_fin(uint32_t th, uint32_t level0, ui
     for (uint32_t level = level0; level < th; ++level) {
         const uint32_t parents = 1u << (th - level - 1);
         typename F::Node o;
-        if (t < parents) F::fin_h(o, hp[tree_pre_slot(th, level, t)], buf[2 * t], buf[2 * t + 1]);
+        const uint32_t m = mirror_lanes(t, parents);  // lanes past the parents hash a copy
+        F::fin_h(o, hp[tree_pre_slot(th, level, m)], buf[2 * m], buf[2 * m + 1]);
         __syncthreads();
         if (t < parents) {
             buf[t] = o;
