@@ -79,9 +79,20 @@ std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* s
 // A source is kind | index (Engine::BdsJob::plan_dag); the level of a merge is
 // 1 + the deepest level among the merges it depends on (leaves and literals
 // are level 0), so that merges of one level can be hashed side by side.
+// post, if given, receives the dry run's sk after the signature: its
+// traversal structure (indices, stack heights, flags) is the real one, its
+// node values are tags.
 void bds_plan(const xmss_params* p, const unsigned char* sk, unsigned long long mlen, std::vector<LeafId>& leaves,
               std::vector<uint32_t>& merges, std::vector<uint32_t>* dag = nullptr,
-              std::vector<unsigned char>* lits = nullptr, std::vector<uint32_t>* wots_up = nullptr);
+              std::vector<unsigned char>* lits = nullptr, std::vector<uint32_t>* wots_up = nullptr,
+              std::vector<unsigned char>* post = nullptr);
+
+// The distinct leaves the signature made with `sk` will consume, from a dry
+// run on a copy.  Which leaves a signature asks for depends on the
+// traversal's structure and the index only, never on node values, so sk
+// may be a dry run's `post` state.  Empty when that signature is the last
+// index's or beyond (those take the host path, bds_sign).
+std::vector<LeafId> bds_leaves_of(const xmss_params* p, const unsigned char* sk);
 
 // A signature whose traversal runs on the device (Engine::bds_sign_batch).
 struct BdsDeviceSign {
@@ -90,6 +101,7 @@ struct BdsDeviceSign {
     std::vector<uint64_t> tree;
     std::vector<unsigned char> seeds;  // SK_SEED, SK_PRF, PUB_SEED, root
     unsigned long long idx = 0;
+    std::vector<LeafId> next;  // the leaves of the key's next signature (bds_leaves_of)
     Engine::BdsJob job{};
 };
 // Host part before the launch: index, plan, index advance, message copy.

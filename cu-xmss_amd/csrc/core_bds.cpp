@@ -527,7 +527,7 @@ std::vector<LeafId> bds_plan_leaves(const xmss_params* p, const unsigned char* s
 
 void bds_plan(const xmss_params* p, const unsigned char* sk, unsigned long long mlen, std::vector<LeafId>& leaves,
               std::vector<uint32_t>& merges, std::vector<uint32_t>* dag, std::vector<unsigned char>* lits,
-              std::vector<uint32_t>* wots_up) {
+              std::vector<uint32_t>* wots_up, std::vector<unsigned char>* post) {
     std::vector<unsigned char> skc(sk, sk + p->sk_bytes);
     std::vector<unsigned char> smc(p->sig_bytes + mlen + 1), mc(mlen + 1, 0);
     unsigned long long l = 0;
@@ -538,6 +538,44 @@ void bds_plan(const xmss_params* p, const unsigned char* sk, unsigned long long 
     if (dag) dag->swap(rec.dag);
     if (lits) lits->swap(rec.lits);
     if (wots_up) wots_up->swap(rec.wots_up);
+    if (post) post->swap(skc);
+}
+
+namespace {
+// Backend that only lists the leaves asked for; every hash result is zeros.
+struct LeafListOps final : BdsOps {
+    unsigned n;
+    std::vector<LeafId> leaves;
+    explicit LeafListOps(unsigned n_) : n(n_) {}
+    void bind_key(const unsigned char*, const unsigned char*, const unsigned char*) override {}
+    void leaf(unsigned char* out, const LeafId& id) override {
+        memset(out, 0, n);
+        leaves.push_back(id);
+    }
+    void thash_h(unsigned char* out, const unsigned char*, const uint32_t*) override { memset(out, 0, n); }
+    void wots_sign(unsigned char* sig, const unsigned char*, uint32_t, uint64_t, uint32_t) override {
+        memset(sig, 0, (size_t)(2 * n + 3) * n);
+    }
+    void hmsg_sign(unsigned char* R, unsigned char* digest, uint64_t, const unsigned char*, const unsigned char*,
+                   uint64_t) override {
+        memset(R, 0, n);
+        memset(digest, 0, n);
+    }
+};
+}  // namespace
+
+std::vector<LeafId> bds_leaves_of(const xmss_params* p, const unsigned char* sk) {
+    unsigned long long idx = 0;
+    for (unsigned i = 0; i < p->index_bytes; ++i) idx |= ((unsigned long long)sk[i]) << 8 * (p->index_bytes - 1 - i);
+    if (idx >= (1ULL << p->full_height) - 1) return {};
+    std::vector<unsigned char> skc(sk, sk + p->sk_bytes), smc(p->sig_bytes + 1);
+    unsigned char m = 0;
+    unsigned long long l = 0;
+    LeafListOps ops(p->n);
+    bds_sign(p, skc.data(), smc.data(), &l, &m, 0, ops);
+    std::sort(ops.leaves.begin(), ops.leaves.end());
+    ops.leaves.erase(std::unique(ops.leaves.begin(), ops.leaves.end()), ops.leaves.end());
+    return ops.leaves;
 }
 
 bool bds_device_prepare(const xmss_params* p, unsigned char* sk, unsigned char* sm, const unsigned char* m,
@@ -547,7 +585,9 @@ bool bds_device_prepare(const xmss_params* p, unsigned char* sk, unsigned char* 
     for (unsigned i = 0; i < ib; ++i) idx |= ((unsigned long long)sk[i]) << 8 * (ib - 1 - i);
     if (idx >= (1ULL << p->full_height) - 1) return false;  // wipe / exhausted: host path
     std::vector<LeafId> plan;
-    bds_plan(p, sk, mlen, plan, ds.merges, &ds.dag, &ds.lits, &ds.wots_up);
+    std::vector<unsigned char> post;
+    bds_plan(p, sk, mlen, plan, ds.merges, &ds.dag, &ds.lits, &ds.wots_up, &post);
+    ds.next = bds_leaves_of(p, post.data());
     ds.layer.resize(plan.size());
     ds.tree.resize(plan.size());
     ds.leaf.resize(plan.size());
@@ -760,7 +800,14 @@ static int bds_sign_api(const xmss_params* p, unsigned char* sk, unsigned char* 
         if (bds_device_prepare(p, sk, sm, m, mlen, ds)) {
             // the key context stays in the device pool across signatures
             e->pool_begin();
-            ds.job.key = (uint32_t)e->pool_slot(fam, {ds.job.sk_seed, ds.job.sk_prf, ds.job.pub_seed});
+            const KeySeeds ks{ds.job.sk_seed, ds.job.sk_prf, ds.job.pub_seed};
+            ds.job.key = (uint32_t)e->pool_slot(fam, ks);
+            // the chains of the leaves this key's next signature will consume
+            // are hashed beside this signature's WOTS signature; this one
+            // finishes the leaves whose chains the previous call hashed
+            std::vector<LeafRun> next(ds.next.size());
+            for (size_t i = 0; i < next.size(); ++i) next[i] = {0, ds.next[i].layer, ds.next[i].tree, ds.next[i].leaf};
+            e->ends_ahead(fam, ks, next.data(), (int)next.size());
             e->bds_sign_batch(fam, 1, &ds.job);
             bds_device_finish(p, sm, smlen, mlen, ds);
             return 0;
@@ -768,6 +815,7 @@ static int bds_sign_api(const xmss_params* p, unsigned char* sk, unsigned char* 
         // last index (the key is wiped now) or exhausted key: the pool keeps no
         // context of the wiped seeds; no traversal work, host control
         const unsigned n = p->n, ib = p->index_bytes;
+        e->ends_ahead_drop();
         e->pool_forget(fam, {sk + ib, sk + ib + n, sk + ib + 3 * n});
         DirectOps ops(e, fam, n, bds_plan_leaves(p, sk, mlen, false));
         const int r = bds_sign(p, sk, sm, smlen, m, mlen, ops);

@@ -292,6 +292,16 @@ public:
     // pool_forget: drops the cached slot of these seeds (a wiped key).
     void pool_forget(int fam, const KeySeeds& seeds);
     size_t pool_cached(int fam) const;
+    // Chain ends hashed one signature ahead, for a key that signs one
+    // signature after another: the next bds_sign_batch of one job for the
+    // key of these seeds also hashes the chains of the leaves `ids`
+    // (LeafRun.key unused, one leaf each) -- those its next signature will
+    // consume -- into device slots, beside its own WOTS signature; a later
+    // signature of the key then hashes only the L-trees of those leaves.
+    // The slots follow one key (found by the keyed digest of its seeds).
+    void ends_ahead(int fam, const KeySeeds& seeds, const LeafRun* ids, int count);
+    // Forgets every chain end held (a wiped key).
+    void ends_ahead_drop();
     void pool_use(int fam);
     size_t pool_key_bytes(int fam) const;
     static size_t pool_key_bytes_of(int fam);

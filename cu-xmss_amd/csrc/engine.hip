@@ -10,6 +10,7 @@
 #include <random>
 #include <array>
 #include <string>
+#include <tuple>
 #include <unordered_map>
 
 #include "engine.h"
@@ -376,8 +377,31 @@ namespace noalloc {
 static __device__ __align__(256) uint8_t scratch[kStaticScratch];
 }
 
+// Chain ends hashed one signature ahead (Engine::ends_ahead): the chains of
+// the leaves a key's next signature will consume are hashed by this
+// signature's finishing launch, beside its WOTS signature, into slots of
+// `buf`, and the next signature hashes only their L-trees.  `have`: the
+// leaves of key d (family fam) whose chain ends a finished launch left in a
+// slot; `want`: the request for the coming launch, for key want_d.
+struct EndsAhead {
+    using Id = std::tuple<uint32_t, uint64_t, uint32_t>;  // layer, tree, leaf
+    DevBuf buf;
+    void* at = nullptr;  // buf.p when `have` was filled
+    int fam = -1, want_fam = -1;
+    Digest d, want_d;
+    std::map<Id, uint32_t> have;
+    std::vector<Id> want;
+    void clear() {
+        have.clear();
+        want.clear();
+        fam = want_fam = -1;
+    }
+};
+constexpr uint32_t kEndsSlots = 64;
+
 struct Engine::Impl {
     Launcher L;
+    EndsAhead ahead;
     void* static_scratch = nullptr;  // this device's noalloc::scratch
     KeyPool pool[F_COUNT];
     const void* kp_override = nullptr;  // pool in use, or null for `keys`
@@ -526,6 +550,8 @@ void Engine::recover() {
     (void)hipStreamSynchronize(I->L.s_aux);
     (void)hipStreamSynchronize(I->s_copy);
     (void)hipGetLastError();
+    // chain ends hashed ahead are not trusted after a failure
+    I->ahead.clear();
 }
 
 void Engine::scrub_keys() {
@@ -568,6 +594,16 @@ void Engine::set_keys(int fam, int nkeys, const KeySeeds* seeds) {
     volatile uint8_t* vh = h.data();
     for (size_t i = 0; i < h.size(); ++i) vh[i] = 0;
 }
+
+void Engine::ends_ahead(int fam, const KeySeeds& s, const LeafRun* ids, int count) {
+    EndsAhead& E = impl_->ahead;
+    E.want.clear();
+    E.want_fam = fam;
+    E.want_d = seed_digest(fam_n(fam), s);
+    for (int i = 0; i < count; ++i) E.want.emplace_back(ids[i].layer, ids[i].tree, ids[i].leaf0);
+}
+
+void Engine::ends_ahead_drop() { impl_->ahead.clear(); }
 
 // A free slot of the pool (grown when none is left), prepared for `s`.
 static int pool_take(Engine::Impl* I, int fam, const KeySeeds& s, bool pinned, const Digest& d) {
@@ -2055,13 +2091,37 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     std::vector<DBds> jd(count);
     std::vector<size_t> tail_bytes(count);
     size_t sk_total = 0, tail_total = 0, lds = 0;
+    // chain ends hashed ahead (ends_ahead): a planned leaf whose chain ends
+    // a slot holds only gets its L-tree here (LAT_ENDS_IN)
+    EndsAhead& E = I->ahead;
+    const size_t eb = (size_t)len * n;
+    uint8_t* ends = nullptr;
+    if (!E.want.empty() || !E.have.empty()) {
+        ends = E.buf.need(kEndsSlots * eb);
+        if (ends != E.at) E.have.clear();  // a new buffer holds no chain ends
+        E.at = ends;
+    }
+    std::vector<EndsAhead::Id> used;
+    std::vector<bool> busy(kEndsSlots, false);
     for (int i = 0; i < count; ++i) {
         const BdsShape& s = jobs[i].shape;
         jd[i].leaf_off = ld.size();
+        const bool ahead = E.fam == fam && !E.have.empty() &&
+                           seed_digest(n, {jobs[i].sk_seed, jobs[i].sk_prf, jobs[i].pub_seed}) == E.d;
         for (uint32_t k = 0; k < jobs[i].nplan; ++k) {
             const uint64_t t = jobs[i].plan_tree[k];
-            ld.push_back({jobs[i].key, jobs[i].plan_layer[k], (uint32_t)(t >> 32), (uint32_t)t, jobs[i].plan_leaf[k],
-                          0, 0, 0});
+            DLeaf q{jobs[i].key, jobs[i].plan_layer[k], (uint32_t)(t >> 32), (uint32_t)t, jobs[i].plan_leaf[k], 0, 0, 0};
+            if (ahead) {
+                const EndsAhead::Id id{q.layer, t, q.leaf};
+                auto it = E.have.find(id);
+                if (it != E.have.end()) {
+                    q.p1 = LAT_ENDS_IN;
+                    q.p2 = it->second;
+                    busy[it->second] = true;
+                    used.push_back(id);
+                }
+            }
+            ld.push_back(q);
         }
         jd[i].merge_off = mg.size();
         for (uint32_t k = 0; k < jobs[i].nmerge; ++k) {
@@ -2103,6 +2163,33 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
         for (uint32_t k = 0; k < jd[i].nwots; ++k) {
             const uint32_t* w = jobs[i].plan_wots + 4 * k;
             up.push_back({jobs[i].key, w[0], w[1], w[2], w[3], 0, LAT_ALL_STEPS, 0});
+        }
+    }
+    // the chains of the leaves the key's next signature will consume
+    // (ends_ahead, one-signature launches), as rows after the planned ones,
+    // into free slots (LAT_ENDS_OUT)
+    const bool same_key = E.fam == fam && E.want_d == E.d;
+    std::vector<std::pair<EndsAhead::Id, uint32_t>> fresh;
+    if (count == 1 && E.want_fam == fam && !E.want.empty() &&
+        seed_digest(n, {jobs[0].sk_seed, jobs[0].sk_prf, jobs[0].pub_seed}) == E.want_d) {
+        std::vector<bool> taken = busy;  // slots still read: this launch's L-trees, and what stays held
+        if (same_key)
+            for (const auto& kv : E.have) taken[kv.second] = true;
+        uint32_t next_slot = 0;
+        for (const EndsAhead::Id& id : E.want) {
+            if (same_key && E.have.count(id)) continue;  // held already
+            bool planned = false;                         // hashed whole here anyway
+            for (uint32_t k = 0; k < jobs[0].nplan && !planned; ++k)
+                planned = std::get<0>(id) == jobs[0].plan_layer[k] && std::get<1>(id) == jobs[0].plan_tree[k] &&
+                          std::get<2>(id) == jobs[0].plan_leaf[k];
+            if (planned) continue;
+            while (next_slot < kEndsSlots && taken[next_slot]) ++next_slot;
+            if (next_slot == kEndsSlots) break;
+            taken[next_slot] = true;
+            const uint64_t t = std::get<1>(id);
+            ld.push_back({jobs[0].key, std::get<0>(id), (uint32_t)(t >> 32), (uint32_t)t, std::get<2>(id), 0,
+                          LAT_ENDS_OUT, next_slot});
+            fresh.emplace_back(id, next_slot);
         }
     }
     // H_msg inputs
@@ -2186,7 +2273,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     for (int i = 0; i < count; ++i) mmax = std::max<uint64_t>(mmax, jobs[i].mlen);
     const bool long_msgs = mmax > kBdsShortMsg;
     o->bds_lat(I->L, dk, dmd, (uint32_t)count, pad, db, dR, dig, dld, (uint32_t)nl, dw, (uint32_t)nw, lat, dmg,
-               (uint32_t)mg.size(), merge_arena, leaf_arena, dW, long_msgs);
+               (uint32_t)mg.size(), merge_arena, leaf_arena, dW, long_msgs, ends);
     o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, ddag, dlit,
                   lat + (nl + count) * o->lat_bytes(0), dT, B.scratch.need((size_t)count * 4 * n),
                   long_msgs ? dw : nullptr, dig, lat + nl * o->lat_bytes(0), dW);
@@ -2197,6 +2284,17 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
         memcpy(jobs[i].R_out, h + o_R + (size_t)i * n, n);
         memcpy(jobs[i].wots0_out, h + o_W + (size_t)i * len * n, (size_t)len * n);
         memcpy(jobs[i].tail_out, h + o_T + jd[i].tail_off, tail_bytes[i]);
+    }
+    // the slots now: the chain ends this launch used are spent, the fresh
+    // ones are held for the requested key (its earlier ones too, if it is
+    // the key the slots held)
+    for (const EndsAhead::Id& id : used) E.have.erase(id);
+    if (!E.want.empty() && E.want_fam == fam) {
+        if (!same_key) E.have.clear();
+        E.fam = fam;
+        E.d = E.want_d;
+        for (const auto& f : fresh) E.have[f.first] = f.second;
+        E.want.clear();
     }
 }
 
@@ -2439,6 +2537,9 @@ void Engine::release_workspace() {
                        &I->bds->merge_arena};
         for (DevBuf* x : b) x->release();
     }
+    I->ahead.clear();
+    I->ahead.at = nullptr;
+    I->ahead.buf.release();
     if (I->ver) {
         DevBuf* v[] = {&I->ver->io, &I->ver->dig, &I->ver->roots, &I->ver->scratch, &I->ver->spec};
         for (DevBuf* x : v) x->release();

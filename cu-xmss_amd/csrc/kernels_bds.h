@@ -420,9 +420,10 @@ struct BdsFinLds<F, false> {
     typename F::Node pk[F::LEN];
 };
 
-// Blocks: nleaf leaves, then nsig WOTS signatures (short messages), then
-// the H_msg blocks (long messages): 64 messages each by two waves
-// (BdsFinPair), else one message per lane.
+// Blocks: nleaf leaves (or, for rows marked LAT_ENDS_OUT / LAT_ENDS_IN, the
+// chains into slot p2 of `ends` / the L-tree from it), then nsig WOTS
+// signatures (short messages), then the H_msg blocks (long messages): 64
+// messages each by two waves (BdsFinPair), else one message per lane.
 template <class F>
 __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restrict__ keys,
                                                   const DMsg* __restrict__ msgs, uint32_t nmsg, uint32_t pad,
@@ -430,10 +431,15 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
                                                   uint8_t* __restrict__ digests, const DLeaf* __restrict__ leaves,
                                                   uint32_t nleaf, const DLeaf* __restrict__ sigs, uint32_t nsig,
                                                   const uint8_t* __restrict__ scratch, size_t stride,
-                                                  uint8_t* __restrict__ leaf_out, uint8_t* __restrict__ sig_out) {
+                                                  uint8_t* __restrict__ leaf_out, uint8_t* __restrict__ sig_out,
+                                                  uint8_t* __restrict__ ends) {
     __shared__ BdsFinLds<F> sh;
     if (blockIdx.x < nleaf) {
-        lat_fin_one<F, LAT_LEAF>(blockIdx.x, sh.pk, leaves, nullptr, nullptr, nullptr, scratch, stride, leaf_out);
+        const DLeaf q = leaves[blockIdx.x];
+        uint8_t* e = ends + (size_t)q.p2 * F::LEN * F::kN;
+        if (q.p1 & LAT_ENDS_OUT) lat_leaf_chains<F>(blockIdx.x, scratch, stride, e);
+        else if (q.p1 & LAT_ENDS_IN) lat_leaf_ltree<F>(blockIdx.x, sh.pk, scratch, stride, e, leaf_out);
+        else lat_fin_one<F, LAT_LEAF>(blockIdx.x, sh.pk, leaves, nullptr, nullptr, nullptr, scratch, stride, leaf_out);
     } else if (blockIdx.x < nleaf + nsig) {
         lat_fin_one<F, LAT_SIGN>(blockIdx.x - nleaf, sh.pk, sigs, digests, nullptr, nullptr,
                                  scratch + (size_t)nleaf * stride, stride, sig_out);

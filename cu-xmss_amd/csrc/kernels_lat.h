@@ -28,8 +28,12 @@ constexpr uint32_t LAT_LEAF = 0, LAT_SIGN = 1, LAT_VERIFY = 2;
 // steps of every chain (the message is not known yet when k_lat_pre runs,
 // as for the upper XMSS^MT layers), p2 = the offset of its auth path, in
 // nodes, from the auth pointer of the k_lat_fin launch.  LAT_SIGN requests
-// use p1 bit 0 the same way (then k_lat_pre reads no message).
-constexpr uint32_t LAT_ALL_STEPS = 1;
+// use p1 bit 0 the same way (then k_lat_pre reads no message).  A BDS
+// signature's LAT_LEAF rows may be split over two signatures of a key:
+// LAT_ENDS_OUT rows hash only the chains and store the LEN chain ends in
+// slot p2 of the ends buffer; LAT_ENDS_IN rows (a later launch) take the
+// chain ends from slot p2 and hash only the L-tree.
+constexpr uint32_t LAT_ALL_STEPS = 1, LAT_ENDS_OUT = 2, LAT_ENDS_IN = 4;
 
 // Per-request scratch: chain starts [LEN], F pre-images [LEN][15], H
 // pre-images [LEN - 1 L-tree merges, in height order][th auth levels].
@@ -91,6 +95,7 @@ __device__ __forceinline__ void lat_pre_one(const typename F::Key* __restrict__ 
     const DLeaf q = reqs[r];
     const uint32_t nf = LEN * 15, nh = (MODE == LAT_SIGN) ? 0 : LEN - 1 + (MODE == LAT_VERIFY ? q.p0 : 0);
     if (t >= LEN + nf + nh) return;
+    if (MODE == LAT_LEAF && (q.p1 & (t < LEN + nf ? LAT_ENDS_IN : LAT_ENDS_OUT))) return;  // the other half's work
     const typename F::Key& k = keys[q.key];
     uint8_t* S = scratch + (size_t)r * stride;
     if (t < LEN) {  // chain start: PRF_keygen (wots.c:14-29)
@@ -250,6 +255,40 @@ __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, co
         }
     }
     if (own == 0) F::store(node, out + (size_t)r * F::kN);
+}
+
+// The two halves of a LAT_LEAF request split over two launches (the flags
+// LAT_ENDS_OUT / LAT_ENDS_IN): the LEN chains of request r to their ends,
+// stored at ends; and the L-tree of LEN chain ends read from ends, its root
+// stored at out[r].  Every thread of the workgroup calls them.
+template <class F>
+__device__ __forceinline__ void lat_leaf_chains(uint32_t r, const uint8_t* __restrict__ scratch, size_t stride,
+                                                uint8_t* __restrict__ ends) {
+    using Lay = LatLayout<F>;
+    using FPre = typename F::FPre;
+    const uint32_t own = threadIdx.x, c = mirror_lanes(own, F::LEN);
+    if (c >= (uint32_t)F::LEN) return;
+    const uint8_t* S = scratch + (size_t)r * stride;
+    typename F::Node x = reinterpret_cast<const typename F::Node*>(S + Lay::starts)[c];
+    const FPre* fp = reinterpret_cast<const FPre*>(S + Lay::fpre) + c * 15;
+    FPre cur = fp[0];
+    for (uint32_t s = 0; s < 15; ++s) {
+        const FPre nxt = fp[s + 1 < 15 ? s + 1 : 14];
+        F::fin_f(x, cur, x);
+        cur = nxt;
+    }
+    if (own == c) F::store(x, ends + (size_t)c * F::kN);
+}
+
+template <class F>
+__device__ __forceinline__ void lat_leaf_ltree(uint32_t r, typename F::Node* pk, const uint8_t* __restrict__ scratch,
+                                               size_t stride, const uint8_t* __restrict__ ends,
+                                               uint8_t* __restrict__ out) {
+    for (uint32_t c = threadIdx.x; c < (uint32_t)F::LEN; c += blockDim.x) F::load(pk[c], ends + (size_t)c * F::kN);
+    __syncthreads();
+    ltree_lds<F>(pk, reinterpret_cast<const typename F::HPre*>(scratch + (size_t)r * stride + LatLayout<F>::hpre),
+                 threadIdx.x);
+    if (threadIdx.x == 0) F::store(pk[0], out + (size_t)r * F::kN);
 }
 
 template <class F, uint32_t MODE>

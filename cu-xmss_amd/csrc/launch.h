@@ -191,11 +191,12 @@ struct FamOps {
     // x lat_bytes(0).
     // long: R / H_msg beside the leaves and the nmsg layer-0 signatures
     // left to bds_update (else H_msg first and the signatures beside the
-    // leaves, into sig_out).
+    // leaves, into sig_out).  ends: the chain-end slots of leaf rows marked
+    // LAT_ENDS_OUT / LAT_ENDS_IN (LEN x n bytes each).
     virtual void bds_lat(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad, uint8_t* blob,
                          uint8_t* R, uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs,
                          uint32_t nsign, uint8_t* scratch, const DThash* merges, uint32_t nmerge, void* merge_out,
-                         uint8_t* leaf_out, uint8_t* sig_out, bool long_msgs) = 0;
+                         uint8_t* leaf_out, uint8_t* sig_out, bool long_msgs, uint8_t* ends) = 0;
     // The state updates, one workgroup per job; dag / lits: the planned
     // merges' sources (3 words per merge, parallel to merge_arena) and the
     // literal node values (jobs with maxlev 0 hash their merges one by one).
@@ -389,7 +390,7 @@ struct FamOpsT final : FamOps {
     void bds_lat(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad, uint8_t* blob,
                  uint8_t* R, uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs, uint32_t nsign,
                  uint8_t* scratch, const DThash* merges, uint32_t nmerge, void* merge_out, uint8_t* leaf_out,
-                 uint8_t* sig_out, bool long_msgs) override {
+                 uint8_t* sig_out, bool long_msgs, uint8_t* ends) override {
         const size_t stride = LatLayout<F>::bytes(0);
         const uint32_t xb = cdiv(F::LEN * 17 - 1, WG);  // LAT_LEAF / LAT_SIGN items per request
         const uint32_t mrows = cdiv(nmerge, (uint64_t)xb * WG);
@@ -403,7 +404,7 @@ struct FamOpsT final : FamOps {
         const uint32_t per = BdsFinPair<F>::value ? 64 : threads;
         L.run("bds_fin", k_bds_fin<F>, dim3(nleaf + nsig + cdiv(nlate, per)), dim3(threads), (const K*)keys,
               msgs, nlate, pad, blob, R, digests, leaves, nleaf, sigs, nsig, (const uint8_t*)scratch, stride,
-              leaf_out, sig_out);
+              leaf_out, sig_out, ends);
     }
     void bds_update(Launcher& L, const void* keys, size_t lds, const DBds* jobs, uint32_t n,
                     uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena, const uint32_t* dag,

@@ -114,3 +114,40 @@ def test_bds_every_hash_family_against_oracle(name):
         assert sk == sk2, "sign %d sk" % i
     vr, mm = lib.sign_open(sm, pk, 1)
     assert vr == 0 and mm == m
+
+
+def test_bds_interleaved_keys_and_an_older_state():
+    """A signature finishes the leaves whose chains its key's previous call
+    hashed ahead; here two keys alternate (the chain-end slots follow the
+    key of each call), and one key then signs again from an older copy of
+    its sk (the chains hashed ahead were for another state): every
+    signature and sk still equals the reference's."""
+    lib = product(1)
+    runs = []
+    for name, mt in (("bds_xmss_sha2_10_256.json", 0), ("bds_xmssmt_sha2_20_4_256.json", 1)):
+        g = golden(name)
+        p = lib.params(g["oid"], mt)
+        if mt:
+            lib.lib.xmss_mi355x_rng_seed(g["rng_seed"])
+        rc, pk, sk = lib.keypair(g["oid"], mt, sk_init=bytes(4 + p.sk_bytes))
+        assert rc == 0 and sk.hex() == g["sk"]
+        runs.append([g, mt, sk, random.Random(g["msg_seed"]), []])
+
+    def step(run, i):
+        g, mt, sk, rng, saved = run
+        m = bytes(rng.getrandbits(8) for _ in range(32))
+        saved.append((sk, m))
+        rc, sm, sk = lib.sign(sk, m, mt)
+        row = g["signs"][i]
+        assert rc == row["rc"] and dg(sm) == row["sm"] and dg(sk) == row["sk"], "%s sign %d" % (g["oid"], i)
+        run[2] = sk
+
+    for i in range(40):
+        for run in runs:
+            step(run, i)
+    g, mt, _, _, saved = runs[0]
+    for i in (17, 18, 5, 6, 7):
+        sk, m = saved[i]
+        rc, sm, sk2 = lib.sign(sk, m, mt)
+        row = g["signs"][i]
+        assert rc == row["rc"] and dg(sm) == row["sm"] and dg(sk2) == row["sk"], "again from state %d" % i
