@@ -2097,7 +2097,11 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     const size_t eb = (size_t)len * n;
     uint8_t* ends = nullptr;
     if (!E.want.empty() || !E.have.empty()) {
-        ends = E.buf.need(kEndsSlots * eb);
+        try {  // the slots are an optimization: without memory for them, every leaf is hashed whole
+            ends = E.buf.need(kEndsSlots * eb);
+        } catch (const DeviceError&) {
+            E.clear();
+        }
         if (ends != E.at) E.have.clear();  // a new buffer holds no chain ends
         E.at = ends;
     }
@@ -2106,7 +2110,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     for (int i = 0; i < count; ++i) {
         const BdsShape& s = jobs[i].shape;
         jd[i].leaf_off = ld.size();
-        const bool ahead = E.fam == fam && !E.have.empty() &&
+        const bool ahead = ends && E.fam == fam && !E.have.empty() &&
                            seed_digest(n, {jobs[i].sk_seed, jobs[i].sk_prf, jobs[i].pub_seed}) == E.d;
         for (uint32_t k = 0; k < jobs[i].nplan; ++k) {
             const uint64_t t = jobs[i].plan_tree[k];
@@ -2170,8 +2174,9 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     // into free slots (LAT_ENDS_OUT)
     const bool same_key = E.fam == fam && E.want_d == E.d;
     std::vector<std::pair<EndsAhead::Id, uint32_t>> fresh;
-    if (count == 1 && E.want_fam == fam && !E.want.empty() &&
-        seed_digest(n, {jobs[0].sk_seed, jobs[0].sk_prf, jobs[0].pub_seed}) == E.want_d) {
+    const bool matched = ends && count == 1 && E.want_fam == fam && !E.want.empty() &&
+                         seed_digest(n, {jobs[0].sk_seed, jobs[0].sk_prf, jobs[0].pub_seed}) == E.want_d;
+    if (matched) {
         std::vector<bool> taken = busy;  // slots still read: this launch's L-trees, and what stays held
         if (same_key)
             for (const auto& kv : E.have) taken[kv.second] = true;
@@ -2289,13 +2294,13 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     // ones are held for the requested key (its earlier ones too, if it is
     // the key the slots held)
     for (const EndsAhead::Id& id : used) E.have.erase(id);
-    if (!E.want.empty() && E.want_fam == fam) {
+    if (matched) {
         if (!same_key) E.have.clear();
         E.fam = fam;
         E.d = E.want_d;
         for (const auto& f : fresh) E.have[f.first] = f.second;
-        E.want.clear();
     }
+    E.want.clear();  // a request is for the launch right after it
 }
 
 
