@@ -17,6 +17,7 @@ library's stream around each launch.
 """
 import argparse
 import ctypes as C
+import gc
 import json
 import math
 import os
@@ -560,14 +561,21 @@ def bench_latency(nsigns, fixed_len=None):
     rng = random.Random(SEED + 9)
     lat = []
     last = None
-    for _ in range(nsigns):
-        mlen = fixed_len if fixed_len else int(32 * 128 ** rng.random())
-        m = bytes(rng.getrandbits(8) for _ in range(mlen))
-        t0 = time.perf_counter()
-        rc, sm, sk = api.sign(sk, m, 0)
-        lat.append(time.perf_counter() - t0)
-        assert rc == 0
-        last = (sm, m)
+    # the interpreter's cyclic collector stays out of the timed calls (the
+    # heap of the earlier sections would otherwise add its pauses to the tail)
+    gc.collect()
+    gc.disable()
+    try:
+        for _ in range(nsigns):
+            mlen = fixed_len if fixed_len else int(32 * 128 ** rng.random())
+            m = bytes(rng.getrandbits(8) for _ in range(mlen))
+            t0 = time.perf_counter()
+            rc, sm, sk = api.sign(sk, m, 0)
+            lat.append(time.perf_counter() - t0)
+            assert rc == 0
+            last = (sm, m)
+    finally:
+        gc.enable()
     vlat = []
     for _ in range(16):
         t0 = time.perf_counter()
