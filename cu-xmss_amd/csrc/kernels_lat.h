@@ -730,10 +730,13 @@ __global__ __launch_bounds__(256) void k_latb_tree(const DLeaf* __restrict__ req
     uint32_t l = LEN, off = 0;
     while (l > 1) {
         const uint32_t parents = l >> 1;
-        const uint32_t j = t / parents, c = t % parents;  // request j of the group, pair c
-        const bool act = j < nr;
+        // lane t: request j of the group, pair c; the idle lanes of a wave
+        // that holds merges hash a copy of one of them (mirror_lanes)
+        const uint32_t m = mirror_lanes(t, nr * parents);
+        const uint32_t j = m / parents, c = m % parents;
+        const bool act = t < nr * parents;  // then m == t
         Node o;
-        if (act) {
+        if (m < nr * parents) {
             const HPre* hp = reinterpret_cast<const HPre*>(scratch + (size_t)(r0 + j) * stride + LatLayout<F>::hpre);
             F::fin_h(o, hp[off + c], pk[j * LEN + 2 * c], pk[j * LEN + 2 * c + 1]);
         }
@@ -744,9 +747,11 @@ __global__ __launch_bounds__(256) void k_latb_tree(const DLeaf* __restrict__ req
         off += parents;
         l = (l + 1) >> 1;
     }
-    if (t >= nr) return;
-    const uint32_t r = r0 + t;
-    Node node = pk[t * LEN];
+    // the auth walks: lane t < nr walks request t, the rest of its wave copies
+    const uint32_t w = mirror_lanes(t, nr);
+    if (w >= nr) return;
+    const uint32_t r = r0 + w;
+    Node node = pk[w * LEN];
     if (AUTH) {
         const DLeaf q = reqs[r];
         const HPre* hp = reinterpret_cast<const HPre*>(scratch + (size_t)r * stride + LatLayout<F>::hpre);
@@ -760,7 +765,7 @@ __global__ __launch_bounds__(256) void k_latb_tree(const DLeaf* __restrict__ req
             idx >>= 1;
         }
     }
-    F::store(node, out + (size_t)r * F::kN);
+    if (w == t) F::store(node, out + (size_t)r * F::kN);
 }
 
 // Auth walk (compute_root, xmss_commons.c:54-96) of every request, from its
