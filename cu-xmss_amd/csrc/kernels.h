@@ -61,6 +61,33 @@ __device__ __forceinline__ uint32_t mirror_lanes(uint32_t i, uint32_t n) {
     return base < n ? base + (i - base) % (n - base) : n;
 }
 
+// Keeps a node in vector registers.  When every lane of a wave holds the
+// same node (a lone request's auth walk, its idle lanes mirroring lane 0),
+// the compiler proves the hashing wave-uniform and moves much of it onto the
+// scalar unit, which runs it at about half the speed (an auth walk measured
+// 8.2 against 4.3 us per level); values made opaque here stay per lane.
+template <class N>
+__device__ __forceinline__ void keep_in_vgprs(N& x) {
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(x.w) / sizeof(x.w[0])); ++i) asm volatile("" : "+v"(x.w[i]));
+}
+
+// The children of an auth-path merge in the order the index bit gives:
+// (node, sib) for a left child, (sib, node) for a right one, selected word by
+// word.  Written as two calls behind a branch on the bit, a wave whose lanes
+// disagree -- or whose bit the compiler cannot prove uniform -- runs the hash
+// twice, once per side.
+template <class F>
+__device__ __forceinline__ void auth_pair(typename F::Node& l, typename F::Node& r, const typename F::Node& node,
+                                          const typename F::Node& sib, uint32_t right) {
+    constexpr int W = sizeof(node.w) / sizeof(node.w[0]);
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        l.w[i] = right ? sib.w[i] : node.w[i];
+        r.w[i] = right ? node.w[i] : sib.w[i];
+    }
+}
+
 // A compiler barrier for memory operations: loads issued before it stay
 // before it, so a prefetch cannot be sunk next to its first use (the
 // hardware waits for a load only where its registers are read).  Loads
@@ -290,8 +317,9 @@ __global__ __launch_bounds__(192) void k_leaf_wg(const typename F::Key* __restri
             F::load(sib, ap + (size_t)i * F::kN);
             a[5] = i;
             a[6] = idx >> 1;
-            if (idx & 1) F::thash_h(node, k, a, sib, node);
-            else F::thash_h(node, k, a, node, sib);
+            typename F::Node l, rr;
+            auth_pair<F>(l, rr, node, sib, idx & 1);
+            F::thash_h(node, k, a, l, rr);
             idx >>= 1;
         }
     }

@@ -142,6 +142,7 @@ struct BdsDev {
             typename F::Node L, R, o;
             F::load(L, hb);
             F::load(R, hb + n);
+            keep_in_vgprs(L);
             if (mcur < nmerge) {
                 F::fin_h(o, merges[mcur], L, R);
             } else {

@@ -195,7 +195,7 @@ __device__ __forceinline__ void ltree_lds(typename F::Node* pk, const typename F
 template <class F, uint32_t MODE>
 __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, const DLeaf* __restrict__ reqs,
                                             const uint8_t* __restrict__ msgs, const uint8_t* __restrict__ sigs,
-                                            const uint8_t* __restrict__ auth, const uint8_t* __restrict__ scratch,
+                                            const uint8_t* auth, const uint8_t* scratch,
                                             size_t stride, uint8_t* __restrict__ out,
                                             const uint8_t* __restrict__ spec = nullptr, uint32_t spec_first = 0) {
     using Lay = LatLayout<F>;
@@ -242,16 +242,32 @@ __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, co
     // the auth walk: lane 0, with the rest of its wave walking copies
     if (own >= 64) return;
     typename F::Node node = pk[0];
+    keep_in_vgprs(node);
     if (MODE == LAT_VERIFY) {
         const DLeaf q = reqs[r];
         uint32_t idx = q.leaf;
         const uint8_t* ap = auth + (size_t)q.p2 * F::kN;
-        for (uint32_t i = 0; i < q.p0; ++i) {
-            typename F::Node sib;
-            F::load(sib, ap + (size_t)i * F::kN);
-            if (idx & 1) F::fin_h(node, hp[F::LEN - 1 + i], sib, node);
-            else F::fin_h(node, hp[F::LEN - 1 + i], node, sib);
+        // each level's sibling and pre-image are loaded one level ahead (a
+        // level's loads waited for in place cost about as much as its two
+        // compressions: 8.6 us per level measured)
+        const uint32_t levels = q.p0;
+        typename F::Node sib, sib_next;
+        HPre h, h_next;
+        if (levels) {
+            F::load(sib, ap);
+            h = hp[F::LEN - 1];
+        }
+        for (uint32_t i = 0; i < levels; ++i) {
+            const uint32_t j = i + 1 < levels ? i + 1 : i;
+            F::load(sib_next, ap + (size_t)j * F::kN);
+            h_next = hp[F::LEN - 1 + j];
+            keep_loads_ahead();
+            typename F::Node l, rr;
+            auth_pair<F>(l, rr, node, sib, idx & 1);
+            F::fin_h(node, h, l, rr);
             idx >>= 1;
+            sib = sib_next;
+            h = h_next;
         }
     }
     if (own == 0) F::store(node, out + (size_t)r * F::kN);
@@ -294,9 +310,9 @@ __device__ __forceinline__ void lat_leaf_ltree(uint32_t r, typename F::Node* pk,
 template <class F, uint32_t MODE>
 __global__ __launch_bounds__(192) void k_lat_fin(const DLeaf* __restrict__ reqs, const uint8_t* __restrict__ msgs,
                                                   const uint8_t* __restrict__ sigs,
-                                                  const uint8_t* __restrict__ auth,
-                                                  const uint8_t* __restrict__ scratch, size_t stride,
-                                                  uint8_t* __restrict__ out) {
+                                                  const uint8_t* auth,  // not restrict: the auth walk's
+                                                  const uint8_t* scratch,  // loads stay one level ahead
+                                                  size_t stride, uint8_t* __restrict__ out) {
     __shared__ typename F::Node pk[F::LEN];
     lat_fin_one<F, MODE>(blockIdx.x, pk, reqs, msgs, sigs, auth, scratch, stride, out);
 }
@@ -323,8 +339,8 @@ __global__ __launch_bounds__(192) void k_lat_verify_layers(const DLeaf* __restri
                                                             uint32_t L0, uint32_t L1,
                                                             const uint8_t* __restrict__ digests,
                                                             const uint8_t* __restrict__ sigs,
-                                                            const uint8_t* __restrict__ auth,
-                                                            const uint8_t* __restrict__ scratch, size_t stride,
+                                                            const uint8_t* auth,  // not restrict: see k_lat_fin
+                                                            const uint8_t* scratch, size_t stride,
                                                             uint8_t* __restrict__ roots, uint32_t nwalk,
                                                             uint8_t* __restrict__ spec, uint32_t spec_first,
                                                             uint32_t nspec) {
@@ -758,10 +774,10 @@ __global__ __launch_bounds__(256) void k_latb_tree(const DLeaf* __restrict__ req
         uint32_t idx = q.leaf;
         const uint8_t* ap = auth + (size_t)q.p2 * F::kN;
         for (uint32_t i = 0; i < q.p0; ++i) {
-            Node sib;
+            Node sib, l, rr;
             F::load(sib, ap + (size_t)i * F::kN);
-            if (idx & 1) F::fin_h(node, hp[LEN - 1 + i], sib, node);
-            else F::fin_h(node, hp[LEN - 1 + i], node, sib);
+            auth_pair<F>(l, rr, node, sib, idx & 1);
+            F::fin_h(node, hp[LEN - 1 + i], l, rr);
             idx >>= 1;
         }
     }
@@ -798,8 +814,9 @@ __global__ __launch_bounds__(WG) void k_latb_auth(const DLeaf* __restrict__ reqs
         F::load(sib_next, ap + (size_t)j * F::kN);
         h_next = hp[F::LEN - 1 + j];
         keep_loads_ahead();
-        if (idx & 1) F::fin_h(node, h, sib, node);
-        else F::fin_h(node, h, node, sib);
+        typename F::Node l, rr;
+        auth_pair<F>(l, rr, node, sib, idx & 1);
+        F::fin_h(node, h, l, rr);
         idx >>= 1;
         sib = sib_next;
         h = h_next;
