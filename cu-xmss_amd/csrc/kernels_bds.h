@@ -431,7 +431,8 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
                                                   uint8_t* __restrict__ blob, uint8_t* __restrict__ R_out,
                                                   uint8_t* __restrict__ digests, const DLeaf* __restrict__ leaves,
                                                   uint32_t nleaf, const DLeaf* __restrict__ sigs, uint32_t nsig,
-                                                  const uint8_t* __restrict__ scratch, size_t stride,
+                                                  const uint8_t* scratch,  // not restrict: the chain
+                                                  size_t stride,           // loops' loads stay ahead
                                                   uint8_t* __restrict__ leaf_out, uint8_t* __restrict__ sig_out,
                                                   uint8_t* __restrict__ ends) {
     __shared__ BdsFinLds<F> sh;

@@ -227,6 +227,7 @@ __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, co
         FPre cur = fp[s0 < 15 ? s0 : 14];
         for (uint32_t s = s0; s < s1; ++s) {
             const FPre nxt = fp[s + 1 < 15 ? s + 1 : 14];
+            keep_loads_ahead();
             F::fin_f(x, cur, x);
             cur = nxt;
         }
@@ -278,7 +279,7 @@ __device__ __forceinline__ void lat_fin_one(uint32_t r, typename F::Node* pk, co
 // stored at ends; and the L-tree of LEN chain ends read from ends, its root
 // stored at out[r].  Every thread of the workgroup calls them.
 template <class F>
-__device__ __forceinline__ void lat_leaf_chains(uint32_t r, const uint8_t* __restrict__ scratch, size_t stride,
+__device__ __forceinline__ void lat_leaf_chains(uint32_t r, const uint8_t* scratch, size_t stride,
                                                 uint8_t* __restrict__ ends) {
     using Lay = LatLayout<F>;
     using FPre = typename F::FPre;
@@ -290,6 +291,7 @@ __device__ __forceinline__ void lat_leaf_chains(uint32_t r, const uint8_t* __res
     FPre cur = fp[0];
     for (uint32_t s = 0; s < 15; ++s) {
         const FPre nxt = fp[s + 1 < 15 ? s + 1 : 14];
+        keep_loads_ahead();
         F::fin_f(x, cur, x);
         cur = nxt;
     }
@@ -516,7 +518,7 @@ __global__ __launch_bounds__(256) void k_pre_need(const typename F::Key* __restr
 // chain (all 15 steps from the pre-images, full waves) and then one workgroup
 // of ceil(LEN/2 / 64) waves per leaf for the L-tree through LDS.
 template <class F>
-__global__ __launch_bounds__(WG) void k_lat_chains_all(uint32_t n, const uint8_t* __restrict__ scratch, size_t stride,
+__global__ __launch_bounds__(WG) void k_lat_chains_all(uint32_t n, const uint8_t* scratch, size_t stride,
                                                         uint8_t* __restrict__ ends) {
     using FPre = typename F::FPre;
     const uint32_t g = blockIdx.x * WG + threadIdx.x;
@@ -528,6 +530,7 @@ __global__ __launch_bounds__(WG) void k_lat_chains_all(uint32_t n, const uint8_t
     FPre cur = fp[0];
     for (uint32_t s = 0; s < 15; ++s) {
         const FPre nxt = fp[s + 1 < 15 ? s + 1 : 14];
+        keep_loads_ahead();  // (scratch is not restrict, so the load stays here)
         F::fin_f(x, cur, x);
         cur = nxt;
     }
