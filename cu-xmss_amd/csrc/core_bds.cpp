@@ -808,6 +808,7 @@ static int bds_sign_api(const xmss_params* p, unsigned char* sk, unsigned char* 
             std::vector<LeafRun> next(ds.next.size());
             for (size_t i = 0; i < next.size(); ++i) next[i] = {0, ds.next[i].layer, ds.next[i].tree, ds.next[i].leaf};
             e->ends_ahead(fam, ks, next.data(), (int)next.size());
+            ds.job.sig_values = true;
             e->bds_sign_batch(fam, 1, &ds.job);
             bds_device_finish(p, sm, smlen, mlen, ds);
             return 0;

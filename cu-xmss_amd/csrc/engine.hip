@@ -2045,7 +2045,7 @@ namespace xm {
 
 
 struct BdsBufs {
-    DevBuf io, leaf_arena, dig, scratch, lat_leaf, merge_arena;
+    DevBuf io, leaf_arena, dig, scratch, lat_leaf, merge_arena, vals;
     PinnedBuf staging;
 };
 
@@ -2277,11 +2277,17 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     uint64_t mmax = 0;
     for (int i = 0; i < count; ++i) mmax = std::max<uint64_t>(mmax, jobs[i].mlen);
     const bool long_msgs = mmax > kBdsShortMsg;
+    // one signature of a long message (the one-call path): its WOTS chains
+    // are walked to every value beside H_msg, and the update launch picks
+    // the signature once the digest is there (the chains leave the serial
+    // path behind H_msg; a group of the serving loop keeps the plain walk,
+    // whose memory its budget counts)
+    uint8_t* vals = long_msgs && count == 1 && jobs[0].sig_values ? B.vals.need((size_t)len * 16 * n) : nullptr;
     o->bds_lat(I->L, dk, dmd, (uint32_t)count, pad, db, dR, dig, dld, (uint32_t)nl, dw, (uint32_t)nw, lat, dmg,
-               (uint32_t)mg.size(), merge_arena, leaf_arena, dW, long_msgs, ends);
+               (uint32_t)mg.size(), merge_arena, leaf_arena, dW, long_msgs, ends, vals);
     o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, ddag, dlit,
                   lat + (nl + count) * o->lat_bytes(0), dT, B.scratch.need((size_t)count * 4 * n),
-                  long_msgs ? dw : nullptr, dig, lat + nl * o->lat_bytes(0), dW);
+                  long_msgs ? dw : nullptr, dig, lat + nl * o->lat_bytes(0), dW, vals);
     I->d2h(h + o_sk, ska, io_bytes - o_sk);
     I->sync();
     for (int i = 0; i < count; ++i) {
@@ -2539,7 +2545,7 @@ void Engine::release_workspace() {
     for (DevBuf* x : bufs) x->release();
     if (I->bds) {
         DevBuf* b[] = {&I->bds->io, &I->bds->leaf_arena, &I->bds->dig, &I->bds->scratch, &I->bds->lat_leaf,
-                       &I->bds->merge_arena};
+                       &I->bds->merge_arena, &I->bds->vals};
         for (DevBuf* x : b) x->release();
     }
     I->ahead.clear();

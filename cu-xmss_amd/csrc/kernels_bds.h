@@ -434,7 +434,8 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
                                                   const uint8_t* scratch,  // not restrict: the chain
                                                   size_t stride,           // loops' loads stay ahead
                                                   uint8_t* __restrict__ leaf_out, uint8_t* __restrict__ sig_out,
-                                                  uint8_t* __restrict__ ends) {
+                                                  uint8_t* __restrict__ ends, uint32_t nvals,
+                                                  uint8_t* __restrict__ vals) {
     __shared__ BdsFinLds<F> sh;
     if (blockIdx.x < nleaf) {
         const DLeaf q = leaves[blockIdx.x];
@@ -445,10 +446,16 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
     } else if (blockIdx.x < nleaf + nsig) {
         lat_fin_one<F, LAT_SIGN>(blockIdx.x - nleaf, sh.pk, sigs, digests, nullptr, nullptr,
                                  scratch + (size_t)nleaf * stride, stride, sig_out);
+    } else if (blockIdx.x < nleaf + nsig + nvals) {
+        // long messages, one signature: every value along its WOTS chains,
+        // beside H_msg (the update launch then picks the signature)
+        const uint32_t j = blockIdx.x - nleaf - nsig;
+        lat_sig_values<F>(j, scratch + (size_t)nleaf * stride, stride, vals + (size_t)j * F::LEN * 16 * F::kN);
     } else if constexpr (BdsFinPair<F>::value) {
-        hmsg_pair_block<F>(keys, msgs, (blockIdx.x - nleaf - nsig) * 64, nmsg, pad, blob, R_out, digests, sh.pair);
+        hmsg_pair_block<F>(keys, msgs, (blockIdx.x - nleaf - nsig - nvals) * 64, nmsg, pad, blob, R_out, digests,
+                           sh.pair);
     } else {
-        hmsg_mirrored<F>(keys, msgs, (blockIdx.x - nleaf - nsig) * blockDim.x + threadIdx.x, nmsg, pad, blob,
+        hmsg_mirrored<F>(keys, msgs, (blockIdx.x - nleaf - nsig - nvals) * blockDim.x + threadIdx.x, nmsg, pad, blob,
                          R_out, digests);
     }
 }
@@ -603,10 +610,15 @@ __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __res
                              const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
                              uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena, uint32_t njobs,
                              const DLeaf* __restrict__ sigs, const uint8_t* __restrict__ digests,
-                             const uint8_t* __restrict__ sign_scratch, uint8_t* __restrict__ sig_out) {
+                             const uint8_t* __restrict__ sign_scratch, uint8_t* __restrict__ sig_out,
+                             const uint8_t* __restrict__ vals) {
     if (blockIdx.x >= njobs) {  // long messages: the layer-0 WOTS signature of job blockIdx.x - njobs
-        lat_fin_one<F, LAT_SIGN>(blockIdx.x - njobs, nullptr, sigs, digests, nullptr, nullptr, sign_scratch, wstride,
-                                 sig_out);
+        const uint32_t j = blockIdx.x - njobs;
+        if (vals)  // its chain values were walked beside H_msg
+            lat_sig_select<F>(digests + (size_t)j * F::kN, vals + (size_t)j * F::LEN * 16 * F::kN,
+                              sig_out + (size_t)j * F::LEN * F::kN);
+        else
+            lat_fin_one<F, LAT_SIGN>(j, nullptr, sigs, digests, nullptr, nullptr, sign_scratch, wstride, sig_out);
         return;
     }
     const DBds job = jobs[blockIdx.x];

@@ -298,6 +298,45 @@ __device__ __forceinline__ void lat_leaf_chains(uint32_t r, const uint8_t* scrat
     if (own == c) F::store(x, ends + (size_t)c * F::kN);
 }
 
+// Every value along the LEN chains of WOTS signature request r (scratch row r
+// holds its chain starts and all 15 steps' F pre-images): values[c * 16 + s]
+// = chain c after s steps.  A signature whose message digest is still being
+// hashed walks its chains this way beside H_msg; the signature is then the
+// value at each chain's digit (lat_sig_select).
+template <class F>
+__device__ __forceinline__ void lat_sig_values(uint32_t r, const uint8_t* scratch, size_t stride,
+                                               uint8_t* __restrict__ values) {
+    using Lay = LatLayout<F>;
+    using FPre = typename F::FPre;
+    const uint32_t own = threadIdx.x, c = mirror_lanes(own, F::LEN);
+    if (c >= (uint32_t)F::LEN) return;
+    const uint8_t* S = scratch + (size_t)r * stride;
+    typename F::Node x = reinterpret_cast<const typename F::Node*>(S + Lay::starts)[c];
+    const FPre* fp = reinterpret_cast<const FPre*>(S + Lay::fpre) + c * 15;
+    uint8_t* v = values + (size_t)c * 16 * F::kN;
+    if (own == c) F::store(x, v);
+    FPre cur = fp[0];
+    for (uint32_t s = 0; s < 15; ++s) {
+        const FPre nxt = fp[s + 1 < 15 ? s + 1 : 14];
+        keep_loads_ahead();
+        F::fin_f(x, cur, x);
+        cur = nxt;
+        if (own == c) F::store(x, v + (size_t)(s + 1) * F::kN);
+    }
+}
+
+// The WOTS signature of message digest msg from the chain values
+// lat_sig_values left (values: LEN x 16 nodes): chain c at its digit.
+template <class F>
+__device__ __forceinline__ void lat_sig_select(const uint8_t* __restrict__ msg, const uint8_t* __restrict__ values,
+                                               uint8_t* __restrict__ out) {
+    for (uint32_t c = threadIdx.x; c < (uint32_t)F::LEN; c += blockDim.x) {
+        typename F::Node x;
+        F::load(x, values + ((size_t)c * 16 + chain_len<F>(msg, c)) * F::kN);
+        F::store(x, out + (size_t)c * F::kN);
+    }
+}
+
 template <class F>
 __device__ __forceinline__ void lat_leaf_ltree(uint32_t r, typename F::Node* pk, const uint8_t* __restrict__ scratch,
                                                size_t stride, const uint8_t* __restrict__ ends,

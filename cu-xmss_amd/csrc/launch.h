@@ -192,11 +192,14 @@ struct FamOps {
     // long: R / H_msg beside the leaves and the nmsg layer-0 signatures
     // left to bds_update (else H_msg first and the signatures beside the
     // leaves, into sig_out).  ends: the chain-end slots of leaf rows marked
-    // LAT_ENDS_OUT / LAT_ENDS_IN (LEN x n bytes each).
+    // LAT_ENDS_OUT / LAT_ENDS_IN (LEN x n bytes each).  vals (long messages
+    // only, or null): every value along the nmsg layer-0 signatures' chains
+    // (LEN x 16 nodes each) is hashed beside H_msg into it, and bds_update
+    // then picks the signatures from it instead of walking the chains.
     virtual void bds_lat(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad, uint8_t* blob,
                          uint8_t* R, uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs,
                          uint32_t nsign, uint8_t* scratch, const DThash* merges, uint32_t nmerge, void* merge_out,
-                         uint8_t* leaf_out, uint8_t* sig_out, bool long_msgs, uint8_t* ends) = 0;
+                         uint8_t* leaf_out, uint8_t* sig_out, bool long_msgs, uint8_t* ends, uint8_t* vals) = 0;
     // The state updates, one workgroup per job; dag / lits: the planned
     // merges' sources (3 words per merge, parallel to merge_arena) and the
     // literal node values (jobs with maxlev 0 hash their merges one by one).
@@ -206,7 +209,7 @@ struct FamOps {
                             uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena,
                             const uint32_t* dag, const uint8_t* lits, const uint8_t* wpre, uint8_t* tails,
                             uint8_t* scratch, const DLeaf* sigs, const uint8_t* digests,
-                            const uint8_t* sign_scratch, uint8_t* sig_out) = 0;
+                            const uint8_t* sign_scratch, uint8_t* sig_out, const uint8_t* vals) = 0;
 };
 
 template <class F>
@@ -390,7 +393,7 @@ struct FamOpsT final : FamOps {
     void bds_lat(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad, uint8_t* blob,
                  uint8_t* R, uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs, uint32_t nsign,
                  uint8_t* scratch, const DThash* merges, uint32_t nmerge, void* merge_out, uint8_t* leaf_out,
-                 uint8_t* sig_out, bool long_msgs, uint8_t* ends) override {
+                 uint8_t* sig_out, bool long_msgs, uint8_t* ends, uint8_t* vals) override {
         const size_t stride = LatLayout<F>::bytes(0);
         const uint32_t xb = cdiv(F::LEN * 17 - 1, WG);  // LAT_LEAF / LAT_SIGN items per request
         const uint32_t mrows = cdiv(nmerge, (uint64_t)xb * WG);
@@ -400,19 +403,21 @@ struct FamOpsT final : FamOps {
               blob, R, digests, hrows, nmsg, leaves, nleaf, sigs, nsign, scratch, stride, merges, nmerge,
               (typename F::HPre*)merge_out);
         const uint32_t nsig = long_msgs ? 0 : nmsg, nlate = long_msgs ? nmsg : 0;
+        const uint32_t nvals = long_msgs && vals ? nmsg : 0;
         // late H_msg: 64 messages per workgroup by its two waves (k_bds_fin)
         const uint32_t per = BdsFinPair<F>::value ? 64 : threads;
-        L.run("bds_fin", k_bds_fin<F>, dim3(nleaf + nsig + cdiv(nlate, per)), dim3(threads), (const K*)keys,
+        L.run("bds_fin", k_bds_fin<F>, dim3(nleaf + nsig + nvals + cdiv(nlate, per)), dim3(threads), (const K*)keys,
               msgs, nlate, pad, blob, R, digests, leaves, nleaf, sigs, nsig, (const uint8_t*)scratch, stride,
-              leaf_out, sig_out, ends);
+              leaf_out, sig_out, ends, nvals, vals);
     }
     void bds_update(Launcher& L, const void* keys, size_t lds, const DBds* jobs, uint32_t n,
                     uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena, const uint32_t* dag,
                     const uint8_t* lits, const uint8_t* wpre, uint8_t* tails, uint8_t* scratch, const DLeaf* sigs,
-                    const uint8_t* digests, const uint8_t* sign_scratch, uint8_t* sig_out) override {
+                    const uint8_t* digests, const uint8_t* sign_scratch, uint8_t* sig_out,
+                    const uint8_t* vals) override {
         L.run_lds("bds_update", k_bds_update<F>, dim3(sigs ? 2 * n : n), dim3(((F::LEN + 63) / 64) * 64), lds, (const K*)keys,
                   jobs, sk_arena, leaf_arena, (const typename F::HPre*)merge_arena, dag, lits, wpre,
-                  LatLayout<F>::bytes(0), tails, scratch, n, sigs, digests, sign_scratch, sig_out);
+                  LatLayout<F>::bytes(0), tails, scratch, n, sigs, digests, sign_scratch, sig_out, vals);
     }
 };
 
