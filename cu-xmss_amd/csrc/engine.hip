@@ -228,6 +228,17 @@ constexpr uint64_t kLatLeaves = 16384;
 constexpr size_t kBdsLds = 64 * 1024;
 // BDS signatures: messages up to this long are hashed before the leaves.
 constexpr uint64_t kBdsShortMsg = 512;
+// Polls (about 2 us apart) the merge-DAG block of a one-call BDS signature's
+// finishing launch makes for the planned leaves before it leaves the DAG to
+// the update launch (BdsFinDag); XMSS_MI355X_DAG_SPIN overrides it, and 0
+// keeps the DAG in the update.
+static uint32_t dag_spin() {
+    static const uint32_t v = [] {
+        const char* e = getenv("XMSS_MI355X_DAG_SPIN");
+        return e ? (uint32_t)strtoul(e, nullptr, 10) : (uint32_t)(1u << 16);
+    }();
+    return v;
+}
 static_assert(Engine::DAG_LEAF == DAG_LEAF && Engine::DAG_MERGE == DAG_MERGE && Engine::DAG_INDEX == DAG_INDEX,
               "merge source codes of the host plan and the kernel");
 static bool use_lat(FamOps* o, size_t count, uint32_t th) {
@@ -2045,7 +2056,7 @@ namespace xm {
 
 
 struct BdsBufs {
-    DevBuf io, leaf_arena, dig, scratch, lat_leaf, merge_arena, vals;
+    DevBuf io, leaf_arena, dig, scratch, lat_leaf, merge_arena, vals, dagv;
     PinnedBuf staging;
 };
 
@@ -2213,7 +2224,8 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     const size_t o_ld = 0, o_md = o_ld + al(sizeof(DLeaf) * ld.size()), o_blob = o_md + al(sizeof(DMsg) * count),
                  o_wd = o_blob + al(total), o_jd = o_wd + al(sizeof(DLeaf) * (count + up.size())),
                  o_mg = o_jd + al(sizeof(DBds) * count), o_dag = o_mg + al(sizeof(DThash) * mg.size()),
-                 o_lit = o_dag + al(sizeof(uint32_t) * dag.size()), o_sk = o_lit + al(lits.size()),
+                 o_lit = o_dag + al(sizeof(uint32_t) * dag.size()), o_sync = o_lit + al(lits.size()),
+                 o_sk = o_sync + al(2 * sizeof(uint32_t)),
                  o_R = o_sk + al(sk_total), o_W = o_R + al((size_t)count * n),
                  o_T = o_W + al((size_t)count * len * n), io_bytes = o_T + al(tail_total);
     uint8_t* h = B.staging.need(io_bytes);
@@ -2225,6 +2237,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     memcpy(h + o_mg, mg.data(), sizeof(DThash) * mg.size());
     memcpy(h + o_dag, dag.data(), sizeof(uint32_t) * dag.size());
     memcpy(h + o_lit, lits.data(), lits.size());
+    memset(h + o_sync, 0, 2 * sizeof(uint32_t));  // BdsFinDag: leaves counted in, DAG hashed
     const size_t nl = ld.size(), nw = count + up.size();
     if (nl + nw + (mg.size() + count) / WG + 2 > 65535 || !use_lat(o, nl + nw, 0))
         fatal("bds_sign_chunk: batch too large for one launch sequence");
@@ -2283,11 +2296,26 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     // path behind H_msg; a group of the serving loop keeps the plain walk,
     // whose memory its budget counts)
     uint8_t* vals = long_msgs && count == 1 && jobs[0].sig_values ? B.vals.need((size_t)len * 16 * n) : nullptr;
+    // one-call signature with a merge DAG: the DAG is hashed in the
+    // finishing launch as soon as the planned leaves are in, beside H_msg
+    // and the WOTS signature, instead of at the start of the update
+    BdsFinDag fd{};
+    if (count == 1 && jobs[0].sig_values && jd[0].maxlev && jobs[0].nplan && dag_spin()) {
+        fd.sync = reinterpret_cast<uint32_t*>(dv + o_sync);
+        fd.wait = jobs[0].nplan;
+        fd.spin = dag_spin();
+        fd.nmerge = jd[0].nmerge;
+        fd.maxlev = jd[0].maxlev;
+        fd.q0 = ddag;  // job 0's merges and literals start at offset 0
+        fd.lits = dlit;
+        fd.pre = merge_arena;
+        fd.vals = B.dagv.need((size_t)jd[0].nmerge * n);
+    }
     o->bds_lat(I->L, dk, dmd, (uint32_t)count, pad, db, dR, dig, dld, (uint32_t)nl, dw, (uint32_t)nw, lat, dmg,
-               (uint32_t)mg.size(), merge_arena, leaf_arena, dW, long_msgs, ends, vals);
+               (uint32_t)mg.size(), merge_arena, leaf_arena, dW, long_msgs, ends, vals, fd);
     o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, ddag, dlit,
                   lat + (nl + count) * o->lat_bytes(0), dT, B.scratch.need((size_t)count * 4 * n),
-                  long_msgs ? dw : nullptr, dig, lat + nl * o->lat_bytes(0), dW, vals);
+                  long_msgs ? dw : nullptr, dig, lat + nl * o->lat_bytes(0), dW, vals, fd.sync, fd.vals);
     I->d2h(h + o_sk, ska, io_bytes - o_sk);
     I->sync();
     for (int i = 0; i < count; ++i) {
@@ -2545,7 +2573,7 @@ void Engine::release_workspace() {
     for (DevBuf* x : bufs) x->release();
     if (I->bds) {
         DevBuf* b[] = {&I->bds->io, &I->bds->leaf_arena, &I->bds->dig, &I->bds->scratch, &I->bds->lat_leaf,
-                       &I->bds->merge_arena, &I->bds->vals};
+                       &I->bds->merge_arena, &I->bds->vals, &I->bds->dagv};
         for (DevBuf* x : b) x->release();
     }
     I->ahead.clear();

@@ -421,6 +421,60 @@ struct BdsFinLds<F, false> {
     typename F::Node pk[F::LEN];
 };
 
+// The planned merges of one signature's state update hashed level by level,
+// one lane per merge, into mv (n bytes per merge): in the update's LDS, or
+// beside H_msg in k_bds_fin (BdsFinDag).  q0: 3 words per merge (sources and
+// level, Engine::BdsJob::plan_dag); pre: the merges' address-only work; every
+// thread of the workgroup calls it.
+template <class F>
+__device__ __forceinline__ void dag_load(typename F::Node& x, uint32_t src, const uint8_t* lits, const uint8_t* leaves,
+                         const uint8_t* mval) {
+    const uint32_t i = src & DAG_INDEX;
+    const uint8_t* p = (src & ~DAG_INDEX) == DAG_LIT ? lits : (src & ~DAG_INDEX) == DAG_LEAF ? leaves : mval;
+    F::load(x, p + (size_t)i * F::kN);
+}
+template <class F>
+__device__ void merge_dag(const uint32_t* q0, uint32_t nmerge, uint32_t maxlev, const uint8_t* lits,
+                          const uint8_t* leaves, const typename F::HPre* pre, uint8_t* mv) {
+    constexpr uint32_t n = F::kN;
+    for (uint32_t lev = 1; lev <= maxlev; ++lev) {
+        for (uint32_t m0 = 0; m0 < nmerge; m0 += blockDim.x) {
+            const uint32_t m = m0 + threadIdx.x;
+            const bool mine = m < nmerge && q0[3 * m + 2] == lev;
+            const uint64_t act = __ballot(mine);
+            if (act == 0) continue;
+            // a lane with no merge of this level hashes a copy of the next
+            // one in its wave, cyclically (mirror_lanes)
+            const uint32_t lane = threadIdx.x & 63;
+            const uint64_t rot = lane ? (act >> lane) | (act << (64 - lane)) : act;
+            const uint32_t src = m - lane + ((lane + (uint32_t)__builtin_ctzll(rot)) & 63);
+            const uint32_t* q = q0 + 3 * src;
+            typename F::Node L, R, o;
+            dag_load<F>(L, q[0], lits, leaves, mv);
+            dag_load<F>(R, q[1], lits, leaves, mv);
+            F::fin_h(o, pre[src], L, R);
+            if (mine) F::store(o, mv + (size_t)m * n);
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+// The merge DAG of a one-signature launch hashed in k_bds_fin, beside H_msg
+// and the WOTS signature, instead of in the update launch: its last block
+// waits until the blocks of the job's `wait` planned leaves have each
+// counted in at sync[0] (a bounded wait: after `spin` polls it gives up and
+// the update hashes the DAG as usual), hashes the DAG into vals and sets
+// sync[1].  vals == nullptr: off.
+struct BdsFinDag {
+    uint32_t* sync;
+    uint32_t wait, spin, nmerge, maxlev;
+    const uint32_t* q0;
+    const uint8_t* lits;
+    const void* pre;
+    uint8_t* vals;
+};
+
 // Blocks: nleaf leaves (or, for rows marked LAT_ENDS_OUT / LAT_ENDS_IN, the
 // chains into slot p2 of `ends` / the L-tree from it), then nsig WOTS
 // signatures (short messages), then the H_msg blocks (long messages): 64
@@ -435,14 +489,42 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
                                                   size_t stride,           // loops' loads stay ahead
                                                   uint8_t* __restrict__ leaf_out, uint8_t* __restrict__ sig_out,
                                                   uint8_t* __restrict__ ends, uint32_t nvals,
-                                                  uint8_t* __restrict__ vals) {
+                                                  uint8_t* __restrict__ vals, BdsFinDag dag) {
     __shared__ BdsFinLds<F> sh;
+    if (dag.vals && blockIdx.x == gridDim.x - 1) {  // the merge DAG, once the planned leaves are in
+        __shared__ uint32_t ready;
+        if (threadIdx.x == 0) {
+            uint32_t polls = 0;
+            while (__hip_atomic_load(dag.sync, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < dag.wait &&
+                   polls < dag.spin) {
+                __builtin_amdgcn_s_sleep(2);
+                ++polls;
+            }
+            ready = __hip_atomic_load(dag.sync, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= dag.wait;
+        }
+        __syncthreads();
+        if (!ready) return;  // the update hashes the DAG
+        merge_dag<F>(dag.q0, dag.nmerge, dag.maxlev, dag.lits, leaf_out,
+                     reinterpret_cast<const typename F::HPre*>(dag.pre), dag.vals);
+        if (threadIdx.x == 0) {
+            __threadfence();
+            dag.sync[1] = 1;
+        }
+        return;
+    }
     if (blockIdx.x < nleaf) {
         const DLeaf q = leaves[blockIdx.x];
         uint8_t* e = ends + (size_t)q.p2 * F::LEN * F::kN;
         if (q.p1 & LAT_ENDS_OUT) lat_leaf_chains<F>(blockIdx.x, scratch, stride, e);
         else if (q.p1 & LAT_ENDS_IN) lat_leaf_ltree<F>(blockIdx.x, sh.pk, scratch, stride, e, leaf_out);
         else lat_fin_one<F, LAT_LEAF>(blockIdx.x, sh.pk, leaves, nullptr, nullptr, nullptr, scratch, stride, leaf_out);
+        if (dag.vals && blockIdx.x < dag.wait) {  // a planned leaf of the DAG's job is in
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                __threadfence();
+                atomicAdd(dag.sync, 1u);
+            }
+        }
     } else if (blockIdx.x < nleaf + nsig) {
         lat_fin_one<F, LAT_SIGN>(blockIdx.x - nleaf, sh.pk, sigs, digests, nullptr, nullptr,
                                  scratch + (size_t)nleaf * stride, stride, sig_out);
@@ -464,20 +546,14 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
 // and the layer-0 WOTS signature, which run in their own launches): the auth
 // paths and lower-layer WOTS signatures go to the signature tail, then the
 // traversal updates the state at skg.  `scratch` holds 4n bytes per workgroup.
-template <class F>
-__device__ void dag_load(typename F::Node& x, uint32_t src, const uint8_t* lits, const uint8_t* leaves,
-                         const uint8_t* mval) {
-    const uint32_t i = src & DAG_INDEX;
-    const uint8_t* p = (src & ~DAG_INDEX) == DAG_LIT ? lits : (src & ~DAG_INDEX) == DAG_LEAF ? leaves : mval;
-    F::load(x, p + (size_t)i * F::kN);
-}
 
 template <class F>
 __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint8_t* skg,
                                const uint8_t* __restrict__ leaf_arena,
                                const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
                                const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
-                               uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena) {
+                               uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena,
+                               const uint8_t* premerged) {
     extern __shared__ __align__(16) uint8_t lds_states[];
     const BdsP& P = job.P;
     BdsDev<F> B(lds_states, skg, P, leaf_arena + job.leaf_off * F::kN, merge_arena + job.merge_off, job.nmerge,
@@ -500,28 +576,11 @@ __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint
     // instead of one per merge.
     if (job.maxlev) {
         uint8_t* mv = lds_states + ((region + 15) & ~15u);
-        const uint32_t* q0 = dag + 3 * job.merge_off;
-        const uint8_t* lits = lit_arena + job.lit_off * n;
-        const typename F::HPre* pre = merge_arena + job.merge_off;
-        for (uint32_t lev = 1; lev <= job.maxlev; ++lev) {
-            for (uint32_t m0 = 0; m0 < job.nmerge; m0 += blockDim.x) {
-                const uint32_t m = m0 + threadIdx.x;
-                const bool mine = m < job.nmerge && q0[3 * m + 2] == lev;
-                const uint64_t act = __ballot(mine);
-                if (act == 0) continue;
-                // a lane with no merge of this level hashes a copy of the
-                // next one in its wave, cyclically (mirror_lanes)
-                const uint32_t lane = threadIdx.x & 63;
-                const uint64_t rot = lane ? (act >> lane) | (act << (64 - lane)) : act;
-                const uint32_t src = m - lane + ((lane + (uint32_t)__builtin_ctzll(rot)) & 63);
-                const uint32_t* q = q0 + 3 * src;
-                typename F::Node L, R, o;
-                dag_load<F>(L, q[0], lits, B.leaves, mv);
-                dag_load<F>(R, q[1], lits, B.leaves, mv);
-                F::fin_h(o, pre[src], L, R);
-                if (mine) F::store(o, mv + (size_t)m * n);
-            }
-            __syncthreads();
+        if (premerged) {  // hashed beside H_msg (BdsFinDag)
+            for (uint32_t b = threadIdx.x; b < job.nmerge * n; b += blockDim.x) mv[b] = premerged[b];
+        } else {
+            merge_dag<F>(dag + 3 * job.merge_off, job.nmerge, job.maxlev, lit_arena + job.lit_off * n, B.leaves,
+                         merge_arena + job.merge_off, mv);
         }
         B.mval = mv;
     }
@@ -611,7 +670,7 @@ __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __res
                              uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena, uint32_t njobs,
                              const DLeaf* __restrict__ sigs, const uint8_t* __restrict__ digests,
                              const uint8_t* __restrict__ sign_scratch, uint8_t* __restrict__ sig_out,
-                             const uint8_t* __restrict__ vals) {
+                             const uint8_t* __restrict__ vals, const uint32_t* dag_sync, const uint8_t* dag_vals) {
     if (blockIdx.x >= njobs) {  // long messages: the layer-0 WOTS signature of job blockIdx.x - njobs
         const uint32_t j = blockIdx.x - njobs;
         if (vals)  // its chain values were walked beside H_msg
@@ -633,8 +692,10 @@ __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __res
         }
         __syncthreads();
     }
+    // job 0's merges, if k_bds_fin hashed them (dag_sync[1] set)
+    const uint8_t* premerged = dag_vals && blockIdx.x == 0 && dag_sync[1] == 1 ? dag_vals : nullptr;
     bds_update_one<F>(keys[job.key], job, skg, leaf_arena, merge_arena, dag, lit_arena, wpre, wstride, tails,
-                      scratch_arena);
+                      scratch_arena, premerged);
     if (res) {
         __syncthreads();
         for (uint32_t b = threadIdx.x; b < job.sk_bytes; b += blockDim.x) res[b] = skg[b];

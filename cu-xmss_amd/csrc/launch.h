@@ -199,7 +199,8 @@ struct FamOps {
     virtual void bds_lat(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad, uint8_t* blob,
                          uint8_t* R, uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs,
                          uint32_t nsign, uint8_t* scratch, const DThash* merges, uint32_t nmerge, void* merge_out,
-                         uint8_t* leaf_out, uint8_t* sig_out, bool long_msgs, uint8_t* ends, uint8_t* vals) = 0;
+                         uint8_t* leaf_out, uint8_t* sig_out, bool long_msgs, uint8_t* ends, uint8_t* vals,
+                         const BdsFinDag& dag) = 0;
     // The state updates, one workgroup per job; dag / lits: the planned
     // merges' sources (3 words per merge, parallel to merge_arena) and the
     // literal node values (jobs with maxlev 0 hash their merges one by one).
@@ -209,7 +210,8 @@ struct FamOps {
                             uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena,
                             const uint32_t* dag, const uint8_t* lits, const uint8_t* wpre, uint8_t* tails,
                             uint8_t* scratch, const DLeaf* sigs, const uint8_t* digests,
-                            const uint8_t* sign_scratch, uint8_t* sig_out, const uint8_t* vals) = 0;
+                            const uint8_t* sign_scratch, uint8_t* sig_out, const uint8_t* vals,
+                            const uint32_t* dag_sync, const uint8_t* dag_vals) = 0;
 };
 
 template <class F>
@@ -393,7 +395,7 @@ struct FamOpsT final : FamOps {
     void bds_lat(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad, uint8_t* blob,
                  uint8_t* R, uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs, uint32_t nsign,
                  uint8_t* scratch, const DThash* merges, uint32_t nmerge, void* merge_out, uint8_t* leaf_out,
-                 uint8_t* sig_out, bool long_msgs, uint8_t* ends, uint8_t* vals) override {
+                 uint8_t* sig_out, bool long_msgs, uint8_t* ends, uint8_t* vals, const BdsFinDag& dag) override {
         const size_t stride = LatLayout<F>::bytes(0);
         const uint32_t xb = cdiv(F::LEN * 17 - 1, WG);  // LAT_LEAF / LAT_SIGN items per request
         const uint32_t mrows = cdiv(nmerge, (uint64_t)xb * WG);
@@ -406,18 +408,21 @@ struct FamOpsT final : FamOps {
         const uint32_t nvals = long_msgs && vals ? nmsg : 0;
         // late H_msg: 64 messages per workgroup by its two waves (k_bds_fin)
         const uint32_t per = BdsFinPair<F>::value ? 64 : threads;
-        L.run("bds_fin", k_bds_fin<F>, dim3(nleaf + nsig + nvals + cdiv(nlate, per)), dim3(threads), (const K*)keys,
-              msgs, nlate, pad, blob, R, digests, leaves, nleaf, sigs, nsig, (const uint8_t*)scratch, stride,
-              leaf_out, sig_out, ends, nvals, vals);
+        // the merge-DAG block, if any, is the last one: it waits for the
+        // planned leaves' blocks, which are dispatched before it
+        L.run("bds_fin", k_bds_fin<F>, dim3(nleaf + nsig + nvals + cdiv(nlate, per) + (dag.vals ? 1 : 0)),
+              dim3(threads), (const K*)keys, msgs, nlate, pad, blob, R, digests, leaves, nleaf, sigs, nsig,
+              (const uint8_t*)scratch, stride, leaf_out, sig_out, ends, nvals, vals, dag);
     }
     void bds_update(Launcher& L, const void* keys, size_t lds, const DBds* jobs, uint32_t n,
                     uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena, const uint32_t* dag,
                     const uint8_t* lits, const uint8_t* wpre, uint8_t* tails, uint8_t* scratch, const DLeaf* sigs,
                     const uint8_t* digests, const uint8_t* sign_scratch, uint8_t* sig_out,
-                    const uint8_t* vals) override {
+                    const uint8_t* vals, const uint32_t* dag_sync, const uint8_t* dag_vals) override {
         L.run_lds("bds_update", k_bds_update<F>, dim3(sigs ? 2 * n : n), dim3(((F::LEN + 63) / 64) * 64), lds, (const K*)keys,
                   jobs, sk_arena, leaf_arena, (const typename F::HPre*)merge_arena, dag, lits, wpre,
-                  LatLayout<F>::bytes(0), tails, scratch, n, sigs, digests, sign_scratch, sig_out, vals);
+                  LatLayout<F>::bytes(0), tails, scratch, n, sigs, digests, sign_scratch, sig_out, vals, dag_sync,
+                  dag_vals);
     }
 };
 

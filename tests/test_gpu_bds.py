@@ -151,3 +151,38 @@ def test_bds_interleaved_keys_and_an_older_state():
         rc, sm, sk2 = lib.sign(sk, m, mt)
         row = g["signs"][i]
         assert rc == row["rc"] and dg(sm) == row["sm"] and dg(sk2) == row["sk"], "again from state %d" % i
+
+
+_DAG_CHILD = r'''
+import os, random, sys
+sys.path.insert(0, sys.argv[1])
+from common import dg, golden, product
+lib = product(1)
+g = golden("bds_xmss_sha2_10_256.json")
+p = lib.params(g["oid"], 0)
+rc, pk, sk = lib.keypair(g["oid"], 0, sk_init=bytes(4 + p.sk_bytes))
+assert rc == 0 and sk.hex() == g["sk"]
+rng = random.Random(g["msg_seed"])
+for i, row in enumerate(g["signs"][:int(sys.argv[2])]):
+    m = bytes(rng.getrandbits(8) for _ in range(32))
+    rc, sm, sk = lib.sign(sk, m, 0)
+    assert rc == row["rc"] and dg(sm) == row["sm"] and dg(sk) == row["sk"], "sign %d" % i
+print("ok")
+'''
+
+
+@pytest.mark.parametrize("spin", ["1", "0"])
+def test_bds_merge_dag_fallbacks(spin):
+    """A one-call signature hashes its merge DAG in the finishing launch once
+    the planned leaves' blocks have counted in, a wait bounded by
+    XMSS_MI355X_DAG_SPIN polls; with one poll the wait mostly gives up and the
+    update launch hashes the DAG, with 0 the finishing launch never tries.
+    Either way the signatures and sk equal the reference's."""
+    import os
+    import subprocess
+    import sys
+    tests = os.path.dirname(os.path.abspath(__file__))
+    env = dict(os.environ, XMSS_MI355X_DAG_SPIN=spin)
+    r = subprocess.run([sys.executable, "-c", _DAG_CHILD, tests, "200"], env=env, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stderr[-2000:]
