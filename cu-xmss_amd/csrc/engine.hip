@@ -2172,7 +2172,12 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
         } else {
             dag.resize(dag.size() + 3 * jobs[i].nmerge, 0);
         }
-        lds = std::max(lds, jd[i].maxlev ? with_dag : region);
+        const size_t used = jd[i].maxlev ? with_dag : region;
+        // the planned leaves' values staged in LDS after the states and merges
+        const size_t lv_off = (used + 15) & ~(size_t)15, with_lv = lv_off + (size_t)jobs[i].nplan * n;
+        jd[i].nleaf = jobs[i].nplan;
+        jd[i].lv_off = jobs[i].nplan && with_lv <= kBdsLds ? (uint32_t)lv_off : 0;
+        lds = std::max(lds, jd[i].lv_off ? with_lv : used);
         jd[i].wots_row = (uint32_t)up.size();
         jd[i].nwots = jobs[i].plan_wots ? jobs[i].nwots : 0;
         for (uint32_t k = 0; k < jd[i].nwots; ++k) {

@@ -40,6 +40,8 @@ struct DBds {
     uint64_t lit_off;     // first literal (n bytes each) in the literal arena
     uint32_t wots_row;    // first pre-image row of its upper-layer WOTS signatures
     uint32_t nwots;       // planned upper-layer WOTS signatures (beyond: whole chains)
+    uint32_t nleaf;       // planned leaves (from leaf_off)
+    uint32_t lv_off;      // LDS offset their values are staged at, or 0 (read from HBM)
 };
 
 // Sources of a planned merge's halves (kind in the top 4 bits, index below).
@@ -185,7 +187,7 @@ struct BdsDev {
     }
 
     // treehash_update (xmss_core_fast.c:406-452) on instance j of state s;
-    // nb is a 2n-byte scratch in global memory
+    // nb is a 2n-byte scratch (LDS)
     __device__ void treehash_update(uint32_t s, uint32_t j, const uint32_t (&addr)[8], uint8_t* nb) {
         uint32_t nd[8] = {addr[0], addr[1], addr[2], 2, 0, 0, 0, 0};
         const uint32_t next = th_next(s, j);
@@ -219,20 +221,25 @@ struct BdsDev {
 
     // bds_treehash_update (xmss_core_fast.c:458-491), `char` return kept
     __device__ uint32_t bds_treehash_update(uint32_t s, uint32_t updates, const uint32_t (&addr)[8], uint8_t* nb) {
-        const uint32_t nth = P.th - P.bds_k;
+        const uint32_t nth = P.th - P.bds_k;  // at most 20 instances
         uint32_t used = 0;
         for (uint32_t u = 0; u < updates; ++u) {
-            uint32_t l_min = P.th, lev = nth;
-            for (uint32_t i = 0; i < nth; ++i) {
+            // The instance with the lowest node height on the stack (the
+            // reference scans them in order, keeping the first strictly
+            // lowest below th): lane i of every wave takes instance i, and a
+            // min over (height, i) across the wave picks the same one.
+            const uint32_t i = tid & 63;
+            uint32_t key = 0xFFFFFFFFu;
+            if (i < nth) {
                 uint32_t low;
                 if (th_done(s, i)) low = P.th;
                 else if (th_usage(s, i) == 0) low = i;
                 else low = minheight(s, i);
-                if (low < l_min) {
-                    lev = i;
-                    l_min = low;
-                }
+                key = (low << 8) | i;
             }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) key = min(key, (uint32_t)__shfl_xor((int)key, o));
+            const uint32_t lev = (key >> 8) < P.th ? (key & 0xFF) : nth;
             if (lev == nth) break;
             treehash_update(s, lev, addr, nb);
             ++used;
@@ -545,7 +552,8 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
 // The state part of one BDS signature per workgroup (everything after R, H_msg
 // and the layer-0 WOTS signature, which run in their own launches): the auth
 // paths and lower-layer WOTS signatures go to the signature tail, then the
-// traversal updates the state at skg.  `scratch` holds 4n bytes per workgroup.
+// traversal updates the state at skg.  Its node scratch is 4n bytes of LDS
+// (scratch_arena is no longer read).
 
 template <class F>
 __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint8_t* skg,
@@ -561,13 +569,26 @@ __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint
     B.wpre = wpre + (size_t)job.wots_row * wstride;
     B.wstride = wstride;
     B.nwots = job.nwots;
-    uint8_t* scratch = scratch_arena + (size_t)blockIdx.x * 4 * F::kN;
+    // the state machine's node scratch (4n bytes) in LDS: its copies are
+    // one byte per lane behind a barrier, and each through HBM cost about a
+    // microsecond (a treehash step is several of them)
+    __shared__ alignas(16) uint8_t scr[4 * F::kN];
+    uint8_t* scratch = scr;
+    (void)scratch_arena;
     uint8_t* tail = tails + job.tail_off;
     const uint32_t n = F::kN, th = P.th, d = P.d;
     const uint64_t idx = job.idx;
     const uint32_t region = (2 * d - 1) * P.state_bytes;
     // stage the serialized states in LDS; written back when the update is done
     for (uint32_t b = threadIdx.x; b < region; b += blockDim.x) lds_states[b] = skg[B.states_base() + b];
+    // and the planned leaves: the state machine takes them one at a time,
+    // each a read of HBM that another launch wrote (about 4 us apiece
+    // measured) when read in place
+    if (job.lv_off) {
+        uint8_t* lv = lds_states + job.lv_off;
+        for (uint32_t b = threadIdx.x; b < job.nleaf * n; b += blockDim.x) lv[b] = B.leaves[b];
+        B.leaves = lv;
+    }
     // The planned merges whose sources the host's dry run resolved (node
     // values of the sk, planned leaves, earlier merges) are hashed here level
     // by level, one lane per merge, into LDS after the states; the state
