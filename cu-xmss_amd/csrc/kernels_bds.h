@@ -91,12 +91,16 @@ struct BdsDev {
         return v;
     }
     __device__ void wr(size_t off, int bytes, uint32_t v) {
+        wr_nosync(off, bytes, v);
+        __syncthreads();
+    }
+    // the same write without the barrier: a run of them ends in one
+    __device__ void wr_nosync(size_t off, int bytes, uint32_t v) {
         if (tid == 0)
             for (int b = bytes - 1; b >= 0; --b) {
                 sk[off + b] = (uint8_t)v;
                 v >>= 8;
             }
-        __syncthreads();
     }
     // field accessors
     __device__ uint32_t stackoffset(uint32_t i) const { return rd(o_stackoffset(i), 4); }
@@ -197,25 +201,35 @@ struct BdsDev {
             const uint32_t use = th_usage(s, j);
             const uint32_t so = stackoffset(s);
             if (!(use > 0 && level(s, so - 1) == nodeheight)) break;
-            copy(nb + n, nb);
-            copy(nb, sk + o_stack(s) + (size_t)(so - 1) * n);
-            nd[5] = nodeheight;
-            nd[6] = next >> (nodeheight + 1);
-            hash_h(nb, nb, nd);
+            if (mval && mcur < nmerge) {  // a planned merge: its value is known, its inputs unused
+                if (tid < n) nb[tid] = mval[(size_t)mcur * n + tid];
+                ++mcur;
+            } else {  // (stack top, node) into the 2n bytes at nb, each lane its own byte
+                if (tid < n) {
+                    nb[n + tid] = nb[tid];
+                    nb[tid] = sk[o_stack(s) + (size_t)(so - 1) * n + tid];
+                }
+                __syncthreads();
+                nd[5] = nodeheight;
+                nd[6] = next >> (nodeheight + 1);
+                hash_h(nb, nb, nd);
+            }
             ++nodeheight;
-            wr(o_th(s, j) + 5, 1, use - 1);
-            set_stackoffset(s, so - 1);
+            wr_nosync(o_th(s, j) + 5, 1, use - 1);
+            wr_nosync(o_stackoffset(s), 4, so - 1);
+            __syncthreads();
         }
         if (nodeheight == th_h(s, j)) {
             copy(sk + th_node(s, j), nb);
             wr(o_th(s, j) + 6, 1, 1);
         } else {
-            const uint32_t so = stackoffset(s);
+            const uint32_t so = stackoffset(s), use = th_usage(s, j);
             copy(sk + o_stack(s) + (size_t)so * n, nb);
-            wr(o_th(s, j) + 5, 1, th_usage(s, j) + 1);
-            set_level(s, so, nodeheight);
-            set_stackoffset(s, so + 1);
-            wr(o_th(s, j) + 1, 4, next + 1);
+            wr_nosync(o_th(s, j) + 5, 1, use + 1);
+            wr_nosync(o_stacklevels(s) + so, 1, nodeheight);
+            wr_nosync(o_stackoffset(s), 4, so + 1);
+            wr_nosync(o_th(s, j) + 1, 4, next + 1);
+            __syncthreads();
         }
     }
 
