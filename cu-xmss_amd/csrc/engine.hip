@@ -228,14 +228,15 @@ constexpr uint64_t kLatLeaves = 16384;
 constexpr size_t kBdsLds = 64 * 1024;
 // BDS signatures: messages up to this long are hashed before the leaves.
 constexpr uint64_t kBdsShortMsg = 512;
-// Polls (about 2 us apart) the merge-DAG block of a one-call BDS signature's
+// Polls (about 0.16 us apart: about 0.65 ms in all, several times the
+// longest leaf) the merge-DAG block of a one-call BDS signature's
 // finishing launch makes for the planned leaves before it leaves the DAG to
 // the update launch (BdsFinDag); XMSS_MI355X_DAG_SPIN overrides it, and 0
 // keeps the DAG in the update.
 static uint32_t dag_spin() {
     static const uint32_t v = [] {
         const char* e = getenv("XMSS_MI355X_DAG_SPIN");
-        return e ? (uint32_t)strtoul(e, nullptr, 10) : (uint32_t)(1u << 16);
+        return e ? (uint32_t)strtoul(e, nullptr, 10) : (uint32_t)(1u << 12);
     }();
     return v;
 }
@@ -2230,7 +2231,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
                  o_wd = o_blob + al(total), o_jd = o_wd + al(sizeof(DLeaf) * (count + up.size())),
                  o_mg = o_jd + al(sizeof(DBds) * count), o_dag = o_mg + al(sizeof(DThash) * mg.size()),
                  o_lit = o_dag + al(sizeof(uint32_t) * dag.size()), o_sync = o_lit + al(lits.size()),
-                 o_sk = o_sync + al(2 * sizeof(uint32_t)),
+                 o_sk = o_sync + al(4 * sizeof(uint32_t)),
                  o_R = o_sk + al(sk_total), o_W = o_R + al((size_t)count * n),
                  o_T = o_W + al((size_t)count * len * n), io_bytes = o_T + al(tail_total);
     uint8_t* h = B.staging.need(io_bytes);
@@ -2242,7 +2243,8 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     memcpy(h + o_mg, mg.data(), sizeof(DThash) * mg.size());
     memcpy(h + o_dag, dag.data(), sizeof(uint32_t) * dag.size());
     memcpy(h + o_lit, lits.data(), lits.size());
-    memset(h + o_sync, 0, 2 * sizeof(uint32_t));  // BdsFinDag: leaves counted in, DAG hashed
+    // BdsFinDag: leaves counted in, DAG (and state) done, values in, signature picked
+    memset(h + o_sync, 0, 4 * sizeof(uint32_t));
     const size_t nl = ld.size(), nw = count + up.size();
     if (nl + nw + (mg.size() + count) / WG + 2 > 65535 || !use_lat(o, nl + nw, 0))
         fatal("bds_sign_chunk: batch too large for one launch sequence");
@@ -2304,9 +2306,13 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     // one-call signature with a merge DAG: the DAG is hashed in the
     // finishing launch as soon as the planned leaves are in, beside H_msg
     // and the WOTS signature, instead of at the start of the update
+    // (and, when its LDS fits the finishing launch's, the state update after
+    // it, so that the update launch only picks a long message's signature)
     BdsFinDag fd{};
-    if (count == 1 && jobs[0].sig_values && jd[0].maxlev && jobs[0].nplan && dag_spin()) {
+    const bool fin_state = lds <= kFinStateLds && !jobs[0].dsk;
+    if (count == 1 && jobs[0].sig_values && (jd[0].maxlev || fin_state) && dag_spin()) {
         fd.sync = reinterpret_cast<uint32_t*>(dv + o_sync);
+        fd.on = 1;
         fd.wait = jobs[0].nplan;
         fd.spin = dag_spin();
         fd.nmerge = jd[0].nmerge;
@@ -2314,15 +2320,41 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
         fd.q0 = ddag;  // job 0's merges and literals start at offset 0
         fd.lits = dlit;
         fd.pre = merge_arena;
-        fd.vals = B.dagv.need((size_t)jd[0].nmerge * n);
+        fd.vals = jd[0].maxlev ? B.dagv.need((size_t)jd[0].nmerge * n) : nullptr;
+        fd.state = fin_state;
+        fd.jobs = djobs;
+        fd.sk_arena = ska;
+        fd.tails = dT;
+        fd.wpre = lat + (nl + count) * o->lat_bytes(0);
+        fd.wstride = o->lat_bytes(0);
+        fd.select = fin_state && vals;
     }
     o->bds_lat(I->L, dk, dmd, (uint32_t)count, pad, db, dR, dig, dld, (uint32_t)nl, dw, (uint32_t)nw, lat, dmg,
                (uint32_t)mg.size(), merge_arena, leaf_arena, dW, long_msgs, ends, vals, fd);
-    o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, ddag, dlit,
-                  lat + (nl + count) * o->lat_bytes(0), dT, B.scratch.need((size_t)count * 4 * n),
-                  long_msgs ? dw : nullptr, dig, lat + nl * o->lat_bytes(0), dW, vals, fd.sync, fd.vals);
-    I->d2h(h + o_sk, ska, io_bytes - o_sk);
-    I->sync();
+    auto update = [&] {
+        o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, ddag, dlit,
+                      lat + (nl + count) * o->lat_bytes(0), dT, B.scratch.need((size_t)count * 4 * n),
+                      long_msgs ? dw : nullptr, dig, lat + nl * o->lat_bytes(0), dW, vals, fd.sync, fd.vals);
+    };
+    // (fd.sync, null when the finishing launch has no merge-DAG block, tells
+    // the update whether job 0's merges or whole state update were done there)
+    if (fd.on && fd.state) {
+        // the finishing launch is to do all of it: the update launch only
+        // if one of its bounded waits gave up (the results then come back
+        // a second time)
+        I->d2h(h + o_sync, dv + o_sync, io_bytes - o_sync);
+        I->sync();
+        const uint32_t* st = reinterpret_cast<const uint32_t*>(h + o_sync);
+        if (st[1] != 2 || (fd.select && st[3] != 1)) {
+            update();
+            I->d2h(h + o_sk, ska, io_bytes - o_sk);
+            I->sync();
+        }
+    } else {
+        update();
+        I->d2h(h + o_sk, ska, io_bytes - o_sk);
+        I->sync();
+    }
     for (int i = 0; i < count; ++i) {
         memcpy(jobs[i].sk, h + o_sk + jd[i].sk_off, jobs[i].shape.sk_bytes);
         memcpy(jobs[i].R_out, h + o_R + (size_t)i * n, n);

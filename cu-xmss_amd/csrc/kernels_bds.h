@@ -430,16 +430,23 @@ template <class F>
 struct BdsFinPair {
     static constexpr bool value = F::kPairHmsg && ((F::LEN + 63) / 64) * 64 == 128;
 };
+// LDS of k_bds_fin's merge-DAG block when it also runs the state update
+// (BdsFinDag::state): the update's states, merge values and leaves.
+constexpr uint32_t kFinStateLds = 32768;
 template <class F, bool PAIR = BdsFinPair<F>::value>
 struct BdsFinLds {
     union {
         typename F::Node pk[F::LEN];
         HmsgPairLds pair;
+        alignas(16) uint8_t raw[kFinStateLds];
     };
 };
 template <class F>
 struct BdsFinLds<F, false> {
-    typename F::Node pk[F::LEN];
+    union {
+        typename F::Node pk[F::LEN];
+        alignas(16) uint8_t raw[kFinStateLds];
+    };
 };
 
 // The planned merges of one signature's state update hashed level by level,
@@ -481,20 +488,69 @@ __device__ void merge_dag(const uint32_t* q0, uint32_t nmerge, uint32_t maxlev, 
     }
 }
 
+template <class F>
+__device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint8_t* skg,
+                               const uint8_t* __restrict__ leaf_arena,
+                               const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
+                               const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
+                               uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena,
+                               const uint8_t* premerged, uint8_t* lds_states);
+
 // The merge DAG of a one-signature launch hashed in k_bds_fin, beside H_msg
 // and the WOTS signature, instead of in the update launch: its last block
 // waits until the blocks of the job's `wait` planned leaves have each
 // counted in at sync[0] (a bounded wait: after `spin` polls it gives up and
 // the update hashes the DAG as usual), hashes the DAG into vals and sets
-// sync[1].  vals == nullptr: off.
+// sync[1] = 1.  on == 0: off.
 struct BdsFinDag {
     uint32_t* sync;
-    uint32_t wait, spin, nmerge, maxlev;
+    uint32_t on, wait, spin, nmerge, maxlev;
     const uint32_t* q0;
     const uint8_t* lits;
     const void* pre;
     uint8_t* vals;
+    // state: the block then runs job 0's whole state update (bds_update_one)
+    // in its LDS and sets sync[1] = 2, and the update launch skips the job
+    uint32_t state;
+    // select (a long message's chain values walked beside H_msg): the
+    // values block counts in at sync[2], and the H_msg block, once its
+    // digest is out and the values are in (the same bounded wait), picks the
+    // WOTS signature and sets sync[3] = 1; with state and select both done
+    // nothing is left for the update launch
+    uint32_t select;
+    const DBds* jobs;
+    uint8_t* sk_arena;
+    uint8_t* tails;
+    const uint8_t* wpre;
+    size_t wstride;
 };
+
+// The waits of k_bds_fin's blocks on blocks dispatched before them (so that
+// the wait always ends, bounded anyway): whether `want` blocks had counted in
+// at *count within `spin` polls.  Every thread of the workgroup calls it; a
+// true return makes the counted blocks' writes visible to all of them.
+__device__ __forceinline__ bool fin_wait(const uint32_t* count, uint32_t want, uint32_t spin) {
+    __shared__ uint32_t ready;
+    if (threadIdx.x == 0) {
+        uint32_t polls = 0;
+        while (__hip_atomic_load(count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want && polls < spin) {
+            __builtin_amdgcn_s_sleep(2);
+            ++polls;
+        }
+        ready = __hip_atomic_load(count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= want;
+    }
+    __syncthreads();
+    return ready;
+}
+// ... and the counting in, once the whole workgroup's writes are done (when on)
+__device__ __forceinline__ void fin_count_in(bool on, uint32_t* count) {
+    if (!on) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(count, 1u);
+    }
+}
 
 // Blocks: nleaf leaves (or, for rows marked LAT_ENDS_OUT / LAT_ENDS_IN, the
 // chains into slot p2 of `ends` / the L-tree from it), then nsig WOTS
@@ -512,24 +568,27 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
                                                   uint8_t* __restrict__ ends, uint32_t nvals,
                                                   uint8_t* __restrict__ vals, BdsFinDag dag) {
     __shared__ BdsFinLds<F> sh;
-    if (dag.vals && blockIdx.x == gridDim.x - 1) {  // the merge DAG, once the planned leaves are in
-        __shared__ uint32_t ready;
-        if (threadIdx.x == 0) {
-            uint32_t polls = 0;
-            while (__hip_atomic_load(dag.sync, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < dag.wait &&
-                   polls < dag.spin) {
-                __builtin_amdgcn_s_sleep(2);
-                ++polls;
+    if (dag.on && blockIdx.x == gridDim.x - 1) {  // the merge DAG, once the planned leaves are in
+        if (!fin_wait(dag.sync, dag.wait, dag.spin)) return;  // the update launch does it all
+        const typename F::HPre* pre = reinterpret_cast<const typename F::HPre*>(dag.pre);
+        const uint8_t* merged = nullptr;
+        if (dag.maxlev) {
+            merge_dag<F>(dag.q0, dag.nmerge, dag.maxlev, dag.lits, leaf_out, pre, dag.vals);
+            merged = dag.vals;
+            if (threadIdx.x == 0) {
+                __threadfence();
+                dag.sync[1] = 1;
             }
-            ready = __hip_atomic_load(dag.sync, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= dag.wait;
         }
-        __syncthreads();
-        if (!ready) return;  // the update hashes the DAG
-        merge_dag<F>(dag.q0, dag.nmerge, dag.maxlev, dag.lits, leaf_out,
-                     reinterpret_cast<const typename F::HPre*>(dag.pre), dag.vals);
-        if (threadIdx.x == 0) {
-            __threadfence();
-            dag.sync[1] = 1;
+        if (dag.state) {  // and the state update, while H_msg or the WOTS signature still runs
+            const DBds job = dag.jobs[0];
+            bds_update_one<F>(keys[job.key], job, dag.sk_arena + job.sk_off, leaf_out, pre, dag.q0, dag.lits,
+                              dag.wpre, dag.wstride, dag.tails, nullptr, merged, sh.raw);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                __threadfence();
+                dag.sync[1] = 2;
+            }
         }
         return;
     }
@@ -539,13 +598,7 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
         if (q.p1 & LAT_ENDS_OUT) lat_leaf_chains<F>(blockIdx.x, scratch, stride, e);
         else if (q.p1 & LAT_ENDS_IN) lat_leaf_ltree<F>(blockIdx.x, sh.pk, scratch, stride, e, leaf_out);
         else lat_fin_one<F, LAT_LEAF>(blockIdx.x, sh.pk, leaves, nullptr, nullptr, nullptr, scratch, stride, leaf_out);
-        if (dag.vals && blockIdx.x < dag.wait) {  // a planned leaf of the DAG's job is in
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                __threadfence();
-                atomicAdd(dag.sync, 1u);
-            }
-        }
+        fin_count_in(dag.on && blockIdx.x < dag.wait, dag.sync);  // a planned leaf of the DAG block's job
     } else if (blockIdx.x < nleaf + nsig) {
         lat_fin_one<F, LAT_SIGN>(blockIdx.x - nleaf, sh.pk, sigs, digests, nullptr, nullptr,
                                  scratch + (size_t)nleaf * stride, stride, sig_out);
@@ -554,12 +607,26 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
         // beside H_msg (the update launch then picks the signature)
         const uint32_t j = blockIdx.x - nleaf - nsig;
         lat_sig_values<F>(j, scratch + (size_t)nleaf * stride, stride, vals + (size_t)j * F::LEN * 16 * F::kN);
-    } else if constexpr (BdsFinPair<F>::value) {
-        hmsg_pair_block<F>(keys, msgs, (blockIdx.x - nleaf - nsig - nvals) * 64, nmsg, pad, blob, R_out, digests,
-                           sh.pair);
+        fin_count_in(dag.select, dag.sync + 2);
     } else {
-        hmsg_mirrored<F>(keys, msgs, (blockIdx.x - nleaf - nsig - nvals) * blockDim.x + threadIdx.x, nmsg, pad, blob,
-                         R_out, digests);
+        if constexpr (BdsFinPair<F>::value)
+            hmsg_pair_block<F>(keys, msgs, (blockIdx.x - nleaf - nsig - nvals) * 64, nmsg, pad, blob, R_out,
+                               digests, sh.pair);
+        else
+            hmsg_mirrored<F>(keys, msgs, (blockIdx.x - nleaf - nsig - nvals) * blockDim.x + threadIdx.x, nmsg,
+                             pad, blob, R_out, digests);
+        // one message (select): its WOTS signature from the values, which
+        // blocks dispatched before this one walk
+        if (dag.select) {
+            __syncthreads();  // the digest is out
+            if (!fin_wait(dag.sync + 2, nvals, dag.spin)) return;  // the update launch picks it
+            lat_sig_select<F>(digests, vals, sig_out);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                __threadfence();
+                dag.sync[3] = 1;
+            }
+        }
     }
 }
 
@@ -575,8 +642,7 @@ __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint
                                const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
                                const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
                                uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena,
-                               const uint8_t* premerged) {
-    extern __shared__ __align__(16) uint8_t lds_states[];
+                               const uint8_t* premerged, uint8_t* lds_states) {
     const BdsP& P = job.P;
     BdsDev<F> B(lds_states, skg, P, leaf_arena + job.leaf_off * F::kN, merge_arena + job.merge_off, job.nmerge,
                 key);
@@ -715,6 +781,8 @@ __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __res
             lat_fin_one<F, LAT_SIGN>(j, nullptr, sigs, digests, nullptr, nullptr, sign_scratch, wstride, sig_out);
         return;
     }
+    if (blockIdx.x == 0 && dag_sync && dag_sync[1] == 2) return;  // done in k_bds_fin (BdsFinDag::state)
+    extern __shared__ __align__(16) uint8_t lds_dyn[];
     const DBds job = jobs[blockIdx.x];
     uint8_t* skg = sk_arena + job.sk_off;
     uint8_t* res = reinterpret_cast<uint8_t*>(job.res);
@@ -730,7 +798,7 @@ __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __res
     // job 0's merges, if k_bds_fin hashed them (dag_sync[1] set)
     const uint8_t* premerged = dag_vals && blockIdx.x == 0 && dag_sync[1] == 1 ? dag_vals : nullptr;
     bds_update_one<F>(keys[job.key], job, skg, leaf_arena, merge_arena, dag, lit_arena, wpre, wstride, tails,
-                      scratch_arena, premerged);
+                      scratch_arena, premerged, lds_dyn);
     if (res) {
         __syncthreads();
         for (uint32_t b = threadIdx.x; b < job.sk_bytes; b += blockDim.x) res[b] = skg[b];

@@ -410,7 +410,7 @@ struct FamOpsT final : FamOps {
         const uint32_t per = BdsFinPair<F>::value ? 64 : threads;
         // the merge-DAG block, if any, is the last one: it waits for the
         // planned leaves' blocks, which are dispatched before it
-        L.run("bds_fin", k_bds_fin<F>, dim3(nleaf + nsig + nvals + cdiv(nlate, per) + (dag.vals ? 1 : 0)),
+        L.run("bds_fin", k_bds_fin<F>, dim3(nleaf + nsig + nvals + cdiv(nlate, per) + (dag.on ? 1 : 0)),
               dim3(threads), (const K*)keys, msgs, nlate, pad, blob, R, digests, leaves, nleaf, sigs, nsig,
               (const uint8_t*)scratch, stride, leaf_out, sig_out, ends, nvals, vals, dag);
     }
