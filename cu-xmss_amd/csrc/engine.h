@@ -449,9 +449,10 @@ public:
         // the pre-image launch; null: whole chains in the update
         const uint32_t* plan_wots;
         uint32_t nwots;
-        // one-call signature: with a long message, its layer-0 WOTS chains are
-        // walked to every value beside H_msg (memory the serving loop's
-        // budget does not count, so its jobs leave this false)
+        // a signature launched alone (the one-call path, a serving group of
+        // one): with a long message its layer-0 WOTS chains are walked to
+        // every value beside H_msg, and the merge DAG and the state update
+        // run in the finishing launch (Engine::bds_bytes counts the memory)
         bool sig_values;
         uint8_t* R_out;        // n
         uint8_t* wots0_out;    // wots_sig_bytes

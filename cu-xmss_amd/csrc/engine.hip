@@ -2297,11 +2297,10 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     uint64_t mmax = 0;
     for (int i = 0; i < count; ++i) mmax = std::max<uint64_t>(mmax, jobs[i].mlen);
     const bool long_msgs = mmax > kBdsShortMsg;
-    // one signature of a long message (the one-call path): its WOTS chains
-    // are walked to every value beside H_msg, and the update launch picks
-    // the signature once the digest is there (the chains leave the serial
-    // path behind H_msg; a group of the serving loop keeps the plain walk,
-    // whose memory its budget counts)
+    // one signature of a long message launched alone (sig_values): its WOTS
+    // chains are walked to every value beside H_msg, and the signature is
+    // picked once the digest is there (the chains leave the serial path
+    // behind H_msg; a group of several keeps the plain walk)
     uint8_t* vals = long_msgs && count == 1 && jobs[0].sig_values ? B.vals.need((size_t)len * 16 * n) : nullptr;
     // one-call signature with a merge DAG: the DAG is hashed in the
     // finishing launch as soon as the planned leaves are in, beside H_msg
@@ -2309,7 +2308,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     // (and, when its LDS fits the finishing launch's, the state update after
     // it, so that the update launch only picks a long message's signature)
     BdsFinDag fd{};
-    const bool fin_state = lds <= kFinStateLds && !jobs[0].dsk;
+    const bool fin_state = lds <= kFinStateLds;
     if (count == 1 && jobs[0].sig_values && (jd[0].maxlev || fin_state) && dag_spin()) {
         fd.sync = reinterpret_cast<uint32_t*>(dv + o_sync);
         fd.on = 1;
@@ -2657,7 +2656,10 @@ size_t Engine::bds_bytes(int fam, const BdsJob& j) {
                       sizeof(DLeaf) * nw + sizeof(DBds) + sizeof(DThash) * j.nmerge + 12 * (size_t)j.nmerge +
                       n * j.nlit + ((s.sk_bytes + 7) & ~7ull) + n + len * n +
                       ((size_t)s.sig_bytes - s.index_bytes - n - s.wots_sig_bytes);
-    return io + j.nplan * n + (size_t)j.nmerge * o->hpre_bytes() + (j.nplan + nw) * o->lat_bytes(0) + 5 * n +
+    // (a group of one signature also walks its WOTS chains to every value
+    // and keeps the merge values: bds_sign_chunk's vals and dagv)
+    const size_t one = (size_t)len * 16 * n + (size_t)j.nmerge * n;
+    return io + j.nplan * n + (size_t)j.nmerge * o->hpre_bytes() + (j.nplan + nw) * o->lat_bytes(0) + 5 * n + one +
            kGroupAllowance;
 }
 

@@ -44,6 +44,57 @@ struct DBds {
     uint32_t lv_off;      // LDS offset their values are staged at, or 0 (read from HBM)
 };
 
+// dst[0, len) = src[0, len), by every thread of the workgroup (no barrier),
+// with several loads in flight per thread: a strided loop of one-byte copies
+// waits out a whole memory latency per iteration (the compiler issues the
+// next load only after the store), which over the kilobytes of an XMSS^MT sk
+// came to tens of microseconds per signature.  Word copies when the two
+// pointers share their alignment mod 4, the rest byte by byte; the ranges
+// must not overlap.
+__device__ __forceinline__ void wg_copy(uint8_t* dst, const uint8_t* src, uint32_t len) {
+    constexpr uint32_t U = 8;
+    const uint32_t t = threadIdx.x, T = blockDim.x;
+    uint32_t done = 0;
+    if ((((uintptr_t)dst ^ (uintptr_t)src) & 3) == 0) {
+        uint32_t head = (uint32_t)(0u - (uint32_t)(uintptr_t)src) & 3u;
+        if (head > len) head = len;
+        if (t < head) dst[t] = src[t];
+        const uint32_t nw = (len - head) >> 2;
+        uint32_t* d = reinterpret_cast<uint32_t*>(dst + head);
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(src + head);
+        for (uint32_t base = 0; base < nw; base += U * T) {
+            uint32_t v[U];
+#pragma unroll
+            for (uint32_t k = 0; k < U; ++k) {
+                const uint32_t i = base + k * T + t;
+                v[k] = w[i < nw ? i : nw - 1];
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < U; ++k) {
+                const uint32_t i = base + k * T + t;
+                if (i < nw) d[i] = v[k];
+            }
+        }
+        done = head + 4 * nw;
+    }
+    const uint32_t rest = len - done;
+    const uint8_t* sb = src + done;
+    uint8_t* db = dst + done;
+    for (uint32_t base = 0; base < rest; base += 4 * U * T) {
+        uint8_t v[4 * U];
+#pragma unroll
+        for (uint32_t k = 0; k < 4 * U; ++k) {
+            const uint32_t i = base + k * T + t;
+            v[k] = sb[i < rest ? i : rest - 1];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4 * U; ++k) {
+            const uint32_t i = base + k * T + t;
+            if (i < rest) db[i] = v[k];
+        }
+    }
+}
+
 // Sources of a planned merge's halves (kind in the top 4 bits, index below).
 enum : uint32_t { DAG_LIT = 0u << 28, DAG_LEAF = 1u << 28, DAG_MERGE = 2u << 28, DAG_INDEX = (1u << 28) - 1 };
 
@@ -191,16 +242,20 @@ struct BdsDev {
     }
 
     // treehash_update (xmss_core_fast.c:406-452) on instance j of state s;
-    // nb is a 2n-byte scratch (LDS)
+    // nb is a 2n-byte scratch (LDS).  The instance's stack usage and the
+    // state's stack offset live in registers while it runs (every thread
+    // keeps the same copies), and thread 0 writes them through to the state:
+    // a field that thread 0 rewrites is never read back by the other waves in
+    // the same step, which a planned merge (no barrier before the writes)
+    // would otherwise race with.
     __device__ void treehash_update(uint32_t s, uint32_t j, const uint32_t (&addr)[8], uint8_t* nb) {
         uint32_t nd[8] = {addr[0], addr[1], addr[2], 2, 0, 0, 0, 0};
         const uint32_t next = th_next(s, j);
+        uint32_t use = th_usage(s, j), so = stackoffset(s);
         uint32_t nodeheight = 0;
         leaf(nb);
-        for (;;) {
-            const uint32_t use = th_usage(s, j);
-            const uint32_t so = stackoffset(s);
-            if (!(use > 0 && level(s, so - 1) == nodeheight)) break;
+        // (the stack levels below the offset are not written in the loop)
+        while (use > 0 && level(s, so - 1) == nodeheight) {
             if (mval && mcur < nmerge) {  // a planned merge: its value is known, its inputs unused
                 if (tid < n) nb[tid] = mval[(size_t)mcur * n + tid];
                 ++mcur;
@@ -215,15 +270,16 @@ struct BdsDev {
                 hash_h(nb, nb, nd);
             }
             ++nodeheight;
-            wr_nosync(o_th(s, j) + 5, 1, use - 1);
-            wr_nosync(o_stackoffset(s), 4, so - 1);
+            --use;
+            --so;
+            wr_nosync(o_th(s, j) + 5, 1, use);
+            wr_nosync(o_stackoffset(s), 4, so);
             __syncthreads();
         }
         if (nodeheight == th_h(s, j)) {
             copy(sk + th_node(s, j), nb);
             wr(o_th(s, j) + 6, 1, 1);
         } else {
-            const uint32_t so = stackoffset(s), use = th_usage(s, j);
             copy(sk + o_stack(s) + (size_t)so * n, nb);
             wr_nosync(o_th(s, j) + 5, 1, use + 1);
             wr_nosync(o_stacklevels(s) + so, 1, nodeheight);
@@ -489,7 +545,7 @@ __device__ void merge_dag(const uint32_t* q0, uint32_t nmerge, uint32_t maxlev, 
 }
 
 template <class F>
-__device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint8_t* skg,
+__device__ __forceinline__ void bds_update_job(const typename F::Key& key, const DBds& job, uint8_t* skg,
                                const uint8_t* __restrict__ leaf_arena,
                                const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
                                const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
@@ -509,7 +565,7 @@ struct BdsFinDag {
     const uint8_t* lits;
     const void* pre;
     uint8_t* vals;
-    // state: the block then runs job 0's whole state update (bds_update_one)
+    // state: the block then runs job 0's whole state update (bds_update_job)
     // in its LDS and sets sync[1] = 2, and the update launch skips the job
     uint32_t state;
     // select (a long message's chain values walked beside H_msg): the
@@ -582,7 +638,7 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
         }
         if (dag.state) {  // and the state update, while H_msg or the WOTS signature still runs
             const DBds job = dag.jobs[0];
-            bds_update_one<F>(keys[job.key], job, dag.sk_arena + job.sk_off, leaf_out, pre, dag.q0, dag.lits,
+            bds_update_job<F>(keys[job.key], job, dag.sk_arena + job.sk_off, leaf_out, pre, dag.q0, dag.lits,
                               dag.wpre, dag.wstride, dag.tails, nullptr, merged, sh.raw);
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -637,7 +693,7 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
 // (scratch_arena is no longer read).
 
 template <class F>
-__device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint8_t* skg,
+__device__ __forceinline__ void bds_update_one(const typename F::Key& key, const DBds& job, uint8_t* skg,
                                const uint8_t* __restrict__ leaf_arena,
                                const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
                                const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
@@ -660,15 +716,16 @@ __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint
     const uint64_t idx = job.idx;
     const uint32_t region = (2 * d - 1) * P.state_bytes;
     // stage the serialized states in LDS; written back when the update is done
-    for (uint32_t b = threadIdx.x; b < region; b += blockDim.x) lds_states[b] = skg[B.states_base() + b];
+    wg_copy(lds_states, skg + B.states_base(), region);
     // and the planned leaves: the state machine takes them one at a time,
     // each a read of HBM that another launch wrote (about 4 us apiece
     // measured) when read in place
     if (job.lv_off) {
         uint8_t* lv = lds_states + job.lv_off;
-        for (uint32_t b = threadIdx.x; b < job.nleaf * n; b += blockDim.x) lv[b] = B.leaves[b];
+        wg_copy(lv, B.leaves, job.nleaf * n);
         B.leaves = lv;
     }
+    __syncthreads();  // the merges below read the staged leaves of every lane
     // The planned merges whose sources the host's dry run resolved (node
     // values of the sk, planned leaves, earlier merges) are hashed here level
     // by level, one lane per merge, into LDS after the states; the state
@@ -678,7 +735,7 @@ __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint
     if (job.maxlev) {
         uint8_t* mv = lds_states + ((region + 15) & ~15u);
         if (premerged) {  // hashed beside H_msg (BdsFinDag)
-            for (uint32_t b = threadIdx.x; b < job.nmerge * n; b += blockDim.x) mv[b] = premerged[b];
+            wg_copy(mv, premerged, job.nmerge * n);
         } else {
             merge_dag<F>(dag + 3 * job.merge_off, job.nmerge, job.maxlev, lit_arena + job.lit_off * n, B.leaves,
                          merge_arena + job.merge_off, mv);
@@ -692,7 +749,7 @@ __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint
         uint32_t len;
         __device__ ~WriteBack() {
             __syncthreads();
-            for (uint32_t b = threadIdx.x; b < len; b += blockDim.x) g[b] = l[b];
+            wg_copy(g, l, len);
         }
     } wb{lds_states, skg + B.states_base(), region};
     // signature tail: auth of layer 0, then (WOTS signature, auth) of layers 1..d-1
@@ -701,11 +758,11 @@ __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint
         for (uint32_t i = 0; i < d; ++i) {
             if (i > 0) {
                 const uint8_t* w = B.wots_sigs() + (size_t)(i - 1) * P.wots_sig_bytes;
-                for (uint32_t b = threadIdx.x; b < P.wots_sig_bytes; b += blockDim.x) tail[o + b] = w[b];
+                wg_copy(tail + o, w, P.wots_sig_bytes);
                 o += P.wots_sig_bytes;
             }
             const uint8_t* a = B.sk + B.o_auth(i);
-            for (uint32_t b = threadIdx.x; b < th * n; b += blockDim.x) tail[o + b] = a[b];
+            wg_copy(tail + o, a, th * n);
             o += (size_t)th * n;
         }
         __syncthreads();
@@ -758,11 +815,38 @@ __device__ void bds_update_one(const typename F::Key& key, const DBds& job, uint
     }
 }
 
-// One signature's state update per workgroup.  The sk is staged in the sk
-// arena; for a key whose state is resident in HBM (job.res), the arena copy
-// is filled from it first, with the index advanced as the signature requires
-// (idx + 1, big-endian), and the updated bytes go back to it at the end, so
-// the arena holds the new sk for the host's copy either way.
+// One signature's state update (bds_update_one) with its sk at skg, in the
+// sk arena; for a key whose state is resident in HBM (job.res), the arena
+// copy is filled from it first, with the index advanced as the signature
+// requires (idx + 1, big-endian), and the updated bytes go back to it at the
+// end, so the arena holds the new sk for the host's copy either way.  Every
+// thread of the workgroup calls it.
+template <class F>
+__device__ __forceinline__ void bds_update_job(const typename F::Key& key, const DBds& job, uint8_t* skg,
+                               const uint8_t* __restrict__ leaf_arena,
+                               const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
+                               const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
+                               uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena,
+                               const uint8_t* premerged, uint8_t* lds_states) {
+    uint8_t* res = reinterpret_cast<uint8_t*>(job.res);
+    if (res) {
+        wg_copy(skg, res, job.sk_bytes);
+        __syncthreads();
+        if (threadIdx.x < job.P.index_bytes) {
+            const uint32_t sh = 8 * (job.P.index_bytes - 1 - threadIdx.x);
+            skg[threadIdx.x] = sh < 64 ? (uint8_t)((job.idx + 1) >> sh) : 0;
+        }
+        __syncthreads();
+    }
+    bds_update_one<F>(key, job, skg, leaf_arena, merge_arena, dag, lit_arena, wpre, wstride, tails, scratch_arena,
+                      premerged, lds_states);
+    if (res) {
+        __syncthreads();
+        wg_copy(res, skg, job.sk_bytes);
+    }
+}
+
+// One signature's state update per workgroup (bds_update_job).
 template <class F>
 __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __restrict__ keys, const DBds* __restrict__ jobs,
                              uint8_t* __restrict__ sk_arena, const uint8_t* __restrict__ leaf_arena,
@@ -784,25 +868,10 @@ __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __res
     if (blockIdx.x == 0 && dag_sync && dag_sync[1] == 2) return;  // done in k_bds_fin (BdsFinDag::state)
     extern __shared__ __align__(16) uint8_t lds_dyn[];
     const DBds job = jobs[blockIdx.x];
-    uint8_t* skg = sk_arena + job.sk_off;
-    uint8_t* res = reinterpret_cast<uint8_t*>(job.res);
-    if (res) {
-        for (uint32_t b = threadIdx.x; b < job.sk_bytes; b += blockDim.x) skg[b] = res[b];
-        __syncthreads();
-        if (threadIdx.x < job.P.index_bytes) {
-            const uint32_t sh = 8 * (job.P.index_bytes - 1 - threadIdx.x);
-            skg[threadIdx.x] = sh < 64 ? (uint8_t)((job.idx + 1) >> sh) : 0;
-        }
-        __syncthreads();
-    }
     // job 0's merges, if k_bds_fin hashed them (dag_sync[1] set)
     const uint8_t* premerged = dag_vals && blockIdx.x == 0 && dag_sync[1] == 1 ? dag_vals : nullptr;
-    bds_update_one<F>(keys[job.key], job, skg, leaf_arena, merge_arena, dag, lit_arena, wpre, wstride, tails,
-                      scratch_arena, premerged, lds_dyn);
-    if (res) {
-        __syncthreads();
-        for (uint32_t b = threadIdx.x; b < job.sk_bytes; b += blockDim.x) res[b] = skg[b];
-    }
+    bds_update_job<F>(keys[job.key], job, sk_arena + job.sk_off, leaf_arena, merge_arena, dag, lit_arena, wpre,
+                      wstride, tails, scratch_arena, premerged, lds_dyn);
 }
 
 }  // namespace xm

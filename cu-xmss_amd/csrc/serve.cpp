@@ -316,6 +316,9 @@ private:
                     // shapes of the family, key contexts from the key table
                     std::vector<Engine::BdsJob> jobs;
                     for (Call* c : cs) jobs.push_back(*c->job);
+                    // a signature alone takes the one-call path's schedule (its
+                    // reservation counts the memory that takes)
+                    if (jobs.size() == 1) jobs[0].sig_values = true;
                     e_->bds_sign_batch(fam, (int)jobs.size(), jobs.data());
                     break;
                 }
