@@ -2057,7 +2057,7 @@ namespace xm {
 
 
 struct BdsBufs {
-    DevBuf io, leaf_arena, dig, scratch, lat_leaf, merge_arena, vals, dagv;
+    DevBuf io, leaf_arena, dig, scratch, lat_leaf, merge_arena, vals, dagv, mid;
     PinnedBuf staging;
 };
 
@@ -2302,6 +2302,9 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     // picked once the digest is there (the chains leave the serial path
     // behind H_msg; a group of several keeps the plain walk)
     uint8_t* vals = long_msgs && count == 1 && jobs[0].sig_values ? B.vals.need((size_t)len * 16 * n) : nullptr;
+    // ... and its first H_msg blocks are hashed in the pre-image launch
+    // (HmsgMid: the SHA-256 families)
+    void* mid = vals ? B.mid.need(sizeof(HmsgMid)) : nullptr;
     // one-call signature with a merge DAG: the DAG is hashed in the
     // finishing launch as soon as the planned leaves are in, beside H_msg
     // and the WOTS signature, instead of at the start of the update
@@ -2329,7 +2332,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
         fd.select = fin_state && vals;
     }
     o->bds_lat(I->L, dk, dmd, (uint32_t)count, pad, db, dR, dig, dld, (uint32_t)nl, dw, (uint32_t)nw, lat, dmg,
-               (uint32_t)mg.size(), merge_arena, leaf_arena, dW, long_msgs, ends, vals, fd);
+               (uint32_t)mg.size(), merge_arena, leaf_arena, dW, long_msgs, ends, vals, fd, mid);
     auto update = [&] {
         o->bds_update(I->L, dk, lds, djobs, (uint32_t)count, ska, leaf_arena, merge_arena, ddag, dlit,
                       lat + (nl + count) * o->lat_bytes(0), dT, B.scratch.need((size_t)count * 4 * n),
@@ -2609,7 +2612,7 @@ void Engine::release_workspace() {
     for (DevBuf* x : bufs) x->release();
     if (I->bds) {
         DevBuf* b[] = {&I->bds->io, &I->bds->leaf_arena, &I->bds->dig, &I->bds->scratch, &I->bds->lat_leaf,
-                       &I->bds->merge_arena, &I->bds->vals, &I->bds->dagv};
+                       &I->bds->merge_arena, &I->bds->vals, &I->bds->dagv, &I->bds->mid};
         for (DevBuf* x : b) x->release();
     }
     I->ahead.clear();
@@ -2656,9 +2659,10 @@ size_t Engine::bds_bytes(int fam, const BdsJob& j) {
                       sizeof(DLeaf) * nw + sizeof(DBds) + sizeof(DThash) * j.nmerge + 12 * (size_t)j.nmerge +
                       n * j.nlit + ((s.sk_bytes + 7) & ~7ull) + n + len * n +
                       ((size_t)s.sig_bytes - s.index_bytes - n - s.wots_sig_bytes);
-    // (a group of one signature also walks its WOTS chains to every value
-    // and keeps the merge values: bds_sign_chunk's vals and dagv)
-    const size_t one = (size_t)len * 16 * n + (size_t)j.nmerge * n;
+    // (a group of one signature also walks its WOTS chains to every value,
+    // keeps the merge values and H_msg's chaining value: bds_sign_chunk's
+    // vals, dagv and mid)
+    const size_t one = (size_t)len * 16 * n + (size_t)j.nmerge * n + sizeof(HmsgMid);
     return io + j.nplan * n + (size_t)j.nmerge * o->hpre_bytes() + (j.nplan + nw) * o->lat_bytes(0) + 5 * n + one +
            kGroupAllowance;
 }

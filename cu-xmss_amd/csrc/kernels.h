@@ -396,16 +396,73 @@ __global__ __launch_bounds__(WG) void k_hmsg(const typename F::Key* __restrict__
 // compression (about 1400 VALU instructions) to its rounds (about 900).
 // Sign requests get R = PRF(SK_PRF, idx) first (wave 0, written into the
 // prefix as hmsg_one does).  Every thread of the workgroup must call it.
+// With mid set, each request resumes from the chaining value hmsg_head left
+// (its R is in the prefix already).
 struct HmsgPairLds {
     uint32_t wk[2][64][64];  // [buffer][round][message]: one word per lane, conflict-free
-    uint32_t nbmax;
+    uint32_t nbmax, b0min;
 };
+
+// The chaining value of an H_msg after its first nb blocks.  A launch
+// sequence of one signature of a long message hashes R and the first
+// kHmsgHeadBlocks blocks in its pre-image launch (hmsg_head, beside the
+// address-only work) and the finishing launch resumes from there, so that
+// this much of H_msg's serial chain is done before the finishing launch
+// starts.  The head is sized to end within the pre-image launch's own
+// length (one wave at a whole compression per block: 6 blocks made that
+// launch longer and 1.6 KB signatures 5 us slower).
+struct HmsgMid {
+    uint32_t st[8];
+    uint32_t nb;
+};
+constexpr uint32_t kHmsgHeadBlocks = 3;
+
+// R and the first `head` blocks of H_msg of request r (SHA-256 families), one
+// lane per request; only the owning lane writes (own: see hmsg_mirrored).
+// A request whose blocks all fit also gets its digest.
+template <class F>
+__device__ __forceinline__ void hmsg_head(const typename F::Key* __restrict__ keys, const DMsg* __restrict__ reqs,
+                                          uint32_t r, uint32_t pad, uint8_t* __restrict__ blob,
+                                          uint8_t* __restrict__ R_out, uint8_t* __restrict__ dig_out,
+                                          HmsgMid* __restrict__ mid, uint32_t head, bool own) {
+    const DMsg q = reqs[r];
+    uint8_t* p = blob + q.off;
+    if (q.sign) {
+        typename F::Node R;
+        F::prf_idx(R, keys[q.key], q.idx);
+        if (own) {
+            F::store(R, p + pad);
+            F::store(R, R_out + (size_t)r * F::kN);
+        }
+    }
+    uint32_t st[8];
+    sha256_iv(st);
+    const uint32_t nb = (q.len + 9 + 63) / 64, b1 = head < nb ? head : nb;
+    for (uint32_t b = 0; b < b1; ++b) {
+        uint4 raw[4];
+        sha256_block_load(raw, p, b);
+        uint32_t w[16];
+        sha256_block_words(w, raw, q.len, b);
+        sha256_compress(st, w);
+    }
+    if (!own) return;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) mid[r].st[i] = st[i];
+    mid[r].nb = b1;
+    if (b1 == nb) {
+        typename F::Node d;
+#pragma unroll
+        for (int i = 0; i < F::NW; ++i) d.w[i] = st[i];
+        F::store(d, dig_out + (size_t)r * F::kN);
+    }
+}
 
 template <class F>
 __device__ __forceinline__ void hmsg_pair_block(const typename F::Key* __restrict__ keys,
                                                 const DMsg* __restrict__ reqs, uint32_t first, uint32_t n,
                                                 uint32_t pad, uint8_t* blob, uint8_t* __restrict__ R_out,
-                                                uint8_t* __restrict__ dig_out, HmsgPairLds& sh) {
+                                                uint8_t* __restrict__ dig_out, HmsgPairLds& sh,
+                                                const HmsgMid* mid = nullptr) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // Lanes past the last request hash a copy of request lane % count: with
     // few messages (a single BDS signature's) the waves run at the speed they
@@ -414,25 +471,37 @@ __device__ __forceinline__ void hmsg_pair_block(const typename F::Key* __restric
     const uint32_t count = first < n ? (n - first < 64 ? n - first : 64) : 0;
     const bool own = lane < count;
     const uint32_t r = count ? first + lane % count : n;
-    uint32_t len = 0, nb = 0;
+    uint32_t len = 0, nb = 0, b0 = 0;
     uint8_t* p = nullptr;
+    uint32_t st[8];
+    sha256_iv(st);
     if (r < n) {
         const DMsg q = reqs[r];
         p = blob + q.off;
         len = q.len;
         nb = (len + 9 + 63) / 64;
-        if (q.sign && wave == 0 && own) {  // R, then the prefix holds it (xmss_core.c:242-248)
+        if (mid) {  // resumed after hmsg_head
+            b0 = mid[r].nb;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) st[i] = mid[r].st[i];
+        } else if (q.sign && wave == 0 && own) {  // R, then the prefix holds it (xmss_core.c:242-248)
             typename F::Node R;
             F::prf_idx(R, keys[q.key], q.idx);
             F::store(R, p + pad);
             F::store(R, R_out + (size_t)r * F::kN);
         }
     }
-    if (threadIdx.x == 0) sh.nbmax = 0;
+    if (threadIdx.x == 0) {
+        sh.nbmax = 0;
+        sh.b0min = 0xFFFFFFFFu;
+    }
     __syncthreads();  // (also orders the R stores before any lane's loads of the prefix)
-    if (wave == 0) atomicMax(&sh.nbmax, nb);
+    if (wave == 0) {
+        atomicMax(&sh.nbmax, nb);
+        atomicMin(&sh.b0min, b0);
+    }
     __syncthreads();
-    const uint32_t NB = sh.nbmax;
+    const uint32_t NB = sh.nbmax, B0 = sh.b0min;
     uint4 raw[4];
     auto schedule = [&](uint32_t blk) {
         uint32_t w[16];
@@ -441,17 +510,17 @@ __device__ __forceinline__ void hmsg_pair_block(const typename F::Key* __restric
         keep_loads_ahead();
         sha256_schedule_k(w, [&](int t, uint32_t v) { sh.wk[blk & 1][t][lane] = v; });
     };
-    uint32_t st[8];
-    sha256_iv(st);
-    if (wave == 0 && nb) {
-        sha256_block_load(raw, p, 0);
-        schedule(0);
+    if (wave == 0 && b0 < nb) {
+        sha256_block_load(raw, p, b0);
+        schedule(b0);
     }
     __syncthreads();
-    for (uint32_t blk = 0; blk < NB; ++blk) {
+    // each lane's blocks are [b0, nb): wave 0 schedules block blk + 1 while
+    // wave 1 compresses block blk
+    for (uint32_t blk = B0; blk < NB; ++blk) {
         if (wave == 0) {
-            if (blk + 1 < nb) schedule(blk + 1);
-        } else if (blk < nb) {
+            if (blk >= b0 && blk + 1 < nb) schedule(blk + 1);
+        } else if (wave == 1 && blk >= b0 && blk < nb) {
             // all 64 round inputs into registers first: the LDS reads are in
             // flight together instead of one per round
             uint32_t x[64];

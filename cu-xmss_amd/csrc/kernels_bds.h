@@ -456,8 +456,16 @@ __global__ __launch_bounds__(WG) void k_bds_pre(const typename F::Key* __restric
                                                  const DLeaf* __restrict__ sigs, uint32_t nsign,
                                                  uint8_t* __restrict__ scratch, size_t stride,
                                                  const DThash* __restrict__ merges, uint32_t nmerge,
-                                                 typename F::HPre* __restrict__ merge_out) {
+                                                 typename F::HPre* __restrict__ merge_out, HmsgMid* __restrict__ mid,
+                                                 uint32_t head) {
     const uint32_t t = blockIdx.x * WG + threadIdx.x;
+    if (head && blockIdx.y == gridDim.y - 1) {  // the last row: the first blocks of long messages' H_msg
+        if constexpr (F::kPairHmsg) {
+            const uint32_t r = mirror_lanes(t, nmsg);
+            if (r < nmsg) hmsg_head<F>(keys, msgs, r, pad, blob, R_out, digests, mid, head, r == t);
+        }
+        return;
+    }
     if (blockIdx.y < hrows) {
         hmsg_mirrored<F>(keys, msgs, blockIdx.y * gridDim.x * WG + t, nmsg, pad, blob, R_out, digests);
         return;
@@ -608,6 +616,8 @@ __device__ __forceinline__ void fin_count_in(bool on, uint32_t* count) {
     }
 }
 
+// mid (long messages, SHA-256 families): k_bds_pre hashed the first blocks of
+// each H_msg (hmsg_head) and the H_msg blocks resume from there.
 // Blocks: nleaf leaves (or, for rows marked LAT_ENDS_OUT / LAT_ENDS_IN, the
 // chains into slot p2 of `ends` / the L-tree from it), then nsig WOTS
 // signatures (short messages), then the H_msg blocks (long messages): 64
@@ -622,7 +632,8 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
                                                   size_t stride,           // loops' loads stay ahead
                                                   uint8_t* __restrict__ leaf_out, uint8_t* __restrict__ sig_out,
                                                   uint8_t* __restrict__ ends, uint32_t nvals,
-                                                  uint8_t* __restrict__ vals, BdsFinDag dag) {
+                                                  uint8_t* __restrict__ vals, BdsFinDag dag,
+                                                  const HmsgMid* __restrict__ mid) {
     __shared__ BdsFinLds<F> sh;
     if (dag.on && blockIdx.x == gridDim.x - 1) {  // the merge DAG, once the planned leaves are in
         if (!fin_wait(dag.sync, dag.wait, dag.spin)) return;  // the update launch does it all
@@ -667,7 +678,7 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
     } else {
         if constexpr (BdsFinPair<F>::value)
             hmsg_pair_block<F>(keys, msgs, (blockIdx.x - nleaf - nsig - nvals) * 64, nmsg, pad, blob, R_out,
-                               digests, sh.pair);
+                               digests, sh.pair, mid);
         else
             hmsg_mirrored<F>(keys, msgs, (blockIdx.x - nleaf - nsig - nvals) * blockDim.x + threadIdx.x, nmsg,
                              pad, blob, R_out, digests);

@@ -200,7 +200,7 @@ struct FamOps {
                          uint8_t* R, uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs,
                          uint32_t nsign, uint8_t* scratch, const DThash* merges, uint32_t nmerge, void* merge_out,
                          uint8_t* leaf_out, uint8_t* sig_out, bool long_msgs, uint8_t* ends, uint8_t* vals,
-                         const BdsFinDag& dag) = 0;
+                         const BdsFinDag& dag, void* mid) = 0;
     // The state updates, one workgroup per job; dag / lits: the planned
     // merges' sources (3 words per merge, parallel to merge_arena) and the
     // literal node values (jobs with maxlev 0 hash their merges one by one).
@@ -395,15 +395,20 @@ struct FamOpsT final : FamOps {
     void bds_lat(Launcher& L, const void* keys, const DMsg* msgs, uint32_t nmsg, uint32_t pad, uint8_t* blob,
                  uint8_t* R, uint8_t* digests, const DLeaf* leaves, uint32_t nleaf, const DLeaf* sigs, uint32_t nsign,
                  uint8_t* scratch, const DThash* merges, uint32_t nmerge, void* merge_out, uint8_t* leaf_out,
-                 uint8_t* sig_out, bool long_msgs, uint8_t* ends, uint8_t* vals, const BdsFinDag& dag) override {
+                 uint8_t* sig_out, bool long_msgs, uint8_t* ends, uint8_t* vals, const BdsFinDag& dag,
+                 void* mid) override {
         const size_t stride = LatLayout<F>::bytes(0);
         const uint32_t xb = cdiv(F::LEN * 17 - 1, WG);  // LAT_LEAF / LAT_SIGN items per request
         const uint32_t mrows = cdiv(nmerge, (uint64_t)xb * WG);
         const uint32_t hrows = long_msgs ? 0 : cdiv(nmsg, (uint64_t)xb * WG);
         const uint32_t threads = ((F::LEN + 63) / 64) * 64;
-        L.run("bds_pre", k_bds_pre<F>, dim3(xb, hrows + nleaf + nsign + mrows), dim3(WG), (const K*)keys, msgs, pad,
-              blob, R, digests, hrows, nmsg, leaves, nleaf, sigs, nsign, scratch, stride, merges, nmerge,
-              (typename F::HPre*)merge_out);
+        // one long message (mid set): its first H_msg blocks in the last row
+        // of the pre-image launch, one wave (mirrored lanes)
+        HmsgMid* hm = BdsFinPair<F>::value && long_msgs && nmsg == 1 ? (HmsgMid*)mid : nullptr;
+        const uint32_t head = hm ? kHmsgHeadBlocks : 0;
+        L.run("bds_pre", k_bds_pre<F>, dim3(xb, hrows + nleaf + nsign + mrows + (head ? 1 : 0)), dim3(WG),
+              (const K*)keys, msgs, pad, blob, R, digests, hrows, nmsg, leaves, nleaf, sigs, nsign, scratch, stride,
+              merges, nmerge, (typename F::HPre*)merge_out, hm, head);
         const uint32_t nsig = long_msgs ? 0 : nmsg, nlate = long_msgs ? nmsg : 0;
         const uint32_t nvals = long_msgs && vals ? nmsg : 0;
         // late H_msg: 64 messages per workgroup by its two waves (k_bds_fin)
@@ -412,7 +417,7 @@ struct FamOpsT final : FamOps {
         // planned leaves' blocks, which are dispatched before it
         L.run("bds_fin", k_bds_fin<F>, dim3(nleaf + nsig + nvals + cdiv(nlate, per) + (dag.on ? 1 : 0)),
               dim3(threads), (const K*)keys, msgs, nlate, pad, blob, R, digests, leaves, nleaf, sigs, nsig,
-              (const uint8_t*)scratch, stride, leaf_out, sig_out, ends, nvals, vals, dag);
+              (const uint8_t*)scratch, stride, leaf_out, sig_out, ends, nvals, vals, dag, (const HmsgMid*)hm);
     }
     void bds_update(Launcher& L, const void* keys, size_t lds, const DBds* jobs, uint32_t n,
                     uint8_t* sk_arena, const uint8_t* leaf_arena, const void* merge_arena, const uint32_t* dag,
