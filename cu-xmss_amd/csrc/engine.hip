@@ -2322,7 +2322,7 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
         fd.q0 = ddag;  // job 0's merges and literals start at offset 0
         fd.lits = dlit;
         fd.pre = merge_arena;
-        fd.vals = jd[0].maxlev ? B.dagv.need((size_t)jd[0].nmerge * n) : nullptr;
+        fd.vals = jd[0].maxlev && !fin_state ? B.dagv.need((size_t)jd[0].nmerge * n) : nullptr;
         fd.state = fin_state;
         fd.jobs = djobs;
         fd.sk_arena = ska;

@@ -552,13 +552,18 @@ __device__ void merge_dag(const uint32_t* q0, uint32_t nmerge, uint32_t maxlev, 
     }
 }
 
+// The two halves of a state update (bds_update_job): STAGE needs only the sk
+// (the resident sk into the arena, the states into LDS, the signature tail
+// from the old states), RUN the planned leaves (their staging, the merges,
+// the state machine, the write-back).
+enum : uint32_t { BDS_PHASE_STAGE = 1, BDS_PHASE_RUN = 2, BDS_PHASE_ALL = 3 };
 template <class F>
 __device__ __forceinline__ void bds_update_job(const typename F::Key& key, const DBds& job, uint8_t* skg,
                                const uint8_t* __restrict__ leaf_arena,
                                const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
                                const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
                                uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena,
-                               const uint8_t* premerged, uint8_t* lds_states);
+                               const uint8_t* premerged, uint8_t* lds_states, uint32_t phases);
 
 // The merge DAG of a one-signature launch hashed in k_bds_fin, beside H_msg
 // and the WOTS signature, instead of in the update launch: its last block
@@ -573,8 +578,9 @@ struct BdsFinDag {
     const uint8_t* lits;
     const void* pre;
     uint8_t* vals;
-    // state: the block then runs job 0's whole state update (bds_update_job)
-    // in its LDS and sets sync[1] = 2, and the update launch skips the job
+    // state: the block instead runs job 0's whole state update
+    // (bds_update_job) in its LDS, the merges included (vals unused), and
+    // sets sync[1] = 2, and the update launch skips the job
     uint32_t state;
     // select (a long message's chain values walked beside H_msg): the
     // values block counts in at sync[2], and the H_msg block, once its
@@ -636,25 +642,28 @@ __global__ __launch_bounds__(192) void k_bds_fin(const typename F::Key* __restri
                                                   const HmsgMid* __restrict__ mid) {
     __shared__ BdsFinLds<F> sh;
     if (dag.on && blockIdx.x == gridDim.x - 1) {  // the merge DAG, once the planned leaves are in
-        if (!fin_wait(dag.sync, dag.wait, dag.spin)) return;  // the update launch does it all
         const typename F::HPre* pre = reinterpret_cast<const typename F::HPre*>(dag.pre);
-        const uint8_t* merged = nullptr;
-        if (dag.maxlev) {
-            merge_dag<F>(dag.q0, dag.nmerge, dag.maxlev, dag.lits, leaf_out, pre, dag.vals);
-            merged = dag.vals;
-            if (threadIdx.x == 0) {
-                __threadfence();
-                dag.sync[1] = 1;
-            }
-        }
-        if (dag.state) {  // and the state update, while H_msg or the WOTS signature still runs
-            const DBds job = dag.jobs[0];
+        const DBds job = dag.jobs[0];
+        // (state) what needs no leaf goes first, while the leaves are hashed;
+        // should the wait give up, the update launch redoes it, to the same
+        // bytes
+        if (dag.state)
             bds_update_job<F>(keys[job.key], job, dag.sk_arena + job.sk_off, leaf_out, pre, dag.q0, dag.lits,
-                              dag.wpre, dag.wstride, dag.tails, nullptr, merged, sh.raw);
+                              dag.wpre, dag.wstride, dag.tails, nullptr, nullptr, sh.raw, BDS_PHASE_STAGE);
+        if (!fin_wait(dag.sync, dag.wait, dag.spin)) return;  // the update launch does it all
+        if (dag.state) {  // the merges into LDS and the state update, while H_msg or the WOTS signature runs
+            bds_update_job<F>(keys[job.key], job, dag.sk_arena + job.sk_off, leaf_out, pre, dag.q0, dag.lits,
+                              dag.wpre, dag.wstride, dag.tails, nullptr, nullptr, sh.raw, BDS_PHASE_RUN);
             __syncthreads();
             if (threadIdx.x == 0) {
                 __threadfence();
                 dag.sync[1] = 2;
+            }
+        } else if (dag.maxlev) {  // the merges alone, for the update launch
+            merge_dag<F>(dag.q0, dag.nmerge, dag.maxlev, dag.lits, leaf_out, pre, dag.vals);
+            if (threadIdx.x == 0) {
+                __threadfence();
+                dag.sync[1] = 1;
             }
         }
         return;
@@ -709,7 +718,7 @@ __device__ __forceinline__ void bds_update_one(const typename F::Key& key, const
                                const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
                                const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
                                uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena,
-                               const uint8_t* premerged, uint8_t* lds_states) {
+                               const uint8_t* premerged, uint8_t* lds_states, uint32_t phases) {
     const BdsP& P = job.P;
     BdsDev<F> B(lds_states, skg, P, leaf_arena + job.leaf_off * F::kN, merge_arena + job.merge_off, job.nmerge,
                 key);
@@ -726,9 +735,27 @@ __device__ __forceinline__ void bds_update_one(const typename F::Key& key, const
     const uint32_t n = F::kN, th = P.th, d = P.d;
     const uint64_t idx = job.idx;
     const uint32_t region = (2 * d - 1) * P.state_bytes;
-    // stage the serialized states in LDS; written back when the update is done
-    wg_copy(lds_states, skg + B.states_base(), region);
-    // and the planned leaves: the state machine takes them one at a time,
+    if (phases & BDS_PHASE_STAGE) {
+        // stage the serialized states in LDS (written back when the update is
+        // done), then the signature tail: auth of layer 0, then (WOTS
+        // signature, auth) of layers 1..d-1
+        wg_copy(lds_states, skg + B.states_base(), region);
+        __syncthreads();
+        size_t o = 0;
+        for (uint32_t i = 0; i < d; ++i) {
+            if (i > 0) {
+                const uint8_t* w = B.wots_sigs() + (size_t)(i - 1) * P.wots_sig_bytes;
+                wg_copy(tail + o, w, P.wots_sig_bytes);
+                o += P.wots_sig_bytes;
+            }
+            const uint8_t* a = B.sk + B.o_auth(i);
+            wg_copy(tail + o, a, th * n);
+            o += (size_t)th * n;
+        }
+        __syncthreads();
+    }
+    if (!(phases & BDS_PHASE_RUN)) return;
+    // the planned leaves: the state machine takes them one at a time,
     // each a read of HBM that another launch wrote (about 4 us apiece
     // measured) when read in place
     if (job.lv_off) {
@@ -763,21 +790,6 @@ __device__ __forceinline__ void bds_update_one(const typename F::Key& key, const
             wg_copy(g, l, len);
         }
     } wb{lds_states, skg + B.states_base(), region};
-    // signature tail: auth of layer 0, then (WOTS signature, auth) of layers 1..d-1
-    {
-        size_t o = 0;
-        for (uint32_t i = 0; i < d; ++i) {
-            if (i > 0) {
-                const uint8_t* w = B.wots_sigs() + (size_t)(i - 1) * P.wots_sig_bytes;
-                wg_copy(tail + o, w, P.wots_sig_bytes);
-                o += P.wots_sig_bytes;
-            }
-            const uint8_t* a = B.sk + B.o_auth(i);
-            wg_copy(tail + o, a, th * n);
-            o += (size_t)th * n;
-        }
-        __syncthreads();
-    }
     if (d == 1) {
         // xmss_core_sign (xmss_core_fast.c:836-839); the host routes the last index elsewhere
         if (idx < (1ull << th) - 1) {
@@ -838,9 +850,9 @@ __device__ __forceinline__ void bds_update_job(const typename F::Key& key, const
                                const typename F::HPre* __restrict__ merge_arena, const uint32_t* __restrict__ dag,
                                const uint8_t* __restrict__ lit_arena, const uint8_t* __restrict__ wpre, size_t wstride,
                                uint8_t* __restrict__ tails, uint8_t* __restrict__ scratch_arena,
-                               const uint8_t* premerged, uint8_t* lds_states) {
+                               const uint8_t* premerged, uint8_t* lds_states, uint32_t phases) {
     uint8_t* res = reinterpret_cast<uint8_t*>(job.res);
-    if (res) {
+    if (res && (phases & BDS_PHASE_STAGE)) {
         wg_copy(skg, res, job.sk_bytes);
         __syncthreads();
         if (threadIdx.x < job.P.index_bytes) {
@@ -850,8 +862,8 @@ __device__ __forceinline__ void bds_update_job(const typename F::Key& key, const
         __syncthreads();
     }
     bds_update_one<F>(key, job, skg, leaf_arena, merge_arena, dag, lit_arena, wpre, wstride, tails, scratch_arena,
-                      premerged, lds_states);
-    if (res) {
+                      premerged, lds_states, phases);
+    if (res && (phases & BDS_PHASE_RUN)) {
         __syncthreads();
         wg_copy(res, skg, job.sk_bytes);
     }
@@ -882,7 +894,7 @@ __global__ __launch_bounds__(192) void k_bds_update(const typename F::Key* __res
     // job 0's merges, if k_bds_fin hashed them (dag_sync[1] set)
     const uint8_t* premerged = dag_vals && blockIdx.x == 0 && dag_sync[1] == 1 ? dag_vals : nullptr;
     bds_update_job<F>(keys[job.key], job, sk_arena + job.sk_off, leaf_arena, merge_arena, dag, lit_arena, wpre,
-                      wstride, tails, scratch_arena, premerged, lds_dyn);
+                      wstride, tails, scratch_arena, premerged, lds_dyn, BDS_PHASE_ALL);
 }
 
 }  // namespace xm
