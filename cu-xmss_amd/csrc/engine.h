@@ -416,8 +416,9 @@ public:
     // For each job: R and H_msg, the planned leaves (in the order the
     // signature consumes them), the layer-0 WOTS signature and the state
     // update of the serialized sk, in three launches on one stream per chunk
-    // of jobs; sk (or its resident copy dsk) is updated in place.  key is
-    // the job's slot in the device key pool.
+    // of jobs (two for a job launched alone, sig_values); sk (or its resident
+    // copy dsk) is updated in place.  key is the job's slot in the device key
+    // pool.
     struct BdsJob {
         const uint8_t* sk_seed;
         const uint8_t* sk_prf;

@@ -2284,7 +2284,8 @@ void Engine::bds_sign_chunk(int fam, int count, BdsJob* jobs) {
     // three launches on the main stream: R / H_msg with the address-only work
     // of the planned leaves, the layer-0 WOTS signatures (all steps: the
     // digests are computed in the same launch) and the merges, then the
-    // leaves' and signatures' finishing work, then the state update
+    // leaves' and signatures' finishing work, then the state update (which a
+    // lone signature's finishing launch usually runs itself: BdsFinDag)
     // scratch rows: the planned leaves, the layer-0 WOTS signatures, then the
     // upper-layer WOTS signatures of the state updates
     uint8_t* lat = B.lat_leaf.need((nl + nw) * o->lat_bytes(0));

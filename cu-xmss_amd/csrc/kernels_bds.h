@@ -431,14 +431,16 @@ struct BdsDev {
 };
 
 // The hashing of BDS signatures, in three launches on one stream (so that a
-// signature costs three launches and no cross-stream waits: the host's
-// launch calls, as much as the GPU, bound a single signature's latency):
+// signature costs no cross-stream waits: the host's launch calls, as much as
+// the GPU, bound a single signature's latency):
 //   k_bds_pre:    the address-only work of the planned leaves (LAT_LEAF
 //                 rows), of the layer-0 WOTS signatures (LAT_SIGN rows, all 15
 //                 steps of every chain) and of the planned tree merges (one
 //                 merge per lane);
 //   k_bds_fin:    one workgroup per planned leaf;
 //   k_bds_update: the state update of each signature.
+// A signature launched alone usually needs only the first two: the last
+// workgroup of k_bds_fin runs its state update (BdsFinDag).
 // R / H_msg (one message per lane) and the layer-0 WOTS signatures (one
 // workgroup each) go one of two ways.  Short messages: H_msg in the first
 // rows of k_bds_pre and the signatures beside the leaves in k_bds_fin.  Long
